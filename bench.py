@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""ns/day of SPONGE's nonbonded force path on one MI355X.
+
+Workload (default): the 23,570-atom PME water box (7,850 TIP3P waters +
+10 Na+ + 10 Cl-, 61.9 A cubic box, rc 10 A, skin 2 A, PME 64^3 order 4,
+2 fs) -- one step = uint coordinates, fused LJ + PME-direct pair force,
+excluded-pair correction, PME reciprocal (spread, R2C FFT, BC convolution,
+C2R FFT, gather) and the neighbor-list refresh every 20 steps, all on the
+GPU through libsponge_nb.so. Inputs are resident in HBM before timing.
+
+Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits:
+this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 (vector) peak
+PEAK_HBM_GBS = 8000.0
+PAIR_FLOPS = 50              # per in-cutoff pair, force-only variant
+
+
+def cpu_baseline(sys_, seconds: float, beta, K):
+    """The FP64 oracle (same formulas, OpenMP) on a bounded sample."""
+    import numpy as np
+    import oracle
+    from oracle import direct, excluded, pme_reciprocal, nl_brute, crd_to_uint, periodic_map
+
+    crd = periodic_map(sys_.crd, sys_.box)
+    u = crd_to_uint(crd, sys_.box)
+    t0 = time.perf_counter()
+    pairs = nl_brute(sys_, u)
+    t_nl = time.perf_counter() - t0
+    n_force = 0
+    t0 = time.perf_counter()
+    while True:
+        direct(sys_, u, pairs, beta)
+        excluded(sys_, u, beta)
+        pme_reciprocal(sys_, u, K, beta)
+        n_force += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    t_force = (time.perf_counter() - t0) / n_force
+    t_step = t_force + t_nl / 20.0
+    return {
+        "value": round(0.1728 / t_step, 4),
+        "unit": "ns/day",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": (f"{n_force} FP64 oracle force evaluations (direct + exclusion + PME "
+                   f"{K[0]}x{K[1]}x{K[2]}) + 1 brute-force neighbor list / 20 on the "
+                   f"{sys_.n}-atom box; {t_step * 1e3:.1f} ms/step"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--config", type=int, default=1, help="0: 648 atoms, 1: 23,570, 2: 96,120")
+    ap.add_argument("--refresh", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard by spatial decomposition, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from sponge_amd import nb, system
+
+    sys_ = system.config(args.config)
+    eng = nb.NonbondedEngine(sys_, refresh_interval=args.refresh)
+    stream = torch.cuda.current_stream()
+
+    # Main_Initial: forced rebuild; then count in-cutoff pairs once.
+    eng.update_neighbor_list(forced=1)
+    eng.crd_to_uint()
+    eng.frc.zero_()
+    eng.force_step(need_energy=1, need_virial=1)
+    torch.cuda.synchronize()
+    parts = eng.energy_parts()
+    pairs_in = parts["pairs_in"]
+    nl_pairs = eng.nl.status()["pairs"]
+
+    def step():
+        eng.crd_to_uint()
+        eng.frc.zero_()
+        eng.force_step(0, 0)
+        eng.update_neighbor_list()
+
+    for _ in range(args.warmup):
+        step()
+    K = args.steps
+    ev_s = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    ev_e = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    for ev in ev_s + ev_e:   # torch creates the hipEvent on first record
+        ev.record(stream)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(K):
+        eng.lj.set_pair_events(ev_s[k], ev_e[k])
+        step()
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    eng.lj.set_pair_events(None, None)
+    ms = t_start.elapsed_time(t_end) / K
+    pair_ms = sum(a.elapsed_time(b) for a, b in zip(ev_s, ev_e)) / K
+
+    flops = PAIR_FLOPS * pairs_in
+    achieved = flops / (pair_ms * 1e-3) / 1e12
+    out = {
+        "metric": "ns/day (24k-atom PME water, 2 fs, nonbonded force path)",
+        "value": round(0.1728 / (ms * 1e-3), 3),
+        "unit": "ns/day",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{sys_.name}: {sys_.n} atoms, box {float(sys_.box[0])} A, rc {sys_.cutoff} "
+                         f"skin {sys_.skin}, PME {eng.pme.fftx}x{eng.pme.ffty}x{eng.pme.fftz} order 4, "
+                         f"neighbor list every {args.refresh} steps, static coordinates"),
+            "atoms": sys_.n,
+            "listed_pairs": int(nl_pairs),
+            "pairs_in_cutoff": int(pairs_in),
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "pair_kernel<false> (LJ + PME direct)",
+            "bound": "mfma",
+            "note": "FP32 VALU bound (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": None,
+            "kernel_ms": round(pair_ms, 6),
+            "flops_per_launch": flops,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sys_, args.cpu_seconds, eng.pme.beta,
+                                           (eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,217 @@
+/*
+ * sponge_nb.h -- C ABI of the MI355X (gfx950) nonbonded hot path of SPONGE.
+ *
+ * Three module handles mirror the reference's three C++ modules; every entry
+ * point below replaces one reference method (cited as file:line relative to
+ * the reference's SPONGE/ directory). All pointers named d_* are device
+ * pointers (HIP), laid out exactly as the reference lays them out
+ * (VECTOR = 3 floats AoS, UNSIGNED_INT_VECTOR = 3 uint32 AoS, original atom
+ * order), so the reference's MD_INFORMATION buffers can be passed unchanged.
+ * Outputs follow the reference's accumulate-into (+=) semantics.
+ *
+ * `stream` is a hipStream_t (NULL = the default stream). No entry point that
+ * takes a stream allocates, frees or synchronises, so a whole MD step can be
+ * captured into one hipGraph; functions that return host values synchronise
+ * and say so.
+ *
+ * Return value: 0 on success, a negative SPONGE_NB_E* code on failure;
+ * sponge_nb_last_error() describes the last failure on the calling thread.
+ */
+#ifndef SPONGE_NB_H
+#define SPONGE_NB_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPONGE_NB_OK 0
+#define SPONGE_NB_EARG (-1)      /* invalid argument / shape */
+#define SPONGE_NB_EHIP (-2)      /* HIP runtime error */
+#define SPONGE_NB_ECAPACITY (-3) /* a fixed capacity was exceeded (see overflow flags) */
+#define SPONGE_NB_ENODEV (-4)    /* no usable gfx950 device */
+
+/* Overflow flags reported by sponge_nl_status (bit set = overflowed). */
+#define SPONGE_NL_OVERFLOW_COLUMN 1  /* too many atoms in one cell column */
+#define SPONGE_NL_OVERFLOW_LIST 2    /* too many cluster pairs for one cluster */
+#define SPONGE_NL_OVERFLOW_CANDIDATES 4
+
+/* Energy / virial parts accumulated in FP64 on the device by the fused step
+ * and by the energy queries (index into a double[SPONGE_NB_NPARTS]). */
+#define SPONGE_PART_LJ 0          /* sum of LJ pair energies (r < rc)          */
+#define SPONGE_PART_DIRECT 1      /* PME direct-space energy                   */
+#define SPONGE_PART_LJ_VIRIAL 2   /* sum of -f_LJ r^2                          */
+#define SPONGE_PART_CORRECTION 3  /* excluded-pair correction energy          */
+#define SPONGE_PART_SELF 4        /* self + neutralising energy                */
+#define SPONGE_PART_RECIPROCAL 5  /* reciprocal-space energy                   */
+#define SPONGE_PART_LRC 6         /* long-range dispersion energy (factor / V) */
+#define SPONGE_PART_PAIRS_IN 7    /* number of listed pairs inside the cutoff  */
+#define SPONGE_NB_NPARTS 8
+
+typedef struct sponge_nl sponge_nl;
+typedef struct sponge_lj sponge_lj;
+typedef struct sponge_pme sponge_pme;
+
+const char* sponge_nb_last_error(void);
+
+/* Number of usable devices and the name of the device the library runs on. */
+int sponge_nb_device_info(int* count, char* name, int name_len);
+
+/* ------------------------------------------------------------------ */
+/* MD_INFORMATION::MD_Information_Crd_To_Uint_Crd (MD_core/MD_core.cu:1528-1532),
+ * kernel Crd_To_Uint_Crd (common.cu:264-287): u = ((int)(x * 2^30 / L)) << 2. */
+int sponge_crd_to_uint(int atom_numbers, const float box_length[3], const float* d_crd,
+                       unsigned int* d_uint_crd, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* NEIGHBOR_LIST (neighbor_list/neighbor_list.cuh:44-105).                */
+
+/* NEIGHBOR_LIST::Initial (neighbor_list/neighbor_list.cu:580-642). The mdin
+ * keys neighbor_list_refresh_interval / _max_atom_in_grid_numbers /
+ * _max_neighbor_numbers / _skin_permit arrive already parsed (skin_permit as
+ * written in mdin; the doubling of neighbor_list.cu:610 happens inside). */
+int sponge_nl_create(int atom_numbers, const float box_length[3], float cutoff, float skin,
+                     int refresh_interval, int max_atom_in_grid_numbers,
+                     int max_neighbor_numbers, float skin_permit, sponge_nl** out);
+/* NEIGHBOR_LIST::Clear (neighbor_list/neighbor_list.cu:659-711). */
+void sponge_nl_destroy(sponge_nl* nl);
+
+/* NEIGHBOR_LIST::Neighbor_List_Update (neighbor_list/neighbor_list.cu:510-556):
+ * forced / every-refresh_interval / skin-triggered rebuild. d_crd (AoS
+ * float3, device) is wrapped into the box in place on a rebuild, as the
+ * reference's Crd_Periodic_Map does. The exclusion CSR is the reference's
+ * (sorted, upper triangular). The skin-triggered path is decided on the
+ * device (no host round trip, no device-side launch). */
+int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_excluded_list_start,
+                     const int* d_excluded_list, const int* d_excluded_numbers,
+                     int forced_update, int forced_check, void* stream);
+
+/* NEIGHBOR_LIST::Update_Volume (neighbor_list/neighbor_list.cu:645-656). */
+int sponge_nl_update_volume(sponge_nl* nl, const float box_length[3]);
+
+/* Synchronises. Number of (i<j) atom pairs currently in the list, overflow
+ * flags (SPONGE_NL_OVERFLOW_*), and how many rebuilds have run. */
+int sponge_nl_status(sponge_nl* nl, long* pair_count, int* overflow_flags, int* rebuilds);
+
+/* Synchronises. The list as canonical atom pairs (i<j, original indices),
+ * unsorted; at most `capacity` pairs are written to h_pairs (2 ints each).
+ * Replaces reading d_nl[i].atom_serial on the host (neighbor_list.cu:490-503). */
+int sponge_nl_get_pairs(sponge_nl* nl, int* h_pairs, long capacity, long* pair_count);
+
+/* ------------------------------------------------------------------ */
+/* LENNARD_JONES_INFORMATION (Lennard_Jones_force/Lennard_Jones_force.cuh:37-149). */
+
+/* LENNARD_JONES_INFORMATION::Initial (Lennard_Jones_force.cu:1067-1139) after
+ * the LJ_in_file / amber_parm7 read: h_LJ_A / h_LJ_B already multiplied by
+ * 12 / 6, lower-triangular pair storage, h_atom_LJ_type per atom. */
+int sponge_lj_create(int atom_numbers, int atom_type_numbers, const float* h_LJ_A,
+                     const float* h_LJ_B, const int* h_atom_LJ_type, float cutoff,
+                     const float box_length[3], sponge_lj** out);
+void sponge_lj_destroy(sponge_lj* lj);
+
+/* LENNARD_JONES_INFORMATION::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial
+ * (Lennard_Jones_force.cu:1318-1352 and kernels :47-562). Accumulates the
+ * fused LJ + erfc direct-Coulomb force into d_frc; with need_atom_energy the
+ * LJ + direct energy into d_atom_energy; with need_virial the LJ virial into
+ * d_atom_lj_virial. d_atom_direct_pme_energy is reset and refilled whenever
+ * energy or virial is requested (the reference resets it only on the virial
+ * path). Per-atom terms are credited to the lower-index atom of each pair,
+ * as the reference's half list does. */
+int sponge_lj_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, const unsigned int* d_uint_crd,
+                               const float* d_charge, float* d_frc, float pme_beta,
+                               int need_atom_energy, float* d_atom_energy, int need_virial,
+                               float* d_atom_lj_virial, float* d_atom_direct_pme_energy,
+                               void* stream);
+
+/* LENNARD_JONES_INFORMATION::Long_Range_Correction (Lennard_Jones_force.cu:1178-1191). */
+int sponge_lj_long_range_correction(sponge_lj* lj, int need_pressure, float* d_virial,
+                                    int need_potential, float* d_potential, void* stream);
+
+/* LENNARD_JONES_INFORMATION::Get_Energy (Lennard_Jones_force.cu:1407-1433):
+ * LJ energy over the list (r < rc) plus the long-range correction, FP64.
+ * Synchronises. */
+int sponge_lj_get_energy(sponge_lj* lj, const sponge_nl* nl, const unsigned int* d_uint_crd,
+                         double* h_energy, void* stream);
+
+/* LENNARD_JONES_INFORMATION::Update_Volume (Lennard_Jones_force.cu:1450-1456). */
+int sponge_lj_update_volume(sponge_lj* lj, const float box_length[3]);
+
+/* Host copies the reference keeps public for nb14 (Lennard_Jones_force.cuh:52-58). */
+int sponge_lj_host_tables(const sponge_lj* lj, const float** h_LJ_A, const float** h_LJ_B,
+                          const int** h_atom_LJ_type);
+
+/* Measurement hook (no counterpart in the reference): when set, every later
+ * pair-kernel launch of this module is bracketed by hipEventRecord of the
+ * two hipEvent_t on the launch stream, so a caller can time the dominant
+ * kernel inside a timed region. NULL events switch the hook off. */
+int sponge_lj_set_pair_events(sponge_lj* lj, void* start_event, void* stop_event);
+
+/* ------------------------------------------------------------------ */
+/* Particle_Mesh_Ewald (PME_force/PME_force.cuh:25-144).               */
+
+/* Particle_Mesh_Ewald::Initial (PME_force.cu:108-306): beta from the
+ * PME_Direct_Tolerance bisection, grid from PME_fftx/y/z (<=0 = automatic,
+ * Get_Fft_Patameter), order-4 BC table. */
+int sponge_pme_create(int atom_numbers, const float box_length[3], float cutoff,
+                      float direct_tolerance, int fftx, int ffty, int fftz, sponge_pme** out);
+void sponge_pme_destroy(sponge_pme* pme);
+
+/* Public members the reference's callers read: beta (PME_force.cuh:62) and
+ * fftx/ffty/fftz (:30-32). */
+int sponge_pme_info(const sponge_pme* pme, float* beta, int fft[3]);
+
+/* Particle_Mesh_Ewald::PME_Excluded_Force_With_Atom_Energy
+ * (PME_force.cu:697-710, kernel :620-695). Also keeps the per-atom
+ * correction energy internally (d_correction_atom_energy). */
+int sponge_pme_excluded_force(sponge_pme* pme, const unsigned int* d_uint_crd,
+                              const float scaler[3], const float* d_charge,
+                              const int* d_excluded_list_start, const int* d_excluded_list,
+                              const int* d_excluded_atom_numbers, float* d_frc,
+                              float* d_atom_energy, void* stream);
+
+/* Particle_Mesh_Ewald::PME_Reciprocal_Force_With_Energy_And_Virial
+ * (PME_force.cu:722-773): spread, 3D R2C FFT, BC convolution, C2R FFT,
+ * gather into d_frc; with need_energy / need_virial adds
+ * correction + self + reciprocal energy to *d_potential and
+ * direct + correction + self + reciprocal to *d_virial (FP64 reductions).
+ * d_direct_atom_energy is the array last filled by
+ * sponge_lj_pme_direct_force (may be NULL when neither is needed). */
+int sponge_pme_reciprocal_force(sponge_pme* pme, const unsigned int* d_uint_crd,
+                                const float* d_charge, float* d_frc, int need_virial,
+                                int need_energy, float* d_virial, float* d_potential,
+                                const float* d_direct_atom_energy, void* stream);
+
+/* Particle_Mesh_Ewald::Get_Energy (PME_force.cu:892-994). which_part:
+ * 0 total, 1 direct, 2 reciprocal, 3 correction, 4 self (PME_force.cuh:86-93).
+ * FP64, synchronises. */
+int sponge_pme_get_energy(sponge_pme* pme, const sponge_nl* nl, const unsigned int* d_uint_crd,
+                          const float* d_charge, const float scaler[3],
+                          const int* d_excluded_list_start, const int* d_excluded_list,
+                          const int* d_excluded_atom_numbers, int which_part, double* h_energy,
+                          void* stream);
+
+/* Particle_Mesh_Ewald::Update_Box_Length (PME_force.cu:1033-1042); the
+ * reference's Update_Volume (:996-999) forwards the stale member box, this
+ * entry point takes the new box for both. */
+int sponge_pme_update_box_length(sponge_pme* pme, const float box_length[3]);
+
+/* ------------------------------------------------------------------ */
+/* Fused per-step force path: the body of Main_Calculate_Force's nonbonded
+ * calls (main.cu:180-193) in the minimum number of launches. Accumulates all
+ * nonbonded forces into d_frc; when need_energy / need_virial is set, the
+ * FP64 parts (SPONGE_PART_*) are written to d_parts (device, 8 doubles) and
+ * the per-atom arrays are filled like the separate calls fill them. */
+int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme,
+                         const unsigned int* d_uint_crd, const float* d_charge,
+                         const int* d_excluded_list_start, const int* d_excluded_list,
+                         const int* d_excluded_numbers, float* d_frc, int need_energy,
+                         int need_virial, float* d_atom_energy, float* d_atom_virial,
+                         double* d_parts, void* stream);
+
+/* Sum_Of_List / Add_Sum_List (common.cu:192-263, MD_core.cu:173-181) with an
+ * FP64 accumulator: *d_out += sum(d_list[0..n)). */
+int sponge_sum_list(int n, const float* d_list, float* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPONGE_NB_H */

@@ -1,0 +1,242 @@
+"""ctypes bindings to the FP64 oracle (oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py -- never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        I, F, D, Lg = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_long
+        sig = {
+            "orc_crd_to_uint": (None, [I, P, P, P]),
+            "orc_periodic_map": (None, [I, P, P]),
+            "orc_nl_brute": (Lg, [I, P, P, F, F, P, P, P, P, Lg]),
+            "orc_nl_grid": (Lg, [I, P, P, F, F, P, P, P, P, Lg, P]),
+            "orc_direct": (None, [I, P, P, P, P, P, P, D, D, Lg, P, P, P, P, P, P]),
+            "orc_excluded": (D, [I, P, P, P, D, P, P, P, P, P]),
+            "orc_pme_beta": (D, [D, D]),
+            "orc_bspline_mod2": (D, [I, I]),
+            "orc_pme_bc_full": (None, [I, I, I, P, D, P]),
+            "orc_fft3_r2c": (None, [I, I, I, P, P]),
+            "orc_fft3_c2r": (None, [I, I, I, P, P]),
+            "orc_pme_reciprocal": (D, [I, P, P, P, I, I, I, D, P, P, P]),
+            "orc_pme_self": (None, [I, P, D, P, P]),
+            "orc_lj_lrc_factor": (D, [I, P, I, P, D]),
+            "orc_ewald_exact": (D, [I, P, P, P, D, D, I, P]),
+            "orc_num_threads": (I, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def crd_to_uint(crd, box):
+    crd = _c(crd, np.float32)
+    box = _c(box, np.float32)
+    out = np.zeros(crd.shape, np.uint32)
+    lib().orc_crd_to_uint(crd.shape[0], _p(crd), _p(box), _p(out))
+    return out
+
+
+def periodic_map(crd, box):
+    crd = _c(crd, np.float32).copy()
+    lib().orc_periodic_map(crd.shape[0], _p(crd), _p(_c(box, np.float32)))
+    return crd
+
+
+def _excl(sys_):
+    return (_c(sys_.excl_start, np.int32), _c(sys_.excl_numbers, np.int32),
+            _c(sys_.excl_list, np.int32) if len(sys_.excl_list) else np.zeros(1, np.int32))
+
+
+def nl_brute(sys_, uint_crd):
+    """Sorted (P,2) int32 array of pairs i<j in the neighbor list."""
+    u = _c(uint_crd, np.uint32)
+    box = _c(sys_.box, np.float32)
+    es, en, el = _excl(sys_)
+    m = lib().orc_nl_brute(sys_.n, _p(u), _p(box), sys_.cutoff, sys_.skin,
+                           _p(es), _p(en), _p(el), None, 0)
+    out = np.zeros((max(m, 1), 2), np.int32)
+    lib().orc_nl_brute(sys_.n, _p(u), _p(box), sys_.cutoff, sys_.skin,
+                       _p(es), _p(en), _p(el), _p(out), m)
+    return out[:m]
+
+
+def nl_grid(sys_, crd):
+    crd = _c(crd, np.float32)
+    box = _c(sys_.box, np.float32)
+    es, en, el = _excl(sys_)
+    dims = np.zeros(3, np.int32)
+    m = lib().orc_nl_grid(sys_.n, _p(crd), _p(box), sys_.cutoff, sys_.skin,
+                          _p(es), _p(en), _p(el), None, 0, _p(dims))
+    out = np.zeros((max(m, 1), 2), np.int32)
+    lib().orc_nl_grid(sys_.n, _p(crd), _p(box), sys_.cutoff, sys_.skin,
+                      _p(es), _p(en), _p(el), _p(out), m, _p(dims))
+    return out[:m], tuple(int(d) for d in dims)
+
+
+def direct(sys_, uint_crd, pairs, beta, charge=None):
+    """FP64 LJ + PME-direct terms. Returns dict of forces, per-atom arrays,
+    totals (E_lj, E_direct, W_lj, n_in_cutoff)."""
+    n = sys_.n
+    u = _c(uint_crd, np.uint32)
+    q = _c(sys_.charge if charge is None else charge, np.float32)
+    t = _c(sys_.lj_type, np.int32)
+    A = _c(sys_.lj_A, np.float32)
+    B = _c(sys_.lj_B, np.float32)
+    pr = _c(pairs, np.int32)
+    frc = np.zeros((n, 3))
+    ae = np.zeros(n)
+    av = np.zeros(n)
+    ad = np.zeros(n)
+    tot = np.zeros(4)
+    lib().orc_direct(n, _p(u), _p(q), _p(t), _p(A), _p(B), _p(_c(sys_.box, np.float32)),
+                     float(sys_.cutoff), float(beta), pr.shape[0], _p(pr), _p(frc),
+                     _p(ae), _p(av), _p(ad), _p(tot))
+    return dict(frc=frc, atom_energy=ae, atom_virial=av, atom_direct=ad,
+                e_lj=tot[0], e_direct=tot[1], w_lj=tot[2], n_in=int(tot[3]))
+
+
+def excluded(sys_, uint_crd, beta, charge=None):
+    n = sys_.n
+    u = _c(uint_crd, np.uint32)
+    q = _c(sys_.charge if charge is None else charge, np.float32)
+    es, en, el = _excl(sys_)
+    frc = np.zeros((n, 3))
+    ac = np.zeros(n)
+    e = lib().orc_excluded(n, _p(u), _p(q), _p(_c(sys_.box, np.float32)), float(beta),
+                           _p(es), _p(en), _p(el), _p(frc), _p(ac))
+    return dict(frc=frc, atom_corr=ac, e_corr=e)
+
+
+def pme_beta(rc, tol=1e-5):
+    return lib().orc_pme_beta(float(rc), float(tol))
+
+
+def bspline_mod2(k, K):
+    return lib().orc_bspline_mod2(int(k), int(K))
+
+
+def pme_bc_full(K, box, beta):
+    bc = np.zeros(tuple(K))
+    lib().orc_pme_bc_full(K[0], K[1], K[2], _p(_c(box, np.float32)), float(beta), _p(bc))
+    return bc
+
+
+def fft3_r2c(a):
+    a = _c(a, np.float64)
+    Kx, Ky, Kz = a.shape
+    out = np.zeros((Kx, Ky, Kz // 2 + 1, 2))
+    lib().orc_fft3_r2c(Kx, Ky, Kz, _p(a), _p(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def fft3_c2r(c, Kz):
+    Kx, Ky, hz = c.shape
+    buf = _c(np.stack([c.real, c.imag], -1), np.float64)
+    out = np.zeros((Kx, Ky, Kz))
+    lib().orc_fft3_c2r(Kx, Ky, Kz, _p(buf), _p(out))
+    return out
+
+
+def pme_reciprocal(sys_, uint_crd, K, beta, charge=None, want_grids=False):
+    n = sys_.n
+    u = _c(uint_crd, np.uint32)
+    q = _c(sys_.charge if charge is None else charge, np.float32)
+    frc = np.zeros((n, 3))
+    Q = np.zeros(tuple(K)) if want_grids else None
+    C = np.zeros(tuple(K)) if want_grids else None
+    e = lib().orc_pme_reciprocal(n, _p(u), _p(q), _p(_c(sys_.box, np.float32)), K[0], K[1], K[2],
+                                 float(beta), _p(frc), _p(Q), _p(C))
+    out = dict(frc=frc, e_rec=e)
+    if want_grids:
+        out.update(Q=Q, conv=C)
+    return out
+
+
+def pme_self(charge, beta, box):
+    q = _c(charge, np.float32)
+    out = np.zeros(2)
+    lib().orc_pme_self(len(q), _p(q), float(beta), _p(_c(box, np.float32)), _p(out))
+    return out[0], out[1]
+
+
+def lj_lrc_factor(sys_):
+    return lib().orc_lj_lrc_factor(sys_.n, _p(_c(sys_.lj_type, np.int32)), sys_.ntypes,
+                                   _p(_c(sys_.lj_B, np.float32)), float(sys_.cutoff))
+
+
+def ewald_exact(xyz, q, box, beta, rreal, kmax):
+    xyz = _c(xyz, np.float64)
+    q = _c(q, np.float64)
+    frc = np.zeros(xyz.shape)
+    e = lib().orc_ewald_exact(len(q), _p(xyz), _p(q), _p(_c(box, np.float32)), float(beta),
+                              float(rreal), int(kmax), _p(frc))
+    return e, frc
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def full_reference(sys_, crd=None, charge=None, beta=None, K=None):
+    """Everything the force path produces for one configuration, FP64.
+
+    NL from brute force on the wrapped coordinates, then direct, exclusion,
+    reciprocal, self/neutralising and long-range terms."""
+    crd = sys_.crd if crd is None else crd
+    crd = periodic_map(crd, sys_.box)
+    u = crd_to_uint(crd, sys_.box)
+    if beta is None:
+        beta = float(np.float32(pme_beta(sys_.cutoff, sys_.pme_tolerance)))
+    if K is None:
+        from sponge_amd.system import fft_size
+        K = tuple(sys_.fft[d] if sys_.fft[d] > 0 else fft_size(sys_.box[d]) for d in range(3))
+    pairs = nl_brute(sys_, u)
+    q = sys_.charge if charge is None else charge
+    d = direct(sys_, u, pairs, beta, q)
+    x = excluded(sys_, u, beta, q)
+    r = pme_reciprocal(sys_, u, K, beta, q)
+    es, en = pme_self(q, beta, sys_.box)
+    V = float(np.prod(sys_.box.astype(np.float64)))
+    lrc = lj_lrc_factor(sys_) / V
+    frc = d["frc"] + x["frc"] + r["frc"]
+    return dict(uint=u, crd=crd, pairs=pairs, beta=beta, K=K, frc=frc,
+                frc_direct=d["frc"], frc_excl=x["frc"], frc_rec=r["frc"],
+                e_lj=d["e_lj"], e_direct=d["e_direct"], w_lj=d["w_lj"],
+                e_corr=x["e_corr"], e_rec=r["e_rec"], e_self=es + en,
+                e_lrc=lrc, atom_energy=d["atom_energy"], atom_virial=d["atom_virial"],
+                atom_direct=d["atom_direct"], atom_corr=x["atom_corr"], n_in=d["n_in"])

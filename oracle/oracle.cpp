@@ -1,0 +1,634 @@
+// FP64 CPU restatement of SPONGE's nonbonded hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP path: only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The
+// product (sponge_amd/) never links or calls it.
+//
+// Every function cites the reference file:line whose formula it restates
+// (paths relative to the reference's SPONGE/ directory). Arithmetic is FP64
+// except where the reference's *definition* is an FP32 expression that the
+// contract requires bit-for-bit (uint coordinates and the neighbor-list
+// distance test), which are restated in FP32 with a fixed operation order
+// (this file is compiled with -ffp-contract=off).
+//
+// Pins (see tests/test_oracle_*.py): closed forms (LJ dimer, Coulomb pair,
+// excluded pair), brute force vs the reference cell-grid algorithm, numpy's
+// pocketfft for the FFT, exact Ewald sums and the NaCl Madelung constant for
+// the reciprocal part, finite differences for every force.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cplx;
+static const double PI_D = 3.14159265358979323846;
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// a2 Crd_To_Uint_Crd (common.cu:264-287) with the NEIGHBOR_LIST cof
+// quarter_crd_to_uint_crd_cof = 0.25f * 2^32 / L (neighbor_list.cu:622).
+// FP32 by definition: u = ((int)(x * cof)) << 2.
+void orc_crd_to_uint(int n, const float* crd, const float* box, uint32_t* out) {
+    float cof[3];
+    for (int d = 0; d < 3; d++) cof[d] = (0.25f * 4294967296.0f) / box[d];
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            float t = crd[3 * i + d] * cof[d];
+            int32_t ti = (int32_t)t;
+            out[3 * i + d] = ((uint32_t)ti) << 2;
+        }
+}
+
+// a4 Crd_Periodic_Map (common.cu:299-352): shift by at most one box length.
+void orc_periodic_map(int n, float* crd, const float* box) {
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            float x = crd[3 * i + d];
+            if (x >= 0) {
+                if (!(x < box[d])) x = x - box[d];
+            } else {
+                x = x + box[d];
+            }
+            crd[3 * i + d] = x;
+        }
+}
+
+// The neighbor-list distance test, FP32 as in Find_atom_neighbors_gridly
+// (neighbor_list.cu:269-274) with cof = (1/2^32) * L (neighbor_list.cu:623).
+// Fixed order: dr2 = (dx*dx + dy*dy) + dz*dz, every operation rounded.
+static inline float nl_dr2(const uint32_t* a, const uint32_t* b, const float* cof) {
+    float dx = (float)(int32_t)(b[0] - a[0]) * cof[0];
+    float dy = (float)(int32_t)(b[1] - a[1]) * cof[1];
+    float dz = (float)(int32_t)(b[2] - a[2]) * cof[2];
+    float s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return s;
+}
+
+static inline bool is_excluded(int a, int b, const int* es, const int* en, const int* el) {
+    // a < b; exclusion lists are upper triangular and sorted (MD_core.cu:441).
+    for (int k = es[a]; k < es[a] + en[a]; k++)
+        if (el[k] == b) return true;
+    return false;
+}
+
+// a5-a9 as a set: every unordered pair i<j whose FP32 dr2 < (rc+skin)^2,
+// minus exclusions (Delete_Excluded_Atoms_Serial_In_Neighbor_List,
+// neighbor_list.cu:353-403). Pairs written as (i, j), i<j, sorted. Returns
+// the pair count; pairs is filled only when cap is large enough.
+long orc_nl_brute(int n, const uint32_t* u, const float* box, float cutoff, float skin,
+                  const int* es, const int* en, const int* el, int* pairs, long cap) {
+    float cof[3];
+    for (int d = 0; d < 3; d++) cof[d] = (1.0f / 4294967296.0f) * box[d];
+    float cws = cutoff + skin;
+    float rl2 = cws * cws;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    std::vector<std::vector<int>> part(nt);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; i++) {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        for (int j = i + 1; j < n; j++) {
+            if (nl_dr2(u + 3 * i, u + 3 * j, cof) < rl2 && !is_excluded(i, j, es, en, el)) {
+                part[t].push_back(i);
+                part[t].push_back(j);
+            }
+        }
+    }
+    std::vector<std::pair<int, int>> all;
+    for (auto& p : part)
+        for (size_t k = 0; k < p.size(); k += 2) all.push_back({p[k], p[k + 1]});
+    std::sort(all.begin(), all.end());
+    long m = (long)all.size();
+    if (pairs && cap >= m)
+        for (long k = 0; k < m; k++) { pairs[2 * k] = all[k].first; pairs[2 * k + 1] = all[k].second; }
+    return m;
+}
+
+// The reference's own cell-grid search restated: grid sizing
+// (neighbor_list.cu:12-25), 5x5x5 stencil with the small-box rules
+// (:57-142), binning with the (n-N)>>31 mask (:165-180), the j>i filter and
+// FP32 test (:260-281), then exclusion deletion. Returns the multiset of
+// pairs (duplicates would show a stencil bug), sorted.
+long orc_nl_grid(int n, const float* crd_in, const float* box, float cutoff, float skin,
+                 const int* es, const int* en, const int* el, int* pairs, long cap, int* grid_dims) {
+    std::vector<float> crd(crd_in, crd_in + 3 * n);
+    orc_periodic_map(n, crd.data(), box);
+    std::vector<uint32_t> u(3 * n);
+    orc_crd_to_uint(n, crd.data(), box, u.data());
+    float cws = cutoff + skin;
+    float half = 0.5f * cws;
+    int N[3];
+    float glen_inv[3];
+    for (int d = 0; d < 3; d++) {
+        N[d] = (int)floorf(box[d] / half);
+        float gl = (float)box[d] / N[d];
+        glen_inv[d] = (float)(1. / gl);
+    }
+    if (grid_dims) { grid_dims[0] = N[0]; grid_dims[1] = N[1]; grid_dims[2] = N[2]; }
+    int Nxy = N[0] * N[1], G = Nxy * N[2];
+    std::vector<std::vector<int>> bucket(G + 1);
+    for (int i = 0; i < n; i++) {
+        int c[3];
+        for (int d = 0; d < 3; d++) {
+            int v = (int)(crd[3 * i + d] * glen_inv[d]);
+            v = v & ((v - N[d]) >> 31);
+            c[d] = v;
+        }
+        bucket[c[2] * Nxy + c[1] * N[0] + c[0]].push_back(i);
+    }
+    float cof[3];
+    for (int d = 0; d < 3; d++) cof[d] = (1.0f / 4294967296.0f) * box[d];
+    float rl2 = cws * cws;
+    std::vector<std::pair<int, int>> all;
+    for (int g = 0; g < G; g++) {
+        int gz = g / Nxy, gy = (g - Nxy * gz) / N[0], gx = g - Nxy * gz - N[0] * gy;
+        int gc[3] = {gx, gy, gz};
+        std::vector<int> sten;
+        for (int l = -2; l <= 2; l++)
+            for (int m = -2; m <= 2; m++)
+                for (int k = -2; k <= 2; k++) {
+                    int off[3] = {l, m, k};
+                    int cc[3];
+                    bool out = false;
+                    for (int d = 0; d < 3; d++) {
+                        int v = gc[d] + off[d];
+                        bool keep = N[d] >= 5 || (v >= 0 && v < N[d]) ||
+                                    (N[d] == 4 && ((gc[d] == 0 && off[d] == -1) || (gc[d] == 3 && off[d] == 1)));
+                        if (!keep) out = true;
+                        if (!out) {
+                            if (v < 0) v += N[d];
+                            else if (v >= N[d]) v -= N[d];
+                        }
+                        cc[d] = v;
+                    }
+                    sten.push_back(out ? G : cc[2] * Nxy + cc[1] * N[0] + cc[0]);
+                }
+        std::sort(sten.begin(), sten.end());
+        for (int s : sten)
+            for (int a : bucket[g])
+                for (int b : bucket[s])
+                    if (b > a && nl_dr2(&u[3 * a], &u[3 * b], cof) < rl2) all.push_back({a, b});
+    }
+    std::vector<std::pair<int, int>> kept;
+    for (auto& p : all)
+        if (!is_excluded(p.first, p.second, es, en, el)) kept.push_back(p);
+    std::sort(kept.begin(), kept.end());
+    long m = (long)kept.size();
+    if (pairs && cap >= m)
+        for (long k = 0; k < m; k++) { pairs[2 * k] = kept[k].first; pairs[2 * k + 1] = kept[k].second; }
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// a13 LJ + PME direct pair terms (Lennard_Jones_force.cu:424-562), FP64, over
+// a half list of pairs (i<j). The per-atom arrays follow the reference's
+// half-list convention: every pair term is credited to atom i (the lower
+// index). dr = r_j - r_i from the wrap-around uint difference (common.cu:89).
+// Stored A, B are x12 / x6 (Lennard_Jones_force.cu:1092,1097).
+//   F_i += (f_LJ - f_C) dr, F_j -= (f_LJ - f_C) dr
+//   f_LJ = (-A r^-6 + B) r^-8,  f_C = q_i q_j (2/sqrt(pi) b r e^{-b^2r^2} + erfc(b r)) r^-3
+//   E_LJ = A/12 r^-12 - B/6 r^-6, E_C = q_i q_j erfc(b r)/r, W_LJ = -f_LJ r^2
+// totals[0]=E_LJ, [1]=E_direct, [2]=W_LJ, [3]=number of pairs inside rc.
+static inline int pair_index(int a, int b) {
+    int hi = a > b ? a : b, lo = a > b ? b : a;
+    return hi * (hi + 1) / 2 + lo;
+}
+
+void orc_direct(int n, const uint32_t* u, const float* q, const int* type, const float* A,
+                const float* B, const float* box, double rc, double beta, long npairs,
+                const int* pairs, double* frc, double* atom_e, double* atom_vir,
+                double* atom_dir, double* totals) {
+    double cof[3];
+    for (int d = 0; d < 3; d++) cof[d] = (double)box[d] / 4294967296.0;
+    const double two_over_sqrtpi = 2.0 / std::sqrt(PI_D);
+    double e_lj = 0, e_c = 0, w_lj = 0, cnt = 0;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    std::vector<std::vector<double>> fpart(nt, std::vector<double>(3 * (size_t)n, 0.0));
+    std::vector<std::vector<double>> epart(nt, std::vector<double>(3 * (size_t)n, 0.0));
+#pragma omp parallel for reduction(+ : e_lj, e_c, w_lj, cnt) schedule(static)
+    for (long k = 0; k < npairs; k++) {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        int i = pairs[2 * k], j = pairs[2 * k + 1];
+        double dr[3];
+        for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+        double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        double r = std::sqrt(r2);
+        if (!(r < rc)) continue;
+        int p = pair_index(type[i], type[j]);
+        double ri2 = 1.0 / r2, ri6 = ri2 * ri2 * ri2, ri8 = ri6 * ri2;
+        double flj = (-(double)A[p] * ri6 + (double)B[p]) * ri8;
+        double br = beta * r;
+        double qq = (double)q[i] * (double)q[j];
+        double erfcv = std::erfc(br);
+        double fc = qq * (two_over_sqrtpi * br * std::exp(-br * br) + erfcv) * ri2 / r;
+        double f = flj - fc;
+        for (int d = 0; d < 3; d++) {
+            fpart[t][3 * (size_t)i + d] += f * dr[d];
+            fpart[t][3 * (size_t)j + d] -= f * dr[d];
+        }
+        double elj = ((double)A[p] / 12.0 * ri6 - (double)B[p] / 6.0) * ri6;
+        double ec = qq * erfcv / r;
+        double wlj = -flj * r2;
+        epart[t][3 * (size_t)i + 0] += elj + ec;
+        epart[t][3 * (size_t)i + 1] += wlj;
+        epart[t][3 * (size_t)i + 2] += ec;
+        e_lj += elj;
+        e_c += ec;
+        w_lj += wlj;
+        cnt += 1;
+    }
+    for (int t = 0; t < nt; t++)
+        for (size_t a = 0; a < (size_t)n; a++) {
+            for (int d = 0; d < 3; d++) frc[3 * a + d] += fpart[t][3 * a + d];
+            if (atom_e) atom_e[a] += epart[t][3 * a + 0];
+            if (atom_vir) atom_vir[a] += epart[t][3 * a + 1];
+            if (atom_dir) atom_dir[a] += epart[t][3 * a + 2];
+        }
+    totals[0] = e_lj;
+    totals[1] = e_c;
+    totals[2] = w_lj;
+    totals[3] = cnt;
+}
+
+// a24 exclusion correction (PME_force.cu:620-695): for every excluded pair,
+// E -= q_i q_j erf(b r)/r, F_i += -q_i q_j (2/sqrt(pi) b r e^{-b^2r^2} + erfc(b r) - 1)/r^3 dr.
+// Energy credited to atom i (the list owner). Returns the total.
+double orc_excluded(int n, const uint32_t* u, const float* q, const float* box, double beta,
+                    const int* es, const int* en, const int* el, double* frc, double* atom_corr) {
+    double cof[3];
+    for (int d = 0; d < 3; d++) cof[d] = (double)box[d] / 4294967296.0;
+    const double two_over_sqrtpi = 2.0 / std::sqrt(PI_D);
+    double tot = 0;
+    for (int i = 0; i < n; i++) {
+        for (int k = es[i]; k < es[i] + en[i]; k++) {
+            int j = el[k];
+            double dr[3];
+            for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+            double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+            double r = std::sqrt(r2);
+            double br = beta * r;
+            double qq = (double)q[i] * (double)q[j];
+            double f = -qq * (two_over_sqrtpi * br * std::exp(-br * br) + std::erfc(br) - 1.0) / r2 / r;
+            for (int d = 0; d < 3; d++) {
+                frc[3 * i + d] += f * dr[d];
+                frc[3 * j + d] -= f * dr[d];
+            }
+            double e = -qq * std::erf(br) / r;
+            tot += e;
+            if (atom_corr) atom_corr[i] += e;
+        }
+    }
+    return tot;
+}
+
+// ---------------------------------------------------------------------------
+// a16 PME setup. Get_Beta (PME_force.cu:71-100) restated in FP64: double the
+// upper bound until erfc(b rc)/rc <= tol, then (iterations+50) bisection
+// steps.
+double orc_pme_beta(double rc, double tol) {
+    double high = 1.0;
+    int ihigh = 1;
+    while (std::erfc(high * rc) / rc > tol) {
+        high *= 2;
+        ihigh++;
+    }
+    ihigh += 50;
+    double low = 0.0, beta = 0.0;
+    for (int ilow = 1; ilow < ihigh; ilow++) {
+        beta = (low + high) / 2;
+        if (std::erfc(beta * rc) / rc >= tol) low = beta;
+        else high = beta;
+    }
+    return beta;
+}
+
+// Cardinal B-spline M_n(u) (PME_force.cu:21-31), recursive definition.
+static double bspline_M(double uu, int n) {
+    if (n == 2) return (uu < 0 || uu > 2) ? 0.0 : 1.0 - std::fabs(uu - 1.0);
+    return uu / (n - 1) * bspline_M(uu, n - 1) + (n - uu) / (n - 1) * bspline_M(uu - 1, n - 1);
+}
+
+// Euler exponential spline modulus |b(k)|^2 (getb, PME_force.cu:44-67;
+// Essmann et al. 1995 eq. 4.4), order 4.
+double orc_bspline_mod2(int k, int K) {
+    const int order = 4;
+    cplx den(0, 0);
+    for (int kk = 0; kk < order - 1; kk++)
+        den += bspline_M(kk + 1, order) * std::exp(cplx(0, 2 * PI_D * k * kk / K));
+    return 1.0 / std::norm(den);
+}
+
+// BC table over the full spectrum (kx, ky, kz all in [0,K)), FP64, folded
+// |m| as in PME_force.cu:238-286:
+//   BC(m) = exp(-pi^2 |m|^2 / b^2) / (pi |m|^2 V) * B1 B2 B3, BC(0) = 0.
+void orc_pme_bc_full(int Kx, int Ky, int Kz, const float* box, double beta, double* bc) {
+    double V = (double)box[0] * box[1] * box[2];
+    std::vector<double> B1(Kx), B2(Ky), B3(Kz);
+    for (int k = 0; k < Kx; k++) B1[k] = orc_bspline_mod2(k, Kx);
+    for (int k = 0; k < Ky; k++) B2[k] = orc_bspline_mod2(k, Ky);
+    for (int k = 0; k < Kz; k++) B3[k] = orc_bspline_mod2(k, Kz);
+    for (int x = 0; x < Kx; x++) {
+        int xr = x > Kx / 2 ? Kx - x : x;
+        for (int y = 0; y < Ky; y++) {
+            int yr = y > Ky / 2 ? Ky - y : y;
+            for (int z = 0; z < Kz; z++) {
+                int zr = z > Kz / 2 ? Kz - z : z;
+                double msq = (double)xr * xr / ((double)box[0] * box[0]) +
+                             (double)yr * yr / ((double)box[1] * box[1]) +
+                             (double)zr * zr / ((double)box[2] * box[2]);
+                size_t idx = ((size_t)x * Ky + y) * Kz + z;
+                if (x == 0 && y == 0 && z == 0) bc[idx] = 0;
+                else bc[idx] = std::exp(-PI_D * PI_D * msq / (beta * beta)) / (PI_D * msq * V) * B1[x] * B2[y] * B3[z];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 mixed-radix FFT (recursive decimation in time; any length, prime
+// factors handled by a direct DFT of that radix). sign=-1 forward.
+static void fft1d_rec(cplx* x, int n, int stride, cplx* out, int sign) {
+    if (n == 1) { out[0] = x[0]; return; }
+    int r = 0;
+    for (int p : {4, 2, 3, 5, 7})
+        if (n % p == 0) { r = p; break; }
+    if (r == 0) {
+        for (int p = 11; p <= n; p++)
+            if (n % p == 0) { r = p; break; }
+    }
+    int m = n / r;
+    std::vector<cplx> sub((size_t)n);
+    for (int q = 0; q < r; q++) fft1d_rec(x + (size_t)q * stride, m, stride * r, sub.data() + (size_t)q * m, sign);
+    for (int k = 0; k < m; k++) {
+        for (int s = 0; s < r; s++) {
+            cplx acc(0, 0);
+            for (int q = 0; q < r; q++) {
+                double ang = sign * 2.0 * PI_D * (double)q * (k + (double)s * m) / n;
+                acc += sub[(size_t)q * m + k] * cplx(std::cos(ang), std::sin(ang));
+            }
+            out[k + (size_t)s * m] = acc;
+        }
+    }
+}
+
+static void fft3d(std::vector<cplx>& a, int Kx, int Ky, int Kz, int sign) {
+    int nmax = std::max(Kx, std::max(Ky, Kz));
+#pragma omp parallel
+    {
+        std::vector<cplx> in(nmax), out(nmax);
+#pragma omp for
+        for (int l = 0; l < Kx * Ky; l++) {
+            cplx* p = a.data() + (size_t)l * Kz;
+            for (int z = 0; z < Kz; z++) in[z] = p[z];
+            fft1d_rec(in.data(), Kz, 1, out.data(), sign);
+            for (int z = 0; z < Kz; z++) p[z] = out[z];
+        }
+#pragma omp for
+        for (int l = 0; l < Kx * Kz; l++) {
+            int x = l / Kz, z = l % Kz;
+            for (int y = 0; y < Ky; y++) in[y] = a[((size_t)x * Ky + y) * Kz + z];
+            fft1d_rec(in.data(), Ky, 1, out.data(), sign);
+            for (int y = 0; y < Ky; y++) a[((size_t)x * Ky + y) * Kz + z] = out[y];
+        }
+#pragma omp for
+        for (int l = 0; l < Ky * Kz; l++) {
+            int y = l / Kz, z = l % Kz;
+            for (int x = 0; x < Kx; x++) in[x] = a[((size_t)x * Ky + y) * Kz + z];
+            fft1d_rec(in.data(), Kx, 1, out.data(), sign);
+            for (int x = 0; x < Kx; x++) a[((size_t)x * Ky + y) * Kz + z] = out[x];
+        }
+    }
+}
+
+// Forward 3D real-to-complex transform of a row-major real grid (z fastest),
+// unnormalised, returning the half spectrum kz in [0, Kz/2] as interleaved
+// (re, im) -- the cufftExecR2C layout the reference relies on
+// (PME_force.cu:159, :741).
+void orc_fft3_r2c(int Kx, int Ky, int Kz, const double* in, double* out) {
+    std::vector<cplx> a((size_t)Kx * Ky * Kz);
+    for (size_t k = 0; k < a.size(); k++) a[k] = cplx(in[k], 0);
+    fft3d(a, Kx, Ky, Kz, -1);
+    int hz = Kz / 2 + 1;
+    for (int x = 0; x < Kx; x++)
+        for (int y = 0; y < Ky; y++)
+            for (int z = 0; z < hz; z++) {
+                cplx v = a[((size_t)x * Ky + y) * Kz + z];
+                size_t o = ((size_t)x * Ky + y) * hz + z;
+                out[2 * o] = v.real();
+                out[2 * o + 1] = v.imag();
+            }
+}
+
+// Inverse (unnormalised) complex-to-real transform from the half spectrum,
+// the cufftExecC2R convention (PME_force.cu:747).
+void orc_fft3_c2r(int Kx, int Ky, int Kz, const double* in, double* out) {
+    int hz = Kz / 2 + 1;
+    std::vector<cplx> a((size_t)Kx * Ky * Kz);
+    for (int x = 0; x < Kx; x++)
+        for (int y = 0; y < Ky; y++)
+            for (int z = 0; z < Kz; z++) {
+                cplx v;
+                if (z < hz) {
+                    size_t o = ((size_t)x * Ky + y) * hz + z;
+                    v = cplx(in[2 * o], in[2 * o + 1]);
+                } else {
+                    int xm = (Kx - x) % Kx, ym = (Ky - y) % Ky, zm = Kz - z;
+                    size_t o = ((size_t)xm * Ky + ym) * hz + zm;
+                    v = std::conj(cplx(in[2 * o], in[2 * o + 1]));
+                }
+                a[((size_t)x * Ky + y) * Kz + z] = v;
+            }
+    fft3d(a, Kx, Ky, Kz, +1);
+    for (size_t k = 0; k < a.size(); k++) out[k] = a[k].real();
+}
+
+// ---------------------------------------------------------------------------
+// a17-a23 SPME reciprocal part, FP64. Fractional grid coordinate
+// t = u K / 2^32 (PME_Atom_Near, PME_force.cu:373-386, exact in FP64);
+// grid point (base - k) mod K gets weight M4(frac + k) (PME_Q_Spread :428-471);
+// convolution with BC over the full spectrum; E = 1/2 sum Q (theta*Q)
+// (:753-755); F = -q sum (theta*Q) dM4 (K/L) (PME_Final :485-550).
+// Returns E_rec; frc += reciprocal forces; optional outputs: the charge grid
+// Q and the convolved grid (theta*Q).
+double orc_pme_reciprocal(int n, const uint32_t* u, const float* q, const float* box, int Kx,
+                          int Ky, int Kz, double beta, double* frc, double* Q_out,
+                          double* conv_out) {
+    int K[3] = {Kx, Ky, Kz};
+    size_t NG = (size_t)Kx * Ky * Kz;
+    std::vector<double> Q(NG, 0.0);
+    std::vector<int> base(3 * (size_t)n);
+    std::vector<double> frac(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            double t = (double)u[3 * i + d] * (double)K[d] / 4294967296.0;
+            double b = std::floor(t);
+            base[3 * i + d] = (int)b;
+            frac[3 * i + d] = t - b;
+        }
+    auto w4 = [](double f, int k) { return bspline_M(f + k, 4); };
+    auto dw4 = [](double f, int k) {
+        double uu = f + k;
+        return bspline_M(uu, 3) - bspline_M(uu - 1, 3);   // d/du M_n = M_{n-1}(u) - M_{n-1}(u-1)
+    };
+    for (int i = 0; i < n; i++) {
+        double wx[4], wy[4], wz[4];
+        for (int k = 0; k < 4; k++) {
+            wx[k] = w4(frac[3 * i], k);
+            wy[k] = w4(frac[3 * i + 1], k);
+            wz[k] = w4(frac[3 * i + 2], k);
+        }
+        for (int a = 0; a < 4; a++) {
+            int gx = ((base[3 * i] - a) % Kx + Kx) % Kx;
+            for (int b = 0; b < 4; b++) {
+                int gy = ((base[3 * i + 1] - b) % Ky + Ky) % Ky;
+                for (int c = 0; c < 4; c++) {
+                    int gz = ((base[3 * i + 2] - c) % Kz + Kz) % Kz;
+                    Q[((size_t)gx * Ky + gy) * Kz + gz] += (double)q[i] * wx[a] * wy[b] * wz[c];
+                }
+            }
+        }
+    }
+    std::vector<double> bc(NG);
+    orc_pme_bc_full(Kx, Ky, Kz, box, beta, bc.data());
+    std::vector<cplx> a(NG);
+    for (size_t k = 0; k < NG; k++) a[k] = cplx(Q[k], 0);
+    fft3d(a, Kx, Ky, Kz, -1);
+    for (size_t k = 0; k < NG; k++) a[k] *= bc[k];
+    fft3d(a, Kx, Ky, Kz, +1);
+    double E = 0;
+    for (size_t k = 0; k < NG; k++) E += Q[k] * a[k].real();
+    E *= 0.5;
+    if (Q_out) for (size_t k = 0; k < NG; k++) Q_out[k] = Q[k];
+    if (conv_out) for (size_t k = 0; k < NG; k++) conv_out[k] = a[k].real();
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        double wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+        for (int k = 0; k < 4; k++) {
+            wx[k] = w4(frac[3 * i], k); dx[k] = dw4(frac[3 * i], k);
+            wy[k] = w4(frac[3 * i + 1], k); dy[k] = dw4(frac[3 * i + 1], k);
+            wz[k] = w4(frac[3 * i + 2], k); dz[k] = dw4(frac[3 * i + 2], k);
+        }
+        double F[3] = {0, 0, 0};
+        for (int aa = 0; aa < 4; aa++) {
+            int gx = ((base[3 * i] - aa) % Kx + Kx) % Kx;
+            for (int b = 0; b < 4; b++) {
+                int gy = ((base[3 * i + 1] - b) % Ky + Ky) % Ky;
+                for (int c = 0; c < 4; c++) {
+                    int gz = ((base[3 * i + 2] - c) % Kz + Kz) % Kz;
+                    double phi = a[((size_t)gx * Ky + gy) * Kz + gz].real();
+                    F[0] += phi * dx[aa] * wy[b] * wz[c];
+                    F[1] += phi * wx[aa] * dy[b] * wz[c];
+                    F[2] += phi * wx[aa] * wy[b] * dz[c];
+                }
+            }
+        }
+        for (int d = 0; d < 3; d++) frc[3 * i + d] += -(double)q[i] * F[d] * (double)K[d] / (double)box[d];
+    }
+    return E;
+}
+
+// Self and neutralising terms (PME_force.cu:757-762, neutralizing_factor :166-167):
+// E_self = -b/sqrt(pi) sum q^2, E_neutral = -pi/(2 b^2 V) (sum q)^2.
+void orc_pme_self(int n, const float* q, double beta, const float* box, double* out2) {
+    double s2 = 0, s1 = 0;
+    for (int i = 0; i < n; i++) { s2 += (double)q[i] * q[i]; s1 += q[i]; }
+    double V = (double)box[0] * box[1] * box[2];
+    out2[0] = -beta / std::sqrt(PI_D) * s2;
+    out2[1] = -PI_D / (2.0 * beta * beta * V) * s1 * s1;
+}
+
+// a11/a14 long-range dispersion factor (Lennard_Jones_force.cu:1042-1065,
+// :1127): factor = -(2/3) pi sum_i sum_j (B_ij/6) / rc^3 over all ordered
+// pairs including i=j; E_lrc = factor/V, W_lrc = 6 factor/V (:1178-1191).
+double orc_lj_lrc_factor(int n, const int* type, int ntypes, const float* B, double rc) {
+    std::vector<double> count(ntypes, 0.0);
+    for (int i = 0; i < n; i++) count[type[i]] += 1;
+    double s = 0;
+    for (int a = 0; a < ntypes; a++)
+        for (int b = 0; b < ntypes; b++) s += count[a] * count[b] * (double)B[pair_index(a, b)];
+    return s * (-2.0 / 3.0) * PI_D / (rc * rc * rc) / 6.0;
+}
+
+// Exact Ewald sum (FP64) for tiny periodic systems of point charges with no
+// exclusions, used only to pin the SPME restatement: real space over all
+// images within rreal with erfc screening, reciprocal space over all |k| <= kmax with the same 1/(pi V m^2) convention,
+// plus self and neutralising terms. Returns the total; frc += forces.
+double orc_ewald_exact(int n, const double* xyz, const double* q, const float* box, double beta,
+                       double rreal, int kmax, double* frc) {
+    double L[3] = {box[0], box[1], box[2]};
+    double V = L[0] * L[1] * L[2];
+    double E = 0;
+    int ni[3];
+    for (int d = 0; d < 3; d++) ni[d] = (int)std::ceil(rreal / L[d]) + 1;
+    const double tsp = 2.0 / std::sqrt(PI_D);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int a = -ni[0]; a <= ni[0]; a++)
+                for (int b = -ni[1]; b <= ni[1]; b++)
+                    for (int c = -ni[2]; c <= ni[2]; c++) {
+                        if (i == j && a == 0 && b == 0 && c == 0) continue;
+                        double dr[3] = {xyz[3 * j] - xyz[3 * i] + a * L[0], xyz[3 * j + 1] - xyz[3 * i + 1] + b * L[1],
+                                        xyz[3 * j + 2] - xyz[3 * i + 2] + c * L[2]};
+                        double r = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+                        if (r > rreal) continue;
+                        double qq = q[i] * q[j];
+                        E += 0.5 * qq * std::erfc(beta * r) / r;
+                        double f = -qq * (tsp * beta * r * std::exp(-beta * beta * r * r) + std::erfc(beta * r)) / (r * r * r);
+                        for (int d = 0; d < 3; d++) frc[3 * i + d] += f * dr[d];
+                    }
+    for (int mx = -kmax; mx <= kmax; mx++)
+        for (int my = -kmax; my <= kmax; my++)
+            for (int mz = -kmax; mz <= kmax; mz++) {
+                if (mx == 0 && my == 0 && mz == 0) continue;
+                if (mx * mx + my * my + mz * mz > kmax * kmax) continue;
+                double m[3] = {mx / L[0], my / L[1], mz / L[2]};
+                double m2 = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+                double pref = std::exp(-PI_D * PI_D * m2 / (beta * beta)) / (PI_D * V * m2);
+                double sr = 0, si = 0;
+                for (int i = 0; i < n; i++) {
+                    double ph = 2 * PI_D * (m[0] * xyz[3 * i] + m[1] * xyz[3 * i + 1] + m[2] * xyz[3 * i + 2]);
+                    sr += q[i] * std::cos(ph);
+                    si += q[i] * std::sin(ph);
+                }
+                E += 0.5 * pref * (sr * sr + si * si);
+                for (int i = 0; i < n; i++) {
+                    double ph = 2 * PI_D * (m[0] * xyz[3 * i] + m[1] * xyz[3 * i + 1] + m[2] * xyz[3 * i + 2]);
+                    // dE/dr_i = pref * q_i * 2 pi m (-sin(ph) sr + cos(ph) si)
+                    double g = pref * q[i] * 2 * PI_D * (-std::sin(ph) * sr + std::cos(ph) * si);
+                    for (int d = 0; d < 3; d++) frc[3 * i + d] -= g * m[d];
+                }
+            }
+    double s2 = 0, s1 = 0;
+    for (int i = 0; i < n; i++) { s2 += q[i] * q[i]; s1 += q[i]; }
+    E += -beta / std::sqrt(PI_D) * s2 - PI_D / (2 * beta * beta * V) * s1 * s1;
+    return E;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

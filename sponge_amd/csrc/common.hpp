@@ -1,0 +1,201 @@
+// Internal definitions shared by the HIP translation units of libsponge_nb.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/sponge_nb.h"
+
+namespace snb {
+
+void set_error(const char* fmt, ...);
+
+#define SNB_HIP(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ::snb::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                             __FILE__, __LINE__);                                  \
+            return SPONGE_NB_EHIP;                                                 \
+        }                                                                          \
+    } while (0)
+
+#define SNB_CHECK_LAUNCH() SNB_HIP(hipGetLastError())
+
+#define SNB_ARG(cond, msg)                   \
+    do {                                         \
+        if (!(cond)) {                           \
+            ::snb::set_error("bad argument: %s", msg); \
+            return SPONGE_NB_EARG;               \
+        }                                        \
+    } while (0)
+
+constexpr int kWave = 64;
+constexpr int kClusterSize = 4;          // atoms per cluster (4x4 cluster pairs, 4 per wave64)
+constexpr int kMaxLjTypes = 64;          // LDS pair table holds kMaxLjTypes^2 (A,B)
+constexpr int kNumXcd = 8;               // MI355X: 8 XCDs, workgroups dealt round-robin
+
+static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+static inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// blockIdx -> logical block so that each XCD gets one contiguous range of
+// logical blocks (workgroup b is dispatched to XCD b % 8). Launch with
+// gridDim.x a multiple of 8; returns -1 for padding blocks.
+__device__ __forceinline__ int xcd_block(int nlogical) {
+    int per = gridDim.x / kNumXcd;
+    int l = (blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
+    return l < nlogical ? l : -1;
+}
+static inline int xcd_grid(int nlogical) { return ceil_div(nlogical, kNumXcd) * kNumXcd; }
+
+// ---- wave64 lane exchange ------------------------------------------------
+// DPP quad permutations (within groups of 4 lanes).
+__device__ __forceinline__ float dpp_quad_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_quad_xor2(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+// Rotate right within each row of 16 lanes.
+template <int N>
+__device__ __forceinline__ float dpp_row_ror(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + N, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wave_sum(float v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 8);
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+// Block-wide FP64 sum; every thread must call it. Result valid in thread 0.
+template <int NT>
+__device__ double block_sum(double v, double* smem /* NT/64 */) {
+    v = wave_sum(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) smem[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NT / 64; k++) r += smem[k];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace snb
+
+// ---- module objects --------------------------------------------------------
+// Cluster-ordered per-step buffers of one pair-kernel user.
+struct PairScratch {
+    int cap;                     // slots (the list's slot capacity)
+    uint4* pk;                   // packed (ux, uy, uz, q)
+    int* pty;                    // LJ type
+    float* fs;                   // 3 * cap SoA force accumulators
+    float* es;                   // 3 * cap: energy, LJ virial, direct energy
+    double* acc;                 // SPONGE_NB_NPARTS FP64 totals
+    hipEvent_t ev_start, ev_stop;  // optional, recorded around the pair kernel
+};
+
+struct sponge_nl {
+    int n;
+    float box[3];
+    float cutoff, skin;
+    int refresh_interval, refresh_count;
+    int max_atom_in_grid_numbers, max_neighbor_numbers;
+    float skin_permit;          // internal value (already doubled like neighbor_list.cu:610)
+    float rl2;                  // (cutoff+skin)^2 in FP32
+    float half_skin_square;     // 0.25 skin^2
+    float cof_q[3];             // 0.25f * 2^32 / L
+    float cof_dr[3];            // (1/2^32) * L
+    // column grid used to form clusters of 4 z-sorted atoms
+    int ncx, ncy, ncol;
+    int col_cap;                // max atoms per column
+    int slot_cap;               // max padded slots (multiple of 4)
+    int ncl_cap;                // slot_cap / 4
+    int list_cap;               // max cluster-pair entries per i-cluster
+    int cand_cap;               // max candidate j-clusters per i-cluster
+    // device buffers
+    float* old_crd;             // 3n
+    unsigned int* nl_uint;      // 3n, uint coordinates at the last rebuild
+    int* col_count;             // ncol
+    int* col_start;             // ncol+1 (slots)
+    int* col_atoms;             // ncol * col_cap
+    int* s2o;                   // slot_cap: slot -> atom (-1 = padding)
+    int* o2s;                   // n: atom -> slot
+    float4* cl_lo;              // ncl_cap bounding boxes (Angstrom)
+    float4* cl_hi;
+    int* list_n;                // ncl_cap
+    uint2* list;                // ncl_cap * list_cap : (j cluster, 16-bit pair mask)
+    uint4* pack;                // slot_cap sorted (ux, uy, uz, atom) at the last rebuild
+    int* flags;                 // [0] rebuild flag, [1] overflow bits, [2] rebuild count,
+                                // [3] slot count, [4] cluster count, [7] skin-check vote,
+                                // [8..9] listed pair count (64-bit)
+    int device;
+};
+
+struct sponge_lj {
+    int n, ntypes;
+    float cutoff;
+    float box[3];
+    float cof_dr[3];
+    float volume;
+    double lrc_factor;           // -(2/3) pi sum_ij B_ij/6 / rc^3
+    float* h_A;                  // triangular, x12
+    float* h_B;                  // triangular, x6
+    int* h_type;
+    float2* d_ab;                // ntypes^2 (A, B) square table
+    int* d_type;                 // n
+    PairScratch ps;              // sized from the neighbor list on first use
+};
+
+struct sponge_pme {
+    int n;
+    float box[3];
+    float cutoff, tolerance, beta;
+    int K[3];
+    int nfft;                    // K0*K1*(K2/2+1)
+    float* d_bc;                 // nfft
+    float2* d_tw[3];             // twiddles exp(-2 pi i j / K) per axis
+    int radix[3][16];
+    int npass[3];
+    float* Q;                    // K^3 charge grid (kept zero between steps)
+    float2* FQ;                  // nfft
+    float* R;                    // K^3 convolved grid
+    int4* base;                  // n grid base indices
+    float4* frac;                // n fractional parts
+    float* corr_atom;            // n correction energies
+    double* acc;                 // SPONGE_NB_NPARTS
+    double neutralizing_factor;  // -pi / (2 beta^2 V)
+    PairScratch ps;              // direct-energy query (Get_Energy DIRECT part)
+    int* zero_type;              // n zeros: every atom of LJ type 0
+    float2* zero_ab;             // one (A, B) = (0, 0) entry
+};
+
+// Internal entry points shared between translation units.
+int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
+              const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
+              hipStream_t s);
+int snb_pairs_scatter(const sponge_nl* nl, const PairScratch* ps, float* d_frc, float* atom_e, float* atom_v,
+                      float* atom_d, hipStream_t s);
+void snb_scratch_free(PairScratch* ps);
+int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float scaler[3], const float* d_charge,
+                 const int* es, const int* el, const int* en, float* d_frc, float* d_atom_energy, hipStream_t s);
+int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
+                       int energy, hipStream_t s);
+int snb_pme_finalize(sponge_pme* pme, const float* d_direct_atom_energy, int need_energy, int need_virial,
+                     float* d_potential, float* d_virial, hipStream_t s);
+int snb_fill_zero_f64(double* p, int n, hipStream_t s);

@@ -1,0 +1,565 @@
+// Neighbor-list rebuild for gfx950: NEIGHBOR_LIST (neighbor_list/neighbor_list.cu)
+// re-designed around 4-atom clusters.
+//
+// The reference bins atoms into cells of edge >= (rc+skin)/2 and appends, per
+// atom i, every j>i in the 125 surrounding cells with FP32 dr2 < (rc+skin)^2
+// (neighbor_list.cu:223-283), then deletes excluded partners (:353-403).
+// Here atoms are counting-sorted into xy columns, z-sorted inside each column
+// and cut into clusters of 4; the list holds, per i-cluster, the j-clusters
+// (index >= i-cluster) whose bounding boxes come within rc+skin, each with a
+// 16-bit mask whose bit (jj*4+ii) is set exactly when the reference would
+// list that atom pair: the same FP32 dr2 expression, the same strict
+// comparison, minus exclusions and padding. The set of listed atom pairs is
+// therefore identical to the reference's; only the storage differs.
+//
+// Rebuild decision stays on the device (no host sync, no device-side
+// launch): every rebuild kernel reads flags[0] and returns at once when it
+// is 0, so the skin-triggered mode can sit inside a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace snb;
+
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kMaxColumn = 1024;     // bitonic sort capacity per column (LDS)
+constexpr int kPairWaves = 4;
+constexpr int kCandCap = 1024;       // candidate j-clusters per i-cluster (LDS)
+
+// Is_need_refresh_neighbor_list_cuda (neighbor_list.cu:336-351): FP32
+// periodic displacement against the coordinates of the last rebuild.
+__global__ void nl_check_kernel(int n, const float* __restrict__ crd, const float* __restrict__ old_crd,
+                                float bx, float by, float bz, float thresh, int* flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float dx = crd[3 * i] - old_crd[3 * i];
+    float dy = crd[3 * i + 1] - old_crd[3 * i + 1];
+    float dz = crd[3 * i + 2] - old_crd[3 * i + 2];
+    dx = dx - floorf(dx / bx + 0.5f) * bx;
+    dy = dy - floorf(dy / by + 0.5f) * by;
+    dz = dz - floorf(dz / bz + 0.5f) * bz;
+    float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    if (r2 > thresh) atomicOr(flags + 7, 1);
+}
+
+// Latches the rebuild decision into flags[0] and clears per-rebuild counters.
+__global__ void nl_begin_kernel(int forced, int* flags, int* col_count, int ncol) {
+    __shared__ int go;
+    if (threadIdx.x == 0) {
+        int f = forced ? 1 : flags[7];
+        flags[7] = 0;
+        flags[0] = f;
+        go = f;
+        if (f) {
+            flags[1] = 0;
+            flags[8] = 0;
+            flags[9] = 0;
+            flags[2] += 1;
+        }
+    }
+    __syncthreads();
+    if (!go) return;
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) col_count[c] = 0;
+}
+
+// Crd_Periodic_Map (common.cu:299-352), old_crd copy (neighbor_list.cu:474)
+// and Crd_To_Uint_Crd with the list's own cof (common.cu:264-287,
+// neighbor_list.cu:622), then the column of each atom from its uint x, y.
+__global__ void nl_bin_kernel(int n, float* __restrict__ crd, float* __restrict__ old_crd,
+                              unsigned int* __restrict__ nl_uint, float bx, float by, float bz,
+                              float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap,
+                              int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags) {
+    if (!flags[0]) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float b[3] = {bx, by, bz};
+    float cq[3] = {cqx, cqy, cqz};
+    unsigned int u[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        float x = crd[3 * i + d];
+        if (x >= 0.0f) {
+            if (!(x < b[d])) x = x - b[d];
+        } else {
+            x = x + b[d];
+        }
+        crd[3 * i + d] = x;
+        old_crd[3 * i + d] = x;
+        int t = (int)__fmul_rn(x, cq[d]);
+        u[d] = ((unsigned int)t) << 2;
+        nl_uint[3 * i + d] = u[d];
+    }
+    int cx = (int)(((unsigned long long)u[0] * (unsigned)ncx) >> 32);
+    int cy = (int)(((unsigned long long)u[1] * (unsigned)ncy) >> 32);
+    int c = cx * ncy + cy;
+    int s = atomicAdd(col_count + c, 1);
+    if (s < col_cap) col_atoms[(size_t)c * col_cap + s] = i;
+    else atomicOr(flags + 1, SPONGE_NL_OVERFLOW_COLUMN);
+}
+
+// Exclusive scan of the padded (multiple of 4) column sizes; one block.
+__global__ void nl_scan_kernel(int ncol, int col_cap, const int* __restrict__ col_count,
+                               int* __restrict__ col_start, int* flags) {
+    if (!flags[0]) return;
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int base = 0; base < ncol; base += blockDim.x) {
+        int c = base + threadIdx.x;
+        int v = 0;
+        if (c < ncol) {
+            int k = min(col_count[c], col_cap);
+            v = (k + 3) & ~3;
+        }
+        int x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        int off = carry;
+        for (int k = 0; k < w; k++) off += wsum[k];
+        if (c < ncol) col_start[c] = off + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int k = 0; k < nw; k++) t += wsum[k];
+            carry += t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        col_start[ncol] = carry;
+        flags[3] = carry;
+        flags[4] = carry / kClusterSize;
+    }
+}
+
+// One block per column: sort its atoms by (uint z, atom index) -- a
+// deterministic order -- write the slot maps, the sorted uint coordinates and
+// the bounding box of every 4-atom cluster.
+__global__ __launch_bounds__(kSortThreads) void nl_sort_kernel(
+    int col_cap, const int* __restrict__ col_count, const int* __restrict__ col_start,
+    const int* __restrict__ col_atoms, const unsigned int* __restrict__ nl_uint,
+    float crx, float cry, float crz, int* __restrict__ s2o, int* __restrict__ o2s,
+    uint4* __restrict__ pack, float4* __restrict__ cl_lo, float4* __restrict__ cl_hi, const int* flags) {
+    if (!flags[0]) return;
+    __shared__ unsigned long long key[kMaxColumn];
+    int c = blockIdx.x;
+    int cnt = min(col_count[c], col_cap);
+    int start = col_start[c];
+    int pad = (cnt + 3) & ~3;
+    int P = 1;
+    while (P < cnt) P <<= 1;
+    for (int k = threadIdx.x; k < P; k += blockDim.x) {
+        if (k < cnt) {
+            int a = col_atoms[(size_t)c * col_cap + k];
+            key[k] = ((unsigned long long)nl_uint[3 * a + 2] << 32) | (unsigned)a;
+        } else {
+            key[k] = ~0ull;
+        }
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < P; k += blockDim.x) {
+                int partner = k ^ stride;
+                if (partner > k) {
+                    bool up = (k & size) == 0;
+                    unsigned long long a = key[k], b = key[partner];
+                    if ((a > b) == up) {
+                        key[k] = b;
+                        key[partner] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < pad; k += blockDim.x) {
+        int slot = start + k;
+        if (k < cnt) {
+            int a = (int)(key[k] & 0xffffffffu);
+            s2o[slot] = a;
+            o2s[a] = slot;
+            pack[slot] = make_uint4(nl_uint[3 * a], nl_uint[3 * a + 1], nl_uint[3 * a + 2], (unsigned)a);
+        } else {
+            s2o[slot] = -1;
+            pack[slot] = make_uint4(0u, 0u, 0u, 0xffffffffu);
+        }
+    }
+    __syncthreads();
+    int ncl = pad / kClusterSize;
+    for (int k = threadIdx.x; k < ncl; k += blockDim.x) {
+        float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+        for (int m = 0; m < kClusterSize; m++) {
+            int kk = k * kClusterSize + m;
+            if (kk >= cnt) break;
+            int a = (int)(key[kk] & 0xffffffffu);
+            float x[3] = {(float)nl_uint[3 * a] * crx, (float)nl_uint[3 * a + 1] * cry,
+                          (float)nl_uint[3 * a + 2] * crz};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                lo[d] = fminf(lo[d], x[d]);
+                hi[d] = fmaxf(hi[d], x[d]);
+            }
+        }
+        int cid = start / kClusterSize + k;
+        cl_lo[cid] = make_float4(lo[0], lo[1], lo[2], 0.f);
+        cl_hi[cid] = make_float4(hi[0], hi[1], hi[2], 0.f);
+    }
+}
+
+// FP32 neighbor-list distance, the reference's expression with every
+// operation rounded in a fixed order (no contraction): must agree bit for
+// bit with the oracle.
+__device__ __forceinline__ float nl_dr2(uint4 a, uint4 b, float cx, float cy, float cz) {
+    float dx = __fmul_rn((float)(int)(b.x - a.x), cx);
+    float dy = __fmul_rn((float)(int)(b.y - a.y), cy);
+    float dz = __fmul_rn((float)(int)(b.z - a.z), cz);
+    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+}
+
+__device__ __forceinline__ bool excluded_pair(int a, int b, const int* __restrict__ es,
+                                              const int* __restrict__ en, const int* __restrict__ el) {
+    int lo = min(a, b), hi = max(a, b);
+    int s = es[lo], e = s + en[lo];
+    for (int k = s; k < e; k++)
+        if (el[k] == hi) return true;
+    return false;
+}
+
+// One wave per i-cluster: gather candidate j-clusters (index >= i) from the
+// columns within rc+skin, test bounding boxes, then compute the exact 16-bit
+// pair masks four cluster pairs at a time and append the non-empty ones.
+__global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
+    int ncx, int ncy, float bx, float by, float bz, float crx, float cry, float crz, float rl2,
+    float rlm, const int* __restrict__ col_start, const float4* __restrict__ cl_lo,
+    const float4* __restrict__ cl_hi, const uint4* __restrict__ pack, const int* __restrict__ es,
+    const int* __restrict__ en, const int* __restrict__ el, int list_cap, int* __restrict__ list_n,
+    uint2* __restrict__ list, int* flags) {
+    if (!flags[0]) return;
+    __shared__ int cand[kPairWaves][kCandCap];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lb = xcd_block(gridDim.x);
+    const int ic = lb * kPairWaves + w;
+    const int ncl = flags[4];
+    if (lb < 0 || ic >= ncl) return;
+    const float4 li = cl_lo[ic], hi = cl_hi[ic];
+    const float cix = 0.5f * (li.x + hi.x), ciy = 0.5f * (li.y + hi.y), ciz = 0.5f * (li.z + hi.z);
+    const float hix = 0.5f * (hi.x - li.x), hiy = 0.5f * (hi.y - li.y), hiz = 0.5f * (hi.z - li.z);
+    const float cwx = bx / ncx, cwy = by / ncy;
+    int ax0 = (int)floorf((li.x - rlm) / cwx), ax1 = (int)floorf((hi.x + rlm) / cwx);
+    int ay0 = (int)floorf((li.y - rlm) / cwy), ay1 = (int)floorf((hi.y + rlm) / cwy);
+    int nxr = ax1 - ax0 + 1, nyr = ay1 - ay0 + 1;
+    if (nxr >= ncx) { ax0 = 0; nxr = ncx; }
+    if (nyr >= ncy) { ay0 = 0; nyr = ncy; }
+    const float rlm2 = rlm * rlm;
+    int ncand = 0;
+    bool over = false;
+    for (int ox = 0; ox < nxr; ox++) {
+        int cx = ((ax0 + ox) % ncx + ncx) % ncx;
+        for (int oy = 0; oy < nyr; oy++) {
+            int cy = ((ay0 + oy) % ncy + ncy) % ncy;
+            int c = cx * ncy + cy;
+            int j0 = col_start[c] / kClusterSize, j1 = col_start[c + 1] / kClusterSize;
+            for (int jb = j0; jb < j1; jb += 64) {
+                int jc = jb + lane;
+                bool ok = false;
+                if (jc < j1 && jc >= ic) {
+                    float4 lj = cl_lo[jc], hj = cl_hi[jc];
+                    float dx = 0.5f * (lj.x + hj.x) - cix, dy = 0.5f * (lj.y + hj.y) - ciy,
+                          dz = 0.5f * (lj.z + hj.z) - ciz;
+                    dx -= bx * rintf(dx / bx);
+                    dy -= by * rintf(dy / by);
+                    dz -= bz * rintf(dz / bz);
+                    float gx = fmaxf(fabsf(dx) - hix - 0.5f * (hj.x - lj.x), 0.f);
+                    float gy = fmaxf(fabsf(dy) - hiy - 0.5f * (hj.y - lj.y), 0.f);
+                    float gz = fmaxf(fabsf(dz) - hiz - 0.5f * (hj.z - lj.z), 0.f);
+                    ok = gx * gx + gy * gy + gz * gz < rlm2;
+                }
+                unsigned long long bal = __ballot(ok);
+                int pos = ncand + __popcll(bal & ((1ull << lane) - 1ull));
+                if (ok) {
+                    if (pos < kCandCap) cand[w][pos] = jc;
+                    else over = true;
+                }
+                ncand += __popcll(bal);
+            }
+        }
+    }
+    if (ncand > kCandCap) {
+        ncand = kCandCap;
+        over = true;
+    }
+    if (over && lane == 0) atomicOr(flags + 1, SPONGE_NL_OVERFLOW_CANDIDATES);
+    // cand[w] is private to this wave: order its LDS writes before the reads.
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+
+    const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4;
+    const uint4 pi = pack[ic * kClusterSize + ii];
+    const int ai = (int)pi.w;
+    int count = 0;
+    long npairs = 0;
+    for (int cb = 0; cb < ncand; cb += 4) {
+        int ci = cb + g;
+        int jc = ci < ncand ? cand[w][ci] : -1;
+        bool bit = false;
+        if (jc >= 0) {
+            uint4 pj = pack[jc * kClusterSize + jj];
+            int aj = (int)pj.w;
+            bool valid = ai >= 0 && aj >= 0 && (jc != ic || jj > ii);
+            if (valid && nl_dr2(pi, pj, crx, cry, crz) < rl2) bit = !excluded_pair(ai, aj, es, en, el);
+        }
+        unsigned long long bal = __ballot(bit);
+        unsigned m = (unsigned)(bal >> (g * 16)) & 0xffffu;
+        bool leader = (lane & 15) == 0 && m != 0;
+        unsigned long long lb2 = __ballot(leader);
+        int slot = count + __popcll(lb2 & ((1ull << lane) - 1ull));
+        if (leader) {
+            if (slot < list_cap) list[(size_t)ic * list_cap + slot] = make_uint2((unsigned)jc, m);
+        }
+        count += __popcll(lb2);
+        npairs += __popcll(bal);
+    }
+    if (lane == 0) {
+        if (count > list_cap) {
+            atomicOr(flags + 1, SPONGE_NL_OVERFLOW_LIST);
+            count = list_cap;
+        }
+        list_n[ic] = count;
+        atomicAdd((unsigned long long*)(flags + 8), (unsigned long long)npairs);
+    }
+}
+
+__global__ void nl_uint_kernel(int n, const float* __restrict__ crd, float cqx, float cqy, float cqz,
+                               unsigned int* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float cq[3] = {cqx, cqy, cqz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int t = (int)__fmul_rn(crd[3 * i + d], cq[d]);
+        out[3 * i + d] = ((unsigned int)t) << 2;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+static void nl_set_box(sponge_nl* nl, const float* box) {
+    for (int d = 0; d < 3; d++) {
+        nl->box[d] = box[d];
+        nl->cof_q[d] = (0.25f * 4294967296.0f) / box[d];
+        nl->cof_dr[d] = (1.0f / 4294967296.0f) * box[d];
+    }
+}
+
+extern "C" int sponge_crd_to_uint(int atom_numbers, const float box_length[3], const float* d_crd,
+                                  unsigned int* d_uint_crd, void* stream) {
+    SNB_ARG(atom_numbers >= 0 && box_length && d_crd && d_uint_crd, "crd_to_uint");
+    if (atom_numbers == 0) return SPONGE_NB_OK;
+    // MD_core's pbc cof (MD_core.cu:485-490): 0.25 * (2^32 / L), equal to the
+    // list's 0.25*2^32 / L bit for bit (scaling by 0.25 is exact).
+    float cq[3];
+    for (int d = 0; d < 3; d++) cq[d] = 0.25f * (4294967296.0f / box_length[d]);
+    nl_uint_kernel<<<ceil_div(atom_numbers, 256), 256, 0, as_stream(stream)>>>(atom_numbers, d_crd, cq[0], cq[1],
+                                                                              cq[2], d_uint_crd);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], float cutoff, float skin,
+                                int refresh_interval, int max_atom_in_grid_numbers,
+                                int max_neighbor_numbers, float skin_permit, sponge_nl** out) {
+    SNB_ARG(out && box_length, "null pointer");
+    SNB_ARG(atom_numbers > 0, "atom_numbers must be positive");
+    SNB_ARG(cutoff > 0 && skin >= 0, "cutoff/skin");
+    for (int d = 0; d < 3; d++) SNB_ARG(box_length[d] > 0, "box_length");
+    *out = nullptr;
+    sponge_nl* nl = new sponge_nl();
+    memset(nl, 0, sizeof(*nl));
+    nl->n = atom_numbers;
+    nl->cutoff = cutoff;
+    nl->skin = skin;
+    nl->refresh_interval = refresh_interval;
+    nl->max_atom_in_grid_numbers = max_atom_in_grid_numbers > 0 ? max_atom_in_grid_numbers : 64;
+    nl->max_neighbor_numbers = max_neighbor_numbers > 0 ? max_neighbor_numbers : 800;
+    nl->skin_permit = skin_permit > 0 ? 2.0f * skin_permit : 1.0f;
+    float cws = cutoff + skin;
+    nl->rl2 = cws * cws;
+    nl->half_skin_square = (float)(0.25 * skin * skin);
+    nl_set_box(nl, box_length);
+    (void)hipGetDevice(&nl->device);
+
+    double V = (double)box_length[0] * box_length[1] * box_length[2];
+    double dens = atom_numbers / V;
+    double w = std::cbrt((double)kClusterSize / dens);
+    nl->ncx = std::max(1, (int)std::floor(box_length[0] / w));
+    nl->ncy = std::max(1, (int)std::floor(box_length[1] / w));
+    nl->ncol = nl->ncx * nl->ncy;
+    double mean = (double)atom_numbers / nl->ncol;
+    int cap = (int)std::ceil(2.5 * mean + 32);
+    cap = (cap + 63) & ~63;
+    nl->col_cap = std::min(std::max(cap, 64), kMaxColumn);
+    nl->slot_cap = atom_numbers + 3 * nl->ncol;
+    nl->slot_cap = (nl->slot_cap + 3) & ~3;
+    nl->ncl_cap = nl->slot_cap / kClusterSize;
+    double rl = cws;
+    double est = (2.0 / 3.0) * M_PI * std::pow(rl + 2.0 * w, 3) * dens / kClusterSize;
+    int lc = (int)std::ceil(2.0 * est) + 64;
+    nl->list_cap = std::min(std::max(lc, 64), std::min(nl->ncl_cap, kCandCap));
+    nl->cand_cap = kCandCap;
+
+    size_t n = atom_numbers;
+    hipError_t e = hipSuccess;
+    auto A = [&](void** p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, bytes > 0 ? bytes : 4);
+    };
+    A((void**)&nl->old_crd, 3 * n * sizeof(float));
+    A((void**)&nl->nl_uint, 3 * n * sizeof(unsigned));
+    A((void**)&nl->col_count, nl->ncol * sizeof(int));
+    A((void**)&nl->col_start, (nl->ncol + 1) * sizeof(int));
+    A((void**)&nl->col_atoms, (size_t)nl->ncol * nl->col_cap * sizeof(int));
+    A((void**)&nl->s2o, nl->slot_cap * sizeof(int));
+    A((void**)&nl->o2s, n * sizeof(int));
+    A((void**)&nl->cl_lo, nl->ncl_cap * sizeof(float4));
+    A((void**)&nl->cl_hi, nl->ncl_cap * sizeof(float4));
+    A((void**)&nl->list_n, nl->ncl_cap * sizeof(int));
+    A((void**)&nl->list, (size_t)nl->ncl_cap * nl->list_cap * sizeof(uint2));
+    A((void**)&nl->pack, nl->slot_cap * sizeof(uint4));
+    A((void**)&nl->flags, 16 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(nl->flags, 0, 16 * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(nl->list_n, 0, nl->ncl_cap * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(nl->s2o, 0xff, nl->slot_cap * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(nl->col_start, 0, (nl->ncol + 1) * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(nl->old_crd, 0, 3 * n * sizeof(float));
+    if (e != hipSuccess) {
+        set_error("sponge_nl_create: %s", hipGetErrorString(e));
+        sponge_nl_destroy(nl);
+        return SPONGE_NB_EHIP;
+    }
+    *out = nl;
+    return SPONGE_NB_OK;
+}
+
+extern "C" void sponge_nl_destroy(sponge_nl* nl) {
+    if (!nl) return;
+    void* ptrs[] = {nl->old_crd, nl->nl_uint, nl->col_count, nl->col_start, nl->col_atoms, nl->s2o, nl->o2s,
+                    nl->cl_lo,   nl->cl_hi,   nl->list_n,    nl->list,      nl->pack,      nl->flags};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete nl;
+}
+
+// Launches the rebuild pipeline; `forced` latches the decision on the host,
+// otherwise the device flag written by nl_check_kernel decides.
+static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el, const int* en, int forced,
+                      hipStream_t s) {
+    const int n = nl->n;
+    nl_begin_kernel<<<1, 256, 0, s>>>(forced, nl->flags, nl->col_count, nl->ncol);
+    SNB_CHECK_LAUNCH();
+    nl_bin_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_crd, nl->old_crd, nl->nl_uint, nl->box[0], nl->box[1],
+                                                   nl->box[2], nl->cof_q[0], nl->cof_q[1], nl->cof_q[2], nl->ncx,
+                                                   nl->ncy, nl->col_cap, nl->col_count, nl->col_atoms, nl->flags);
+    SNB_CHECK_LAUNCH();
+    nl_scan_kernel<<<1, 1024, 0, s>>>(nl->ncol, nl->col_cap, nl->col_count, nl->col_start, nl->flags);
+    SNB_CHECK_LAUNCH();
+    nl_sort_kernel<<<nl->ncol, kSortThreads, 0, s>>>(nl->col_cap, nl->col_count, nl->col_start, nl->col_atoms,
+                                                      nl->nl_uint, nl->cof_dr[0], nl->cof_dr[1], nl->cof_dr[2],
+                                                      nl->s2o, nl->o2s, nl->pack, nl->cl_lo, nl->cl_hi, nl->flags);
+    SNB_CHECK_LAUNCH();
+    const float rlm = std::sqrt(nl->rl2) + 1.0e-3f * (1.0f + std::sqrt(nl->rl2));
+    int nblk = xcd_grid(ceil_div(nl->ncl_cap, kPairWaves));
+    nl_pairs_kernel<<<nblk, kPairWaves * 64, 0, s>>>(
+        nl->ncx, nl->ncy, nl->box[0], nl->box[1], nl->box[2], nl->cof_dr[0], nl->cof_dr[1], nl->cof_dr[2], nl->rl2,
+        rlm, nl->col_start, nl->cl_lo, nl->cl_hi, nl->pack, es, en, el, nl->list_cap, nl->list_n, nl->list,
+        nl->flags);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_excluded_list_start,
+                                const int* d_excluded_list, const int* d_excluded_numbers, int forced_update,
+                                int forced_check, void* stream) {
+    SNB_ARG(nl && d_crd && d_excluded_list_start && d_excluded_list && d_excluded_numbers, "nl_update");
+    hipStream_t s = as_stream(stream);
+    if (forced_update) {
+        return nl_rebuild(nl, d_crd, d_excluded_list_start, d_excluded_list, d_excluded_numbers, 1, s);
+    }
+    if (nl->refresh_interval > 0 && !forced_check) {
+        int rc = SPONGE_NB_OK;
+        if (nl->refresh_count % nl->refresh_interval == 0)
+            rc = nl_rebuild(nl, d_crd, d_excluded_list_start, d_excluded_list, d_excluded_numbers, 1, s);
+        nl->refresh_count += 1;
+        return rc;
+    }
+    float thresh = nl->skin_permit * nl->skin_permit * nl->half_skin_square;
+    nl_check_kernel<<<ceil_div(nl->n, 256), 256, 0, s>>>(nl->n, d_crd, nl->old_crd, nl->box[0], nl->box[1],
+                                                         nl->box[2], thresh, nl->flags);
+    SNB_CHECK_LAUNCH();
+    return nl_rebuild(nl, d_crd, d_excluded_list_start, d_excluded_list, d_excluded_numbers, 0, s);
+}
+
+extern "C" int sponge_nl_update_volume(sponge_nl* nl, const float box_length[3]) {
+    SNB_ARG(nl && box_length, "nl_update_volume");
+    for (int d = 0; d < 3; d++) SNB_ARG(box_length[d] > 0, "box_length");
+    nl_set_box(nl, box_length);
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_nl_status(sponge_nl* nl, long* pair_count, int* overflow_flags, int* rebuilds) {
+    SNB_ARG(nl, "nl_status");
+    int f[16];
+    SNB_HIP(hipDeviceSynchronize());
+    SNB_HIP(hipMemcpy(f, nl->flags, sizeof(f), hipMemcpyDeviceToHost));
+    unsigned long long pc;
+    memcpy(&pc, f + 8, sizeof(pc));
+    if (pair_count) *pair_count = (long)pc;
+    if (overflow_flags) *overflow_flags = f[1];
+    if (rebuilds) *rebuilds = f[2];
+    return f[1] ? SPONGE_NB_ECAPACITY : SPONGE_NB_OK;
+}
+
+extern "C" int sponge_nl_get_pairs(sponge_nl* nl, int* h_pairs, long capacity, long* pair_count) {
+    SNB_ARG(nl, "nl_get_pairs");
+    int f[16];
+    SNB_HIP(hipDeviceSynchronize());
+    SNB_HIP(hipMemcpy(f, nl->flags, sizeof(f), hipMemcpyDeviceToHost));
+    int nslot = f[3], ncl = f[4];
+    std::vector<int> ln(ncl), s2o(nslot);
+    std::vector<uint2> lst((size_t)ncl * nl->list_cap);
+    if (ncl > 0) {
+        SNB_HIP(hipMemcpy(ln.data(), nl->list_n, ncl * sizeof(int), hipMemcpyDeviceToHost));
+        SNB_HIP(hipMemcpy(s2o.data(), nl->s2o, nslot * sizeof(int), hipMemcpyDeviceToHost));
+        SNB_HIP(hipMemcpy(lst.data(), nl->list, lst.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+    }
+    long m = 0;
+    for (int ic = 0; ic < ncl; ic++)
+        for (int e = 0; e < ln[ic]; e++) {
+            uint2 ent = lst[(size_t)ic * nl->list_cap + e];
+            for (int b = 0; b < 16; b++)
+                if (ent.y >> b & 1u) {
+                    int ii = b & 3, jj = b >> 2;
+                    int a = s2o[ic * kClusterSize + ii], c = s2o[ent.x * kClusterSize + jj];
+                    if (h_pairs && m < capacity) {
+                        h_pairs[2 * m] = std::min(a, c);
+                        h_pairs[2 * m + 1] = std::max(a, c);
+                    }
+                    m++;
+                }
+        }
+    if (pair_count) *pair_count = m;
+    return f[1] ? SPONGE_NB_ECAPACITY : SPONGE_NB_OK;
+}

@@ -1,0 +1,360 @@
+"""Python mirror of SPONGE's three nonbonded modules over the C ABI.
+
+`NEIGHBOR_LIST`, `LENNARD_JONES_INFORMATION` and `Particle_Mesh_Ewald` keep
+the reference's method names and argument meaning
+(neighbor_list/neighbor_list.cuh:83-105, Lennard_Jones_force/
+Lennard_Jones_force.cuh:72-101, PME_force/PME_force.cuh:96-121); every call
+goes straight to libsponge_nb.so (hand-written HIP for gfx950). Device memory
+and streams come from PyTorch; there is no CPU fallback: without the built
+library or a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsponge_nb.so")
+_lib = None
+
+NPARTS = 8
+PART_NAMES = ["lj", "direct", "lj_virial", "correction", "self", "reciprocal", "lrc", "pairs_in"]
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_L = ctypes.c_long
+_D = ctypes.c_double
+_FP3 = ctypes.POINTER(ctypes.c_float)
+
+_SIGS = {
+    "sponge_nb_last_error": (ctypes.c_char_p, []),
+    "sponge_nb_device_info": (_I, [ctypes.POINTER(_I), ctypes.c_char_p, _I]),
+    "sponge_crd_to_uint": (_I, [_I, _FP3, _P, _P, _P]),
+    "sponge_nl_create": (_I, [_I, _FP3, _F, _F, _I, _I, _I, _F, ctypes.POINTER(_P)]),
+    "sponge_nl_destroy": (None, [_P]),
+    "sponge_nl_update": (_I, [_P, _P, _P, _P, _P, _I, _I, _P]),
+    "sponge_nl_update_volume": (_I, [_P, _FP3]),
+    "sponge_nl_status": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "sponge_nl_get_pairs": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
+    "sponge_lj_create": (_I, [_I, _I, _P, _P, _P, _F, _FP3, ctypes.POINTER(_P)]),
+    "sponge_lj_destroy": (None, [_P]),
+    "sponge_lj_pme_direct_force": (_I, [_P, _P, _P, _P, _P, _F, _I, _P, _I, _P, _P, _P]),
+    "sponge_lj_long_range_correction": (_I, [_P, _I, _P, _I, _P, _P]),
+    "sponge_lj_get_energy": (_I, [_P, _P, _P, ctypes.POINTER(_D), _P]),
+    "sponge_lj_update_volume": (_I, [_P, _FP3]),
+    "sponge_lj_host_tables": (_I, [_P, _P, _P, _P]),
+    "sponge_lj_set_pair_events": (_I, [_P, _P, _P]),
+    "sponge_pme_create": (_I, [_I, _FP3, _F, _F, _I, _I, _I, ctypes.POINTER(_P)]),
+    "sponge_pme_destroy": (None, [_P]),
+    "sponge_pme_info": (_I, [_P, ctypes.POINTER(_F), ctypes.POINTER(_I)]),
+    "sponge_pme_excluded_force": (_I, [_P, _P, _FP3, _P, _P, _P, _P, _P, _P, _P]),
+    "sponge_pme_reciprocal_force": (_I, [_P, _P, _P, _P, _I, _I, _P, _P, _P, _P]),
+    "sponge_pme_get_energy": (_I, [_P, _P, _P, _P, _FP3, _P, _P, _P, _I, ctypes.POINTER(_D), _P]),
+    "sponge_pme_update_box_length": (_I, [_P, _FP3]),
+    "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P]),
+    "sponge_sum_list": (_I, [_I, _P, _P, _P]),
+}
+
+
+class SpongeError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libsponge_nb.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SpongeError(f"{path} is missing: run `python -m sponge_amd.build` first "
+                          "(there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = load_library().sponge_nb_last_error().decode(errors="replace")
+        raise SpongeError(f"{what} failed ({rc}): {msg}")
+
+
+def _vec3(v):
+    a = (ctypes.c_float * 3)(*[float(x) for x in v])
+    return a
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise SpongeError("no GPU visible: the HIP path has no CPU fallback")
+    lib = load_library()
+    cnt = _I(0)
+    name = ctypes.create_string_buffer(64)
+    torch.cuda.init()
+    rc = lib.sponge_nb_device_info(ctypes.byref(cnt), name, 64)
+    _check(rc, "device check")
+    return name.value.decode()
+
+
+class NEIGHBOR_LIST:
+    """neighbor_list/neighbor_list.cuh:44-105 over sponge_nl_*."""
+    CONDITIONAL_UPDATE, FORCED_UPDATE = 0, 1
+    CONDITIONAL_CHECK, FORCED_CHECK = 0, 1
+
+    def __init__(self, atom_numbers, box_length, cut, skin, refresh_interval=0,
+                 max_atom_in_grid_numbers=64, max_neighbor_numbers=800, skin_permit=0.5):
+        lib = load_library()
+        h = _P()
+        _check(lib.sponge_nl_create(int(atom_numbers), _vec3(box_length), float(cut), float(skin),
+                                    int(refresh_interval), int(max_atom_in_grid_numbers),
+                                    int(max_neighbor_numbers), float(skin_permit), ctypes.byref(h)),
+               "NEIGHBOR_LIST::Initial")
+        self.h = h
+        self.atom_numbers = int(atom_numbers)
+
+    def Neighbor_List_Update(self, crd, excluded_list_start, excluded_list, excluded_numbers,
+                             forced_update=0, forced_check=0, stream=None):
+        _check(load_library().sponge_nl_update(self.h, _ptr(crd), _ptr(excluded_list_start),
+                                               _ptr(excluded_list), _ptr(excluded_numbers),
+                                               int(forced_update), int(forced_check), _stream(stream)),
+               "Neighbor_List_Update")
+
+    def Update_Volume(self, box_length):
+        _check(load_library().sponge_nl_update_volume(self.h, _vec3(box_length)), "Update_Volume")
+
+    def status(self):
+        pc, fl, rb = _L(0), _I(0), _I(0)
+        rc = load_library().sponge_nl_status(self.h, ctypes.byref(pc), ctypes.byref(fl), ctypes.byref(rb))
+        return dict(pairs=pc.value, overflow=fl.value, rebuilds=rb.value, ok=rc == 0)
+
+    def pairs(self):
+        """(P,2) int32 canonical pairs (i<j), sorted lexicographically."""
+        lib = load_library()
+        n = _L(0)
+        lib.sponge_nl_get_pairs(self.h, None, 0, ctypes.byref(n))
+        out = np.zeros((max(n.value, 1), 2), np.int32)
+        rc = lib.sponge_nl_get_pairs(self.h, out.ctypes.data_as(_P), n.value, ctypes.byref(n))
+        _check(rc, "neighbor list capacity")
+        out = out[:n.value]
+        order = np.lexsort((out[:, 1], out[:, 0]))
+        return out[order]
+
+    def Clear(self):
+        if self.h:
+            load_library().sponge_nl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.Clear()
+        except Exception:
+            pass
+
+
+class LENNARD_JONES_INFORMATION:
+    """Lennard_Jones_force/Lennard_Jones_force.cuh:37-149 over sponge_lj_*."""
+
+    def __init__(self, atom_numbers, atom_type_numbers, h_LJ_A, h_LJ_B, h_atom_LJ_type, cutoff,
+                 box_length):
+        lib = load_library()
+        self._A = np.ascontiguousarray(h_LJ_A, np.float32)
+        self._B = np.ascontiguousarray(h_LJ_B, np.float32)
+        self._t = np.ascontiguousarray(h_atom_LJ_type, np.int32)
+        h = _P()
+        _check(lib.sponge_lj_create(int(atom_numbers), int(atom_type_numbers), self._A.ctypes.data_as(_P),
+                                    self._B.ctypes.data_as(_P), self._t.ctypes.data_as(_P), float(cutoff),
+                                    _vec3(box_length), ctypes.byref(h)),
+               "LENNARD_JONES_INFORMATION::Initial")
+        self.h = h
+        self.cutoff = float(cutoff)
+
+    def LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(self, nl, uint_crd, charge, frc, pme_beta,
+                                                         need_atom_energy=0, atom_energy=None,
+                                                         need_virial=0, atom_lj_virial=None,
+                                                         atom_direct_pme_energy=None, stream=None):
+        _check(load_library().sponge_lj_pme_direct_force(
+            self.h, nl.h, _ptr(uint_crd), _ptr(charge), _ptr(frc), float(pme_beta), int(need_atom_energy),
+            _ptr(atom_energy), int(need_virial), _ptr(atom_lj_virial), _ptr(atom_direct_pme_energy),
+            _stream(stream)), "LJ_PME_Direct_Force_With_Atom_Energy_And_Virial")
+
+    def Long_Range_Correction(self, need_pressure, d_virial, need_potential, d_potential, stream=None):
+        _check(load_library().sponge_lj_long_range_correction(
+            self.h, int(need_pressure), _ptr(d_virial), int(need_potential), _ptr(d_potential),
+            _stream(stream)), "Long_Range_Correction")
+
+    def Get_Energy(self, nl, uint_crd, stream=None):
+        e = _D(0)
+        _check(load_library().sponge_lj_get_energy(self.h, nl.h, _ptr(uint_crd), ctypes.byref(e),
+                                                   _stream(stream)), "LJ Get_Energy")
+        return e.value
+
+    def Update_Volume(self, box_length):
+        _check(load_library().sponge_lj_update_volume(self.h, _vec3(box_length)), "LJ Update_Volume")
+
+    def set_pair_events(self, start=None, stop=None):
+        """Brackets later pair-kernel launches with these torch.cuda.Events."""
+        e0 = ctypes.c_void_p(start.cuda_event) if start is not None else None
+        e1 = ctypes.c_void_p(stop.cuda_event) if stop is not None else None
+        _check(load_library().sponge_lj_set_pair_events(self.h, e0, e1), "set_pair_events")
+
+    def Clear(self):
+        if self.h:
+            load_library().sponge_lj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.Clear()
+        except Exception:
+            pass
+
+
+class Particle_Mesh_Ewald:
+    """PME_force/PME_force.cuh:25-144 over sponge_pme_*."""
+    TOTAL, DIRECT, RECIPROCAL, CORRECTION, SELF = 0, 1, 2, 3, 4
+
+    def __init__(self, atom_numbers, boxlength, cutoff, direct_tolerance=1e-5, fftx=-1, ffty=-1,
+                 fftz=-1):
+        lib = load_library()
+        h = _P()
+        _check(lib.sponge_pme_create(int(atom_numbers), _vec3(boxlength), float(cutoff),
+                                     float(direct_tolerance), int(fftx), int(ffty), int(fftz),
+                                     ctypes.byref(h)), "Particle_Mesh_Ewald::Initial")
+        self.h = h
+        b = _F(0)
+        k = (_I * 3)()
+        lib.sponge_pme_info(h, ctypes.byref(b), k)
+        self.beta = b.value
+        self.fftx, self.ffty, self.fftz = k[0], k[1], k[2]
+
+    def PME_Excluded_Force_With_Atom_Energy(self, uint_crd, scaler, charge, excluded_list_start,
+                                            excluded_list, excluded_atom_numbers, frc, atom_energy=None,
+                                            stream=None):
+        _check(load_library().sponge_pme_excluded_force(
+            self.h, _ptr(uint_crd), _vec3(scaler), _ptr(charge), _ptr(excluded_list_start),
+            _ptr(excluded_list), _ptr(excluded_atom_numbers), _ptr(frc), _ptr(atom_energy),
+            _stream(stream)), "PME_Excluded_Force_With_Atom_Energy")
+
+    def PME_Reciprocal_Force_With_Energy_And_Virial(self, uint_crd, charge, force, need_virial=0,
+                                                    need_energy=0, d_virial=None, d_potential=None,
+                                                    d_direct_atom_energy=None, stream=None):
+        _check(load_library().sponge_pme_reciprocal_force(
+            self.h, _ptr(uint_crd), _ptr(charge), _ptr(force), int(need_virial), int(need_energy),
+            _ptr(d_virial), _ptr(d_potential), _ptr(d_direct_atom_energy), _stream(stream)),
+            "PME_Reciprocal_Force_With_Energy_And_Virial")
+
+    def Get_Energy(self, uint_crd, charge, nl, scaler, excluded_list_start, excluded_list,
+                   excluded_atom_numbers, which_part=0, stream=None):
+        e = _D(0)
+        _check(load_library().sponge_pme_get_energy(
+            self.h, nl.h if nl is not None else None, _ptr(uint_crd), _ptr(charge), _vec3(scaler),
+            _ptr(excluded_list_start), _ptr(excluded_list), _ptr(excluded_atom_numbers), int(which_part),
+            ctypes.byref(e), _stream(stream)), "PME Get_Energy")
+        return e.value
+
+    def Update_Box_Length(self, boxlength):
+        _check(load_library().sponge_pme_update_box_length(self.h, _vec3(boxlength)),
+               "PME Update_Box_Length")
+
+    Update_Volume = Update_Box_Length
+
+    def Clear(self):
+        if self.h:
+            load_library().sponge_pme_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.Clear()
+        except Exception:
+            pass
+
+
+def crd_to_uint(crd, box_length, uint_out, stream=None):
+    """MD_INFORMATION::MD_Information_Crd_To_Uint_Crd (MD_core/MD_core.cu:1528)."""
+    _check(load_library().sponge_crd_to_uint(int(crd.shape[0]), _vec3(box_length), _ptr(crd),
+                                              _ptr(uint_out), _stream(stream)), "Crd_To_Uint_Crd")
+
+
+def sum_list(lst, out, stream=None):
+    _check(load_library().sponge_sum_list(int(lst.numel()), _ptr(lst), _ptr(out), _stream(stream)),
+           "Sum_Of_List")
+
+
+class NonbondedEngine:
+    """The three modules plus the device state MD_INFORMATION owns for them.
+
+    Built from a `sponge_amd.system.System`; `force_step` runs the fused
+    nonbonded force of one MD step (main.cu:180-193) into `self.frc`.
+    """
+
+    def __init__(self, sys_, refresh_interval=20, device="cuda"):
+        import torch
+        require_gpu()
+        self.sys = sys_
+        self.device = device
+        n = sys_.n
+        self.n = n
+        self.box = [float(x) for x in sys_.box]
+        t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+        self.crd = t(sys_.crd, torch.float32).contiguous()
+        self.charge = t(sys_.charge, torch.float32)
+        self.uint_crd = torch.zeros((n, 3), dtype=torch.int32, device=device)
+        self.frc = torch.zeros((n, 3), dtype=torch.float32, device=device)
+        self.atom_energy = torch.zeros(n, dtype=torch.float32, device=device)
+        self.atom_virial = torch.zeros(n, dtype=torch.float32, device=device)
+        self.atom_direct = torch.zeros(n, dtype=torch.float32, device=device)
+        self.parts = torch.zeros(NPARTS, dtype=torch.float64, device=device)
+        self.excl_start = t(sys_.excl_start, torch.int32)
+        self.excl_numbers = t(sys_.excl_numbers, torch.int32)
+        el = sys_.excl_list if len(sys_.excl_list) else np.zeros(1, np.int32)
+        self.excl_list = t(el, torch.int32)
+        self.nl = NEIGHBOR_LIST(n, self.box, sys_.cutoff, sys_.skin, refresh_interval=refresh_interval)
+        self.lj = LENNARD_JONES_INFORMATION(n, sys_.ntypes, sys_.lj_A, sys_.lj_B, sys_.lj_type,
+                                            sys_.cutoff, self.box)
+        self.pme = Particle_Mesh_Ewald(n, self.box, sys_.cutoff, sys_.pme_tolerance, *sys_.fft)
+        self.scaler = [b / 4294967296.0 for b in self.box]
+
+    def update_neighbor_list(self, forced=0, stream=None):
+        self.nl.Neighbor_List_Update(self.crd, self.excl_start, self.excl_list, self.excl_numbers,
+                                     forced_update=forced, stream=stream)
+
+    def crd_to_uint(self, stream=None):
+        crd_to_uint(self.crd, self.box, self.uint_crd, stream)
+
+    def force_step(self, need_energy=0, need_virial=0, stream=None):
+        _check(load_library().sponge_nb_force_step(
+            self.nl.h, self.lj.h, self.pme.h, _ptr(self.uint_crd), _ptr(self.charge),
+            _ptr(self.excl_start), _ptr(self.excl_list), _ptr(self.excl_numbers), _ptr(self.frc),
+            int(need_energy), int(need_virial), _ptr(self.atom_energy), _ptr(self.atom_virial),
+            _ptr(self.parts), _stream(stream)), "nb_force_step")
+
+    def energy_parts(self):
+        import torch
+        torch.cuda.synchronize()
+        p = self.parts.cpu().numpy()
+        return {k: float(v) for k, v in zip(PART_NAMES, p)}

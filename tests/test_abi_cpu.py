@@ -1,0 +1,91 @@
+"""CPU checks of the C ABI library and the host-side helpers (no GPU calls)."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from sponge_amd import system
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "sponge_nb.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sponge_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from sponge_amd import nb
+    path = nb.LIB_PATH
+    if not os.path.exists(path):
+        pytest.skip("libsponge_nb.so not built")
+    lib = ctypes.CDLL(path)
+    names = header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    # the Python mirror binds exactly the header's functions
+    assert sorted(nb.exported_symbols()) == names
+
+
+def test_library_is_gfx950_code_object():
+    from sponge_amd import nb
+    if not os.path.exists(nb.LIB_PATH):
+        pytest.skip("libsponge_nb.so not built")
+    blob = open(nb.LIB_PATH, "rb").read()
+    assert b"amdgcn-amd-amdhsa--gfx950" in blob
+
+
+def test_product_has_no_oracle_dependency():
+    """The shipped package never imports or links the oracle."""
+    pkg = os.path.join(ROOT, "sponge_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+                src = open(os.path.join(dirpath, f), errors="replace").read()
+                assert "import oracle" not in src and "from oracle" not in src, f
+                assert "liboracle" not in src, f
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_water_box_generator():
+    s = system.config(1)
+    assert s.n == 23570 and s.waters == 7850
+    assert (s.crd >= 0).all() and (s.crd < s.box).all()
+    assert abs(float(s.charge.sum())) < 1e-2
+    # TIP3P geometry
+    o, h1, h2 = s.crd[0].astype(float), s.crd[1].astype(float), s.crd[2].astype(float)
+    d = lambda a, b: np.linalg.norm(((a - b + s.box / 2) % s.box) - s.box / 2)
+    assert d(o, h1) == pytest.approx(0.9572, abs=1e-5)
+    assert d(o, h2) == pytest.approx(0.9572, abs=1e-5)
+    # density of water ~1.0 g/cm^3
+    mass = float(s.mass.astype(float).sum()) / 6.02214076e23
+    vol = float(np.prod(s.box.astype(float))) * 1e-24
+    assert mass / vol == pytest.approx(1.0, abs=0.02)
+    # deterministic for a seed
+    s2 = system.config(1)
+    assert np.array_equal(s.crd, s2.crd)
+    # exclusions: O->H1,H2 and H1->H2, upper triangular and sorted
+    assert s.excl_numbers[:3].tolist() == [2, 1, 0]
+    assert s.excl_list[:3].tolist() == [1, 2, 2]
+
+
+def test_lj_tables_lorentz_berthelot():
+    s = system.config(0)
+    A, B = s.pair_table_square()
+    sig, eps = 3.15061, 0.1521
+    assert A[0, 0] / 12 == pytest.approx(4 * eps * sig ** 12, rel=1e-6)
+    assert B[0, 0] / 6 == pytest.approx(4 * eps * sig ** 6, rel=1e-6)
+    assert A[1, 1] == 0 and B[0, 1] == 0
+    assert np.array_equal(A, A.T)
