@@ -94,9 +94,8 @@ def main():
     nl_pairs = eng.nl.status()["pairs"]
 
     def step():
-        eng.crd_to_uint()
-        eng.frc.zero_()
-        eng.force_step(0, 0)
+        # uint conversion fused into the step prep; the pair kernel assigns frc
+        eng.force_step(0, 0, from_crd=True, overwrite=True)
         eng.update_neighbor_list()
 
     for _ in range(args.warmup):

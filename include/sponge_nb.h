@@ -196,14 +196,19 @@ int sponge_pme_update_box_length(sponge_pme* pme, const float box_length[3]);
 
 /* ------------------------------------------------------------------ */
 /* Fused per-step force path: the body of Main_Calculate_Force's nonbonded
- * calls (main.cu:180-193) in the minimum number of launches. Accumulates all
- * nonbonded forces into d_frc; when need_energy / need_virial is set, the
- * FP64 parts (SPONGE_PART_*) are written to d_parts (device, 8 doubles) and
- * the per-atom arrays are filled like the separate calls fill them. */
-int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme,
-                         const unsigned int* d_uint_crd, const float* d_charge,
+ * calls (main.cu:170-193) in the minimum number of launches. With d_crd
+ * non-NULL the uint coordinates are first computed from it and written to
+ * d_uint_crd (MD_Information_Crd_To_Uint_Crd, MD_core.cu:1528-1532, fused);
+ * with d_crd NULL d_uint_crd is read. All nonbonded forces are added to d_frc
+ * (or, with SPONGE_STEP_OVERWRITE_FORCE, d_frc is overwritten: the caller need
+ * not zero it). When need_energy / need_virial is set, the FP64 parts
+ * (SPONGE_PART_*) are written to d_parts (device, 8 doubles) and the per-atom
+ * arrays are filled like the separate calls fill them. */
+#define SPONGE_STEP_OVERWRITE_FORCE 1
+int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd,
+                         unsigned int* d_uint_crd, const float* d_charge,
                          const int* d_excluded_list_start, const int* d_excluded_list,
-                         const int* d_excluded_numbers, float* d_frc, int need_energy,
+                         const int* d_excluded_numbers, float* d_frc, int flags, int need_energy,
                          int need_virial, float* d_atom_energy, float* d_atom_virial,
                          double* d_parts, void* stream);
 

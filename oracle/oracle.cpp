@@ -197,6 +197,10 @@ long orc_nl_grid(int n, const float* crd_in, const float* box, float cutoff, flo
 // a half list of pairs (i<j). The per-atom arrays follow the reference's
 // half-list convention: every pair term is credited to atom i (the lower
 // index). dr = r_j - r_i from the wrap-around uint difference (common.cu:89).
+// The cutoff test is the FP32 expression of the neighbor list (nl_dr2 against
+// rc*rc in FP32); the reference's norm3df(dr) < rc (:480-482) can differ only
+// for pairs within one FP32 ulp of the cutoff, and a fixed FP32 predicate
+// lets the GPU path and this oracle agree on every pair.
 // Stored A, B are x12 / x6 (Lennard_Jones_force.cu:1092,1097).
 //   F_i += (f_LJ - f_C) dr, F_j -= (f_LJ - f_C) dr
 //   f_LJ = (-A r^-6 + B) r^-8,  f_C = q_i q_j (2/sqrt(pi) b r e^{-b^2r^2} + erfc(b r)) r^-3
@@ -212,7 +216,13 @@ void orc_direct(int n, const uint32_t* u, const float* q, const int* type, const
                 const int* pairs, double* frc, double* atom_e, double* atom_vir,
                 double* atom_dir, double* totals) {
     double cof[3];
-    for (int d = 0; d < 3; d++) cof[d] = (double)box[d] / 4294967296.0;
+    float coff[3];
+    for (int d = 0; d < 3; d++) {
+        cof[d] = (double)box[d] / 4294967296.0;
+        coff[d] = (1.0f / 4294967296.0f) * box[d];
+    }
+    const float rcf = (float)rc;
+    const float rc2f = rcf * rcf;
     const double two_over_sqrtpi = 2.0 / std::sqrt(PI_D);
     double e_lj = 0, e_c = 0, w_lj = 0, cnt = 0;
     int nt = 1;
@@ -232,7 +242,7 @@ void orc_direct(int n, const uint32_t* u, const float* q, const int* type, const
         for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
         double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
         double r = std::sqrt(r2);
-        if (!(r < rc)) continue;
+        if (!(nl_dr2(u + 3 * i, u + 3 * j, coff) < rc2f)) continue;
         int p = pair_index(type[i], type[j]);
         double ri2 = 1.0 / r2, ri6 = ri2 * ri2 * ri2, ri8 = ri6 * ri2;
         double flj = (-(double)A[p] * ri6 + (double)B[p]) * ri8;

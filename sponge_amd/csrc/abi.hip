@@ -87,26 +87,29 @@ extern "C" int sponge_sum_list(int n, const float* d_list, float* d_out, void* s
     return SPONGE_NB_OK;
 }
 
-extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const unsigned int* d_uint_crd,
-                                    const float* d_charge, const int* d_excluded_list_start,
-                                    const int* d_excluded_list, const int* d_excluded_numbers, float* d_frc,
+extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd,
+                                    unsigned int* d_uint_crd, const float* d_charge, const int* d_excluded_list_start,
+                                    const int* d_excluded_list, const int* d_excluded_numbers, float* d_frc, int flags,
                                     int need_energy, int need_virial, float* d_atom_energy, float* d_atom_virial,
                                     double* d_parts, void* stream) {
     SNB_ARG(nl && lj && pme && d_uint_crd && d_charge && d_frc, "nb_force_step");
     SNB_ARG(d_excluded_list_start && d_excluded_list && d_excluded_numbers, "exclusion CSR");
     SNB_ARG(nl->n == lj->n && nl->n == pme->n, "atom counts of the modules differ");
+    SNB_ARG(pme->box[0] == lj->box[0] && pme->box[1] == lj->box[1] && pme->box[2] == lj->box[2],
+            "LJ and PME boxes differ");
     hipStream_t s = as_stream(stream);
     const int energy = (need_energy > 0 || need_virial > 0) ? 1 : 0;
-    int rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme->beta,
-                       energy, &lj->ps, s);
+    int rc = snb_step_prep(nl, lj, pme, d_crd, d_uint_crd, d_charge, energy, s);
+    if (rc) return rc;
+    rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme->beta,
+                   energy, &lj->ps, d_frc, need_energy > 0 ? d_atom_energy : nullptr,
+                   need_virial > 0 ? d_atom_virial : nullptr, nullptr, s, 1,
+                   (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0);
     if (rc) return rc;
     rc = snb_excluded(pme, d_uint_crd, lj->cof_dr, d_charge, d_excluded_list_start, d_excluded_list,
                       d_excluded_numbers, d_frc, nullptr, s);
     if (rc) return rc;
-    rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, d_frc, energy, s);
-    if (rc) return rc;
-    rc = snb_pairs_scatter(nl, &lj->ps, d_frc, need_energy > 0 ? d_atom_energy : nullptr,
-                           need_virial > 0 ? d_atom_virial : nullptr, nullptr, s);
+    rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, d_frc, energy, 1, s);
     if (rc) return rc;
     if (energy) {
         rc = snb_pme_finalize(pme, nullptr, 0, 0, nullptr, nullptr, s);
@@ -129,7 +132,7 @@ extern "C" int sponge_pme_get_energy(sponge_pme* pme, const sponge_nl* nl, const
     const bool all = which_part == 0;
     double e_rec = 0, e_self = 0, e_dir = 0, e_corr = 0;
     if (all || which_part == 2 || which_part == 4) {
-        int rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, 1, s);
+        int rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, 1, 0, s);
         if (rc) return rc;
     }
     if (all || which_part == 3) {
@@ -159,7 +162,7 @@ extern "C" int sponge_pme_get_energy(sponge_pme* pme, const sponge_nl* nl, const
         float cof[3];
         for (int d = 0; d < 3; d++) cof[d] = (1.0f / 4294967296.0f) * pme->box[d];
         int rc = snb_pairs(nl, pme->zero_ab, 1, pme->zero_type, pme->cutoff, cof, d_uint_crd, d_charge, pme->beta, 1,
-                           &pme->ps, s);
+                           &pme->ps, nullptr, nullptr, nullptr, nullptr, s);
         if (rc) return rc;
         double acc[SPONGE_NB_NPARTS];
         SNB_HIP(hipMemcpyAsync(acc, pme->ps.acc, sizeof(acc), hipMemcpyDeviceToHost, s));

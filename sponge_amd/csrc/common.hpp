@@ -102,10 +102,7 @@ __device__ double block_sum(double v, double* smem /* NT/64 */) {
 // Cluster-ordered per-step buffers of one pair-kernel user.
 struct PairScratch {
     int cap;                     // slots (the list's slot capacity)
-    uint4* pk;                   // packed (ux, uy, uz, q)
-    int* pty;                    // LJ type
-    float* fs;                   // 3 * cap SoA force accumulators
-    float* es;                   // 3 * cap: energy, LJ virial, direct energy
+    uint4* pk;                   // packed (ux|t, uy|t, uz|t, q): LJ type in the low bits
     double* acc;                 // SPONGE_NB_NPARTS FP64 totals
     hipEvent_t ev_start, ev_stop;  // optional, recorded around the pair kernel
 };
@@ -172,10 +169,12 @@ struct sponge_pme {
     float2* d_tw[3];             // twiddles exp(-2 pi i j / K) per axis
     int radix[3][16];
     int npass[3];
-    float* Q;                    // K^3 charge grid (kept zero between steps)
     float2* FQ;                  // nfft
     float* R;                    // K^3 convolved grid
-    int4* base;                  // n grid base indices
+    unsigned* base;              // n grid base indices, x | y << 10 | z << 20
+    int* bin_count;              // K0 atoms filed under each x base plane
+    int* bin_atoms;              // K0 * bin_cap atom indices
+    int bin_cap;
     float4* frac;                // n fractional parts
     float* corr_atom;            // n correction energies
     double* acc;                 // SPONGE_NB_NPARTS
@@ -188,14 +187,15 @@ struct sponge_pme {
 // Internal entry points shared between translation units.
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
-              hipStream_t s);
-int snb_pairs_scatter(const sponge_nl* nl, const PairScratch* ps, float* d_frc, float* atom_e, float* atom_v,
-                      float* atom_d, hipStream_t s);
+              float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered = 0,
+              int overwrite = 0);
 void snb_scratch_free(PairScratch* ps);
 int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float scaler[3], const float* d_charge,
                  const int* es, const int* el, const int* en, float* d_frc, float* d_atom_energy, hipStream_t s);
 int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
-                       int energy, hipStream_t s);
+                       int energy, int prep_done, hipStream_t s);
+int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd, unsigned int* d_uint_crd,
+                  const float* d_charge, int energy, hipStream_t s);
 int snb_pme_finalize(sponge_pme* pme, const float* d_direct_atom_energy, int need_energy, int need_virial,
                      float* d_potential, float* d_virial, hipStream_t s);
 int snb_fill_zero_f64(double* p, int n, hipStream_t s);

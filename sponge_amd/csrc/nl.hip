@@ -5,12 +5,14 @@
 // atom i, every j>i in the 125 surrounding cells with FP32 dr2 < (rc+skin)^2
 // (neighbor_list.cu:223-283), then deletes excluded partners (:353-403).
 // Here atoms are counting-sorted into xy columns, z-sorted inside each column
-// and cut into clusters of 4; the list holds, per i-cluster, the j-clusters
-// (index >= i-cluster) whose bounding boxes come within rc+skin, each with a
-// 16-bit mask whose bit (jj*4+ii) is set exactly when the reference would
+// and cut into clusters of 4; the list holds, per i-cluster, every j-cluster
+// (both directions, so each unordered cluster pair appears twice and the pair
+// kernel needs no atomics) whose bounding box comes within rc+skin, each with
+// a 16-bit mask whose bit (jj*4+ii) is set exactly when the reference would
 // list that atom pair: the same FP32 dr2 expression, the same strict
-// comparison, minus exclusions and padding. The set of listed atom pairs is
-// therefore identical to the reference's; only the storage differs.
+// comparison, minus exclusions and padding. The set of listed unordered atom
+// pairs is therefore identical to the reference's half list; only the storage
+// differs.
 //
 // Rebuild decision stays on the device (no host sync, no device-side
 // launch): every rebuild kernel reads flags[0] and returns at once when it
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
             for (int jb = j0; jb < j1; jb += 64) {
                 int jc = jb + lane;
                 bool ok = false;
-                if (jc < j1 && jc >= ic) {
+                if (jc < j1) {
                     float4 lj = cl_lo[jc], hj = cl_hi[jc];
                     float dx = 0.5f * (lj.x + hj.x) - cix, dy = 0.5f * (lj.y + hj.y) - ciy,
                           dz = 0.5f * (lj.z + hj.z) - ciz;
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
         if (jc >= 0) {
             uint4 pj = pack[jc * kClusterSize + jj];
             int aj = (int)pj.w;
-            bool valid = ai >= 0 && aj >= 0 && (jc != ic || jj > ii);
+            bool valid = ai >= 0 && aj >= 0 && (jc != ic || jj != ii);
             if (valid && nl_dr2(pi, pj, crx, cry, crz) < rl2) bit = !excluded_pair(ai, aj, es, en, el);
         }
         unsigned long long bal = __ballot(bit);
@@ -339,6 +341,7 @@ __global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
             count = list_cap;
         }
         list_n[ic] = count;
+        // every unordered pair is listed from both of its atoms
         atomicAdd((unsigned long long*)(flags + 8), (unsigned long long)npairs);
     }
 }
@@ -417,7 +420,7 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
     nl->slot_cap = (nl->slot_cap + 3) & ~3;
     nl->ncl_cap = nl->slot_cap / kClusterSize;
     double rl = cws;
-    double est = (2.0 / 3.0) * M_PI * std::pow(rl + 2.0 * w, 3) * dens / kClusterSize;
+    double est = (4.0 / 3.0) * M_PI * std::pow(rl + 2.0 * w, 3) * dens / kClusterSize;
     int lc = (int)std::ceil(2.0 * est) + 64;
     nl->list_cap = std::min(std::max(lc, 64), std::min(nl->ncl_cap, kCandCap));
     nl->cand_cap = kCandCap;
@@ -526,7 +529,7 @@ extern "C" int sponge_nl_status(sponge_nl* nl, long* pair_count, int* overflow_f
     SNB_HIP(hipMemcpy(f, nl->flags, sizeof(f), hipMemcpyDeviceToHost));
     unsigned long long pc;
     memcpy(&pc, f + 8, sizeof(pc));
-    if (pair_count) *pair_count = (long)pc;
+    if (pair_count) *pair_count = (long)(pc / 2);
     if (overflow_flags) *overflow_flags = f[1];
     if (rebuilds) *rebuilds = f[2];
     return f[1] ? SPONGE_NB_ECAPACITY : SPONGE_NB_OK;
@@ -553,6 +556,7 @@ extern "C" int sponge_nl_get_pairs(sponge_nl* nl, int* h_pairs, long capacity, l
                 if (ent.y >> b & 1u) {
                     int ii = b & 3, jj = b >> 2;
                     int a = s2o[ic * kClusterSize + ii], c = s2o[ent.x * kClusterSize + jj];
+                    if (a > c) continue;  // each unordered pair is listed from both atoms
                     if (h_pairs && m < capacity) {
                         h_pairs[2 * m] = std::min(a, c);
                         h_pairs[2 * m + 1] = std::max(a, c);

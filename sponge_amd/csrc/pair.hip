@@ -5,11 +5,12 @@
 //
 // Pair kernel layout: one wave64 per 4-atom i-cluster; each iteration takes
 // four j-clusters from the cluster-pair list, lane = (g:2, jj:2, ii:2) holds
-// the pair (ii, jj) of j-cluster g. Coordinates and charges are gathered once
-// per step into cluster order as one 16-byte record per atom; the i force
-// stays in registers for the whole list, the j force is reduced over the four
-// i lanes with DPP quad permutes and leaves with one atomic per component and
-// j atom.
+// the pair (ii, jj) of j-cluster g. Coordinates, charges and LJ types are
+// gathered once per step into cluster order as one 16-byte record per atom.
+// The list names every neighbouring cluster from both sides, so each wave
+// only accumulates the force on its own four atoms (in registers, reduced
+// with cross-lane adds at the end) and adds it to the caller's force array
+// with plain read-modify-writes: no atomics anywhere in the pair path.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "pme_prep.hpp"
 
 using namespace snb;
 
@@ -26,11 +28,13 @@ constexpr int kPairWaves = 4;
 constexpr float kTwoOverSqrtPi = 1.1283791670955126f;
 
 // Per-step gather into cluster order (replaces Copy_Crd_And_Charge_To_New_Crd,
-// Lennard_Jones_force.cu:24-34) and reset of the sorted accumulators.
-__global__ void gather_kernel(int cap, const int* __restrict__ flags, const int* __restrict__ s2o,
+// Lennard_Jones_force.cu:24-34). The LJ type (< 64) rides in the low two bits
+// of each uint coordinate, which Crd_To_Uint_Crd always leaves zero (`<< 2`,
+// common.cu:283-285): one 16-byte load per j atom carries position, charge
+// and type.
+__global__ void gather_kernel(const int* __restrict__ flags, const int* __restrict__ s2o,
                               const unsigned int* __restrict__ uc, const float* __restrict__ q,
-                              const int* __restrict__ type, uint4* __restrict__ pk, int* __restrict__ pty,
-                              float* __restrict__ fs, float* __restrict__ es, int energy,
+                              const int* __restrict__ type, uint4* __restrict__ pk, int energy,
                               double* __restrict__ acc) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (energy && p < 3) acc[p] = 0.0;  // SPONGE_PART_LJ, _DIRECT, _LJ_VIRIAL
@@ -38,49 +42,84 @@ __global__ void gather_kernel(int cap, const int* __restrict__ flags, const int*
     if (p >= flags[3]) return;
     int a = s2o[p];
     if (a >= 0) {
-        pk[p] = make_uint4(uc[3 * a], uc[3 * a + 1], uc[3 * a + 2], __float_as_uint(q[a]));
-        pty[p] = type[a];
+        unsigned t = (unsigned)type[a];
+        pk[p] = make_uint4((uc[3 * a] & ~3u) | (t & 3u), (uc[3 * a + 1] & ~3u) | ((t >> 2) & 3u),
+                           (uc[3 * a + 2] & ~3u) | ((t >> 4) & 3u), __float_as_uint(q[a]));
     } else {
         pk[p] = make_uint4(0u, 0u, 0u, 0u);
-        pty[p] = 0;
-    }
-    fs[p] = 0.f;
-    fs[cap + p] = 0.f;
-    fs[2 * cap + p] = 0.f;
-    if (energy) {
-        es[p] = 0.f;
-        es[cap + p] = 0.f;
-        es[2 * cap + p] = 0.f;
     }
 }
 
-// Adds the cluster-ordered accumulators into the caller's original-order
-// arrays (frc AoS float3, per-atom energy arrays).
-__global__ void scatter_kernel(int cap, const int* __restrict__ flags, const int* __restrict__ s2o,
-                               const float* __restrict__ fs, const float* __restrict__ es,
-                               float* __restrict__ frc, float* __restrict__ atom_e, float* __restrict__ atom_v,
-                               float* __restrict__ atom_d) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= flags[3]) return;
-    int a = s2o[p];
-    if (a < 0) return;
-    if (frc) {
-        frc[3 * a] += fs[p];
-        frc[3 * a + 1] += fs[cap + p];
-        frc[3 * a + 2] += fs[2 * cap + p];
+// Fused per-step prep of the whole force path, one thread per cluster slot:
+// optional MD_Information_Crd_To_Uint_Crd (MD_core.cu:1528-1532, written back
+// to the caller's uint array), the pair-kernel record, and the PME base,
+// fraction and x-plane bin of the slot's atom.
+__global__ __launch_bounds__(256) void step_prep_kernel(const int* __restrict__ flags, const int* __restrict__ s2o,
+                                                        const float* __restrict__ crd, float cqx, float cqy, float cqz,
+                                                        unsigned int* __restrict__ uc, const float* __restrict__ q,
+                                                        const int* __restrict__ type, uint4* __restrict__ pk,
+                                                        PmePrep P, int energy, double* __restrict__ acc) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (energy && p < 3) acc[p] = 0.0;  // SPONGE_PART_LJ, _DIRECT, _LJ_VIRIAL
+    if (energy && p == 3) acc[SPONGE_PART_PAIRS_IN] = 0.0;
+    const int nslot = flags[3];
+    int a = p < nslot ? s2o[p] : -1;
+    unsigned u0 = 0u, u1 = 0u, u2 = 0u;
+    float qa = 0.f;
+    if (a >= 0) {
+        if (crd) {
+            u0 = ((unsigned)(int)__fmul_rn(crd[3 * a], cqx)) << 2;
+            u1 = ((unsigned)(int)__fmul_rn(crd[3 * a + 1], cqy)) << 2;
+            u2 = ((unsigned)(int)__fmul_rn(crd[3 * a + 2], cqz)) << 2;
+            uc[3 * a] = u0;
+            uc[3 * a + 1] = u1;
+            uc[3 * a + 2] = u2;
+        } else {
+            u0 = uc[3 * a];
+            u1 = uc[3 * a + 1];
+            u2 = uc[3 * a + 2];
+        }
+        qa = q[a];
+        const unsigned t = (unsigned)type[a];
+        pk[p] = make_uint4((u0 & ~3u) | (t & 3u), (u1 & ~3u) | ((t >> 2) & 3u), (u2 & ~3u) | ((t >> 4) & 3u),
+                           __float_as_uint(qa));
+    } else if (p < nslot) {
+        pk[p] = make_uint4(0u, 0u, 0u, 0u);
     }
-    if (atom_e) atom_e[a] += es[p];
-    if (atom_v) atom_v[a] += es[cap + p];
-    if (atom_d) atom_d[a] = es[2 * cap + p];
+    pme_prep_block(a, u0, u1, u2, qa, P, energy);
 }
 
+__device__ __forceinline__ int unpack_type(uint4 v) { return (int)((v.x & 3u) | ((v.y & 3u) << 2) | ((v.z & 3u) << 4)); }
+
+// erfc(x) = exp(-x^2) * erfcx(x) with erfcx(x) ~ t P7(t), t = 1/(1 + p x):
+// the form of Abramowitz & Stegun 7.1.26 with eight coefficients refitted
+// (iteratively reweighted least squares on [0, 5]) for a relative error of
+// 6.3e-8 in erfcx; evaluated in FP32 the relative error of erfc and of the
+// force factor 2x/sqrt(pi) e^{-x^2} + erfc(x) stays below 5.2e-7 on [0, 4.5].
+__device__ __forceinline__ float erfcx_poly(float x) {
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3195f, x, 1.0f));
+    float p = -0.181163783781557f;
+    p = fmaf(p, t, 0.7263733451877996f);
+    p = fmaf(p, t, -0.8054935211563752f);
+    p = fmaf(p, t, 0.8311706379959746f);
+    p = fmaf(p, t, -0.18734797244699758f);
+    p = fmaf(p, t, 0.27142876592060167f);
+    p = fmaf(p, t, 0.16357488112143515f);
+    p = fmaf(p, t, 0.18145758631968023f);
+    return t * p;
+}
+
+// Cutoff test shared with the oracle: FP32 dr2 = (dx*dx + dy*dy) + dz*dz with
+// every operation rounded (the neighbor list's expression) against rc*rc in
+// FP32, so both sides agree on every pair, including those within one ulp of
+// the cutoff.
 template <bool ENERGY>
 __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
     const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list, int list_cap,
-    const uint4* __restrict__ pk, const int* __restrict__ pty, const float2* __restrict__ ab, int nt, float cx,
-    float cy, float cz, float rc2, float beta, float* __restrict__ fs, int cap, const int* __restrict__ s2o,
-    float* __restrict__ es, double* __restrict__ acc) {
-    __shared__ float2 sab[kMaxLjTypes * kMaxLjTypes];
+    const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz, float rc2,
+    float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
+    float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite) {
+    extern __shared__ float2 sab[];
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -90,8 +129,10 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
     const int islot = ic * kClusterSize + ii;
     const uint4 a = pk[islot];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
     const float qi = __uint_as_float(a.w);
-    const float2* srow = sab + pty[islot] * nt;
+    const float2* srow = sab + unpack_type(a) * nt;
+    const float mlog2e_b2 = -1.4426950408889634f * (beta * beta);
     int ai = 0;
     if (ENERGY) ai = s2o[islot];
     float fix = 0.f, fiy = 0.f, fiz = 0.f;
@@ -99,115 +140,105 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
     double tlj = 0.0, tdir = 0.0, tvir = 0.0, tcnt = 0.0;
     const int n = list_n[ic];
     const uint2* lst = list + (size_t)ic * list_cap;
-    for (int e0 = 0; e0 < n; e0 += 4) {
-        const int e = e0 + g;
-        uint2 ent = e < n ? lst[e] : make_uint2((unsigned)ic, 0u);
-        const int jslot = (int)ent.x * kClusterSize + jj;
-        const uint4 b = pk[jslot];
-        const float dx = (float)(int)(b.x - a.x) * cx;
-        const float dy = (float)(int)(b.y - a.y) * cy;
-        const float dz = (float)(int)(b.z - a.z) * cz;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool in = ((ent.y >> bit) & 1u) && r2 < rc2;
-        const float r2s = in ? r2 : 1.0f;
-        const float rinv = rsqrtf(r2s);
-        const float rinv2 = rinv * rinv;
-        const float rinv6 = rinv2 * rinv2 * rinv2;
-        const float2 AB = srow[pty[jslot]];
-        const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
-        const float r = r2s * rinv;
-        const float br = beta * r;
-        const float qq = qi * __uint_as_float(b.w);
-        const float ex = expf(-br * br);
-        const float ec = erfcf(br);
-        const float fc = qq * (kTwoOverSqrtPi * br * ex + ec) * rinv * rinv2;
-        const float f = in ? flj - fc : 0.f;
-        const float fx = f * dx, fy = f * dy, fz = f * dz;
-        fix += fx;
-        fiy += fy;
-        fiz += fz;
-        // j force: reduce over the four i lanes of each (g, jj) quad.
-        float jx = fx + dpp_quad_xor1(fx), jy = fy + dpp_quad_xor1(fy), jz = fz + dpp_quad_xor1(fz);
-        jx += dpp_quad_xor2(jx);
-        jy += dpp_quad_xor2(jy);
-        jz += dpp_quad_xor2(jz);
-        const bool jlead = ii == 0 && e < n;
-        if (jlead) {
-            atomicAdd(fs + jslot, -jx);
-            atomicAdd(fs + cap + jslot, -jy);
-            atomicAdd(fs + 2 * cap + jslot, -jz);
-        }
-        if (ENERGY) {
-            // Energy steps evaluate the pair energies in FP64 from the exact
-            // integer displacement: the totals are nets of large cancelling
-            // terms, which FP32 terms cannot hold to 1e-6.
-            double elj = 0.0, edir = 0.0, vlj = 0.0;
-            if (in) {
-                const double ddx = (double)(int)(b.x - a.x) * (double)cx;
-                const double ddy = (double)(int)(b.y - a.y) * (double)cy;
-                const double ddz = (double)(int)(b.z - a.z) * (double)cz;
-                const double dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                const double dr = sqrt(dr2);
-                const double ir2 = 1.0 / dr2, ir6 = ir2 * ir2 * ir2;
-                elj = ((double)AB.x * (1.0 / 12.0) * ir6 - (double)AB.y * (1.0 / 6.0)) * ir6;
-                edir = (double)qi * (double)__uint_as_float(b.w) * erfc((double)beta * dr) / dr;
-                vlj = ((double)AB.x * ir6 - (double)AB.y) * ir6;
+    for (int base = 0; base < n; base += 64) {
+        // 64 list entries (16 iterations of 4 j-clusters) held one per lane.
+        const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
+        const int cnt = min(64, n - base);
+        int jc = __shfl((int)my.x, g);
+        unsigned m = (unsigned)__shfl((int)my.y, g);
+        uint4 b = pk[jc * kClusterSize + jj];
+        for (int it = 0; it < cnt; it += 4) {
+            // prefetch the next iteration's j record
+            const int src = min(it + 4 + g, 63);
+            const int jcn = __shfl((int)my.x, src);
+            const unsigned mn = (unsigned)__shfl((int)my.y, src);
+            const uint4 bn = pk[jcn * kClusterSize + jj];
+            const float dx = __fmul_rn((float)(int)((b.x & ~3u) - ax), cx);
+            const float dy = __fmul_rn((float)(int)((b.y & ~3u) - ay), cy);
+            const float dz = __fmul_rn((float)(int)((b.z & ~3u) - az), cz);
+            const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+            const bool in = ((m >> bit) & 1u) && r2 < rc2;
+            const float r2s = in ? r2 : 1.0f;
+            const float rinv = __builtin_amdgcn_rsqf(r2s);
+            const float rinv2 = rinv * rinv;
+            const float rinv6 = rinv2 * rinv2 * rinv2;
+            const float2 AB = srow[unpack_type(b)];
+            const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
+            const float x = beta * (r2s * rinv);
+            const float ex = __builtin_amdgcn_exp2f(mlog2e_b2 * r2s);
+            const float ecx = erfcx_poly(x);
+            const float qq = qi * __uint_as_float(b.w);
+            const float fc = qq * ex * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+            const float f = in ? flj - fc : 0.f;
+            fix += f * dx;
+            fiy += f * dy;
+            fiz += f * dz;
+            if (ENERGY) {
+                // Energy steps evaluate the pair energies in FP64 from the exact
+                // integer displacement: the totals are nets of large cancelling
+                // terms, which FP32 terms cannot hold to 1e-6. Each unordered
+                // pair is credited once, to its lower-index atom, as the
+                // reference's half list does.
+                const int aj = s2o[jc * kClusterSize + jj];
+                if (in && ai < aj) {
+                    const double ddx = (double)(int)((b.x & ~3u) - ax) * (double)cx;
+                    const double ddy = (double)(int)((b.y & ~3u) - ay) * (double)cy;
+                    const double ddz = (double)(int)((b.z & ~3u) - az) * (double)cz;
+                    const double dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    const double dr = sqrt(dr2);
+                    const double ir2 = 1.0 / dr2, ir6 = ir2 * ir2 * ir2;
+                    const double elj = ((double)AB.x * (1.0 / 12.0) * ir6 - (double)AB.y * (1.0 / 6.0)) * ir6;
+                    const double edir = (double)qi * (double)__uint_as_float(b.w) * erfc((double)beta * dr) / dr;
+                    const double vlj = ((double)AB.x * ir6 - (double)AB.y) * ir6;
+                    tlj += elj;
+                    tdir += edir;
+                    tvir += vlj;
+                    tcnt += 1.0;
+                    eie += (float)(elj + edir);
+                    eiv += (float)vlj;
+                    eid += (float)edir;
+                }
             }
-            tlj += elj;
-            tdir += edir;
-            tvir += vlj;
-            tcnt += in ? 1.0 : 0.0;
-            const int aj = s2o[jslot];
-            const bool to_i = ai < aj;  // the reference credits the lower-index atom
-            const float fe = (float)(elj + edir), fv = (float)vlj, fd = (float)edir;
-            float ce = to_i ? 0.f : fe, cv = to_i ? 0.f : fv, cd = to_i ? 0.f : fd;
-            if (to_i) {
-                eie += fe;
-                eiv += fv;
-                eid += fd;
-            }
-            ce += dpp_quad_xor1(ce);
-            cv += dpp_quad_xor1(cv);
-            cd += dpp_quad_xor1(cd);
-            ce += dpp_quad_xor2(ce);
-            cv += dpp_quad_xor2(cv);
-            cd += dpp_quad_xor2(cd);
-            if (jlead) {
-                atomicAdd(es + jslot, ce);
-                atomicAdd(es + cap + jslot, cv);
-                atomicAdd(es + 2 * cap + jslot, cd);
-            }
+            b = bn;
+            m = mn;
+            jc = jcn;
         }
     }
     // i force: sum the 16 lanes sharing ii (lanes ii, ii+4, ..., ii+60).
-    fix += __shfl_xor(fix, 4);
-    fiy += __shfl_xor(fiy, 4);
-    fiz += __shfl_xor(fiz, 4);
-    fix += __shfl_xor(fix, 8);
-    fiy += __shfl_xor(fiy, 8);
-    fiz += __shfl_xor(fiz, 8);
-    fix += __shfl_xor(fix, 16);
-    fiy += __shfl_xor(fiy, 16);
-    fiz += __shfl_xor(fiz, 16);
-    fix += __shfl_xor(fix, 32);
-    fiy += __shfl_xor(fiy, 32);
-    fiz += __shfl_xor(fiz, 32);
-    if (lane < 4) {
-        atomicAdd(fs + islot, fix);
-        atomicAdd(fs + cap + islot, fiy);
-        atomicAdd(fs + 2 * cap + islot, fiz);
-    }
-    if (ENERGY) {
-        for (int o = 4; o < 64; o <<= 1) {
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) {
+        fix += __shfl_xor(fix, o);
+        fiy += __shfl_xor(fiy, o);
+        fiz += __shfl_xor(fiz, o);
+        if (ENERGY) {
             eie += __shfl_xor(eie, o);
             eiv += __shfl_xor(eiv, o);
             eid += __shfl_xor(eid, o);
         }
-        if (lane < 4) {
-            atomicAdd(es + islot, eie);
-            atomicAdd(es + cap + islot, eiv);
-            atomicAdd(es + 2 * cap + islot, eid);
+    }
+    if (lane < 4) {
+        const int at = s2o[islot];
+        if (at >= 0) {
+            if (frc) {
+                // every atom has exactly one slot, so this is the only writer
+                if (overwrite) {
+                    frc[3 * at] = fix;
+                    frc[3 * at + 1] = fiy;
+                    frc[3 * at + 2] = fiz;
+                } else {
+                    frc[3 * at] += fix;
+                    frc[3 * at + 1] += fiy;
+                    frc[3 * at + 2] += fiz;
+                }
+            }
+            if (ENERGY) {
+                if (atom_e) atom_e[at] += eie;
+                if (atom_v) atom_v[at] += eiv;
+                if (atom_d) atom_d[at] = eid;
+            }
         }
+    }
+    if (ENERGY) {
         tlj = wave_sum(tlj);
         tdir = wave_sum(tdir);
         tvir = wave_sum(tvir);
@@ -364,9 +395,6 @@ static int scratch_ensure(PairScratch* ps, const sponge_nl* nl) {
     snb_scratch_free(ps);
     int cap = nl->slot_cap;
     SNB_HIP(hipMalloc((void**)&ps->pk, cap * sizeof(uint4)));
-    SNB_HIP(hipMalloc((void**)&ps->pty, cap * sizeof(int)));
-    SNB_HIP(hipMalloc((void**)&ps->fs, 3 * (size_t)cap * sizeof(float)));
-    SNB_HIP(hipMalloc((void**)&ps->es, 3 * (size_t)cap * sizeof(float)));
     SNB_HIP(hipMalloc((void**)&ps->acc, SPONGE_NB_NPARTS * sizeof(double)));
     SNB_HIP(hipMemset(ps->acc, 0, SPONGE_NB_NPARTS * sizeof(double)));
     ps->cap = cap;
@@ -374,7 +402,7 @@ static int scratch_ensure(PairScratch* ps, const sponge_nl* nl) {
 }
 
 void snb_scratch_free(PairScratch* ps) {
-    void* ptrs[] = {ps->pk, ps->pty, ps->fs, ps->es, ps->acc};
+    void* ptrs[] = {ps->pk, ps->acc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     hipEvent_t e0 = ps->ev_start, e1 = ps->ev_stop;
@@ -383,31 +411,49 @@ void snb_scratch_free(PairScratch* ps) {
     ps->ev_stop = e1;
 }
 
-// Gather + pair kernel; results stay cluster-ordered in ps->fs / ps->es and
-// the FP64 totals in ps->acc.
+// Gather + pair kernel: forces added to d_frc (original order, may be NULL),
+// per-atom energies to the given arrays, FP64 totals to ps->acc.
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
-              hipStream_t s) {
+              float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered,
+              int overwrite) {
     int rc = scratch_ensure(ps, nl);
     if (rc) return rc;
     const int cap = ps->cap;
-    gather_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(cap, nl->flags, nl->s2o, d_uint_crd, d_charge, d_type, ps->pk,
-                                                     ps->pty, ps->fs, ps->es, energy, ps->acc);
-    SNB_CHECK_LAUNCH();
+    if (!gathered) {
+        gather_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(nl->flags, nl->s2o, d_uint_crd, d_charge, d_type, ps->pk,
+                                                         energy, ps->acc);
+        SNB_CHECK_LAUNCH();
+    }
     const int nblk = xcd_grid(ceil_div(nl->ncl_cap, kPairWaves));
     const float rc2 = cutoff * cutoff;
+    const size_t lds = (size_t)nt * nt * sizeof(float2);
     if (ps->ev_start) SNB_HIP(hipEventRecord(ps->ev_start, s));
     if (energy) {
-        pair_kernel<true><<<nblk, kPairWaves * 64, 0, s>>>(nl->flags, nl->list_n, nl->list, nl->list_cap, ps->pk,
-                                                           ps->pty, ab, nt, cof[0], cof[1], cof[2], rc2, beta, ps->fs,
-                                                           cap, nl->s2o, ps->es, ps->acc);
+        pair_kernel<true><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, nl->list_n, nl->list, nl->list_cap, ps->pk,
+                                                             ab, nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,
+                                                             atom_e, atom_v, atom_d, ps->acc, overwrite);
     } else {
-        pair_kernel<false><<<nblk, kPairWaves * 64, 0, s>>>(nl->flags, nl->list_n, nl->list, nl->list_cap, ps->pk,
-                                                            ps->pty, ab, nt, cof[0], cof[1], cof[2], rc2, beta, ps->fs,
-                                                            cap, nl->s2o, ps->es, ps->acc);
+        pair_kernel<false><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, nl->list_n, nl->list, nl->list_cap, ps->pk,
+                                                              ab, nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,
+                                                              nullptr, nullptr, nullptr, ps->acc, overwrite);
     }
     SNB_CHECK_LAUNCH();
     if (ps->ev_stop) SNB_HIP(hipEventRecord(ps->ev_stop, s));
+    return SPONGE_NB_OK;
+}
+
+int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd, unsigned int* d_uint_crd,
+                  const float* d_charge, int energy, hipStream_t s) {
+    int rc = scratch_ensure(&lj->ps, nl);
+    if (rc) return rc;
+    float cq[3];
+    for (int d = 0; d < 3; d++) cq[d] = 0.25f * (4294967296.0f / lj->box[d]);
+    const int cap = lj->ps.cap;
+    step_prep_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(nl->flags, nl->s2o, d_crd, cq[0], cq[1], cq[2], d_uint_crd,
+                                                        d_charge, lj->d_type, lj->ps.pk, pme_prep_args(pme), energy,
+                                                        lj->ps.acc);
+    SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
 
@@ -418,14 +464,6 @@ extern "C" int sponge_lj_set_pair_events(sponge_lj* lj, void* start_event, void*
     return SPONGE_NB_OK;
 }
 
-int snb_pairs_scatter(const sponge_nl* nl, const PairScratch* ps, float* d_frc, float* atom_e, float* atom_v,
-                      float* atom_d, hipStream_t s) {
-    const int cap = ps->cap;
-    scatter_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(cap, nl->flags, nl->s2o, ps->fs, ps->es, d_frc, atom_e, atom_v,
-                                                      atom_d);
-    SNB_CHECK_LAUNCH();
-    return SPONGE_NB_OK;
-}
 
 extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, const unsigned int* d_uint_crd,
                                           const float* d_charge, float* d_frc, float pme_beta, int need_atom_energy,
@@ -435,12 +473,9 @@ extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, co
     SNB_ARG(nl->n == lj->n, "atom count of LJ and neighbor list differ");
     hipStream_t s = as_stream(stream);
     const int energy = (need_atom_energy > 0 || need_virial > 0) ? 1 : 0;
-    int rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme_beta,
-                       energy, &lj->ps, s);
-    if (rc) return rc;
-    return snb_pairs_scatter(nl, &lj->ps, d_frc, need_atom_energy > 0 ? d_atom_energy : nullptr,
-                             need_virial > 0 ? d_atom_lj_virial : nullptr,
-                             energy ? d_atom_direct_pme_energy : nullptr, s);
+    return snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme_beta,
+                     energy, &lj->ps, d_frc, need_atom_energy > 0 ? d_atom_energy : nullptr,
+                     need_virial > 0 ? d_atom_lj_virial : nullptr, energy ? d_atom_direct_pme_energy : nullptr, s);
 }
 
 extern "C" int sponge_lj_long_range_correction(sponge_lj* lj, int need_pressure, float* d_virial, int need_potential,
@@ -467,7 +502,7 @@ extern "C" int sponge_lj_get_energy(sponge_lj* lj, const sponge_nl* nl, const un
     SNB_HIP(hipMalloc((void**)&zq, lj->n * sizeof(float)));
     SNB_HIP(hipMemset(zq, 0, lj->n * sizeof(float)));
     int rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, zq, 0.0f, 1,
-                       &lj->ps, s);
+                       &lj->ps, nullptr, nullptr, nullptr, nullptr, s);
     double acc[SPONGE_NB_NPARTS];
     if (rc == 0) {
         SNB_HIP(hipMemcpyAsync(acc, lj->ps.acc, sizeof(acc), hipMemcpyDeviceToHost, s));

@@ -6,11 +6,11 @@
 // [kx][ky][kz <= Kz/2] exactly as the reference's PME_Q / PME_FQ / PME_BC):
 //   pme_prep      per atom: grid base index and spline fraction from the uint
 //                 coordinate with one 32x32->64-bit multiply (t = u K / 2^32)
-//   pme_spread    one wave per atom, lane = one of the 64 order-4 stencil
-//                 points, FP32 atomics into Q
-//   fwd_plane     one block per x plane: real z-rows packed two at a time
-//                 into complex z-FFTs and unpacked (R2C), then y-FFTs; the plane
-//                 of Q is zeroed for the next call
+//   fwd_plane     one block per x plane: every atom whose order-4 stencil
+//                 touches the plane is spread into the plane in LDS
+//                 (ds_add_f32, no global atomics and no charge grid in HBM),
+//                 then real z-rows are packed two at a time into complex
+//                 z-FFTs and unpacked (R2C), then y-FFTs
 //   xconv         one block per (y, kz chunk): x-FFT, multiply by BC, energy
 //                 1/2 sum BC |F|^2 (FP64), inverse x-FFT
 //   inv_plane     one block per x plane: inverse y-FFTs, Hermitian repack and
@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "pme_prep.hpp"
 
 using namespace snb;
 
@@ -38,14 +39,11 @@ struct FftPlan {
     int radix[12];
 };
 
-constexpr int kPlaneThreads = 1024;
+constexpr int kPlaneThreads = 512;
+constexpr int kSpreadPerThread = 4;
 constexpr int kConvThreads = 256;
 constexpr int kConvChunk = 16;
 constexpr int kGatherThreads = 256;
-constexpr int kSpreadAtomsPerWave = 4;
-// PME acc slots 6 and 7 hold sum q^2 and sum q between prep and finalize.
-constexpr int kAccQ2 = 6;
-constexpr int kAccQ1 = 7;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -102,93 +100,121 @@ __device__ __forceinline__ void dft(float2* v) {
     }
 }
 
-// One Stockham radix-R step over nlines lines of length K held in LDS; element
-// (line l, position p) lives at buf[l*ls + p*ks]. Every thread first loads and
-// transforms all of its butterflies into registers, then all threads write.
-template <int R, int SIGN, int NT, int MAXB>
-__device__ void fft_pass(float2* buf, int nlines, int ls, int ks, int K, int Ns, const float2* __restrict__ tw) {
+// q = n / d for n * d < 2^32 via one multiply-high with m = ceil(2^32 / d);
+// d == 1 (m would be 2^32) is passed through.
+__device__ __forceinline__ unsigned div_magic(unsigned d) { return d > 1u ? 0xFFFFFFFFu / d + 1u : 0u; }
+__device__ __forceinline__ unsigned fast_div(unsigned n, unsigned d, unsigned m) { return d > 1u ? __umulhi(n, m) : n; }
+
+// One Stockham radix-R step over nlines lines of length K held in LDS;
+// element (line l, position p) lives at buf[l*ls + p*ks]; tw is the LDS
+// twiddle table exp(-2 pi i j / K). Line l is owned by wave l % nw in every
+// pass (a wave works on G of its lines at a time), and a wave's LDS
+// operations execute in program order, so a line's reads all precede its
+// writes and the next pass sees them without any workgroup barrier.
+template <int R, int SIGN>
+__device__ void wave_pass(float2* buf, int nlines, int ls, int ks, int K, int Ns, const float2* tw, int wid,
+                          int nw, int lane) {
     const int per = K / R;
-    const int nb = nlines * per;
     const int tws = K / (Ns * R);
-    float2 v[MAXB][R];
+    const unsigned mns = div_magic((unsigned)Ns);
+    if (per <= 64) {
+        const int G = 64 / per;
+        const int sub = (int)fast_div((unsigned)lane, (unsigned)per, div_magic((unsigned)per));
+        const int j = lane - sub * per;
+        const int jn = (int)fast_div((unsigned)j, Ns, mns);
+        const int k = j - jn * Ns;
+        const int d = jn * Ns * R + k;
+        for (int i0 = 0; wid + nw * i0 < nlines; i0 += G) {
+            const int l = wid + nw * (i0 + sub);
+            if (sub < G && l < nlines) {
+                float2 v[R];
 #pragma unroll
-    for (int s = 0; s < MAXB; s++) {
-        int b = threadIdx.x + s * NT;
-        if (b < nb) {
-            int l = b / per, j = b - l * per;
-            int k = j % Ns;
-#pragma unroll
-            for (int q = 0; q < R; q++) {
-                float2 x = buf[l * ls + (j + q * per) * ks];
-                if (q > 0) {
-                    float2 t = tw[q * k * tws];
-                    if (SIGN > 0) t.y = -t.y;
-                    x = cmul(x, t);
+                for (int q = 0; q < R; q++) {
+                    float2 x = buf[l * ls + (j + q * per) * ks];
+                    if (q > 0) {
+                        float2 t = tw[q * k * tws];
+                        if (SIGN > 0) t.y = -t.y;
+                        x = cmul(x, t);
+                    }
+                    v[q] = x;
                 }
-                v[s][q] = x;
+                dft<R, SIGN>(v);
+#pragma unroll
+                for (int q = 0; q < R; q++) buf[l * ls + (d + q * Ns) * ks] = v[q];
             }
-            dft<R, SIGN>(v[s]);
+        }
+    } else {
+        // long lines (per > 64, at most 128 here): one line per wave at a time
+        for (int l = wid; l < nlines; l += nw) {
+            float2 v[2][R];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int j = lane + 64 * c;
+                if (j < per) {
+                    const int k = j - (int)fast_div((unsigned)j, Ns, mns) * Ns;
+#pragma unroll
+                    for (int q = 0; q < R; q++) {
+                        float2 x = buf[l * ls + (j + q * per) * ks];
+                        if (q > 0) {
+                            float2 t = tw[q * k * tws];
+                            if (SIGN > 0) t.y = -t.y;
+                            x = cmul(x, t);
+                        }
+                        v[c][q] = x;
+                    }
+                    dft<R, SIGN>(v[c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int j = lane + 64 * c;
+                if (j < per) {
+                    const int jn = (int)fast_div((unsigned)j, Ns, mns);
+                    const int d = jn * Ns * R + (j - jn * Ns);
+#pragma unroll
+                    for (int q = 0; q < R; q++) buf[l * ls + (d + q * Ns) * ks] = v[c][q];
+                }
+            }
         }
     }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < MAXB; s++) {
-        int b = threadIdx.x + s * NT;
-        if (b < nb) {
-            int l = b / per, j = b - l * per;
-            int k = j % Ns;
-            int d = (j / Ns) * Ns * R + k;
-#pragma unroll
-            for (int q = 0; q < R; q++) buf[l * ls + (d + q * Ns) * ks] = v[s][q];
-        }
-    }
-    __syncthreads();
 }
 
-template <int SIGN, int NT, int MAXB>
-__device__ void lds_fft(float2* buf, int nlines, int ls, int ks, const FftPlan& p, const float2* __restrict__ tw) {
+// Full multi-pass transform of the wave's lines; callers put a workgroup
+// barrier before (data written by other waves) and after (data read by other
+// waves).
+template <int SIGN, int NT>
+__device__ void lds_fft(float2* buf, int nlines, int ls, int ks, const FftPlan& p, const float2* tw) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = NT / 64;
     int Ns = 1;
     for (int ps = 0; ps < p.npass; ps++) {
         int R = p.radix[ps];
         switch (R) {
-            case 4: fft_pass<4, SIGN, NT, MAXB>(buf, nlines, ls, ks, p.K, Ns, tw); break;
-            case 2: fft_pass<2, SIGN, NT, MAXB>(buf, nlines, ls, ks, p.K, Ns, tw); break;
-            case 3: fft_pass<3, SIGN, NT, MAXB>(buf, nlines, ls, ks, p.K, Ns, tw); break;
-            case 5: fft_pass<5, SIGN, NT, MAXB>(buf, nlines, ls, ks, p.K, Ns, tw); break;
-            default: fft_pass<7, SIGN, NT, MAXB>(buf, nlines, ls, ks, p.K, Ns, tw); break;
+            case 4: wave_pass<4, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
+            case 2: wave_pass<2, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
+            case 3: wave_pass<3, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
+            case 5: wave_pass<5, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
+            default: wave_pass<7, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
         }
         Ns *= R;
     }
 }
 
+template <int NT>
+__device__ void load_twiddles(float2* dst, const float2* __restrict__ src, int K) {
+    for (int j = threadIdx.x; j < K; j += NT) dst[j] = src[j];
+}
+
 // ---------------------------------------------------------------------------
-// PME_Atom_Near (PME_force.cu:367-421) without the 64-index cache: base and
-// fraction of t = u K / 2^32 from the exact 64-bit product.
-__global__ void pme_prep_kernel(int n, const unsigned int* __restrict__ uc, const float* __restrict__ q, int Kx,
-                                int Ky, int Kz, int4* __restrict__ base, float4* __restrict__ frac, int energy,
-                                double* __restrict__ acc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double q2 = 0.0, q1 = 0.0;
-    if (i < n) {
-        const float s = 2.3283064365386963e-10f;  // 2^-32
-        unsigned long long tx = (unsigned long long)uc[3 * i] * (unsigned)Kx;
-        unsigned long long ty = (unsigned long long)uc[3 * i + 1] * (unsigned)Ky;
-        unsigned long long tz = (unsigned long long)uc[3 * i + 2] * (unsigned)Kz;
-        base[i] = make_int4((int)(tx >> 32), (int)(ty >> 32), (int)(tz >> 32), 0);
-        frac[i] = make_float4((float)(unsigned)tx * s, (float)(unsigned)ty * s, (float)(unsigned)tz * s, q[i]);
-        if (energy) {
-            q1 = q[i];
-            q2 = q1 * q1;
-        }
-    }
-    if (energy) {
-        q2 = wave_sum(q2);
-        q1 = wave_sum(q1);
-        if ((threadIdx.x & 63) == 0) {
-            atomicAdd(acc + kAccQ2, q2);
-            atomicAdd(acc + kAccQ1, q1);
-        }
-    }
+// PME_Atom_Near (PME_force.cu:367-421): per-atom base, fraction and x-plane
+// bin (pme_prep_block).
+__global__ __launch_bounds__(256) void pme_prep_kernel(int n, const unsigned int* __restrict__ uc,
+                                                       const float* __restrict__ q, PmePrep P, int energy) {
+    // pme_prep_block holds workgroup barriers: every thread reaches the one
+    // call site, atoms past n pass i = -1.
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = i < n;
+    pme_prep_block(ok ? i : -1, ok ? uc[3 * i] : 0u, ok ? uc[3 * i + 1] : 0u, ok ? uc[3 * i + 2] : 0u,
+                   ok ? q[i] : 0.f, P, energy);
 }
 
 // Order-4 cardinal B-spline weights M4(f + k), k = 0..3, and derivatives
@@ -218,47 +244,93 @@ __device__ __forceinline__ float bsp_dw(float f, int k) {
     }
 }
 
-// PME_Q_Spread (PME_force.cu:428-471): grid point (base - k) gets
-// q * M4(fx+kx) M4(fy+ky) M4(fz+kz).
-__global__ void pme_spread_kernel(int n, const int4* __restrict__ base, const float4* __restrict__ frac, int Kx,
-                                  int Ky, int Kz, float* __restrict__ Q) {
-    const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int kx = lane >> 4, ky = (lane >> 2) & 3, kz = lane & 3;
-    for (int t = 0; t < kSpreadAtomsPerWave; t++) {
-        int i = wave * kSpreadAtomsPerWave + t;
-        if (i >= n) return;
-        int4 b = base[i];
-        float4 f = frac[i];
-        float w = f.w * bsp_w(f.x, kx) * bsp_w(f.y, ky) * bsp_w(f.z, kz);
-        int gx = b.x - kx, gy = b.y - ky, gz = b.z - kz;
-        gx += gx < 0 ? Kx : 0;
-        gy += gy < 0 ? Ky : 0;
-        gz += gz < 0 ? Kz : 0;
-        atomicAdd(Q + ((size_t)gx * Ky + gy) * Kz + gz, w);
-    }
-}
-
+// PME_Q_Spread (PME_force.cu:428-471) fused with the forward transform of
+// one x plane: grid point (base - k) gets q M4(fx+kx) M4(fy+ky) M4(fz+kz);
+// plane x receives the atoms filed under planes x .. x+3 (with kx = their
+// offset). The real plane is held directly in the packed complex layout of the
+// z-FFT: row y is the real (y even) or imaginary (y odd) part of line y/2.
 template <int MAXB>
-__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(float* __restrict__ Q, float2* __restrict__ FQ,
-                                                                  int Ky, int Kz, FftPlan pz, FftPlan py,
-                                                                  const float2* __restrict__ twz,
-                                                                  const float2* __restrict__ twy,
-                                                                  double* __restrict__ acc) {
+__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
+    const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
+    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, FftPlan pz,
+    FftPlan py, const float2* __restrict__ twz, const float2* __restrict__ twy, double* __restrict__ acc) {
     extern __shared__ float2 sm[];
     const int x = blockIdx.x;
     const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
-    float* Qp = Q + (size_t)x * Ky * Kz;
+    const int plane = max(Ky * H, nrp * Kz);
+    float2* tz = sm + plane;
+    float2* ty = tz + Kz;
+    load_twiddles<kPlaneThreads>(tz, twz, Kz);
+    load_twiddles<kPlaneThreads>(ty, twy, Ky);
     if (x == 0 && threadIdx.x == 0 && acc) acc[SPONGE_PART_RECIPROCAL] = 0.0;
-    for (int idx = threadIdx.x; idx < nrp * Kz; idx += kPlaneThreads) {
-        int p = idx / Kz, z = idx - p * Kz;
-        float a = Qp[(2 * p) * Kz + z];
-        float b = (2 * p + 1 < Ky) ? Qp[(2 * p + 1) * Kz + z] : 0.f;
-        sm[idx] = make_float2(a, b);
+    for (int idx = threadIdx.x; idx < nrp * Kz; idx += kPlaneThreads) sm[idx] = make_float2(0.f, 0.f);
+    __syncthreads();
+    float* smf = reinterpret_cast<float*>(sm);
+    // The four bins are walked as one flat range; each thread issues the
+    // index, base and fraction loads of up to kSpreadPerThread atoms before
+    // any LDS update, so their latencies overlap.
+    int c[4], off[5];
+    off[0] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int b = x + k;
+        b -= b >= Kx ? Kx : 0;
+        c[k] = min(bin_count[b], bin_cap);
+        off[k + 1] = off[k] + c[k];
+    }
+    const int total = off[4];
+    for (int t0 = 0; t0 < total; t0 += kPlaneThreads * kSpreadPerThread) {
+        int ai[kSpreadPerThread], akx[kSpreadPerThread];
+#pragma unroll
+        for (int r = 0; r < kSpreadPerThread; r++) {
+            const int t = t0 + r * kPlaneThreads + (int)threadIdx.x;
+            ai[r] = -1;
+            akx[r] = 0;
+            if (t < total) {
+                const int k = (t >= off[1]) + (t >= off[2]) + (t >= off[3]);
+                int b = x + k;
+                b -= b >= Kx ? Kx : 0;
+                ai[r] = bin_atoms[(size_t)b * bin_cap + (t - off[k])];
+                akx[r] = k;
+            }
+        }
+        unsigned bb[kSpreadPerThread];
+        float4 ff[kSpreadPerThread];
+#pragma unroll
+        for (int r = 0; r < kSpreadPerThread; r++) {
+            if (ai[r] >= 0) {
+                bb[r] = base[ai[r]];
+                ff[r] = frac[ai[r]];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kSpreadPerThread; r++) {
+            if (ai[r] < 0) continue;
+            const int by = (int)((bb[r] >> 10) & 1023u), bz = (int)(bb[r] >> 20);
+            const float4 f = ff[r];
+            const float wx = f.w * bsp_w(f.x, akx[r]);
+            float wz[4];
+            int gz[4];
+#pragma unroll
+            for (int kz = 0; kz < 4; kz++) {
+                wz[kz] = bsp_w(f.z, kz);
+                int tt = bz - kz;
+                gz[kz] = tt + (tt < 0 ? Kz : 0);
+            }
+#pragma unroll
+            for (int ky = 0; ky < 4; ky++) {
+                int gy = by - ky;
+                gy += gy < 0 ? Ky : 0;
+                const float wy = wx * bsp_w(f.y, ky);
+                float* row = smf + 2 * (gy >> 1) * Kz + (gy & 1);
+#pragma unroll
+                for (int kz = 0; kz < 4; kz++) atomicAdd(row + 2 * gz[kz], wy * wz[kz]);
+            }
+        }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < Ky * Kz; idx += kPlaneThreads) Qp[idx] = 0.f;
-    lds_fft<-1, kPlaneThreads, MAXB>(sm, nrp, Kz, 1, pz, twz);
+    lds_fft<-1, kPlaneThreads>(sm, nrp, Kz, 1, pz, tz);
+    __syncthreads();
     // Unpack the paired transforms Z = FFT(a + i b): A_k = (Z_k + conj Z_-k)/2,
     // B_k = (Z_k - conj Z_-k)/(2i), into rows y of the half spectrum.
     float2 st[MAXB];
@@ -281,7 +353,8 @@ __global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(float* __restr
         if (o < Ky * H) sm[o] = st[s];
     }
     __syncthreads();
-    lds_fft<-1, kPlaneThreads, MAXB>(sm, H, 1, H, py, twy);
+    lds_fft<-1, kPlaneThreads>(sm, H, 1, H, py, ty);
+    __syncthreads();
     float2* out = FQ + (size_t)x * Ky * H;
     for (int idx = threadIdx.x; idx < Ky * H; idx += kPlaneThreads) out[idx] = sm[idx];
 }
@@ -290,18 +363,24 @@ template <int MAXB>
 __global__ __launch_bounds__(kConvThreads) void xconv_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
                                                              int Kx, int Ky, int Kz, FftPlan px,
                                                              const float2* __restrict__ twx, int energy,
-                                                             double* __restrict__ acc) {
+                                                             double* __restrict__ acc, int* __restrict__ bin_count) {
     __shared__ float2 sm[256 * kConvChunk];
+    __shared__ float2 tx[256];
     __shared__ double red[kConvThreads / 64];
     const int H = Kz / 2 + 1;
     const int y = blockIdx.x, kz0 = blockIdx.y * kConvChunk;
     const int cw = min(kConvChunk, H - kz0);
+    // the forward plane kernels have consumed the x-plane bins: clear them
+    if (blockIdx.x == 0 && blockIdx.y == 0)
+        for (int b = threadIdx.x; b < Kx; b += kConvThreads) bin_count[b] = 0;
+    load_twiddles<kConvThreads>(tx, twx, Kx);
     for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
         int xx = idx / cw, c = idx - xx * cw;
         sm[xx * kConvChunk + c] = FQ[((size_t)xx * Ky + y) * H + kz0 + c];
     }
     __syncthreads();
-    lds_fft<-1, kConvThreads, MAXB>(sm, cw, 1, kConvChunk, px, twx);
+    lds_fft<-1, kConvThreads>(sm, cw, 1, kConvChunk, px, tx);
+    __syncthreads();
     double e = 0.0;
     for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
         int xx = idx / cw, c = idx - xx * cw;
@@ -315,7 +394,8 @@ __global__ __launch_bounds__(kConvThreads) void xconv_kernel(float2* __restrict_
         sm[xx * kConvChunk + c] = make_float2(v.x * b, v.y * b);
     }
     __syncthreads();
-    lds_fft<1, kConvThreads, MAXB>(sm, cw, 1, kConvChunk, px, twx);
+    lds_fft<1, kConvThreads>(sm, cw, 1, kConvChunk, px, tx);
+    __syncthreads();
     for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
         int xx = idx / cw, c = idx - xx * cw;
         FQ[((size_t)xx * Ky + y) * H + kz0 + c] = sm[xx * kConvChunk + c];
@@ -334,10 +414,16 @@ __global__ __launch_bounds__(kPlaneThreads) void inv_plane_kernel(const float2* 
     extern __shared__ float2 sm[];
     const int x = blockIdx.x;
     const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
+    const int plane = max(Ky * H, nrp * Kz);
+    float2* tz = sm + plane;
+    float2* ty = tz + Kz;
+    load_twiddles<kPlaneThreads>(tz, twz, Kz);
+    load_twiddles<kPlaneThreads>(ty, twy, Ky);
     const float2* in = FQ + (size_t)x * Ky * H;
     for (int idx = threadIdx.x; idx < Ky * H; idx += kPlaneThreads) sm[idx] = in[idx];
     __syncthreads();
-    lds_fft<1, kPlaneThreads, MAXB>(sm, H, 1, H, py, twy);
+    lds_fft<1, kPlaneThreads>(sm, H, 1, H, py, ty);
+    __syncthreads();
     // Repack rows (2p, 2p+1) into one complex line Z = A + i B over all kz,
     // extending each row by Hermitian symmetry; self-conjugate bins keep only
     // their real part (the C2R convention).
@@ -370,7 +456,8 @@ __global__ __launch_bounds__(kPlaneThreads) void inv_plane_kernel(const float2* 
         if (o < nrp * Kz) sm[o] = st[s];
     }
     __syncthreads();
-    lds_fft<1, kPlaneThreads, MAXB>(sm, nrp, Kz, 1, pz, twz);
+    lds_fft<1, kPlaneThreads>(sm, nrp, Kz, 1, pz, tz);
+    __syncthreads();
     float* Rp = R + (size_t)x * Ky * Kz;
     for (int idx = threadIdx.x; idx < nrp * Kz; idx += kPlaneThreads) {
         int p = idx / Kz, z = idx - p * Kz;
@@ -382,7 +469,7 @@ __global__ __launch_bounds__(kPlaneThreads) void inv_plane_kernel(const float2* 
 
 // PME_Final (PME_force.cu:485-550): 16 lanes per atom (lane = kx, ky), each
 // walking the 4 kz points; force += -q sum R dtheta (K/L).
-__global__ __launch_bounds__(kGatherThreads) void pme_gather_kernel(int n, const int4* __restrict__ base,
+__global__ __launch_bounds__(kGatherThreads) void pme_gather_kernel(int n, const unsigned* __restrict__ base,
                                                                     const float4* __restrict__ frac,
                                                                     const float* __restrict__ R, int Kx, int Ky,
                                                                     int Kz, float ibx, float iby, float ibz,
@@ -392,7 +479,8 @@ __global__ __launch_bounds__(kGatherThreads) void pme_gather_kernel(int n, const
     const int kx = sub >> 2, ky = sub & 3;
     float sx = 0.f, sy = 0.f, sz = 0.f, qi = 0.f;
     if (i < n) {
-        int4 b = base[i];
+        const unsigned bb = base[i];
+        const int4 b = make_int4((int)(bb & 1023u), (int)((bb >> 10) & 1023u), (int)(bb >> 20), 0);
         float4 f = frac[i];
         qi = f.w;
         float wx = bsp_w(f.x, kx), dwx = bsp_dw(f.x, kx);
@@ -572,7 +660,7 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     }
     const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
     long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
-    if (pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads) < 0 || plane * 8 > 64 * 1024 ||
+    if (pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads) < 0 || (plane + Ky + Kz) * 8 > 64 * 1024 ||
         Kx > 256 || pick_maxb(((long)Kx * kConvChunk / 2 + kConvThreads - 1) / kConvThreads) < 0) {
         set_error("PME grid %dx%dx%d too large for the in-LDS plane FFT", Kx, Ky, Kz);
         delete pme;
@@ -587,12 +675,14 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
         if (e == hipSuccess) e = hipMalloc(ptr, bytes);
     };
     A((void**)&pme->d_bc, hbc.size() * sizeof(float));
-    A((void**)&pme->Q, NG * sizeof(float));
     A((void**)&pme->FQ, (size_t)pme->nfft * sizeof(float2));
     A((void**)&pme->R, NG * sizeof(float));
-    A((void**)&pme->base, (size_t)atom_numbers * sizeof(int4));
+    A((void**)&pme->base, (size_t)atom_numbers * sizeof(unsigned));
     A((void**)&pme->frac, (size_t)atom_numbers * sizeof(float4));
     A((void**)&pme->corr_atom, (size_t)atom_numbers * sizeof(float));
+    pme->bin_cap = atom_numbers;  // a plane can never hold more than every atom
+    A((void**)&pme->bin_count, (size_t)Kx * sizeof(int));
+    A((void**)&pme->bin_atoms, (size_t)Kx * pme->bin_cap * sizeof(int));
     A((void**)&pme->acc, SPONGE_NB_NPARTS * sizeof(double));
     for (int d = 0; d < 3; d++) {
         std::vector<float2> tw(pme->K[d]);
@@ -604,8 +694,8 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
         if (e == hipSuccess) e = hipMemcpy(pme->d_tw[d], tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) e = hipMemcpy(pme->d_bc, hbc.data(), hbc.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(pme->Q, 0, NG * sizeof(float));
     if (e == hipSuccess) e = hipMemset(pme->corr_atom, 0, (size_t)atom_numbers * sizeof(float));
+    if (e == hipSuccess) e = hipMemset(pme->bin_count, 0, (size_t)Kx * sizeof(int));
     if (e == hipSuccess) e = hipMemset(pme->acc, 0, SPONGE_NB_NPARTS * sizeof(double));
     if (e != hipSuccess) {
         set_error("sponge_pme_create: %s", hipGetErrorString(e));
@@ -618,11 +708,13 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
 
 extern "C" void sponge_pme_destroy(sponge_pme* pme) {
     if (!pme) return;
-    void* ptrs[] = {pme->d_bc, pme->Q,         pme->FQ,       pme->R,        pme->base,
+    void* ptrs[] = {pme->d_bc, pme->FQ,       pme->R,        pme->base,
                     pme->frac, pme->corr_atom, pme->acc,      pme->d_tw[0],  pme->d_tw[1],
                     pme->d_tw[2]};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    if (pme->bin_count) (void)hipFree(pme->bin_count);
+    if (pme->bin_atoms) (void)hipFree(pme->bin_atoms);
     if (pme->zero_type) (void)hipFree(pme->zero_type);
     if (pme->zero_ab) (void)hipFree(pme->zero_ab);
     snb_scratch_free(&pme->ps);
@@ -651,36 +743,34 @@ extern "C" int sponge_pme_update_box_length(sponge_pme* pme, const float box_len
 
 // The reciprocal pipeline; forces added to d_frc (original atom order).
 int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
-                       int energy, hipStream_t s) {
+                       int energy, int prep_done, hipStream_t s) {
     const int n = pme->n, Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
     FftPlan px, py, pz;
     plan_axis(Kx, &px);
     plan_axis(Ky, &py);
     plan_axis(Kz, &pz);
-    pme_prep_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_uint_crd, d_charge, Kx, Ky, Kz, pme->base, pme->frac,
-                                                     energy, pme->acc);
-    SNB_CHECK_LAUNCH();
-    int nw = ceil_div(n, kSpreadAtomsPerWave);
-    pme_spread_kernel<<<ceil_div(nw, 4), 256, 0, s>>>(n, pme->base, pme->frac, Kx, Ky, Kz, pme->Q);
-    SNB_CHECK_LAUNCH();
+    if (!prep_done) {
+        pme_prep_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_uint_crd, d_charge, pme_prep_args(pme), energy);
+        SNB_CHECK_LAUNCH();
+    }
     long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
     int mb = pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
-    size_t lds = plane * sizeof(float2);
+    size_t lds = (plane + Ky + Kz) * sizeof(float2);
     double* acc = energy ? pme->acc : nullptr;
     switch (mb) {
-        case 1: fwd_plane_kernel<1><<<Kx, kPlaneThreads, lds, s>>>(pme->Q, pme->FQ, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
-        case 2: fwd_plane_kernel<2><<<Kx, kPlaneThreads, lds, s>>>(pme->Q, pme->FQ, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
-        case 4: fwd_plane_kernel<4><<<Kx, kPlaneThreads, lds, s>>>(pme->Q, pme->FQ, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
-        default: fwd_plane_kernel<8><<<Kx, kPlaneThreads, lds, s>>>(pme->Q, pme->FQ, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
+        case 1: fwd_plane_kernel<1><<<Kx, kPlaneThreads, lds, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
+        case 2: fwd_plane_kernel<2><<<Kx, kPlaneThreads, lds, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
+        case 4: fwd_plane_kernel<4><<<Kx, kPlaneThreads, lds, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
+        default: fwd_plane_kernel<8><<<Kx, kPlaneThreads, lds, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
     }
     SNB_CHECK_LAUNCH();
     dim3 cg(Ky, ceil_div(H, kConvChunk));
     int mbc = pick_maxb(((long)Kx * kConvChunk / 2 + kConvThreads - 1) / kConvThreads);
     switch (mbc) {
-        case 1: xconv_kernel<1><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc); break;
-        case 2: xconv_kernel<2><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc); break;
-        case 4: xconv_kernel<4><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc); break;
-        default: xconv_kernel<8><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc); break;
+        case 1: xconv_kernel<1><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count); break;
+        case 2: xconv_kernel<2><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count); break;
+        case 4: xconv_kernel<4><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count); break;
+        default: xconv_kernel<8><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count); break;
     }
     SNB_CHECK_LAUNCH();
     switch (mb) {
@@ -714,7 +804,7 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* pme, const unsigned int* 
     SNB_ARG(pme && d_uint_crd && d_charge && d_frc, "pme_reciprocal_force");
     hipStream_t s = as_stream(stream);
     int energy = (need_virial > 0 || need_energy > 0) ? 1 : 0;
-    int rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, d_frc, energy, s);
+    int rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, d_frc, energy, 0, s);
     if (rc || !energy) return rc;
     return snb_pme_finalize(pme, d_direct_atom_energy, need_energy > 0, need_virial > 0, d_potential, d_virial, s);
 }

@@ -1,0 +1,83 @@
+// Per-atom PME preparation shared by the PME module's own prep kernel and the
+// fused per-step prep kernel (PME_Atom_Near, PME_force.cu:367-421, without
+// the 64-index cache).
+#pragma once
+#include "common.hpp"
+
+namespace snb {
+
+// PME acc slots 6 and 7 hold sum q^2 and sum q between prep and finalize.
+constexpr int kAccQ2 = 6;
+constexpr int kAccQ1 = 7;
+
+struct PmePrep {
+    int Kx, Ky, Kz;
+    unsigned* base;     // x | y << 10 | z << 20
+    float4* frac;       // (fx, fy, fz, q)
+    int* bin_count;     // atoms filed under each x base plane
+    int* bin_atoms;
+    int bin_cap;
+    double* acc;
+};
+
+static inline PmePrep pme_prep_args(sponge_pme* pme) {
+    PmePrep p;
+    p.Kx = pme->K[0];
+    p.Ky = pme->K[1];
+    p.Kz = pme->K[2];
+    p.base = pme->base;
+    p.frac = pme->frac;
+    p.bin_count = pme->bin_count;
+    p.bin_atoms = pme->bin_atoms;
+    p.bin_cap = pme->bin_cap;
+    p.acc = pme->acc;
+    return p;
+}
+
+// Block-cooperative: it contains workgroup barriers, so every thread of the
+// block must reach the same single call site (no call under divergent control
+// flow, no early return before it); i < 0 = no atom. t = u K / 2^32 from one exact 32x32->64-bit product per
+// axis; base index and fraction stored; the atom filed under its x base
+// plane (LDS histogram, one global atomic per block and plane); with energy,
+// sum q^2 and sum q added to acc in FP64.
+__device__ __forceinline__ void pme_prep_block(int i, unsigned ux, unsigned uy, unsigned uz, float q,
+                                               const PmePrep& P, int energy) {
+    __shared__ int hist[256];
+    __shared__ int boff[256];
+    for (int b = threadIdx.x; b < P.Kx; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    double q2 = 0.0, q1 = 0.0;
+    int bx = -1, rank = 0;
+    if (i >= 0) {
+        const float s = 2.3283064365386963e-10f;  // 2^-32
+        unsigned long long tx = (unsigned long long)ux * (unsigned)P.Kx;
+        unsigned long long ty = (unsigned long long)uy * (unsigned)P.Ky;
+        unsigned long long tz = (unsigned long long)uz * (unsigned)P.Kz;
+        bx = (int)(tx >> 32);
+        P.base[i] = (unsigned)bx | ((unsigned)(ty >> 32) << 10) | ((unsigned)(tz >> 32) << 20);
+        P.frac[i] = make_float4((float)(unsigned)tx * s, (float)(unsigned)ty * s, (float)(unsigned)tz * s, q);
+        rank = atomicAdd(hist + bx, 1);
+        if (energy) {
+            q1 = q;
+            q2 = q1 * q1;
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < P.Kx; b += blockDim.x)
+        if (hist[b] > 0) boff[b] = atomicAdd(P.bin_count + b, hist[b]);
+    __syncthreads();
+    if (bx >= 0) {
+        int slot = boff[bx] + rank;
+        if (slot < P.bin_cap) P.bin_atoms[(size_t)bx * P.bin_cap + slot] = i;
+    }
+    if (energy) {
+        q2 = wave_sum(q2);
+        q1 = wave_sum(q1);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(P.acc + kAccQ2, q2);
+            atomicAdd(P.acc + kAccQ1, q1);
+        }
+    }
+}
+
+}  // namespace snb

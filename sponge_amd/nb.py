@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libsponge_nb.so")
 _lib = None
 
 NPARTS = 8
+OVERWRITE_FORCE = 1
 PART_NAMES = ["lj", "direct", "lj_virial", "correction", "self", "reciprocal", "lrc", "pairs_in"]
 
 _P = ctypes.c_void_p
@@ -54,7 +55,7 @@ _SIGS = {
     "sponge_pme_reciprocal_force": (_I, [_P, _P, _P, _P, _I, _I, _P, _P, _P, _P]),
     "sponge_pme_get_energy": (_I, [_P, _P, _P, _P, _FP3, _P, _P, _P, _I, ctypes.POINTER(_D), _P]),
     "sponge_pme_update_box_length": (_I, [_P, _FP3]),
-    "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _P, _P, _P]),
+    "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "sponge_sum_list": (_I, [_I, _P, _P, _P]),
 }
 
@@ -346,12 +347,16 @@ class NonbondedEngine:
     def crd_to_uint(self, stream=None):
         crd_to_uint(self.crd, self.box, self.uint_crd, stream)
 
-    def force_step(self, need_energy=0, need_virial=0, stream=None):
+    def force_step(self, need_energy=0, need_virial=0, from_crd=False, overwrite=False, stream=None):
+        """Fused nonbonded force of one step into self.frc. from_crd: compute the
+        uint coordinates from self.crd first; overwrite: assign self.frc
+        instead of adding to it."""
+        flags = OVERWRITE_FORCE if overwrite else 0
         _check(load_library().sponge_nb_force_step(
-            self.nl.h, self.lj.h, self.pme.h, _ptr(self.uint_crd), _ptr(self.charge),
-            _ptr(self.excl_start), _ptr(self.excl_list), _ptr(self.excl_numbers), _ptr(self.frc),
-            int(need_energy), int(need_virial), _ptr(self.atom_energy), _ptr(self.atom_virial),
-            _ptr(self.parts), _stream(stream)), "nb_force_step")
+            self.nl.h, self.lj.h, self.pme.h, _ptr(self.crd) if from_crd else None, _ptr(self.uint_crd),
+            _ptr(self.charge), _ptr(self.excl_start), _ptr(self.excl_list), _ptr(self.excl_numbers),
+            _ptr(self.frc), int(flags), int(need_energy), int(need_virial), _ptr(self.atom_energy),
+            _ptr(self.atom_virial), _ptr(self.parts), _stream(stream)), "nb_force_step")
 
     def energy_parts(self):
         import torch

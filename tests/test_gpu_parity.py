@@ -162,6 +162,20 @@ def test_separate_entry_points_match_fused(case):
     assert rel_rms(corr.cpu().numpy(), ref["atom_corr"]) < FORCE_TOL
 
 
+def test_fused_from_crd_overwrite(case):
+    """sponge_nb_force_step computing the uint coordinates itself and
+    overwriting a garbage-filled force array gives the same forces."""
+    import torch
+    name, sys_, eng, ref = case
+    eng.uint_crd.fill_(12345)
+    eng.frc.fill_(float("nan"))
+    eng.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+    u = eng.uint_crd.cpu().numpy().view(np.uint32)
+    np.testing.assert_array_equal(u, ref["uint"])
+    assert rel_rms(eng.frc.cpu().numpy(), ref["frc"]) < FORCE_TOL
+
+
 def test_get_energy_queries(case):
     name, sys_, eng, ref = case
     args = (eng.uint_crd, eng.charge, eng.nl, eng.scaler, eng.excl_start, eng.excl_list,
