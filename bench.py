@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--refresh", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 5 times (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -78,6 +80,15 @@ def main():
 
     import torch
     from sponge_amd import nb, system
+
+    if args.copy_probe:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4, "launches": 5}))
+        return
 
     sys_ = system.config(args.config)
     eng = nb.NonbondedEngine(sys_, refresh_interval=args.refresh)

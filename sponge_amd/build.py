@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsponge_nb.so")
+DRIVER = os.path.join(HERE, "sponge_nb_driver")
+HOST_SOURCES = ["host/sponge_host.cpp", "host/nb_driver.cpp"]
 SOURCES = ["abi.hip", "nl.hip", "pair.hip", "pme.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -45,9 +47,23 @@ def build(verbose: bool = True) -> str:
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    _build_driver()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER}")
     return LIB
+
+
+def _build_driver() -> None:
+    """Host-only C++ (the SPONGE class mirror and the step-loop driver),
+    linked against libsponge_nb.so through an $ORIGIN rpath."""
+    tmp = DRIVER + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           *[os.path.join(CSRC, s) for s in HOST_SOURCES], "-o", tmp,
+           "-L", HERE, "-lsponge_nb", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for the driver:\n{r.stdout}{r.stderr}")
+    os.replace(tmp, DRIVER)
 
 
 if __name__ == "__main__":

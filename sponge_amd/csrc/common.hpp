@@ -82,6 +82,18 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// FP32 squared distance (dx*dx + dy*dy) + dz*dz with every operation rounded
+// separately, in this order. This is the neighbor list's distance test (and
+// the pair kernel's cutoff test); the oracle evaluates the identical
+// expression, so it must never be contracted into FMAs.
+__device__ __forceinline__ float dr2_exact(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+    float s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return s;
+}
+
 // Block-wide FP64 sum; every thread must call it. Result valid in thread 0.
 template <int NT>
 __device__ double block_sum(double v, double* smem /* NT/64 */) {

@@ -1,0 +1,231 @@
+// sponge_nb_driver: the nonbonded part of SPONGE's step loop (main.cu:33-46,
+// 48-79, 162-194, 380-392) written against the C++ mirror classes, so the
+// call order a SPONGE build would keep is exercised end to end. Coordinates
+// do not move (no integrator); each step converts coordinates to the uint
+// grid, clears the force, runs the LJ + PME direct pair force, the
+// long-range correction, the excluded-pair correction and the reciprocal
+// force, then updates the neighbor list as Main_Iteration does.
+//
+// Inputs (mdin keys, as MD_core.cu:257-298, 308-345, 864-866, 1053-1080,
+// 1212-1256 read them): coordinate_in_file, charge_in_file, exclude_in_file,
+// LJ_in_file, cutoff, skin, step_limit, write_mdout_interval,
+// neighbor_list_*, PME_*. Extra keys of this driver: frc_out_file (forces
+// after the first step: a line `N`, then N lines `fx fy fz`).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "sponge_host.h"
+
+static void hip_ok(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "Error: %s: %s\n", what, hipGetErrorString(e));
+        std::exit(1);
+    }
+}
+
+static FILE* open_or_die(const char* path, const char* mode) {
+    FILE* fp = std::fopen(path, mode);
+    if (!fp) {
+        std::fprintf(stderr, "Error: cannot open %s\n", path);
+        std::exit(1);
+    }
+    return fp;
+}
+
+struct MdInputs {
+    int atom_numbers = 0;
+    VECTOR box{0, 0, 0};
+    std::vector<VECTOR> crd;
+    std::vector<float> charge;
+    std::vector<int> ex_start, ex_numbers, ex_list;
+};
+
+static void check_count(MdInputs& in, int n, const char* what) {
+    if (n <= 0) {
+        std::fprintf(stderr, "Error: atom_numbers not found in %s\n", what);
+        std::exit(1);
+    }
+    if (in.atom_numbers > 0 && in.atom_numbers != n) {
+        std::fprintf(stderr, "Error: atom_numbers is not equal: %d %d\n", in.atom_numbers, n);
+        std::exit(1);
+    }
+    in.atom_numbers = n;
+}
+
+static void read_inputs(CONTROLLER& c, MdInputs& in) {
+    if (!c.Command_Exist("coordinate_in_file")) {
+        std::fprintf(stderr, "Error: coordinate_in_file is required\n");
+        std::exit(1);
+    }
+    {
+        FILE* fp = open_or_die(c.Command("coordinate_in_file"), "r");
+        char line[1024];
+        int n = 0;
+        if (!std::fgets(line, sizeof(line), fp) || std::sscanf(line, "%d", &n) != 1) n = 0;
+        check_count(in, n, "coordinate_in_file");
+        in.crd.resize(n);
+        for (int i = 0; i < n; i++)
+            if (std::fscanf(fp, "%f %f %f", &in.crd[i].x, &in.crd[i].y, &in.crd[i].z) != 3) {
+                std::fprintf(stderr, "Error: coordinate_in_file ends early\n");
+                std::exit(1);
+            }
+        if (std::fscanf(fp, "%f %f %f", &in.box.x, &in.box.y, &in.box.z) != 3) {
+            std::fprintf(stderr, "Error: box_length not found in coordinate_in_file\n");
+            std::exit(1);
+        }
+        std::fclose(fp);
+    }
+    in.charge.assign(in.atom_numbers, 0.f);
+    if (c.Command_Exist("charge_in_file")) {
+        FILE* fp = open_or_die(c.Command("charge_in_file"), "r");
+        int n = 0;
+        if (std::fscanf(fp, "%d", &n) != 1) n = 0;
+        check_count(in, n, "charge_in_file");
+        for (int i = 0; i < n; i++)
+            if (std::fscanf(fp, "%f", &in.charge[i]) != 1) {
+                std::fprintf(stderr, "Error: charge_in_file ends early\n");
+                std::exit(1);
+            }
+        std::fclose(fp);
+    }
+    in.ex_start.assign(in.atom_numbers, 0);
+    in.ex_numbers.assign(in.atom_numbers, 0);
+    if (c.Command_Exist("exclude_in_file")) {
+        FILE* fp = open_or_die(c.Command("exclude_in_file"), "r");
+        int n = 0, total = 0;
+        if (std::fscanf(fp, "%d %d", &n, &total) != 2) n = 0;
+        check_count(in, n, "exclude_in_file");
+        in.ex_list.reserve(total);
+        for (int i = 0; i < n; i++) {
+            int k = 0;
+            if (std::fscanf(fp, "%d", &k) != 1) k = -1;
+            in.ex_start[i] = (int)in.ex_list.size();
+            in.ex_numbers[i] = k;
+            for (int j = 0; j < k; j++) {
+                int a = -1;
+                if (std::fscanf(fp, "%d", &a) != 1 || a < 0 || a >= n) k = -1;
+                in.ex_list.push_back(a);
+            }
+            if (k < 0 || (int)in.ex_list.size() > total) {
+                std::fprintf(stderr, "Error: malformed exclude_in_file at atom %d\n", i);
+                std::exit(1);
+            }
+        }
+        std::fclose(fp);
+    }
+}
+
+template <class T>
+static T* to_device(const std::vector<T>& h) {
+    T* d = nullptr;
+    size_t bytes = (h.empty() ? 1 : h.size()) * sizeof(T);
+    hip_ok(hipMalloc((void**)&d, bytes), "hipMalloc");
+    if (!h.empty()) hip_ok(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+    return d;
+}
+
+int main(int argc, char** argv) {
+    CONTROLLER controller;
+    if (!controller.Initial(argc, argv)) return 1;
+    controller.printf("%s\n", controller.md_name.c_str());
+
+    MdInputs in;
+    read_inputs(controller, in);
+    const int n = in.atom_numbers;
+    float cutoff = 10.0f, skin = 2.0f;
+    int step_limit = 1, mdout_interval = 1000;
+    if (controller.Command_Exist("cutoff")) cutoff = (float)std::atof(controller.Command("cutoff"));
+    if (controller.Command_Exist("skin")) skin = (float)std::atof(controller.Command("skin"));
+    if (controller.Command_Exist("step_limit")) step_limit = std::atoi(controller.Command("step_limit"));
+    if (controller.Command_Exist("write_mdout_interval"))
+        mdout_interval = std::atoi(controller.Command("write_mdout_interval"));
+    if (mdout_interval <= 0) mdout_interval = 1;
+    if (controller.Command_Exist("check_input")) {  // parse and report only; no device call
+        for (auto& kv : controller.commands) controller.printf("command %s = %s\n", kv.first.c_str(), kv.second.c_str());
+        double qsum = 0;
+        for (float q : in.charge) qsum += q;
+        controller.printf("atoms %d box %g %g %g excluded %zu charge_sum %.6f cutoff %g skin %g steps %d\n", n,
+                          in.box.x, in.box.y, in.box.z, in.ex_list.size(), qsum, cutoff, skin, step_limit);
+        return 0;
+    }
+
+    VECTOR* crd = to_device(in.crd);
+    float* d_charge = to_device(in.charge);
+    int* d_es = to_device(in.ex_start);
+    int* d_en = to_device(in.ex_numbers);
+    int* d_el = to_device(in.ex_list);
+    UNSIGNED_INT_VECTOR* uint_crd = nullptr;
+    VECTOR* frc = nullptr;
+    float *d_atom_energy = nullptr, *d_atom_virial = nullptr, *d_scalars = nullptr;
+    hip_ok(hipMalloc((void**)&uint_crd, n * sizeof(UNSIGNED_INT_VECTOR)), "hipMalloc");
+    hip_ok(hipMalloc((void**)&frc, n * sizeof(VECTOR)), "hipMalloc");
+    hip_ok(hipMalloc((void**)&d_atom_energy, n * sizeof(float)), "hipMalloc");
+    hip_ok(hipMalloc((void**)&d_atom_virial, n * sizeof(float)), "hipMalloc");
+    hip_ok(hipMalloc((void**)&d_scalars, 2 * sizeof(float)), "hipMalloc");  // d_virial, d_potential
+    const VECTOR scaler{in.box.x / 4294967296.0f, in.box.y / 4294967296.0f, in.box.z / 4294967296.0f};
+    const float box3[3] = {in.box.x, in.box.y, in.box.z};
+
+    NEIGHBOR_LIST neighbor_list;
+    LENNARD_JONES_INFORMATION lj;
+    Particle_Mesh_Ewald pme;
+    neighbor_list.Initial(&controller, n, in.box, cutoff, skin);
+    neighbor_list.Neighbor_List_Update(crd, d_es, d_el, d_en, NEIGHBOR_LIST::FORCED_UPDATE);
+    lj.Initial(&controller, cutoff, in.box);
+    pme.Initial(&controller, n, in.box, cutoff);
+
+    auto t0 = std::chrono::steady_clock::now();
+    for (int step = 1; step <= step_limit; step++) {
+        const int need_potential = step % mdout_interval == 0;
+        const int need_pressure = 0;
+        sponge_host_check(sponge_crd_to_uint(n, box3, reinterpret_cast<const float*>(crd),
+                                             reinterpret_cast<unsigned*>(uint_crd), nullptr),
+                          "Crd_To_Uint_Crd");
+        hip_ok(hipMemsetAsync(frc, 0, n * sizeof(VECTOR), nullptr), "reset force");
+        hip_ok(hipMemsetAsync(d_atom_energy, 0, n * sizeof(float), nullptr), "reset atom energy");
+        hip_ok(hipMemsetAsync(d_atom_virial, 0, n * sizeof(float), nullptr), "reset atom virial");
+        hip_ok(hipMemsetAsync(d_scalars, 0, 2 * sizeof(float), nullptr), "reset virial");
+        lj.LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(n, uint_crd, d_charge, frc, neighbor_list.d_nl, pme.beta,
+                                                           need_potential, d_atom_energy, need_pressure,
+                                                           d_atom_virial, pme.d_direct_atom_energy);
+        lj.Long_Range_Correction(need_pressure, d_scalars, need_potential, d_scalars + 1);
+        pme.PME_Excluded_Force_With_Atom_Energy(uint_crd, scaler, d_charge, d_es, d_el, d_en, frc,
+                                                pme.d_correction_atom_energy);
+        pme.PME_Reciprocal_Force_With_Energy_And_Virial(uint_crd, d_charge, frc, need_pressure, need_potential,
+                                                        d_scalars, d_scalars + 1);
+        if (step == 1 && controller.Command_Exist("frc_out_file")) {
+            std::vector<VECTOR> h(n);
+            hip_ok(hipMemcpy(h.data(), frc, n * sizeof(VECTOR), hipMemcpyDeviceToHost), "hipMemcpy");
+            FILE* fp = open_or_die(controller.Command("frc_out_file"), "w");
+            std::fprintf(fp, "%d\n", n);
+            for (int i = 0; i < n; i++) std::fprintf(fp, "%.9g %.9g %.9g\n", h[i].x, h[i].y, h[i].z);
+            std::fclose(fp);
+        }
+        if (need_potential) {
+            float e_pme = pme.Get_Energy(uint_crd, d_charge, neighbor_list.d_nl, scaler, d_es, d_el, d_en);
+            float e_lj = lj.Get_Energy(uint_crd, neighbor_list.d_nl);
+            controller.printf("step %d LJ %.6f PME %.6f reciprocal %.6f self %.6f direct %.6f correction %.6f\n",
+                              step, e_lj, e_pme, pme.reciprocal_ene, pme.self_ene, pme.direct_ene,
+                              pme.correction_ene);
+        }
+        neighbor_list.Neighbor_List_Update(crd, d_es, d_el, d_en);
+    }
+    hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    long pairs = 0;
+    int overflow = 0, rebuilds = 0;
+    sponge_host_check(sponge_nl_status(neighbor_list.d_nl, &pairs, &overflow, &rebuilds), "sponge_nl_status");
+    controller.printf("neighbor pairs %ld overflow %d rebuilds %d\n", pairs, overflow, rebuilds);
+    controller.printf("Core Run Wall Time: %f second(s), %f steps/s\n", secs, step_limit / secs);
+
+    pme.Clear();
+    lj.Clear();
+    neighbor_list.Clear();
+    for (void* p : {(void*)crd, (void*)d_charge, (void*)d_es, (void*)d_en, (void*)d_el, (void*)uint_crd, (void*)frc,
+                    (void*)d_atom_energy, (void*)d_atom_virial, (void*)d_scalars})
+        (void)hipFree(p);
+    return overflow ? 2 : 0;
+}

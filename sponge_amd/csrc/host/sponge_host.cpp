@@ -1,0 +1,459 @@
+// Implementation of the C++ host mirror (see sponge_host.h).
+#include "sponge_host.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <vector>
+
+void sponge_host_check(int rc, const char* what) {
+    if (rc != 0) {
+        std::fprintf(stderr, "Error: %s failed (%d): %s\n", what, rc, sponge_nb_last_error());
+        std::exit(1);
+    }
+}
+
+static void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "Error: %s: %s\n", what, hipGetErrorString(e));
+        std::exit(1);
+    }
+}
+
+// ---------------------------------------------------------------- CONTROLLER
+static std::string first_token(const std::string& v) {
+    std::istringstream is(v);
+    std::string t;
+    is >> t;
+    return t;
+}
+
+bool CONTROLLER::Set_Command(const char* flag, const char* value, const char* prefix) {
+    if (prefix && std::strcmp(prefix, "comments") == 0) return true;
+    std::string key;
+    if (prefix && prefix[0] && std::strcmp(prefix, "main") != 0) key = std::string(prefix) + "_";
+    key += flag;
+    if (commands.count(key)) {
+        std::fprintf(stderr, "Error: %s is set more than once.\n", key.c_str());
+        return false;
+    }
+    original_commands[key] = value;
+    commands[key] = first_token(value);
+    return true;
+}
+
+// Statements end at any of "\n , { }"; the run of enders and blanks after a
+// statement is its ender set (control.cu:229-278). `name {` opens a block
+// whose keys are stored as name_key, `}` closes it, `#` starts a comment that
+// runs to the end of the line and `##name {` opens a commented-out block
+// (control.cu:294-341, Get_Command :205-225).
+bool CONTROLLER::Commands_From_In_File(const char* path) {
+    std::ifstream in(path);
+    if (!in) {
+        std::fprintf(stderr, "Error: cannot open %s\n", path);
+        return false;
+    }
+    std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    size_t p = text.find('\n');
+    md_name = text.substr(0, p);
+    if (p == std::string::npos) return true;
+    auto is_ender = [](char c) { return c == '\n' || c == ',' || c == '{' || c == '}'; };
+    auto is_blank = [](char c) { return c == ' ' || c == '\t'; };
+    std::string prefix;
+    size_t i = p + 1;
+    const size_t n = text.size();
+    while (i < n) {
+        std::string line, ender;
+        while (i < n && line.empty() && is_blank(text[i])) i++;
+        while (i < n && !is_ender(text[i])) line.push_back(text[i++]);
+        if (i < n) ender.push_back(text[i++]);
+        while (i < n && (is_blank(text[i]) || is_ender(text[i])))
+            if (is_ender(text[i++])) ender.push_back(text[i - 1]);
+        const bool opens = ender.find('{') != std::string::npos;
+        const bool closes = ender.find('}') != std::string::npos;
+        if (!line.empty() && line[0] == '#') {
+            if (line.size() > 1 && line[1] == '#') {
+                if (opens) prefix = first_token(line);
+                if (closes) prefix.clear();
+            }
+            if (ender.find('\n') == std::string::npos) {  // the comment runs to the end of the line
+                while (i < n && text[i] != '\n') i++;
+            }
+            continue;
+        }
+        if (opens) {
+            prefix = first_token(line);
+        } else if (!(prefix.size() > 1 && prefix[0] == '#' && prefix[1] == '#')) {
+            size_t eq = line.find('=');
+            if (eq != std::string::npos) {
+                size_t eq2 = line.find('=', eq + 1);
+                std::string flag = first_token(line.substr(0, eq));
+                std::string value = line.substr(eq + 1, eq2 == std::string::npos ? std::string::npos : eq2 - eq - 1);
+                if (!flag.empty() && !first_token(value).empty() &&
+                    !Set_Command(flag.c_str(), value.c_str(), prefix.c_str()))
+                    return false;
+            }
+        }
+        if (closes) prefix.clear();
+    }
+    return true;
+}
+
+bool CONTROLLER::Initial(int argc, char** argv) {
+    // control.cu:164-204: `-flag v1 v2 ...`; a flag is `-` followed by a non-digit
+    auto is_flag = [](const char* a) { return std::strlen(a) > 1 && a[0] == '-' && !(a[1] >= '0' && a[1] <= '9'); };
+    for (int i = 1; i < argc; i++) {
+        if (!is_flag(argv[i])) continue;
+        std::string v;
+        for (int j = i + 1; j < argc && !is_flag(argv[j]); j++) {
+            v += " ";
+            v += argv[j];
+        }
+        if (!Set_Command(argv[i] + 1, v.c_str())) return false;
+    }
+    if (Command_Exist("mdin")) return Commands_From_In_File(Command("mdin"));
+    std::ifstream probe("mdin.txt");
+    if (probe) return Commands_From_In_File("mdin.txt");
+    md_name = "Default SPONGE MD Task Name";
+    return true;
+}
+
+// A missing `<name>_in_file` falls back to `<default_in_file_prefix>_<name>.txt`
+// when that file exists (control.cu:52-87).
+bool CONTROLLER::Command_Exist(const char* key) {
+    if (commands.count(key)) return true;
+    const std::string k(key), tail("_in_file");
+    if (k.size() > tail.size() && k.compare(k.size() - tail.size(), tail.size(), tail) == 0 &&
+        commands.count("default_in_file_prefix")) {
+        std::string path = commands["default_in_file_prefix"] + "_" + k.substr(0, k.size() - tail.size()) + ".txt";
+        if (FILE* f = std::fopen(path.c_str(), "r")) {
+            std::fclose(f);
+            commands[k] = path;
+            return true;
+        }
+    }
+    return false;
+}
+bool CONTROLLER::Command_Exist(const char* prefix, const char* key) {
+    return Command_Exist((std::string(prefix) + "_" + key).c_str());
+}
+const char* CONTROLLER::Command(const char* key) const {
+    auto it = commands.find(key);
+    return it == commands.end() ? "" : it->second.c_str();
+}
+const char* CONTROLLER::Command(const char* prefix, const char* key) const {
+    return Command((std::string(prefix) + "_" + key).c_str());
+}
+bool CONTROLLER::Command_Choice(const char* key, const char* value) const {
+    auto it = commands.find(key);
+    if (it == commands.end()) return false;
+    const std::string& a = it->second;
+    if (a.size() != std::strlen(value)) return false;
+    for (size_t i = 0; i < a.size(); i++)
+        if (std::tolower((unsigned char)a[i]) != std::tolower((unsigned char)value[i])) return false;
+    return true;
+}
+void CONTROLLER::printf(const char* fmt, ...) const {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vfprintf(stdout, fmt, ap);
+    va_end(ap);
+}
+
+// ------------------------------------------------------------- NEIGHBOR_LIST
+void NEIGHBOR_LIST::Initial(CONTROLLER* controller, int md_atom_numbers, VECTOR box, float cut, float sk,
+                            const char* name) {
+    std::snprintf(module_name, sizeof(module_name), "%s", name ? name : "neighbor_list");
+    atom_numbers = md_atom_numbers;
+    float permit_in_mdin = 0.5f;  // mdin value 0.5 = internal 1.0 (neighbor_list.cu:608-611)
+    if (controller->Command_Exist(module_name, "refresh_interval"))
+        refresh_interval = std::atoi(controller->Command(module_name, "refresh_interval"));
+    if (controller->Command_Exist(module_name, "max_atom_in_grid_numbers"))
+        max_atom_in_grid_numbers = std::atoi(controller->Command(module_name, "max_atom_in_grid_numbers"));
+    if (controller->Command_Exist(module_name, "max_neighbor_numbers"))
+        max_neighbor_numbers = std::atoi(controller->Command(module_name, "max_neighbor_numbers"));
+    if (controller->Command_Exist(module_name, "skin_permit"))
+        permit_in_mdin = (float)std::atof(controller->Command(module_name, "skin_permit"));
+    skin_permit = 2.0f * permit_in_mdin;
+    skin = sk;
+    cutoff = cut;
+    box_length = box;
+    float b[3] = {box.x, box.y, box.z};
+    sponge_host_check(sponge_nl_create(atom_numbers, b, cutoff, skin, refresh_interval, max_atom_in_grid_numbers,
+                                       max_neighbor_numbers, permit_in_mdin, &d_nl),
+                      "NEIGHBOR_LIST::Initial");
+    is_initialized = 1;
+}
+
+void NEIGHBOR_LIST::Neighbor_List_Update(VECTOR* crd, int* es, int* el, int* en, int forced_update, int forced_check) {
+    if (!is_initialized) return;
+    sponge_host_check(sponge_nl_update(d_nl, reinterpret_cast<float*>(crd), es, el, en, forced_update, forced_check,
+                                       nullptr),
+                      "Neighbor_List_Update");
+}
+
+void NEIGHBOR_LIST::Update_Volume(VECTOR box) {
+    if (!is_initialized) return;
+    box_length = box;
+    float b[3] = {box.x, box.y, box.z};
+    sponge_host_check(sponge_nl_update_volume(d_nl, b), "NEIGHBOR_LIST::Update_Volume");
+}
+
+void NEIGHBOR_LIST::Clear() {
+    if (is_initialized) {
+        sponge_nl_destroy(d_nl);
+        d_nl = nullptr;
+        is_initialized = 0;
+    }
+}
+
+// ------------------------------------------------- LENNARD_JONES_INFORMATION
+static bool read_parm7_lj(const char* path, int* n, int* ntypes, std::vector<float>& A, std::vector<float>& B,
+                          std::vector<int>& types) {
+    // Lennard_Jones_force.cu:1201-1276: POINTERS (atoms, types),
+    // ATOM_TYPE_INDEX (1-based), LENNARD_JONES_ACOEF / _BCOEF (x12 / x6).
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string line;
+    auto read_block = [&](int count, auto&& sink) {
+        std::getline(in, line);  // %FORMAT line
+        int got = 0;
+        while (got < count && std::getline(in, line)) {
+            std::istringstream is(line);
+            std::string tok;
+            while (got < count && is >> tok) sink(got++, tok);
+        }
+    };
+    while (std::getline(in, line)) {
+        std::istringstream is(line);
+        std::string a, b;
+        if (!(is >> a >> b) || a != "%FLAG") continue;
+        if (b == "POINTERS") {
+            read_block(2, [&](int k, const std::string& t) {
+                if (k == 0) *n = std::atoi(t.c_str());
+                else *ntypes = std::atoi(t.c_str());
+            });
+            int P = *ntypes * (*ntypes + 1) / 2;
+            A.assign(P, 0.f);
+            B.assign(P, 0.f);
+            types.assign(*n, 0);
+        } else if (b == "ATOM_TYPE_INDEX") {
+            read_block(*n, [&](int k, const std::string& t) { types[k] = std::atoi(t.c_str()) - 1; });
+        } else if (b == "LENNARD_JONES_ACOEF") {
+            read_block((int)A.size(), [&](int k, const std::string& t) { A[k] = (float)(12.0 * std::atof(t.c_str())); });
+        } else if (b == "LENNARD_JONES_BCOEF") {
+            read_block((int)B.size(), [&](int k, const std::string& t) { B[k] = (float)(6.0 * std::atof(t.c_str())); });
+        }
+    }
+    return *n > 0 && *ntypes > 0;
+}
+
+void LENNARD_JONES_INFORMATION::Initial(CONTROLLER* controller, float cut, VECTOR box, const char* name) {
+    std::snprintf(module_name, sizeof(module_name), "%s", name ? name : "LJ");
+    std::vector<float> A, B;
+    std::vector<int> t;
+    if (controller->Command_Exist(module_name, "in_file")) {
+        FILE* fp = std::fopen(controller->Command(module_name, "in_file"), "r");
+        if (!fp) {
+            std::fprintf(stderr, "Error: cannot open %s\n", controller->Command(module_name, "in_file"));
+            std::exit(1);
+        }
+        if (std::fscanf(fp, "%d %d", &atom_numbers, &atom_type_numbers) != 2) std::exit(1);
+        pair_type_numbers = atom_type_numbers * (atom_type_numbers + 1) / 2;
+        A.resize(pair_type_numbers);
+        B.resize(pair_type_numbers);
+        t.resize(atom_numbers);
+        for (int i = 0; i < pair_type_numbers; i++) {
+            if (std::fscanf(fp, "%f", &A[i]) != 1) std::exit(1);
+            A[i] *= 12.0f;
+        }
+        for (int i = 0; i < pair_type_numbers; i++) {
+            if (std::fscanf(fp, "%f", &B[i]) != 1) std::exit(1);
+            B[i] *= 6.0f;
+        }
+        for (int i = 0; i < atom_numbers; i++)
+            if (std::fscanf(fp, "%d", &t[i]) != 1) std::exit(1);
+        std::fclose(fp);
+    } else if (controller->Command_Exist("amber_parm7")) {
+        if (!read_parm7_lj(controller->Command("amber_parm7"), &atom_numbers, &atom_type_numbers, A, B, t)) {
+            std::fprintf(stderr, "Error: LJ information not found in %s\n", controller->Command("amber_parm7"));
+            std::exit(1);
+        }
+        pair_type_numbers = atom_type_numbers * (atom_type_numbers + 1) / 2;
+    } else {
+        return;  // not initialised: every method is a no-op, as in the reference
+    }
+    cutoff = cut;
+    h_LJ_A = new float[pair_type_numbers];
+    h_LJ_B = new float[pair_type_numbers];
+    h_atom_LJ_type = new int[atom_numbers];
+    std::memcpy(h_LJ_A, A.data(), pair_type_numbers * sizeof(float));
+    std::memcpy(h_LJ_B, B.data(), pair_type_numbers * sizeof(float));
+    std::memcpy(h_atom_LJ_type, t.data(), atom_numbers * sizeof(int));
+    float b[3] = {box.x, box.y, box.z};
+    sponge_host_check(sponge_lj_create(atom_numbers, atom_type_numbers, h_LJ_A, h_LJ_B, h_atom_LJ_type, cutoff, b,
+                                       &handle),
+                      "LENNARD_JONES_INFORMATION::Initial");
+    // long_range_factor (Lennard_Jones_force.cu:1115-1127), for callers that read it
+    std::vector<double> cnt(atom_type_numbers, 0.0);
+    for (int i = 0; i < atom_numbers; i++) cnt[t[i]] += 1;
+    double s = 0;
+    for (int x = 0; x < atom_type_numbers; x++)
+        for (int y = 0; y < atom_type_numbers; y++) {
+            int hi = x > y ? x : y, lo = x > y ? y : x;
+            s += cnt[x] * cnt[y] * B[hi * (hi + 1) / 2 + lo];
+        }
+    long_range_factor = (float)(s * (-2.0 / 3.0) * M_PI / ((double)cut * cut * cut) / 6.0);
+    is_initialized = 1;
+}
+
+void LENNARD_JONES_INFORMATION::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(
+    const int n, const UNSIGNED_INT_VECTOR* uint_crd, const float* charge, VECTOR* frc, const sponge_nl* nl,
+    const float pme_beta, const int need_atom_energy, float* atom_energy, const int need_virial, float* atom_lj_virial,
+    float* atom_direct_pme_energy) {
+    if (!is_initialized) return;
+    (void)n;
+    sponge_host_check(sponge_lj_pme_direct_force(handle, nl, reinterpret_cast<const unsigned*>(uint_crd), charge,
+                                                 reinterpret_cast<float*>(frc), pme_beta, need_atom_energy,
+                                                 atom_energy, need_virial, atom_lj_virial, atom_direct_pme_energy,
+                                                 nullptr),
+                      "LJ_PME_Direct_Force_With_Atom_Energy_And_Virial");
+}
+
+void LENNARD_JONES_INFORMATION::Long_Range_Correction(int need_pressure, float* d_virial, int need_potential,
+                                                      float* d_potential) {
+    if (!is_initialized) return;
+    sponge_host_check(sponge_lj_long_range_correction(handle, need_pressure, d_virial, need_potential, d_potential,
+                                                      nullptr),
+                      "Long_Range_Correction");
+}
+
+float LENNARD_JONES_INFORMATION::Get_Energy(const UNSIGNED_INT_VECTOR* uint_crd, const sponge_nl* nl, int) {
+    if (!is_initialized) return NAN;
+    double e = 0;
+    sponge_host_check(sponge_lj_get_energy(handle, nl, reinterpret_cast<const unsigned*>(uint_crd), &e, nullptr),
+                      "LJ Get_Energy");
+    h_LJ_energy_sum = (float)e;
+    return h_LJ_energy_sum;
+}
+
+void LENNARD_JONES_INFORMATION::Update_Volume(VECTOR box) {
+    if (!is_initialized) return;
+    float b[3] = {box.x, box.y, box.z};
+    sponge_host_check(sponge_lj_update_volume(handle, b), "LJ Update_Volume");
+}
+
+void LENNARD_JONES_INFORMATION::Clear() {
+    if (!is_initialized) return;
+    sponge_lj_destroy(handle);
+    handle = nullptr;
+    delete[] h_LJ_A;
+    delete[] h_LJ_B;
+    delete[] h_atom_LJ_type;
+    h_LJ_A = h_LJ_B = nullptr;
+    h_atom_LJ_type = nullptr;
+    is_initialized = 0;
+}
+
+// ------------------------------------------------------ Particle_Mesh_Ewald
+void Particle_Mesh_Ewald::Initial(CONTROLLER* controller, int n, VECTOR box, float cut, const char* name) {
+    std::snprintf(module_name, sizeof(module_name), "%s", name ? name : "PME");
+    atom_numbers = n;
+    cutoff = cut;
+    boxlength = box;
+    tolerance = 1e-5f;
+    if (controller->Command_Exist(module_name, "Direct_Tolerance"))
+        tolerance = (float)std::atof(controller->Command(module_name, "Direct_Tolerance"));
+    fftx = ffty = fftz = -1;
+    if (controller->Command_Exist(module_name, "fftx")) fftx = std::atoi(controller->Command(module_name, "fftx"));
+    if (controller->Command_Exist(module_name, "ffty")) ffty = std::atoi(controller->Command(module_name, "ffty"));
+    if (controller->Command_Exist(module_name, "fftz")) fftz = std::atoi(controller->Command(module_name, "fftz"));
+    float b[3] = {box.x, box.y, box.z};
+    sponge_host_check(sponge_pme_create(n, b, cut, tolerance, fftx, ffty, fftz, &handle), "Particle_Mesh_Ewald::Initial");
+    int k[3];
+    sponge_host_check(sponge_pme_info(handle, &beta, k), "PME info");
+    fftx = k[0];
+    ffty = k[1];
+    fftz = k[2];
+    hip_check(hipMalloc((void**)&d_direct_atom_energy, n * sizeof(float)), "hipMalloc");
+    hip_check(hipMalloc((void**)&d_correction_atom_energy, n * sizeof(float)), "hipMalloc");
+    hip_check(hipMemset(d_direct_atom_energy, 0, n * sizeof(float)), "hipMemset");
+    hip_check(hipMemset(d_correction_atom_energy, 0, n * sizeof(float)), "hipMemset");
+    controller->printf("    PME beta %f, fft %d %d %d\n", beta, fftx, ffty, fftz);
+    is_initialized = 1;
+}
+
+void Particle_Mesh_Ewald::PME_Excluded_Force_With_Atom_Energy(const UNSIGNED_INT_VECTOR* uint_crd, const VECTOR sacler,
+                                                              const float* charge, const int* es, const int* el,
+                                                              const int* en, VECTOR* frc, float* atom_energy) {
+    if (!is_initialized) return;
+    float s[3] = {sacler.x, sacler.y, sacler.z};
+    // The reference resets d_correction_atom_energy before its kernel adds
+    // into atom_energy (usually that same array); the library's kernel assigns
+    // each atom's correction energy, so no reset is needed.
+    sponge_host_check(sponge_pme_excluded_force(handle, reinterpret_cast<const unsigned*>(uint_crd), s, charge, es, el,
+                                                en, reinterpret_cast<float*>(frc), atom_energy, nullptr),
+                      "PME_Excluded_Force_With_Atom_Energy");
+}
+
+void Particle_Mesh_Ewald::PME_Reciprocal_Force_With_Energy_And_Virial(const UNSIGNED_INT_VECTOR* uint_crd,
+                                                                      const float* charge, VECTOR* force,
+                                                                      int need_virial, int need_energy,
+                                                                      float* d_virial, float* d_potential) {
+    if (!is_initialized) return;
+    sponge_host_check(sponge_pme_reciprocal_force(handle, reinterpret_cast<const unsigned*>(uint_crd), charge,
+                                                  reinterpret_cast<float*>(force), need_virial, need_energy, d_virial,
+                                                  d_potential, d_direct_atom_energy, nullptr),
+                      "PME_Reciprocal_Force_With_Energy_And_Virial");
+}
+
+float Particle_Mesh_Ewald::Get_Energy(const UNSIGNED_INT_VECTOR* uint_crd, const float* charge, const sponge_nl* nl,
+                                      const VECTOR scaler, const int* es, const int* el, const int* en,
+                                      int which_part, int) {
+    if (!is_initialized) return NAN;
+    float s[3] = {scaler.x, scaler.y, scaler.z};
+    auto part = [&](int w) {
+        double e = 0;
+        sponge_host_check(sponge_pme_get_energy(handle, nl, reinterpret_cast<const unsigned*>(uint_crd), charge, s, es,
+                                                el, en, w, &e, nullptr),
+                          "PME Get_Energy");
+        return (float)e;
+    };
+    if (which_part == TOTAL || which_part == RECIPROCAL) reciprocal_ene = part(RECIPROCAL);
+    if (which_part == TOTAL || which_part == SELF) self_ene = part(SELF);
+    if (which_part == TOTAL || which_part == DIRECT) direct_ene = part(DIRECT);
+    if (which_part == TOTAL || which_part == CORRECTION) correction_ene = part(CORRECTION);
+    switch (which_part) {
+        case TOTAL: return ee_ene = reciprocal_ene + self_ene + direct_ene + correction_ene;
+        case RECIPROCAL: return reciprocal_ene;
+        case SELF: return self_ene;
+        case DIRECT: return direct_ene;
+        default: return correction_ene;
+    }
+}
+
+void Particle_Mesh_Ewald::Update_Box_Length(VECTOR box) {
+    if (!is_initialized) return;
+    boxlength = box;
+    float b[3] = {box.x, box.y, box.z};
+    sponge_host_check(sponge_pme_update_box_length(handle, b), "PME Update_Box_Length");
+}
+
+// The reference's Update_Volume forwards its stale member box
+// (PME_force.cu:996-999); the new box is used here.
+void Particle_Mesh_Ewald::Update_Volume(VECTOR box) { Update_Box_Length(box); }
+
+void Particle_Mesh_Ewald::Clear() {
+    if (!is_initialized) return;
+    sponge_pme_destroy(handle);
+    handle = nullptr;
+    (void)hipFree(d_direct_atom_energy);
+    (void)hipFree(d_correction_atom_energy);
+    d_direct_atom_energy = d_correction_atom_energy = nullptr;
+    is_initialized = 0;
+}

@@ -39,6 +39,7 @@ constexpr int kCandCap = 1024;       // candidate j-clusters per i-cluster (LDS)
 // periodic displacement against the coordinates of the last rebuild.
 __global__ void nl_check_kernel(int n, const float* __restrict__ crd, const float* __restrict__ old_crd,
                                 float bx, float by, float bz, float thresh, int* flags) {
+#pragma clang fp contract(off)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float dx = crd[3 * i] - old_crd[3 * i];
@@ -47,7 +48,7 @@ __global__ void nl_check_kernel(int n, const float* __restrict__ crd, const floa
     dx = dx - floorf(dx / bx + 0.5f) * bx;
     dy = dy - floorf(dy / by + 0.5f) * by;
     dz = dz - floorf(dz / bz + 0.5f) * bz;
-    float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    float r2 = dr2_exact(dx, dy, dz);
     if (r2 > thresh) atomicOr(flags + 7, 1);
 }
 
@@ -226,10 +227,10 @@ __global__ __launch_bounds__(kSortThreads) void nl_sort_kernel(
 // operation rounded in a fixed order (no contraction): must agree bit for
 // bit with the oracle.
 __device__ __forceinline__ float nl_dr2(uint4 a, uint4 b, float cx, float cy, float cz) {
-    float dx = __fmul_rn((float)(int)(b.x - a.x), cx);
-    float dy = __fmul_rn((float)(int)(b.y - a.y), cy);
-    float dz = __fmul_rn((float)(int)(b.z - a.z), cz);
-    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    float dx = (float)(int)(b.x - a.x) * cx;
+    float dy = (float)(int)(b.y - a.y) * cy;
+    float dz = (float)(int)(b.z - a.z) * cz;
+    return dr2_exact(dx, dy, dz);
 }
 
 __device__ __forceinline__ bool excluded_pair(int a, int b, const int* __restrict__ es,

@@ -153,10 +153,10 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
             const int jcn = __shfl((int)my.x, src);
             const unsigned mn = (unsigned)__shfl((int)my.y, src);
             const uint4 bn = pk[jcn * kClusterSize + jj];
-            const float dx = __fmul_rn((float)(int)((b.x & ~3u) - ax), cx);
-            const float dy = __fmul_rn((float)(int)((b.y & ~3u) - ay), cy);
-            const float dz = __fmul_rn((float)(int)((b.z & ~3u) - az), cz);
-            const float r2 = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+            const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+            const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
+            const float dz = (float)(int)((b.z & ~3u) - az) * cz;
+            const float r2 = dr2_exact(dx, dy, dz);
             const bool in = ((m >> bit) & 1u) && r2 < rc2;
             const float r2s = in ? r2 : 1.0f;
             const float rinv = __builtin_amdgcn_rsqf(r2s);

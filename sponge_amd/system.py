@@ -248,3 +248,64 @@ def fft_size(length: float) -> int:
     while not check_2357(t):
         t += 4
     return t
+
+
+# ---------------------------------------------------------------------------
+# SPONGE input files
+# ---------------------------------------------------------------------------
+
+def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
+                        write_mdout_interval: int = 1, refresh_interval: int = 0,
+                        extra: dict | None = None) -> str:
+    """Writes `sys_` as SPONGE text inputs plus an mdin and returns the mdin path.
+
+    Formats are the ones MD_core.cu reads: coordinate_in_file (`N start_time`,
+    N lines `x y z`, then the box; :1212-1256), charge_in_file (`N`, then N
+    charges already x18.2223; :1053-1080), exclude_in_file (`N total`, then
+    per atom `count j...`; :308-345) and LJ_in_file (`N T`, A and B of the
+    T(T+1)/2 pairs before the x12 / x6 scaling, then N types;
+    Lennard_Jones_force.cu:1076-1108).
+    """
+    import os
+    os.makedirs(directory, exist_ok=True)
+    p = lambda name: os.path.join(directory, name)
+    with open(p("coordinate.txt"), "w") as f:
+        f.write(f"{sys_.n} 0.0\n")
+        for x, y, z in sys_.crd.astype(np.float32):
+            f.write(f"{x:.9g} {y:.9g} {z:.9g}\n")
+        f.write(f"{sys_.box[0]:.9g} {sys_.box[1]:.9g} {sys_.box[2]:.9g} 90.0 90.0 90.0\n")
+    with open(p("charge.txt"), "w") as f:
+        f.write(f"{sys_.n}\n")
+        f.write("\n".join(f"{q:.9g}" for q in sys_.charge.astype(np.float32)) + "\n")
+    with open(p("exclude.txt"), "w") as f:
+        f.write(f"{sys_.n} {len(sys_.excl_list)}\n")
+        for i in range(sys_.n):
+            s, k = int(sys_.excl_start[i]), int(sys_.excl_numbers[i])
+            f.write(" ".join([str(k)] + [str(int(j)) for j in sys_.excl_list[s:s + k]]) + "\n")
+    with open(p("LJ.txt"), "w") as f:
+        f.write(f"{sys_.n} {sys_.ntypes}\n")
+        f.write(" ".join(f"{float(a) / 12.0:.17g}" for a in sys_.lj_A) + "\n")
+        f.write(" ".join(f"{float(b) / 6.0:.17g}" for b in sys_.lj_B) + "\n")
+        f.write("\n".join(str(int(t)) for t in sys_.lj_type) + "\n")
+    keys = {
+        "coordinate_in_file": p("coordinate.txt"),
+        "charge_in_file": p("charge.txt"),
+        "exclude_in_file": p("exclude.txt"),
+        "LJ_in_file": p("LJ.txt"),
+        "cutoff": sys_.cutoff,
+        "skin": sys_.skin,
+        "step_limit": step_limit,
+        "write_mdout_interval": write_mdout_interval,
+        "neighbor_list_refresh_interval": refresh_interval,
+        "PME_Direct_Tolerance": sys_.pme_tolerance,
+    }
+    for axis, k in zip("xyz", sys_.fft):
+        if k > 0:
+            keys[f"PME_fft{axis}"] = k
+    keys.update(extra or {})
+    mdin = p("mdin.txt")
+    with open(mdin, "w") as f:
+        f.write(f"{sys_.name}\n")
+        for k, v in keys.items():
+            f.write(f"{k} = {v}\n")
+    return mdin
