@@ -36,6 +36,9 @@ def lib():
             "orc_nl_grid": (Lg, [I, P, P, F, F, P, P, P, P, Lg, P]),
             "orc_direct": (None, [I, P, P, P, P, P, P, D, D, Lg, P, P, P, P, P, P]),
             "orc_excluded": (D, [I, P, P, P, D, P, P, P, P, P]),
+            "orc_direct_soft": (None, [I, P, P, P, P, P, P, P, P, P, D, D, D, D, D, P, D, D, Lg, P,
+                                       P, P, P, P, P]),
+            "orc_lj_soft_lrc_factor": (D, [I, P, P, I, I, P, P, D, D]),
             "orc_pme_beta": (D, [D, D]),
             "orc_bspline_mod2": (D, [I, I]),
             "orc_pme_bc_full": (None, [I, I, I, P, D, P]),
@@ -128,6 +131,36 @@ def direct(sys_, uint_crd, pairs, beta, charge=None):
                      _p(ae), _p(av), _p(ad), _p(tot))
     return dict(frc=frc, atom_energy=ae, atom_virial=av, atom_direct=ad,
                 e_lj=tot[0], e_direct=tot[1], w_lj=tot[2], n_in=int(tot[3]))
+
+
+def direct_soft(fs, uint_crd, pairs, beta, charge=None, lam=None):
+    """FP64 soft-core LJ + PME-direct terms of a FepSystem (LJ_soft_core.cu
+    restated in orc_direct_soft); same return dict as direct()."""
+    n = fs.n
+    u = _c(uint_crd, np.uint32)
+    q = _c(fs.charge if charge is None else charge, np.float32)
+    pr = _c(pairs, np.int32)
+    frc = np.zeros((n, 3))
+    ae = np.zeros(n)
+    av = np.zeros(n)
+    ad = np.zeros(n)
+    tot = np.zeros(4)
+    lib().orc_direct_soft(
+        n, _p(u), _p(q), _p(_c(fs.lj_type, np.int32)), _p(_c(fs.lj_type_B, np.int32)),
+        _p(_c(fs.mask, np.int32)), _p(_c(fs.lj_A, np.float32)), _p(_c(fs.lj_B, np.float32)),
+        _p(_c(fs.lj_A_B, np.float32)), _p(_c(fs.lj_B_B, np.float32)),
+        float(fs.lam if lam is None else lam), float(fs.sc_alpha), float(fs.sc_p), float(fs.sc_sigma),
+        float(fs.sc_sigma_min), _p(_c(fs.box, np.float32)), float(fs.cutoff), float(beta),
+        pr.shape[0], _p(pr), _p(frc), _p(ae), _p(av), _p(ad), _p(tot))
+    return dict(frc=frc, atom_energy=ae, atom_virial=av, atom_direct=ad,
+                e_lj=tot[0], e_direct=tot[1], w_lj=tot[2], n_in=int(tot[3]))
+
+
+def lj_soft_lrc_factor(fs, lam=None):
+    return lib().orc_lj_soft_lrc_factor(
+        fs.n, _p(_c(fs.lj_type, np.int32)), _p(_c(fs.lj_type_B, np.int32)), fs.ntypes, fs.ntypes_B,
+        _p(_c(fs.lj_B, np.float32)), _p(_c(fs.lj_B_B, np.float32)),
+        float(fs.lam if lam is None else lam), float(fs.cutoff))
 
 
 def excluded(sys_, uint_crd, beta, charge=None):

@@ -279,6 +279,138 @@ void orc_direct(int n, const uint32_t* u, const float* q, const int* type, const
     totals[3] = cnt;
 }
 
+// ---------------------------------------------------------------------------
+// a27 soft-core LJ + PME direct pair terms (LJ_soft_core.cu:159-317 force,
+// :320-489 energy, :491-662 virial), FP64, half list, same cutoff predicate
+// and per-atom crediting as orc_direct. Per atom: LJ type in state A and in
+// state B, and a subsystem mask. With lam = lambda, l_ = 1 - lambda, pA / pB
+// the pair indices of the A / B types (tables stored x12 / x6):
+//   soft = mask_i != mask_j or exactly one of AA[pA], BA[pB] is below 1e-6.
+//   not soft: plain LJ with A = l_ AA + lam BA, B = l_ AB + lam BB, plain
+//             Coulomb q_i q_j erfc(b r)/r.
+//   soft:     sA^6 = r^6 + alpha lam^p sigA6, sB^6 = r^6 + alpha l_^p sigB6,
+//             sig6 = soft_core_sigma^6 when A or B of the state is below 1e-6,
+//             else max(A / (2B), sigma_min^6);
+//             E_LJ = l_ (AA/12 sA^-12 - AB/6 sA^-6) + lam (BA/12 sB^-12 - BB/6 sB^-6)
+//             E_C  = q_i q_j (l_ erfc(b sA)/sA + lam erfc(b sB)/sB)
+//             f_LJ = r^4 (l_ (-AA sA^-6 + AB) sA^-12 + lam (-BA sB^-6 + BB) sB^-12)
+//             f_C  = q_i q_j r^4 (l_ (2b/sqrt(pi) e^{-b^2 sA^2} + erfc(b sA)/sA) sA^-6 + (same for B))
+//   F_i += (f_LJ - f_C) dr, W_LJ = -f_LJ r^2. totals as orc_direct.
+void orc_direct_soft(int n, const uint32_t* u, const float* q, const int* typeA, const int* typeB,
+                     const int* mask, const float* AA, const float* AB, const float* BA, const float* BB,
+                     double lambda, double alpha, double p, double sigma, double sigma_min,
+                     const float* box, double rc, double beta, long npairs, const int* pairs, double* frc,
+                     double* atom_e, double* atom_vir, double* atom_dir, double* totals) {
+    double cof[3];
+    float coff[3];
+    for (int d = 0; d < 3; d++) {
+        cof[d] = (double)box[d] / 4294967296.0;
+        coff[d] = (1.0f / 4294967296.0f) * box[d];
+    }
+    const float rcf = (float)rc;
+    const float rc2f = rcf * rcf;
+    const double two_over_sqrtpi = 2.0 / std::sqrt(PI_D);
+    const double l_ = 1.0 - lambda;
+    const double alp = alpha * std::pow(lambda, p), alp_ = alpha * std::pow(l_, p);
+    const double sig6 = std::pow(sigma, 6.0), sig6min = std::pow(sigma_min, 6.0);
+    double e_lj = 0, e_c = 0, w_lj = 0, cnt = 0;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    std::vector<std::vector<double>> fpart(nt, std::vector<double>(3 * (size_t)n, 0.0));
+    std::vector<std::vector<double>> epart(nt, std::vector<double>(3 * (size_t)n, 0.0));
+#pragma omp parallel for reduction(+ : e_lj, e_c, w_lj, cnt) schedule(static)
+    for (long k = 0; k < npairs; k++) {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        int i = pairs[2 * k], j = pairs[2 * k + 1];
+        if (!(nl_dr2(u + 3 * i, u + 3 * j, coff) < rc2f)) continue;
+        double dr[3];
+        for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+        const double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        const double r = std::sqrt(r2);
+        const int pa = pair_index(typeA[i], typeA[j]), pb = pair_index(typeB[i], typeB[j]);
+        const double aa = AA[pa], ab = AB[pa], ba = BA[pb], bb = BB[pb];
+        const double qq = (double)q[i] * (double)q[j];
+        const bool soft = mask[i] != mask[j] || (ba > 1e-6 && aa < 1e-6) || (ba < 1e-6 && aa > 1e-6);
+        double flj, fc, elj, ec;
+        if (!soft) {
+            const double A = l_ * aa + lambda * ba, B = l_ * ab + lambda * bb;
+            const double ri2 = 1.0 / r2, ri6 = ri2 * ri2 * ri2, ri8 = ri6 * ri2;
+            flj = (-A * ri6 + B) * ri8;
+            const double br = beta * r, erfcv = std::erfc(br);
+            fc = qq * (two_over_sqrtpi * br * std::exp(-br * br) + erfcv) * ri2 / r;
+            elj = (A / 12.0 * ri6 - B / 6.0) * ri6;
+            ec = qq * erfcv / r;
+        } else {
+            const double sa6 = (aa < 1e-6 || ab < 1e-6) ? sig6 : std::max(0.5 * aa / ab, sig6min);
+            const double sb6 = (ba < 1e-6 || bb < 1e-6) ? sig6 : std::max(0.5 * ba / bb, sig6min);
+            const double r4 = r2 * r2, r6 = r4 * r2;
+            const double SA6 = r6 + alp * sa6, SB6 = r6 + alp_ * sb6;  // s^6
+            const double iA6 = 1.0 / SA6, iB6 = 1.0 / SB6;
+            const double sA = std::pow(SA6, 1.0 / 6.0), sB = std::pow(SB6, 1.0 / 6.0);
+            flj = r4 * (l_ * (-aa * iA6 + ab) * iA6 * iA6 + lambda * (-ba * iB6 + bb) * iB6 * iB6);
+            const double bA = beta * sA, bB = beta * sB;
+            const double eA = std::erfc(bA), eB = std::erfc(bB);
+            fc = qq * r4 *
+                 (l_ * (two_over_sqrtpi * beta * std::exp(-bA * bA) + eA / sA) * iA6 +
+                  lambda * (two_over_sqrtpi * beta * std::exp(-bB * bB) + eB / sB) * iB6);
+            elj = l_ * (aa / 12.0 * iA6 - ab / 6.0) * iA6 + lambda * (ba / 12.0 * iB6 - bb / 6.0) * iB6;
+            ec = qq * (l_ * eA / sA + lambda * eB / sB);
+        }
+        const double f = flj - fc;
+        for (int d = 0; d < 3; d++) {
+            fpart[t][3 * (size_t)i + d] += f * dr[d];
+            fpart[t][3 * (size_t)j + d] -= f * dr[d];
+        }
+        const double wlj = -flj * r2;
+        epart[t][3 * (size_t)i + 0] += elj + ec;
+        epart[t][3 * (size_t)i + 1] += wlj;
+        epart[t][3 * (size_t)i + 2] += ec;
+        e_lj += elj;
+        e_c += ec;
+        w_lj += wlj;
+        cnt += 1;
+    }
+    for (int t = 0; t < nt; t++)
+        for (size_t a = 0; a < (size_t)n; a++) {
+            for (int d = 0; d < 3; d++) frc[3 * a + d] += fpart[t][3 * a + d];
+            if (atom_e) atom_e[a] += epart[t][3 * a + 0];
+            if (atom_vir) atom_vir[a] += epart[t][3 * a + 1];
+            if (atom_dir) atom_dir[a] += epart[t][3 * a + 2];
+        }
+    totals[0] = e_lj;
+    totals[1] = e_c;
+    totals[2] = w_lj;
+    totals[3] = cnt;
+}
+
+// Long-range dispersion factor of the soft-core module (LJ_soft_core.cu:
+// 88-118, 1588-1600): -2/3 pi / rc^3 / 6 * sum_ij (l_ AB[pA_ij] + lam BB[pB_ij]),
+// every ordered pair (i, j) including i = j; divide by the volume for the energy.
+double orc_lj_soft_lrc_factor(int n, const int* typeA, const int* typeB, int ntA, int ntB, const float* AB,
+                              const float* BB, double lambda, double rc) {
+    std::vector<double> ca(ntA, 0.0), cb(ntB, 0.0);
+    std::vector<double> cab((size_t)ntA * ntB, 0.0);
+    for (int i = 0; i < n; i++) cab[(size_t)typeA[i] * ntB + typeB[i]] += 1.0;
+    double s = 0;
+    for (int a1 = 0; a1 < ntA; a1++)
+        for (int b1 = 0; b1 < ntB; b1++) {
+            const double c1 = cab[(size_t)a1 * ntB + b1];
+            if (c1 == 0) continue;
+            for (int a2 = 0; a2 < ntA; a2++)
+                for (int b2 = 0; b2 < ntB; b2++) {
+                    const double c2 = cab[(size_t)a2 * ntB + b2];
+                    if (c2 == 0) continue;
+                    s += c1 * c2 * ((1.0 - lambda) * AB[pair_index(a1, a2)] + lambda * BB[pair_index(b1, b2)]);
+                }
+        }
+    return s * (-2.0 / 3.0) * PI_D / (rc * rc * rc) / 6.0;
+}
+
 // a24 exclusion correction (PME_force.cu:620-695): for every excluded pair,
 // E -= q_i q_j erf(b r)/r, F_i += -q_i q_j (2/sqrt(pi) b r e^{-b^2r^2} + erfc(b r) - 1)/r^3 dr.
 // Energy credited to atom i (the list owner). Returns the total.

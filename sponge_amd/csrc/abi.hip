@@ -101,6 +101,18 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     const int energy = (need_energy > 0 || need_virial > 0) ? 1 : 0;
     int rc = snb_step_prep(nl, lj, pme, d_crd, d_uint_crd, d_charge, energy, s);
     if (rc) return rc;
+    // Fork: the reciprocal pipeline (spread, FFTs, convolution) runs on the
+    // side stream while the pair and exclusion kernels run on s; it reads only
+    // what step_prep wrote. Join before the gather, which adds into d_frc.
+    const bool serial = (flags & SPONGE_STEP_SERIAL) != 0;
+    hipStream_t ps = serial ? s : pme->side;
+    if (!serial) {
+        SNB_HIP(hipEventRecord(pme->fork, s));
+        SNB_HIP(hipStreamWaitEvent(ps, pme->fork, 0));
+    }
+    rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, energy, 1, ps);
+    if (rc) return rc;
+    if (!serial) SNB_HIP(hipEventRecord(pme->join, ps));
     rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme->beta,
                    energy, &lj->ps, d_frc, need_energy > 0 ? d_atom_energy : nullptr,
                    need_virial > 0 ? d_atom_virial : nullptr, nullptr, s, 1,
@@ -109,7 +121,8 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     rc = snb_excluded(pme, d_uint_crd, lj->cof_dr, d_charge, d_excluded_list_start, d_excluded_list,
                       d_excluded_numbers, d_frc, nullptr, s);
     if (rc) return rc;
-    rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, d_frc, energy, 1, s);
+    if (!serial) SNB_HIP(hipStreamWaitEvent(s, pme->join, 0));
+    rc = snb_pme_gather(pme, d_frc, s);
     if (rc) return rc;
     if (energy) {
         rc = snb_pme_finalize(pme, nullptr, 0, 0, nullptr, nullptr, s);

@@ -194,6 +194,8 @@ struct sponge_pme {
     PairScratch ps;              // direct-energy query (Get_Energy DIRECT part)
     int* zero_type;              // n zeros: every atom of LJ type 0
     float2* zero_ab;             // one (A, B) = (0, 0) entry
+    hipStream_t side;            // reciprocal pipeline of the fused step runs here
+    hipEvent_t fork, join;       // concurrently with the pair kernel
 };
 
 // Internal entry points shared between translation units.
@@ -206,6 +208,7 @@ int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float sc
                  const int* es, const int* el, const int* en, float* d_frc, float* d_atom_energy, hipStream_t s);
 int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
                        int energy, int prep_done, hipStream_t s);
+int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s);
 int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd, unsigned int* d_uint_crd,
                   const float* d_charge, int energy, hipStream_t s);
 int snb_pme_finalize(sponge_pme* pme, const float* d_direct_atom_energy, int need_energy, int need_virial,

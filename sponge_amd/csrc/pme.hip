@@ -41,6 +41,9 @@ struct FftPlan {
 
 constexpr int kPlaneThreads = 512;
 constexpr int kSpreadPerThread = 4;
+// Fixed-point unit of the spread grid: one contribution q*Mx*My*Mz must stay
+// below 2^31 units, i.e. |q| < 128 / (2/3)^3 = 432 (18.2223 x 23.7 e).
+constexpr float kSpreadScale = 16777216.0f;
 constexpr int kConvThreads = 256;
 constexpr int kConvChunk = 16;
 constexpr int kGatherThreads = 256;
@@ -258,14 +261,20 @@ __global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
     const int x = blockIdx.x;
     const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
     const int plane = max(Ky * H, nrp * Kz);
-    float2* tz = sm + plane;
+    float2* tz = sm + max(plane, Ky * Kz);  // past the fixed-point grid too
     float2* ty = tz + Kz;
     load_twiddles<kPlaneThreads>(tz, twz, Kz);
     load_twiddles<kPlaneThreads>(ty, twy, Ky);
     if (x == 0 && threadIdx.x == 0 && acc) acc[SPONGE_PART_RECIPROCAL] = 0.0;
-    for (int idx = threadIdx.x; idx < nrp * Kz; idx += kPlaneThreads) sm[idx] = make_float2(0.f, 0.f);
+    // Spread into a row-major Ky x Kz grid of 64-bit fixed-point sums
+    // (kSpreadScale units) that aliases the front of the plane buffer. LDS
+    // integer atomics are an order of magnitude faster than LDS float atomics
+    // on this chip, and integer sums make the grid independent of the order
+    // in which atoms arrive.
+    unsigned long long* qacc = reinterpret_cast<unsigned long long*>(sm);
+    const int npt = Ky * Kz;
+    for (int idx = threadIdx.x; idx < npt; idx += kPlaneThreads) qacc[idx] = 0ull;
     __syncthreads();
-    float* smf = reinterpret_cast<float*>(sm);
     // The four bins are walked as one flat range; each thread issues the
     // index, base and fraction loads of up to kSpreadPerThread atoms before
     // any LDS update, so their latencies overlap.
@@ -308,7 +317,7 @@ __global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
             if (ai[r] < 0) continue;
             const int by = (int)((bb[r] >> 10) & 1023u), bz = (int)(bb[r] >> 20);
             const float4 f = ff[r];
-            const float wx = f.w * bsp_w(f.x, akx[r]);
+            const float wx = f.w * kSpreadScale * bsp_w(f.x, akx[r]);
             float wz[4];
             int gz[4];
 #pragma unroll
@@ -322,12 +331,35 @@ __global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
                 int gy = by - ky;
                 gy += gy < 0 ? Ky : 0;
                 const float wy = wx * bsp_w(f.y, ky);
-                float* row = smf + 2 * (gy >> 1) * Kz + (gy & 1);
+                unsigned long long* row = qacc + gy * Kz;
 #pragma unroll
-                for (int kz = 0; kz < 4; kz++) atomicAdd(row + 2 * gz[kz], wy * wz[kz]);
+                for (int kz = 0; kz < 4; kz++)
+                    atomicAdd(row + gz[kz], (unsigned long long)(long long)__float2int_rn(wy * wz[kz]));
             }
         }
     }
+    __syncthreads();
+    // Fixed point -> float, into the packed complex layout of the z-FFT: row y
+    // is the real (y even) or imaginary (y odd) part of line y/2; with Ky odd
+    // the imaginary part of the last line is zero.
+    float qv[2 * MAXB];
+#pragma unroll
+    for (int s = 0; s < 2 * MAXB; s++) {
+        const int o = threadIdx.x + s * kPlaneThreads;
+        if (o < npt) qv[s] = (float)((double)(long long)qacc[o] * (1.0 / kSpreadScale));
+    }
+    __syncthreads();
+    float* smf = reinterpret_cast<float*>(sm);
+#pragma unroll
+    for (int s = 0; s < 2 * MAXB; s++) {
+        const int o = threadIdx.x + s * kPlaneThreads;
+        if (o < npt) {
+            const int y = o / Kz, z = o - y * Kz;
+            smf[2 * ((y >> 1) * Kz + z) + (y & 1)] = qv[s];
+        }
+    }
+    if (Ky & 1)
+        for (int z = threadIdx.x; z < Kz; z += kPlaneThreads) smf[2 * ((Ky >> 1) * Kz + z) + 1] = 0.f;
     __syncthreads();
     lds_fft<-1, kPlaneThreads>(sm, nrp, Kz, 1, pz, tz);
     __syncthreads();
@@ -558,8 +590,17 @@ int plan_axis(int K, FftPlan* p) {
     return m == 1 ? 0 : -1;
 }
 
+// Dynamic LDS of the forward plane kernel: the fixed-point spread grid
+// (8 B per point) or the complex plane, whichever is larger, then the z and
+// y twiddles.
+size_t fwd_plane_lds(int Ky, int Kz) {
+    const long H = Kz / 2 + 1;
+    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
+    return (size_t)(std::max(plane, (long)Ky * Kz) + Ky + Kz) * sizeof(float2);
+}
+
 int pick_maxb(long need) {
-    for (int b : {1, 2, 4, 8})
+    for (int b : {1, 2, 4, 8, 16})
         if (need <= b) return b;
     return -1;
 }
@@ -660,11 +701,30 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     }
     const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
     long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
-    if (pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads) < 0 || (plane + Ky + Kz) * 8 > 64 * 1024 ||
-        Kx > 256 || pick_maxb(((long)Kx * kConvChunk / 2 + kConvThreads - 1) / kConvThreads) < 0) {
-        set_error("PME grid %dx%dx%d too large for the in-LDS plane FFT", Kx, Ky, Kz);
+    int dev = 0, max_lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        max_lds = 64 * 1024;
+    const size_t lds_fwd = fwd_plane_lds(Ky, Kz);
+    const int mb = pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
+    if (mb < 0 || lds_fwd > (size_t)max_lds || Kx > 256 ||
+        pick_maxb(((long)Kx * kConvChunk / 2 + kConvThreads - 1) / kConvThreads) < 0) {
+        set_error("PME grid %dx%dx%d too large for the in-LDS plane FFT (%zu B of LDS, %d available)", Kx, Ky, Kz,
+                  lds_fwd, max_lds);
         delete pme;
         return SPONGE_NB_EARG;
+    }
+    if (lds_fwd > 64 * 1024) {
+        const void* fn = mb == 1 ? (const void*)fwd_plane_kernel<1>
+                         : mb == 2 ? (const void*)fwd_plane_kernel<2>
+                         : mb == 4 ? (const void*)fwd_plane_kernel<4>
+                         : mb == 8 ? (const void*)fwd_plane_kernel<8>
+                                   : (const void*)fwd_plane_kernel<16>;
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fwd) != hipSuccess) {
+            set_error("cannot raise the LDS limit of the plane kernel to %zu B", lds_fwd);
+            delete pme;
+            return SPONGE_NB_EHIP;
+        }
     }
     pme->nfft = Kx * Ky * H;
     std::vector<float> hbc;
@@ -697,6 +757,10 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     if (e == hipSuccess) e = hipMemset(pme->corr_atom, 0, (size_t)atom_numbers * sizeof(float));
     if (e == hipSuccess) e = hipMemset(pme->bin_count, 0, (size_t)Kx * sizeof(int));
     if (e == hipSuccess) e = hipMemset(pme->acc, 0, SPONGE_NB_NPARTS * sizeof(double));
+    // default-priority side stream and fork / join events of the fused step
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&pme->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pme->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pme->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         set_error("sponge_pme_create: %s", hipGetErrorString(e));
         sponge_pme_destroy(pme);
@@ -708,6 +772,12 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
 
 extern "C" void sponge_pme_destroy(sponge_pme* pme) {
     if (!pme) return;
+    if (pme->side) {
+        (void)hipStreamSynchronize(pme->side);
+        (void)hipStreamDestroy(pme->side);
+    }
+    if (pme->fork) (void)hipEventDestroy(pme->fork);
+    if (pme->join) (void)hipEventDestroy(pme->join);
     void* ptrs[] = {pme->d_bc, pme->FQ,       pme->R,        pme->base,
                     pme->frac, pme->corr_atom, pme->acc,      pme->d_tw[0],  pme->d_tw[1],
                     pme->d_tw[2]};
@@ -756,13 +826,17 @@ int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const fl
     long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
     int mb = pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
     size_t lds = (plane + Ky + Kz) * sizeof(float2);
+    size_t lds_fwd = fwd_plane_lds(Ky, Kz);
     double* acc = energy ? pme->acc : nullptr;
+#define SNB_FWD(MB) fwd_plane_kernel<MB><<<Kx, kPlaneThreads, lds_fwd, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc)
     switch (mb) {
-        case 1: fwd_plane_kernel<1><<<Kx, kPlaneThreads, lds, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
-        case 2: fwd_plane_kernel<2><<<Kx, kPlaneThreads, lds, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
-        case 4: fwd_plane_kernel<4><<<Kx, kPlaneThreads, lds, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
-        default: fwd_plane_kernel<8><<<Kx, kPlaneThreads, lds, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc); break;
+        case 1: SNB_FWD(1); break;
+        case 2: SNB_FWD(2); break;
+        case 4: SNB_FWD(4); break;
+        case 8: SNB_FWD(8); break;
+        default: SNB_FWD(16); break;
     }
+#undef SNB_FWD
     SNB_CHECK_LAUNCH();
     dim3 cg(Ky, ceil_div(H, kConvChunk));
     int mbc = pick_maxb(((long)Kx * kConvChunk / 2 + kConvThreads - 1) / kConvThreads);
@@ -777,15 +851,21 @@ int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const fl
         case 1: inv_plane_kernel<1><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
         case 2: inv_plane_kernel<2><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
         case 4: inv_plane_kernel<4><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        default: inv_plane_kernel<8><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
+        case 8: inv_plane_kernel<8><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
+        default: inv_plane_kernel<16><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
     }
     SNB_CHECK_LAUNCH();
-    if (d_frc) {
-        pme_gather_kernel<<<ceil_div((long)n * 16, kGatherThreads), kGatherThreads, 0, s>>>(
-            n, pme->base, pme->frac, pme->R, Kx, Ky, Kz, (float)Kx / pme->box[0], (float)Ky / pme->box[1],
-            (float)Kz / pme->box[2], d_frc);
-        SNB_CHECK_LAUNCH();
-    }
+    (void)n;
+    return d_frc ? snb_pme_gather(pme, d_frc, s) : SPONGE_NB_OK;
+}
+
+// Reciprocal forces from the convolved grid R, added to d_frc.
+int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s) {
+    const int n = pme->n, Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2];
+    pme_gather_kernel<<<ceil_div((long)n * 16, kGatherThreads), kGatherThreads, 0, s>>>(
+        n, pme->base, pme->frac, pme->R, Kx, Ky, Kz, (float)Kx / pme->box[0], (float)Ky / pme->box[1],
+        (float)Kz / pme->box[2], d_frc);
+    SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
 

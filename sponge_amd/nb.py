@@ -21,6 +21,7 @@ _lib = None
 
 NPARTS = 8
 OVERWRITE_FORCE = 1
+SERIAL = 2
 PART_NAMES = ["lj", "direct", "lj_virial", "correction", "self", "reciprocal", "lrc", "pairs_in"]
 
 _P = ctypes.c_void_p
@@ -347,11 +348,13 @@ class NonbondedEngine:
     def crd_to_uint(self, stream=None):
         crd_to_uint(self.crd, self.box, self.uint_crd, stream)
 
-    def force_step(self, need_energy=0, need_virial=0, from_crd=False, overwrite=False, stream=None):
+    def force_step(self, need_energy=0, need_virial=0, from_crd=False, overwrite=False, serial=False,
+                   stream=None):
         """Fused nonbonded force of one step into self.frc. from_crd: compute the
         uint coordinates from self.crd first; overwrite: assign self.frc
-        instead of adding to it."""
-        flags = OVERWRITE_FORCE if overwrite else 0
+        instead of adding to it; serial: keep the reciprocal pipeline on the
+        caller's stream instead of overlapping it with the pair kernel."""
+        flags = (OVERWRITE_FORCE if overwrite else 0) | (SERIAL if serial else 0)
         _check(load_library().sponge_nb_force_step(
             self.nl.h, self.lj.h, self.pme.h, _ptr(self.crd) if from_crd else None, _ptr(self.uint_crd),
             _ptr(self.charge), _ptr(self.excl_start), _ptr(self.excl_list), _ptr(self.excl_numbers),

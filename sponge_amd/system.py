@@ -73,6 +73,141 @@ class System:
         return A, B
 
 
+@dataclasses.dataclass
+class FepSystem(System):
+    """A System with SPONGE's soft-core FEP description (LJ_soft_core.cu:
+    1420-1610): LJ types and pair tables of state A (lj_type, lj_A, lj_B) and
+    of state B (lj_type_B, lj_A_B, lj_B_B), the subsystem mask, lambda and the
+    soft-core parameters. `charge` holds the charges at lambda,
+    (1 - lambda) charge_A + lambda charge_B (linear charge scaling)."""
+    lj_type_B: np.ndarray = None
+    lj_A_B: np.ndarray = None
+    lj_B_B: np.ndarray = None
+    mask: np.ndarray = None
+    charge_A: np.ndarray = None
+    charge_B: np.ndarray = None
+    lam: float = 0.0
+    sc_alpha: float = 0.5
+    sc_p: float = 1.0
+    sc_sigma: float = 3.0
+    sc_sigma_min: float = 0.0
+
+    @property
+    def ntypes_B(self) -> int:
+        t = len(self.lj_A_B)
+        return int(round((math.sqrt(8 * t + 1) - 1) / 2))
+
+
+def _lb_table(sig, eps):
+    """Lorentz-Berthelot pair tables of per-type sigma / epsilon, x12 / x6;
+    a type with sigma or epsilon 0 has A = B = 0 with every partner."""
+    T = len(sig)
+    A = np.zeros(T * (T + 1) // 2, np.float64)
+    B = np.zeros_like(A)
+    for a in range(T):
+        for b in range(a + 1):
+            if sig[a] == 0 or sig[b] == 0 or eps[a] == 0 or eps[b] == 0:
+                continue
+            s = 0.5 * (sig[a] + sig[b])
+            e = math.sqrt(eps[a] * eps[b])
+            k = a * (a + 1) // 2 + b
+            A[k] = 4.0 * e * s ** 12
+            B[k] = 4.0 * e * s ** 6
+    return (A * 12.0).astype(np.float32), (B * 6.0).astype(np.float32)
+
+
+WATER_VOLUME = 29.915  # A^3 per water at 1.0 g/cm^3 (18.015 g/mol)
+
+
+def fep_system(n_water: int, lam: float, seed: int = 11, n_solute: int = 12,
+               cutoff: float = 9.0, skin: float = 2.0, name: str = None) -> FepSystem:
+    """One solute of `n_solute` atoms decoupled by lambda in a TIP3P box.
+
+    The box holds `n_water` waters at 1.0 g/cm^3 before the solute is placed
+    at its centre (vertices of a randomly rotated icosahedron of radius 2.6 A
+    for 12 atoms, a random compact cluster otherwise); waters with an atom
+    closer than 2.6 A to a solute atom are removed. Solute charges are
+    N(0, 0.3) e shifted to zero net charge; LJ sigma U(3.0, 3.8) A, epsilon
+    U(0.05, 0.2) kcal/mol, one type per solute atom. State A is the coupled
+    solute, state B has zero solute charge and LJ (type `zero`); the solute
+    is mask 1, water mask 0; solute-solute pairs are excluded.
+    """
+    rng = np.random.default_rng(seed)
+    L = (n_water * WATER_VOLUME) ** (1.0 / 3.0)
+    w = water_box(n_water, L, seed=seed, cutoff=cutoff, skin=skin)
+    c = np.full(3, L / 2)
+    if n_solute == 12:
+        g = (1 + math.sqrt(5)) / 2
+        v = np.array([[0, 1, g], [0, -1, g], [0, 1, -g], [0, -1, -g], [1, g, 0], [-1, g, 0],
+                      [1, -g, 0], [-1, -g, 0], [g, 0, 1], [-g, 0, 1], [g, 0, -1], [-g, 0, -1]], float)
+        v *= 2.6 / np.linalg.norm(v[0])
+    else:
+        v = rng.normal(size=(n_solute, 3))
+        v *= 2.6 / np.linalg.norm(v, axis=1)[:, None]
+    sol = c + v @ _random_rotations(rng, 1)[0].T
+    # drop overlapping waters (minimum image)
+    crd = w.crd.astype(np.float64).reshape(-1, 3, 3)
+    d = crd[:, :, None, :] - sol[None, None, :, :]
+    d -= L * np.round(d / L)
+    keep = ~(np.sqrt((d ** 2).sum(-1)) < 2.6).any(axis=(1, 2))
+    keep &= np.linalg.norm(crd[:, 0] - c, axis=1) > 3.0
+    nw = int(keep.sum())
+    wat = crd[keep].reshape(-1, 3)
+    xyz = np.concatenate([wat, sol]).astype(np.float32)
+    xyz = np.mod(xyz, np.float32(L)).astype(np.float32)
+    xyz[xyz >= np.float32(L)] = 0.0
+    N = 3 * nw + n_solute
+    # types: O 0, H 1, solute 2..2+n_solute-1, zero type last
+    sig = [SPECIES["O"][1], 0.0] + list(rng.uniform(3.0, 3.8, n_solute)) + [0.0]
+    eps = [SPECIES["O"][2], 0.0] + list(rng.uniform(0.05, 0.2, n_solute)) + [0.0]
+    A, B = _lb_table(sig, eps)
+    zero = 2 + n_solute
+    tA = np.concatenate([np.tile([0, 1, 1], nw), 2 + np.arange(n_solute)]).astype(np.int32)
+    tB = tA.copy()
+    tB[3 * nw:] = zero
+    qs = rng.normal(0.0, 0.3, n_solute)
+    qs -= qs.mean()
+    qA = np.concatenate([np.tile([SPECIES["O"][0], SPECIES["H"][0], SPECIES["H"][0]], nw), qs])
+    qA = (qA * CHARGE_SCALE).astype(np.float32)
+    qB = qA.copy()
+    qB[3 * nw:] = 0.0
+    q = ((1.0 - lam) * qA.astype(np.float64) + lam * qB.astype(np.float64)).astype(np.float32)
+    mask = np.zeros(N, np.int32)
+    mask[3 * nw:] = 1
+    mass = np.concatenate([np.tile([15.999, 1.008, 1.008], nw), np.full(n_solute, 12.011)]).astype(np.float32)
+    excl = [[] for _ in range(N)]
+    for k in range(nw):
+        o = 3 * k
+        excl[o] = [o + 1, o + 2]
+        excl[o + 1] = [o + 2]
+    for a in range(n_solute):
+        excl[3 * nw + a] = [3 * nw + b for b in range(a + 1, n_solute)]
+    numbers = np.array([len(e) for e in excl], np.int32)
+    start = np.zeros(N, np.int32)
+    start[1:] = np.cumsum(numbers)[:-1]
+    lst = np.array([j for e in excl for j in e], np.int32)
+    return FepSystem(name=name or f"fep{N}_l{lam:.3f}", box=np.full(3, L, np.float32), crd=xyz,
+                     charge=q, mass=mass, lj_type=tA, lj_A=A, lj_B=B, excl_start=start,
+                     excl_numbers=numbers, excl_list=lst, cutoff=cutoff, skin=skin,
+                     waters=nw, lj_type_B=tB, lj_A_B=A.copy(), lj_B_B=B.copy(), mask=mask,
+                     charge_A=qA, charge_B=qB, lam=float(lam))
+
+
+def fep_batch(seed: int = 11, window_waters: int = 1000, n_windows: int = 16,
+              replica_atoms=(1500, 3000, 4500, 6000, 7500, 9000, 10500, 12000)):
+    """The batched-FEP workload: `n_windows` lambda windows (lambda = k/(n-1))
+    of one solvated solute, plus replicas of other sizes (about the given atom
+    counts) at seeded random lambdas; cutoff 9 A, skin 2 A, PME grids from
+    SPONGE's size rule (about 1 A spacing)."""
+    rng = np.random.default_rng(seed + 1000)
+    out = [fep_system(window_waters, k / (n_windows - 1), seed=seed, name=f"window{k:02d}")
+           for k in range(n_windows)]
+    for r, n in enumerate(replica_atoms):
+        lam = float(rng.uniform(0.0, 1.0))
+        out.append(fep_system(max(1, (n - 12) // 3), lam, seed=seed + 1 + r, name=f"replica{r}"))
+    return out
+
+
 def _random_rotations(rng, n):
     q = rng.normal(size=(n, 4))
     q /= np.linalg.norm(q, axis=1)[:, None]

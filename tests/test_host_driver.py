@@ -122,7 +122,7 @@ def test_driver_step_loop_matches_oracle(tmp_path, name):
     r = run(["-mdin", mdin], timeout=100)
     sys.stdout.write(r.stdout[-2000:])
     assert r.returncode == 0, r.stderr
-    beta = float(re.search(r"PME beta (\S+)", r.stdout).group(1))
+    beta = float(re.search(r"PME beta ([0-9.eE+-]+)", r.stdout).group(1))
     K = tuple(int(v) for v in re.search(r"fft (\d+) (\d+) (\d+)", r.stdout).groups())
     ref = oracle.full_reference(sys_, beta=np.float32(beta), K=K)
     frc = np.loadtxt(tmp_path / "frc.txt", skiprows=1)
