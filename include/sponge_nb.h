@@ -139,6 +139,23 @@ int sponge_lj_update_volume(sponge_lj* lj, const float box_length[3]);
 int sponge_lj_host_tables(const sponge_lj* lj, const float** h_LJ_A, const float** h_LJ_B,
                           const int** h_atom_LJ_type);
 
+/* LJ_SOFT_CORE (Lennard_Jones_force/LJ_soft_core.cuh:37-160): Initial
+ * (LJ_soft_core.cu:1420-1610) after its reads of LJ_soft_core_in_file
+ * (`N TA TB`, AA and AB of state A, BA and BB of state B, then per atom
+ * `typeA typeB`; tables x12 / x6 as stored) and subsys_division_in_file
+ * (h_subsys_division may be NULL: every mask 0). lambda_lj, soft_core_alpha
+ * (0.5), soft_core_powfer (1), soft_core_sigma (3.0), soft_core_sigma_min (0).
+ * The handle is a sponge_lj: LJ_Soft_Core_PME_Direct_Force_With_Atom_Energy_And_Virial
+ * (:1740-1774) is sponge_lj_pme_direct_force on it, and Long_Range_Correction,
+ * Get_Energy, Update_Volume, sponge_nb_force_step take it like a plain handle.
+ * At most 64 distinct (type A, type B, mask) combinations. */
+int sponge_lj_soft_create(int atom_numbers, int atom_type_numbers_A, int atom_type_numbers_B,
+                          const float* h_LJ_AA, const float* h_LJ_AB, const float* h_LJ_BA,
+                          const float* h_LJ_BB, const int* h_atom_LJ_type_A,
+                          const int* h_atom_LJ_type_B, const int* h_subsys_division, float lambda,
+                          float alpha, float p, float sigma, float sigma_min, float cutoff,
+                          const float box_length[3], sponge_lj** out);
+
 /* Measurement hook (no counterpart in the reference): when set, every later
  * pair-kernel launch of this module is bracketed by hipEventRecord of the
  * two hipEvent_t on the launch stream, so a caller can time the dominant
@@ -216,6 +233,45 @@ int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const fl
                          const int* d_excluded_numbers, float* d_frc, int flags, int need_energy,
                          int need_virial, float* d_atom_energy, float* d_atom_virial,
                          double* d_parts, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Batched systems (no counterpart in the reference, which runs one system
+ * per process; the FEP windows of main_fep.cu / TI_core each get their own).
+ * A batch groups already-created module handles of independent systems --
+ * FEP lambda windows (soft-core handles) and replicas of any size -- and runs
+ * each stage of the step as ONE launch over all of them (block maps with
+ * per-system offsets), so small systems fill the GPU together. Per system
+ * the caller names its device arrays once, like MD_INFORMATION owns them. */
+typedef struct sponge_batch sponge_batch;
+typedef struct {
+    float* crd;                     /* wrapped in place by list rebuilds */
+    unsigned int* uint_crd;
+    const float* charge;
+    const int* excluded_list_start;
+    const int* excluded_list;
+    const int* excluded_numbers;
+    float* frc;
+    float* atom_energy;             /* energy steps, may be NULL */
+    float* atom_virial;             /* energy steps, may be NULL */
+    double* parts;                  /* SPONGE_NB_NPARTS doubles, energy steps, may be NULL */
+} sponge_batch_io;
+
+/* Handles stay owned by the caller and must outlive the batch; every system's
+ * list, LJ and PME handles must agree on atom count and box, and all lists on
+ * refresh_interval. Call sponge_batch_sync after changing any member handle
+ * (e.g. *_update_volume). */
+int sponge_batch_create(int nsys, sponge_nl* const* nl, sponge_lj* const* lj, sponge_pme* const* pme,
+                        const sponge_batch_io* io, sponge_batch** out);
+void sponge_batch_destroy(sponge_batch* batch);
+int sponge_batch_sync(sponge_batch* batch);
+/* Neighbor_List_Update of every system (refresh_interval > 0: rebuild all on
+ * the interval; 0: each system's own skin check decides on the device). */
+int sponge_batch_neighbor_list_update(sponge_batch* batch, int forced_update, void* stream);
+/* sponge_nb_force_step of every system; SPONGE_STEP_FROM_CRD computes each
+ * system's uint coordinates from its crd first. */
+#define SPONGE_STEP_FROM_CRD 4
+int sponge_batch_force_step(sponge_batch* batch, int flags, int need_energy, int need_virial,
+                            void* stream);
 
 /* Sum_Of_List / Add_Sum_List (common.cu:192-263, MD_core.cu:173-181) with an
  * FP64 accumulator: *d_out += sum(d_list[0..n)). */

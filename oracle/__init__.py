@@ -260,12 +260,13 @@ def full_reference(sys_, crd=None, charge=None, beta=None, K=None):
         K = tuple(sys_.fft[d] if sys_.fft[d] > 0 else fft_size(sys_.box[d]) for d in range(3))
     pairs = nl_brute(sys_, u)
     q = sys_.charge if charge is None else charge
-    d = direct(sys_, u, pairs, beta, q)
+    soft = getattr(sys_, "lj_type_B", None) is not None
+    d = direct_soft(sys_, u, pairs, beta, q) if soft else direct(sys_, u, pairs, beta, q)
     x = excluded(sys_, u, beta, q)
     r = pme_reciprocal(sys_, u, K, beta, q)
     es, en = pme_self(q, beta, sys_.box)
     V = float(np.prod(sys_.box.astype(np.float64)))
-    lrc = lj_lrc_factor(sys_) / V
+    lrc = (lj_soft_lrc_factor(sys_) if soft else lj_lrc_factor(sys_)) / V
     frc = d["frc"] + x["frc"] + r["frc"]
     return dict(uint=u, crd=crd, pairs=pairs, beta=beta, K=K, frc=frc,
                 frc_direct=d["frc"], frc_excl=x["frc"], frc_rec=r["frc"],

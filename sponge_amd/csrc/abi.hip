@@ -116,7 +116,7 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme->beta,
                    energy, &lj->ps, d_frc, need_energy > 0 ? d_atom_energy : nullptr,
                    need_virial > 0 ? d_atom_virial : nullptr, nullptr, s, 1,
-                   (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0);
+                   (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, lj->d_sc);
     if (rc) return rc;
     rc = snb_excluded(pme, d_uint_crd, lj->cof_dr, d_charge, d_excluded_list_start, d_excluded_list,
                       d_excluded_numbers, d_frc, nullptr, s);

@@ -1,0 +1,161 @@
+// Batched systems: one launch per stage of the nonbonded step over many
+// independent systems (FEP lambda windows, replicas of uneven size).
+//
+// Each batched kernel reads a block map entry {system, block within the
+// system} and the system's BatchSys (its module structs by value plus the
+// caller's arrays) and runs the same device body as the single-system
+// kernel. Maps are built once here; grids are sized from capacities, so the
+// launches are static and the whole step can be captured into one hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace snb;
+
+static int stage_blocks(int st, const sponge_nl* nl, const sponge_pme* pme) {
+    const int n = pme->n, H = pme->K[2] / 2 + 1;
+    switch (st) {
+        case kStPrep: return ceil_div(nl->slot_cap, 256);
+        case kStPair: return ceil_div(nl->ncl_cap, 4);
+        case kStExcl: return ceil_div(n, 256);
+        case kStFwd: return pme->K[0];
+        case kStConv: return pme->K[1] * ceil_div(H, 16);
+        case kStInv: return pme->K[0];
+        case kStGather: return ceil_div(16 * n, 256);
+        case kStFin: return 1;
+        case kStNlCheck: return ceil_div(n, 256);
+        case kStNlBegin: return 1;
+        case kStNlBin: return ceil_div(n, 256);
+        case kStNlScan: return 1;
+        case kStNlSort: return nl->ncol;
+        default: return ceil_div(nl->ncl_cap, 4);  // kStNlPairs
+    }
+}
+
+static void batch_free(sponge_batch* b) {
+    if (!b) return;
+    if (b->d) (void)hipFree(b->d);
+    for (int st = 0; st < kNumStages; st++)
+        if (b->map[st]) (void)hipFree(b->map[st]);
+    delete b;
+}
+
+extern "C" int sponge_batch_sync(sponge_batch* b) {
+    SNB_ARG(b, "batch_sync");
+    for (int s = 0; s < b->nsys; s++) {
+        BatchSys& v = b->h[s];
+        const sponge_nl* nl = b->nl[s];
+        const sponge_lj* lj = b->lj[s];
+        const sponge_pme* pme = b->pme[s];
+        SNB_ARG(nl->box[0] == pme->box[0] && nl->box[1] == pme->box[1] && nl->box[2] == pme->box[2] &&
+                    lj->box[0] == pme->box[0] && lj->box[1] == pme->box[1] && lj->box[2] == pme->box[2],
+                "the modules of a system disagree on the box");
+        v.nl = *nl;
+        v.lj = *lj;
+        v.pme = *pme;
+        v.rlm = snb_nl_rlm(nl);
+        v.thresh = nl->skin_permit * nl->skin_permit * nl->half_skin_square;
+        for (int d = 0; d < 3; d++) v.cq[d] = 0.25f * (4294967296.0f / pme->box[d]);
+        v.lrc = lj->lrc_factor / lj->volume;
+    }
+    SNB_HIP(hipMemcpy(b->d, b->h.data(), b->nsys * sizeof(BatchSys), hipMemcpyHostToDevice));
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_batch_create(int nsys, sponge_nl* const* nl, sponge_lj* const* lj, sponge_pme* const* pme,
+                                   const sponge_batch_io* io, sponge_batch** out) {
+    SNB_ARG(out && nl && lj && pme && io && nsys > 0, "batch_create");
+    *out = nullptr;
+    for (int s = 0; s < nsys; s++) {
+        SNB_ARG(nl[s] && lj[s] && pme[s], "null module handle");
+        SNB_ARG(nl[s]->n == lj[s]->n && nl[s]->n == pme[s]->n, "atom counts of a system's modules differ");
+        SNB_ARG(nl[s]->refresh_interval == nl[0]->refresh_interval, "refresh_interval differs between systems");
+        const sponge_batch_io& o = io[s];
+        SNB_ARG(o.crd && o.uint_crd && o.charge && o.excluded_list_start && o.excluded_list && o.excluded_numbers &&
+                    o.frc,
+                "null system array");
+    }
+    sponge_batch* b = new sponge_batch();
+    b->nsys = nsys;
+    b->nl.assign(nl, nl + nsys);
+    b->lj.assign(lj, lj + nsys);
+    b->pme.assign(pme, pme + nsys);
+    b->h.resize(nsys);
+    b->d = nullptr;
+    for (int st = 0; st < kNumStages; st++) b->map[st] = nullptr;
+    b->refresh_interval = nl[0]->refresh_interval;
+    b->refresh_count = 0;
+    b->max_nt = 0;
+    b->mb_plane = 1;
+    b->soft = 0;
+    b->lds_fwd = b->lds_inv = 0;
+    int rc = SPONGE_NB_OK;
+    for (int s = 0; s < nsys && rc == SPONGE_NB_OK; s++) {
+        rc = snb_lj_scratch(lj[s], nl[s]);
+        b->h[s].io = io[s];
+        snb_pme_plans(pme[s], &b->h[s].px, &b->h[s].py, &b->h[s].pz);
+        b->max_nt = std::max(b->max_nt, lj[s]->ntypes);
+        b->mb_plane = std::max(b->mb_plane, snb_pme_plane_maxb(pme[s]));
+        b->soft |= lj[s]->d_sc != nullptr;
+        b->lds_fwd = std::max(b->lds_fwd, snb_pme_lds_fwd(pme[s]));
+        b->lds_inv = std::max(b->lds_inv, snb_pme_lds_inv(pme[s]));
+    }
+    b->lds_pair = (size_t)b->max_nt * b->max_nt * sizeof(float2);
+    if (rc == SPONGE_NB_OK) rc = snb_batch_pme_attributes(b->mb_plane, b->lds_fwd);
+    if (rc == SPONGE_NB_OK && hipMalloc((void**)&b->d, nsys * sizeof(BatchSys)) != hipSuccess) {
+        set_error("batch_create: out of device memory");
+        rc = SPONGE_NB_EHIP;
+    }
+    for (int st = 0; st < kNumStages && rc == SPONGE_NB_OK; st++) {
+        std::vector<int2> m;
+        for (int s = 0; s < nsys; s++) {
+            const int k = stage_blocks(st, nl[s], pme[s]);
+            for (int j = 0; j < k; j++) m.push_back(make_int2(s, j));
+        }
+        b->nblk[st] = (int)m.size();
+        if (hipMalloc((void**)&b->map[st], m.size() * sizeof(int2)) != hipSuccess ||
+            hipMemcpy(b->map[st], m.data(), m.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("batch_create: block map upload failed");
+            rc = SPONGE_NB_EHIP;
+        }
+    }
+    if (rc == SPONGE_NB_OK) rc = sponge_batch_sync(b);
+    if (rc != SPONGE_NB_OK) {
+        batch_free(b);
+        return rc;
+    }
+    *out = b;
+    return SPONGE_NB_OK;
+}
+
+extern "C" void sponge_batch_destroy(sponge_batch* b) { batch_free(b); }
+
+extern "C" int sponge_batch_neighbor_list_update(sponge_batch* b, int forced_update, void* stream) {
+    SNB_ARG(b, "batch_neighbor_list_update");
+    hipStream_t s = as_stream(stream);
+    if (forced_update) return snb_batch_nl(b, 1, s);
+    if (b->refresh_interval > 0) {
+        int rc = SPONGE_NB_OK;
+        if (b->refresh_count % b->refresh_interval == 0) rc = snb_batch_nl(b, 1, s);
+        b->refresh_count += 1;
+        return rc;
+    }
+    return snb_batch_nl(b, 0, s);
+}
+
+extern "C" int sponge_batch_force_step(sponge_batch* b, int flags, int need_energy, int need_virial, void* stream) {
+    SNB_ARG(b, "batch_force_step");
+    hipStream_t s = as_stream(stream);
+    const int energy = (need_energy > 0 || need_virial > 0) ? 1 : 0;
+    int rc = snb_batch_prep(b, energy, (flags & SPONGE_STEP_FROM_CRD) ? 1 : 0, s);
+    if (!rc) rc = snb_batch_pairs(b, energy, (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, s);
+    if (!rc) rc = snb_batch_excluded(b, s);
+    if (!rc) rc = snb_batch_reciprocal(b, energy, s);
+    if (!rc) rc = snb_batch_gather(b, s);
+    if (!rc && energy) rc = snb_batch_finalize(b, s);
+    return rc;
+}

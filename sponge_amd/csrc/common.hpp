@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <cstdint>
 #include <cstdio>
 
@@ -119,6 +121,13 @@ struct PairScratch {
     hipEvent_t ev_start, ev_stop;  // optional, recorded around the pair kernel
 };
 
+// Mixed-radix pass list of one FFT axis (radix 4 first, then 2, 3, 5, 7).
+struct FftPlan {
+    int K;
+    int npass;
+    int radix[12];
+};
+
 struct sponge_nl {
     int n;
     float box[3];
@@ -166,8 +175,9 @@ struct sponge_lj {
     float* h_A;                  // triangular, x12
     float* h_B;                  // triangular, x6
     int* h_type;
-    float2* d_ab;                // ntypes^2 (A, B) square table
-    int* d_type;                 // n
+    float2* d_ab;                // ntypes^2 (A, B) square table; A < 0 marks a soft-core pair type
+    float4* d_sc;                // soft-core handles: 2 ntypes^2 per-pair-type records, else NULL
+    int* d_type;                 // n (soft-core handles: combined (type A, type B, mask) index)
     PairScratch ps;              // sized from the neighbor list on first use
 };
 
@@ -198,11 +208,48 @@ struct sponge_pme {
     hipEvent_t fork, join;       // concurrently with the pair kernel
 };
 
+// ---- batched systems ------------------------------------------------------
+// Caller-owned device arrays of one system of a batch (MD_INFORMATION's).
+using BatchIo = sponge_batch_io;
+
+// Everything a batched kernel needs of one system: the module structs by
+// value (their device pointers and scalars) plus derived launch constants.
+struct BatchSys {
+    sponge_nl nl;
+    sponge_lj lj;
+    sponge_pme pme;
+    BatchIo io;
+    FftPlan px, py, pz;
+    float rlm;                  // list radius with the bounding-box margin
+    float thresh;               // skin-check threshold (squared)
+    float cq[3];                // MD_core's 0.25 * 2^32 / L
+    double lrc;                 // lrc_factor / volume
+};
+
+enum BatchStage {
+    kStPrep, kStPair, kStExcl, kStFwd, kStConv, kStInv, kStGather, kStFin,
+    kStNlCheck, kStNlBegin, kStNlBin, kStNlScan, kStNlSort, kStNlPairs, kNumStages
+};
+
+struct sponge_batch {
+    int nsys;
+    std::vector<sponge_nl*> nl;
+    std::vector<sponge_lj*> lj;
+    std::vector<sponge_pme*> pme;
+    std::vector<BatchSys> h;    // host copy of d
+    BatchSys* d;
+    int2* map[kNumStages];      // block -> (system, block within the system); x = -1 past the end
+    int nblk[kNumStages];
+    int max_nt, mb_plane, soft;
+    size_t lds_pair, lds_fwd, lds_inv;
+    int refresh_interval, refresh_count;
+};
+
 // Internal entry points shared between translation units.
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered = 0,
-              int overwrite = 0);
+              int overwrite = 0, const float4* sc = nullptr);
 void snb_scratch_free(PairScratch* ps);
 int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float scaler[3], const float* d_charge,
                  const int* es, const int* el, const int* en, float* d_frc, float* d_atom_energy, hipStream_t s);
@@ -214,3 +261,17 @@ int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const flo
 int snb_pme_finalize(sponge_pme* pme, const float* d_direct_atom_energy, int need_energy, int need_virial,
                      float* d_potential, float* d_virial, hipStream_t s);
 int snb_fill_zero_f64(double* p, int n, hipStream_t s);
+void snb_pme_plans(const sponge_pme* pme, FftPlan* px, FftPlan* py, FftPlan* pz);
+int snb_pme_plane_maxb(const sponge_pme* pme);
+size_t snb_pme_lds_fwd(const sponge_pme* pme);
+size_t snb_pme_lds_inv(const sponge_pme* pme);
+float snb_nl_rlm(const sponge_nl* nl);
+int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s);
+int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s);
+int snb_batch_excluded(sponge_batch* b, hipStream_t s);
+int snb_lj_scratch(sponge_lj* lj, const sponge_nl* nl);
+int snb_batch_reciprocal(sponge_batch* b, int energy, hipStream_t s);
+int snb_batch_gather(sponge_batch* b, hipStream_t s);
+int snb_batch_finalize(sponge_batch* b, hipStream_t s);
+int snb_batch_pme_attributes(int mb, size_t lds_fwd);
+int snb_batch_nl(sponge_batch* b, int forced, hipStream_t s);

@@ -37,10 +37,11 @@ constexpr int kCandCap = 1024;       // candidate j-clusters per i-cluster (LDS)
 
 // Is_need_refresh_neighbor_list_cuda (neighbor_list.cu:336-351): FP32
 // periodic displacement against the coordinates of the last rebuild.
-__global__ void nl_check_kernel(int n, const float* __restrict__ crd, const float* __restrict__ old_crd,
-                                float bx, float by, float bz, float thresh, int* flags) {
+__device__ __forceinline__ void nl_check_body(int blk, int n, const float* __restrict__ crd,
+                                              const float* __restrict__ old_crd, float bx, float by, float bz,
+                                              float thresh, int* flags) {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float dx = crd[3 * i] - old_crd[3 * i];
     float dy = crd[3 * i + 1] - old_crd[3 * i + 1];
@@ -52,8 +53,13 @@ __global__ void nl_check_kernel(int n, const float* __restrict__ crd, const floa
     if (r2 > thresh) atomicOr(flags + 7, 1);
 }
 
+__global__ void nl_check_kernel(int n, const float* __restrict__ crd, const float* __restrict__ old_crd,
+                                float bx, float by, float bz, float thresh, int* flags) {
+    nl_check_body(blockIdx.x, n, crd, old_crd, bx, by, bz, thresh, flags);
+}
+
 // Latches the rebuild decision into flags[0] and clears per-rebuild counters.
-__global__ void nl_begin_kernel(int forced, int* flags, int* col_count, int ncol) {
+__device__ __forceinline__ void nl_begin_body(int forced, int* flags, int* col_count, int ncol) {
     __shared__ int go;
     if (threadIdx.x == 0) {
         int f = forced ? 1 : flags[7];
@@ -72,15 +78,19 @@ __global__ void nl_begin_kernel(int forced, int* flags, int* col_count, int ncol
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) col_count[c] = 0;
 }
 
+__global__ void nl_begin_kernel(int forced, int* flags, int* col_count, int ncol) {
+    nl_begin_body(forced, flags, col_count, ncol);
+}
+
 // Crd_Periodic_Map (common.cu:299-352), old_crd copy (neighbor_list.cu:474)
 // and Crd_To_Uint_Crd with the list's own cof (common.cu:264-287,
 // neighbor_list.cu:622), then the column of each atom from its uint x, y.
-__global__ void nl_bin_kernel(int n, float* __restrict__ crd, float* __restrict__ old_crd,
-                              unsigned int* __restrict__ nl_uint, float bx, float by, float bz,
-                              float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap,
-                              int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags) {
+__device__ __forceinline__ void nl_bin_body(int blk, int n, float* __restrict__ crd, float* __restrict__ old_crd,
+                                            unsigned int* __restrict__ nl_uint, float bx, float by, float bz,
+                                            float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap,
+                                            int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags) {
     if (!flags[0]) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float b[3] = {bx, by, bz};
     float cq[3] = {cqx, cqy, cqz};
@@ -107,9 +117,17 @@ __global__ void nl_bin_kernel(int n, float* __restrict__ crd, float* __restrict_
     else atomicOr(flags + 1, SPONGE_NL_OVERFLOW_COLUMN);
 }
 
+__global__ void nl_bin_kernel(int n, float* __restrict__ crd, float* __restrict__ old_crd,
+                              unsigned int* __restrict__ nl_uint, float bx, float by, float bz,
+                              float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap,
+                              int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags) {
+    nl_bin_body(blockIdx.x, n, crd, old_crd, nl_uint, bx, by, bz, cqx, cqy, cqz, ncx, ncy, col_cap, col_count,
+                col_atoms, flags);
+}
+
 // Exclusive scan of the padded (multiple of 4) column sizes; one block.
-__global__ void nl_scan_kernel(int ncol, int col_cap, const int* __restrict__ col_count,
-                               int* __restrict__ col_start, int* flags) {
+__device__ __forceinline__ void nl_scan_body(int ncol, int col_cap, const int* __restrict__ col_count,
+                                             int* __restrict__ col_start, int* flags) {
     if (!flags[0]) return;
     __shared__ int wsum[16];
     __shared__ int carry;
@@ -148,17 +166,21 @@ __global__ void nl_scan_kernel(int ncol, int col_cap, const int* __restrict__ co
     }
 }
 
+__global__ void nl_scan_kernel(int ncol, int col_cap, const int* __restrict__ col_count,
+                               int* __restrict__ col_start, int* flags) {
+    nl_scan_body(ncol, col_cap, col_count, col_start, flags);
+}
+
 // One block per column: sort its atoms by (uint z, atom index) -- a
 // deterministic order -- write the slot maps, the sorted uint coordinates and
 // the bounding box of every 4-atom cluster.
-__global__ __launch_bounds__(kSortThreads) void nl_sort_kernel(
-    int col_cap, const int* __restrict__ col_count, const int* __restrict__ col_start,
+__device__ __forceinline__ void nl_sort_body(
+    int c, int col_cap, const int* __restrict__ col_count, const int* __restrict__ col_start,
     const int* __restrict__ col_atoms, const unsigned int* __restrict__ nl_uint,
     float crx, float cry, float crz, int* __restrict__ s2o, int* __restrict__ o2s,
     uint4* __restrict__ pack, float4* __restrict__ cl_lo, float4* __restrict__ cl_hi, const int* flags) {
     if (!flags[0]) return;
     __shared__ unsigned long long key[kMaxColumn];
-    int c = blockIdx.x;
     int cnt = min(col_count[c], col_cap);
     int start = col_start[c];
     int pad = (cnt + 3) & ~3;
@@ -223,6 +245,15 @@ __global__ __launch_bounds__(kSortThreads) void nl_sort_kernel(
     }
 }
 
+__global__ __launch_bounds__(kSortThreads) void nl_sort_kernel(
+    int col_cap, const int* __restrict__ col_count, const int* __restrict__ col_start,
+    const int* __restrict__ col_atoms, const unsigned int* __restrict__ nl_uint,
+    float crx, float cry, float crz, int* __restrict__ s2o, int* __restrict__ o2s,
+    uint4* __restrict__ pack, float4* __restrict__ cl_lo, float4* __restrict__ cl_hi, const int* flags) {
+    nl_sort_body(blockIdx.x, col_cap, col_count, col_start, col_atoms, nl_uint, crx, cry, crz, s2o, o2s, pack, cl_lo,
+                 cl_hi, flags);
+}
+
 // FP32 neighbor-list distance, the reference's expression with every
 // operation rounded in a fixed order (no contraction): must agree bit for
 // bit with the oracle.
@@ -245,8 +276,8 @@ __device__ __forceinline__ bool excluded_pair(int a, int b, const int* __restric
 // One wave per i-cluster: gather candidate j-clusters (index >= i) from the
 // columns within rc+skin, test bounding boxes, then compute the exact 16-bit
 // pair masks four cluster pairs at a time and append the non-empty ones.
-__global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
-    int ncx, int ncy, float bx, float by, float bz, float crx, float cry, float crz, float rl2,
+__device__ __forceinline__ void nl_pairs_body(
+    int lb, int ncx, int ncy, float bx, float by, float bz, float crx, float cry, float crz, float rl2,
     float rlm, const int* __restrict__ col_start, const float4* __restrict__ cl_lo,
     const float4* __restrict__ cl_hi, const uint4* __restrict__ pack, const int* __restrict__ es,
     const int* __restrict__ en, const int* __restrict__ el, int list_cap, int* __restrict__ list_n,
@@ -254,7 +285,6 @@ __global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
     if (!flags[0]) return;
     __shared__ int cand[kPairWaves][kCandCap];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int lb = xcd_block(gridDim.x);
     const int ic = lb * kPairWaves + w;
     const int ncl = flags[4];
     if (lb < 0 || ic >= ncl) return;
@@ -345,6 +375,69 @@ __global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
         // every unordered pair is listed from both of its atoms
         atomicAdd((unsigned long long*)(flags + 8), (unsigned long long)npairs);
     }
+}
+
+__global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
+    int ncx, int ncy, float bx, float by, float bz, float crx, float cry, float crz, float rl2,
+    float rlm, const int* __restrict__ col_start, const float4* __restrict__ cl_lo,
+    const float4* __restrict__ cl_hi, const uint4* __restrict__ pack, const int* __restrict__ es,
+    const int* __restrict__ en, const int* __restrict__ el, int list_cap, int* __restrict__ list_n,
+    uint2* __restrict__ list, int* flags) {
+    nl_pairs_body(xcd_block(gridDim.x), ncx, ncy, bx, by, bz, crx, cry, crz, rl2, rlm, col_start, cl_lo, cl_hi, pack,
+                  es, en, el, list_cap, list_n, list, flags);
+}
+
+// ---- batched rebuild: one launch per stage over every system --------------
+__global__ void nl_check_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    nl_check_body(m.y, b.nl.n, b.io.crd, b.nl.old_crd, b.nl.box[0], b.nl.box[1], b.nl.box[2], b.thresh, b.nl.flags);
+}
+
+__global__ void nl_begin_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map, int forced) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    nl_begin_body(forced, b.nl.flags, b.nl.col_count, b.nl.ncol);
+}
+
+__global__ void nl_bin_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const sponge_nl& L = b.nl;
+    nl_bin_body(m.y, L.n, b.io.crd, L.old_crd, L.nl_uint, L.box[0], L.box[1], L.box[2], L.cof_q[0], L.cof_q[1],
+                L.cof_q[2], L.ncx, L.ncy, L.col_cap, L.col_count, L.col_atoms, L.flags);
+}
+
+__global__ void nl_scan_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const sponge_nl& L = bs[m.x].nl;
+    nl_scan_body(L.ncol, L.col_cap, L.col_count, L.col_start, L.flags);
+}
+
+__global__ __launch_bounds__(kSortThreads) void nl_sort_batch(const BatchSys* __restrict__ bs,
+                                                              const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const sponge_nl& L = bs[m.x].nl;
+    nl_sort_body(m.y, L.col_cap, L.col_count, L.col_start, L.col_atoms, L.nl_uint, L.cof_dr[0], L.cof_dr[1],
+                 L.cof_dr[2], L.s2o, L.o2s, L.pack, L.cl_lo, L.cl_hi, L.flags);
+}
+
+__global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_batch(const BatchSys* __restrict__ bs,
+                                                                  const int2* __restrict__ map, int nmap) {
+    const int g = xcd_block(nmap);
+    if (g < 0) return;
+    const int2 m = map[g];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const sponge_nl& L = b.nl;
+    nl_pairs_body(m.y, L.ncx, L.ncy, L.box[0], L.box[1], L.box[2], L.cof_dr[0], L.cof_dr[1], L.cof_dr[2], L.rl2,
+                  b.rlm, L.col_start, L.cl_lo, L.cl_hi, L.pack, b.io.excluded_list_start, b.io.excluded_numbers, b.io.excluded_list,
+                  L.list_cap, L.list_n, L.list, L.flags);
 }
 
 __global__ void nl_uint_kernel(int n, const float* __restrict__ crd, float cqx, float cqy, float cqz,
@@ -467,6 +560,10 @@ extern "C" void sponge_nl_destroy(sponge_nl* nl) {
     delete nl;
 }
 
+// Bounding-box search radius: rc + skin with a small margin for the FP32
+// rounding of the cluster boxes.
+float snb_nl_rlm(const sponge_nl* nl) { return std::sqrt(nl->rl2) + 1.0e-3f * (1.0f + std::sqrt(nl->rl2)); }
+
 // Launches the rebuild pipeline; `forced` latches the decision on the host,
 // otherwise the device flag written by nl_check_kernel decides.
 static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el, const int* en, int forced,
@@ -484,7 +581,7 @@ static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el,
                                                       nl->nl_uint, nl->cof_dr[0], nl->cof_dr[1], nl->cof_dr[2],
                                                       nl->s2o, nl->o2s, nl->pack, nl->cl_lo, nl->cl_hi, nl->flags);
     SNB_CHECK_LAUNCH();
-    const float rlm = std::sqrt(nl->rl2) + 1.0e-3f * (1.0f + std::sqrt(nl->rl2));
+    const float rlm = snb_nl_rlm(nl);
     int nblk = xcd_grid(ceil_div(nl->ncl_cap, kPairWaves));
     nl_pairs_kernel<<<nblk, kPairWaves * 64, 0, s>>>(
         nl->ncx, nl->ncy, nl->box[0], nl->box[1], nl->box[2], nl->cof_dr[0], nl->cof_dr[1], nl->cof_dr[2], nl->rl2,
@@ -567,4 +664,25 @@ extern "C" int sponge_nl_get_pairs(sponge_nl* nl, int* h_pairs, long capacity, l
         }
     if (pair_count) *pair_count = m;
     return f[1] ? SPONGE_NB_ECAPACITY : SPONGE_NB_OK;
+}
+
+// Batched Neighbor_List_Update: forced = rebuild every system; otherwise the
+// skin check of every system votes into its own flag (skin-triggered mode).
+int snb_batch_nl(sponge_batch* b, int forced, hipStream_t s) {
+    if (!forced) {
+        nl_check_batch<<<b->nblk[kStNlCheck], 256, 0, s>>>(b->d, b->map[kStNlCheck]);
+        SNB_CHECK_LAUNCH();
+    }
+    nl_begin_batch<<<b->nblk[kStNlBegin], 256, 0, s>>>(b->d, b->map[kStNlBegin], forced);
+    SNB_CHECK_LAUNCH();
+    nl_bin_batch<<<b->nblk[kStNlBin], 256, 0, s>>>(b->d, b->map[kStNlBin]);
+    SNB_CHECK_LAUNCH();
+    nl_scan_batch<<<b->nblk[kStNlScan], 1024, 0, s>>>(b->d, b->map[kStNlScan]);
+    SNB_CHECK_LAUNCH();
+    nl_sort_batch<<<b->nblk[kStNlSort], kSortThreads, 0, s>>>(b->d, b->map[kStNlSort]);
+    SNB_CHECK_LAUNCH();
+    const int nmap = b->nblk[kStNlPairs];
+    nl_pairs_batch<<<xcd_grid(nmap), kPairWaves * 64, 0, s>>>(b->d, b->map[kStNlPairs], nmap);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
 }

@@ -13,8 +13,11 @@
 // with plain read-modify-writes: no atomics anywhere in the pair path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "common.hpp"
@@ -54,12 +57,12 @@ __global__ void gather_kernel(const int* __restrict__ flags, const int* __restri
 // optional MD_Information_Crd_To_Uint_Crd (MD_core.cu:1528-1532, written back
 // to the caller's uint array), the pair-kernel record, and the PME base,
 // fraction and x-plane bin of the slot's atom.
-__global__ __launch_bounds__(256) void step_prep_kernel(const int* __restrict__ flags, const int* __restrict__ s2o,
-                                                        const float* __restrict__ crd, float cqx, float cqy, float cqz,
-                                                        unsigned int* __restrict__ uc, const float* __restrict__ q,
-                                                        const int* __restrict__ type, uint4* __restrict__ pk,
-                                                        PmePrep P, int energy, double* __restrict__ acc) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void step_prep_body(int blk, const int* __restrict__ flags, const int* __restrict__ s2o,
+                                               const float* __restrict__ crd, float cqx, float cqy, float cqz,
+                                               unsigned int* __restrict__ uc, const float* __restrict__ q,
+                                               const int* __restrict__ type, uint4* __restrict__ pk,
+                                               const PmePrep& P, int energy, double* __restrict__ acc) {
+    const int p = blk * blockDim.x + threadIdx.x;
     if (energy && p < 3) acc[p] = 0.0;  // SPONGE_PART_LJ, _DIRECT, _LJ_VIRIAL
     if (energy && p == 3) acc[SPONGE_PART_PAIRS_IN] = 0.0;
     const int nslot = flags[3];
@@ -89,6 +92,24 @@ __global__ __launch_bounds__(256) void step_prep_kernel(const int* __restrict__ 
     pme_prep_block(a, u0, u1, u2, qa, P, energy);
 }
 
+__global__ __launch_bounds__(256) void step_prep_kernel(const int* __restrict__ flags, const int* __restrict__ s2o,
+                                                        const float* __restrict__ crd, float cqx, float cqy, float cqz,
+                                                        unsigned int* __restrict__ uc, const float* __restrict__ q,
+                                                        const int* __restrict__ type, uint4* __restrict__ pk,
+                                                        PmePrep P, int energy, double* __restrict__ acc) {
+    step_prep_body(blockIdx.x, flags, s2o, crd, cqx, cqy, cqz, uc, q, type, pk, P, energy, acc);
+}
+
+__global__ __launch_bounds__(256) void step_prep_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map,
+                                                       int energy, int from_crd) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const PmePrep P = pme_prep_args(&b.pme);
+    step_prep_body(m.y, b.nl.flags, b.nl.s2o, from_crd ? b.io.crd : nullptr, b.cq[0], b.cq[1], b.cq[2],
+                   b.io.uint_crd, b.io.charge, b.lj.d_type, b.lj.ps.pk, P, energy, b.lj.ps.acc);
+}
+
 __device__ __forceinline__ int unpack_type(uint4 v) { return (int)((v.x & 3u) | ((v.y & 3u) << 2) | ((v.z & 3u) << 4)); }
 
 // erfc(x) = exp(-x^2) * erfcx(x) with erfcx(x) ~ t P7(t), t = 1/(1 + p x):
@@ -109,21 +130,64 @@ __device__ __forceinline__ float erfcx_poly(float x) {
     return t * p;
 }
 
+// Soft-core pair terms (LJ_soft_core.cu:254-300): with s^6 = r^6 + c per end
+// state, F = (f_LJ - f_C) dr where
+//   f_LJ = r^4 [(l_AB - l_AA s_A^-6) s_A^-12 + (lB_B - lB_A s_B^-6) s_B^-12]
+//   f_C  = q_i q_j r^4 [l_ (2b/sqrt(pi) e^{-b^2 s_A^2} + erfc(b s_A)/s_A) s_A^-6 + (B alike)]
+// s0 = {l_ AA, l_ AB, lam BA, lam BB} (l_ = 1 - lambda), s1 = {c_A, c_B, l_, lam}.
+// s = (s^6)^(1/6) is taken as exp2(log2(s^6) / 6).
+__device__ __forceinline__ void soft_pair(float r2, float qq, float beta, float mlog2e_b2, float4 s0, float4 s1,
+                                          float& flj, float& fc) {
+    const float r4 = r2 * r2, r6 = r4 * r2;
+    const float SA6 = r6 + s1.x, SB6 = r6 + s1.y;
+    const float iA6 = __builtin_amdgcn_rcpf(SA6), iB6 = __builtin_amdgcn_rcpf(SB6);
+    flj = r4 * ((s0.y - s0.x * iA6) * iA6 * iA6 + (s0.w - s0.z * iB6) * iB6 * iB6);
+    const float sA = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(SA6) * (1.0f / 6.0f));
+    const float sB = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(SB6) * (1.0f / 6.0f));
+    const float isA = __builtin_amdgcn_rcpf(sA), isB = __builtin_amdgcn_rcpf(sB);
+    const float exA = __builtin_amdgcn_exp2f(mlog2e_b2 * (sA * sA));
+    const float exB = __builtin_amdgcn_exp2f(mlog2e_b2 * (sB * sB));
+    const float tA = fmaf(exA * erfcx_poly(beta * sA), isA, kTwoOverSqrtPi * beta * exA);
+    const float tB = fmaf(exB * erfcx_poly(beta * sB), isB, kTwoOverSqrtPi * beta * exB);
+    fc = qq * r4 * (s1.z * tA * iA6 + s1.w * tB * iB6);
+}
+
+// FP64 energy-step terms of one soft-core pair from the exact displacement:
+// E_LJ, E_C and the LJ virial -f_LJ r^2 (LJ_soft_core.cu:434-473, 598-641).
+__device__ __forceinline__ void soft_pair_energy(double r2, double qq, double beta, float4 s0, float4 s1,
+                                                 double& elj, double& edir, double& vlj) {
+    const double r4 = r2 * r2, r6 = r4 * r2;
+    const double SA6 = r6 + (double)s1.x, SB6 = r6 + (double)s1.y;
+    const double iA6 = 1.0 / SA6, iB6 = 1.0 / SB6;
+    const double sA = cbrt(sqrt(SA6)), sB = cbrt(sqrt(SB6));
+    elj = ((double)s0.x * (1.0 / 12.0) * iA6 - (double)s0.y * (1.0 / 6.0)) * iA6 +
+          ((double)s0.z * (1.0 / 12.0) * iB6 - (double)s0.w * (1.0 / 6.0)) * iB6;
+    edir = qq * ((double)s1.z * erfc(beta * sA) / sA + (double)s1.w * erfc(beta * sB) / sB);
+    const double flj = r4 * (((double)s0.y - (double)s0.x * iA6) * iA6 * iA6 +
+                             ((double)s0.w - (double)s0.z * iB6) * iB6 * iB6);
+    vlj = -flj * r2;
+}
+
 // Cutoff test shared with the oracle: FP32 dr2 = (dx*dx + dy*dy) + dz*dz with
 // every operation rounded (the neighbor list's expression) against rc*rc in
 // FP32, so both sides agree on every pair, including those within one ulp of
 // the cutoff.
-template <bool ENERGY>
-__global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
-    const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list, int list_cap,
-    const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz, float rc2,
-    float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
-    float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite) {
+// SOFT: the type index is the combined (type A, type B, mask) index of a
+// soft-core handle; `ab` holds the lambda-mixed (A, B) of ordinary pair
+// types and A < 0 for soft-core pair types, whose records are sc[t_i nt + t_j]
+// (s0) and sc[nt^2 + t_i nt + t_j] (s1), read from global memory only by
+// waves that meet such a pair.
+template <bool ENERGY, bool SOFT>
+__device__ __forceinline__ void pair_body(
+    int lb, const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list,
+    int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
+    float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
+    float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
+    const float4* __restrict__ sc) {
     extern __shared__ float2 sab[];
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int lb = xcd_block(gridDim.x);
     const int ic = lb * kPairWaves + w;
     if (lb < 0 || ic >= flags[4]) return;
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
@@ -131,7 +195,8 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
     const uint4 a = pk[islot];
     const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
     const float qi = __uint_as_float(a.w);
-    const float2* srow = sab + unpack_type(a) * nt;
+    const int ti = unpack_type(a);
+    const float2* srow = sab + ti * nt;
     const float mlog2e_b2 = -1.4426950408889634f * (beta * beta);
     int ai = 0;
     if (ENERGY) ai = s2o[islot];
@@ -162,13 +227,24 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
             const float rinv = __builtin_amdgcn_rsqf(r2s);
             const float rinv2 = rinv * rinv;
             const float rinv6 = rinv2 * rinv2 * rinv2;
-            const float2 AB = srow[unpack_type(b)];
-            const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
+            const int tj = unpack_type(b);
+            const float2 AB = srow[tj];
+            float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
             const float x = beta * (r2s * rinv);
             const float ex = __builtin_amdgcn_exp2f(mlog2e_b2 * r2s);
             const float ecx = erfcx_poly(x);
             const float qq = qi * __uint_as_float(b.w);
-            const float fc = qq * ex * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+            float fc = qq * ex * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+            bool sp = false;
+            float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+            if (SOFT) {
+                sp = in && AB.x < 0.f;
+                if (sp) {
+                    s0 = sc[ti * nt + tj];
+                    s1 = sc[nt * nt + ti * nt + tj];
+                    soft_pair(r2s, qq, beta, mlog2e_b2, s0, s1, flj, fc);
+                }
+            }
             const float f = in ? flj - fc : 0.f;
             fix += f * dx;
             fiy += f * dy;
@@ -185,11 +261,17 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
                     const double ddy = (double)(int)((b.y & ~3u) - ay) * (double)cy;
                     const double ddz = (double)(int)((b.z & ~3u) - az) * (double)cz;
                     const double dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    const double dr = sqrt(dr2);
-                    const double ir2 = 1.0 / dr2, ir6 = ir2 * ir2 * ir2;
-                    const double elj = ((double)AB.x * (1.0 / 12.0) * ir6 - (double)AB.y * (1.0 / 6.0)) * ir6;
-                    const double edir = (double)qi * (double)__uint_as_float(b.w) * erfc((double)beta * dr) / dr;
-                    const double vlj = ((double)AB.x * ir6 - (double)AB.y) * ir6;
+                    double elj, edir, vlj;
+                    if (SOFT && sp) {
+                        soft_pair_energy(dr2, (double)qi * (double)__uint_as_float(b.w), (double)beta, s0, s1, elj,
+                                         edir, vlj);
+                    } else {
+                        const double dr = sqrt(dr2);
+                        const double ir2 = 1.0 / dr2, ir6 = ir2 * ir2 * ir2;
+                        elj = ((double)AB.x * (1.0 / 12.0) * ir6 - (double)AB.y * (1.0 / 6.0)) * ir6;
+                        edir = (double)qi * (double)__uint_as_float(b.w) * erfc((double)beta * dr) / dr;
+                        vlj = ((double)AB.x * ir6 - (double)AB.y) * ir6;
+                    }
                     tlj += elj;
                     tdir += edir;
                     tvir += vlj;
@@ -252,6 +334,35 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
     }
 }
 
+template <bool ENERGY, bool SOFT>
+__global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
+    const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list, int list_cap,
+    const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz, float rc2,
+    float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
+    float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
+    const float4* __restrict__ sc) {
+    pair_body<ENERGY, SOFT>(xcd_block(gridDim.x), flags, list_n, list, list_cap, pk, ab, nt, cx, cy, cz, rc2, beta,
+                            s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc);
+}
+
+// Batched: logical blocks are spread over the XCDs as in the single-system
+// launch, then mapped to (system, block of 4 i-clusters).
+template <bool ENERGY, bool SOFT>
+__global__ __launch_bounds__(kPairWaves * 64) void pair_batch(const BatchSys* __restrict__ bs,
+                                                              const int2* __restrict__ map, int nmap,
+                                                              int overwrite) {
+    const int g = xcd_block(nmap);
+    if (g < 0) return;
+    const int2 m = map[g];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const float rc = b.lj.cutoff;
+    pair_body<ENERGY, SOFT>(m.y, b.nl.flags, b.nl.list_n, b.nl.list, b.nl.list_cap, b.lj.ps.pk, b.lj.d_ab,
+                            b.lj.ntypes, b.lj.cof_dr[0], b.lj.cof_dr[1], b.lj.cof_dr[2], rc * rc, b.pme.beta,
+                            b.nl.s2o, b.io.frc, ENERGY ? b.io.atom_energy : nullptr,
+                            ENERGY ? b.io.atom_virial : nullptr, nullptr, b.lj.ps.acc, overwrite, b.lj.d_sc);
+}
+
 __global__ void add_scalar_kernel(float* a, float va, float* b, float vb) {
     if (a) a[0] += va;
     if (b) b[0] += vb;
@@ -260,11 +371,12 @@ __global__ void add_scalar_kernel(float* a, float va, float* b, float vb) {
 // PME_Excluded_Force_With_Atom_Energy_Correction (PME_force.cu:620-695):
 // one thread per atom over its exclusion CSR row, FP32; energy
 // -q_i q_j erf(b r)/r credited to atom i; FP64 total in acc.
-__global__ void excluded_kernel(int n, const unsigned int* __restrict__ uc, float sx, float sy, float sz,
-                                const float* __restrict__ q, float beta, const int* __restrict__ es,
-                                const int* __restrict__ el, const int* __restrict__ en, float* __restrict__ frc,
-                                float* __restrict__ atom_e, float* __restrict__ corr) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void excluded_body(int blk, int n, const unsigned int* __restrict__ uc, float sx, float sy,
+                                              float sz, const float* __restrict__ q, float beta,
+                                              const int* __restrict__ es, const int* __restrict__ el,
+                                              const int* __restrict__ en, float* __restrict__ frc,
+                                              float* __restrict__ atom_e, float* __restrict__ corr) {
+    int i = blk * blockDim.x + threadIdx.x;
     if (i < n) {
         int ne = en[i];
         float e = 0.f;
@@ -304,6 +416,22 @@ __global__ void excluded_kernel(int n, const unsigned int* __restrict__ uc, floa
         }
         if (corr) corr[i] = e;
     }
+}
+
+__global__ void excluded_kernel(int n, const unsigned int* __restrict__ uc, float sx, float sy, float sz,
+                                const float* __restrict__ q, float beta, const int* __restrict__ es,
+                                const int* __restrict__ el, const int* __restrict__ en, float* __restrict__ frc,
+                                float* __restrict__ atom_e, float* __restrict__ corr) {
+    excluded_body(blockIdx.x, n, uc, sx, sy, sz, q, beta, es, el, en, frc, atom_e, corr);
+}
+
+__global__ __launch_bounds__(256) void excluded_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    excluded_body(m.y, b.pme.n, b.io.uint_crd, b.lj.cof_dr[0], b.lj.cof_dr[1], b.lj.cof_dr[2], b.io.charge,
+                  b.pme.beta, b.io.excluded_list_start, b.io.excluded_list, b.io.excluded_numbers, b.io.frc, nullptr,
+                  b.pme.corr_atom);
 }
 
 }  // namespace
@@ -357,9 +485,105 @@ extern "C" int sponge_lj_create(int atom_numbers, int atom_type_numbers, const f
     return SPONGE_NB_OK;
 }
 
+// LJ_SOFT_CORE::Initial (LJ_soft_core.cu:1420-1610) after its file reads.
+// Atoms are grouped into combined types c = (type A, type B, mask); every
+// (c_i, c_j) pair type is decided once on the host: ordinary pair types get
+// the lambda-mixed (A, B) of the plain kernel, soft-core ones (mask differs,
+// or exactly one end state has A below 1e-6, LJ_soft_core.cu:251) get the
+// records soft_pair reads. The long-range factor sums (1 - lambda) AB + lambda BB
+// over all ordered atom pairs in FP64 (Total_C6_Get, :88-118, :1588-1600).
+extern "C" int sponge_lj_soft_create(int atom_numbers, int atom_type_numbers_A, int atom_type_numbers_B,
+                                     const float* h_LJ_AA, const float* h_LJ_AB, const float* h_LJ_BA,
+                                     const float* h_LJ_BB, const int* h_atom_LJ_type_A,
+                                     const int* h_atom_LJ_type_B, const int* h_subsys_division, float lambda,
+                                     float alpha, float p, float sigma, float sigma_min, float cutoff,
+                                     const float box_length[3], sponge_lj** out) {
+    SNB_ARG(out && h_LJ_AA && h_LJ_AB && h_LJ_BA && h_LJ_BB && h_atom_LJ_type_A && h_atom_LJ_type_B && box_length,
+            "null pointer");
+    SNB_ARG(atom_numbers > 0, "atom_numbers");
+    SNB_ARG(atom_type_numbers_A > 0 && atom_type_numbers_B > 0, "atom_type_numbers");
+    SNB_ARG(lambda >= 0.f && lambda <= 1.f, "lambda_lj must be in [0, 1]");
+    *out = nullptr;
+    const int TA = atom_type_numbers_A, TB = atom_type_numbers_B;
+    std::map<std::tuple<int, int, int>, int> combo;
+    std::vector<int> ctype(atom_numbers);
+    std::vector<std::tuple<int, int, int>> cdesc;
+    for (int i = 0; i < atom_numbers; i++) {
+        const int a = h_atom_LJ_type_A[i], b = h_atom_LJ_type_B[i], m = h_subsys_division ? h_subsys_division[i] : 0;
+        SNB_ARG(a >= 0 && a < TA && b >= 0 && b < TB, "atom LJ type out of range");
+        auto key = std::make_tuple(a, b, m);
+        auto it = combo.find(key);
+        if (it == combo.end()) {
+            SNB_ARG((int)cdesc.size() < kMaxLjTypes, "more than 64 distinct (type A, type B, mask) combinations");
+            it = combo.emplace(key, (int)cdesc.size()).first;
+            cdesc.push_back(key);
+        }
+        ctype[i] = it->second;
+    }
+    const int C = (int)cdesc.size();
+    auto pidx = [](int x, int y) {
+        const int hi = x > y ? x : y, lo = x > y ? y : x;
+        return hi * (hi + 1) / 2 + lo;
+    };
+    const double lam = lambda, l_ = 1.0 - lam;
+    const double alp = alpha * std::pow(lam, (double)p), alp_ = alpha * std::pow(l_, (double)p);
+    const double sig6 = std::pow((double)sigma, 6.0), sig6min = std::pow((double)sigma_min, 6.0);
+    std::vector<float2> sq((size_t)C * C);
+    std::vector<float4> sc(2 * (size_t)C * C, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<double> cnt(C, 0.0);
+    for (int i = 0; i < atom_numbers; i++) cnt[ctype[i]] += 1;
+    double lrc = 0;
+    for (int x = 0; x < C; x++)
+        for (int y = 0; y < C; y++) {
+            const int ax = std::get<0>(cdesc[x]), bx = std::get<1>(cdesc[x]), mx = std::get<2>(cdesc[x]);
+            const int ay = std::get<0>(cdesc[y]), by = std::get<1>(cdesc[y]), my = std::get<2>(cdesc[y]);
+            const int pa = pidx(ax, ay), pb = pidx(bx, by);
+            const double aa = h_LJ_AA[pa], ab = h_LJ_AB[pa], ba = h_LJ_BA[pb], bb = h_LJ_BB[pb];
+            lrc += cnt[x] * cnt[y] * (l_ * ab + lam * bb);
+            const bool soft = mx != my || (ba > 1e-6 && aa < 1e-6) || (ba < 1e-6 && aa > 1e-6);
+            const size_t k = (size_t)x * C + y;
+            if (!soft) {
+                sq[k] = make_float2((float)(l_ * aa + lam * ba), (float)(l_ * ab + lam * bb));
+            } else {
+                const double sa6 = (aa < 1e-6 || ab < 1e-6) ? sig6 : std::max(0.5 * aa / ab, sig6min);
+                const double sb6 = (ba < 1e-6 || bb < 1e-6) ? sig6 : std::max(0.5 * ba / bb, sig6min);
+                sq[k] = make_float2(-1.0f, 0.0f);
+                sc[k] = make_float4((float)(l_ * aa), (float)(l_ * ab), (float)(lam * ba), (float)(lam * bb));
+                sc[(size_t)C * C + k] = make_float4((float)(alp * sa6), (float)(alp_ * sb6), (float)l_, (float)lam);
+            }
+        }
+    sponge_lj* lj = new sponge_lj();
+    memset(lj, 0, sizeof(*lj));
+    lj->n = atom_numbers;
+    lj->ntypes = C;
+    lj->cutoff = cutoff;
+    const int P = TA * (TA + 1) / 2;
+    lj->h_A = new float[P];
+    lj->h_B = new float[P];
+    lj->h_type = new int[atom_numbers];
+    memcpy(lj->h_A, h_LJ_AA, P * sizeof(float));
+    memcpy(lj->h_B, h_LJ_AB, P * sizeof(float));
+    memcpy(lj->h_type, h_atom_LJ_type_A, atom_numbers * sizeof(int));
+    sponge_lj_update_volume(lj, box_length);
+    lj->lrc_factor = lrc * (-2.0 / 3.0) * M_PI / ((double)cutoff * cutoff * cutoff) / 6.0;
+    hipError_t e = hipMalloc((void**)&lj->d_ab, sq.size() * sizeof(float2));
+    if (e == hipSuccess) e = hipMemcpy(lj->d_ab, sq.data(), sq.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&lj->d_sc, sc.size() * sizeof(float4));
+    if (e == hipSuccess) e = hipMemcpy(lj->d_sc, sc.data(), sc.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&lj->d_type, atom_numbers * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(lj->d_type, ctype.data(), atom_numbers * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error("sponge_lj_soft_create: %s", hipGetErrorString(e));
+        sponge_lj_destroy(lj);
+        return SPONGE_NB_EHIP;
+    }
+    *out = lj;
+    return SPONGE_NB_OK;
+}
+
 extern "C" void sponge_lj_destroy(sponge_lj* lj) {
     if (!lj) return;
-    void* ptrs[] = {lj->d_ab, lj->d_type};
+    void* ptrs[] = {lj->d_ab, lj->d_sc, lj->d_type};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     snb_scratch_free(&lj->ps);
@@ -416,7 +640,7 @@ void snb_scratch_free(PairScratch* ps) {
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered,
-              int overwrite) {
+              int overwrite, const float4* sc) {
     int rc = scratch_ensure(ps, nl);
     if (rc) return rc;
     const int cap = ps->cap;
@@ -429,15 +653,19 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     const float rc2 = cutoff * cutoff;
     const size_t lds = (size_t)nt * nt * sizeof(float2);
     if (ps->ev_start) SNB_HIP(hipEventRecord(ps->ev_start, s));
+#define SNB_PAIR(E, S)                                                                                          \
+    pair_kernel<E, S><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, nl->list_n, nl->list, nl->list_cap, ps->pk, ab, \
+                                                         nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,         \
+                                                         E ? atom_e : nullptr, E ? atom_v : nullptr,                    \
+                                                         E ? atom_d : nullptr, ps->acc, overwrite, sc)
     if (energy) {
-        pair_kernel<true><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, nl->list_n, nl->list, nl->list_cap, ps->pk,
-                                                             ab, nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,
-                                                             atom_e, atom_v, atom_d, ps->acc, overwrite);
+        if (sc) SNB_PAIR(true, true);
+        else SNB_PAIR(true, false);
     } else {
-        pair_kernel<false><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, nl->list_n, nl->list, nl->list_cap, ps->pk,
-                                                              ab, nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,
-                                                              nullptr, nullptr, nullptr, ps->acc, overwrite);
+        if (sc) SNB_PAIR(false, true);
+        else SNB_PAIR(false, false);
     }
+#undef SNB_PAIR
     SNB_CHECK_LAUNCH();
     if (ps->ev_stop) SNB_HIP(hipEventRecord(ps->ev_stop, s));
     return SPONGE_NB_OK;
@@ -475,7 +703,8 @@ extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, co
     const int energy = (need_atom_energy > 0 || need_virial > 0) ? 1 : 0;
     return snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme_beta,
                      energy, &lj->ps, d_frc, need_atom_energy > 0 ? d_atom_energy : nullptr,
-                     need_virial > 0 ? d_atom_lj_virial : nullptr, energy ? d_atom_direct_pme_energy : nullptr, s);
+                     need_virial > 0 ? d_atom_lj_virial : nullptr, energy ? d_atom_direct_pme_energy : nullptr, s, 0,
+                     0, lj->d_sc);
 }
 
 extern "C" int sponge_lj_long_range_correction(sponge_lj* lj, int need_pressure, float* d_virial, int need_potential,
@@ -502,7 +731,7 @@ extern "C" int sponge_lj_get_energy(sponge_lj* lj, const sponge_nl* nl, const un
     SNB_HIP(hipMalloc((void**)&zq, lj->n * sizeof(float)));
     SNB_HIP(hipMemset(zq, 0, lj->n * sizeof(float)));
     int rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, zq, 0.0f, 1,
-                       &lj->ps, nullptr, nullptr, nullptr, nullptr, s);
+                       &lj->ps, nullptr, nullptr, nullptr, nullptr, s, 0, 0, lj->d_sc);
     double acc[SPONGE_NB_NPARTS];
     if (rc == 0) {
         SNB_HIP(hipMemcpyAsync(acc, lj->ps.acc, sizeof(acc), hipMemcpyDeviceToHost, s));
@@ -534,3 +763,36 @@ extern "C" int sponge_pme_excluded_force(sponge_pme* pme, const unsigned int* d_
     return snb_excluded(pme, d_uint_crd, scaler, d_charge, d_excluded_list_start, d_excluded_list,
                         d_excluded_atom_numbers, d_frc, d_atom_energy, s);
 }
+
+// ---------------------------------------------------------------------------
+// Batched launches (one per stage over every system of a batch).
+int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s) {
+    step_prep_batch<<<b->nblk[kStPrep], 256, 0, s>>>(b->d, b->map[kStPrep], energy, from_crd);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s) {
+    const int nmap = b->nblk[kStPair];
+    const int grid = xcd_grid(nmap);
+#define SNB_PAIRB(E, S) \
+    pair_batch<E, S><<<grid, kPairWaves * 64, b->lds_pair, s>>>(b->d, b->map[kStPair], nmap, overwrite)
+    if (energy) {
+        if (b->soft) SNB_PAIRB(true, true);
+        else SNB_PAIRB(true, false);
+    } else {
+        if (b->soft) SNB_PAIRB(false, true);
+        else SNB_PAIRB(false, false);
+    }
+#undef SNB_PAIRB
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+int snb_batch_excluded(sponge_batch* b, hipStream_t s) {
+    excluded_batch<<<b->nblk[kStExcl], 256, 0, s>>>(b->d, b->map[kStExcl]);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+int snb_lj_scratch(sponge_lj* lj, const sponge_nl* nl) { return scratch_ensure(&lj->ps, nl); }

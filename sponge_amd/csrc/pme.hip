@@ -33,12 +33,6 @@ using namespace snb;
 
 namespace {
 
-struct FftPlan {
-    int K;
-    int npass;
-    int radix[12];
-};
-
 constexpr int kPlaneThreads = 512;
 constexpr int kSpreadPerThread = 4;
 // Fixed-point unit of the spread grid: one contribution q*Mx*My*Mz must stay
@@ -253,12 +247,12 @@ __device__ __forceinline__ float bsp_dw(float f, int k) {
 // offset). The real plane is held directly in the packed complex layout of the
 // z-FFT: row y is the real (y even) or imaginary (y odd) part of line y/2.
 template <int MAXB>
-__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
-    const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
-    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, FftPlan pz,
-    FftPlan py, const float2* __restrict__ twz, const float2* __restrict__ twy, double* __restrict__ acc) {
+__device__ __forceinline__ void fwd_plane_body(
+    int x, const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
+    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz,
+    const FftPlan& pz, const FftPlan& py, const float2* __restrict__ twz, const float2* __restrict__ twy,
+    double* __restrict__ acc) {
     extern __shared__ float2 sm[];
-    const int x = blockIdx.x;
     const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
     const int plane = max(Ky * H, nrp * Kz);
     float2* tz = sm + max(plane, Ky * Kz);  // past the fixed-point grid too
@@ -392,18 +386,40 @@ __global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
 }
 
 template <int MAXB>
-__global__ __launch_bounds__(kConvThreads) void xconv_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
-                                                             int Kx, int Ky, int Kz, FftPlan px,
-                                                             const float2* __restrict__ twx, int energy,
-                                                             double* __restrict__ acc, int* __restrict__ bin_count) {
+__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
+    const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
+    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, FftPlan pz,
+    FftPlan py, const float2* __restrict__ twz, const float2* __restrict__ twy, double* __restrict__ acc) {
+    fwd_plane_body<MAXB>(blockIdx.x, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, py, twz, twy,
+                         acc);
+}
+
+// Batched: block -> (system, x plane) from the block map.
+template <int MAXB>
+__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_batch(const BatchSys* __restrict__ bs,
+                                                                 const int2* __restrict__ map, int energy) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const sponge_pme& P = b.pme;
+    fwd_plane_body<MAXB>(m.y, P.base, P.frac, P.bin_count, P.bin_atoms, P.bin_cap, P.FQ, P.K[0], P.K[1], P.K[2],
+                         b.pz, b.py, P.d_tw[2], P.d_tw[1], energy ? P.acc : nullptr);
+}
+
+// One block per (y, chunk of kConvChunk kz): FFT along x, energy, BC
+// multiply, inverse FFT along x.
+__device__ __forceinline__ void xconv_body(int y, int chunk, float2* __restrict__ FQ, const float* __restrict__ bc,
+                                           int Kx, int Ky, int Kz, const FftPlan& px,
+                                           const float2* __restrict__ twx, int energy, double* __restrict__ acc,
+                                           int* __restrict__ bin_count) {
     __shared__ float2 sm[256 * kConvChunk];
     __shared__ float2 tx[256];
     __shared__ double red[kConvThreads / 64];
     const int H = Kz / 2 + 1;
-    const int y = blockIdx.x, kz0 = blockIdx.y * kConvChunk;
+    const int kz0 = chunk * kConvChunk;
     const int cw = min(kConvChunk, H - kz0);
     // the forward plane kernels have consumed the x-plane bins: clear them
-    if (blockIdx.x == 0 && blockIdx.y == 0)
+    if (y == 0 && chunk == 0)
         for (int b = threadIdx.x; b < Kx; b += kConvThreads) bin_count[b] = 0;
     load_twiddles<kConvThreads>(tx, twx, Kx);
     for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
@@ -438,13 +454,31 @@ __global__ __launch_bounds__(kConvThreads) void xconv_kernel(float2* __restrict_
     }
 }
 
+__global__ __launch_bounds__(kConvThreads) void xconv_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
+                                                             int Kx, int Ky, int Kz, FftPlan px,
+                                                             const float2* __restrict__ twx, int energy,
+                                                             double* __restrict__ acc, int* __restrict__ bin_count) {
+    xconv_body(blockIdx.x, blockIdx.y, FQ, bc, Kx, Ky, Kz, px, twx, energy, acc, bin_count);
+}
+
+// Batched: the local block l is (y = l mod Ky, chunk = l / Ky).
+__global__ __launch_bounds__(kConvThreads) void xconv_batch(const BatchSys* __restrict__ bs,
+                                                            const int2* __restrict__ map, int energy) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const sponge_pme& P = b.pme;
+    const int Ky = P.K[1];
+    const int chunk = m.y / Ky;
+    xconv_body(m.y - chunk * Ky, chunk, P.FQ, P.d_bc, P.K[0], Ky, P.K[2], b.px, P.d_tw[0], energy, P.acc,
+               P.bin_count);
+}
+
 template <int MAXB>
-__global__ __launch_bounds__(kPlaneThreads) void inv_plane_kernel(const float2* __restrict__ FQ,
-                                                                  float* __restrict__ R, int Ky, int Kz, FftPlan pz,
-                                                                  FftPlan py, const float2* __restrict__ twz,
-                                                                  const float2* __restrict__ twy) {
+__device__ __forceinline__ void inv_plane_body(int x, const float2* __restrict__ FQ, float* __restrict__ R, int Ky,
+                                               int Kz, const FftPlan& pz, const FftPlan& py,
+                                               const float2* __restrict__ twz, const float2* __restrict__ twy) {
     extern __shared__ float2 sm[];
-    const int x = blockIdx.x;
     const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
     const int plane = max(Ky * H, nrp * Kz);
     float2* tz = sm + plane;
@@ -499,14 +533,31 @@ __global__ __launch_bounds__(kPlaneThreads) void inv_plane_kernel(const float2* 
     }
 }
 
+template <int MAXB>
+__global__ __launch_bounds__(kPlaneThreads) void inv_plane_kernel(const float2* __restrict__ FQ,
+                                                                  float* __restrict__ R, int Ky, int Kz, FftPlan pz,
+                                                                  FftPlan py, const float2* __restrict__ twz,
+                                                                  const float2* __restrict__ twy) {
+    inv_plane_body<MAXB>(blockIdx.x, FQ, R, Ky, Kz, pz, py, twz, twy);
+}
+
+template <int MAXB>
+__global__ __launch_bounds__(kPlaneThreads) void inv_plane_batch(const BatchSys* __restrict__ bs,
+                                                                 const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const sponge_pme& P = b.pme;
+    inv_plane_body<MAXB>(m.y, P.FQ, P.R, P.K[1], P.K[2], b.pz, b.py, P.d_tw[2], P.d_tw[1]);
+}
+
 // PME_Final (PME_force.cu:485-550): 16 lanes per atom (lane = kx, ky), each
 // walking the 4 kz points; force += -q sum R dtheta (K/L).
-__global__ __launch_bounds__(kGatherThreads) void pme_gather_kernel(int n, const unsigned* __restrict__ base,
-                                                                    const float4* __restrict__ frac,
-                                                                    const float* __restrict__ R, int Kx, int Ky,
-                                                                    int Kz, float ibx, float iby, float ibz,
-                                                                    float* __restrict__ frc) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pme_gather_body(int blk, int n, const unsigned* __restrict__ base,
+                                                const float4* __restrict__ frac, const float* __restrict__ R, int Kx,
+                                                int Ky, int Kz, float ibx, float iby, float ibz,
+                                                float* __restrict__ frc) {
+    const int gid = blk * kGatherThreads + threadIdx.x;
     const int i = gid >> 4, sub = gid & 15;
     const int kx = sub >> 2, ky = sub & 3;
     float sx = 0.f, sy = 0.f, sz = 0.f, qi = 0.f;
@@ -545,6 +596,24 @@ __global__ __launch_bounds__(kGatherThreads) void pme_gather_kernel(int n, const
     }
 }
 
+__global__ __launch_bounds__(kGatherThreads) void pme_gather_kernel(int n, const unsigned* __restrict__ base,
+                                                                    const float4* __restrict__ frac,
+                                                                    const float* __restrict__ R, int Kx, int Ky,
+                                                                    int Kz, float ibx, float iby, float ibz,
+                                                                    float* __restrict__ frc) {
+    pme_gather_body(blockIdx.x, n, base, frac, R, Kx, Ky, Kz, ibx, iby, ibz, frc);
+}
+
+__global__ __launch_bounds__(kGatherThreads) void pme_gather_batch(const BatchSys* __restrict__ bs,
+                                                                   const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const sponge_pme& P = b.pme;
+    pme_gather_body(m.y, P.n, P.base, P.frac, P.R, P.K[0], P.K[1], P.K[2], (float)P.K[0] / P.box[0],
+                    (float)P.K[1] / P.box[1], (float)P.K[2] / P.box[2], b.io.frc);
+}
+
 // Energy / virial bookkeeping of PME_Reciprocal_Force_With_Energy_And_Virial
 // (PME_force.cu:753-771) with FP64 sums: potential += corr + self + rec,
 // virial += direct + corr + self + rec.
@@ -571,6 +640,41 @@ __global__ void pme_finalize_kernel(int n, const float* __restrict__ dir_atom, c
         acc[kAccQ1] = 0.0;
         if (need_energy && d_potential) d_potential[0] += (float)(tsc + eself + erec);
         if (need_virial && d_virial) d_virial[0] += (float)(tsd + tsc + eself + erec);
+    }
+}
+
+// Batched energy-step bookkeeping: per system the finalize sums above, then
+// the parts array of the fused step.
+__global__ __launch_bounds__(1024) void pme_finalize_batch(const BatchSys* __restrict__ bs,
+                                                           const int2* __restrict__ map) {
+    __shared__ double red[16];
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const sponge_pme& P = b.pme;
+    double sc = 0.0;
+    for (int i = threadIdx.x; i < P.n; i += blockDim.x) sc += P.corr_atom[i];
+    const double tsc = block_sum<1024>(sc, red);
+    if (threadIdx.x == 0) {
+        double* acc = P.acc;
+        const double* la = b.lj.ps.acc;
+        const double q2 = acc[kAccQ2], q1 = acc[kAccQ1];
+        const double eself = -(double)P.beta / sqrt(M_PI) * q2 + P.neutralizing_factor * q1 * q1;
+        acc[SPONGE_PART_SELF] = eself;
+        acc[SPONGE_PART_CORRECTION] = tsc;
+        acc[kAccQ2] = 0.0;
+        acc[kAccQ1] = 0.0;
+        double* parts = b.io.parts;
+        if (parts) {
+            parts[SPONGE_PART_LJ] = la[SPONGE_PART_LJ];
+            parts[SPONGE_PART_DIRECT] = la[SPONGE_PART_DIRECT];
+            parts[SPONGE_PART_LJ_VIRIAL] = la[SPONGE_PART_LJ_VIRIAL];
+            parts[SPONGE_PART_CORRECTION] = tsc;
+            parts[SPONGE_PART_SELF] = eself;
+            parts[SPONGE_PART_RECIPROCAL] = acc[SPONGE_PART_RECIPROCAL];
+            parts[SPONGE_PART_LRC] = b.lrc;
+            parts[SPONGE_PART_PAIRS_IN] = la[SPONGE_PART_PAIRS_IN];
+        }
     }
 }
 
@@ -839,13 +943,8 @@ int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const fl
 #undef SNB_FWD
     SNB_CHECK_LAUNCH();
     dim3 cg(Ky, ceil_div(H, kConvChunk));
-    int mbc = pick_maxb(((long)Kx * kConvChunk / 2 + kConvThreads - 1) / kConvThreads);
-    switch (mbc) {
-        case 1: xconv_kernel<1><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count); break;
-        case 2: xconv_kernel<2><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count); break;
-        case 4: xconv_kernel<4><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count); break;
-        default: xconv_kernel<8><<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count); break;
-    }
+    xconv_kernel<<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc,
+                                             pme->bin_count);
     SNB_CHECK_LAUNCH();
     switch (mb) {
         case 1: inv_plane_kernel<1><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
@@ -893,5 +992,80 @@ extern "C" int sponge_pme_reciprocal_force(sponge_pme* pme, const unsigned int* 
 int snb_pme_parts(sponge_pme* pme, double* parts, hipStream_t s) {
     SNB_HIP(hipMemcpyAsync(parts, pme->acc, SPONGE_NB_NPARTS * sizeof(double), hipMemcpyDeviceToHost, s));
     SNB_HIP(hipStreamSynchronize(s));
+    return SPONGE_NB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched reciprocal pipeline: one launch per stage over every system of a
+// batch (block maps built by sponge_batch_create).
+void snb_pme_plans(const sponge_pme* pme, FftPlan* px, FftPlan* py, FftPlan* pz) {
+    plan_axis(pme->K[0], px);
+    plan_axis(pme->K[1], py);
+    plan_axis(pme->K[2], pz);
+}
+
+int snb_pme_plane_maxb(const sponge_pme* pme) {
+    const int Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
+    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
+    return pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
+}
+
+size_t snb_pme_lds_fwd(const sponge_pme* pme) { return fwd_plane_lds(pme->K[1], pme->K[2]); }
+
+size_t snb_pme_lds_inv(const sponge_pme* pme) {
+    const int Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
+    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
+    return (size_t)(plane + Ky + Kz) * sizeof(float2);
+}
+
+int snb_batch_reciprocal(sponge_batch* b, int energy, hipStream_t s) {
+#define SNB_FWDB(MB)                                                                                         \
+    fwd_plane_batch<MB><<<b->nblk[kStFwd], kPlaneThreads, b->lds_fwd, s>>>(b->d, b->map[kStFwd], energy)
+#define SNB_INVB(MB) inv_plane_batch<MB><<<b->nblk[kStInv], kPlaneThreads, b->lds_inv, s>>>(b->d, b->map[kStInv])
+    switch (b->mb_plane) {
+        case 1: SNB_FWDB(1); break;
+        case 2: SNB_FWDB(2); break;
+        case 4: SNB_FWDB(4); break;
+        case 8: SNB_FWDB(8); break;
+        default: SNB_FWDB(16); break;
+    }
+    SNB_CHECK_LAUNCH();
+    xconv_batch<<<b->nblk[kStConv], kConvThreads, 0, s>>>(b->d, b->map[kStConv], energy);
+    SNB_CHECK_LAUNCH();
+    switch (b->mb_plane) {
+        case 1: SNB_INVB(1); break;
+        case 2: SNB_INVB(2); break;
+        case 4: SNB_INVB(4); break;
+        case 8: SNB_INVB(8); break;
+        default: SNB_INVB(16); break;
+    }
+    SNB_CHECK_LAUNCH();
+#undef SNB_FWDB
+#undef SNB_INVB
+    return SPONGE_NB_OK;
+}
+
+int snb_batch_gather(sponge_batch* b, hipStream_t s) {
+    pme_gather_batch<<<b->nblk[kStGather], kGatherThreads, 0, s>>>(b->d, b->map[kStGather]);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+int snb_batch_finalize(sponge_batch* b, hipStream_t s) {
+    pme_finalize_batch<<<b->nblk[kStFin], 1024, 0, s>>>(b->d, b->map[kStFin]);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+// Raises the dynamic-LDS limit of the batched forward plane kernel when the
+// largest system's plane needs more than 64 KB.
+int snb_batch_pme_attributes(int mb, size_t lds_fwd) {
+    if (lds_fwd <= 64 * 1024) return SPONGE_NB_OK;
+    const void* fn = mb == 1 ? (const void*)fwd_plane_batch<1>
+                     : mb == 2 ? (const void*)fwd_plane_batch<2>
+                     : mb == 4 ? (const void*)fwd_plane_batch<4>
+                     : mb == 8 ? (const void*)fwd_plane_batch<8>
+                               : (const void*)fwd_plane_batch<16>;
+    SNB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fwd));
     return SPONGE_NB_OK;
 }

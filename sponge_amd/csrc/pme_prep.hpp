@@ -20,7 +20,7 @@ struct PmePrep {
     double* acc;
 };
 
-static inline PmePrep pme_prep_args(sponge_pme* pme) {
+__host__ __device__ static inline PmePrep pme_prep_args(const sponge_pme* pme) {
     PmePrep p;
     p.Kx = pme->K[0];
     p.Ky = pme->K[1];

@@ -22,6 +22,7 @@ _lib = None
 NPARTS = 8
 OVERWRITE_FORCE = 1
 SERIAL = 2
+FROM_CRD = 4
 PART_NAMES = ["lj", "direct", "lj_virial", "correction", "self", "reciprocal", "lrc", "pairs_in"]
 
 _P = ctypes.c_void_p
@@ -49,6 +50,8 @@ _SIGS = {
     "sponge_lj_update_volume": (_I, [_P, _FP3]),
     "sponge_lj_host_tables": (_I, [_P, _P, _P, _P]),
     "sponge_lj_set_pair_events": (_I, [_P, _P, _P]),
+    "sponge_lj_soft_create": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _F, _F, _F, _F, _F, _F, _FP3,
+                                   ctypes.POINTER(_P)]),
     "sponge_pme_create": (_I, [_I, _FP3, _F, _F, _I, _I, _I, ctypes.POINTER(_P)]),
     "sponge_pme_destroy": (None, [_P]),
     "sponge_pme_info": (_I, [_P, ctypes.POINTER(_F), ctypes.POINTER(_I)]),
@@ -58,6 +61,11 @@ _SIGS = {
     "sponge_pme_update_box_length": (_I, [_P, _FP3]),
     "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "sponge_sum_list": (_I, [_I, _P, _P, _P]),
+    "sponge_batch_create": (_I, [_I, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    "sponge_batch_destroy": (None, [_P]),
+    "sponge_batch_sync": (_I, [_P]),
+    "sponge_batch_neighbor_list_update": (_I, [_P, _I, _P]),
+    "sponge_batch_force_step": (_I, [_P, _I, _I, _I, _P]),
 }
 
 
@@ -235,6 +243,32 @@ class LENNARD_JONES_INFORMATION:
             pass
 
 
+class LJ_SOFT_CORE(LENNARD_JONES_INFORMATION):
+    """Lennard_Jones_force/LJ_soft_core.cuh:37-160 over sponge_lj_soft_create;
+    the force, long-range, energy and volume methods are those of the plain
+    module (the handle type is shared). Tables are x12 / x6 as stored."""
+
+    def __init__(self, atom_numbers, atom_type_numbers_A, atom_type_numbers_B, h_LJ_AA, h_LJ_AB, h_LJ_BA,
+                 h_LJ_BB, h_atom_LJ_type_A, h_atom_LJ_type_B, h_subsys_division, lambda_lj, cutoff,
+                 box_length, soft_core_alpha=0.5, soft_core_powfer=1.0, soft_core_sigma=3.0,
+                 soft_core_sigma_min=0.0):
+        lib = load_library()
+        keep = [np.ascontiguousarray(a, dt) for a, dt in (
+            (h_LJ_AA, np.float32), (h_LJ_AB, np.float32), (h_LJ_BA, np.float32), (h_LJ_BB, np.float32),
+            (h_atom_LJ_type_A, np.int32), (h_atom_LJ_type_B, np.int32))]
+        mask = None if h_subsys_division is None else np.ascontiguousarray(h_subsys_division, np.int32)
+        self._keep = keep + [mask]
+        h = _P()
+        _check(lib.sponge_lj_soft_create(
+            int(atom_numbers), int(atom_type_numbers_A), int(atom_type_numbers_B),
+            *[k.ctypes.data_as(_P) for k in keep], mask.ctypes.data_as(_P) if mask is not None else None,
+            float(lambda_lj), float(soft_core_alpha), float(soft_core_powfer), float(soft_core_sigma),
+            float(soft_core_sigma_min), float(cutoff), _vec3(box_length), ctypes.byref(h)),
+            "LJ_SOFT_CORE::Initial")
+        self.h = h
+        self.cutoff = float(cutoff)
+
+
 class Particle_Mesh_Ewald:
     """PME_force/PME_force.cuh:25-144 over sponge_pme_*."""
     TOTAL, DIRECT, RECIPROCAL, CORRECTION, SELF = 0, 1, 2, 3, 4
@@ -336,8 +370,14 @@ class NonbondedEngine:
         el = sys_.excl_list if len(sys_.excl_list) else np.zeros(1, np.int32)
         self.excl_list = t(el, torch.int32)
         self.nl = NEIGHBOR_LIST(n, self.box, sys_.cutoff, sys_.skin, refresh_interval=refresh_interval)
-        self.lj = LENNARD_JONES_INFORMATION(n, sys_.ntypes, sys_.lj_A, sys_.lj_B, sys_.lj_type,
-                                            sys_.cutoff, self.box)
+        if getattr(sys_, "lj_type_B", None) is not None:
+            self.lj = LJ_SOFT_CORE(n, sys_.ntypes, sys_.ntypes_B, sys_.lj_A, sys_.lj_B, sys_.lj_A_B,
+                                   sys_.lj_B_B, sys_.lj_type, sys_.lj_type_B, sys_.mask, sys_.lam,
+                                   sys_.cutoff, self.box, sys_.sc_alpha, sys_.sc_p, sys_.sc_sigma,
+                                   sys_.sc_sigma_min)
+        else:
+            self.lj = LENNARD_JONES_INFORMATION(n, sys_.ntypes, sys_.lj_A, sys_.lj_B, sys_.lj_type,
+                                                sys_.cutoff, self.box)
         self.pme = Particle_Mesh_Ewald(n, self.box, sys_.cutoff, sys_.pme_tolerance, *sys_.fft)
         self.scaler = [b / 4294967296.0 for b in self.box]
 
@@ -366,3 +406,58 @@ class NonbondedEngine:
         torch.cuda.synchronize()
         p = self.parts.cpu().numpy()
         return {k: float(v) for k, v in zip(PART_NAMES, p)}
+
+
+class _BatchIo(ctypes.Structure):
+    """sponge_batch_io (include/sponge_nb.h)."""
+    _fields_ = [(name, _P) for name in (
+        "crd", "uint_crd", "charge", "excluded_list_start", "excluded_list", "excluded_numbers",
+        "frc", "atom_energy", "atom_virial", "parts")]
+
+
+class BatchedSystems:
+    """Independent systems (FEP windows, replicas) stepped together: every
+    stage of the nonbonded step is one launch over all of them
+    (sponge_batch_*). Built from NonbondedEngine objects, whose module
+    handles and device arrays it uses in place."""
+
+    def __init__(self, engines):
+        lib = load_library()
+        self.engines = list(engines)
+        k = len(self.engines)
+        nl = (_P * k)(*[e.nl.h for e in self.engines])
+        lj = (_P * k)(*[e.lj.h for e in self.engines])
+        pme = (_P * k)(*[e.pme.h for e in self.engines])
+        io = (_BatchIo * k)()
+        for i, e in enumerate(self.engines):
+            io[i] = _BatchIo(_ptr(e.crd), _ptr(e.uint_crd), _ptr(e.charge), _ptr(e.excl_start),
+                             _ptr(e.excl_list), _ptr(e.excl_numbers), _ptr(e.frc), _ptr(e.atom_energy),
+                             _ptr(e.atom_virial), _ptr(e.parts))
+        self._keep = (nl, lj, pme, io)
+        h = _P()
+        _check(lib.sponge_batch_create(k, ctypes.cast(nl, _P), ctypes.cast(lj, _P), ctypes.cast(pme, _P),
+                                       ctypes.cast(io, _P), ctypes.byref(h)), "sponge_batch_create")
+        self.h = h
+
+    def neighbor_list_update(self, forced=0, stream=None):
+        _check(load_library().sponge_batch_neighbor_list_update(self.h, int(forced), _stream(stream)),
+               "sponge_batch_neighbor_list_update")
+
+    def force_step(self, need_energy=0, need_virial=0, from_crd=True, overwrite=True, stream=None):
+        flags = (OVERWRITE_FORCE if overwrite else 0) | (FROM_CRD if from_crd else 0)
+        _check(load_library().sponge_batch_force_step(self.h, flags, int(need_energy), int(need_virial),
+                                                      _stream(stream)), "sponge_batch_force_step")
+
+    def sync(self):
+        _check(load_library().sponge_batch_sync(self.h), "sponge_batch_sync")
+
+    def Clear(self):
+        if self.h:
+            load_library().sponge_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.Clear()
+        except Exception:
+            pass

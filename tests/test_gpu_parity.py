@@ -59,6 +59,8 @@ CASES = {
     # than 64 KB of LDS and 16 plane points per thread
     "random_biggrid": lambda: system.small_random(600, (14.0, 30.0, 28.0), seed=5, cutoff=5.0,
                                                   skin=1.0, fft=(16, 108, 100)),
+    # soft-core FEP window: 12-atom solute decoupled at lambda 0.35 (LJ_SOFT_CORE)
+    "fep_window": lambda: system.fep_system(700, 0.35, seed=3),
 }
 
 
