@@ -61,6 +61,82 @@ def cpu_baseline(sys_, seconds: float, beta, K):
     }
 
 
+def fep_batch_bench(steps: int, warmup: int, refresh: int):
+    """Batched FEP: 16 soft-core lambda windows of one solvated 12-atom solute
+    plus 8 replicas of 1.4k-12k atoms (cutoff 9 A, skin 2 A, PME ~1 A grids),
+    stepped (a) one system after another with per-system launches and (b) as
+    one batch, one launch per stage, eagerly and (c) replayed from one
+    hipGraph per MD step (one graph for steps with the list refresh, one
+    without). Returns window-steps/s of each."""
+    import torch
+    from sponge_amd import nb, system
+
+    systems = system.fep_batch()
+    engines = [nb.NonbondedEngine(s, refresh_interval=refresh) for s in systems]
+    stream = torch.cuda.current_stream()
+    for e in engines:
+        e.update_neighbor_list(forced=1)
+        e.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+
+    def timed(fn, n):
+        for k in range(warmup):
+            fn(k)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for k in range(n):
+            fn(k)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) * 1e-3
+
+    def seq_step(k):
+        for e in engines:
+            e.force_step(0, 0, from_crd=True, overwrite=True)
+            e.update_neighbor_list()
+
+    t_seq = timed(seq_step, steps)
+    batch = nb.BatchedSystems(engines)
+    batch.neighbor_list_update(forced=1)
+
+    def batch_step(k):
+        batch.force_step(0, 0)
+        batch.neighbor_list_update()
+
+    t_batch = timed(batch_step, steps)
+
+    # one graph per MD step: with and without the list rebuild
+    g_plain, g_rebuild = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+    batch.force_step(0, 0)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g_plain):
+        batch.force_step(0, 0, stream=torch.cuda.current_stream())
+    with torch.cuda.graph(g_rebuild):
+        batch.force_step(0, 0, stream=torch.cuda.current_stream())
+        batch.neighbor_list_update(forced=1, stream=torch.cuda.current_stream())
+
+    def graph_step(k):
+        (g_rebuild if k % refresh == 0 else g_plain).replay()
+
+    t_graph = timed(graph_step, steps)
+    nsys = len(systems)
+    ws = lambda t: round(nsys * steps / t, 1)
+    return {
+        "metric": "window-steps/s, 16 soft-core FEP windows + 8 replicas",
+        "systems": nsys,
+        "atoms": int(sum(s.n for s in systems)),
+        "sizes": [int(s.n) for s in systems],
+        "steps": steps,
+        "sequential": ws(t_seq),
+        "batched": ws(t_batch),
+        "batched_graph": ws(t_graph),
+        "speedup_batched": round(t_seq / t_batch, 3),
+        "speedup_batched_graph": round(t_seq / t_graph, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +146,8 @@ def main():
     ap.add_argument("--refresh", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--fep-steps", type=int, default=200,
+                    help="steps of the batched-FEP comparison (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -166,6 +244,8 @@ def main():
             "flops_per_launch": flops,
         },
     }
+    if args.fep_steps > 0:
+        out["fep_batch"] = fep_batch_bench(args.fep_steps, 20, args.refresh)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sys_, args.cpu_seconds, eng.pme.beta,
                                            (eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))

@@ -79,7 +79,7 @@ def test_batch_matches_oracle(batch_run):
         assert abs(p["self"] - ref["e_self"]) <= 1e-6 * abs(ref["e_self"]), s.name
 
 
-def test_batch_repeated_steps_are_stable(batch_run):
+def test_batch_repeated_steps_are_stable(batch_run):  # noqa: D401
     """Skin-check updates with unmoved atoms rebuild nothing; repeated force
     steps reproduce the first."""
     import torch
@@ -93,3 +93,29 @@ def test_batch_repeated_steps_are_stable(batch_run):
     assert after == before
     for s, e, b in zip(systems, engines, out):
         assert rel_rms(e.frc.cpu().numpy(), b["frc"]) < 1e-6, s.name
+
+
+def test_batch_graph_replays_match_eager(batch_run):
+    """One hipGraph per MD step: the captured batched step (with a forced
+    list rebuild) replays to the eager result, with eager work between
+    replays."""
+    import torch
+    systems, engines, batch, single, out = batch_run
+    batch.neighbor_list_update(forced=1)
+    batch.force_step(0, 0)
+    torch.cuda.synchronize()
+    eager = [e.frc.cpu().numpy().copy() for e in engines]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        cur = torch.cuda.current_stream()
+        batch.force_step(0, 0, stream=cur)
+        batch.neighbor_list_update(forced=1, stream=cur)
+    for r in range(3):
+        for e in engines:
+            e.frc.fill_(float("nan"))
+        g.replay()
+        torch.cuda.synchronize()
+        for s, e, ref in zip(systems, engines, eager):
+            assert rel_rms(e.frc.cpu().numpy(), ref) < 1e-6, (r, s.name)
+        batch.force_step(0, 0)
+        torch.cuda.synchronize()
