@@ -135,6 +135,12 @@ int sponge_lj_get_energy(sponge_lj* lj, const sponge_nl* nl, const unsigned int*
 /* LENNARD_JONES_INFORMATION::Update_Volume (Lennard_Jones_force.cu:1450-1456). */
 int sponge_lj_update_volume(sponge_lj* lj, const float box_length[3]);
 
+/* Replaces the A / B tables of a plain handle (same layout as
+ * sponge_lj_create; used to evaluate new parameters, e.g. by the gradient
+ * path's callers). Synchronous; not for soft-core handles. No counterpart in
+ * the reference, whose tables are fixed after Initial. */
+int sponge_lj_set_tables(sponge_lj* lj, const float* h_LJ_A, const float* h_LJ_B);
+
 /* Host copies the reference keeps public for nb14 (Lennard_Jones_force.cuh:52-58). */
 int sponge_lj_host_tables(const sponge_lj* lj, const float** h_LJ_A, const float** h_LJ_B,
                           const int** h_atom_LJ_type);
@@ -233,6 +239,29 @@ int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const fl
                          const int* d_excluded_numbers, float* d_frc, int flags, int need_energy,
                          int need_virial, float* d_atom_energy, float* d_atom_virial,
                          double* d_parts, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Gradients (no counterpart in the reference, which has no adjoint). Both
+ * recompute the pair terms over the current neighbor list, run the
+ * reciprocal pipeline, and write FP64 device arrays: d_grad_charge (n, in
+ * the units of the stored charges), d_grad_A / d_grad_B (T(T+1)/2, with
+ * respect to the stored x12 / x6 pair tables). Plain LJ handles with at most
+ * 8 types. The first call allocates the adjoint scratch.
+ * Energy: E = E_LJ + E_lrc + E_direct + E_correction + E_reciprocal + E_self.
+ * Force matching: L = sum_i |F_i - T_i|^2 from the caller's forces F (of the
+ * same parameters, e.g. from sponge_nb_force_step) and targets T; *d_loss = L. */
+int sponge_nb_energy_gradient(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme,
+                              const unsigned int* d_uint_crd, const float* d_charge,
+                              const int* d_excluded_list_start, const int* d_excluded_list,
+                              const int* d_excluded_numbers, double* d_grad_charge,
+                              double* d_grad_A, double* d_grad_B, void* stream);
+int sponge_nb_force_matching_gradient(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme,
+                                      const unsigned int* d_uint_crd, const float* d_charge,
+                                      const int* d_excluded_list_start, const int* d_excluded_list,
+                                      const int* d_excluded_numbers, const float* d_frc,
+                                      const float* d_target_frc, double* d_loss,
+                                      double* d_grad_charge, double* d_grad_A, double* d_grad_B,
+                                      void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched systems (no counterpart in the reference, which runs one system

@@ -39,6 +39,8 @@ def lib():
             "orc_direct_soft": (None, [I, P, P, P, P, P, P, P, P, P, D, D, D, D, D, P, D, D, Lg, P,
                                        P, P, P, P, P]),
             "orc_lj_soft_lrc_factor": (D, [I, P, P, I, I, P, P, D, D]),
+            "orc_energy_grad": (None, [I, P, P, P, I, P, P, P, D, D, Lg, P, P, P, P, I, I, I, P, P, P]),
+            "orc_loss_grad": (D, [I, P, P, P, I, P, P, P, D, D, Lg, P, P, P, P, I, I, I, P, P, P, P, P]),
             "orc_pme_beta": (D, [D, D]),
             "orc_bspline_mod2": (D, [I, I]),
             "orc_pme_bc_full": (None, [I, I, I, P, D, P]),
@@ -161,6 +163,46 @@ def lj_soft_lrc_factor(fs, lam=None):
         fs.n, _p(_c(fs.lj_type, np.int32)), _p(_c(fs.lj_type_B, np.int32)), fs.ntypes, fs.ntypes_B,
         _p(_c(fs.lj_B, np.float32)), _p(_c(fs.lj_B_B, np.float32)),
         float(fs.lam if lam is None else lam), float(fs.cutoff))
+
+
+def _grad_args(sys_, uint_crd, pairs, beta, K, charge, A, B):
+    es, en, el = _excl(sys_)
+    return [sys_.n, _p(_c(uint_crd, np.uint32)), _p(_c(sys_.charge if charge is None else charge, np.float32)),
+            _p(_c(sys_.lj_type, np.int32)), sys_.ntypes, _p(_c(sys_.lj_A if A is None else A, np.float32)),
+            _p(_c(sys_.lj_B if B is None else B, np.float32)), _p(_c(sys_.box, np.float32)), float(sys_.cutoff),
+            float(beta), len(pairs), _p(_c(pairs, np.int32)), _p(es), _p(en), _p(el), int(K[0]), int(K[1]),
+            int(K[2])], (es, en, el)
+
+
+def energy_grad(sys_, uint_crd, pairs, beta, K, charge=None, A=None, B=None):
+    """FP64 dE/dq (n), dE/dA and dE/dB (stored x12 / x6 triangular tables) of
+    the total nonbonded energy (orc_energy_grad)."""
+    args, keep = _grad_args(sys_, uint_crd, pairs, beta, K, charge, A, B)
+    P = len(sys_.lj_A)
+    gq, gA, gB = np.zeros(sys_.n), np.zeros(P), np.zeros(P)
+    lib().orc_energy_grad(*args, _p(gq), _p(gA), _p(gB))
+    return gq, gA, gB
+
+
+def loss_grad(sys_, uint_crd, pairs, beta, K, F, T, charge=None, A=None, B=None):
+    """Force-matching loss L = sum |F - T|^2 and its FP64 gradients
+    (orc_loss_grad); F are the forces of the same parameters."""
+    args, keep = _grad_args(sys_, uint_crd, pairs, beta, K, charge, A, B)
+    P = len(sys_.lj_A)
+    gq, gA, gB = np.zeros(sys_.n), np.zeros(P), np.zeros(P)
+    L = lib().orc_loss_grad(*args, _p(_c(F, np.float64)), _p(_c(T, np.float64)), _p(gq), _p(gA), _p(gB))
+    return L, gq, gA, gB
+
+
+def total_energy(sys_, charge=None, A=None, B=None, beta=None, K=None):
+    """E_LJ + E_lrc + E_direct + E_correction + E_reciprocal + E_self (FP64)."""
+    import dataclasses
+    s = sys_
+    if A is not None or B is not None:
+        s = dataclasses.replace(sys_, lj_A=np.asarray(A if A is not None else sys_.lj_A, np.float32),
+                                lj_B=np.asarray(B if B is not None else sys_.lj_B, np.float32))
+    r = full_reference(s, charge=charge, beta=beta, K=K)
+    return r["e_lj"] + r["e_lrc"] + r["e_direct"] + r["e_corr"] + r["e_rec"] + r["e_self"]
 
 
 def excluded(sys_, uint_crd, beta, charge=None):

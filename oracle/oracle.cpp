@@ -688,6 +688,235 @@ double orc_pme_reciprocal(int n, const uint32_t* u, const float* q, const float*
     return E;
 }
 
+// ---------------------------------------------------------------------------
+// Gradients of the nonbonded energy and of a force-matching loss with respect
+// to per-atom charges and the (stored, x12 / x6) LJ pair tables. The reference
+// has no adjoint; these are the analytic derivatives of the formulas restated
+// above (orc_direct, orc_excluded, orc_pme_reciprocal, orc_pme_self,
+// orc_lj_lrc_factor), FP64, over the same half list and cutoff predicate.
+//
+// Energy: dE/dq_i = sum_j q_j erfc(b r)/r - sum_excl q_j erf(b r)/r
+//                   + sum_g R[g] M_i(g) - 2b/sqrt(pi) q_i + 2 nf sum q,
+//         dE/dA_p = sum_{pairs of type p} r^-12/12,
+//         dE/dB_p = -sum r^-6/6 + d(lrc/V)/dB_p,
+// with R = theta*Q the convolved grid and nf = -pi/(2 b^2 V).
+// Loss L = sum_i |F_i - T_i|^2, w_i = 2 (F_i - T_i), d = r_j - r_i:
+//   pair:      dL/dq_i += -q_j g(r) d.(w_i - w_j),  g = (2b/sqrt(pi) r e^{-b^2r^2} + erfc(b r))/r^3
+//              dL/dA_p += -r^-14 d.(w_i - w_j),     dL/dB_p += r^-8 d.(w_i - w_j)
+//   excluded:  dL/dq_i += -q_j h(r) d.(w_i - w_j),  h = g - 1/r^3 (and i <-> j)
+//   reciprocal: dL/dq_k += -sum_g R[g] w_k.grad M_k(g) - sum_g (theta*D)[g] M_k(g),
+//              D[g] = sum_i q_i w_i.grad M_i(g)  (grad M carries K/L per axis).
+static void pme_frac_base(int n, const uint32_t* u, const int* K, std::vector<int>& base, std::vector<double>& frac) {
+    base.resize(3 * (size_t)n);
+    frac.resize(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            double t = (double)u[3 * i + d] * (double)K[d] / 4294967296.0;
+            double b = std::floor(t);
+            base[3 * i + d] = (int)b;
+            frac[3 * i + d] = t - b;
+        }
+}
+
+// sum over the 64 grid points of atom i of grid[g] * (weight chosen by mode):
+// mode 0: M(g); mode 1: w.grad M(g) with wv = (w_x Kx/Lx, w_y Ky/Ly, w_z Kz/Lz).
+static double pme_atom_gather(const double* grid, const int* base, const double* frac, const int* K,
+                              const double* wv, int mode) {
+    double wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+    for (int k = 0; k < 4; k++) {
+        wx[k] = bspline_M(frac[0] + k, 4);
+        wy[k] = bspline_M(frac[1] + k, 4);
+        wz[k] = bspline_M(frac[2] + k, 4);
+        dx[k] = bspline_M(frac[0] + k, 3) - bspline_M(frac[0] + k - 1, 3);
+        dy[k] = bspline_M(frac[1] + k, 3) - bspline_M(frac[1] + k - 1, 3);
+        dz[k] = bspline_M(frac[2] + k, 3) - bspline_M(frac[2] + k - 1, 3);
+    }
+    double s = 0;
+    for (int a = 0; a < 4; a++) {
+        int gx = ((base[0] - a) % K[0] + K[0]) % K[0];
+        for (int b = 0; b < 4; b++) {
+            int gy = ((base[1] - b) % K[1] + K[1]) % K[1];
+            for (int c = 0; c < 4; c++) {
+                int gz = ((base[2] - c) % K[2] + K[2]) % K[2];
+                double m = mode == 0 ? wx[a] * wy[b] * wz[c]
+                                     : wv[0] * dx[a] * wy[b] * wz[c] + wv[1] * wx[a] * dy[b] * wz[c] +
+                                           wv[2] * wx[a] * wy[b] * dz[c];
+                s += grid[((size_t)gx * K[1] + gy) * K[2] + gz] * m;
+            }
+        }
+    }
+    return s;
+}
+
+void orc_energy_grad(int n, const uint32_t* u, const float* q, const int* type, int ntypes, const float* A,
+                     const float* B, const float* box, double rc, double beta, long npairs, const int* pairs,
+                     const int* es, const int* en, const int* el, int Kx, int Ky, int Kz, double* gq, double* gA,
+                     double* gB) {
+    double cof[3];
+    float coff[3];
+    for (int d = 0; d < 3; d++) {
+        cof[d] = (double)box[d] / 4294967296.0;
+        coff[d] = (1.0f / 4294967296.0f) * box[d];
+    }
+    const float rcf = (float)rc;
+    const float rc2f = rcf * rcf;
+    const int P = ntypes * (ntypes + 1) / 2;
+    for (int i = 0; i < n; i++) gq[i] = 0;
+    for (int p = 0; p < P; p++) gA[p] = gB[p] = 0;
+    for (long k = 0; k < npairs; k++) {
+        int i = pairs[2 * k], j = pairs[2 * k + 1];
+        if (!(nl_dr2(u + 3 * i, u + 3 * j, coff) < rc2f)) continue;
+        double dr[3];
+        for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+        double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2], r = std::sqrt(r2);
+        double pot = std::erfc(beta * r) / r;
+        gq[i] += (double)q[j] * pot;
+        gq[j] += (double)q[i] * pot;
+        double ri6 = 1.0 / (r2 * r2 * r2);
+        int p = pair_index(type[i], type[j]);
+        gA[p] += ri6 * ri6 / 12.0;
+        gB[p] += -ri6 / 6.0;
+    }
+    for (int i = 0; i < n; i++)
+        for (int k = es[i]; k < es[i] + en[i]; k++) {
+            int j = el[k];
+            double dr[3];
+            for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+            double r = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+            double pot = std::erf(beta * r) / r;
+            gq[i] -= (double)q[j] * pot;
+            gq[j] -= (double)q[i] * pot;
+        }
+    int K[3] = {Kx, Ky, Kz};
+    size_t NG = (size_t)Kx * Ky * Kz;
+    std::vector<double> R(NG), frc(3 * (size_t)n, 0.0);
+    orc_pme_reciprocal(n, u, q, box, Kx, Ky, Kz, beta, frc.data(), nullptr, R.data());
+    std::vector<int> base;
+    std::vector<double> frac;
+    pme_frac_base(n, u, K, base, frac);
+    double sq = 0;
+    for (int i = 0; i < n; i++) sq += q[i];
+    const double V = (double)box[0] * box[1] * box[2];
+    const double nf = -PI_D / (2.0 * beta * beta * V);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++)
+        gq[i] += pme_atom_gather(R.data(), &base[3 * (size_t)i], &frac[3 * (size_t)i], K, nullptr, 0) -
+                 2.0 * beta / std::sqrt(PI_D) * q[i] + 2.0 * nf * sq;
+    // long-range dispersion: lrc/V = c sum_{ordered ij} B[p_ij], c = -2/3 pi / rc^3 / 6 / V
+    std::vector<double> cnt(ntypes, 0.0);
+    for (int i = 0; i < n; i++) cnt[type[i]] += 1;
+    const double c = (-2.0 / 3.0) * PI_D / (rc * rc * rc) / 6.0 / V;
+    for (int a = 0; a < ntypes; a++)
+        for (int b = 0; b < ntypes; b++) gB[pair_index(a, b)] += c * cnt[a] * cnt[b];
+}
+
+double orc_loss_grad(int n, const uint32_t* u, const float* q, const int* type, int ntypes, const float* A,
+                     const float* B, const float* box, double rc, double beta, long npairs, const int* pairs,
+                     const int* es, const int* en, const int* el, int Kx, int Ky, int Kz, const double* F,
+                     const double* T, double* gq, double* gA, double* gB) {
+    (void)A;
+    (void)B;
+    double cof[3];
+    float coff[3];
+    for (int d = 0; d < 3; d++) {
+        cof[d] = (double)box[d] / 4294967296.0;
+        coff[d] = (1.0f / 4294967296.0f) * box[d];
+    }
+    const float rcf = (float)rc;
+    const float rc2f = rcf * rcf;
+    const double tsp = 2.0 / std::sqrt(PI_D);
+    const int P = ntypes * (ntypes + 1) / 2;
+    std::vector<double> w(3 * (size_t)n);
+    double L = 0;
+    for (size_t a = 0; a < 3 * (size_t)n; a++) {
+        double e = F[a] - T[a];
+        L += e * e;
+        w[a] = 2.0 * e;
+    }
+    for (int i = 0; i < n; i++) gq[i] = 0;
+    for (int p = 0; p < P; p++) gA[p] = gB[p] = 0;
+    for (long k = 0; k < npairs; k++) {
+        int i = pairs[2 * k], j = pairs[2 * k + 1];
+        if (!(nl_dr2(u + 3 * i, u + 3 * j, coff) < rc2f)) continue;
+        double dr[3];
+        for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+        double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2], r = std::sqrt(r2);
+        double sd = 0;
+        for (int d = 0; d < 3; d++) sd += dr[d] * (w[3 * (size_t)i + d] - w[3 * (size_t)j + d]);
+        double br = beta * r;
+        double g = (tsp * br * std::exp(-br * br) + std::erfc(br)) / (r2 * r);
+        gq[i] += -(double)q[j] * g * sd;
+        gq[j] += -(double)q[i] * g * sd;
+        double ri2 = 1.0 / r2, ri6 = ri2 * ri2 * ri2, ri8 = ri6 * ri2;
+        int p = pair_index(type[i], type[j]);
+        gA[p] += -ri6 * ri8 * sd;
+        gB[p] += ri8 * sd;
+    }
+    for (int i = 0; i < n; i++)
+        for (int k = es[i]; k < es[i] + en[i]; k++) {
+            int j = el[k];
+            double dr[3];
+            for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+            double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2], r = std::sqrt(r2);
+            double sd = 0;
+            for (int d = 0; d < 3; d++) sd += dr[d] * (w[3 * (size_t)i + d] - w[3 * (size_t)j + d]);
+            double br = beta * r;
+            double h = (tsp * br * std::exp(-br * br) + std::erfc(br) - 1.0) / (r2 * r);
+            gq[i] += -(double)q[j] * h * sd;
+            gq[j] += -(double)q[i] * h * sd;
+        }
+    int K[3] = {Kx, Ky, Kz};
+    size_t NG = (size_t)Kx * Ky * Kz;
+    std::vector<double> R(NG), frc(3 * (size_t)n, 0.0);
+    orc_pme_reciprocal(n, u, q, box, Kx, Ky, Kz, beta, frc.data(), nullptr, R.data());
+    std::vector<int> base;
+    std::vector<double> frac;
+    pme_frac_base(n, u, K, base, frac);
+    std::vector<double> wv(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) wv[3 * (size_t)i + d] = w[3 * (size_t)i + d] * K[d] / (double)box[d];
+    // D grid: sum_i q_i w_i.grad M_i
+    std::vector<cplx> a(NG, cplx(0, 0));
+    for (int i = 0; i < n; i++) {
+        const double* f = &frac[3 * (size_t)i];
+        const int* b0 = &base[3 * (size_t)i];
+        const double* v = &wv[3 * (size_t)i];
+        double wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+        for (int k = 0; k < 4; k++) {
+            wx[k] = bspline_M(f[0] + k, 4);
+            wy[k] = bspline_M(f[1] + k, 4);
+            wz[k] = bspline_M(f[2] + k, 4);
+            dx[k] = bspline_M(f[0] + k, 3) - bspline_M(f[0] + k - 1, 3);
+            dy[k] = bspline_M(f[1] + k, 3) - bspline_M(f[1] + k - 1, 3);
+            dz[k] = bspline_M(f[2] + k, 3) - bspline_M(f[2] + k - 1, 3);
+        }
+        for (int aa = 0; aa < 4; aa++) {
+            int gx = ((b0[0] - aa) % Kx + Kx) % Kx;
+            for (int bb = 0; bb < 4; bb++) {
+                int gy = ((b0[1] - bb) % Ky + Ky) % Ky;
+                for (int cc = 0; cc < 4; cc++) {
+                    int gz = ((b0[2] - cc) % Kz + Kz) % Kz;
+                    double m = v[0] * dx[aa] * wy[bb] * wz[cc] + v[1] * wx[aa] * dy[bb] * wz[cc] +
+                               v[2] * wx[aa] * wy[bb] * dz[cc];
+                    a[((size_t)gx * Ky + gy) * Kz + gz] += (double)q[i] * m;
+                }
+            }
+        }
+    }
+    std::vector<double> bc(NG);
+    orc_pme_bc_full(Kx, Ky, Kz, box, beta, bc.data());
+    fft3d(a, Kx, Ky, Kz, -1);
+    for (size_t k = 0; k < NG; k++) a[k] *= bc[k];
+    fft3d(a, Kx, Ky, Kz, +1);
+    std::vector<double> RD(NG);
+    for (size_t k = 0; k < NG; k++) RD[k] = a[k].real();
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++)
+        gq[i] += -pme_atom_gather(R.data(), &base[3 * (size_t)i], &frac[3 * (size_t)i], K, &wv[3 * (size_t)i], 1) -
+                 pme_atom_gather(RD.data(), &base[3 * (size_t)i], &frac[3 * (size_t)i], K, nullptr, 0);
+    return L;
+}
+
 // Self and neutralising terms (PME_force.cu:757-762, neutralizing_factor :166-167):
 // E_self = -b/sqrt(pi) sum q^2, E_neutral = -pi/(2 b^2 V) (sum q)^2.
 void orc_pme_self(int n, const float* q, double beta, const float* box, double* out2) {

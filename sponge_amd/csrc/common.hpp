@@ -179,6 +179,7 @@ struct sponge_lj {
     float4* d_sc;                // soft-core handles: 2 ntypes^2 per-pair-type records, else NULL
     int* d_type;                 // n (soft-core handles: combined (type A, type B, mask) index)
     PairScratch ps;              // sized from the neighbor list on first use
+    void* adj;                   // adjoint scratch, allocated by the first gradient call
 };
 
 struct sponge_pme {
@@ -270,8 +271,12 @@ int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s);
 int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s);
 int snb_batch_excluded(sponge_batch* b, hipStream_t s);
 int snb_lj_scratch(sponge_lj* lj, const sponge_nl* nl);
+void snb_adj_free(sponge_lj* lj);
 int snb_batch_reciprocal(sponge_batch* b, int energy, hipStream_t s);
 int snb_batch_gather(sponge_batch* b, hipStream_t s);
 int snb_batch_finalize(sponge_batch* b, hipStream_t s);
 int snb_batch_pme_attributes(int mb, size_t lds_fwd);
+int snb_pme_prep(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, hipStream_t s);
+int snb_pme_adjoint_grid(sponge_pme* pme, const float4* wv, const float* scale, hipStream_t s);
+int snb_pme_adj_gather(sponge_pme* pme, int mode, const float4* wv, float sign, float* out, hipStream_t s);
 int snb_batch_nl(sponge_batch* b, int forced, hipStream_t s);

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "pair_math.hpp"
 #include "pme_prep.hpp"
 
 using namespace snb;
@@ -28,7 +29,6 @@ using namespace snb;
 namespace {
 
 constexpr int kPairWaves = 4;
-constexpr float kTwoOverSqrtPi = 1.1283791670955126f;
 
 // Per-step gather into cluster order (replaces Copy_Crd_And_Charge_To_New_Crd,
 // Lennard_Jones_force.cu:24-34). The LJ type (< 64) rides in the low two bits
@@ -110,25 +110,7 @@ __global__ __launch_bounds__(256) void step_prep_batch(const BatchSys* __restric
                    b.io.uint_crd, b.io.charge, b.lj.d_type, b.lj.ps.pk, P, energy, b.lj.ps.acc);
 }
 
-__device__ __forceinline__ int unpack_type(uint4 v) { return (int)((v.x & 3u) | ((v.y & 3u) << 2) | ((v.z & 3u) << 4)); }
 
-// erfc(x) = exp(-x^2) * erfcx(x) with erfcx(x) ~ t P7(t), t = 1/(1 + p x):
-// the form of Abramowitz & Stegun 7.1.26 with eight coefficients refitted
-// (iteratively reweighted least squares on [0, 5]) for a relative error of
-// 6.3e-8 in erfcx; evaluated in FP32 the relative error of erfc and of the
-// force factor 2x/sqrt(pi) e^{-x^2} + erfc(x) stays below 5.2e-7 on [0, 4.5].
-__device__ __forceinline__ float erfcx_poly(float x) {
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3195f, x, 1.0f));
-    float p = -0.181163783781557f;
-    p = fmaf(p, t, 0.7263733451877996f);
-    p = fmaf(p, t, -0.8054935211563752f);
-    p = fmaf(p, t, 0.8311706379959746f);
-    p = fmaf(p, t, -0.18734797244699758f);
-    p = fmaf(p, t, 0.27142876592060167f);
-    p = fmaf(p, t, 0.16357488112143515f);
-    p = fmaf(p, t, 0.18145758631968023f);
-    return t * p;
-}
 
 // Soft-core pair terms (LJ_soft_core.cu:254-300): with s^6 = r^6 + c per end
 // state, F = (f_LJ - f_C) dr where
@@ -581,8 +563,32 @@ extern "C" int sponge_lj_soft_create(int atom_numbers, int atom_type_numbers_A, 
     return SPONGE_NB_OK;
 }
 
+// New A / B tables for a plain handle (x12 / x6 triangular, as created): the
+// square device table, the host copies and the long-range factor. Synchronous.
+extern "C" int sponge_lj_set_tables(sponge_lj* lj, const float* h_LJ_A, const float* h_LJ_B) {
+    SNB_ARG(lj && h_LJ_A && h_LJ_B, "lj_set_tables");
+    SNB_ARG(!lj->d_sc, "lj_set_tables: soft-core handles keep their tables");
+    const int T = lj->ntypes, P = T * (T + 1) / 2;
+    memcpy(lj->h_A, h_LJ_A, P * sizeof(float));
+    memcpy(lj->h_B, h_LJ_B, P * sizeof(float));
+    std::vector<double> cnt(T, 0.0);
+    for (int i = 0; i < lj->n; i++) cnt[lj->h_type[i]] += 1;
+    std::vector<float2> sq((size_t)T * T);
+    double sum = 0;
+    for (int x = 0; x < T; x++)
+        for (int y = 0; y < T; y++) {
+            const int hi = x > y ? x : y, lo = x > y ? y : x, k = hi * (hi + 1) / 2 + lo;
+            sum += cnt[x] * cnt[y] * (double)h_LJ_B[k];
+            sq[(size_t)x * T + y] = make_float2(h_LJ_A[k], h_LJ_B[k]);
+        }
+    lj->lrc_factor = sum * (-2.0 / 3.0) * M_PI / ((double)lj->cutoff * lj->cutoff * lj->cutoff) / 6.0;
+    SNB_HIP(hipMemcpy(lj->d_ab, sq.data(), sq.size() * sizeof(float2), hipMemcpyHostToDevice));
+    return SPONGE_NB_OK;
+}
+
 extern "C" void sponge_lj_destroy(sponge_lj* lj) {
     if (!lj) return;
+    snb_adj_free(lj);
     void* ptrs[] = {lj->d_ab, lj->d_sc, lj->d_type};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -246,13 +246,19 @@ __device__ __forceinline__ float bsp_dw(float f, int k) {
 // plane x receives the atoms filed under planes x .. x+3 (with kx = their
 // offset). The real plane is held directly in the packed complex layout of the
 // z-FFT: row y is the real (y even) or imaginary (y odd) part of line y/2.
-template <int MAXB>
+//
+// MODE 1 spreads, instead of the charge, the adjoint source of the
+// force-matching gradient, q_i w_i . grad M_i(g): wv[i] = (w_x Kx/Lx,
+// w_y Ky/Ly, w_z Kz/Lz), with the fixed-point unit read from scale_ptr (set
+// on the device so that no contribution reaches 2^31).
+template <int MAXB, int MODE = 0>
 __device__ __forceinline__ void fwd_plane_body(
     int x, const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
     const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz,
     const FftPlan& pz, const FftPlan& py, const float2* __restrict__ twz, const float2* __restrict__ twy,
-    double* __restrict__ acc) {
+    double* __restrict__ acc, const float4* __restrict__ wv = nullptr, const float* __restrict__ scale_ptr = nullptr) {
     extern __shared__ float2 sm[];
+    const float unit = MODE ? scale_ptr[0] : kSpreadScale;
     const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
     const int plane = max(Ky * H, nrp * Kz);
     float2* tz = sm + max(plane, Ky * Kz);  // past the fixed-point grid too
@@ -298,12 +304,13 @@ __device__ __forceinline__ void fwd_plane_body(
             }
         }
         unsigned bb[kSpreadPerThread];
-        float4 ff[kSpreadPerThread];
+        float4 ff[kSpreadPerThread], ww[kSpreadPerThread];
 #pragma unroll
         for (int r = 0; r < kSpreadPerThread; r++) {
             if (ai[r] >= 0) {
                 bb[r] = base[ai[r]];
                 ff[r] = frac[ai[r]];
+                if (MODE) ww[r] = wv[ai[r]];
             }
         }
 #pragma unroll
@@ -311,24 +318,44 @@ __device__ __forceinline__ void fwd_plane_body(
             if (ai[r] < 0) continue;
             const int by = (int)((bb[r] >> 10) & 1023u), bz = (int)(bb[r] >> 20);
             const float4 f = ff[r];
-            const float wx = f.w * kSpreadScale * bsp_w(f.x, akx[r]);
-            float wz[4];
+            float wz[4], dz[4];
             int gz[4];
 #pragma unroll
             for (int kz = 0; kz < 4; kz++) {
                 wz[kz] = bsp_w(f.z, kz);
+                if (MODE) dz[kz] = bsp_dw(f.z, kz);
                 int tt = bz - kz;
                 gz[kz] = tt + (tt < 0 ? Kz : 0);
             }
+            if (MODE == 0) {
+                const float wx = f.w * unit * bsp_w(f.x, akx[r]);
 #pragma unroll
-            for (int ky = 0; ky < 4; ky++) {
-                int gy = by - ky;
-                gy += gy < 0 ? Ky : 0;
-                const float wy = wx * bsp_w(f.y, ky);
-                unsigned long long* row = qacc + gy * Kz;
+                for (int ky = 0; ky < 4; ky++) {
+                    int gy = by - ky;
+                    gy += gy < 0 ? Ky : 0;
+                    const float wy = wx * bsp_w(f.y, ky);
+                    unsigned long long* row = qacc + gy * Kz;
 #pragma unroll
-                for (int kz = 0; kz < 4; kz++)
-                    atomicAdd(row + gz[kz], (unsigned long long)(long long)__float2int_rn(wy * wz[kz]));
+                    for (int kz = 0; kz < 4; kz++)
+                        atomicAdd(row + gz[kz], (unsigned long long)(long long)__float2int_rn(wy * wz[kz]));
+                }
+            } else {
+                const float4 w = ww[r];
+                const float qs = f.w * unit;
+                const float ax = bsp_w(f.x, akx[r]), dax = bsp_dw(f.x, akx[r]);
+#pragma unroll
+                for (int ky = 0; ky < 4; ky++) {
+                    int gy = by - ky;
+                    gy += gy < 0 ? Ky : 0;
+                    const float ay = bsp_w(f.y, ky), day = bsp_dw(f.y, ky);
+                    const float c1 = qs * (w.x * dax * ay + w.y * ax * day);
+                    const float c2 = qs * w.z * ax * ay;
+                    unsigned long long* row = qacc + gy * Kz;
+#pragma unroll
+                    for (int kz = 0; kz < 4; kz++)
+                        atomicAdd(row + gz[kz],
+                                  (unsigned long long)(long long)__float2int_rn(fmaf(c1, wz[kz], c2 * dz[kz])));
+                }
             }
         }
     }
@@ -340,7 +367,7 @@ __device__ __forceinline__ void fwd_plane_body(
 #pragma unroll
     for (int s = 0; s < 2 * MAXB; s++) {
         const int o = threadIdx.x + s * kPlaneThreads;
-        if (o < npt) qv[s] = (float)((double)(long long)qacc[o] * (1.0 / kSpreadScale));
+        if (o < npt) qv[s] = (float)((double)(long long)qacc[o] / (double)unit);
     }
     __syncthreads();
     float* smf = reinterpret_cast<float*>(sm);
@@ -392,6 +419,16 @@ __global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
     FftPlan py, const float2* __restrict__ twz, const float2* __restrict__ twy, double* __restrict__ acc) {
     fwd_plane_body<MAXB>(blockIdx.x, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, py, twz, twy,
                          acc);
+}
+
+template <int MAXB>
+__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_adjoint_kernel(
+    const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
+    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, FftPlan pz,
+    FftPlan py, const float2* __restrict__ twz, const float2* __restrict__ twy, const float4* __restrict__ wv,
+    const float* __restrict__ scale_ptr) {
+    fwd_plane_body<MAXB, 1>(blockIdx.x, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, py, twz,
+                            twy, nullptr, wv, scale_ptr);
 }
 
 // Batched: block -> (system, x plane) from the block map.
@@ -594,6 +631,52 @@ __device__ __forceinline__ void pme_gather_body(int blk, int n, const unsigned* 
         frc[3 * i + 1] += -qi * sy * iby;
         frc[3 * i + 2] += -qi * sz * ibz;
     }
+}
+
+// Adjoint gathers, 16 lanes per atom as pme_gather_body:
+// out[i] += sign * sum_g R[g] m_i(g), m = M (MODE 0) or wv_i . grad M (MODE 1,
+// wv carrying K/L per axis like the force).
+template <int MODE>
+__global__ __launch_bounds__(kGatherThreads) void pme_adj_gather_kernel(
+    int n, const unsigned* __restrict__ base, const float4* __restrict__ frac, const float* __restrict__ R, int Kx,
+    int Ky, int Kz, const float4* __restrict__ wv, float sign, float* __restrict__ out) {
+    const int gid = blockIdx.x * kGatherThreads + threadIdx.x;
+    const int i = gid >> 4, sub = gid & 15;
+    const int kx = sub >> 2, ky = sub & 3;
+    float acc = 0.f;
+    if (i < n) {
+        const unsigned bb = base[i];
+        const int bx = (int)(bb & 1023u), by = (int)((bb >> 10) & 1023u), bz = (int)(bb >> 20);
+        const float4 f = frac[i];
+        const float wx = bsp_w(f.x, kx), wy = bsp_w(f.y, ky);
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        float dwx = 0.f, dwy = 0.f;
+        if (MODE) {
+            w = wv[i];
+            dwx = bsp_dw(f.x, kx);
+            dwy = bsp_dw(f.y, ky);
+        }
+        int gx = bx - kx, gy = by - ky;
+        gx += gx < 0 ? Kx : 0;
+        gy += gy < 0 ? Ky : 0;
+        const float* row = R + ((size_t)gx * Ky + gy) * Kz;
+#pragma unroll
+        for (int kz = 0; kz < 4; kz++) {
+            int gz = bz - kz;
+            gz += gz < 0 ? Kz : 0;
+            const float phi = row[gz];
+            const float wz = bsp_w(f.z, kz);
+            if (MODE == 0) {
+                acc += phi * wx * wy * wz;
+            } else {
+                const float dwz = bsp_dw(f.z, kz);
+                acc += phi * (w.x * dwx * wy * wz + w.y * wx * dwy * wz + w.z * wx * wy * dwz);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) acc += __shfl_xor(acc, o);
+    if (i < n && sub == 0) out[i] += sign * acc;
 }
 
 __global__ __launch_bounds__(kGatherThreads) void pme_gather_kernel(int n, const unsigned* __restrict__ base,
@@ -1067,5 +1150,71 @@ int snb_batch_pme_attributes(int mb, size_t lds_fwd) {
                      : mb == 8 ? (const void*)fwd_plane_batch<8>
                                : (const void*)fwd_plane_batch<16>;
     SNB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fwd));
+    return SPONGE_NB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Pieces of the adjoint (gradient) path.
+int snb_pme_prep(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, hipStream_t s) {
+    pme_prep_kernel<<<ceil_div(pme->n, 256), 256, 0, s>>>(pme->n, d_uint_crd, d_charge, pme_prep_args(pme), 0);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+// R = theta * D for the adjoint source D[g] = sum_i q_i wv_i . grad M_i(g);
+// the atom bins must have been filled (snb_pme_prep) since the last pipeline.
+int snb_pme_adjoint_grid(sponge_pme* pme, const float4* wv, const float* scale, hipStream_t s) {
+    const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
+    FftPlan px, py, pz;
+    snb_pme_plans(pme, &px, &py, &pz);
+    const int mb = snb_pme_plane_maxb(pme);
+    const size_t lds_fwd = fwd_plane_lds(Ky, Kz), lds = snb_pme_lds_inv(pme);
+    if (lds_fwd > 64 * 1024) {
+        const void* fn = mb == 1 ? (const void*)fwd_plane_adjoint_kernel<1>
+                         : mb == 2 ? (const void*)fwd_plane_adjoint_kernel<2>
+                         : mb == 4 ? (const void*)fwd_plane_adjoint_kernel<4>
+                         : mb == 8 ? (const void*)fwd_plane_adjoint_kernel<8>
+                                   : (const void*)fwd_plane_adjoint_kernel<16>;
+        SNB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fwd));
+    }
+#define SNB_FWDA(MB)                                                                                           \
+    fwd_plane_adjoint_kernel<MB><<<Kx, kPlaneThreads, lds_fwd, s>>>(pme->base, pme->frac, pme->bin_count,      \
+                                                                    pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, \
+                                                                    Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], \
+                                                                    wv, scale)
+    switch (mb) {
+        case 1: SNB_FWDA(1); break;
+        case 2: SNB_FWDA(2); break;
+        case 4: SNB_FWDA(4); break;
+        case 8: SNB_FWDA(8); break;
+        default: SNB_FWDA(16); break;
+    }
+#undef SNB_FWDA
+    SNB_CHECK_LAUNCH();
+    dim3 cg(Ky, ceil_div(H, kConvChunk));
+    xconv_kernel<<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], 0, pme->acc,
+                                             pme->bin_count);
+    SNB_CHECK_LAUNCH();
+    switch (mb) {
+        case 1: inv_plane_kernel<1><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
+        case 2: inv_plane_kernel<2><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
+        case 4: inv_plane_kernel<4><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
+        case 8: inv_plane_kernel<8><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
+        default: inv_plane_kernel<16><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
+    }
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+int snb_pme_adj_gather(sponge_pme* pme, int mode, const float4* wv, float sign, float* out, hipStream_t s) {
+    const int n = pme->n;
+    const int grid = ceil_div((long)n * 16, kGatherThreads);
+    if (mode == 0)
+        pme_adj_gather_kernel<0><<<grid, kGatherThreads, 0, s>>>(n, pme->base, pme->frac, pme->R, pme->K[0],
+                                                                 pme->K[1], pme->K[2], wv, sign, out);
+    else
+        pme_adj_gather_kernel<1><<<grid, kGatherThreads, 0, s>>>(n, pme->base, pme->frac, pme->R, pme->K[0],
+                                                                 pme->K[1], pme->K[2], wv, sign, out);
+    SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }

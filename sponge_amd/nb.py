@@ -61,6 +61,9 @@ _SIGS = {
     "sponge_pme_update_box_length": (_I, [_P, _FP3]),
     "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "sponge_sum_list": (_I, [_I, _P, _P, _P]),
+    "sponge_lj_set_tables": (_I, [_P, _P, _P]),
+    "sponge_nb_energy_gradient": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "sponge_nb_force_matching_gradient": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "sponge_batch_create": (_I, [_I, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "sponge_batch_destroy": (None, [_P]),
     "sponge_batch_sync": (_I, [_P]),
@@ -224,6 +227,13 @@ class LENNARD_JONES_INFORMATION:
 
     def Update_Volume(self, box_length):
         _check(load_library().sponge_lj_update_volume(self.h, _vec3(box_length)), "LJ Update_Volume")
+
+    def set_tables(self, h_LJ_A, h_LJ_B):
+        """New x12 / x6 triangular tables (plain handles)."""
+        self._A = np.ascontiguousarray(h_LJ_A, np.float32)
+        self._B = np.ascontiguousarray(h_LJ_B, np.float32)
+        _check(load_library().sponge_lj_set_tables(self.h, self._A.ctypes.data_as(_P), self._B.ctypes.data_as(_P)),
+               "sponge_lj_set_tables")
 
     def set_pair_events(self, start=None, stop=None):
         """Brackets later pair-kernel launches with these torch.cuda.Events."""
@@ -400,6 +410,21 @@ class NonbondedEngine:
             _ptr(self.charge), _ptr(self.excl_start), _ptr(self.excl_list), _ptr(self.excl_numbers),
             _ptr(self.frc), int(flags), int(need_energy), int(need_virial), _ptr(self.atom_energy),
             _ptr(self.atom_virial), _ptr(self.parts), _stream(stream)), "nb_force_step")
+
+    def energy_gradient(self, grad_charge, grad_A, grad_B, stream=None):
+        """dE/dq, dE/dA, dE/dB (float64 device tensors) of the total energy
+        at the current coordinates (the list must be current)."""
+        _check(load_library().sponge_nb_energy_gradient(
+            self.nl.h, self.lj.h, self.pme.h, _ptr(self.uint_crd), _ptr(self.charge), _ptr(self.excl_start),
+            _ptr(self.excl_list), _ptr(self.excl_numbers), _ptr(grad_charge), _ptr(grad_A), _ptr(grad_B),
+            _stream(stream)), "sponge_nb_energy_gradient")
+
+    def force_matching_gradient(self, frc, target, loss, grad_charge, grad_A, grad_B, stream=None):
+        """L = sum |frc - target|^2 into loss[0] and its gradients."""
+        _check(load_library().sponge_nb_force_matching_gradient(
+            self.nl.h, self.lj.h, self.pme.h, _ptr(self.uint_crd), _ptr(self.charge), _ptr(self.excl_start),
+            _ptr(self.excl_list), _ptr(self.excl_numbers), _ptr(frc), _ptr(target), _ptr(loss),
+            _ptr(grad_charge), _ptr(grad_A), _ptr(grad_B), _stream(stream)), "sponge_nb_force_matching_gradient")
 
     def energy_parts(self):
         import torch
