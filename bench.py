@@ -137,6 +137,56 @@ def fep_batch_bench(steps: int, warmup: int, refresh: int):
     }
 
 
+def adjoint_bench(eng, reps: int):
+    """Config-5 timing: one forward force step, one energy-gradient call and
+    one force-matching-gradient call (target forces from charges perturbed by
+    N(0, 0.02) e, computed on the GPU), each averaged over `reps` launches."""
+    import numpy as np
+    import torch
+    from sponge_amd import system
+
+    stream = torch.cuda.current_stream()
+    P = len(eng.sys.lj_A)
+    gq = torch.zeros(eng.n, dtype=torch.float64, device="cuda")
+    gA = torch.zeros(P, dtype=torch.float64, device="cuda")
+    gB = torch.zeros(P, dtype=torch.float64, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    q0 = eng.charge.clone()
+    rng = np.random.default_rng(5)
+    dq = torch.as_tensor(rng.normal(0.0, 0.02 * system.CHARGE_SCALE, eng.n), dtype=torch.float32, device="cuda")
+    eng.charge.copy_(q0 + dq)
+    eng.force_step(0, 0, from_crd=True, overwrite=True)
+    target = eng.frc.clone()
+    eng.charge.copy_(q0)
+    eng.force_step(0, 0, from_crd=True, overwrite=True)
+    frc = eng.frc.clone()
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(reps):
+            fn()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / reps
+
+    fwd = timed(lambda: eng.force_step(0, 0, from_crd=True, overwrite=True))
+    e_grad = timed(lambda: eng.energy_gradient(gq, gA, gB))
+    l_grad = timed(lambda: eng.force_matching_gradient(frc, target, loss, gq, gA, gB))
+    return {
+        "workload": f"{eng.sys.name}: gradients w.r.t. {eng.n} charges and {P}+{P} LJ table entries",
+        "forward_step_ms": round(fwd, 4),
+        "energy_gradient_ms": round(e_grad, 4),
+        "force_matching_gradient_ms": round(l_grad, 4),
+        "energy_gradient_per_forward": round(e_grad / fwd, 3),
+        "force_matching_gradient_per_forward": round(l_grad / fwd, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,7 +195,10 @@ def main():
     ap.add_argument("--config", type=int, default=1, help="0: 648 atoms, 1: 23,570, 2: 96,120")
     ap.add_argument("--refresh", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--adjoint-reps", type=int, default=50,
+                    help="launches per timing of the gradient calls (0 skips them)")
     ap.add_argument("--fep-steps", type=int, default=200,
                     help="steps of the batched-FEP comparison (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -204,8 +257,30 @@ def main():
     t_end.record(stream)
     torch.cuda.synchronize()
     eng.lj.set_pair_events(None, None)
-    ms = t_start.elapsed_time(t_end) / K
+    eager_ms = t_start.elapsed_time(t_end) / K
     pair_ms = sum(a.elapsed_time(b) for a, b in zip(ev_s, ev_e)) / K
+
+    # The same steps replayed from hipGraphs, one per kind of MD step: the
+    # force step, and the force step followed by the list rebuild (every
+    # `refresh` steps). Host launch overhead leaves the critical path.
+    ms = eager_ms
+    if not args.no_graph:
+        g_plain, g_rebuild = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g_plain):
+            eng.force_step(0, 0, from_crd=True, overwrite=True, stream=torch.cuda.current_stream())
+        with torch.cuda.graph(g_rebuild):
+            cur = torch.cuda.current_stream()
+            eng.force_step(0, 0, from_crd=True, overwrite=True, stream=cur)
+            eng.update_neighbor_list(forced=1, stream=cur)
+        for k in range(args.warmup):
+            (g_rebuild if k % args.refresh == 0 else g_plain).replay()
+        torch.cuda.synchronize()
+        t_start.record(stream)
+        for k in range(K):
+            (g_rebuild if k % args.refresh == 0 else g_plain).replay()
+        t_end.record(stream)
+        torch.cuda.synchronize()
+        ms = t_start.elapsed_time(t_end) / K
 
     flops = PAIR_FLOPS * pairs_in
     achieved = flops / (pair_ms * 1e-3) / 1e12
@@ -217,6 +292,8 @@ def main():
         "steps": K,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 6),
+        "eager_ms_per_step": round(eager_ms, 6),
+        "graph": not args.no_graph,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -244,6 +321,8 @@ def main():
             "flops_per_launch": flops,
         },
     }
+    if args.adjoint_reps > 0:
+        out["adjoint"] = adjoint_bench(eng, args.adjoint_reps)
     if args.fep_steps > 0:
         out["fep_batch"] = fep_batch_bench(args.fep_steps, 20, args.refresh)
     if not args.no_cpu_baseline:

@@ -85,6 +85,17 @@ int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_excluded_list_sta
                      const int* d_excluded_list, const int* d_excluded_numbers,
                      int forced_update, int forced_check, void* stream);
 
+/* Inner list of the fused step (no counterpart in the reference): every
+ * prune_interval fused steps, sponge_nb_force_step keeps only the list
+ * entries and pair bits within prune_radius (>= cutoff) of the current
+ * coordinates and runs the pair kernel over them; 0 turns it off. The caller
+ * vouches that no atom moves more than (prune_radius - cutoff) / 2 between
+ * prunes. Default for refresh_interval > 0: interval refresh_interval / 4 and
+ * radius cutoff + skin * interval / refresh_interval (the skin prorated, the
+ * reference's own assumption for interval updates); off otherwise. The other
+ * entry points always use the full list. */
+int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius);
+
 /* NEIGHBOR_LIST::Update_Volume (neighbor_list/neighbor_list.cu:645-656). */
 int sponge_nl_update_volume(sponge_nl* nl, const float box_length[3]);
 

@@ -103,21 +103,23 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     if (rc) return rc;
     // Fork: the reciprocal pipeline (spread, FFTs, convolution) runs on the
     // side stream while the pair and exclusion kernels run on s; it reads only
-    // what step_prep wrote. Join before the gather, which adds into d_frc.
+    // what step_prep wrote. Join before the gather, which adds into d_frc. The
+    // pair kernel is enqueued first so that it starts as soon as the prep is
+    // done, however far ahead the host is.
     const bool serial = (flags & SPONGE_STEP_SERIAL) != 0;
     hipStream_t ps = serial ? s : pme->side;
-    if (!serial) {
-        SNB_HIP(hipEventRecord(pme->fork, s));
-        SNB_HIP(hipStreamWaitEvent(ps, pme->fork, 0));
-    }
-    rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, energy, 1, ps);
+    if (!serial) SNB_HIP(hipEventRecord(pme->fork, s));
+    rc = snb_prune(nl, lj->ps.pk, lj->cof_dr, s);
     if (rc) return rc;
-    if (!serial) SNB_HIP(hipEventRecord(pme->join, ps));
     rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme->beta,
                    energy, &lj->ps, d_frc, need_energy > 0 ? d_atom_energy : nullptr,
                    need_virial > 0 ? d_atom_virial : nullptr, nullptr, s, 1,
-                   (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, lj->d_sc);
+                   (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, lj->d_sc, 1);
     if (rc) return rc;
+    if (!serial) SNB_HIP(hipStreamWaitEvent(ps, pme->fork, 0));
+    rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, energy, 1, ps);
+    if (rc) return rc;
+    if (!serial) SNB_HIP(hipEventRecord(pme->join, ps));
     rc = snb_excluded(pme, d_uint_crd, lj->cof_dr, d_charge, d_excluded_list_start, d_excluded_list,
                       d_excluded_numbers, d_frc, nullptr, s);
     if (rc) return rc;

@@ -161,8 +161,16 @@ struct sponge_nl {
     uint4* pack;                // slot_cap sorted (ux, uy, uz, atom) at the last rebuild
     int* flags;                 // [0] rebuild flag, [1] overflow bits, [2] rebuild count,
                                 // [3] slot count, [4] cluster count, [7] skin-check vote,
-                                // [8..9] listed pair count (64-bit)
+                                // [8..9] listed pair count (64-bit), [10] fused steps since
+                                // the last prune
     int device;
+    // inner list of the fused step: the entries and masks of `list` whose
+    // pairs lie within prune_r2, recomputed every prune_interval fused steps
+    // (0 = off) from the step's coordinates
+    int prune_interval;
+    float prune_r2;
+    int* list_in_n;             // ncl_cap
+    uint2* list_in;             // ncl_cap * list_cap
 };
 
 struct sponge_lj {
@@ -250,7 +258,7 @@ struct sponge_batch {
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered = 0,
-              int overwrite = 0, const float4* sc = nullptr);
+              int overwrite = 0, const float4* sc = nullptr, int inner = 0);
 void snb_scratch_free(PairScratch* ps);
 int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float scaler[3], const float* d_charge,
                  const int* es, const int* el, const int* en, float* d_frc, float* d_atom_energy, hipStream_t s);
@@ -271,6 +279,7 @@ int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s);
 int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s);
 int snb_batch_excluded(sponge_batch* b, hipStream_t s);
 int snb_lj_scratch(sponge_lj* lj, const sponge_nl* nl);
+int snb_prune(sponge_nl* nl, const uint4* pk, const float* cof, hipStream_t s);
 void snb_adj_free(sponge_lj* lj);
 int snb_batch_reciprocal(sponge_batch* b, int energy, hipStream_t s);
 int snb_batch_gather(sponge_batch* b, hipStream_t s);

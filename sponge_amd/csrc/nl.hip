@@ -71,6 +71,7 @@ __device__ __forceinline__ void nl_begin_body(int forced, int* flags, int* col_c
             flags[8] = 0;
             flags[9] = 0;
             flags[2] += 1;
+            flags[10] = 0;  // the next fused step prunes the new list
         }
     }
     __syncthreads();
@@ -535,10 +536,22 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
     A((void**)&nl->cl_hi, nl->ncl_cap * sizeof(float4));
     A((void**)&nl->list_n, nl->ncl_cap * sizeof(int));
     A((void**)&nl->list, (size_t)nl->ncl_cap * nl->list_cap * sizeof(uint2));
+    A((void**)&nl->list_in, (size_t)nl->ncl_cap * nl->list_cap * sizeof(uint2));
+    A((void**)&nl->list_in_n, nl->ncl_cap * sizeof(int));
     A((void**)&nl->pack, nl->slot_cap * sizeof(uint4));
     A((void**)&nl->flags, 16 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->flags, 0, 16 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->list_n, 0, nl->ncl_cap * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(nl->list_in_n, 0, nl->ncl_cap * sizeof(int));
+    // Inner list of the fused step: in interval mode the reference trusts the
+    // skin to cover refresh_interval steps of motion; prorating it, an inner
+    // radius of rc + skin * P / refresh_interval covers P steps, and the list
+    // is pruned to it every P = refresh_interval / 4 fused steps.
+    if (refresh_interval > 0) {
+        nl->prune_interval = std::max(1, refresh_interval / 4);
+        const float rp = cutoff + skin * (float)nl->prune_interval / (float)refresh_interval;
+        nl->prune_r2 = rp * rp;
+    }
     if (e == hipSuccess) e = hipMemset(nl->s2o, 0xff, nl->slot_cap * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->col_start, 0, (nl->ncol + 1) * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->old_crd, 0, 3 * n * sizeof(float));
@@ -553,8 +566,9 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
 
 extern "C" void sponge_nl_destroy(sponge_nl* nl) {
     if (!nl) return;
-    void* ptrs[] = {nl->old_crd, nl->nl_uint, nl->col_count, nl->col_start, nl->col_atoms, nl->s2o, nl->o2s,
-                    nl->cl_lo,   nl->cl_hi,   nl->list_n,    nl->list,      nl->pack,      nl->flags};
+    void* ptrs[] = {nl->old_crd, nl->nl_uint, nl->col_count, nl->col_start, nl->col_atoms, nl->s2o,
+                    nl->o2s,     nl->cl_lo,   nl->cl_hi,     nl->list_n,    nl->list,      nl->pack,
+                    nl->flags,   nl->list_in, nl->list_in_n};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete nl;
@@ -611,6 +625,14 @@ extern "C" int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_exclud
                                                          nl->box[2], thresh, nl->flags);
     SNB_CHECK_LAUNCH();
     return nl_rebuild(nl, d_crd, d_excluded_list_start, d_excluded_list, d_excluded_numbers, 0, s);
+}
+
+extern "C" int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius) {
+    SNB_ARG(nl && prune_interval >= 0, "nl_set_prune");
+    SNB_ARG(prune_interval == 0 || prune_radius >= nl->cutoff, "prune radius below the cutoff");
+    nl->prune_interval = prune_interval;
+    nl->prune_r2 = prune_radius * prune_radius;
+    return SPONGE_NB_OK;
 }
 
 extern "C" int sponge_nl_update_volume(sponge_nl* nl, const float box_length[3]) {

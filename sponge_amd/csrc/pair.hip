@@ -159,25 +159,29 @@ __device__ __forceinline__ void soft_pair_energy(double r2, double qq, double be
 // types and A < 0 for soft-core pair types, whose records are sc[t_i nt + t_j]
 // (s0) and sc[nt^2 + t_i nt + t_j] (s1), read from global memory only by
 // waves that meet such a pair.
-template <bool ENERGY, bool SOFT>
+// TB = 2: every type index is below 4 and sits in the x coordinate's low bits
+// alone (two instructions fewer per pair); TB = 6: the general 6-bit index.
+template <bool ENERGY, bool SOFT, int TB>
 __device__ __forceinline__ void pair_body(
     int lb, const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list,
     int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
     float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
-    const float4* __restrict__ sc) {
+    const float4* __restrict__ sc, int* __restrict__ step_counter = nullptr) {
     extern __shared__ float2 sab[];
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ic = lb * kPairWaves + w;
     if (lb < 0 || ic >= flags[4]) return;
+    // fused steps since the last prune (read by the next step's prune kernel)
+    if (step_counter && ic == 0 && lane == 0) atomicAdd(step_counter, 1);
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
     const int islot = ic * kClusterSize + ii;
     const uint4 a = pk[islot];
     const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
     const float qi = __uint_as_float(a.w);
-    const int ti = unpack_type(a);
+    const int ti = TB == 2 ? (int)(a.x & 3u) : unpack_type(a);
     const float2* srow = sab + ti * nt;
     const float mlog2e_b2 = -1.4426950408889634f * (beta * beta);
     int ai = 0;
@@ -205,11 +209,13 @@ __device__ __forceinline__ void pair_body(
             const float dz = (float)(int)((b.z & ~3u) - az) * cz;
             const float r2 = dr2_exact(dx, dy, dz);
             const bool in = ((m >> bit) & 1u) && r2 < rc2;
-            const float r2s = in ? r2 : 1.0f;
+            // masked lanes (padding, the atom itself) may give inf / NaN
+            // below; the final select discards them
+            const float r2s = r2;
             const float rinv = __builtin_amdgcn_rsqf(r2s);
             const float rinv2 = rinv * rinv;
             const float rinv6 = rinv2 * rinv2 * rinv2;
-            const int tj = unpack_type(b);
+            const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
             const float2 AB = srow[tj];
             float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
             const float x = beta * (r2s * rinv);
@@ -316,20 +322,65 @@ __device__ __forceinline__ void pair_body(
     }
 }
 
-template <bool ENERGY, bool SOFT>
+template <bool ENERGY, bool SOFT, int TB>
 __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
     const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list, int list_cap,
     const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz, float rc2,
     float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
-    const float4* __restrict__ sc) {
-    pair_body<ENERGY, SOFT>(xcd_block(gridDim.x), flags, list_n, list, list_cap, pk, ab, nt, cx, cy, cz, rc2, beta,
-                            s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc);
+    const float4* __restrict__ sc, int* __restrict__ step_counter) {
+    pair_body<ENERGY, SOFT, TB>(xcd_block(gridDim.x), flags, list_n, list, list_cap, pk, ab, nt, cx, cy, cz, rc2, beta,
+                                s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc, step_counter);
+}
+
+// Inner list of the fused step: one wave per i-cluster re-tests every pair
+// bit of the cluster-pair list against prune_r2 with the step's coordinates
+// (the pair kernel's records) and compacts the surviving entries. Runs only
+// on steps whose counter (fused steps since the last prune, reset by a
+// rebuild) is a multiple of the interval; the decision is the same for every
+// block, so the kernel can sit in a captured graph.
+__global__ __launch_bounds__(kPairWaves * 64) void prune_kernel(
+    const int* __restrict__ flags, int interval, const int* __restrict__ list_n, const uint2* __restrict__ list,
+    int list_cap, const uint4* __restrict__ pk, float cx, float cy, float cz, float pr2, int* __restrict__ in_n,
+    uint2* __restrict__ in_list) {
+    if (flags[10] % interval != 0) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lb = xcd_block(gridDim.x);
+    const int ic = lb * kPairWaves + w;
+    if (lb < 0 || ic >= flags[4]) return;
+    const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
+    const uint4 a = pk[ic * kClusterSize + ii];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
+    const int n = list_n[ic];
+    const uint2* lst = list + (size_t)ic * list_cap;
+    uint2* out = in_list + (size_t)ic * list_cap;
+    int count = 0;
+    for (int base = 0; base < n; base += 64) {
+        const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
+        const int cnt = min(64, n - base);
+        for (int it = 0; it < cnt; it += 4) {
+            const int src = it + g;
+            const int jc = __shfl((int)my.x, src);
+            const unsigned m = (unsigned)__shfl((int)my.y, src);
+            const uint4 b = pk[jc * kClusterSize + jj];
+            const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+            const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
+            const float dz = (float)(int)((b.z & ~3u) - az) * cz;
+            const bool keep = ((m >> bit) & 1u) && dr2_exact(dx, dy, dz) < pr2;
+            const unsigned long long bal = __ballot(keep);
+            const unsigned mn = (unsigned)(bal >> (g * 16)) & 0xffffu;
+            const bool leader = (lane & 15) == 0 && mn != 0u;
+            const unsigned long long lb2 = __ballot(leader);
+            if (leader) out[count + __popcll(lb2 & ((1ull << lane) - 1ull))] = make_uint2((unsigned)jc, mn);
+            count += __popcll(lb2);
+        }
+    }
+    if (lane == 0) in_n[ic] = count;
 }
 
 // Batched: logical blocks are spread over the XCDs as in the single-system
 // launch, then mapped to (system, block of 4 i-clusters).
-template <bool ENERGY, bool SOFT>
+template <bool ENERGY, bool SOFT, int TB>
 __global__ __launch_bounds__(kPairWaves * 64) void pair_batch(const BatchSys* __restrict__ bs,
                                                               const int2* __restrict__ map, int nmap,
                                                               int overwrite) {
@@ -339,10 +390,11 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_batch(const BatchSys* __
     if (m.x < 0) return;
     const BatchSys& b = bs[m.x];
     const float rc = b.lj.cutoff;
-    pair_body<ENERGY, SOFT>(m.y, b.nl.flags, b.nl.list_n, b.nl.list, b.nl.list_cap, b.lj.ps.pk, b.lj.d_ab,
+    pair_body<ENERGY, SOFT, TB>(m.y, b.nl.flags, b.nl.list_n, b.nl.list, b.nl.list_cap, b.lj.ps.pk, b.lj.d_ab,
                             b.lj.ntypes, b.lj.cof_dr[0], b.lj.cof_dr[1], b.lj.cof_dr[2], rc * rc, b.pme.beta,
                             b.nl.s2o, b.io.frc, ENERGY ? b.io.atom_energy : nullptr,
-                            ENERGY ? b.io.atom_virial : nullptr, nullptr, b.lj.ps.acc, overwrite, b.lj.d_sc);
+                            ENERGY ? b.io.atom_virial : nullptr, nullptr, b.lj.ps.acc, overwrite, b.lj.d_sc,
+                            nullptr);
 }
 
 __global__ void add_scalar_kernel(float* a, float va, float* b, float vb) {
@@ -646,7 +698,11 @@ void snb_scratch_free(PairScratch* ps) {
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered,
-              int overwrite, const float4* sc) {
+              int overwrite, const float4* sc, int inner) {
+    inner = inner && nl->prune_interval > 0;
+    const int* list_n = inner ? nl->list_in_n : nl->list_n;
+    const uint2* list = inner ? nl->list_in : nl->list;
+    int* counter = inner ? nl->flags + 10 : nullptr;
     int rc = scratch_ensure(ps, nl);
     if (rc) return rc;
     const int cap = ps->cap;
@@ -659,11 +715,16 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     const float rc2 = cutoff * cutoff;
     const size_t lds = (size_t)nt * nt * sizeof(float2);
     if (ps->ev_start) SNB_HIP(hipEventRecord(ps->ev_start, s));
-#define SNB_PAIR(E, S)                                                                                          \
-    pair_kernel<E, S><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, nl->list_n, nl->list, nl->list_cap, ps->pk, ab, \
+#define SNB_PAIR(E, S)                      \
+    do {                                    \
+        if (nt <= 4) SNB_PAIR_TB(E, S, 2);  \
+        else SNB_PAIR_TB(E, S, 6);          \
+    } while (0)
+#define SNB_PAIR_TB(E, S, TB)                                                                                   \
+    pair_kernel<E, S, TB><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, list_n, list, nl->list_cap, ps->pk, ab,   \
                                                          nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,         \
                                                          E ? atom_e : nullptr, E ? atom_v : nullptr,                    \
-                                                         E ? atom_d : nullptr, ps->acc, overwrite, sc)
+                                                         E ? atom_d : nullptr, ps->acc, overwrite, sc, counter)
     if (energy) {
         if (sc) SNB_PAIR(true, true);
         else SNB_PAIR(true, false);
@@ -672,6 +733,7 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
         else SNB_PAIR(false, false);
     }
 #undef SNB_PAIR
+#undef SNB_PAIR_TB
     SNB_CHECK_LAUNCH();
     if (ps->ev_stop) SNB_HIP(hipEventRecord(ps->ev_stop, s));
     return SPONGE_NB_OK;
@@ -781,8 +843,15 @@ int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s) {
 int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s) {
     const int nmap = b->nblk[kStPair];
     const int grid = xcd_grid(nmap);
-#define SNB_PAIRB(E, S) \
-    pair_batch<E, S><<<grid, kPairWaves * 64, b->lds_pair, s>>>(b->d, b->map[kStPair], nmap, overwrite)
+#define SNB_PAIRB(E, S)                                                                                          \
+    do {                                                                                                         \
+        if (b->max_nt <= 4)                                                                                      \
+            pair_batch<E, S, 2><<<grid, kPairWaves * 64, b->lds_pair, s>>>(b->d, b->map[kStPair], nmap,          \
+                                                                          overwrite);                           \
+        else                                                                                                     \
+            pair_batch<E, S, 6><<<grid, kPairWaves * 64, b->lds_pair, s>>>(b->d, b->map[kStPair], nmap,          \
+                                                                          overwrite);                           \
+    } while (0)
     if (energy) {
         if (b->soft) SNB_PAIRB(true, true);
         else SNB_PAIRB(true, false);
@@ -802,3 +871,15 @@ int snb_batch_excluded(sponge_batch* b, hipStream_t s) {
 }
 
 int snb_lj_scratch(sponge_lj* lj, const sponge_nl* nl) { return scratch_ensure(&lj->ps, nl); }
+
+// Prunes the fused step's inner list when its step counter says so (see
+// prune_kernel); pk holds the step's coordinates.
+int snb_prune(sponge_nl* nl, const uint4* pk, const float* cof, hipStream_t s) {
+    if (nl->prune_interval <= 0) return SPONGE_NB_OK;
+    const int nblk = xcd_grid(ceil_div(nl->ncl_cap, kPairWaves));
+    prune_kernel<<<nblk, kPairWaves * 64, 0, s>>>(nl->flags, nl->prune_interval, nl->list_n, nl->list, nl->list_cap,
+                                                  pk, cof[0], cof[1], cof[2], nl->prune_r2, nl->list_in_n,
+                                                  nl->list_in);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}

@@ -40,6 +40,7 @@ _SIGS = {
     "sponge_nl_destroy": (None, [_P]),
     "sponge_nl_update": (_I, [_P, _P, _P, _P, _P, _I, _I, _P]),
     "sponge_nl_update_volume": (_I, [_P, _FP3]),
+    "sponge_nl_set_prune": (_I, [_P, _I, _F]),
     "sponge_nl_status": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "sponge_nl_get_pairs": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
     "sponge_lj_create": (_I, [_I, _I, _P, _P, _P, _F, _FP3, ctypes.POINTER(_P)]),
@@ -155,6 +156,11 @@ class NEIGHBOR_LIST:
                                                _ptr(excluded_list), _ptr(excluded_numbers),
                                                int(forced_update), int(forced_check), _stream(stream)),
                "Neighbor_List_Update")
+
+    def set_prune(self, interval, radius=0.0):
+        """Inner list of the fused step: re-pruned every `interval` fused
+        steps to `radius` (>= cutoff); interval 0 turns it off."""
+        _check(load_library().sponge_nl_set_prune(self.h, int(interval), float(radius)), "sponge_nl_set_prune")
 
     def Update_Volume(self, box_length):
         _check(load_library().sponge_nl_update_volume(self.h, _vec3(box_length)), "Update_Volume")

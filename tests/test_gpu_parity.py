@@ -225,3 +225,55 @@ def test_get_energy_queries(case):
     assert abs(P.Get_Energy(*args, which_part=P.TOTAL) - tot) < ENERGY_TOL * scale
     e_lj = eng.lj.Get_Energy(eng.nl, eng.uint_crd)
     assert rel(e_lj, ref["e_lj"] + ref["e_lrc"]) < ENERGY_TOL
+
+
+def test_graph_replay_matches_eager(case):
+    """The fused step (with its side-stream fork / join) and the forced list
+    rebuild captured into one hipGraph replay to the eager forces, with eager
+    steps between the replays."""
+    import torch
+    name, sys_, eng, ref = case
+    eng.force_step(0, 0, from_crd=True, overwrite=True)
+    eng.update_neighbor_list(forced=1)
+    eng.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+    eager = eng.frc.cpu().numpy().copy()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        cur = torch.cuda.current_stream()
+        eng.update_neighbor_list(forced=1, stream=cur)
+        eng.force_step(0, 0, from_crd=True, overwrite=True, stream=cur)
+    for r in range(3):
+        eng.frc.fill_(float("nan"))
+        g.replay()
+        torch.cuda.synchronize()
+        assert rel_rms(eng.frc.cpu().numpy(), eager) < 1e-6, r
+        assert rel_rms(eng.frc.cpu().numpy(), ref["frc"]) < FORCE_TOL, r
+        eng.force_step(0, 0, from_crd=True, overwrite=True)
+        torch.cuda.synchronize()
+
+
+def test_pruned_inner_list(case):
+    """Interval mode prunes the fused step's list to an inner radius (default
+    rc + skin / 4, every refresh / 4 steps); forces stay those of the full list,
+    also when pruned to exactly rc every step."""
+    import torch
+    from sponge_amd import nb
+    name, sys_, eng, ref = case
+    e2 = nb.NonbondedEngine(sys_, refresh_interval=20)
+    for interval, radius in ((None, None), (1, sys_.cutoff)):
+        if interval is not None:
+            e2.nl.set_prune(interval, radius)
+        e2.update_neighbor_list(forced=1)
+        for _ in range(3):
+            e2.force_step(0, 0, from_crd=True, overwrite=True)
+        torch.cuda.synchronize()
+        f = e2.frc.cpu().numpy()
+        assert rel_rms(f, ref["frc"]) < FORCE_TOL, (interval, radius)
+        e2.force_step(1, 1, from_crd=True, overwrite=True)
+        p = e2.energy_parts()
+        assert p["pairs_in"] == ref["n_in"], (interval, radius)
+    e2.nl.set_prune(0)
+    e2.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+    assert rel_rms(e2.frc.cpu().numpy(), ref["frc"]) < FORCE_TOL
