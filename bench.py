@@ -26,6 +26,11 @@ if ROOT not in sys.path:
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 (vector) peak
 PEAK_HBM_GBS = 8000.0
 PAIR_FLOPS = 50              # per in-cutoff pair, force-only variant
+# HBM bytes per launch of the force-only pair kernel on the 23,570-atom box:
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same launch, each scaled by
+# what the counter reads for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0);
+# the runs and their commands are kept under profiles/.
+PAIR_TRAFFIC_BYTES = 12.64e6
 
 
 def cpu_baseline(sys_, seconds: float, beta, K):
@@ -316,7 +321,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": PAIR_TRAFFIC_BYTES if args.config == 1 else None,
+            "traffic_unit": "bytes per launch",
             "kernel_ms": round(pair_ms, 6),
             "flops_per_launch": flops,
         },

@@ -40,6 +40,9 @@ constexpr int kSpreadPerThread = 4;
 constexpr float kSpreadScale = 16777216.0f;
 constexpr int kConvThreads = 256;
 constexpr int kConvChunk = 16;
+// LDS pitch of one x position in the convolution kernel: odd, so that the
+// x-FFT's element stride (the pitch) spreads a wave's accesses over the banks
+constexpr int kConvPitch = kConvChunk + 1;
 constexpr int kGatherThreads = 256;
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -260,7 +263,10 @@ __device__ __forceinline__ void fwd_plane_body(
     extern __shared__ float2 sm[];
     const float unit = MODE ? scale_ptr[0] : kSpreadScale;
     const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
-    const int plane = max(Ky * H, nrp * Kz);
+    // packed z-lines sit at an odd pitch, so that the lines a wave transforms
+    // together fall on different LDS banks
+    const int zp = Kz + 1;
+    const int plane = max(Ky * H, nrp * zp);
     float2* tz = sm + max(plane, Ky * Kz);  // past the fixed-point grid too
     float2* ty = tz + Kz;
     load_twiddles<kPlaneThreads>(tz, twz, Kz);
@@ -376,13 +382,13 @@ __device__ __forceinline__ void fwd_plane_body(
         const int o = threadIdx.x + s * kPlaneThreads;
         if (o < npt) {
             const int y = o / Kz, z = o - y * Kz;
-            smf[2 * ((y >> 1) * Kz + z) + (y & 1)] = qv[s];
+            smf[2 * ((y >> 1) * zp + z) + (y & 1)] = qv[s];
         }
     }
     if (Ky & 1)
-        for (int z = threadIdx.x; z < Kz; z += kPlaneThreads) smf[2 * ((Ky >> 1) * Kz + z) + 1] = 0.f;
+        for (int z = threadIdx.x; z < Kz; z += kPlaneThreads) smf[2 * ((Ky >> 1) * zp + z) + 1] = 0.f;
     __syncthreads();
-    lds_fft<-1, kPlaneThreads>(sm, nrp, Kz, 1, pz, tz);
+    lds_fft<-1, kPlaneThreads>(sm, nrp, zp, 1, pz, tz);
     __syncthreads();
     // Unpack the paired transforms Z = FFT(a + i b): A_k = (Z_k + conj Z_-k)/2,
     // B_k = (Z_k - conj Z_-k)/(2i), into rows y of the half spectrum.
@@ -393,8 +399,8 @@ __device__ __forceinline__ void fwd_plane_body(
         if (o < Ky * H) {
             int y = o / H, k = o - y * H;
             int p = y >> 1;
-            float2 zk = sm[p * Kz + k];
-            float2 zm = sm[p * Kz + (k == 0 ? 0 : Kz - k)];
+            float2 zk = sm[p * zp + k];
+            float2 zm = sm[p * zp + (k == 0 ? 0 : Kz - k)];
             if ((y & 1) == 0) st[s] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
             else st[s] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
         }
@@ -449,7 +455,7 @@ __device__ __forceinline__ void xconv_body(int y, int chunk, float2* __restrict_
                                            int Kx, int Ky, int Kz, const FftPlan& px,
                                            const float2* __restrict__ twx, int energy, double* __restrict__ acc,
                                            int* __restrict__ bin_count) {
-    __shared__ float2 sm[256 * kConvChunk];
+    __shared__ float2 sm[256 * kConvPitch];
     __shared__ float2 tx[256];
     __shared__ double red[kConvThreads / 64];
     const int H = Kz / 2 + 1;
@@ -461,29 +467,29 @@ __device__ __forceinline__ void xconv_body(int y, int chunk, float2* __restrict_
     load_twiddles<kConvThreads>(tx, twx, Kx);
     for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
         int xx = idx / cw, c = idx - xx * cw;
-        sm[xx * kConvChunk + c] = FQ[((size_t)xx * Ky + y) * H + kz0 + c];
+        sm[xx * kConvPitch + c] = FQ[((size_t)xx * Ky + y) * H + kz0 + c];
     }
     __syncthreads();
-    lds_fft<-1, kConvThreads>(sm, cw, 1, kConvChunk, px, tx);
+    lds_fft<-1, kConvThreads>(sm, cw, 1, kConvPitch, px, tx);
     __syncthreads();
     double e = 0.0;
     for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
         int xx = idx / cw, c = idx - xx * cw;
         int kz = kz0 + c;
         float b = bc[((size_t)xx * Ky + y) * H + kz];
-        float2 v = sm[xx * kConvChunk + c];
+        float2 v = sm[xx * kConvPitch + c];
         if (energy) {
             double m = (kz == 0 || (2 * kz == Kz)) ? 1.0 : 2.0;
             e += m * (double)b * ((double)v.x * v.x + (double)v.y * v.y);
         }
-        sm[xx * kConvChunk + c] = make_float2(v.x * b, v.y * b);
+        sm[xx * kConvPitch + c] = make_float2(v.x * b, v.y * b);
     }
     __syncthreads();
-    lds_fft<1, kConvThreads>(sm, cw, 1, kConvChunk, px, tx);
+    lds_fft<1, kConvThreads>(sm, cw, 1, kConvPitch, px, tx);
     __syncthreads();
     for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
         int xx = idx / cw, c = idx - xx * cw;
-        FQ[((size_t)xx * Ky + y) * H + kz0 + c] = sm[xx * kConvChunk + c];
+        FQ[((size_t)xx * Ky + y) * H + kz0 + c] = sm[xx * kConvPitch + c];
     }
     if (energy) {
         double t = block_sum<kConvThreads>(e, red);
@@ -517,7 +523,8 @@ __device__ __forceinline__ void inv_plane_body(int x, const float2* __restrict__
                                                const float2* __restrict__ twz, const float2* __restrict__ twy) {
     extern __shared__ float2 sm[];
     const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
-    const int plane = max(Ky * H, nrp * Kz);
+    const int zp = Kz + 1;  // odd pitch of the packed z-lines
+    const int plane = max(Ky * H, nrp * zp);
     float2* tz = sm + plane;
     float2* ty = tz + Kz;
     load_twiddles<kPlaneThreads>(tz, twz, Kz);
@@ -556,15 +563,18 @@ __device__ __forceinline__ void inv_plane_body(int x, const float2* __restrict__
 #pragma unroll
     for (int s = 0; s < MAXB; s++) {
         int o = threadIdx.x + s * kPlaneThreads;
-        if (o < nrp * Kz) sm[o] = st[s];
+        if (o < nrp * Kz) {
+            const int p = o / Kz;
+            sm[o + p] = st[s];  // line p at pitch zp = Kz + 1
+        }
     }
     __syncthreads();
-    lds_fft<1, kPlaneThreads>(sm, nrp, Kz, 1, pz, tz);
+    lds_fft<1, kPlaneThreads>(sm, nrp, zp, 1, pz, tz);
     __syncthreads();
     float* Rp = R + (size_t)x * Ky * Kz;
     for (int idx = threadIdx.x; idx < nrp * Kz; idx += kPlaneThreads) {
         int p = idx / Kz, z = idx - p * Kz;
-        float2 v = sm[idx];
+        float2 v = sm[p * zp + z];
         Rp[(2 * p) * Kz + z] = v.x;
         if (2 * p + 1 < Ky) Rp[(2 * p + 1) * Kz + z] = v.y;
     }
@@ -782,7 +792,7 @@ int plan_axis(int K, FftPlan* p) {
 // y twiddles.
 size_t fwd_plane_lds(int Ky, int Kz) {
     const long H = Kz / 2 + 1;
-    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
+    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
     return (size_t)(std::max(plane, (long)Ky * Kz) + Ky + Kz) * sizeof(float2);
 }
 
@@ -887,7 +897,7 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
         }
     }
     const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
-    long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
+    long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
     int dev = 0, max_lds = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
@@ -1010,7 +1020,7 @@ int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const fl
         pme_prep_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_uint_crd, d_charge, pme_prep_args(pme), energy);
         SNB_CHECK_LAUNCH();
     }
-    long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
+    long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
     int mb = pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
     size_t lds = (plane + Ky + Kz) * sizeof(float2);
     size_t lds_fwd = fwd_plane_lds(Ky, Kz);
@@ -1089,7 +1099,7 @@ void snb_pme_plans(const sponge_pme* pme, FftPlan* px, FftPlan* py, FftPlan* pz)
 
 int snb_pme_plane_maxb(const sponge_pme* pme) {
     const int Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
-    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
+    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
     return pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
 }
 
@@ -1097,7 +1107,7 @@ size_t snb_pme_lds_fwd(const sponge_pme* pme) { return fwd_plane_lds(pme->K[1], 
 
 size_t snb_pme_lds_inv(const sponge_pme* pme) {
     const int Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
-    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * Kz);
+    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
     return (size_t)(plane + Ky + Kz) * sizeof(float2);
 }
 
