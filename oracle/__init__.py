@@ -51,6 +51,9 @@ def lib():
             "orc_lj_lrc_factor": (D, [I, P, I, P, D]),
             "orc_ewald_exact": (D, [I, P, P, P, D, D, I, P]),
             "orc_num_threads": (I, []),
+            "orc_settle_geometry": (None, [D, D, D, D, D, D, P]),
+            "orc_settle_groups": (None, [I, I, P, P, P, P, P, P, P, P, P]),
+            "orc_md_iteration": (D, [I, P, P, P, P, P, P, P, D, D, D, D, I, P, P, I, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -316,3 +319,66 @@ def full_reference(sys_, crd=None, charge=None, beta=None, K=None):
                 e_corr=x["e_corr"], e_rec=r["e_rec"], e_self=es + en,
                 e_lrc=lrc, atom_energy=d["atom_energy"], atom_virial=d["atom_virial"],
                 atom_direct=d["atom_direct"], atom_corr=x["atom_corr"], n_in=d["n_in"])
+
+
+# ---------------------------------------------------------------------------
+# Integrator and SETTLE (orc_settle_* / orc_md_iteration).
+
+def settle_geometry(rAB, rAC, rBC, mA, mB, mC):
+    g = np.zeros(5)
+    lib().orc_settle_geometry(float(rAB), float(rAC), float(rBC), float(mA), float(mB), float(mC), _p(g))
+    return g
+
+
+def settle_groups(n, pair_i, pair_j, pair_r, mass):
+    """SETTLE's rigid triangles / pairs found in a constraint-pair list."""
+    pi, pj = _c(pair_i, np.int32), _c(pair_j, np.int32)
+    pr, m = _c(pair_r, np.float32), _c(mass, np.float32)
+    k = len(pi)
+    tri = np.zeros((max(k, 1), 3), np.int32)
+    geo = np.zeros((max(k, 1), 5))
+    prs = np.zeros((max(k, 1), 2), np.int32)
+    plen = np.zeros(max(k, 1))
+    cnt = np.zeros(3, np.int32)
+    lib().orc_settle_groups(int(n), k, _p(pi), _p(pj), _p(pr), _p(m), _p(tri), _p(geo), _p(prs), _p(plen),
+                            _p(cnt))
+    nt, npr = int(cnt[0]), int(cnt[1])
+    return dict(tri=tri[:nt].copy(), geo=geo[:nt].copy(), pairs=prs[:npr].copy(), plen=plen[:npr].copy(),
+                rest=int(cnt[2]))
+
+
+def md_iteration(crd, vel, frc, mass, uint_crd, box, dt, groups, vmax=0.0, vfac=1.0, xfac=1.0):
+    """One leap-frog + SETTLE step in FP64; crd / vel (float64, (N,3)) are
+    updated in place. Returns the constraint virial sum."""
+    assert crd.dtype == np.float64 and vel.dtype == np.float64
+    assert crd.flags.c_contiguous and vel.flags.c_contiguous
+    m = _c(mass, np.float64)
+    im = np.where(m > 0, 1.0 / np.where(m > 0, m, 1.0), 0.0)
+    f = _c(frc, np.float64)
+    u = _c(uint_crd, np.uint32)
+    tri, geo = _c(groups["tri"], np.int32), _c(groups["geo"], np.float64)
+    prs, plen = _c(groups["pairs"], np.int32), _c(groups["plen"], np.float64)
+    return lib().orc_md_iteration(len(m), _p(crd), _p(vel), _p(f), _p(im), _p(m), _p(u), _p(_c(box, np.float32)),
+                                  float(dt), float(vmax), float(vfac), float(xfac), len(tri), _p(tri), _p(geo),
+                                  len(prs), _p(prs), _p(plen))
+
+
+def md_run(sys_, vel0, nsteps, dt, groups, crd0=None, beta=None, K=None, record=None):
+    """NVE trajectory of main.cu's loop (force, then leap-frog + SETTLE) in
+    FP64 state, forces from full_reference at the FP32-rounded coordinates.
+    Returns (crd, vel, log) with per-step potential and kinetic energies."""
+    crd = np.array(sys_.crd if crd0 is None else crd0, np.float64)
+    vel = np.array(vel0, np.float64)
+    m = sys_.mass.astype(np.float64)
+    log = []
+    for step in range(nsteps):
+        ref = full_reference(sys_, crd=crd.astype(np.float32), beta=beta, K=K)
+        beta, K = ref["beta"], ref["K"]
+        pot = ref["e_lj"] + ref["e_lrc"] + ref["e_direct"] + ref["e_corr"] + ref["e_rec"] + ref["e_self"]
+        ke_before = 0.5 * float((m[:, None] * vel * vel).sum())
+        md_iteration(crd, vel, ref["frc"], m, ref["uint"], sys_.box, dt, groups)
+        ke_after = 0.5 * float((m[:, None] * vel * vel).sum())
+        log.append(dict(potential=pot, ke_half_before=ke_before, ke_half_after=ke_after))
+        if record is not None:
+            record(step, crd, vel)
+    return crd, vel, log

@@ -994,6 +994,210 @@ double orc_ewald_exact(int n, const double* xyz, const double* q, const float* b
     return E;
 }
 
+// ---------------------------------------------------------------------------
+// Integrator and SETTLE (next to the force path).
+
+// Get_Rabcde_From_SSS (constrain/SETTLE.cu:8-47) in FP64: a rigid triangle
+// with sides |AB|, |AC|, |BC| placed with its centre of mass at the origin,
+// A at (0, ra), B at (rc, rb), C at (rd, re).
+void orc_settle_geometry(double rAB, double rAC, double rBC, double mA, double mB, double mC, double* g) {
+    double mT = mA + mB + mC;
+    double Ax = 0, Ay = 0, Bx = -rAB, By = 0;
+    double c = (rBC * rBC - rAC * rAC - rAB * rAB) / (2 * rAC * rAB);
+    double Cx = rAC * c, Cy = rAC * std::sqrt(1.0 - c * c);
+    double Ox = (Bx * mB + Cx * mC) / mT, Oy = Cy * mC / mT;
+    Ax -= Ox; Ay -= Oy; Bx -= Ox; By -= Oy; Cx -= Ox; Cy -= Oy;
+    c = 1.0 / std::sqrt(1.0 + Ax * Ax / Ay / Ay);
+    double s = c * Ax / Ay;
+    double ra = Ax * s + Ay * c;
+    double rc = Bx * c - By * s, rb = Bx * s + By * c;
+    double rd = Cx * c - Cy * s, re = Cx * s + Cy * c;
+    if (ra < 0) { ra = -ra; rb = -rb; re = -re; }
+    g[0] = ra; g[1] = rb; g[2] = rc; g[3] = rd; g[4] = re;
+}
+
+// SETTLE::Initial group discovery (constrain/SETTLE.cu:209-331, pairs' k at
+// :325, leftover count :339). counts = {triangles, pairs, leftover pairs};
+// tri = (A, B, C) per triangle with geo = its 5 canonical coordinates;
+// prs = (i, j) per pair with plen = its length.
+void orc_settle_groups(int n, int np, const int* ai, const int* aj, const float* r, const float* mass, int* tri,
+                       double* geo, int* prs, double* plen, int* counts) {
+    std::vector<int> ln(n, 0), la(2 * (size_t)n, -1);
+    std::vector<double> lr(3 * (size_t)n, 0.0);
+    for (int k = 0; k < np; k++) {
+        int i = ai[k], j = aj[k];
+        if (ln[i] < 2 && ln[j] < 2) {
+            la[2 * i + ln[i]] = j; la[2 * j + ln[j]] = i;
+            lr[3 * i + ln[i]] = r[k]; lr[3 * j + ln[j]] = r[k];
+            ln[i]++; ln[j]++;
+        } else {
+            ln[i] = 3; ln[j] = 3;
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        if (ln[i] == 2) {
+            int a = la[2 * i], b = la[2 * i + 1];
+            if (ln[a] == 2 && ln[b] == 2 &&
+                ((la[2 * a] == i && la[2 * a + 1] == b) || (la[2 * a + 1] == i && la[2 * a] == b))) {
+                ln[a] = -2; ln[b] = -2;
+                lr[3 * i + 2] = (la[2 * a + 1] == b) ? lr[3 * a + 1] : lr[3 * a];
+            } else {
+                ln[i] = 3; ln[a] = 3; ln[b] = 3;
+            }
+        } else if (ln[i] == 1) {
+            int a = la[2 * i];
+            if (ln[a] == 1) ln[a] = -1;
+            else { ln[i] = 3; ln[a] = 3; }
+        }
+    }
+    int nt = 0, npr = 0, rest = 0;
+    for (int i = 0; i < n; i++) {
+        if (ln[i] == 2) {
+            ln[i] = -2;
+            tri[3 * nt] = i; tri[3 * nt + 1] = la[2 * i]; tri[3 * nt + 2] = la[2 * i + 1];
+            orc_settle_geometry(lr[3 * i], lr[3 * i + 1], lr[3 * i + 2], mass[i], mass[la[2 * i]],
+                                mass[la[2 * i + 1]], geo + 5 * nt);
+            nt++;
+        } else if (ln[i] == 1) {
+            ln[i] = -1;
+            prs[2 * npr] = i; prs[2 * npr + 1] = la[2 * i]; plen[npr] = lr[3 * i];
+            npr++;
+        }
+    }
+    for (int k = 0; k < np; k++) if (ln[ai[k]] > 0) rest++;
+    counts[0] = nt; counts[1] = npr; counts[2] = rest;
+}
+
+static inline void v_sub(const double* a, const double* b, double* o) { for (int d = 0; d < 3; d++) o[d] = a[d] - b[d]; }
+static inline double v_dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void v_cross(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline void v_unit(double* a) { double s = 1.0 / std::sqrt(v_dot(a, a)); for (int d = 0; d < 3; d++) a[d] *= s; }
+// Get_Periodic_Displacement of float coordinates (common.cu:99-107)
+static inline void v_pbc(const double* a, const double* b, const float* L, double* o) {
+    for (int d = 0; d < 3; d++) { double x = a[d] - b[d]; o[d] = x - std::floor(x / L[d] + 0.5) * L[d]; }
+}
+
+// SETTLE_DO_TRIANGLE (constrain/SETTLE.cu:100-188) in FP64. rB0, rC0: old
+// B-A, C-A; rA1..rC1: new positions relative to the new centre of mass,
+// replaced by the constrained ones; v: the three velocities, corrected.
+// Returns the triangle's virial F_B . rB0 + F_C . rC0.
+static double settle_do_triangle(const double* rB0, const double* rC0, double* rA1, double* rB1, double* rC1,
+                                 const double* g, double mB, double mC, double dt, double vfac, double xfac,
+                                 double* vA, double* vB, double* vC) {
+    double ra = g[0], rb = g[1], rc = g[2], rd = g[3], re = g[4];
+    double ez[3], ex[3], ey[3];
+    v_cross(rB0, rC0, ez);
+    v_cross(rA1, ez, ex);
+    v_cross(ez, ex, ey);
+    v_unit(ex); v_unit(ey); v_unit(ez);
+    double B0x = v_dot(ex, rB0), B0y = v_dot(ey, rB0), C0x = v_dot(ex, rC0), C0y = v_dot(ey, rC0);
+    double A1z = v_dot(ez, rA1);
+    double B1x = v_dot(ex, rB1), B1y = v_dot(ey, rB1), B1z = v_dot(ez, rB1);
+    double C1x = v_dot(ex, rC1), C1y = v_dot(ey, rC1), C1z = v_dot(ez, rC1);
+    double sinphi = A1z / ra, cosphi = std::sqrt(1.0 - sinphi * sinphi);
+    double sinpsi = (B1z - C1z - (rb - re) * sinphi) / ((rd - rc) * cosphi);
+    double cospsi = std::sqrt(1.0 - sinpsi * sinpsi);
+    double A2y = ra * cosphi;
+    double B2x = rc * cospsi, B2y = rb * cosphi + rc * sinpsi * sinphi;
+    double C2x = rd * cospsi, C2y = re * cosphi + rd * sinpsi * sinphi;
+    double alpha = B2x * B0x + C2x * C0x + B2y * B0y + C2y * C0y;
+    double beta = -B2x * B0y - C2x * C0y + B2y * B0x + C2y * C0x;
+    double gamma = B1y * B0x - B1x * B0y + C1y * C0x - C1x * C0y;
+    double t = alpha * alpha + beta * beta;
+    double sinth = (alpha * gamma - beta * std::sqrt(t - gamma * gamma)) / t;
+    double costh = std::sqrt(1.0 - sinth * sinth);
+    double A3[3] = {-A2y * sinth, A2y * costh, A1z};
+    double B3[3] = {B2x * costh - B2y * sinth, B2x * sinth + B2y * costh, B1z};
+    double C3[3] = {C2x * costh - C2y * sinth, C2x * sinth + C2y * costh, C1z};
+    double a3[3], b3[3], c3[3];
+    for (int d = 0; d < 3; d++) {
+        a3[d] = A3[0] * ex[d] + A3[1] * ey[d] + A3[2] * ez[d];
+        b3[d] = B3[0] * ex[d] + B3[1] * ey[d] + B3[2] * ez[d];
+        c3[d] = C3[0] * ex[d] + C3[1] * ey[d] + C3[2] * ez[d];
+    }
+    double kv = vfac / dt / xfac, kf = 1.0 / dt / dt / xfac, vir = 0;
+    for (int d = 0; d < 3; d++) {
+        vA[d] += kv * (a3[d] - rA1[d]);
+        vB[d] += kv * (b3[d] - rB1[d]);
+        vC[d] += kv * (c3[d] - rC1[d]);
+        vir += kf * mB * (b3[d] - rB1[d]) * rB0[d] + kf * mC * (c3[d] - rC1[d]) * rC0[d];
+        rA1[d] = a3[d]; rB1[d] = b3[d]; rC1[d] = c3[d];
+    }
+    return vir;
+}
+
+// One NVE step of main.cu:282-320 in FP64: SETTLE::Remember_Last_Coordinates
+// from the step's uint coordinates (SETTLE.cu:385-410, scaler L / 2^32),
+// MD_Iteration_Leap_Frog(_With_Max_Velocity) (MD_core.cu:13-45) on every
+// atom, then settle_pair / settle_triangle (SETTLE.cu:412-483) with
+// Get_Periodic_Displacement of the float coordinates. crd / vel updated in
+// place; returns the constraint virial sum (before the 1/(3V) factor).
+double orc_md_iteration(int n, double* crd, double* vel, const double* frc, const double* inv_mass,
+                        const double* mass, const uint32_t* u, const float* box, double dt, double vmax, double vfac,
+                        double xfac, int ntri, const int* tri, const double* geo, int npair, const int* prs,
+                        const double* plen) {
+    double sc[3];
+    for (int d = 0; d < 3; d++) sc[d] = (double)box[d] / 4294967296.0;
+    auto udisp = [&](int a, int b, double* o) {
+        for (int d = 0; d < 3; d++) o[d] = (double)(int32_t)(u[3 * a + d] - u[3 * b + d]) * sc[d];
+    };
+    std::vector<double> last(6 * (size_t)ntri + 3 * (size_t)npair);
+    for (int k = 0; k < ntri; k++) {
+        udisp(tri[3 * k + 1], tri[3 * k], &last[6 * k]);
+        udisp(tri[3 * k + 2], tri[3 * k], &last[6 * k + 3]);
+    }
+    for (int k = 0; k < npair; k++) udisp(prs[2 * k + 1], prs[2 * k], &last[6 * (size_t)ntri + 3 * k]);
+    for (int i = 0; i < n; i++) {
+        double v[3];
+        for (int d = 0; d < 3; d++) v[d] = vel[3 * i + d] + dt * inv_mass[i] * frc[3 * i + d];
+        if (vmax > 0) {
+            double s = std::min(1.0, vmax / std::sqrt(v_dot(v, v)));
+            for (int d = 0; d < 3; d++) v[d] *= s;
+        }
+        for (int d = 0; d < 3; d++) { vel[3 * i + d] = v[d]; crd[3 * i + d] += dt * v[d]; }
+    }
+    double vir = 0;
+    for (int k = 0; k < npair; k++) {
+        int i = prs[2 * k], j = prs[2 * k + 1];
+        double r1[3], *r2 = &last[6 * (size_t)ntri + 3 * k];
+        v_pbc(&crd[3 * j], &crd[3 * i], box, r1);
+        double mA = mass[i], mB = mass[j], r0r0 = plen[k] * plen[k];
+        double r1r1 = v_dot(r1, r1), r1r2 = v_dot(r1, r2), r2r2 = v_dot(r2, r2);
+        double kk = (std::sqrt(r1r2 * r1r2 - r1r1 * r2r2 + r2r2 * r0r0) - r1r2) / r2r2;
+        double ck = 1.0 / (mA + mB);
+        for (int d = 0; d < 3; d++) {
+            double di = -mB * ck * kk * r2[d], dj = mA * ck * kk * r2[d];
+            crd[3 * i + d] += di; crd[3 * j + d] += dj;
+            vel[3 * i + d] += vfac / dt / xfac * di; vel[3 * j + d] += vfac / dt / xfac * dj;
+            vir += mB / dt / dt / xfac * dj * r2[d];
+        }
+    }
+    for (int k = 0; k < ntri; k++) {
+        int a = tri[3 * k], b = tri[3 * k + 1], c = tri[3 * k + 2];
+        double rA[3], rB[3], rC[3], rO[3];
+        for (int d = 0; d < 3; d++) rA[d] = crd[3 * a + d];
+        v_pbc(&crd[3 * b], rA, box, rB);
+        v_pbc(&crd[3 * c], rA, box, rC);
+        double mA = mass[a], mB = mass[b], mC = mass[c];
+        for (int d = 0; d < 3; d++) {
+            rO[d] = (mB * rB[d] + mC * rC[d]) / (mA + mB + mC) + rA[d];
+            rA[d] -= rO[d];
+            rB[d] += rA[d];
+            rC[d] += rA[d];
+        }
+        vir += settle_do_triangle(&last[6 * k], &last[6 * k + 3], rA, rB, rC, geo + 5 * k, mB, mC, dt, vfac, xfac,
+                                  &vel[3 * a], &vel[3 * b], &vel[3 * c]);
+        for (int d = 0; d < 3; d++) {
+            crd[3 * a + d] = rA[d] + rO[d];
+            crd[3 * b + d] = rB[d] + rO[d];
+            crd[3 * c + d] = rC[d] + rO[d];
+        }
+    }
+    return vir;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

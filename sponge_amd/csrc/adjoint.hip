@@ -169,7 +169,6 @@ __global__ __launch_bounds__(kAdjWaves * 64) void adj_pair_kernel(
             const float r2s = in ? r2 : 1.0f;
             const float rinv = __builtin_amdgcn_rsqf(r2s);
             const float rinv2 = rinv * rinv;
-            const float rinv6 = rinv2 * rinv2 * rinv2;
             const float x = beta * (r2s * rinv);
             const float ex = __builtin_amdgcn_exp2f(mlog2e_b2 * r2s);
             const float ecx = erfcx_poly(x);

@@ -70,6 +70,16 @@ _SIGS = {
     "sponge_batch_sync": (_I, [_P]),
     "sponge_batch_neighbor_list_update": (_I, [_P, _I, _P]),
     "sponge_batch_force_step": (_I, [_P, _I, _I, _I, _P]),
+    # include/sponge_md.h
+    "sponge_md_leap_frog": (_I, [_I, _P, _P, _P, _P, _P, _F, _F, _P]),
+    "sponge_settle_create": (_I, [_I, _I, _P, _P, _P, _P, _F, _F, _I, ctypes.POINTER(_P)]),
+    "sponge_settle_destroy": (None, [_P]),
+    "sponge_settle_counts": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "sponge_settle_groups": (_I, [_P, _P, _P, _P, _P]),
+    "sponge_settle_set_factors": (_I, [_P, _F, _F]),
+    "sponge_settle_remember_last_coordinates": (_I, [_P, _P, _FP3, _P]),
+    "sponge_settle_do": (_I, [_P, _P, _P, _FP3, _P, _I, _P, _P]),
+    "sponge_md_iteration": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _FP3, _F, _F, _I, _P, _P]),
 }
 
 

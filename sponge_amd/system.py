@@ -20,6 +20,8 @@ import numpy as np
 CHARGE_SCALE = 18.2223          # sqrt(kcal*A/mol) per elementary charge
 TIP3P_ROH = 0.9572
 TIP3P_HOH_DEG = 104.52
+TIME_UNIT_PER_PS = 20.455       # SPONGE's internal time unit is 1/20.455 ps (common.cuh)
+KB = 0.00198716                 # kcal/mol/K
 
 # (name, charge e, sigma A, epsilon kcal/mol)
 SPECIES = {
@@ -361,6 +363,48 @@ def small_random(n: int, box, seed: int = 7, cutoff: float = 4.0,
 # ---------------------------------------------------------------------------
 # SPONGE host-side conventions restated (pure arithmetic, used by host code)
 # ---------------------------------------------------------------------------
+
+def water_constraints(sys_: System):
+    """Constraint pairs of the rigid waters, in the order SPONGE's CONSTRAIN
+    module lists a water's bonds and its H-O-H angle: (O, H1), (O, H2),
+    (H1, H2); lengths 0.9572 and 2 * 0.9572 * sin(104.52 / 2) Angstrom."""
+    nw = sys_.waters
+    o = 3 * np.arange(nw)
+    r_hh = 2.0 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH_DEG) / 2)
+    pi = np.stack([o, o, o + 1], 1).reshape(-1).astype(np.int32)
+    pj = np.stack([o + 1, o + 2, o + 2], 1).reshape(-1).astype(np.int32)
+    pr = np.tile(np.array([TIP3P_ROH, TIP3P_ROH, r_hh], np.float32), nw)
+    return pi, pj, pr
+
+
+def rigid_velocities(sys_: System, temperature: float = 300.0, seed: int = 5) -> np.ndarray:
+    """Maxwell-Boltzmann velocities (Angstrom per internal time unit) with
+    every water's velocities projected onto rigid-body motion (centre of
+    mass velocity + rotation), and the total momentum removed."""
+    rng = np.random.default_rng(seed)
+    m = sys_.mass.astype(np.float64)
+    v = rng.normal(size=(sys_.n, 3)) * np.sqrt(KB * temperature / m)[:, None]
+    nw = sys_.waters
+    if nw:
+        x = sys_.crd[:3 * nw].astype(np.float64).reshape(nw, 3, 3)
+        # unwrap each water around its oxygen
+        L = sys_.box.astype(np.float64)
+        x = x[:, :1] + (x - x[:, :1]) - L * np.round((x - x[:, :1]) / L)
+        mw = m[:3 * nw].reshape(nw, 3)
+        vw = v[:3 * nw].reshape(nw, 3, 3)
+        M = mw.sum(1)
+        com = (mw[..., None] * x).sum(1) / M[:, None]
+        V = (mw[..., None] * vw).sum(1) / M[:, None]
+        r = x - com[:, None]
+        Lm = (mw[..., None] * np.cross(r, vw - V[:, None])).sum(1)
+        r2 = (r * r).sum(2)
+        inertia = (mw * r2).sum(1)[:, None, None] * np.eye(3) - np.einsum("wa,wai,waj->wij", mw, r, r)
+        omega = np.linalg.solve(inertia, Lm[..., None])[..., 0]
+        vw = V[:, None] + np.cross(omega[:, None], r)
+        v[:3 * nw] = vw.reshape(-1, 3)
+    v -= (m[:, None] * v).sum(0) / m.sum()
+    return v.astype(np.float32)
+
 
 def check_2357(n: int) -> bool:
     """common.cu:467-511 Check_2357_Factor."""
