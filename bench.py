@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--refresh", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
+    ap.add_argument("--serial", action="store_true",
+                    help="run the reciprocal pipeline after the pair kernel on one stream")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--adjoint-reps", type=int, default=50,
                     help="launches per timing of the gradient calls (0 skips them)")
@@ -231,6 +233,7 @@ def main():
 
     sys_ = system.config(args.config)
     mde = md.MDEngine(sys_, refresh_interval=args.refresh, temperature=300.0, seed=5)
+    mde.serial = args.serial
     eng = mde.nb
     stream = torch.cuda.current_stream()
 
@@ -310,7 +313,8 @@ def main():
         "ms_per_step": round(ms, 6),
         "eager_ms_per_step": round(eager_ms, 6),
         "graph_ms_per_step": None if graph_ms is None else round(graph_ms, 6),
-        "timed_as": "eager" if ms == eager_ms else "graph",
+        "timed_as": ("eager" if ms == eager_ms else "graph") + (", one stream" if args.serial else
+                                                                 ", PME on a second stream"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

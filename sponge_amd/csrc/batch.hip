@@ -22,9 +22,9 @@ static int stage_blocks(int st, const sponge_nl* nl, const sponge_pme* pme) {
         case kStPrep: return ceil_div(nl->slot_cap, 256);
         case kStPair: return ceil_div(nl->ncl_cap, 4);
         case kStExcl: return ceil_div(n, 256);
-        case kStFwd: return pme->K[0];
-        case kStConv: return pme->K[1] * ceil_div(H, 16);
-        case kStInv: return pme->K[0];
+        case kStFwd: return pme->K[0] * snb_pme_row_blocks(pme);
+        case kStConv: return H;
+        case kStInv: return pme->K[0] * snb_pme_row_blocks(pme);
         case kStGather: return ceil_div(16 * n, 256);
         case kStFin: return 1;
         case kStNlCheck: return ceil_div(n, 256);
@@ -90,22 +90,20 @@ extern "C" int sponge_batch_create(int nsys, sponge_nl* const* nl, sponge_lj* co
     b->refresh_interval = nl[0]->refresh_interval;
     b->refresh_count = 0;
     b->max_nt = 0;
-    b->mb_plane = 1;
     b->soft = 0;
-    b->lds_fwd = b->lds_inv = 0;
+    b->lds_fwd = b->lds_conv = b->lds_inv = 0;
     int rc = SPONGE_NB_OK;
     for (int s = 0; s < nsys && rc == SPONGE_NB_OK; s++) {
         rc = snb_lj_scratch(lj[s], nl[s]);
         b->h[s].io = io[s];
         snb_pme_plans(pme[s], &b->h[s].px, &b->h[s].py, &b->h[s].pz);
         b->max_nt = std::max(b->max_nt, lj[s]->ntypes);
-        b->mb_plane = std::max(b->mb_plane, snb_pme_plane_maxb(pme[s]));
         b->soft |= lj[s]->d_sc != nullptr;
         b->lds_fwd = std::max(b->lds_fwd, snb_pme_lds_fwd(pme[s]));
+        b->lds_conv = std::max(b->lds_conv, snb_pme_lds_conv(pme[s]));
         b->lds_inv = std::max(b->lds_inv, snb_pme_lds_inv(pme[s]));
     }
     b->lds_pair = (size_t)b->max_nt * b->max_nt * sizeof(float2);
-    if (rc == SPONGE_NB_OK) rc = snb_batch_pme_attributes(b->mb_plane, b->lds_fwd);
     if (rc == SPONGE_NB_OK && hipMalloc((void**)&b->d, nsys * sizeof(BatchSys)) != hipSuccess) {
         set_error("batch_create: out of device memory");
         rc = SPONGE_NB_EHIP;

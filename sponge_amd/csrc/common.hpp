@@ -121,11 +121,14 @@ struct PairScratch {
     hipEvent_t ev_start, ev_stop;  // optional, recorded around the pair kernel
 };
 
-// Mixed-radix pass list of one FFT axis (radix 4 first, then 2, 3, 5, 7).
+// Mixed-radix pass list of one FFT axis (radix 4 first, then 2, 3, 5, 7),
+// radix of pass p in bits 4p..4p+3 of `code`: a scalar, so that kernels
+// never index an array with the pass number (which would put the plan in
+// scratch memory).
 struct FftPlan {
     int K;
     int npass;
-    int radix[12];
+    unsigned long long code;
 };
 
 struct sponge_nl {
@@ -196,11 +199,9 @@ struct sponge_pme {
     float cutoff, tolerance, beta;
     int K[3];
     int nfft;                    // K0*K1*(K2/2+1)
-    float* d_bc;                 // nfft
+    float* d_bc;                 // nfft, stored [kz][x][y] like FQ
     float2* d_tw[3];             // twiddles exp(-2 pi i j / K) per axis
-    int radix[3][16];
-    int npass[3];
-    float2* FQ;                  // nfft
+    float2* FQ;                  // nfft half spectrum, stored [kz][x][y]
     float* R;                    // K^3 convolved grid
     unsigned* base;              // n grid base indices, x | y << 10 | z << 20
     int* bin_count;              // K0 atoms filed under each x base plane
@@ -249,8 +250,8 @@ struct sponge_batch {
     BatchSys* d;
     int2* map[kNumStages];      // block -> (system, block within the system); x = -1 past the end
     int nblk[kNumStages];
-    int max_nt, mb_plane, soft;
-    size_t lds_pair, lds_fwd, lds_inv;
+    int max_nt, soft;
+    size_t lds_pair, lds_fwd, lds_conv, lds_inv;
     int refresh_interval, refresh_count;
 };
 
@@ -271,8 +272,9 @@ int snb_pme_finalize(sponge_pme* pme, const float* d_direct_atom_energy, int nee
                      float* d_potential, float* d_virial, hipStream_t s);
 int snb_fill_zero_f64(double* p, int n, hipStream_t s);
 void snb_pme_plans(const sponge_pme* pme, FftPlan* px, FftPlan* py, FftPlan* pz);
-int snb_pme_plane_maxb(const sponge_pme* pme);
+int snb_pme_row_blocks(const sponge_pme* pme);
 size_t snb_pme_lds_fwd(const sponge_pme* pme);
+size_t snb_pme_lds_conv(const sponge_pme* pme);
 size_t snb_pme_lds_inv(const sponge_pme* pme);
 float snb_nl_rlm(const sponge_nl* nl);
 int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s);
@@ -284,7 +286,6 @@ void snb_adj_free(sponge_lj* lj);
 int snb_batch_reciprocal(sponge_batch* b, int energy, hipStream_t s);
 int snb_batch_gather(sponge_batch* b, hipStream_t s);
 int snb_batch_finalize(sponge_batch* b, hipStream_t s);
-int snb_batch_pme_attributes(int mb, size_t lds_fwd);
 int snb_pme_prep(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, hipStream_t s);
 int snb_pme_adjoint_grid(sponge_pme* pme, const float4* wv, const float* scale, hipStream_t s);
 int snb_pme_adj_gather(sponge_pme* pme, int mode, const float4* wv, float sign, float* out, hipStream_t s);

@@ -2,23 +2,24 @@
 // PME_force/PME_force.cu), with a hand-written 3D real-to-complex FFT in
 // place of cuFFT.
 //
-// Pipeline per call (grid layout [x][y][z], z fastest, half spectrum
-// [kx][ky][kz <= Kz/2] exactly as the reference's PME_Q / PME_FQ / PME_BC):
+// Pipeline per call (real grid [x][y][z], z fastest, as the reference's
+// PME_Q; the half spectrum and the BC table are stored [kz][x][y], kz <= Kz/2,
+// so that one kz plane is contiguous):
 //   pme_prep      per atom: grid base index and spline fraction from the uint
 //                 coordinate with one 32x32->64-bit multiply (t = u K / 2^32)
-//   fwd_plane     one block per x plane: every atom whose order-4 stencil
-//                 touches the plane is spread into the plane in LDS
-//                 (ds_add_f32, no global atomics and no charge grid in HBM),
-//                 then real z-rows are packed two at a time into complex
-//                 z-FFTs and unpacked (R2C), then y-FFTs
-//   xconv         one block per (y, kz chunk): x-FFT, multiply by BC, energy
-//                 1/2 sum BC |F|^2 (FP64), inverse x-FFT
-//   inv_plane     one block per x plane: inverse y-FFTs, Hermitian repack and
+//   spread_z      one block per (x plane, 16 y rows): the stencil rows of every
+//                 atom filed under planes x..x+3 that fall in the block are
+//                 spread in LDS as 64-bit fixed-point sums (no global atomics
+//                 and no charge grid in HBM), then the real z-rows are paired
+//                 into complex z-FFTs and unpacked (R2C)
+//   conv2d        one block per kz: y- and x-FFT of the plane, energy
+//                 1/2 sum m BC |F|^2 (FP64), multiply by BC, inverse x- and
+//                 y-FFT
+//   inv_z         one block per (x plane, 16 y rows): Hermitian repack and
 //                 paired complex z-IFFT (C2R), write the convolved grid
 //   pme_gather    16 lanes per atom: force -q sum grid * dtheta (K/L)
-// All FFT passes are mixed-radix (2,3,4,5,7) Stockham steps over a whole
-// plane held in LDS, each step staged through registers so one LDS buffer
-// suffices.
+// All FFT passes are mixed-radix (2,3,4,5,7) Stockham steps over lines held
+// in LDS, each line owned by one wave for all of its passes.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -33,17 +34,27 @@ using namespace snb;
 
 namespace {
 
-constexpr int kPlaneThreads = 512;
-constexpr int kSpreadPerThread = 4;
+// y rows per block of the spread / z-transform kernels (even: rows pair into
+// complex z-lines)
+constexpr int kRowBlock = 16;
+constexpr int kZThreads = 256;
+constexpr int kSpreadPerThread = 8;
 // Fixed-point unit of the spread grid: one contribution q*Mx*My*Mz must stay
 // below 2^31 units, i.e. |q| < 128 / (2/3)^3 = 432 (18.2223 x 23.7 e).
 constexpr float kSpreadScale = 16777216.0f;
-constexpr int kConvThreads = 256;
-constexpr int kConvChunk = 16;
-// LDS pitch of one x position in the convolution kernel: odd, so that the
-// x-FFT's element stride (the pitch) spreads a wave's accesses over the banks
-constexpr int kConvPitch = kConvChunk + 1;
+// one block of the convolution kernel holds a whole Kx x Ky plane
+constexpr int kConvThreads = 512;
+// global loads in flight per thread in the convolution kernel (a 64 x 64
+// plane in one round trip), and spectrum
+// points per thread of the inverse z kernel (16 rows x (Kz/2 + 1), Kz <= 512)
+constexpr int kConvBatch = 8;
+constexpr int kZPer = 17;
 constexpr int kGatherThreads = 256;
+
+__host__ __device__ __forceinline__ int row_blocks(int Ky) { return (Ky + kRowBlock - 1) / kRowBlock; }
+// LDS pitch of one x row of the convolution plane: odd, so that the x-FFT's
+// element stride (the pitch) spreads a wave's accesses over the banks
+__host__ __device__ __forceinline__ int conv_pitch(int Ky) { return Ky | 1; }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -100,28 +111,31 @@ __device__ __forceinline__ void dft(float2* v) {
     }
 }
 
-// q = n / d for n * d < 2^32 via one multiply-high with m = ceil(2^32 / d);
-// d == 1 (m would be 2^32) is passed through.
-__device__ __forceinline__ unsigned div_magic(unsigned d) { return d > 1u ? 0xFFFFFFFFu / d + 1u : 0u; }
-__device__ __forceinline__ unsigned fast_div(unsigned n, unsigned d, unsigned m) { return d > 1u ? __umulhi(n, m) : n; }
+// n / d for 0 <= n < 2^21 from rd = 1.0f / d (d >= 1): (n + 1/2) / d lies at
+// least 1/(2d) away from an integer and the float product is within
+// 2^-22 (n + 1/2) / d of it, so the truncation is exact. A few VALU
+// operations instead of an integer division.
+__device__ __forceinline__ int div_small(int n, float rd) { return (int)(((float)n + 0.5f) * rd); }
 
 // One Stockham radix-R step over nlines lines of length K held in LDS;
 // element (line l, position p) lives at buf[l*ls + p*ks]; tw is the LDS
-// twiddle table exp(-2 pi i j / K). Line l is owned by wave l % nw in every
+// twiddle table exp(-2 pi i j / K); Ns = product of the radices of the
+// earlier passes, kons = K / Ns. Line l is owned by wave l % nw in every
 // pass (a wave works on G of its lines at a time), and a wave's LDS
 // operations execute in program order, so a line's reads all precede its
 // writes and the next pass sees them without any workgroup barrier.
 template <int R, int SIGN>
-__device__ void wave_pass(float2* buf, int nlines, int ls, int ks, int K, int Ns, const float2* tw, int wid,
-                          int nw, int lane) {
+__device__ __forceinline__ void wave_pass(float2* buf, int nlines, int ls, int ks, int K, int Ns, int kons,
+                                          const float2* tw, int wid, int nw, int lane) {
     const int per = K / R;
-    const int tws = K / (Ns * R);
-    const unsigned mns = div_magic((unsigned)Ns);
+    const int tws = kons / R;
+    const float rns = 1.0f / (float)Ns;
     if (per <= 64) {
-        const int G = 64 / per;
-        const int sub = (int)fast_div((unsigned)lane, (unsigned)per, div_magic((unsigned)per));
+        const float rper = 1.0f / (float)per;
+        const int G = div_small(64, rper);
+        const int sub = div_small(lane, rper);
         const int j = lane - sub * per;
-        const int jn = (int)fast_div((unsigned)j, Ns, mns);
+        const int jn = div_small(j, rns);
         const int k = j - jn * Ns;
         const int d = jn * Ns * R + k;
         for (int i0 = 0; wid + nw * i0 < nlines; i0 += G) {
@@ -151,7 +165,7 @@ __device__ void wave_pass(float2* buf, int nlines, int ls, int ks, int K, int Ns
             for (int c = 0; c < 2; c++) {
                 const int j = lane + 64 * c;
                 if (j < per) {
-                    const int k = j - (int)fast_div((unsigned)j, Ns, mns) * Ns;
+                    const int k = j - div_small(j, rns) * Ns;
 #pragma unroll
                     for (int q = 0; q < R; q++) {
                         float2 x = buf[l * ls + (j + q * per) * ks];
@@ -169,7 +183,7 @@ __device__ void wave_pass(float2* buf, int nlines, int ls, int ks, int K, int Ns
             for (int c = 0; c < 2; c++) {
                 const int j = lane + 64 * c;
                 if (j < per) {
-                    const int jn = (int)fast_div((unsigned)j, Ns, mns);
+                    const int jn = div_small(j, rns);
                     const int d = jn * Ns * R + (j - jn * Ns);
 #pragma unroll
                     for (int q = 0; q < R; q++) buf[l * ls + (d + q * Ns) * ks] = v[c][q];
@@ -183,24 +197,24 @@ __device__ void wave_pass(float2* buf, int nlines, int ls, int ks, int K, int Ns
 // barrier before (data written by other waves) and after (data read by other
 // waves).
 template <int SIGN, int NT>
-__device__ void lds_fft(float2* buf, int nlines, int ls, int ks, const FftPlan& p, const float2* tw) {
+__device__ __forceinline__ void lds_fft(float2* buf, int nlines, int ls, int ks, const FftPlan& p, const float2* tw) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = NT / 64;
-    int Ns = 1;
+    int Ns = 1, kons = p.K;
     for (int ps = 0; ps < p.npass; ps++) {
-        int R = p.radix[ps];
+        const int R = (int)((p.code >> (4 * ps)) & 15ull);
         switch (R) {
-            case 4: wave_pass<4, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
-            case 2: wave_pass<2, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
-            case 3: wave_pass<3, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
-            case 5: wave_pass<5, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
-            default: wave_pass<7, SIGN>(buf, nlines, ls, ks, p.K, Ns, tw, wid, nw, lane); break;
+            case 4: wave_pass<4, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 4; break;
+            case 2: wave_pass<2, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 2; break;
+            case 3: wave_pass<3, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 3; break;
+            case 5: wave_pass<5, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 5; break;
+            default: wave_pass<7, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 7; break;
         }
         Ns *= R;
     }
 }
 
 template <int NT>
-__device__ void load_twiddles(float2* dst, const float2* __restrict__ src, int K) {
+__device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ src, int K) {
     for (int j = threadIdx.x; j < K; j += NT) dst[j] = src[j];
 }
 
@@ -244,86 +258,94 @@ __device__ __forceinline__ float bsp_dw(float f, int k) {
     }
 }
 
-// PME_Q_Spread (PME_force.cu:428-471) fused with the forward transform of
-// one x plane: grid point (base - k) gets q M4(fx+kx) M4(fy+ky) M4(fz+kz);
-// plane x receives the atoms filed under planes x .. x+3 (with kx = their
-// offset). The real plane is held directly in the packed complex layout of the
-// z-FFT: row y is the real (y even) or imaginary (y odd) part of line y/2.
+// PME_Q_Spread (PME_force.cu:428-471) fused with the forward z transform of
+// one block of kRowBlock y rows of one x plane: grid point (base - k) gets
+// q M4(fx+kx) M4(fy+ky) M4(fz+kz); plane x receives the atoms filed under
+// planes x .. x+3 (kx = their offset), of which the block keeps the stencil
+// rows that fall inside it. The rows are summed as 64-bit fixed-point
+// integers in LDS (LDS integer atomics are an order of magnitude faster than
+// LDS float atomics on this chip, and integer sums do not depend on the order
+// in which atoms arrive), converted, paired into complex z-lines (row r is
+// the real (r even) or imaginary (r odd) part of line r/2), transformed, and
+// unpacked (R2C) straight into the spectrum, stored [kz][x][y].
 //
 // MODE 1 spreads, instead of the charge, the adjoint source of the
 // force-matching gradient, q_i w_i . grad M_i(g): wv[i] = (w_x Kx/Lx,
 // w_y Ky/Ly, w_z Kz/Lz), with the fixed-point unit read from scale_ptr (set
 // on the device so that no contribution reaches 2^31).
-template <int MAXB, int MODE = 0>
-__device__ __forceinline__ void fwd_plane_body(
-    int x, const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
+template <int MODE = 0>
+__device__ __forceinline__ void spread_z_body(
+    int x, int yb, const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
     const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz,
-    const FftPlan& pz, const FftPlan& py, const float2* __restrict__ twz, const float2* __restrict__ twy,
-    double* __restrict__ acc, const float4* __restrict__ wv = nullptr, const float* __restrict__ scale_ptr = nullptr) {
+    const FftPlan& pz, const float2* __restrict__ twz, double* __restrict__ acc, const float4* __restrict__ wv = nullptr,
+    const float* __restrict__ scale_ptr = nullptr) {
     extern __shared__ float2 sm[];
     const float unit = MODE ? scale_ptr[0] : kSpreadScale;
-    const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
-    // packed z-lines sit at an odd pitch, so that the lines a wave transforms
-    // together fall on different LDS banks
-    const int zp = Kz + 1;
-    const int plane = max(Ky * H, nrp * zp);
-    float2* tz = sm + max(plane, Ky * Kz);  // past the fixed-point grid too
-    float2* ty = tz + Kz;
-    load_twiddles<kPlaneThreads>(tz, twz, Kz);
-    load_twiddles<kPlaneThreads>(ty, twy, Ky);
-    if (x == 0 && threadIdx.x == 0 && acc) acc[SPONGE_PART_RECIPROCAL] = 0.0;
-    // Spread into a row-major Ky x Kz grid of 64-bit fixed-point sums
-    // (kSpreadScale units) that aliases the front of the plane buffer. LDS
-    // integer atomics are an order of magnitude faster than LDS float atomics
-    // on this chip, and integer sums make the grid independent of the order
-    // in which atoms arrive.
-    unsigned long long* qacc = reinterpret_cast<unsigned long long*>(sm);
-    const int npt = Ky * Kz;
-    for (int idx = threadIdx.x; idx < npt; idx += kPlaneThreads) qacc[idx] = 0ull;
+    const int H = Kz / 2 + 1, zp = Kz + 1;  // odd pitch: a wave's lines fall on different banks
+    const int y0 = yb * kRowBlock, nr = min(kRowBlock, Ky - y0), nrp = (nr + 1) / 2;
+    unsigned long long* qacc = reinterpret_cast<unsigned long long*>(sm);  // nr x Kz
+    float2* pk = sm + kRowBlock * Kz;                                       // nrp lines at pitch zp
+    float2* tz = pk + (kRowBlock / 2) * zp;
+    load_twiddles<kZThreads>(tz, twz, Kz);
+    if (acc && x == 0 && yb == 0 && threadIdx.x == 0) acc[SPONGE_PART_RECIPROCAL] = 0.0;
+    const int npt = nr * Kz;
+    const float rkz = 1.0f / (float)Kz, rnr = 1.0f / (float)nr;
+    for (int idx = threadIdx.x; idx < npt; idx += kZThreads) qacc[idx] = 0ull;
     __syncthreads();
     // The four bins are walked as one flat range; each thread issues the
     // index, base and fraction loads of up to kSpreadPerThread atoms before
     // any LDS update, so their latencies overlap.
-    int c[4], off[5];
-    off[0] = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        int b = x + k;
-        b -= b >= Kx ? Kx : 0;
-        c[k] = min(bin_count[b], bin_cap);
-        off[k + 1] = off[k] + c[k];
-    }
-    const int total = off[4];
-    for (int t0 = 0; t0 < total; t0 += kPlaneThreads * kSpreadPerThread) {
+    // prefix offsets of the four bins, kept in scalars (an array indexed by
+    // a per-lane bin number would live in scratch memory)
+    int b0 = x, b1 = x + 1, b2 = x + 2, b3 = x + 3;
+    b1 -= b1 >= Kx ? Kx : 0;
+    b2 -= b2 >= Kx ? Kx : 0;
+    b3 -= b3 >= Kx ? Kx : 0;
+    const int o1 = min(bin_count[b0], bin_cap);
+    const int o2 = o1 + min(bin_count[b1], bin_cap);
+    const int o3 = o2 + min(bin_count[b2], bin_cap);
+    const int total = o3 + min(bin_count[b3], bin_cap);
+    for (int t0 = 0; t0 < total; t0 += kZThreads * kSpreadPerThread) {
         int ai[kSpreadPerThread], akx[kSpreadPerThread];
 #pragma unroll
         for (int r = 0; r < kSpreadPerThread; r++) {
-            const int t = t0 + r * kPlaneThreads + (int)threadIdx.x;
+            const int t = t0 + r * kZThreads + (int)threadIdx.x;
             ai[r] = -1;
             akx[r] = 0;
             if (t < total) {
-                const int k = (t >= off[1]) + (t >= off[2]) + (t >= off[3]);
-                int b = x + k;
-                b -= b >= Kx ? Kx : 0;
-                ai[r] = bin_atoms[(size_t)b * bin_cap + (t - off[k])];
+                const int k = (t >= o1) + (t >= o2) + (t >= o3);
+                const int b = k == 0 ? b0 : k == 1 ? b1 : k == 2 ? b2 : b3;
+                const int o = k == 0 ? 0 : k == 1 ? o1 : k == 2 ? o2 : o3;
+                ai[r] = bin_atoms[(size_t)b * bin_cap + (t - o)];
                 akx[r] = k;
             }
         }
+        // base and fraction loads of every candidate are issued together (one
+        // memory round trip), the rows are tested afterwards
         unsigned bb[kSpreadPerThread];
-        float4 ff[kSpreadPerThread], ww[kSpreadPerThread];
+        float4 ff[kSpreadPerThread];
 #pragma unroll
-        for (int r = 0; r < kSpreadPerThread; r++) {
+        for (int r = 0; r < kSpreadPerThread; r++)
             if (ai[r] >= 0) {
                 bb[r] = base[ai[r]];
                 ff[r] = frac[ai[r]];
-                if (MODE) ww[r] = wv[ai[r]];
             }
-        }
 #pragma unroll
         for (int r = 0; r < kSpreadPerThread; r++) {
             if (ai[r] < 0) continue;
-            const int by = (int)((bb[r] >> 10) & 1023u), bz = (int)(bb[r] >> 20);
+            // stencil rows by - ky, ky = 0..3, relative to the block
+            int d = (int)((bb[r] >> 10) & 1023u) - y0;
+            d += d < 0 ? Ky : 0;
+            bool inside = false;
+#pragma unroll
+            for (int ky = 0; ky < 4; ky++) {
+                int rr = d - ky;
+                rr += rr < 0 ? Ky : 0;
+                inside |= rr < nr;
+            }
+            if (!inside) continue;
             const float4 f = ff[r];
+            const int bz = (int)(bb[r] >> 20);
             float wz[4], dz[4];
             int gz[4];
 #pragma unroll
@@ -333,30 +355,27 @@ __device__ __forceinline__ void fwd_plane_body(
                 int tt = bz - kz;
                 gz[kz] = tt + (tt < 0 ? Kz : 0);
             }
-            if (MODE == 0) {
-                const float wx = f.w * unit * bsp_w(f.x, akx[r]);
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (MODE) w = wv[ai[r]];
+            const float qs = f.w * unit;
+            const float ax = bsp_w(f.x, akx[r]);
+            const float dax = MODE ? bsp_dw(f.x, akx[r]) : 0.f;
 #pragma unroll
-                for (int ky = 0; ky < 4; ky++) {
-                    int gy = by - ky;
-                    gy += gy < 0 ? Ky : 0;
-                    const float wy = wx * bsp_w(f.y, ky);
-                    unsigned long long* row = qacc + gy * Kz;
+            for (int ky = 0; ky < 4; ky++) {
+                int rr = d - ky;
+                rr += rr < 0 ? Ky : 0;
+                if (rr >= nr) continue;
+                const float ay = bsp_w(f.y, ky);
+                unsigned long long* row = qacc + rr * Kz;
+                if (MODE == 0) {
+                    const float wy = qs * ax * ay;
 #pragma unroll
                     for (int kz = 0; kz < 4; kz++)
                         atomicAdd(row + gz[kz], (unsigned long long)(long long)__float2int_rn(wy * wz[kz]));
-                }
-            } else {
-                const float4 w = ww[r];
-                const float qs = f.w * unit;
-                const float ax = bsp_w(f.x, akx[r]), dax = bsp_dw(f.x, akx[r]);
-#pragma unroll
-                for (int ky = 0; ky < 4; ky++) {
-                    int gy = by - ky;
-                    gy += gy < 0 ? Ky : 0;
-                    const float ay = bsp_w(f.y, ky), day = bsp_dw(f.y, ky);
+                } else {
+                    const float day = bsp_dw(f.y, ky);
                     const float c1 = qs * (w.x * dax * ay + w.y * ax * day);
                     const float c2 = qs * w.z * ax * ay;
-                    unsigned long long* row = qacc + gy * Kz;
 #pragma unroll
                     for (int kz = 0; kz < 4; kz++)
                         atomicAdd(row + gz[kz],
@@ -366,236 +385,245 @@ __device__ __forceinline__ void fwd_plane_body(
         }
     }
     __syncthreads();
-    // Fixed point -> float, into the packed complex layout of the z-FFT: row y
-    // is the real (y even) or imaginary (y odd) part of line y/2; with Ky odd
-    // the imaginary part of the last line is zero.
-    float qv[2 * MAXB];
-#pragma unroll
-    for (int s = 0; s < 2 * MAXB; s++) {
-        const int o = threadIdx.x + s * kPlaneThreads;
-        if (o < npt) qv[s] = (float)((double)(long long)qacc[o] / (double)unit);
+    float* pkf = reinterpret_cast<float*>(pk);
+    for (int idx = threadIdx.x; idx < npt; idx += kZThreads) {
+        const int r = div_small(idx, rkz), z = idx - r * Kz;
+        pkf[2 * ((r >> 1) * zp + z) + (r & 1)] = (float)((double)(long long)qacc[idx] / (double)unit);
     }
+    if (nr & 1)
+        for (int z = threadIdx.x; z < Kz; z += kZThreads) pkf[2 * ((nr >> 1) * zp + z) + 1] = 0.f;
     __syncthreads();
-    float* smf = reinterpret_cast<float*>(sm);
-#pragma unroll
-    for (int s = 0; s < 2 * MAXB; s++) {
-        const int o = threadIdx.x + s * kPlaneThreads;
-        if (o < npt) {
-            const int y = o / Kz, z = o - y * Kz;
-            smf[2 * ((y >> 1) * zp + z) + (y & 1)] = qv[s];
-        }
-    }
-    if (Ky & 1)
-        for (int z = threadIdx.x; z < Kz; z += kPlaneThreads) smf[2 * ((Ky >> 1) * zp + z) + 1] = 0.f;
-    __syncthreads();
-    lds_fft<-1, kPlaneThreads>(sm, nrp, zp, 1, pz, tz);
+    lds_fft<-1, kZThreads>(pk, nrp, zp, 1, pz, tz);
     __syncthreads();
     // Unpack the paired transforms Z = FFT(a + i b): A_k = (Z_k + conj Z_-k)/2,
-    // B_k = (Z_k - conj Z_-k)/(2i), into rows y of the half spectrum.
-    float2 st[MAXB];
-#pragma unroll
-    for (int s = 0; s < MAXB; s++) {
-        int o = threadIdx.x + s * kPlaneThreads;
-        if (o < Ky * H) {
-            int y = o / H, k = o - y * H;
-            int p = y >> 1;
-            float2 zk = sm[p * zp + k];
-            float2 zm = sm[p * zp + (k == 0 ? 0 : Kz - k)];
-            if ((y & 1) == 0) st[s] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-            else st[s] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-        }
+    // B_k = (Z_k - conj Z_-k)/(2i); consecutive threads take consecutive rows
+    // so the spectrum is written in runs of nr y values.
+    for (int idx = threadIdx.x; idx < nr * H; idx += kZThreads) {
+        const int k = div_small(idx, rnr), r = idx - k * nr;
+        const int p = r >> 1;
+        const float2 zk = pk[p * zp + k];
+        const float2 zm = pk[p * zp + (k == 0 ? 0 : Kz - k)];
+        const float2 v = (r & 1) == 0 ? make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y))
+                                      : make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+        FQ[((size_t)k * Kx + x) * Ky + y0 + r] = v;
     }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < MAXB; s++) {
-        int o = threadIdx.x + s * kPlaneThreads;
-        if (o < Ky * H) sm[o] = st[s];
-    }
-    __syncthreads();
-    lds_fft<-1, kPlaneThreads>(sm, H, 1, H, py, ty);
-    __syncthreads();
-    float2* out = FQ + (size_t)x * Ky * H;
-    for (int idx = threadIdx.x; idx < Ky * H; idx += kPlaneThreads) out[idx] = sm[idx];
 }
 
-template <int MAXB>
-__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_kernel(
+__global__ __launch_bounds__(kZThreads) void spread_z_kernel(
     const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
-    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, FftPlan pz,
-    FftPlan py, const float2* __restrict__ twz, const float2* __restrict__ twy, double* __restrict__ acc) {
-    fwd_plane_body<MAXB>(blockIdx.x, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, py, twz, twy,
-                         acc);
+    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, int nyb,
+    FftPlan pz, const float2* __restrict__ twz, double* __restrict__ acc) {
+    const int x = blockIdx.x / nyb;
+    spread_z_body(x, blockIdx.x - x * nyb, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, twz, acc);
 }
 
-template <int MAXB>
-__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_adjoint_kernel(
+__global__ __launch_bounds__(kZThreads) void spread_z_adjoint_kernel(
     const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
-    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, FftPlan pz,
-    FftPlan py, const float2* __restrict__ twz, const float2* __restrict__ twy, const float4* __restrict__ wv,
-    const float* __restrict__ scale_ptr) {
-    fwd_plane_body<MAXB, 1>(blockIdx.x, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, py, twz,
-                            twy, nullptr, wv, scale_ptr);
+    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, int nyb,
+    FftPlan pz, const float2* __restrict__ twz, const float4* __restrict__ wv, const float* __restrict__ scale_ptr) {
+    const int x = blockIdx.x / nyb;
+    spread_z_body<1>(x, blockIdx.x - x * nyb, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, twz,
+                     nullptr, wv, scale_ptr);
 }
 
-// Batched: block -> (system, x plane) from the block map.
-template <int MAXB>
-__global__ __launch_bounds__(kPlaneThreads) void fwd_plane_batch(const BatchSys* __restrict__ bs,
-                                                                 const int2* __restrict__ map, int energy) {
-    const int2 m = map[blockIdx.x];
-    if (m.x < 0) return;
-    const BatchSys& b = bs[m.x];
-    const sponge_pme& P = b.pme;
-    fwd_plane_body<MAXB>(m.y, P.base, P.frac, P.bin_count, P.bin_atoms, P.bin_cap, P.FQ, P.K[0], P.K[1], P.K[2],
-                         b.pz, b.py, P.d_tw[2], P.d_tw[1], energy ? P.acc : nullptr);
-}
-
-// One block per (y, chunk of kConvChunk kz): FFT along x, energy, BC
-// multiply, inverse FFT along x.
-__device__ __forceinline__ void xconv_body(int y, int chunk, float2* __restrict__ FQ, const float* __restrict__ bc,
-                                           int Kx, int Ky, int Kz, const FftPlan& px,
-                                           const float2* __restrict__ twx, int energy, double* __restrict__ acc,
-                                           int* __restrict__ bin_count) {
-    __shared__ float2 sm[256 * kConvPitch];
-    __shared__ float2 tx[256];
-    __shared__ double red[kConvThreads / 64];
-    const int H = Kz / 2 + 1;
-    const int kz0 = chunk * kConvChunk;
-    const int cw = min(kConvChunk, H - kz0);
-    // the forward plane kernels have consumed the x-plane bins: clear them
-    if (y == 0 && chunk == 0)
-        for (int b = threadIdx.x; b < Kx; b += kConvThreads) bin_count[b] = 0;
-    load_twiddles<kConvThreads>(tx, twx, Kx);
-    for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
-        int xx = idx / cw, c = idx - xx * cw;
-        sm[xx * kConvPitch + c] = FQ[((size_t)xx * Ky + y) * H + kz0 + c];
-    }
-    __syncthreads();
-    lds_fft<-1, kConvThreads>(sm, cw, 1, kConvPitch, px, tx);
-    __syncthreads();
-    double e = 0.0;
-    for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
-        int xx = idx / cw, c = idx - xx * cw;
-        int kz = kz0 + c;
-        float b = bc[((size_t)xx * Ky + y) * H + kz];
-        float2 v = sm[xx * kConvPitch + c];
-        if (energy) {
-            double m = (kz == 0 || (2 * kz == Kz)) ? 1.0 : 2.0;
-            e += m * (double)b * ((double)v.x * v.x + (double)v.y * v.y);
-        }
-        sm[xx * kConvPitch + c] = make_float2(v.x * b, v.y * b);
-    }
-    __syncthreads();
-    lds_fft<1, kConvThreads>(sm, cw, 1, kConvPitch, px, tx);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < Kx * cw; idx += kConvThreads) {
-        int xx = idx / cw, c = idx - xx * cw;
-        FQ[((size_t)xx * Ky + y) * H + kz0 + c] = sm[xx * kConvPitch + c];
-    }
-    if (energy) {
-        double t = block_sum<kConvThreads>(e, red);
-        if (threadIdx.x == 0) atomicAdd(acc + SPONGE_PART_RECIPROCAL, 0.5 * t);
-    }
-}
-
-__global__ __launch_bounds__(kConvThreads) void xconv_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
-                                                             int Kx, int Ky, int Kz, FftPlan px,
-                                                             const float2* __restrict__ twx, int energy,
-                                                             double* __restrict__ acc, int* __restrict__ bin_count) {
-    xconv_body(blockIdx.x, blockIdx.y, FQ, bc, Kx, Ky, Kz, px, twx, energy, acc, bin_count);
-}
-
-// Batched: the local block l is (y = l mod Ky, chunk = l / Ky).
-__global__ __launch_bounds__(kConvThreads) void xconv_batch(const BatchSys* __restrict__ bs,
+// Batched: local block l = x * (row blocks) + row block.
+__global__ __launch_bounds__(kZThreads) void spread_z_batch(const BatchSys* __restrict__ bs,
                                                             const int2* __restrict__ map, int energy) {
     const int2 m = map[blockIdx.x];
     if (m.x < 0) return;
     const BatchSys& b = bs[m.x];
     const sponge_pme& P = b.pme;
-    const int Ky = P.K[1];
-    const int chunk = m.y / Ky;
-    xconv_body(m.y - chunk * Ky, chunk, P.FQ, P.d_bc, P.K[0], Ky, P.K[2], b.px, P.d_tw[0], energy, P.acc,
-               P.bin_count);
+    const int nyb = row_blocks(P.K[1]);
+    const int x = m.y / nyb;
+    spread_z_body(x, m.y - x * nyb, P.base, P.frac, P.bin_count, P.bin_atoms, P.bin_cap, P.FQ, P.K[0], P.K[1], P.K[2],
+                  b.pz, P.d_tw[2], energy ? P.acc : nullptr);
 }
 
-template <int MAXB>
-__device__ __forceinline__ void inv_plane_body(int x, const float2* __restrict__ FQ, float* __restrict__ R, int Ky,
-                                               int Kz, const FftPlan& pz, const FftPlan& py,
-                                               const float2* __restrict__ twz, const float2* __restrict__ twy) {
+// One block per kz: the Kx x Ky plane of the spectrum in LDS (x rows at an
+// odd pitch), FFT along y then x, energy 1/2 sum m BC |F|^2 (FP64; m = 2 for
+// the kz that stand for their conjugates too), multiply by BC, inverse FFT
+// along x then y.
+__device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, const float* __restrict__ bc, int Kx,
+                                            int Ky, int Kz, const FftPlan& px, const FftPlan& py,
+                                            const float2* __restrict__ twx, const float2* __restrict__ twy, int energy,
+                                            double* __restrict__ acc, int* __restrict__ bin_count) {
     extern __shared__ float2 sm[];
-    const int H = Kz / 2 + 1, nrp = (Ky + 1) / 2;
-    const int zp = Kz + 1;  // odd pitch of the packed z-lines
-    const int plane = max(Ky * H, nrp * zp);
-    float2* tz = sm + plane;
-    float2* ty = tz + Kz;
-    load_twiddles<kPlaneThreads>(tz, twz, Kz);
-    load_twiddles<kPlaneThreads>(ty, twy, Ky);
-    const float2* in = FQ + (size_t)x * Ky * H;
-    for (int idx = threadIdx.x; idx < Ky * H; idx += kPlaneThreads) sm[idx] = in[idx];
-    __syncthreads();
-    lds_fft<1, kPlaneThreads>(sm, H, 1, H, py, ty);
-    __syncthreads();
-    // Repack rows (2p, 2p+1) into one complex line Z = A + i B over all kz,
-    // extending each row by Hermitian symmetry; self-conjugate bins keep only
-    // their real part (the C2R convention).
-    float2 st[MAXB];
+    __shared__ double red[kConvThreads / 64];
+    const int pitch = conv_pitch(Ky);
+    float2* tx = sm + Kx * pitch;
+    float2* ty = tx + Kx;
+    // the spread has consumed the x-plane bins: clear them for the next prep
+    if (kz == 0)
+        for (int b = threadIdx.x; b < Kx; b += kConvThreads) bin_count[b] = 0;
+    load_twiddles<kConvThreads>(tx, twx, Kx);
+    load_twiddles<kConvThreads>(ty, twy, Ky);
+    float2* plane = FQ + (size_t)kz * Kx * Ky;
+    const float* bcp = bc + (size_t)kz * Kx * Ky;
+    const int np = Kx * Ky;
+    const float rky = 1.0f / (float)Ky;
+    // The plane and its BC values are loaded in one memory round trip when
+    // they fit kConvBatch points per thread (the BC values wait in registers
+    // through the forward transforms): with one block per CU nothing else
+    // hides the memory latency.
+    float bv[kConvBatch];
+    const bool onepass = np <= kConvBatch * kConvThreads;
+    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * kConvThreads) {
+        float2 v[kConvBatch];
 #pragma unroll
-    for (int s = 0; s < MAXB; s++) {
-        int o = threadIdx.x + s * kPlaneThreads;
-        if (o < nrp * Kz) {
-            int p = o / Kz, k = o - p * Kz;
-            bool hi = k > Kz / 2;
-            int kk = hi ? Kz - k : k;
-            bool selfc = (kk == 0) || (2 * kk == Kz);
-            float2 A = sm[(2 * p) * H + kk];
-            float2 B = (2 * p + 1 < Ky) ? sm[(2 * p + 1) * H + kk] : make_float2(0.f, 0.f);
-            if (hi) {
-                A.y = -A.y;
-                B.y = -B.y;
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * kConvThreads;
+            if (idx < np) {
+                v[s] = plane[idx];
+                if (onepass) bv[s] = bcp[idx];
             }
-            if (selfc) {
-                A.y = 0.f;
-                B.y = 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * kConvThreads;
+            if (idx < np) {
+                const int xx = div_small(idx, rky), y = idx - xx * Ky;
+                sm[xx * pitch + y] = v[s];
             }
-            st[s] = make_float2(A.x - B.y, A.y + B.x);
         }
     }
     __syncthreads();
+    lds_fft<-1, kConvThreads>(sm, Kx, pitch, 1, py, ty);
+    __syncthreads();
+    lds_fft<-1, kConvThreads>(sm, Ky, 1, pitch, px, tx);
+    __syncthreads();
+    double e = 0.0;
+    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * kConvThreads) {
+        if (!onepass) {
 #pragma unroll
-    for (int s = 0; s < MAXB; s++) {
-        int o = threadIdx.x + s * kPlaneThreads;
-        if (o < nrp * Kz) {
-            const int p = o / Kz;
-            sm[o + p] = st[s];  // line p at pitch zp = Kz + 1
+            for (int s = 0; s < kConvBatch; s++) {
+                const int idx = s0 + s * kConvThreads;
+                if (idx < np) bv[s] = bcp[idx];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * kConvThreads;
+            if (idx < np) {
+                const int xx = div_small(idx, rky), y = idx - xx * Ky;
+                const float b = bv[s];
+                const float2 w = sm[xx * pitch + y];
+                if (energy) e += (double)b * ((double)w.x * w.x + (double)w.y * w.y);
+                sm[xx * pitch + y] = make_float2(w.x * b, w.y * b);
+            }
         }
     }
     __syncthreads();
-    lds_fft<1, kPlaneThreads>(sm, nrp, zp, 1, pz, tz);
+    lds_fft<1, kConvThreads>(sm, Ky, 1, pitch, px, tx);
     __syncthreads();
-    float* Rp = R + (size_t)x * Ky * Kz;
-    for (int idx = threadIdx.x; idx < nrp * Kz; idx += kPlaneThreads) {
-        int p = idx / Kz, z = idx - p * Kz;
-        float2 v = sm[p * zp + z];
-        Rp[(2 * p) * Kz + z] = v.x;
-        if (2 * p + 1 < Ky) Rp[(2 * p + 1) * Kz + z] = v.y;
+    lds_fft<1, kConvThreads>(sm, Kx, pitch, 1, py, ty);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < np; idx += kConvThreads) {
+        const int xx = div_small(idx, rky), y = idx - xx * Ky;
+        plane[idx] = sm[xx * pitch + y];
+    }
+    if (energy) {
+        const double t = block_sum<kConvThreads>(e, red);
+        const double m = (kz == 0 || 2 * kz == Kz) ? 1.0 : 2.0;
+        if (threadIdx.x == 0) atomicAdd(acc + SPONGE_PART_RECIPROCAL, 0.5 * m * t);
     }
 }
 
-template <int MAXB>
-__global__ __launch_bounds__(kPlaneThreads) void inv_plane_kernel(const float2* __restrict__ FQ,
-                                                                  float* __restrict__ R, int Ky, int Kz, FftPlan pz,
-                                                                  FftPlan py, const float2* __restrict__ twz,
-                                                                  const float2* __restrict__ twy) {
-    inv_plane_body<MAXB>(blockIdx.x, FQ, R, Ky, Kz, pz, py, twz, twy);
+__global__ __launch_bounds__(kConvThreads) void conv2d_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
+                                                              int Kx, int Ky, int Kz, FftPlan px, FftPlan py,
+                                                              const float2* __restrict__ twx,
+                                                              const float2* __restrict__ twy, int energy,
+                                                              double* __restrict__ acc, int* __restrict__ bin_count) {
+    conv2d_body(blockIdx.x, FQ, bc, Kx, Ky, Kz, px, py, twx, twy, energy, acc, bin_count);
 }
 
-template <int MAXB>
-__global__ __launch_bounds__(kPlaneThreads) void inv_plane_batch(const BatchSys* __restrict__ bs,
-                                                                 const int2* __restrict__ map) {
+__global__ __launch_bounds__(kConvThreads) void conv2d_batch(const BatchSys* __restrict__ bs,
+                                                             const int2* __restrict__ map, int energy) {
     const int2 m = map[blockIdx.x];
     if (m.x < 0) return;
     const BatchSys& b = bs[m.x];
     const sponge_pme& P = b.pme;
-    inv_plane_body<MAXB>(m.y, P.FQ, P.R, P.K[1], P.K[2], b.pz, b.py, P.d_tw[2], P.d_tw[1]);
+    conv2d_body(m.y, P.FQ, P.d_bc, P.K[0], P.K[1], P.K[2], b.px, b.py, P.d_tw[0], P.d_tw[1], energy, P.acc,
+                P.bin_count);
+}
+
+// Inverse z transform (C2R) of one block of kRowBlock y rows of one x plane:
+// rows (2p, 2p+1) are repacked into one complex line Z = A + i B over all kz,
+// each row extended by Hermitian symmetry (self-conjugate bins keep only their
+// real part, the C2R convention), transformed, and written to the real grid
+// R [x][y][z].
+__device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
+                                           int Ky, int Kz, const FftPlan& pz, const float2* __restrict__ twz) {
+    extern __shared__ float2 sm[];
+    const int H = Kz / 2 + 1, zp = Kz + 1;
+    const int y0 = yb * kRowBlock, nr = min(kRowBlock, Ky - y0), nrp = (nr + 1) / 2;
+    float2* rows = sm;                      // nr x H
+    float2* pk = sm + kRowBlock * H;        // nrp lines at pitch zp
+    float2* tz = pk + (kRowBlock / 2) * zp;
+    load_twiddles<kZThreads>(tz, twz, Kz);
+    const float rkz = 1.0f / (float)Kz, rnr = 1.0f / (float)nr;
+    float2 v[kZPer];
+#pragma unroll
+    for (int s = 0; s < kZPer; s++) {
+        const int idx = threadIdx.x + s * kZThreads;
+        if (idx < nr * H) {
+            const int k = div_small(idx, rnr), r = idx - k * nr;
+            v[s] = FQ[((size_t)k * Kx + x) * Ky + y0 + r];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kZPer; s++) {
+        const int idx = threadIdx.x + s * kZThreads;
+        if (idx < nr * H) {
+            const int k = div_small(idx, rnr), r = idx - k * nr;
+            rows[r * H + k] = v[s];
+        }
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < nrp * Kz; o += kZThreads) {
+        const int p = div_small(o, rkz), k = o - p * Kz;
+        const bool hi = k > Kz / 2;
+        const int kk = hi ? Kz - k : k;
+        const bool selfc = (kk == 0) || (2 * kk == Kz);
+        float2 A = rows[(2 * p) * H + kk];
+        float2 B = (2 * p + 1 < nr) ? rows[(2 * p + 1) * H + kk] : make_float2(0.f, 0.f);
+        if (hi) {
+            A.y = -A.y;
+            B.y = -B.y;
+        }
+        if (selfc) {
+            A.y = 0.f;
+            B.y = 0.f;
+        }
+        pk[p * zp + k] = make_float2(A.x - B.y, A.y + B.x);
+    }
+    __syncthreads();
+    lds_fft<1, kZThreads>(pk, nrp, zp, 1, pz, tz);
+    __syncthreads();
+    float* Rp = R + ((size_t)x * Ky + y0) * Kz;
+    for (int idx = threadIdx.x; idx < nrp * Kz; idx += kZThreads) {
+        const int p = div_small(idx, rkz), z = idx - p * Kz;
+        const float2 v = pk[p * zp + z];
+        Rp[(2 * p) * Kz + z] = v.x;
+        if (2 * p + 1 < nr) Rp[(2 * p + 1) * Kz + z] = v.y;
+    }
+}
+
+__global__ __launch_bounds__(kZThreads) void inv_z_kernel(const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
+                                                          int Ky, int Kz, int nyb, FftPlan pz,
+                                                          const float2* __restrict__ twz) {
+    const int x = blockIdx.x / nyb;
+    inv_z_body(x, blockIdx.x - x * nyb, FQ, R, Kx, Ky, Kz, pz, twz);
+}
+
+__global__ __launch_bounds__(kZThreads) void inv_z_batch(const BatchSys* __restrict__ bs,
+                                                         const int2* __restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const BatchSys& b = bs[m.x];
+    const sponge_pme& P = b.pme;
+    const int nyb = row_blocks(P.K[1]);
+    const int x = m.y / nyb;
+    inv_z_body(x, m.y - x * nyb, P.FQ, P.R, P.K[0], P.K[1], P.K[2], b.pz, P.d_tw[2]);
 }
 
 // PME_Final (PME_force.cu:485-550): 16 lanes per atom (lane = kx, ky), each
@@ -774,32 +802,44 @@ __global__ __launch_bounds__(1024) void pme_finalize_batch(const BatchSys* __res
 int plan_axis(int K, FftPlan* p) {
     p->K = K;
     p->npass = 0;
+    p->code = 0;
     int m = K;
-    while (m % 4 == 0 && p->npass < 12) {
-        p->radix[p->npass++] = 4;
-        m /= 4;
-    }
+    auto push = [&](int r) {
+        p->code |= (unsigned long long)r << (4 * p->npass);
+        p->npass++;
+        m /= r;
+    };
+    while (m % 4 == 0 && p->npass < 16) push(4);
     for (int r : {2, 3, 5, 7})
-        while (m % r == 0 && p->npass < 12) {
-            p->radix[p->npass++] = r;
-            m /= r;
-        }
+        while (m % r == 0 && p->npass < 16) push(r);
     return m == 1 ? 0 : -1;
 }
 
-// Dynamic LDS of the forward plane kernel: the fixed-point spread grid
-// (8 B per point) or the complex plane, whichever is larger, then the z and
-// y twiddles.
-size_t fwd_plane_lds(int Ky, int Kz) {
-    const long H = Kz / 2 + 1;
-    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
-    return (size_t)(std::max(plane, (long)Ky * Kz) + Ky + Kz) * sizeof(float2);
+// Dynamic LDS of the three transform kernels (bytes).
+size_t spread_z_lds(int Kz) {
+    return (size_t)(kRowBlock * Kz + (kRowBlock / 2) * (Kz + 1) + Kz) * sizeof(float2);
 }
+size_t inv_z_lds(int Kz) {
+    return (size_t)(kRowBlock * (Kz / 2 + 1) + (kRowBlock / 2) * (Kz + 1) + Kz) * sizeof(float2);
+}
+size_t conv2d_lds(int Kx, int Ky) { return (size_t)(Kx * conv_pitch(Ky) + Kx + Ky) * sizeof(float2); }
 
-int pick_maxb(long need) {
-    for (int b : {1, 2, 4, 8, 16})
-        if (need <= b) return b;
-    return -1;
+// Lets every transform kernel use up to `bytes` of dynamic LDS (once per
+// size; the attribute is per function).
+int raise_lds_limits(size_t bytes) {
+    static size_t done = 64 * 1024;
+    if (bytes <= done) return SPONGE_NB_OK;
+    const void* fns[] = {(const void*)spread_z_kernel, (const void*)spread_z_adjoint_kernel,
+                         (const void*)spread_z_batch,  (const void*)conv2d_kernel,
+                         (const void*)conv2d_batch,    (const void*)inv_z_kernel,
+                         (const void*)inv_z_batch};
+    for (const void* f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+            set_error("cannot raise the LDS limit of the PME kernels to %zu B", bytes);
+            return SPONGE_NB_EHIP;
+        }
+    done = bytes;
+    return SPONGE_NB_OK;
 }
 
 }  // namespace
@@ -864,7 +904,7 @@ static void pme_fill_bc(sponge_pme* pme, std::vector<float>& h) {
             int yr = y > Ky / 2 ? Ky - y : y;
             for (int z = 0; z < H; z++) {
                 double msq = xr * xr / (Lx * Lx) + yr * yr / (Ly * Ly) + (double)z * z / (Lz * Lz);
-                size_t idx = ((size_t)x * Ky + y) * H + z;
+                size_t idx = ((size_t)z * Kx + x) * Ky + y;  // spectrum layout [kz][x][y]
                 if (x == 0 && y == 0 && z == 0) h[idx] = 0.f;
                 else h[idx] = (float)(std::exp(-M_PI * M_PI * msq / (b * b)) / (M_PI * msq * V) * B1[x] * B2[y] * B3[z]);
             }
@@ -897,30 +937,22 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
         }
     }
     const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
-    long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
     int dev = 0, max_lds = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
         max_lds = 64 * 1024;
-    const size_t lds_fwd = fwd_plane_lds(Ky, Kz);
-    const int mb = pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
-    if (mb < 0 || lds_fwd > (size_t)max_lds || Kx > 256 ||
-        pick_maxb(((long)Kx * kConvChunk / 2 + kConvThreads - 1) / kConvThreads) < 0) {
-        set_error("PME grid %dx%dx%d too large for the in-LDS plane FFT (%zu B of LDS, %d available)", Kx, Ky, Kz,
-                  lds_fwd, max_lds);
+    const size_t lds_need = std::max({spread_z_lds(Kz), inv_z_lds(Kz), conv2d_lds(Kx, Ky)});
+    if (lds_need > (size_t)max_lds || Kx > 512 || Ky > 512 || Kz > 512) {
+        set_error("PME grid %dx%dx%d too large for the in-LDS transforms (%zu B of LDS, %d available)", Kx, Ky, Kz,
+                  lds_need, max_lds);
         delete pme;
         return SPONGE_NB_EARG;
     }
-    if (lds_fwd > 64 * 1024) {
-        const void* fn = mb == 1 ? (const void*)fwd_plane_kernel<1>
-                         : mb == 2 ? (const void*)fwd_plane_kernel<2>
-                         : mb == 4 ? (const void*)fwd_plane_kernel<4>
-                         : mb == 8 ? (const void*)fwd_plane_kernel<8>
-                                   : (const void*)fwd_plane_kernel<16>;
-        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fwd) != hipSuccess) {
-            set_error("cannot raise the LDS limit of the plane kernel to %zu B", lds_fwd);
+    {
+        const int rc = raise_lds_limits(lds_need);
+        if (rc) {
             delete pme;
-            return SPONGE_NB_EHIP;
+            return rc;
         }
     }
     pme->nfft = Kx * Ky * H;
@@ -1020,32 +1052,16 @@ int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const fl
         pme_prep_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_uint_crd, d_charge, pme_prep_args(pme), energy);
         SNB_CHECK_LAUNCH();
     }
-    long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
-    int mb = pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
-    size_t lds = (plane + Ky + Kz) * sizeof(float2);
-    size_t lds_fwd = fwd_plane_lds(Ky, Kz);
+    const int nyb = row_blocks(Ky);
     double* acc = energy ? pme->acc : nullptr;
-#define SNB_FWD(MB) fwd_plane_kernel<MB><<<Kx, kPlaneThreads, lds_fwd, s>>>(pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc)
-    switch (mb) {
-        case 1: SNB_FWD(1); break;
-        case 2: SNB_FWD(2); break;
-        case 4: SNB_FWD(4); break;
-        case 8: SNB_FWD(8); break;
-        default: SNB_FWD(16); break;
-    }
-#undef SNB_FWD
+    spread_z_kernel<<<Kx * nyb, kZThreads, spread_z_lds(Kz), s>>>(pme->base, pme->frac, pme->bin_count,
+                                                                  pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz,
+                                                                  nyb, pz, pme->d_tw[2], acc);
     SNB_CHECK_LAUNCH();
-    dim3 cg(Ky, ceil_div(H, kConvChunk));
-    xconv_kernel<<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc,
-                                             pme->bin_count);
+    conv2d_kernel<<<H, kConvThreads, conv2d_lds(Kx, Ky), s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, py, pme->d_tw[0],
+                                                               pme->d_tw[1], energy, pme->acc, pme->bin_count);
     SNB_CHECK_LAUNCH();
-    switch (mb) {
-        case 1: inv_plane_kernel<1><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        case 2: inv_plane_kernel<2><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        case 4: inv_plane_kernel<4><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        case 8: inv_plane_kernel<8><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        default: inv_plane_kernel<16><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-    }
+    inv_z_kernel<<<Kx * nyb, kZThreads, inv_z_lds(Kz), s>>>(pme->FQ, pme->R, Kx, Ky, Kz, nyb, pz, pme->d_tw[2]);
     SNB_CHECK_LAUNCH();
     (void)n;
     return d_frc ? snb_pme_gather(pme, d_frc, s) : SPONGE_NB_OK;
@@ -1097,44 +1113,21 @@ void snb_pme_plans(const sponge_pme* pme, FftPlan* px, FftPlan* py, FftPlan* pz)
     plan_axis(pme->K[2], pz);
 }
 
-int snb_pme_plane_maxb(const sponge_pme* pme) {
-    const int Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
-    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
-    return pick_maxb((plane + kPlaneThreads - 1) / kPlaneThreads);
-}
+int snb_pme_row_blocks(const sponge_pme* pme) { return row_blocks(pme->K[1]); }
 
-size_t snb_pme_lds_fwd(const sponge_pme* pme) { return fwd_plane_lds(pme->K[1], pme->K[2]); }
+size_t snb_pme_lds_fwd(const sponge_pme* pme) { return spread_z_lds(pme->K[2]); }
 
-size_t snb_pme_lds_inv(const sponge_pme* pme) {
-    const int Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
-    const long plane = std::max((long)Ky * H, (long)((Ky + 1) / 2) * (Kz + 1));
-    return (size_t)(plane + Ky + Kz) * sizeof(float2);
-}
+size_t snb_pme_lds_conv(const sponge_pme* pme) { return conv2d_lds(pme->K[0], pme->K[1]); }
+
+size_t snb_pme_lds_inv(const sponge_pme* pme) { return inv_z_lds(pme->K[2]); }
 
 int snb_batch_reciprocal(sponge_batch* b, int energy, hipStream_t s) {
-#define SNB_FWDB(MB)                                                                                         \
-    fwd_plane_batch<MB><<<b->nblk[kStFwd], kPlaneThreads, b->lds_fwd, s>>>(b->d, b->map[kStFwd], energy)
-#define SNB_INVB(MB) inv_plane_batch<MB><<<b->nblk[kStInv], kPlaneThreads, b->lds_inv, s>>>(b->d, b->map[kStInv])
-    switch (b->mb_plane) {
-        case 1: SNB_FWDB(1); break;
-        case 2: SNB_FWDB(2); break;
-        case 4: SNB_FWDB(4); break;
-        case 8: SNB_FWDB(8); break;
-        default: SNB_FWDB(16); break;
-    }
+    spread_z_batch<<<b->nblk[kStFwd], kZThreads, b->lds_fwd, s>>>(b->d, b->map[kStFwd], energy);
     SNB_CHECK_LAUNCH();
-    xconv_batch<<<b->nblk[kStConv], kConvThreads, 0, s>>>(b->d, b->map[kStConv], energy);
+    conv2d_batch<<<b->nblk[kStConv], kConvThreads, b->lds_conv, s>>>(b->d, b->map[kStConv], energy);
     SNB_CHECK_LAUNCH();
-    switch (b->mb_plane) {
-        case 1: SNB_INVB(1); break;
-        case 2: SNB_INVB(2); break;
-        case 4: SNB_INVB(4); break;
-        case 8: SNB_INVB(8); break;
-        default: SNB_INVB(16); break;
-    }
+    inv_z_batch<<<b->nblk[kStInv], kZThreads, b->lds_inv, s>>>(b->d, b->map[kStInv]);
     SNB_CHECK_LAUNCH();
-#undef SNB_FWDB
-#undef SNB_INVB
     return SPONGE_NB_OK;
 }
 
@@ -1147,19 +1140,6 @@ int snb_batch_gather(sponge_batch* b, hipStream_t s) {
 int snb_batch_finalize(sponge_batch* b, hipStream_t s) {
     pme_finalize_batch<<<b->nblk[kStFin], 1024, 0, s>>>(b->d, b->map[kStFin]);
     SNB_CHECK_LAUNCH();
-    return SPONGE_NB_OK;
-}
-
-// Raises the dynamic-LDS limit of the batched forward plane kernel when the
-// largest system's plane needs more than 64 KB.
-int snb_batch_pme_attributes(int mb, size_t lds_fwd) {
-    if (lds_fwd <= 64 * 1024) return SPONGE_NB_OK;
-    const void* fn = mb == 1 ? (const void*)fwd_plane_batch<1>
-                     : mb == 2 ? (const void*)fwd_plane_batch<2>
-                     : mb == 4 ? (const void*)fwd_plane_batch<4>
-                     : mb == 8 ? (const void*)fwd_plane_batch<8>
-                               : (const void*)fwd_plane_batch<16>;
-    SNB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fwd));
     return SPONGE_NB_OK;
 }
 
@@ -1177,41 +1157,15 @@ int snb_pme_adjoint_grid(sponge_pme* pme, const float4* wv, const float* scale, 
     const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
     FftPlan px, py, pz;
     snb_pme_plans(pme, &px, &py, &pz);
-    const int mb = snb_pme_plane_maxb(pme);
-    const size_t lds_fwd = fwd_plane_lds(Ky, Kz), lds = snb_pme_lds_inv(pme);
-    if (lds_fwd > 64 * 1024) {
-        const void* fn = mb == 1 ? (const void*)fwd_plane_adjoint_kernel<1>
-                         : mb == 2 ? (const void*)fwd_plane_adjoint_kernel<2>
-                         : mb == 4 ? (const void*)fwd_plane_adjoint_kernel<4>
-                         : mb == 8 ? (const void*)fwd_plane_adjoint_kernel<8>
-                                   : (const void*)fwd_plane_adjoint_kernel<16>;
-        SNB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_fwd));
-    }
-#define SNB_FWDA(MB)                                                                                           \
-    fwd_plane_adjoint_kernel<MB><<<Kx, kPlaneThreads, lds_fwd, s>>>(pme->base, pme->frac, pme->bin_count,      \
-                                                                    pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, \
-                                                                    Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], \
-                                                                    wv, scale)
-    switch (mb) {
-        case 1: SNB_FWDA(1); break;
-        case 2: SNB_FWDA(2); break;
-        case 4: SNB_FWDA(4); break;
-        case 8: SNB_FWDA(8); break;
-        default: SNB_FWDA(16); break;
-    }
-#undef SNB_FWDA
+    const int nyb = row_blocks(Ky);
+    spread_z_adjoint_kernel<<<Kx * nyb, kZThreads, spread_z_lds(Kz), s>>>(
+        pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb, pz,
+        pme->d_tw[2], wv, scale);
     SNB_CHECK_LAUNCH();
-    dim3 cg(Ky, ceil_div(H, kConvChunk));
-    xconv_kernel<<<cg, kConvThreads, 0, s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], 0, pme->acc,
-                                             pme->bin_count);
+    conv2d_kernel<<<H, kConvThreads, conv2d_lds(Kx, Ky), s>>>(pme->FQ, pme->d_bc, Kx, Ky, Kz, px, py, pme->d_tw[0],
+                                                               pme->d_tw[1], 0, pme->acc, pme->bin_count);
     SNB_CHECK_LAUNCH();
-    switch (mb) {
-        case 1: inv_plane_kernel<1><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        case 2: inv_plane_kernel<2><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        case 4: inv_plane_kernel<4><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        case 8: inv_plane_kernel<8><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-        default: inv_plane_kernel<16><<<Kx, kPlaneThreads, lds, s>>>(pme->FQ, pme->R, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]); break;
-    }
+    inv_z_kernel<<<Kx * nyb, kZThreads, inv_z_lds(Kz), s>>>(pme->FQ, pme->R, Kx, Ky, Kz, nyb, pz, pme->d_tw[2]);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }

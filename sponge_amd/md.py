@@ -119,6 +119,9 @@ class MDEngine:
             pi, pj, pr = system.water_constraints(sys_)
             self.settle = SETTLE(self.n, pi, pj, pr, m, self.dt)
         self.steps = 0
+        # True: the reciprocal pipeline runs on the caller's stream after the
+        # pair kernel instead of concurrently on the PME handle's stream
+        self.serial = False
 
     @property
     def crd(self):
@@ -129,7 +132,8 @@ class MDEngine:
         self.nb.update_neighbor_list(forced=1)
 
     def force(self, need_energy=0, need_virial=0, stream=None):
-        self.nb.force_step(need_energy, need_virial, from_crd=True, overwrite=True, stream=stream)
+        self.nb.force_step(need_energy, need_virial, from_crd=True, overwrite=True, serial=self.serial,
+                           stream=stream)
 
     def iterate(self, need_pressure=0, stream=None):
         md_iteration(self.settle, self.n, self.vel, self.nb.crd, self.nb.frc, self.acc, self.inverse_mass,
