@@ -109,8 +109,6 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     const bool serial = (flags & SPONGE_STEP_SERIAL) != 0;
     hipStream_t ps = serial ? s : pme->side;
     if (!serial) SNB_HIP(hipEventRecord(pme->fork, s));
-    rc = snb_prune(nl, lj->ps.pk, lj->cof_dr, s);
-    if (rc) return rc;
     rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme->beta,
                    energy, &lj->ps, d_frc, need_energy > 0 ? d_atom_energy : nullptr,
                    need_virial > 0 ? d_atom_virial : nullptr, nullptr, s, 1,

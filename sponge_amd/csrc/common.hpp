@@ -165,7 +165,7 @@ struct sponge_nl {
     int* flags;                 // [0] rebuild flag, [1] overflow bits, [2] rebuild count,
                                 // [3] slot count, [4] cluster count, [7] skin-check vote,
                                 // [8..9] listed pair count (64-bit), [10] fused steps since
-                                // the last prune
+                                // the last rebuild
     int device;
     // inner list of the fused step: the entries and masks of `list` whose
     // pairs lie within prune_r2, recomputed every prune_interval fused steps
@@ -255,6 +255,14 @@ struct sponge_batch {
     int refresh_interval, refresh_count;
 };
 
+// Inner-list arguments of the pair kernel (interval 0: no inner list).
+struct PruneArgs {
+    int interval;
+    float r2;
+    int* in_n;
+    uint2* in_list;
+};
+
 // Internal entry points shared between translation units.
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
@@ -281,7 +289,6 @@ int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s);
 int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s);
 int snb_batch_excluded(sponge_batch* b, hipStream_t s);
 int snb_lj_scratch(sponge_lj* lj, const sponge_nl* nl);
-int snb_prune(sponge_nl* nl, const uint4* pk, const float* cof, hipStream_t s);
 void snb_adj_free(sponge_lj* lj);
 int snb_batch_reciprocal(sponge_batch* b, int energy, hipStream_t s);
 int snb_batch_gather(sponge_batch* b, hipStream_t s);

@@ -34,6 +34,7 @@ constexpr int kSortThreads = 256;
 constexpr int kMaxColumn = 1024;     // bitonic sort capacity per column (LDS)
 constexpr int kPairWaves = 4;
 constexpr int kCandCap = 1024;       // candidate j-clusters per i-cluster (LDS)
+constexpr int kColCap = 128;         // cell columns of one search window held in LDS
 
 // Is_need_refresh_neighbor_list_cuda (neighbor_list.cu:336-351): FP32
 // periodic displacement against the coordinates of the last rebuild.
@@ -285,6 +286,8 @@ __device__ __forceinline__ void nl_pairs_body(
     uint2* __restrict__ list, int* flags) {
     if (!flags[0]) return;
     __shared__ int cand[kPairWaves][kCandCap];
+    __shared__ int colj0[kPairWaves][kColCap];
+    __shared__ int colpre[kPairWaves][kColCap];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ic = lb * kPairWaves + w;
     const int ncl = flags[4];
@@ -301,36 +304,85 @@ __device__ __forceinline__ void nl_pairs_body(
     const float rlm2 = rlm * rlm;
     int ncand = 0;
     bool over = false;
+    // bounding-box test of candidate j-cluster jc; appends it to cand[w]
+    auto test_box = [&](int jc, bool live) {
+        bool ok = false;
+        if (live) {
+            const float4 lj = cl_lo[jc], hj = cl_hi[jc];
+            float dx = 0.5f * (lj.x + hj.x) - cix, dy = 0.5f * (lj.y + hj.y) - ciy, dz = 0.5f * (lj.z + hj.z) - ciz;
+            dx -= bx * rintf(dx / bx);
+            dy -= by * rintf(dy / by);
+            dz -= bz * rintf(dz / bz);
+            const float gx = fmaxf(fabsf(dx) - hix - 0.5f * (hj.x - lj.x), 0.f);
+            const float gy = fmaxf(fabsf(dy) - hiy - 0.5f * (hj.y - lj.y), 0.f);
+            const float gz = fmaxf(fabsf(dz) - hiz - 0.5f * (hj.z - lj.z), 0.f);
+            ok = gx * gx + gy * gy + gz * gz < rlm2;
+        }
+        const unsigned long long bal = __ballot(ok);
+        const int pos = ncand + __popcll(bal & ((1ull << lane) - 1ull));
+        if (ok) {
+            if (pos < kCandCap) cand[w][pos] = jc;
+            else over = true;
+        }
+        ncand += __popcll(bal);
+    };
+    const int ncols = nxr * nyr;
+    if (ncols <= kColCap) {
+        // The window's columns as one flat range of j-clusters: a table of
+        // each column's first cluster and running cluster count, then 64
+        // clusters per step whatever their column (instead of one column per
+        // step, most of whose lanes would idle). Same candidate order.
+        int total = 0;
+        for (int base = 0; base < ncols; base += 64) {
+            const int k = base + lane;
+            int j0 = 0, cnt = 0;
+            if (k < ncols) {
+                const int ox = k / nyr, oy = k - ox * nyr;
+                const int cx = ((ax0 + ox) % ncx + ncx) % ncx, cy = ((ay0 + oy) % ncy + ncy) % ncy;
+                const int c = cx * ncy + cy;
+                j0 = col_start[c] / kClusterSize;
+                cnt = col_start[c + 1] / kClusterSize - j0;
+            }
+            int incl = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o);
+                if (lane >= o) incl += t;
+            }
+            if (k < ncols) {
+                colj0[w][k] = j0;
+                colpre[w][k] = total + incl - cnt;
+            }
+            total += __shfl(incl, 63);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        for (int t0 = 0; t0 < total; t0 += 64) {
+            const int t = t0 + lane;
+            int jc = 0;
+            if (t < total) {
+                // last column whose run starts at or before t (empty columns
+                // share their successor's start and are skipped)
+                int lo = 0, hi = ncols - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (colpre[w][mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                jc = colj0[w][lo] + (t - colpre[w][lo]);
+            }
+            test_box(jc, t < total);
+        }
+    } else {
     for (int ox = 0; ox < nxr; ox++) {
         int cx = ((ax0 + ox) % ncx + ncx) % ncx;
         for (int oy = 0; oy < nyr; oy++) {
             int cy = ((ay0 + oy) % ncy + ncy) % ncy;
             int c = cx * ncy + cy;
             int j0 = col_start[c] / kClusterSize, j1 = col_start[c + 1] / kClusterSize;
-            for (int jb = j0; jb < j1; jb += 64) {
-                int jc = jb + lane;
-                bool ok = false;
-                if (jc < j1) {
-                    float4 lj = cl_lo[jc], hj = cl_hi[jc];
-                    float dx = 0.5f * (lj.x + hj.x) - cix, dy = 0.5f * (lj.y + hj.y) - ciy,
-                          dz = 0.5f * (lj.z + hj.z) - ciz;
-                    dx -= bx * rintf(dx / bx);
-                    dy -= by * rintf(dy / by);
-                    dz -= bz * rintf(dz / bz);
-                    float gx = fmaxf(fabsf(dx) - hix - 0.5f * (hj.x - lj.x), 0.f);
-                    float gy = fmaxf(fabsf(dy) - hiy - 0.5f * (hj.y - lj.y), 0.f);
-                    float gz = fmaxf(fabsf(dz) - hiz - 0.5f * (hj.z - lj.z), 0.f);
-                    ok = gx * gx + gy * gy + gz * gz < rlm2;
-                }
-                unsigned long long bal = __ballot(ok);
-                int pos = ncand + __popcll(bal & ((1ull << lane) - 1ull));
-                if (ok) {
-                    if (pos < kCandCap) cand[w][pos] = jc;
-                    else over = true;
-                }
-                ncand += __popcll(bal);
-            }
+            for (int jb = j0; jb < j1; jb += 64) test_box(jb + lane, jb + lane < j1);
         }
+    }
     }
     if (ncand > kCandCap) {
         ncand = kCandCap;
@@ -346,12 +398,14 @@ __device__ __forceinline__ void nl_pairs_body(
     const int ai = (int)pi.w;
     int count = 0;
     long npairs = 0;
+    // the next group's j record is loaded while this group is tested
+    int jc = g < ncand ? cand[w][g] : -1;
+    uint4 pj = jc >= 0 ? pack[jc * kClusterSize + jj] : make_uint4(0u, 0u, 0u, 0u);
     for (int cb = 0; cb < ncand; cb += 4) {
-        int ci = cb + g;
-        int jc = ci < ncand ? cand[w][ci] : -1;
+        const int jcn = cb + 4 + g < ncand ? cand[w][cb + 4 + g] : -1;
+        const uint4 pjn = jcn >= 0 ? pack[jcn * kClusterSize + jj] : make_uint4(0u, 0u, 0u, 0u);
         bool bit = false;
         if (jc >= 0) {
-            uint4 pj = pack[jc * kClusterSize + jj];
             int aj = (int)pj.w;
             bool valid = ai >= 0 && aj >= 0 && (jc != ic || jj != ii);
             if (valid && nl_dr2(pi, pj, crx, cry, crz) < rl2) bit = !excluded_pair(ai, aj, es, en, el);
@@ -366,6 +420,8 @@ __device__ __forceinline__ void nl_pairs_body(
         }
         count += __popcll(lb2);
         npairs += __popcll(bal);
+        jc = jcn;
+        pj = pjn;
     }
     if (lane == 0) {
         if (count > list_cap) {

@@ -61,8 +61,12 @@ __device__ __forceinline__ void step_prep_body(int blk, const int* __restrict__ 
                                                const float* __restrict__ crd, float cqx, float cqy, float cqz,
                                                unsigned int* __restrict__ uc, const float* __restrict__ q,
                                                const int* __restrict__ type, uint4* __restrict__ pk,
-                                               const PmePrep& P, int energy, double* __restrict__ acc) {
+                                               const PmePrep& P, int energy, double* __restrict__ acc,
+                                               int* __restrict__ step_counter = nullptr) {
     const int p = blk * blockDim.x + threadIdx.x;
+    // fused steps since the last list rebuild, this one included (read by
+    // this step's pair kernel to decide whether it re-prunes the inner list)
+    if (step_counter && p == 0) atomicAdd(step_counter, 1);
     if (energy && p < 3) acc[p] = 0.0;  // SPONGE_PART_LJ, _DIRECT, _LJ_VIRIAL
     if (energy && p == 3) acc[SPONGE_PART_PAIRS_IN] = 0.0;
     const int nslot = flags[3];
@@ -96,8 +100,9 @@ __global__ __launch_bounds__(256) void step_prep_kernel(const int* __restrict__ 
                                                         const float* __restrict__ crd, float cqx, float cqy, float cqz,
                                                         unsigned int* __restrict__ uc, const float* __restrict__ q,
                                                         const int* __restrict__ type, uint4* __restrict__ pk,
-                                                        PmePrep P, int energy, double* __restrict__ acc) {
-    step_prep_body(blockIdx.x, flags, s2o, crd, cqx, cqy, cqz, uc, q, type, pk, P, energy, acc);
+                                                        PmePrep P, int energy, double* __restrict__ acc,
+                                                        int* __restrict__ step_counter) {
+    step_prep_body(blockIdx.x, flags, s2o, crd, cqx, cqy, cqz, uc, q, type, pk, P, energy, acc, step_counter);
 }
 
 __global__ __launch_bounds__(256) void step_prep_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map,
@@ -167,15 +172,23 @@ __device__ __forceinline__ void pair_body(
     int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
     float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
-    const float4* __restrict__ sc, int* __restrict__ step_counter = nullptr) {
+    const float4* __restrict__ sc, PruneArgs pa = PruneArgs{0, 0.f, nullptr, nullptr}) {
     extern __shared__ float2 sab[];
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ic = lb * kPairWaves + w;
     if (lb < 0 || ic >= flags[4]) return;
-    // fused steps since the last prune (read by the next step's prune kernel)
-    if (step_counter && ic == 0 && lane == 0) atomicAdd(step_counter, 1);
+    // Inner list (pa.interval > 0): on every pa.interval-th fused step since
+    // the last rebuild this launch reads the full list and writes the entries
+    // and bits within pa.r2 as the inner list; the other steps read the inner
+    // list. The counter was advanced by this step's prep kernel.
+    bool use_inner = false, write_inner = false;
+    if (pa.interval > 0) {
+        const int c = flags[10];
+        write_inner = c <= 1 || (c - 1) % pa.interval == 0;
+        use_inner = !write_inner;
+    }
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
     const int islot = ic * kClusterSize + ii;
     const uint4 a = pk[islot];
@@ -189,8 +202,10 @@ __device__ __forceinline__ void pair_body(
     float fix = 0.f, fiy = 0.f, fiz = 0.f;
     float eie = 0.f, eiv = 0.f, eid = 0.f;
     double tlj = 0.0, tdir = 0.0, tvir = 0.0, tcnt = 0.0;
-    const int n = list_n[ic];
-    const uint2* lst = list + (size_t)ic * list_cap;
+    const int n = use_inner ? pa.in_n[ic] : list_n[ic];
+    const uint2* lst = (use_inner ? pa.in_list : list) + (size_t)ic * list_cap;
+    uint2* inner_out = write_inner ? pa.in_list + (size_t)ic * list_cap : nullptr;
+    int inner_count = 0;
     for (int base = 0; base < n; base += 64) {
         // 64 list entries (16 iterations of 4 j-clusters) held one per lane.
         const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
@@ -209,6 +224,16 @@ __device__ __forceinline__ void pair_body(
             const float dz = (float)(int)((b.z & ~3u) - az) * cz;
             const float r2 = dr2_exact(dx, dy, dz);
             const bool in = ((m >> bit) & 1u) && r2 < rc2;
+            if (write_inner) {
+                const bool keep = ((m >> bit) & 1u) && r2 < pa.r2;
+                const unsigned long long kb = __ballot(keep);
+                const unsigned mk = (unsigned)(kb >> (g * 16)) & 0xffffu;
+                const bool leader = (lane & 15) == 0 && mk != 0u;
+                const unsigned long long lb2 = __ballot(leader);
+                if (leader)
+                    inner_out[inner_count + __popcll(lb2 & ((1ull << lane) - 1ull))] = make_uint2((unsigned)jc, mk);
+                inner_count += __popcll(lb2);
+            }
             // masked lanes (padding, the atom itself) may give inf / NaN
             // below; the final select discards them
             const float r2s = r2;
@@ -274,6 +299,7 @@ __device__ __forceinline__ void pair_body(
             jc = jcn;
         }
     }
+    if (write_inner && lane == 0) pa.in_n[ic] = inner_count;
     // i force: sum the 16 lanes sharing ii (lanes ii, ii+4, ..., ii+60).
 #pragma unroll
     for (int o = 4; o < 64; o <<= 1) {
@@ -328,54 +354,9 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
     const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz, float rc2,
     float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
-    const float4* __restrict__ sc, int* __restrict__ step_counter) {
+    const float4* __restrict__ sc, PruneArgs pa) {
     pair_body<ENERGY, SOFT, TB>(xcd_block(gridDim.x), flags, list_n, list, list_cap, pk, ab, nt, cx, cy, cz, rc2, beta,
-                                s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc, step_counter);
-}
-
-// Inner list of the fused step: one wave per i-cluster re-tests every pair
-// bit of the cluster-pair list against prune_r2 with the step's coordinates
-// (the pair kernel's records) and compacts the surviving entries. Runs only
-// on steps whose counter (fused steps since the last prune, reset by a
-// rebuild) is a multiple of the interval; the decision is the same for every
-// block, so the kernel can sit in a captured graph.
-__global__ __launch_bounds__(kPairWaves * 64) void prune_kernel(
-    const int* __restrict__ flags, int interval, const int* __restrict__ list_n, const uint2* __restrict__ list,
-    int list_cap, const uint4* __restrict__ pk, float cx, float cy, float cz, float pr2, int* __restrict__ in_n,
-    uint2* __restrict__ in_list) {
-    if (flags[10] % interval != 0) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int lb = xcd_block(gridDim.x);
-    const int ic = lb * kPairWaves + w;
-    if (lb < 0 || ic >= flags[4]) return;
-    const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
-    const uint4 a = pk[ic * kClusterSize + ii];
-    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
-    const int n = list_n[ic];
-    const uint2* lst = list + (size_t)ic * list_cap;
-    uint2* out = in_list + (size_t)ic * list_cap;
-    int count = 0;
-    for (int base = 0; base < n; base += 64) {
-        const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
-        const int cnt = min(64, n - base);
-        for (int it = 0; it < cnt; it += 4) {
-            const int src = it + g;
-            const int jc = __shfl((int)my.x, src);
-            const unsigned m = (unsigned)__shfl((int)my.y, src);
-            const uint4 b = pk[jc * kClusterSize + jj];
-            const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
-            const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
-            const float dz = (float)(int)((b.z & ~3u) - az) * cz;
-            const bool keep = ((m >> bit) & 1u) && dr2_exact(dx, dy, dz) < pr2;
-            const unsigned long long bal = __ballot(keep);
-            const unsigned mn = (unsigned)(bal >> (g * 16)) & 0xffffu;
-            const bool leader = (lane & 15) == 0 && mn != 0u;
-            const unsigned long long lb2 = __ballot(leader);
-            if (leader) out[count + __popcll(lb2 & ((1ull << lane) - 1ull))] = make_uint2((unsigned)jc, mn);
-            count += __popcll(lb2);
-        }
-    }
-    if (lane == 0) in_n[ic] = count;
+                                s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc, pa);
 }
 
 // Batched: logical blocks are spread over the XCDs as in the single-system
@@ -393,8 +374,7 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_batch(const BatchSys* __
     pair_body<ENERGY, SOFT, TB>(m.y, b.nl.flags, b.nl.list_n, b.nl.list, b.nl.list_cap, b.lj.ps.pk, b.lj.d_ab,
                             b.lj.ntypes, b.lj.cof_dr[0], b.lj.cof_dr[1], b.lj.cof_dr[2], rc * rc, b.pme.beta,
                             b.nl.s2o, b.io.frc, ENERGY ? b.io.atom_energy : nullptr,
-                            ENERGY ? b.io.atom_virial : nullptr, nullptr, b.lj.ps.acc, overwrite, b.lj.d_sc,
-                            nullptr);
+                            ENERGY ? b.io.atom_virial : nullptr, nullptr, b.lj.ps.acc, overwrite, b.lj.d_sc);
 }
 
 __global__ void add_scalar_kernel(float* a, float va, float* b, float vb) {
@@ -700,9 +680,10 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered,
               int overwrite, const float4* sc, int inner) {
     inner = inner && nl->prune_interval > 0;
-    const int* list_n = inner ? nl->list_in_n : nl->list_n;
-    const uint2* list = inner ? nl->list_in : nl->list;
-    int* counter = inner ? nl->flags + 10 : nullptr;
+    const PruneArgs pa = inner ? PruneArgs{nl->prune_interval, nl->prune_r2, nl->list_in_n, nl->list_in}
+                               : PruneArgs{0, 0.f, nullptr, nullptr};
+    const int* list_n = nl->list_n;
+    const uint2* list = nl->list;
     int rc = scratch_ensure(ps, nl);
     if (rc) return rc;
     const int cap = ps->cap;
@@ -724,7 +705,7 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     pair_kernel<E, S, TB><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, list_n, list, nl->list_cap, ps->pk, ab,   \
                                                          nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,         \
                                                          E ? atom_e : nullptr, E ? atom_v : nullptr,                    \
-                                                         E ? atom_d : nullptr, ps->acc, overwrite, sc, counter)
+                                                         E ? atom_d : nullptr, ps->acc, overwrite, sc, pa)
     if (energy) {
         if (sc) SNB_PAIR(true, true);
         else SNB_PAIR(true, false);
@@ -748,7 +729,7 @@ int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const flo
     const int cap = lj->ps.cap;
     step_prep_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(nl->flags, nl->s2o, d_crd, cq[0], cq[1], cq[2], d_uint_crd,
                                                         d_charge, lj->d_type, lj->ps.pk, pme_prep_args(pme), energy,
-                                                        lj->ps.acc);
+                                                        lj->ps.acc, nl->prune_interval > 0 ? nl->flags + 10 : nullptr);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
@@ -872,14 +853,3 @@ int snb_batch_excluded(sponge_batch* b, hipStream_t s) {
 
 int snb_lj_scratch(sponge_lj* lj, const sponge_nl* nl) { return scratch_ensure(&lj->ps, nl); }
 
-// Prunes the fused step's inner list when its step counter says so (see
-// prune_kernel); pk holds the step's coordinates.
-int snb_prune(sponge_nl* nl, const uint4* pk, const float* cof, hipStream_t s) {
-    if (nl->prune_interval <= 0) return SPONGE_NB_OK;
-    const int nblk = xcd_grid(ceil_div(nl->ncl_cap, kPairWaves));
-    prune_kernel<<<nblk, kPairWaves * 64, 0, s>>>(nl->flags, nl->prune_interval, nl->list_n, nl->list, nl->list_cap,
-                                                  pk, cof[0], cof[1], cof[2], nl->prune_r2, nl->list_in_n,
-                                                  nl->list_in);
-    SNB_CHECK_LAUNCH();
-    return SPONGE_NB_OK;
-}
