@@ -109,18 +109,27 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     const bool serial = (flags & SPONGE_STEP_SERIAL) != 0;
     hipStream_t ps = serial ? s : pme->side;
     if (!serial) SNB_HIP(hipEventRecord(pme->fork, s));
+    // Force-only steps fold the excluded-pair correction into the pair kernel
+    // when the neighbor list holds the symmetric form of this exclusion CSR;
+    // energy steps keep the separate kernel, which also fills the per-atom
+    // correction energies.
+    const bool fold_excl = !energy && nl->sym_start && nl->ex_es == d_excluded_list_start &&
+                           nl->ex_el == d_excluded_list && nl->ex_en == d_excluded_numbers;
+    const ExclArgs ex = fold_excl ? ExclArgs{nl->sym_start, nl->sym_list, nl->o2s} : ExclArgs{nullptr, nullptr, nullptr};
     rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme->beta,
                    energy, &lj->ps, d_frc, need_energy > 0 ? d_atom_energy : nullptr,
                    need_virial > 0 ? d_atom_virial : nullptr, nullptr, s, 1,
-                   (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, lj->d_sc, 1);
+                   (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, lj->d_sc, 1, ex);
     if (rc) return rc;
     if (!serial) SNB_HIP(hipStreamWaitEvent(ps, pme->fork, 0));
     rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, energy, 1, ps);
     if (rc) return rc;
     if (!serial) SNB_HIP(hipEventRecord(pme->join, ps));
-    rc = snb_excluded(pme, d_uint_crd, lj->cof_dr, d_charge, d_excluded_list_start, d_excluded_list,
-                      d_excluded_numbers, d_frc, nullptr, s);
-    if (rc) return rc;
+    if (!fold_excl) {
+        rc = snb_excluded(pme, d_uint_crd, lj->cof_dr, d_charge, d_excluded_list_start, d_excluded_list,
+                          d_excluded_numbers, d_frc, nullptr, s);
+        if (rc) return rc;
+    }
     if (!serial) SNB_HIP(hipStreamWaitEvent(s, pme->join, 0));
     rc = snb_pme_gather(pme, d_frc, s);
     if (rc) return rc;

@@ -174,6 +174,14 @@ struct sponge_nl {
     float prune_r2;
     int* list_in_n;             // ncl_cap
     uint2* list_in;             // ncl_cap * list_cap
+    // symmetric exclusion lists in original atom order (partners of atom a:
+    // sym_list[sym_start[a] .. sym_start[a+1])), built on the host from the
+    // exclusion CSR the first time an update passes those arrays
+    const int* ex_es;
+    const int* ex_el;
+    const int* ex_en;
+    int* sym_start;             // n + 1
+    int* sym_list;              // 2 x excluded pairs
 };
 
 struct sponge_lj {
@@ -263,11 +271,20 @@ struct PruneArgs {
     uint2* in_list;
 };
 
+// Excluded-pair correction folded into the pair kernel (start NULL: off):
+// the symmetric exclusion lists and the atom -> slot map of the list.
+struct ExclArgs {
+    const int* start;
+    const int* list;
+    const int* o2s;
+};
+
 // Internal entry points shared between translation units.
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered = 0,
-              int overwrite = 0, const float4* sc = nullptr, int inner = 0);
+              int overwrite = 0, const float4* sc = nullptr, int inner = 0,
+              ExclArgs ex = ExclArgs{nullptr, nullptr, nullptr});
 void snb_scratch_free(PairScratch* ps);
 int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float scaler[3], const float* d_charge,
                  const int* es, const int* el, const int* en, float* d_frc, float* d_atom_energy, hipStream_t s);

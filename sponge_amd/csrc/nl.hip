@@ -283,7 +283,8 @@ __device__ __forceinline__ void nl_pairs_body(
     float rlm, const int* __restrict__ col_start, const float4* __restrict__ cl_lo,
     const float4* __restrict__ cl_hi, const uint4* __restrict__ pack, const int* __restrict__ es,
     const int* __restrict__ en, const int* __restrict__ el, int list_cap, int* __restrict__ list_n,
-    uint2* __restrict__ list, int* flags) {
+    uint2* __restrict__ list, int* flags, const int* __restrict__ sym_start = nullptr,
+    const int* __restrict__ sym_list = nullptr) {
     if (!flags[0]) return;
     __shared__ int cand[kPairWaves][kCandCap];
     __shared__ int colj0[kPairWaves][kColCap];
@@ -396,6 +397,25 @@ __device__ __forceinline__ void nl_pairs_body(
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4;
     const uint4 pi = pack[ic * kClusterSize + ii];
     const int ai = (int)pi.w;
+    // The lane's i-atom exclusion partners, the first four in registers, from
+    // the symmetric lists when there are some (else the CSR is searched).
+    int e0 = -1, e1 = -1, e2 = -1, e3 = -1, r0 = 0, r1 = 0;
+    if (sym_start && ai >= 0) {
+        r0 = sym_start[ai];
+        r1 = sym_start[ai + 1];
+        const int len = r1 - r0;
+        if (len > 0) e0 = sym_list[r0];
+        if (len > 1) e1 = sym_list[r0 + 1];
+        if (len > 2) e2 = sym_list[r0 + 2];
+        if (len > 3) e3 = sym_list[r0 + 3];
+    }
+    auto is_excluded = [&](int aj) -> bool {
+        if (!sym_start) return excluded_pair(ai, aj, es, en, el);
+        if (aj == e0 || aj == e1 || aj == e2 || aj == e3) return true;
+        for (int k = r0 + 4; k < r1; k++)
+            if (sym_list[k] == aj) return true;
+        return false;
+    };
     int count = 0;
     long npairs = 0;
     // the next group's j record is loaded while this group is tested
@@ -408,7 +428,7 @@ __device__ __forceinline__ void nl_pairs_body(
         if (jc >= 0) {
             int aj = (int)pj.w;
             bool valid = ai >= 0 && aj >= 0 && (jc != ic || jj != ii);
-            if (valid && nl_dr2(pi, pj, crx, cry, crz) < rl2) bit = !excluded_pair(ai, aj, es, en, el);
+            if (valid && nl_dr2(pi, pj, crx, cry, crz) < rl2) bit = !is_excluded(aj);
         }
         unsigned long long bal = __ballot(bit);
         unsigned m = (unsigned)(bal >> (g * 16)) & 0xffffu;
@@ -439,9 +459,9 @@ __global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
     float rlm, const int* __restrict__ col_start, const float4* __restrict__ cl_lo,
     const float4* __restrict__ cl_hi, const uint4* __restrict__ pack, const int* __restrict__ es,
     const int* __restrict__ en, const int* __restrict__ el, int list_cap, int* __restrict__ list_n,
-    uint2* __restrict__ list, int* flags) {
+    uint2* __restrict__ list, int* flags, const int* __restrict__ sym_start, const int* __restrict__ sym_list) {
     nl_pairs_body(xcd_block(gridDim.x), ncx, ncy, bx, by, bz, crx, cry, crz, rl2, rlm, col_start, cl_lo, cl_hi, pack,
-                  es, en, el, list_cap, list_n, list, flags);
+                  es, en, el, list_cap, list_n, list, flags, sym_start, sym_list);
 }
 
 // ---- batched rebuild: one launch per stage over every system --------------
@@ -624,7 +644,7 @@ extern "C" void sponge_nl_destroy(sponge_nl* nl) {
     if (!nl) return;
     void* ptrs[] = {nl->old_crd, nl->nl_uint, nl->col_count, nl->col_start, nl->col_atoms, nl->s2o,
                     nl->o2s,     nl->cl_lo,   nl->cl_hi,     nl->list_n,    nl->list,      nl->pack,
-                    nl->flags,   nl->list_in, nl->list_in_n};
+                    nl->flags,   nl->list_in, nl->list_in_n, nl->sym_start, nl->sym_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete nl;
@@ -656,8 +676,55 @@ static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el,
     nl_pairs_kernel<<<nblk, kPairWaves * 64, 0, s>>>(
         nl->ncx, nl->ncy, nl->box[0], nl->box[1], nl->box[2], nl->cof_dr[0], nl->cof_dr[1], nl->cof_dr[2], nl->rl2,
         rlm, nl->col_start, nl->cl_lo, nl->cl_hi, nl->pack, es, en, el, nl->list_cap, nl->list_n, nl->list,
-        nl->flags);
+        nl->flags, nl->sym_start, nl->sym_list);
     SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+// Symmetric exclusion lists from the reference's CSR (each excluded pair
+// listed once, in the row of one of its atoms; MD_core/MD_core.cu:284-483):
+// built on the host the first time these CSR arrays are passed (one
+// synchronising copy), reused while the same arrays come back.
+static int nl_sym_exclusions(sponge_nl* nl, const int* es, const int* el, const int* en) {
+    if (es == nl->ex_es && el == nl->ex_el && en == nl->ex_en && nl->sym_start) return SPONGE_NB_OK;
+    const int n = nl->n;
+    std::vector<int> hs(n), hn(n);
+    SNB_HIP(hipDeviceSynchronize());
+    SNB_HIP(hipMemcpy(hs.data(), es, n * sizeof(int), hipMemcpyDeviceToHost));
+    SNB_HIP(hipMemcpy(hn.data(), en, n * sizeof(int), hipMemcpyDeviceToHost));
+    long end = 0;
+    for (int i = 0; i < n; i++) {
+        SNB_ARG(hn[i] >= 0 && hs[i] >= 0, "exclusion CSR: negative start or count");
+        end = std::max(end, (long)hs[i] + hn[i]);
+    }
+    std::vector<int> hl(std::max(end, 1L));
+    if (end > 0) SNB_HIP(hipMemcpy(hl.data(), el, end * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> start(n + 1, 0);
+    for (int i = 0; i < n; i++)
+        for (int k = hs[i]; k < hs[i] + hn[i]; k++) {
+            SNB_ARG(hl[k] >= 0 && hl[k] < n, "exclusion CSR: partner index out of range");
+            start[i + 1]++;
+            start[hl[k] + 1]++;
+        }
+    for (int i = 0; i < n; i++) start[i + 1] += start[i];
+    std::vector<int> pos(start.begin(), start.end() - 1), sym(std::max(start[n], 1));
+    for (int i = 0; i < n; i++)
+        for (int k = hs[i]; k < hs[i] + hn[i]; k++) {
+            sym[pos[i]++] = hl[k];
+            sym[pos[hl[k]]++] = i;
+        }
+    (void)hipFree(nl->sym_start);
+    (void)hipFree(nl->sym_list);
+    nl->sym_start = nullptr;
+    nl->sym_list = nullptr;
+    nl->ex_es = nl->ex_el = nl->ex_en = nullptr;
+    SNB_HIP(hipMalloc((void**)&nl->sym_start, (n + 1) * sizeof(int)));
+    SNB_HIP(hipMalloc((void**)&nl->sym_list, sym.size() * sizeof(int)));
+    SNB_HIP(hipMemcpy(nl->sym_start, start.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+    SNB_HIP(hipMemcpy(nl->sym_list, sym.data(), sym.size() * sizeof(int), hipMemcpyHostToDevice));
+    nl->ex_es = es;
+    nl->ex_el = el;
+    nl->ex_en = en;
     return SPONGE_NB_OK;
 }
 
@@ -666,6 +733,10 @@ extern "C" int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_exclud
                                 int forced_check, void* stream) {
     SNB_ARG(nl && d_crd && d_excluded_list_start && d_excluded_list && d_excluded_numbers, "nl_update");
     hipStream_t s = as_stream(stream);
+    {
+        const int rc = nl_sym_exclusions(nl, d_excluded_list_start, d_excluded_list, d_excluded_numbers);
+        if (rc) return rc;
+    }
     if (forced_update) {
         return nl_rebuild(nl, d_crd, d_excluded_list_start, d_excluded_list, d_excluded_numbers, 1, s);
     }

@@ -172,7 +172,8 @@ __device__ __forceinline__ void pair_body(
     int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
     float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
-    const float4* __restrict__ sc, PruneArgs pa = PruneArgs{0, 0.f, nullptr, nullptr}) {
+    const float4* __restrict__ sc, PruneArgs pa = PruneArgs{0, 0.f, nullptr, nullptr},
+    ExclArgs ex = ExclArgs{nullptr, nullptr, nullptr}) {
     extern __shared__ float2 sab[];
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
     __syncthreads();
@@ -300,6 +301,31 @@ __device__ __forceinline__ void pair_body(
         }
     }
     if (write_inner && lane == 0) pa.in_n[ic] = inner_count;
+    if (ex.start) {
+        // PME_Excluded_Force_With_Atom_Energy's force (PME_force.cu:620-695)
+        // for this cluster's atoms: lane (ii, k) takes the k-th exclusion
+        // partner of atom ii (16 per pass) from the step's slot-ordered
+        // records; each atom collects its own side of every excluded pair, so
+        // no other atom's force is written.
+        const int at = s2o[islot];
+        if (at >= 0) {
+            const int r1 = ex.start[at + 1];
+            for (int k = ex.start[at] + (lane >> 2); k < r1; k += 16) {
+                const uint4 b = pk[ex.o2s[ex.list[k]]];
+                const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+                const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
+                const float dz = (float)(int)((b.z & ~3u) - az) * cz;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float r = sqrtf(r2);
+                const float br = beta * r;
+                const float qq = qi * __uint_as_float(b.w);
+                const float fa = -qq * ((kTwoOverSqrtPi * br * expf(-br * br) + erfcf(br) - 1.0f) / r2 / r);
+                fix += fa * dx;
+                fiy += fa * dy;
+                fiz += fa * dz;
+            }
+        }
+    }
     // i force: sum the 16 lanes sharing ii (lanes ii, ii+4, ..., ii+60).
 #pragma unroll
     for (int o = 4; o < 64; o <<= 1) {
@@ -354,9 +380,9 @@ __global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
     const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz, float rc2,
     float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
-    const float4* __restrict__ sc, PruneArgs pa) {
+    const float4* __restrict__ sc, PruneArgs pa, ExclArgs ex) {
     pair_body<ENERGY, SOFT, TB>(xcd_block(gridDim.x), flags, list_n, list, list_cap, pk, ab, nt, cx, cy, cz, rc2, beta,
-                                s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc, pa);
+                                s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc, pa, ex);
 }
 
 // Batched: logical blocks are spread over the XCDs as in the single-system
@@ -678,7 +704,7 @@ void snb_scratch_free(PairScratch* ps) {
 int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, float cutoff, const float* cof,
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered,
-              int overwrite, const float4* sc, int inner) {
+              int overwrite, const float4* sc, int inner, ExclArgs ex) {
     inner = inner && nl->prune_interval > 0;
     const PruneArgs pa = inner ? PruneArgs{nl->prune_interval, nl->prune_r2, nl->list_in_n, nl->list_in}
                                : PruneArgs{0, 0.f, nullptr, nullptr};
@@ -705,7 +731,7 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     pair_kernel<E, S, TB><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, list_n, list, nl->list_cap, ps->pk, ab,   \
                                                          nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,         \
                                                          E ? atom_e : nullptr, E ? atom_v : nullptr,                    \
-                                                         E ? atom_d : nullptr, ps->acc, overwrite, sc, pa)
+                                                         E ? atom_d : nullptr, ps->acc, overwrite, sc, pa, ex)
     if (energy) {
         if (sc) SNB_PAIR(true, true);
         else SNB_PAIR(true, false);
