@@ -121,7 +121,7 @@ struct PairScratch {
     hipEvent_t ev_start, ev_stop;  // optional, recorded around the pair kernel
 };
 
-// Mixed-radix pass list of one FFT axis (radix 4 first, then 2, 3, 5, 7),
+// Mixed-radix pass list of one FFT axis (radix 8 first, then 4, 2, 3, 5, 7),
 // radix of pass p in bits 4p..4p+3 of `code`: a scalar, so that kernels
 // never index an array with the pass number (which would put the plan in
 // scratch memory).

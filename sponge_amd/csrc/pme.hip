@@ -18,7 +18,7 @@
 //   inv_z         one block per (x plane, 16 y rows): Hermitian repack and
 //                 paired complex z-IFFT (C2R), write the convolved grid
 //   pme_gather    16 lanes per atom: force -q sum grid * dtheta (K/L)
-// All FFT passes are mixed-radix (2,3,4,5,7) Stockham steps over lines held
+// All FFT passes are mixed-radix (8,4,2,3,5,7) Stockham steps over lines held
 // in LDS, each line owned by one wave for all of its passes.
 #include <hip/hip_runtime.h>
 
@@ -52,9 +52,16 @@ constexpr int kZPer = 17;
 constexpr int kGatherThreads = 256;
 
 __host__ __device__ __forceinline__ int row_blocks(int Ky) { return (Ky + kRowBlock - 1) / kRowBlock; }
-// LDS pitch of one x row of the convolution plane: odd, so that the x-FFT's
-// element stride (the pitch) spreads a wave's accesses over the banks
-__host__ __device__ __forceinline__ int conv_pitch(int Ky) { return Ky | 1; }
+// LDS layout of transform lines. Stockham passes write each line at strides
+// of the radix (positions R j + q), which on a plain layout puts a wave's
+// stores on a few banks; one pad slot after every 16 positions, pd(p), and
+// line pitches of 1 mod 16 (in 8-byte elements) spread them over all banks.
+// Both indices of a 2D plane are padded, so rows and columns stay lines.
+__host__ __device__ __forceinline__ int pd(int p) { return p + (p >> 4); }
+__host__ __device__ __forceinline__ int padded_len(int K) { return pd(K - 1) + 1; }
+__host__ __device__ __forceinline__ int pitch16(int n) { return ((n + 14) / 16) * 16 + 1; }
+// pitch of the packed z-lines and of the y direction of the convolution plane
+__host__ __device__ __forceinline__ int line_pitch(int K) { return pitch16(padded_len(K)); }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -81,6 +88,25 @@ __device__ __forceinline__ void dft(float2* v) {
         v[2] = csub(s02, s13);
         v[1] = cadd(d02, d13);
         v[3] = csub(d02, d13);
+    } else if (R == 8) {
+        // two radix-4 transforms of the even and odd elements, then the
+        // eighth-root twiddles exp(SIGN i pi k / 4)
+        const float c = 0.70710678118654752f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        dft<4, SIGN>(e);
+        dft<4, SIGN>(o);
+        const float2 o1 = cmul(o[1], make_float2(c, SIGN * c));
+        const float2 o2 = mul_i<SIGN>(o[2]);
+        const float2 o3 = cmul(o[3], make_float2(-c, SIGN * c));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
     } else if (R == 3) {
         const float c = -0.5f, s = 0.86602540378443865f;
         float2 t = cadd(v[1], v[2]);
@@ -144,7 +170,7 @@ __device__ __forceinline__ void wave_pass(float2* buf, int nlines, int ls, int k
                 float2 v[R];
 #pragma unroll
                 for (int q = 0; q < R; q++) {
-                    float2 x = buf[l * ls + (j + q * per) * ks];
+                    float2 x = buf[pd(l) * ls + pd(j + q * per) * ks];
                     if (q > 0) {
                         float2 t = tw[q * k * tws];
                         if (SIGN > 0) t.y = -t.y;
@@ -154,7 +180,7 @@ __device__ __forceinline__ void wave_pass(float2* buf, int nlines, int ls, int k
                 }
                 dft<R, SIGN>(v);
 #pragma unroll
-                for (int q = 0; q < R; q++) buf[l * ls + (d + q * Ns) * ks] = v[q];
+                for (int q = 0; q < R; q++) buf[pd(l) * ls + pd(d + q * Ns) * ks] = v[q];
             }
         }
     } else {
@@ -168,7 +194,7 @@ __device__ __forceinline__ void wave_pass(float2* buf, int nlines, int ls, int k
                     const int k = j - div_small(j, rns) * Ns;
 #pragma unroll
                     for (int q = 0; q < R; q++) {
-                        float2 x = buf[l * ls + (j + q * per) * ks];
+                        float2 x = buf[pd(l) * ls + pd(j + q * per) * ks];
                         if (q > 0) {
                             float2 t = tw[q * k * tws];
                             if (SIGN > 0) t.y = -t.y;
@@ -186,7 +212,7 @@ __device__ __forceinline__ void wave_pass(float2* buf, int nlines, int ls, int k
                     const int jn = div_small(j, rns);
                     const int d = jn * Ns * R + (j - jn * Ns);
 #pragma unroll
-                    for (int q = 0; q < R; q++) buf[l * ls + (d + q * Ns) * ks] = v[c][q];
+                    for (int q = 0; q < R; q++) buf[pd(l) * ls + pd(d + q * Ns) * ks] = v[c][q];
                 }
             }
         }
@@ -203,6 +229,7 @@ __device__ __forceinline__ void lds_fft(float2* buf, int nlines, int ls, int ks,
     for (int ps = 0; ps < p.npass; ps++) {
         const int R = (int)((p.code >> (4 * ps)) & 15ull);
         switch (R) {
+            case 8: wave_pass<8, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 8; break;
             case 4: wave_pass<4, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 4; break;
             case 2: wave_pass<2, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 2; break;
             case 3: wave_pass<3, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 3; break;
@@ -281,7 +308,7 @@ __device__ __forceinline__ void spread_z_body(
     const float* __restrict__ scale_ptr = nullptr) {
     extern __shared__ float2 sm[];
     const float unit = MODE ? scale_ptr[0] : kSpreadScale;
-    const int H = Kz / 2 + 1, zp = Kz + 1;  // odd pitch: a wave's lines fall on different banks
+    const int H = Kz / 2 + 1, zp = line_pitch(Kz);
     const int y0 = yb * kRowBlock, nr = min(kRowBlock, Ky - y0), nrp = (nr + 1) / 2;
     unsigned long long* qacc = reinterpret_cast<unsigned long long*>(sm);  // nr x Kz
     float2* pk = sm + kRowBlock * Kz;                                       // nrp lines at pitch zp
@@ -388,10 +415,10 @@ __device__ __forceinline__ void spread_z_body(
     float* pkf = reinterpret_cast<float*>(pk);
     for (int idx = threadIdx.x; idx < npt; idx += kZThreads) {
         const int r = div_small(idx, rkz), z = idx - r * Kz;
-        pkf[2 * ((r >> 1) * zp + z) + (r & 1)] = (float)((double)(long long)qacc[idx] / (double)unit);
+        pkf[2 * ((r >> 1) * zp + pd(z)) + (r & 1)] = (float)((double)(long long)qacc[idx] / (double)unit);
     }
     if (nr & 1)
-        for (int z = threadIdx.x; z < Kz; z += kZThreads) pkf[2 * ((nr >> 1) * zp + z) + 1] = 0.f;
+        for (int z = threadIdx.x; z < Kz; z += kZThreads) pkf[2 * ((nr >> 1) * zp + pd(z)) + 1] = 0.f;
     __syncthreads();
     lds_fft<-1, kZThreads>(pk, nrp, zp, 1, pz, tz);
     __syncthreads();
@@ -401,8 +428,8 @@ __device__ __forceinline__ void spread_z_body(
     for (int idx = threadIdx.x; idx < nr * H; idx += kZThreads) {
         const int k = div_small(idx, rnr), r = idx - k * nr;
         const int p = r >> 1;
-        const float2 zk = pk[p * zp + k];
-        const float2 zm = pk[p * zp + (k == 0 ? 0 : Kz - k)];
+        const float2 zk = pk[p * zp + pd(k)];
+        const float2 zm = pk[p * zp + pd(k == 0 ? 0 : Kz - k)];
         const float2 v = (r & 1) == 0 ? make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y))
                                       : make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
         FQ[((size_t)k * Kx + x) * Ky + y0 + r] = v;
@@ -439,8 +466,8 @@ __global__ __launch_bounds__(kZThreads) void spread_z_batch(const BatchSys* __re
                   b.pz, P.d_tw[2], energy ? P.acc : nullptr);
 }
 
-// One block per kz: the Kx x Ky plane of the spectrum in LDS (x rows at an
-// odd pitch), FFT along y then x, energy 1/2 sum m BC |F|^2 (FP64; m = 2 for
+// One block per kz: the Kx x Ky plane of the spectrum in LDS (padded rows
+// and columns), FFT along y then x, energy 1/2 sum m BC |F|^2 (FP64; m = 2 for
 // the kz that stand for their conjugates too), multiply by BC, inverse FFT
 // along x then y.
 __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, const float* __restrict__ bc, int Kx,
@@ -449,8 +476,8 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
                                             double* __restrict__ acc, int* __restrict__ bin_count) {
     extern __shared__ float2 sm[];
     __shared__ double red[kConvThreads / 64];
-    const int pitch = conv_pitch(Ky);
-    float2* tx = sm + Kx * pitch;
+    const int pitch = line_pitch(Ky);
+    float2* tx = sm + padded_len(Kx) * pitch;
     float2* ty = tx + Kx;
     // the spread has consumed the x-plane bins: clear them for the next prep
     if (kz == 0)
@@ -482,7 +509,7 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
             const int idx = s0 + s * kConvThreads;
             if (idx < np) {
                 const int xx = div_small(idx, rky), y = idx - xx * Ky;
-                sm[xx * pitch + y] = v[s];
+                sm[pd(xx) * pitch + pd(y)] = v[s];
             }
         }
     }
@@ -506,9 +533,9 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
             if (idx < np) {
                 const int xx = div_small(idx, rky), y = idx - xx * Ky;
                 const float b = bv[s];
-                const float2 w = sm[xx * pitch + y];
+                const float2 w = sm[pd(xx) * pitch + pd(y)];
                 if (energy) e += (double)b * ((double)w.x * w.x + (double)w.y * w.y);
-                sm[xx * pitch + y] = make_float2(w.x * b, w.y * b);
+                sm[pd(xx) * pitch + pd(y)] = make_float2(w.x * b, w.y * b);
             }
         }
     }
@@ -519,7 +546,7 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
     __syncthreads();
     for (int idx = threadIdx.x; idx < np; idx += kConvThreads) {
         const int xx = div_small(idx, rky), y = idx - xx * Ky;
-        plane[idx] = sm[xx * pitch + y];
+        plane[idx] = sm[pd(xx) * pitch + pd(y)];
     }
     if (energy) {
         const double t = block_sum<kConvThreads>(e, red);
@@ -554,7 +581,7 @@ __global__ __launch_bounds__(kConvThreads) void conv2d_batch(const BatchSys* __r
 __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
                                            int Ky, int Kz, const FftPlan& pz, const float2* __restrict__ twz) {
     extern __shared__ float2 sm[];
-    const int H = Kz / 2 + 1, zp = Kz + 1;
+    const int H = Kz / 2 + 1, zp = line_pitch(Kz);
     const int y0 = yb * kRowBlock, nr = min(kRowBlock, Ky - y0), nrp = (nr + 1) / 2;
     float2* rows = sm;                      // nr x H
     float2* pk = sm + kRowBlock * H;        // nrp lines at pitch zp
@@ -594,7 +621,7 @@ __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restri
             A.y = 0.f;
             B.y = 0.f;
         }
-        pk[p * zp + k] = make_float2(A.x - B.y, A.y + B.x);
+        pk[p * zp + pd(k)] = make_float2(A.x - B.y, A.y + B.x);
     }
     __syncthreads();
     lds_fft<1, kZThreads>(pk, nrp, zp, 1, pz, tz);
@@ -602,7 +629,7 @@ __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restri
     float* Rp = R + ((size_t)x * Ky + y0) * Kz;
     for (int idx = threadIdx.x; idx < nrp * Kz; idx += kZThreads) {
         const int p = div_small(idx, rkz), z = idx - p * Kz;
-        const float2 v = pk[p * zp + z];
+        const float2 v = pk[p * zp + pd(z)];
         Rp[(2 * p) * Kz + z] = v.x;
         if (2 * p + 1 < nr) Rp[(2 * p + 1) * Kz + z] = v.y;
     }
@@ -809,20 +836,26 @@ int plan_axis(int K, FftPlan* p) {
         p->npass++;
         m /= r;
     };
+    while (m % 8 == 0 && p->npass < 16) push(8);
     while (m % 4 == 0 && p->npass < 16) push(4);
     for (int r : {2, 3, 5, 7})
         while (m % r == 0 && p->npass < 16) push(r);
-    return m == 1 ? 0 : -1;
+    if (m != 1) return -1;
+    // every pass's butterflies (K / R of them) must fit the two lanes per
+    // element of the long-line path
+    for (int ps = 0; ps < p->npass; ps++)
+        if (K / (int)((p->code >> (4 * ps)) & 15ull) > 128) return -1;
+    return 0;
 }
 
 // Dynamic LDS of the three transform kernels (bytes).
 size_t spread_z_lds(int Kz) {
-    return (size_t)(kRowBlock * Kz + (kRowBlock / 2) * (Kz + 1) + Kz) * sizeof(float2);
+    return (size_t)(kRowBlock * Kz + (kRowBlock / 2) * line_pitch(Kz) + Kz) * sizeof(float2);
 }
 size_t inv_z_lds(int Kz) {
-    return (size_t)(kRowBlock * (Kz / 2 + 1) + (kRowBlock / 2) * (Kz + 1) + Kz) * sizeof(float2);
+    return (size_t)(kRowBlock * (Kz / 2 + 1) + (kRowBlock / 2) * line_pitch(Kz) + Kz) * sizeof(float2);
 }
-size_t conv2d_lds(int Kx, int Ky) { return (size_t)(Kx * conv_pitch(Ky) + Kx + Ky) * sizeof(float2); }
+size_t conv2d_lds(int Kx, int Ky) { return (size_t)(padded_len(Kx) * line_pitch(Ky) + Kx + Ky) * sizeof(float2); }
 
 // Lets every transform kernel use up to `bytes` of dynamic LDS (once per
 // size; the attribute is per function).
@@ -931,7 +964,9 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     FftPlan p;
     for (int d = 0; d < 3; d++) {
         if (pme->K[d] < 4 || plan_axis(pme->K[d], &p) != 0) {
-            set_error("PME grid size %d must be >= 4 with prime factors 2,3,5,7 only", pme->K[d]);
+            set_error("PME grid size %d unsupported: it must be >= 4 with prime factors 2, 3, 5, 7 only, and "
+                      "K / r <= 128 for each of its radix-r passes (8 and 4 first)",
+                      pme->K[d]);
             delete pme;
             return SPONGE_NB_EARG;
         }

@@ -93,3 +93,17 @@ def test_lj_tables_lorentz_berthelot():
     assert B[0, 0] / 6 == pytest.approx(4 * eps * sig ** 6, rel=1e-6)
     assert A[1, 1] == 0 and B[0, 1] == 0
     assert np.array_equal(A, A.T)
+
+
+def test_pme_rejects_unsupported_fft_size():
+    """A grid whose radix passes exceed the in-LDS line kernels is refused at
+    creation (before any device call) instead of being transformed wrongly."""
+    from sponge_amd import nb
+    if not os.path.exists(nb.LIB_PATH):
+        pytest.skip("libsponge_nb.so not built")
+    lib = nb.load_library()
+    h = ctypes.c_void_p()
+    box = (ctypes.c_float * 3)(400.0, 400.0, 400.0)
+    rc = lib.sponge_pme_create(1000, box, 10.0, 1e-5, 384, 384, 384, ctypes.byref(h))
+    assert rc == -1
+    assert b"384" in lib.sponge_nb_last_error()
