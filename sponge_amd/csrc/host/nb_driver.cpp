@@ -1,21 +1,30 @@
-// sponge_nb_driver: the nonbonded part of SPONGE's step loop (main.cu:33-46,
-// 48-79, 162-194, 380-392) written against the C++ mirror classes, so the
-// call order a SPONGE build would keep is exercised end to end. Coordinates
-// do not move (no integrator); each step converts coordinates to the uint
-// grid, clears the force, runs the LJ + PME direct pair force, the
+// sponge_nb_driver: SPONGE's step loop (main.cu:33-46, 48-79, 162-194,
+// 282-320, 376-434) over the C++ mirror classes, so the call order a SPONGE
+// build keeps is exercised end to end. Each step converts the coordinates to
+// the uint grid, clears the force, runs the LJ + PME direct pair force, the
 // long-range correction, the excluded-pair correction and the reciprocal
-// force, then updates the neighbor list as Main_Iteration does.
+// force; with a molecular-dynamics mode it then remembers the constrained
+// bond vectors, takes the leap-frog step and applies SETTLE (NVE); finally it
+// updates the neighbor list as Main_Iteration does and writes the outputs of
+// Main_Print. Without `mode` or `mass_in_file` the coordinates stay fixed.
 //
-// Inputs (mdin keys, as MD_core.cu:257-298, 308-345, 864-866, 1053-1080,
-// 1212-1256 read them): coordinate_in_file, charge_in_file, exclude_in_file,
-// LJ_in_file, cutoff, skin, step_limit, write_mdout_interval,
-// neighbor_list_*, PME_*. Extra keys of this driver: frc_out_file (forces
-// after the first step: a line `N`, then N lines `fx fy fz`).
+// Inputs (mdin keys, as MD_core.cu:257-298, 308-345, 494-551, 864-1049,
+// 1053-1080, 1212-1256 read them): coordinate_in_file, velocity_in_file,
+// mass_in_file, charge_in_file, exclude_in_file, LJ_in_file,
+// constrain_in_file (count, then `i j r`; constrain.cu:27-64), mode (NVE),
+// dt (ps), cutoff, skin, step_limit, write_mdout_interval,
+// write_information_interval, write_restart_file_interval, crd, box, rst,
+// neighbor_list_*, PME_*. Outputs: mdcrd.dat (float32 xyz frames,
+// MD_core.cu:1349-1360), mdbox.txt (MD_core.cu:1402-1412) and
+// <rst>_coordinate.txt / <rst>_velocity.txt (MD_core.cu:1466-1487). Extra key
+// of this driver: frc_out_file (forces after the first step: a line `N`,
+// then N lines `fx fy fz`).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "sponge_host.h"
@@ -39,8 +48,9 @@ static FILE* open_or_die(const char* path, const char* mode) {
 struct MdInputs {
     int atom_numbers = 0;
     VECTOR box{0, 0, 0};
-    std::vector<VECTOR> crd;
-    std::vector<float> charge;
+    double start_time = 0.0;
+    std::vector<VECTOR> crd, vel;
+    std::vector<float> charge, mass;
     std::vector<int> ex_start, ex_numbers, ex_list;
 };
 
@@ -65,7 +75,7 @@ static void read_inputs(CONTROLLER& c, MdInputs& in) {
         FILE* fp = open_or_die(c.Command("coordinate_in_file"), "r");
         char line[1024];
         int n = 0;
-        if (!std::fgets(line, sizeof(line), fp) || std::sscanf(line, "%d", &n) != 1) n = 0;
+        if (!std::fgets(line, sizeof(line), fp) || std::sscanf(line, "%d %lf", &n, &in.start_time) < 1) n = 0;
         check_count(in, n, "coordinate_in_file");
         in.crd.resize(n);
         for (int i = 0; i < n; i++)
@@ -77,6 +87,33 @@ static void read_inputs(CONTROLLER& c, MdInputs& in) {
             std::fprintf(stderr, "Error: box_length not found in coordinate_in_file\n");
             std::exit(1);
         }
+        std::fclose(fp);
+    }
+    in.vel.assign(in.atom_numbers, VECTOR{0.f, 0.f, 0.f});
+    if (c.Command_Exist("velocity_in_file")) {
+        FILE* fp = open_or_die(c.Command("velocity_in_file"), "r");
+        char line[1024];
+        int n = 0;
+        if (!std::fgets(line, sizeof(line), fp) || std::sscanf(line, "%d", &n) != 1) n = 0;
+        check_count(in, n, "velocity_in_file");
+        for (int i = 0; i < n; i++)
+            if (std::fscanf(fp, "%f %f %f", &in.vel[i].x, &in.vel[i].y, &in.vel[i].z) != 3) {
+                std::fprintf(stderr, "Error: velocity_in_file ends early\n");
+                std::exit(1);
+            }
+        std::fclose(fp);
+    }
+    in.mass.assign(in.atom_numbers, 20.f);  // MD_core.cu:1025-1037 default
+    if (c.Command_Exist("mass_in_file")) {
+        FILE* fp = open_or_die(c.Command("mass_in_file"), "r");
+        int n = 0;
+        if (std::fscanf(fp, "%d", &n) != 1) n = 0;
+        check_count(in, n, "mass_in_file");
+        for (int i = 0; i < n; i++)
+            if (std::fscanf(fp, "%f", &in.mass[i]) != 1 || in.mass[i] < 0.f) {
+                std::fprintf(stderr, "Error: mass_in_file ends early or holds a negative mass\n");
+                std::exit(1);
+            }
         std::fclose(fp);
     }
     in.charge.assign(in.atom_numbers, 0.f);
@@ -144,6 +181,22 @@ int main(int argc, char** argv) {
     if (controller.Command_Exist("write_mdout_interval"))
         mdout_interval = std::atoi(controller.Command("write_mdout_interval"));
     if (mdout_interval <= 0) mdout_interval = 1;
+    // MD_INFORMATION::Read_Mode / Read_dt (MD_core.cu:494-551): this build
+    // integrates NVE only
+    const bool md = controller.Command_Exist("mode") || controller.Command_Exist("mass_in_file");
+    if (controller.Command_Exist("mode") && std::strcmp(controller.Command("mode"), "NVE") != 0) {
+        std::fprintf(stderr, "Error: mode %s is not provided here (NVE only)\n", controller.Command("mode"));
+        return 1;
+    }
+    double dt_ps = 0.001;
+    if (controller.Command_Exist("dt")) dt_ps = std::atof(controller.Command("dt"));
+    const float dt = (float)dt_ps * 20.455f;
+    int info_interval = 1000;
+    if (controller.Command_Exist("write_information_interval"))
+        info_interval = std::atoi(controller.Command("write_information_interval"));
+    int restart_interval = info_interval;
+    if (controller.Command_Exist("write_restart_file_interval"))
+        restart_interval = std::atoi(controller.Command("write_restart_file_interval"));
     if (controller.Command_Exist("check_input")) {  // parse and report only; no device call
         for (auto& kv : controller.commands) controller.printf("command %s = %s\n", kv.first.c_str(), kv.second.c_str());
         double qsum = 0;
@@ -154,6 +207,12 @@ int main(int argc, char** argv) {
     }
 
     VECTOR* crd = to_device(in.crd);
+    VECTOR* vel = to_device(in.vel);
+    std::vector<float> inv_mass(n);
+    for (int i = 0; i < n; i++) inv_mass[i] = in.mass[i] == 0.f ? 0.f : 1.0f / in.mass[i];
+    float* d_mass = to_device(in.mass);
+    float* d_mass_inverse = to_device(inv_mass);
+    VECTOR* acc = to_device(std::vector<VECTOR>(n, VECTOR{0.f, 0.f, 0.f}));
     float* d_charge = to_device(in.charge);
     int* d_es = to_device(in.ex_start);
     int* d_en = to_device(in.ex_numbers);
@@ -176,6 +235,29 @@ int main(int argc, char** argv) {
     neighbor_list.Neighbor_List_Update(crd, d_es, d_el, d_en, NEIGHBOR_LIST::FORCED_UPDATE);
     lj.Initial(&controller, cutoff, in.box);
     pme.Initial(&controller, n, in.box, cutoff);
+    CONSTRAIN constrain;
+    SETTLE settle;
+    FILE* crd_traj = nullptr;
+    FILE* box_traj = nullptr;
+    if (md) {
+        int freedom = 3 * n;
+        constrain.Initial_Constrain(&controller, n, dt, in.box, 1.0f, 0, in.mass.data(), &freedom);
+        settle.Initial(&controller, &constrain, in.mass.data());
+        if (constrain.constrain_pair_numbers > 0) {
+            std::fprintf(stderr, "Error: %d constraints are not rigid triangles or pairs (SIMPLE_CONSTRAIN and "
+                                 "SHAKE are not provided here)\n",
+                         constrain.constrain_pair_numbers);
+            return 1;
+        }
+        if (info_interval > 0) {
+            crd_traj = open_or_die(controller.Command_Exist("crd") ? controller.Command("crd") : "mdcrd.dat", "wb");
+            box_traj = open_or_die(controller.Command_Exist("box") ? controller.Command("box") : "mdbox.txt", "w");
+        }
+    }
+    auto download = [&](VECTOR* d, std::vector<VECTOR>& h) {
+        h.resize(n);
+        hip_ok(hipMemcpy(h.data(), d, n * sizeof(VECTOR), hipMemcpyDeviceToHost), "hipMemcpy");
+    };
 
     auto t0 = std::chrono::steady_clock::now();
     for (int step = 1; step <= step_limit; step++) {
@@ -211,9 +293,47 @@ int main(int argc, char** argv) {
                               step, e_lj, e_pme, pme.reciprocal_ene, pme.self_ene, pme.direct_ene,
                               pme.correction_ene);
         }
+        if (md) {
+            // Main_Iteration (main.cu:282-320), NVE
+            settle.Remember_Last_Coordinates(uint_crd, scaler);
+            Leap_Frog(n, vel, crd, frc, acc, d_mass_inverse, dt, -1.0f);
+            settle.Do_SETTLE(d_mass, crd, in.box, vel, 0, nullptr);
+        }
         neighbor_list.Neighbor_List_Update(crd, d_es, d_el, d_en);
+        if (md && info_interval > 0 && step % info_interval == 0) {
+            // Main_Print's trajectory frames (MD_core.cu:1349-1360, 1402-1412)
+            std::vector<VECTOR> h;
+            download(crd, h);
+            std::fwrite(&h[0].x, sizeof(VECTOR), n, crd_traj);
+            std::fprintf(box_traj, "%f %f %f %.0f %.0f %.0f\n", in.box.x, in.box.y, in.box.z, 90.0f, 90.0f, 90.0f);
+        }
+        if (md && restart_interval > 0 && step % restart_interval == 0) {
+            // Export_Restart_File without an AMBER restart (MD_core.cu:1466-1487)
+            std::vector<VECTOR> hc, hv;
+            download(crd, hc);
+            download(vel, hv);
+            const char* rst = controller.Command_Exist("rst") ? controller.Command("rst") : "restart";
+            const double now = in.start_time + dt_ps * step;
+            char path[1100];
+            std::snprintf(path, sizeof(path), "%s_coordinate.txt", rst);
+            FILE* fc = open_or_die(path, "w");
+            std::snprintf(path, sizeof(path), "%s_velocity.txt", rst);
+            FILE* fv = open_or_die(path, "w");
+            std::fprintf(fc, "%d %.3lf\n", n, now);
+            std::fprintf(fv, "%d %.3lf\n", n, now);
+            for (int i = 0; i < n; i++) {
+                std::fprintf(fc, "%12.7f %12.7f %12.7f\n", hc[i].x, hc[i].y, hc[i].z);
+                std::fprintf(fv, "%12.7f %12.7f %12.7f\n", hv[i].x, hv[i].y, hv[i].z);
+            }
+            std::fprintf(fc, "%12.7f %12.7f %12.7f %12.7f %12.7f %12.7f", in.box.x, in.box.y, in.box.z, 90.0f, 90.0f,
+                         90.0f);
+            std::fclose(fc);
+            std::fclose(fv);
+        }
     }
     hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (crd_traj) std::fclose(crd_traj);
+    if (box_traj) std::fclose(box_traj);
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     long pairs = 0;
     int overflow = 0, rebuilds = 0;
@@ -221,10 +341,11 @@ int main(int argc, char** argv) {
     controller.printf("neighbor pairs %ld overflow %d rebuilds %d\n", pairs, overflow, rebuilds);
     controller.printf("Core Run Wall Time: %f second(s), %f steps/s\n", secs, step_limit / secs);
 
+    settle.Clear();
     pme.Clear();
     lj.Clear();
     neighbor_list.Clear();
-    for (void* p : {(void*)crd, (void*)d_charge, (void*)d_es, (void*)d_en, (void*)d_el, (void*)uint_crd, (void*)frc,
+    for (void* p : {(void*)vel, (void*)d_mass, (void*)d_mass_inverse, (void*)acc, (void*)crd, (void*)d_charge, (void*)d_es, (void*)d_en, (void*)d_el, (void*)uint_crd, (void*)frc,
                     (void*)d_atom_energy, (void*)d_atom_virial, (void*)d_scalars})
         (void)hipFree(p);
     return overflow ? 2 : 0;

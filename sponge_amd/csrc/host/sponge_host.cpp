@@ -457,3 +457,77 @@ void Particle_Mesh_Ewald::Clear() {
     d_direct_atom_energy = d_correction_atom_energy = nullptr;
     is_initialized = 0;
 }
+
+// ---- the step on either side of the force path --------------------------
+
+void CONSTRAIN::Initial_Constrain(CONTROLLER* controller, int n, float step, VECTOR box_length, float exp_gamma,
+                                  int is_Minimization, const float* atom_mass, int* system_freedom) {
+    (void)box_length;
+    atom_numbers = n;
+    dt = step;
+    dt_inverse = 1.0f / step;
+    v_factor = exp_gamma;
+    x_factor = 0.5f * (1.0f + exp_gamma);
+    if (is_Minimization) v_factor = 0.0f;
+    pairs.clear();
+    if (controller->Command_Exist(module_name, "in_file")) {
+        const char* path = controller->Command(module_name, "in_file");
+        FILE* fp = std::fopen(path, "r");
+        if (!fp) {
+            std::fprintf(stderr, "Error: cannot open %s\n", path);
+            std::exit(1);
+        }
+        int count = 0;
+        if (std::fscanf(fp, "%d", &count) != 1 || count < 0) {
+            std::fprintf(stderr, "Error: %s: constraint count not found\n", path);
+            std::exit(1);
+        }
+        for (int k = 0; k < count; k++) {
+            CONSTRAIN_PAIR p{};
+            if (std::fscanf(fp, "%d %d %f", &p.atom_i_serial, &p.atom_j_serial, &p.constant_r) != 3 ||
+                p.atom_i_serial < 0 || p.atom_i_serial >= n || p.atom_j_serial < 0 || p.atom_j_serial >= n) {
+                std::fprintf(stderr, "Error: %s: malformed constraint %d\n", path, k);
+                std::exit(1);
+            }
+            const float mi = atom_mass[p.atom_i_serial], mj = atom_mass[p.atom_j_serial];
+            p.constrain_k = mi * mj / (mi + mj) / x_factor;
+            pairs.push_back(p);
+        }
+        std::fclose(fp);
+    }
+    constrain_pair_numbers = (int)pairs.size();
+    h_constrain_pair = pairs.data();
+    if (system_freedom) *system_freedom -= constrain_pair_numbers;
+    controller->printf("    constrain pair number is %d\n", constrain_pair_numbers);
+}
+
+void SETTLE::Remember_Last_Coordinates(UNSIGNED_INT_VECTOR* uint_crd, VECTOR scaler) {
+    if (!is_initialized) return;
+    const float s[3] = {scaler.x, scaler.y, scaler.z};
+    sponge_host_check(sponge_settle_remember_last_coordinates(handle, reinterpret_cast<const unsigned*>(uint_crd), s,
+                                                              nullptr),
+                      "SETTLE::Remember_Last_Coordinates");
+}
+
+void SETTLE::Do_SETTLE(const float* d_mass, VECTOR* crd, VECTOR box_length, VECTOR* vel, int need_pressure,
+                       float* d_pressure) {
+    if (!is_initialized) return;
+    const float b[3] = {box_length.x, box_length.y, box_length.z};
+    sponge_host_check(sponge_settle_do(handle, d_mass, reinterpret_cast<float*>(crd), b,
+                                       reinterpret_cast<float*>(vel), need_pressure, d_pressure, nullptr),
+                      "SETTLE::Do_SETTLE");
+}
+
+void SETTLE::Clear() {
+    if (handle) sponge_settle_destroy(handle);
+    handle = nullptr;
+    is_initialized = 0;
+}
+
+void Leap_Frog(int atom_numbers, VECTOR* vel, VECTOR* crd, VECTOR* frc, VECTOR* acc, const float* d_mass_inverse,
+               float dt, float max_velocity) {
+    sponge_host_check(sponge_md_leap_frog(atom_numbers, reinterpret_cast<float*>(vel), reinterpret_cast<float*>(crd),
+                                          reinterpret_cast<const float*>(frc), reinterpret_cast<float*>(acc),
+                                          d_mass_inverse, dt, max_velocity, nullptr),
+                      "Leap_Frog");
+}

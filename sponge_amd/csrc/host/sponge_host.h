@@ -12,7 +12,9 @@
 #include <cstdio>
 #include <map>
 #include <string>
+#include <vector>
 
+#include "../../../include/sponge_md.h"
 #include "../../../include/sponge_nb.h"
 
 struct VECTOR {
@@ -137,3 +139,97 @@ struct Particle_Mesh_Ewald {
 // status 1, the reference's behaviour for fatal input errors without the
 // interactive getchar() pause.
 void sponge_host_check(int rc, const char* what);
+
+// ---- the step on either side of the force path --------------------------
+
+// constrain/constrain.cuh: one distance constraint, the reference's layout.
+struct CONSTRAIN_PAIR {
+    int atom_i_serial;
+    int atom_j_serial;
+    float constant_r;
+    float constrain_k;
+};
+
+// CONSTRAIN (constrain/constrain.cuh) for a host without SPONGE's own: the
+// pairs of <module>_in_file (a count, then `i j r` lines) and the thermostat
+// factors (constrain/constrain.cu:4-80). A SPONGE build keeps its own
+// CONSTRAIN, which SETTLE::Initial accepts as well.
+struct CONSTRAIN {
+    char module_name[256] = "constrain";
+    int atom_numbers = 0;
+    float dt = 0.f, dt_inverse = 0.f;
+    float v_factor = 1.f, x_factor = 1.f;
+    int constrain_pair_numbers = 0;
+    std::vector<CONSTRAIN_PAIR> pairs;
+    CONSTRAIN_PAIR* h_constrain_pair = nullptr;
+    void Initial_Constrain(CONTROLLER* controller, int atom_numbers, float dt, VECTOR box_length, float exp_gamma,
+                           int is_Minimization, const float* atom_mass, int* system_freedom);
+};
+
+// SETTLE (constrain/SETTLE.cuh:37-67) over sponge_settle_*: rigid triangles
+// and pairs found among the constraint pairs; the pairs it takes over are
+// removed from the constraint's host list, as SETTLE.cu:338-360 does (a
+// build that also runs SIMPLE_CONSTRAIN re-uploads that list).
+struct SETTLE {
+    char module_name[256] = "settle";
+    int is_initialized = 0;
+    int triangle_numbers = 0;
+    int pair_numbers = 0;
+    sponge_settle* handle = nullptr;
+
+    template <class CONSTRAIN_T>
+    void Initial(CONTROLLER* controller, CONSTRAIN_T* constrain, float* h_mass, const char* module_name = nullptr) {
+        if (module_name) std::snprintf(this->module_name, sizeof(this->module_name), "%s", module_name);
+        const int np = constrain->constrain_pair_numbers;
+        if (np <= 0) {
+            controller->printf("SETTLE IS NOT INITIALIZED\n\n");
+            return;
+        }
+        controller->printf("START INITIALIZING SETTLE:\n");
+        std::vector<int> ai(np), aj(np);
+        std::vector<float> r(np);
+        for (int k = 0; k < np; k++) {
+            ai[k] = constrain->h_constrain_pair[k].atom_i_serial;
+            aj[k] = constrain->h_constrain_pair[k].atom_j_serial;
+            r[k] = constrain->h_constrain_pair[k].constant_r;
+        }
+        sponge_host_check(sponge_settle_create(constrain->atom_numbers, np, ai.data(), aj.data(), r.data(), h_mass,
+                                               constrain->dt, constrain->v_factor, 0, &handle),
+                          "SETTLE::Initial");
+        sponge_host_check(sponge_settle_set_factors(handle, constrain->v_factor, constrain->x_factor),
+                          "SETTLE factors");
+        int rest = 0;
+        sponge_host_check(sponge_settle_counts(handle, &triangle_numbers, &pair_numbers, &rest), "SETTLE counts");
+        controller->printf("    rigid triangle numbers is %d\n", triangle_numbers);
+        controller->printf("    rigid pair numbers is %d\n", pair_numbers);
+        if (triangle_numbers + pair_numbers == 0) {
+            controller->printf("SETTLE IS NOT INITIALIZED\n\n");
+            return;
+        }
+        // keep the constraints whose first atom is in no rigid group
+        std::vector<char> grouped(constrain->atom_numbers, 0);
+        std::vector<int> tri(3 * (size_t)triangle_numbers), prs(2 * (size_t)pair_numbers);
+        sponge_host_check(sponge_settle_groups(handle, tri.data(), nullptr, prs.data(), nullptr), "SETTLE groups");
+        for (int a : tri) grouped[a] = 1;
+        for (int a : prs) grouped[a] = 1;
+        int kept = 0;
+        for (int k = 0; k < np; k++)
+            if (!grouped[constrain->h_constrain_pair[k].atom_i_serial])
+                constrain->h_constrain_pair[kept++] = constrain->h_constrain_pair[k];
+        constrain->constrain_pair_numbers = kept;
+        controller->printf("    remaining simple constrain pair numbers is %d\n", kept);
+        is_initialized = 1;
+        controller->printf("END INITIALIZING SETTLE\n\n");
+    }
+    // SETTLE.cu:494-505; scaler = box_length / 2^32
+    void Remember_Last_Coordinates(UNSIGNED_INT_VECTOR* uint_crd, VECTOR scaler);
+    // SETTLE.cu:507-525
+    void Do_SETTLE(const float* d_mass, VECTOR* crd, VECTOR box_length, VECTOR* vel, int need_pressure,
+                   float* d_pressure);
+    void Clear();
+};
+
+// MD_INFORMATION::NVE_iteration::Leap_Frog (MD_core/MD_core.cu:1535-1547)
+// on MD_INFORMATION's arrays.
+void Leap_Frog(int atom_numbers, VECTOR* vel, VECTOR* crd, VECTOR* frc, VECTOR* acc, const float* d_mass_inverse,
+               float dt, float max_velocity);

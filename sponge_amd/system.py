@@ -435,7 +435,8 @@ def fft_size(length: float) -> int:
 
 def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
                         write_mdout_interval: int = 1, refresh_interval: int = 0,
-                        extra: dict | None = None) -> str:
+                        extra: dict | None = None, velocities: np.ndarray | None = None,
+                        dt_ps: float | None = None) -> str:
     """Writes `sys_` as SPONGE text inputs plus an mdin and returns the mdin path.
 
     Formats are the ones MD_core.cu reads: coordinate_in_file (`N start_time`,
@@ -444,6 +445,11 @@ def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
     per atom `count j...`; :308-345) and LJ_in_file (`N T`, A and B of the
     T(T+1)/2 pairs before the x12 / x6 scaling, then N types;
     Lennard_Jones_force.cu:1076-1108).
+
+    With `velocities` the molecular-dynamics inputs follow: velocity_in_file
+    (`N`, then N lines; MD_core.cu:864-890), mass_in_file (`N`, then N masses;
+    :922-949), constrain_in_file (the rigid waters' constraint pairs: a count,
+    then `i j r`; constrain/constrain.cu:27-64), mode NVE and dt (ps).
     """
     import os
     os.makedirs(directory, exist_ok=True)
@@ -466,6 +472,24 @@ def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
         f.write(" ".join(f"{float(a) / 12.0:.17g}" for a in sys_.lj_A) + "\n")
         f.write(" ".join(f"{float(b) / 6.0:.17g}" for b in sys_.lj_B) + "\n")
         f.write("\n".join(str(int(t)) for t in sys_.lj_type) + "\n")
+    md_keys = {}
+    if velocities is not None:
+        v = np.asarray(velocities, np.float32).reshape(-1, 3)
+        with open(p("velocity.txt"), "w") as f:
+            f.write(f"{sys_.n}\n")
+            for x, y, z in v:
+                f.write(f"{x:.9g} {y:.9g} {z:.9g}\n")
+        with open(p("mass.txt"), "w") as f:
+            f.write(f"{sys_.n}\n")
+            f.write("\n".join(f"{m:.9g}" for m in sys_.mass.astype(np.float32)) + "\n")
+        pi, pj, pr = water_constraints(sys_)
+        with open(p("constrain.txt"), "w") as f:
+            f.write(f"{len(pi)}\n")
+            for a, b, r in zip(pi, pj, pr):
+                f.write(f"{int(a)} {int(b)} {float(r):.9g}\n")
+        md_keys = {"velocity_in_file": p("velocity.txt"), "mass_in_file": p("mass.txt"),
+                   "constrain_in_file": p("constrain.txt"), "mode": "NVE",
+                   "dt": 0.001 if dt_ps is None else dt_ps}
     keys = {
         "coordinate_in_file": p("coordinate.txt"),
         "charge_in_file": p("charge.txt"),
@@ -481,6 +505,7 @@ def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
     for axis, k in zip("xyz", sys_.fft):
         if k > 0:
             keys[f"PME_fft{axis}"] = k
+    keys.update(md_keys)
     keys.update(extra or {})
     mdin = p("mdin.txt")
     with open(mdin, "w") as f:

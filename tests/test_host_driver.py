@@ -5,7 +5,9 @@ input files and touches no device: they pin the mdin rules of control.cu
 (blocks, comments, commas, `main` / `comments` blocks, duplicate keys,
 default_in_file_prefix, command-line flags) and the file formats. The GPU test
 runs the SPONGE-ordered step loop on SPONGE-format files and checks the forces
-and the printed energies against the FP64 oracle.
+and the printed energies against the FP64 oracle, and its NVE mode (leap-frog
+with SETTLE) against the oracle's trajectory, with the trajectory and restart
+files it writes.
 """
 import os
 import re
@@ -92,6 +94,29 @@ def test_input_files_round_trip(tmp_path):
 
 
 @needs_driver
+def test_only_nve_is_accepted(tmp_path):
+    sys_ = system.config(0)
+    mdin = system.write_sponge_inputs(sys_, str(tmp_path), velocities=np.zeros((sys_.n, 3)),
+                                      extra={"mode": "NVT"})
+    r = run(["-mdin", mdin])
+    assert r.returncode != 0 and "NVE only" in r.stderr
+
+
+@needs_driver
+def test_md_inputs_parse(tmp_path):
+    sys_ = system.config(0)
+    v = system.rigid_velocities(sys_)
+    mdin = system.write_sponge_inputs(sys_, str(tmp_path), velocities=v, dt_ps=0.002,
+                                      extra={"check_input": 1})
+    r = run(["-mdin", mdin])
+    assert r.returncode == 0, r.stderr
+    c = commands(r.stdout)
+    assert c["mode"] == "NVE" and c["dt"] == "0.002"
+    lines = (tmp_path / "constrain.txt").read_text().split("\n")
+    assert int(lines[0]) == 3 * sys_.waters and lines[1].split()[:2] == ["0", "1"]
+
+
+@needs_driver
 def test_malformed_exclusions_are_rejected(tmp_path):
     sys_ = system.config(0)
     mdin = system.write_sponge_inputs(sys_, str(tmp_path), extra={"check_input": 1})
@@ -139,3 +164,47 @@ def test_driver_step_loop_matches_oracle(tmp_path, name):
         assert abs(e_rec - ref["e_rec"]) < 1e-5 * scale
         assert abs(e_lj - (ref["e_lj"] + ref["e_lrc"])) < 1e-5 * abs(ref["e_lj"]) + 1e-5
     assert "overflow 0" in r.stdout
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_driver_nve_matches_oracle(tmp_path):
+    """SPONGE's NVE loop on the 648-atom rigid-water box, 10 steps of 2 fs:
+    the restart coordinates after step 10 against the oracle's FP64
+    trajectory from the same velocity file (1e-4 Angstrom RMS, the bound of
+    the chaotic growth of FP32 rounding over 10 steps), every water rigid,
+    one trajectory frame per write_information_interval."""
+    import oracle
+    sys_ = system.config(0)
+    v = system.rigid_velocities(sys_)
+    print("[driver] nve water648 10 steps", flush=True)
+    mdin = system.write_sponge_inputs(sys_, str(tmp_path), step_limit=10, write_mdout_interval=5,
+                                      refresh_interval=20, velocities=v, dt_ps=0.002,
+                                      extra={"write_information_interval": 5,
+                                             "crd": str(tmp_path / "mdcrd.dat"),
+                                             "box": str(tmp_path / "mdbox.txt"),
+                                             "rst": str(tmp_path / "restart")})
+    r = run(["-mdin", mdin], timeout=100)
+    sys.stdout.write(r.stdout[-1500:])
+    assert r.returncode == 0, r.stderr
+    assert "rigid triangle numbers is 216" in r.stdout
+    frames = np.fromfile(tmp_path / "mdcrd.dat", dtype=np.float32).reshape(-1, sys_.n, 3)
+    assert frames.shape[0] == 2
+    assert len((tmp_path / "mdbox.txt").read_text().splitlines()) == 2
+    head = (tmp_path / "restart_coordinate.txt").read_text().splitlines()
+    assert head[0].split()[0] == str(sys_.n) and abs(float(head[0].split()[1]) - 0.02) < 1e-9
+    crd = np.loadtxt(tmp_path / "restart_coordinate.txt", skiprows=1, max_rows=sys_.n)
+    vel = np.loadtxt(tmp_path / "restart_velocity.txt", skiprows=1)
+    np.testing.assert_allclose(crd, frames[-1], atol=1e-6)
+    groups = oracle.settle_groups(sys_.n, *system.water_constraints(sys_), sys_.mass)
+    c_ref, v_ref, _ = oracle.md_run(sys_, v, 10, 0.002 * system.TIME_UNIT_PER_PS, groups)
+    L = sys_.box.astype(np.float64)
+    d = crd - c_ref
+    d -= L * np.round(d / L)
+    rms = float(np.sqrt((d ** 2).mean()))
+    print(f"    RMS |dx| {rms:.3e}, vel rel RMS "
+          f"{float(np.sqrt(((vel - v_ref) ** 2).sum() / (v_ref ** 2).sum())):.3e}", flush=True)
+    assert rms < 1e-4
+    x = crd.reshape(-1, 3, 3)
+    oh = np.linalg.norm((x[:, 1:] - x[:, :1]) - L * np.round((x[:, 1:] - x[:, :1]) / L), axis=2)
+    assert np.abs(oh - system.TIP3P_ROH).max() < 1e-4
