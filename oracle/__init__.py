@@ -54,6 +54,9 @@ def lib():
             "orc_settle_geometry": (None, [D, D, D, D, D, D, P]),
             "orc_settle_groups": (None, [I, I, P, P, P, P, P, P, P, P, P]),
             "orc_md_iteration": (D, [I, P, P, P, P, P, P, P, D, D, D, D, I, P, P, I, P, P]),
+            "orc_fep_windows": (None, [I, P, P, P, P, P, P, P, P, P, P, D, D, D, D, D, P, D, D, Lg, P,
+                                       I, P, I, P]),
+            "orc_excluded_cross": (D, [I, P, P, P, P, D, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -248,6 +251,47 @@ def fft3_c2r(c, Kz):
     out = np.zeros((Kx, Ky, Kz))
     lib().orc_fft3_c2r(Kx, Ky, Kz, _p(buf), _p(out))
     return out
+
+
+FEP_FIELDS = ["vdw_intersys", "vdw_intrasys", "coul_direct_intersys", "coul_direct_intrasys",
+              "dhdl_lj", "dhdl_direct", "dhdl_direct_exact"]
+
+
+def fep_windows(fs, uint_crd, pairs, beta, lambdas, charge=None, dq=None, lambda0=None, truncate_soft=True):
+    """FP64 FEP / TI pair terms of a FepSystem at every lambda window
+    (orc_fep_windows): array (nwin, 7) in FEP_FIELDS order. Window charges
+    are charge + (lambda - lambda0) dq (charge: fs.charge, lambda0: fs.lam).
+    truncate_soft=False drops the energy kernel's soft-core-radius cutoff."""
+    u = _c(uint_crd, np.uint32)
+    q = _c(fs.charge if charge is None else charge, np.float32)
+    lams = _c(np.atleast_1d(lambdas), np.float64)
+    out = np.zeros((len(lams), 7))
+    lib().orc_fep_windows(
+        fs.n, _p(u), _p(q), _p(None if dq is None else _c(dq, np.float32)), _p(_c(fs.lj_type, np.int32)),
+        _p(_c(fs.lj_type_B, np.int32)), _p(_c(fs.mask, np.int32)), _p(_c(fs.lj_A, np.float32)),
+        _p(_c(fs.lj_B, np.float32)), _p(_c(fs.lj_A_B, np.float32)), _p(_c(fs.lj_B_B, np.float32)),
+        float(fs.lam if lambda0 is None else lambda0), float(fs.sc_alpha), float(fs.sc_p), float(fs.sc_sigma),
+        float(fs.sc_sigma_min), _p(_c(fs.box, np.float32)), float(fs.cutoff), float(beta),
+        _c(pairs, np.int32).shape[0], _p(_c(pairs, np.int32)), len(lams), _p(lams), int(bool(truncate_soft)),
+        _p(out))
+    return out
+
+
+def excluded_cross(sys_, uint_crd, beta, charge, dq):
+    """-sum over excluded pairs (q_i dq_j + dq_i q_j) erf(b r)/r, FP64."""
+    es, en, el = _excl(sys_)
+    return lib().orc_excluded_cross(sys_.n, _p(_c(uint_crd, np.uint32)), _p(_c(charge, np.float32)),
+                                    _p(_c(dq, np.float32)), _p(_c(sys_.box, np.float32)), float(beta),
+                                    _p(es), _p(en), _p(el))
+
+
+def pme_cross_reciprocal(sys_, uint_crd, K, beta, charge, dq):
+    """The reference's cross reciprocal term (TI_core.cu:686-697): the grid of
+    dq spread like the charges, dotted with the convolved grid of the
+    charges, FP64."""
+    a = pme_reciprocal(sys_, uint_crd, K, beta, charge, want_grids=True)
+    b = pme_reciprocal(sys_, uint_crd, K, beta, dq, want_grids=True)
+    return float((b["Q"] * a["conv"]).sum())
 
 
 def pme_reciprocal(sys_, uint_crd, K, beta, charge=None, want_grids=False):

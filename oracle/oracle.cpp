@@ -411,6 +411,123 @@ double orc_lj_soft_lrc_factor(int n, const int* typeA, const int* typeB, int ntA
     return s * (-2.0 / 3.0) * PI_D / (rc * rc * rc) / 6.0;
 }
 
+// FEP / TI pair terms of one frame at several lambda windows, each window a
+// literal restatement of LJ_Soft_Core_Energy_With_Coulumb_Direct_CUDA
+// (LJ_soft_core.cu:970-1135) and LJ_Soft_Core_With_Drect_Columb_dH_dlambda_CUDA
+// (:1137-1284), FP64, half list with the same cutoff predicate as orc_direct.
+// Window w: lam = lambdas[w], l_ = 1 - lam, charges q + (lam - lambda0) dq
+// (dq may be NULL), alp = alpha lam^p, alp_ = alpha l_^p, alp1 = alpha
+// lam^(p-1), alp1_ = alpha l_^(p-1), sig6 as orc_direct_soft.
+//   ordinary pair: E_LJ = (A/12 r^-6 - B/6) r^-6 with A = l_ AA + lam BA,
+//     B = l_ AB + lam BB; E_C = q_i q_j erfc(b r)/r;
+//     dH_LJ = ((BA - AA)/12 r^-6 - (BB - AB)/6) r^-6;
+//     dH_C (reference, :1271) = (dq_j q_i + q_j dq_i) erfc(b r);
+//     dH_C (exact) = (dq_j q_i + q_j dq_i) erfc(b r)/r.
+//   soft pair: uA = r^6 + alp sigA6, uB = r^6 + alp_ sigB6, s = u^(1/6);
+//     energies keep state X only while s_X < rc (:1073, :1087):
+//     E_LJ = l_ (AA/12 uA^-2 - AB/6 uA^-1) + lam (BA/12 uB^-2 - BB/6 uB^-1),
+//     E_C = q_i q_j (l_ erfc(b sA)/sA + lam erfc(b sB)/sB);
+//     dH_LJ = uB^-2 BA/12 - BB/6 uB^-1 - AA/12 uA^-2 + AB/6 uA^-1
+//             + lam uB^-2 p sigB6 alp1_ (BA/6 uB^-1 - BB/6)
+//             + l_ uA^-2 p sigA6 alp1 (-AA/6 uA^-1 + AB/6);
+//     dH_C = q_i q_j [lam p sigB6 alp1_ uB^-1 (b/(3 sqrt pi) e^{-b^2 sB^2} + erfc(b sB)/(6 sB))
+//                     - l_ p sigA6 alp1 uA^-1 (b/(3 sqrt pi) e^{-b^2 sA^2} + erfc(b sA)/(6 sA))
+//                     - erfc(b sA)/sA + erfc(b sB)/sB]
+//            + (dq_j q_i + q_j dq_i)(l_ erfc(b sA)/sA + lam erfc(b sB)/sB)   (both forms).
+// truncate_soft = 0 drops the soft-core-radius test (the energy whose lambda
+// derivative the dH/dlambda formulas are).
+// out[w * 7 + k]: LJ intersys, LJ intrasys, direct intersys, direct intrasys,
+// dH LJ, dH direct (reference form), dH direct (exact); intrasys = same mask.
+void orc_fep_windows(int n, const uint32_t* u, const float* q, const float* dq, const int* typeA,
+                     const int* typeB, const int* mask, const float* AA, const float* AB, const float* BA,
+                     const float* BB, double lambda0, double alpha, double p, double sigma, double sigma_min,
+                     const float* box, double rc, double beta, long npairs, const int* pairs, int nwin,
+                     const double* lambdas, int truncate_soft, double* out) {
+    double cof[3];
+    float coff[3];
+    for (int d = 0; d < 3; d++) {
+        cof[d] = (double)box[d] / 4294967296.0;
+        coff[d] = (1.0f / 4294967296.0f) * box[d];
+    }
+    const float rcf = (float)rc;
+    const float rc2f = rcf * rcf;
+    const double k3 = 1.0 / (3.0 * std::sqrt(PI_D));
+    const double sig6 = std::pow(sigma, 6.0), sig6min = std::pow(sigma_min, 6.0);
+    for (int w = 0; w < nwin; w++) {
+        const double lam = lambdas[w], l_ = 1.0 - lam, t = lam - lambda0;
+        const double alp = alpha * std::pow(lam, p), alp_ = alpha * std::pow(l_, p);
+        const double alp1 = alpha * std::pow(lam, p - 1.0), alp1_ = alpha * std::pow(l_, p - 1.0);
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0;
+#pragma omp parallel for reduction(+ : s0, s1, s2, s3, s4, s5, s6) schedule(static)
+        for (long k = 0; k < npairs; k++) {
+            const int i = pairs[2 * k], j = pairs[2 * k + 1];
+            if (!(nl_dr2(u + 3 * i, u + 3 * j, coff) < rc2f)) continue;
+            double dr[3];
+            for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+            const double r2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+            const double r = std::sqrt(r2);
+            const int pa = pair_index(typeA[i], typeA[j]), pb = pair_index(typeB[i], typeB[j]);
+            const double aa = AA[pa], ab = AB[pa], ba = BA[pb], bb = BB[pb];
+            const double dqi = dq ? (double)dq[i] : 0.0, dqj = dq ? (double)dq[j] : 0.0;
+            const double qi = (double)q[i] + t * dqi, qj = (double)q[j] + t * dqj;
+            const double cross = dqj * qi + qj * dqi;
+            const bool soft = mask[i] != mask[j] || (ba > 1e-6 && aa < 1e-6) || (ba < 1e-6 && aa > 1e-6);
+            double elj, ec, dlj, dref, dexact;
+            if (!soft) {
+                const double ri2 = 1.0 / r2, ri6 = ri2 * ri2 * ri2;
+                const double A = l_ * aa + lam * ba, B = l_ * ab + lam * bb;
+                const double erfcv = std::erfc(beta * r);
+                elj = (A / 12.0 * ri6 - B / 6.0) * ri6;
+                ec = qi * qj * erfcv / r;
+                dlj = ((ba - aa) / 12.0 * ri6 - (bb - ab) / 6.0) * ri6;
+                dref = erfcv * cross;
+                dexact = erfcv / r * cross;
+            } else {
+                const double sa6 = (aa < 1e-6 || ab < 1e-6) ? sig6 : std::max(0.5 * aa / ab, sig6min);
+                const double sb6 = (ba < 1e-6 || bb < 1e-6) ? sig6 : std::max(0.5 * ba / bb, sig6min);
+                const double r6 = r2 * r2 * r2;
+                const double uA = r6 + alp * sa6, uB = r6 + alp_ * sb6;
+                const double iA = 1.0 / uA, iB = 1.0 / uB;
+                const double sA = std::pow(uA, 1.0 / 6.0), sB = std::pow(uB, 1.0 / 6.0);
+                const double gA = std::erfc(beta * sA) / sA, gB = std::erfc(beta * sB) / sB;
+                const bool inA = !truncate_soft || sA < rc, inB = !truncate_soft || sB < rc;
+                const double vA = inA ? (aa / 12.0 * iA - ab / 6.0) * iA : 0.0;
+                const double vB = inB ? (ba / 12.0 * iB - bb / 6.0) * iB : 0.0;
+                const double eA = inA ? gA : 0.0, eB = inB ? gB : 0.0;
+                elj = l_ * vA + lam * vB;
+                ec = qi * qj * (l_ * eA + lam * eB);
+                dlj = iB * iB * ba / 12.0 - bb / 6.0 * iB - aa / 12.0 * iA * iA + ab / 6.0 * iA +
+                      lam * iB * iB * p * sb6 * alp1_ * (ba / 6.0 * iB - bb / 6.0) +
+                      l_ * iA * iA * p * sa6 * alp1 * (-aa / 6.0 * iA + ab / 6.0);
+                double dc = lam * p * sb6 * alp1_ * iB * (beta * k3 * std::exp(-beta * beta * sB * sB) + gB / 6.0);
+                dc -= l_ * p * sa6 * alp1 * iA * (beta * k3 * std::exp(-beta * beta * sA * sA) + gA / 6.0);
+                dc += gB - gA;
+                dc = dc * qi * qj + cross * (l_ * gA + lam * gB);
+                dref = dc;
+                dexact = dc;
+            }
+            if (mask[i] == mask[j]) {
+                s1 += elj;
+                s3 += ec;
+            } else {
+                s0 += elj;
+                s2 += ec;
+            }
+            s4 += dlj;
+            s5 += dref;
+            s6 += dexact;
+        }
+        double* o = out + 7 * (size_t)w;
+        o[0] = s0;
+        o[1] = s1;
+        o[2] = s2;
+        o[3] = s3;
+        o[4] = s4;
+        o[5] = s5;
+        o[6] = s6;
+    }
+}
+
 // a24 exclusion correction (PME_force.cu:620-695): for every excluded pair,
 // E -= q_i q_j erf(b r)/r, F_i += -q_i q_j (2/sqrt(pi) b r e^{-b^2r^2} + erfc(b r) - 1)/r^3 dr.
 // Energy credited to atom i (the list owner). Returns the total.
@@ -437,6 +554,26 @@ double orc_excluded(int n, const uint32_t* u, const float* q, const float* box, 
             double e = -qq * std::erf(br) / r;
             tot += e;
             if (atom_corr) atom_corr[i] += e;
+        }
+    }
+    return tot;
+}
+
+// PME_Cross_Excluded_Energy_Correction (TI_core.cu:69-125), FP64: the lambda
+// derivative of the exclusion correction when the charges move along dq,
+// -sum over excluded pairs (q_i dq_j + dq_i q_j) erf(b r)/r.
+double orc_excluded_cross(int n, const uint32_t* u, const float* q, const float* dq, const float* box,
+                          double beta, const int* es, const int* en, const int* el) {
+    double cof[3];
+    for (int d = 0; d < 3; d++) cof[d] = (double)box[d] / 4294967296.0;
+    double tot = 0;
+    for (int i = 0; i < n; i++) {
+        for (int k = es[i]; k < es[i] + en[i]; k++) {
+            const int j = el[k];
+            double dr[3];
+            for (int d = 0; d < 3; d++) dr[d] = (double)(int32_t)(u[3 * j + d] - u[3 * i + d]) * cof[d];
+            const double r = std::sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+            tot -= ((double)q[i] * (double)dq[j] + (double)dq[i] * (double)q[j]) * std::erf(beta * r) / r;
         }
     }
     return tot;

@@ -15,7 +15,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsponge_nb.so")
 DRIVER = os.path.join(HERE, "sponge_nb_driver")
 HOST_SOURCES = ["host/sponge_host.cpp", "host/nb_driver.cpp"]
-SOURCES = ["abi.hip", "nl.hip", "pair.hip", "pme.hip", "batch.hip", "adjoint.hip", "md.hip"]
+SOURCES = ["abi.hip", "nl.hip", "pair.hip", "pme.hip", "batch.hip", "adjoint.hip", "md.hip", "fep.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",

@@ -199,6 +199,13 @@ struct sponge_lj {
     int* d_type;                 // n (soft-core handles: combined (type A, type B, mask) index)
     PairScratch ps;              // sized from the neighbor list on first use
     void* adj;                   // adjoint scratch, allocated by the first gradient call
+    // soft-core handles: raw end-state records of every combined pair type,
+    // 2 ntypes^2 float4 (AA, AB, BA, BB), (soft, same mask, 0, 0), and the
+    // parameters the lambda-window evaluators need
+    float4* d_fe;
+    double fep_lambda, fep_alpha, fep_p, fep_sig6, fep_sig6min;
+    double lrc_A, lrc_B;         // long-range factors of state A (sum AB) and B (sum BB)
+    void* fep;                   // frame-evaluator scratch, allocated by the first call
 };
 
 struct sponge_pme {
@@ -224,6 +231,7 @@ struct sponge_pme {
     float2* zero_ab;             // one (A, B) = (0, 0) entry
     hipStream_t side;            // reciprocal pipeline of the fused step runs here
     hipEvent_t fork, join;       // concurrently with the pair kernel
+    void* fep;                   // frame-evaluator scratch, allocated by the first call
 };
 
 // ---- batched systems ------------------------------------------------------
@@ -286,6 +294,10 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
               int overwrite = 0, const float4* sc = nullptr, int inner = 0,
               ExclArgs ex = ExclArgs{nullptr, nullptr, nullptr});
 void snb_scratch_free(PairScratch* ps);
+// Cluster-order records (uint coordinates, charge, type) into ps->pk.
+int snb_gather(const sponge_nl* nl, const unsigned int* d_uint_crd, const float* d_charge, const int* d_type,
+               PairScratch* ps, hipStream_t s);
+void snb_fep_free(void* fep_scratch);
 int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float scaler[3], const float* d_charge,
                  const int* es, const int* el, const int* en, float* d_frc, float* d_atom_energy, hipStream_t s);
 int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,

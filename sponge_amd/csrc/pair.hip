@@ -570,9 +570,10 @@ extern "C" int sponge_lj_soft_create(int atom_numbers, int atom_type_numbers_A, 
     const double sig6 = std::pow((double)sigma, 6.0), sig6min = std::pow((double)sigma_min, 6.0);
     std::vector<float2> sq((size_t)C * C);
     std::vector<float4> sc(2 * (size_t)C * C, make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<float4> fe(2 * (size_t)C * C);
     std::vector<double> cnt(C, 0.0);
     for (int i = 0; i < atom_numbers; i++) cnt[ctype[i]] += 1;
-    double lrc = 0;
+    double lrc = 0, lrc_a = 0, lrc_b = 0;
     for (int x = 0; x < C; x++)
         for (int y = 0; y < C; y++) {
             const int ax = std::get<0>(cdesc[x]), bx = std::get<1>(cdesc[x]), mx = std::get<2>(cdesc[x]);
@@ -580,13 +581,17 @@ extern "C" int sponge_lj_soft_create(int atom_numbers, int atom_type_numbers_A, 
             const int pa = pidx(ax, ay), pb = pidx(bx, by);
             const double aa = h_LJ_AA[pa], ab = h_LJ_AB[pa], ba = h_LJ_BA[pb], bb = h_LJ_BB[pb];
             lrc += cnt[x] * cnt[y] * (l_ * ab + lam * bb);
+            lrc_a += cnt[x] * cnt[y] * ab;
+            lrc_b += cnt[x] * cnt[y] * bb;
             const bool soft = mx != my || (ba > 1e-6 && aa < 1e-6) || (ba < 1e-6 && aa > 1e-6);
             const size_t k = (size_t)x * C + y;
+            const double sa6 = (aa < 1e-6 || ab < 1e-6) ? sig6 : std::max(0.5 * aa / ab, sig6min);
+            const double sb6 = (ba < 1e-6 || bb < 1e-6) ? sig6 : std::max(0.5 * ba / bb, sig6min);
+            fe[k] = make_float4((float)aa, (float)ab, (float)ba, (float)bb);
+            fe[(size_t)C * C + k] = make_float4(soft ? 1.f : 0.f, mx == my ? 1.f : 0.f, 0.f, 0.f);
             if (!soft) {
                 sq[k] = make_float2((float)(l_ * aa + lam * ba), (float)(l_ * ab + lam * bb));
             } else {
-                const double sa6 = (aa < 1e-6 || ab < 1e-6) ? sig6 : std::max(0.5 * aa / ab, sig6min);
-                const double sb6 = (ba < 1e-6 || bb < 1e-6) ? sig6 : std::max(0.5 * ba / bb, sig6min);
                 sq[k] = make_float2(-1.0f, 0.0f);
                 sc[k] = make_float4((float)(l_ * aa), (float)(l_ * ab), (float)(lam * ba), (float)(lam * bb));
                 sc[(size_t)C * C + k] = make_float4((float)(alp * sa6), (float)(alp_ * sb6), (float)l_, (float)lam);
@@ -605,9 +610,19 @@ extern "C" int sponge_lj_soft_create(int atom_numbers, int atom_type_numbers_A, 
     memcpy(lj->h_B, h_LJ_AB, P * sizeof(float));
     memcpy(lj->h_type, h_atom_LJ_type_A, atom_numbers * sizeof(int));
     sponge_lj_update_volume(lj, box_length);
-    lj->lrc_factor = lrc * (-2.0 / 3.0) * M_PI / ((double)cutoff * cutoff * cutoff) / 6.0;
+    const double kf = (-2.0 / 3.0) * M_PI / ((double)cutoff * cutoff * cutoff) / 6.0;
+    lj->lrc_factor = lrc * kf;
+    lj->lrc_A = lrc_a * kf;
+    lj->lrc_B = lrc_b * kf;
+    lj->fep_lambda = lam;
+    lj->fep_alpha = alpha;
+    lj->fep_p = p;
+    lj->fep_sig6 = sig6;
+    lj->fep_sig6min = sig6min;
     hipError_t e = hipMalloc((void**)&lj->d_ab, sq.size() * sizeof(float2));
     if (e == hipSuccess) e = hipMemcpy(lj->d_ab, sq.data(), sq.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc((void**)&lj->d_fe, fe.size() * sizeof(float4));
+    if (e == hipSuccess) e = hipMemcpy(lj->d_fe, fe.data(), fe.size() * sizeof(float4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void**)&lj->d_sc, sc.size() * sizeof(float4));
     if (e == hipSuccess) e = hipMemcpy(lj->d_sc, sc.data(), sc.size() * sizeof(float4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc((void**)&lj->d_type, atom_numbers * sizeof(int));
@@ -647,7 +662,8 @@ extern "C" int sponge_lj_set_tables(sponge_lj* lj, const float* h_LJ_A, const fl
 extern "C" void sponge_lj_destroy(sponge_lj* lj) {
     if (!lj) return;
     snb_adj_free(lj);
-    void* ptrs[] = {lj->d_ab, lj->d_sc, lj->d_type};
+    snb_fep_free(lj->fep);
+    void* ptrs[] = {lj->d_ab, lj->d_sc, lj->d_fe, lj->d_type};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     snb_scratch_free(&lj->ps);
@@ -743,6 +759,16 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
 #undef SNB_PAIR_TB
     SNB_CHECK_LAUNCH();
     if (ps->ev_stop) SNB_HIP(hipEventRecord(ps->ev_stop, s));
+    return SPONGE_NB_OK;
+}
+
+int snb_gather(const sponge_nl* nl, const unsigned int* d_uint_crd, const float* d_charge, const int* d_type,
+               PairScratch* ps, hipStream_t s) {
+    int rc = scratch_ensure(ps, nl);
+    if (rc) return rc;
+    gather_kernel<<<ceil_div(ps->cap, 256), 256, 0, s>>>(nl->flags, nl->s2o, d_uint_crd, d_charge, d_type, ps->pk, 0,
+                                                         ps->acc);
+    SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
 

@@ -1051,6 +1051,7 @@ extern "C" void sponge_pme_destroy(sponge_pme* pme) {
     if (pme->bin_atoms) (void)hipFree(pme->bin_atoms);
     if (pme->zero_type) (void)hipFree(pme->zero_type);
     if (pme->zero_ab) (void)hipFree(pme->zero_ab);
+    snb_fep_free(pme->fep);
     snb_scratch_free(&pme->ps);
     delete pme;
 }

@@ -80,7 +80,18 @@ _SIGS = {
     "sponge_settle_remember_last_coordinates": (_I, [_P, _P, _FP3, _P]),
     "sponge_settle_do": (_I, [_P, _P, _P, _FP3, _P, _I, _P, _P]),
     "sponge_md_iteration": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _FP3, _F, _F, _I, _P, _P]),
+    # include/sponge_fep.h
+    "sponge_lj_soft_energy_with_coulomb_direct": (_I, [_P, _P, _P, _P, _F, _P, _P]),
+    "sponge_lj_soft_dhdl_with_coulomb_direct": (_I, [_P, _P, _P, _P, _P, _I, _F, _P, _P]),
+    "sponge_lj_soft_long_range_correction": (_I, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "sponge_lj_soft_lambda_windows": (_I, [_P, _P, _P, _P, _P, _F, _I, _P, _P, _P]),
+    "sponge_pme_cross_dhdl": (_I, [_P, _P, _P, _P, _P, _FP3, _P, _P, _P, _I, _P, _P]),
+    "sponge_pme_lambda_windows": (_I, [_P, _P, _P, _P, _FP3, _P, _P, _P, _F, _I, _P, _P, _P]),
+    "sponge_fep_separate_direct_energy": (_I, [_P, _P, _P, _P, _P, _P, _P]),
 }
+
+FEP_FIELDS = ["vdw_intersys", "vdw_intrasys", "coul_direct_intersys", "coul_direct_intrasys", "dhdl_lj",
+              "dhdl_direct", "dhdl_direct_exact", "vdw_lrc", "dhdl_lrc"]
 
 
 class SpongeError(RuntimeError):
@@ -293,6 +304,60 @@ class LJ_SOFT_CORE(LENNARD_JONES_INFORMATION):
             "LJ_SOFT_CORE::Initial")
         self.h = h
         self.cutoff = float(cutoff)
+        self.lambda_lj = float(lambda_lj)
+        self.h_LJ_energy_sum_intersys = self.h_LJ_energy_sum_intrasys = 0.0
+        self.h_direct_ene_sum_intersys = self.h_direct_ene_sum_intrasys = 0.0
+        self.h_sigma_of_dH_dlambda_lj = self.h_sigma_of_dH_dlambda_direct = 0.0
+
+    def Get_Energy_With_Coulomb_Direct(self, uint_crd, nl, charge, pme_beta, stream=None):
+        """LJ_soft_core.cu:1809-1852: fills the four h_*_sum_* members and
+        returns the LJ total (intersys + intrasys)."""
+        out = np.zeros(4)
+        _check(load_library().sponge_lj_soft_energy_with_coulomb_direct(
+            self.h, nl.h, _ptr(uint_crd), _ptr(charge), float(pme_beta), out.ctypes.data_as(_P),
+            _stream(stream)), "Get_Energy_With_Coulomb_Direct")
+        (self.h_LJ_energy_sum_intersys, self.h_LJ_energy_sum_intrasys, self.h_direct_ene_sum_intersys,
+         self.h_direct_ene_sum_intrasys) = (float(x) for x in out)
+        return self.h_LJ_energy_sum_intersys + self.h_LJ_energy_sum_intrasys
+
+    def Get_Partial_H_Partial_Lambda_With_Columb_Direct(self, uint_crd, charge, nl, charge_B_A,
+                                                         charge_pertubated, pme_beta, stream=None):
+        """LJ_soft_core.cu:1854-1894: returns the LJ dH/dlambda and keeps the
+        direct-Coulomb one in h_sigma_of_dH_dlambda_direct."""
+        out = np.zeros(2)
+        _check(load_library().sponge_lj_soft_dhdl_with_coulomb_direct(
+            self.h, nl.h, _ptr(uint_crd), _ptr(charge), _ptr(charge_B_A), int(charge_pertubated),
+            float(pme_beta), out.ctypes.data_as(_P), _stream(stream)),
+            "Get_Partial_H_Partial_Lambda_With_Columb_Direct")
+        self.h_sigma_of_dH_dlambda_lj, self.h_sigma_of_dH_dlambda_direct = float(out[0]), float(out[1])
+        return self.h_sigma_of_dH_dlambda_lj
+
+    def Long_Range_Correction(self, *args, **kw):
+        """No arguments: the host value factor / V (LJ_soft_core.cu:1927-1937);
+        with arguments: the device accumulation of the plain module."""
+        if args or kw:
+            return super().Long_Range_Correction(*args, **kw)
+        v = _D(0)
+        _check(load_library().sponge_lj_soft_long_range_correction(self.h, ctypes.byref(v), None),
+               "Long_Range_Correction")
+        return v.value
+
+    def Partial_H_Partial_Lambda_Long_Range_Correction(self):
+        """LJ_soft_core.cu:1939-1955."""
+        v = _D(0)
+        _check(load_library().sponge_lj_soft_long_range_correction(self.h, None, ctypes.byref(v)),
+               "Partial_H_Partial_Lambda_Long_Range_Correction")
+        return v.value
+
+    def lambda_windows(self, nl, uint_crd, charge, charge_B_A, pme_beta, lambdas, stream=None):
+        """Every window of one frame (sponge_lj_soft_lambda_windows): array
+        (nwin, 9) in FEP_FIELDS order."""
+        lams = np.ascontiguousarray(np.atleast_1d(lambdas), np.float64)
+        out = np.zeros((len(lams), len(FEP_FIELDS)))
+        _check(load_library().sponge_lj_soft_lambda_windows(
+            self.h, nl.h, _ptr(uint_crd), _ptr(charge), _ptr(charge_B_A), float(pme_beta), len(lams),
+            lams.ctypes.data_as(_P), out.ctypes.data_as(_P), _stream(stream)), "lambda windows")
+        return out
 
 
 class Particle_Mesh_Ewald:
@@ -337,6 +402,38 @@ class Particle_Mesh_Ewald:
             _ptr(excluded_list_start), _ptr(excluded_list), _ptr(excluded_atom_numbers), int(which_part),
             ctypes.byref(e), _stream(stream)), "PME Get_Energy")
         return e.value
+
+    def cross_dhdl(self, nl, uint_crd, charge, charge_B_A, scaler, excluded_list_start, excluded_list,
+                   excluded_atom_numbers, lj_pertubated, stream=None):
+        """TI_CORE::Get_Cross_PME_Partial_H_Partial_Lambda (TI_core.cu:670-739):
+        dict of the reciprocal, self, correction and direct cross terms."""
+        out = np.zeros(4)
+        _check(load_library().sponge_pme_cross_dhdl(
+            self.h, nl.h if nl is not None else None, _ptr(uint_crd), _ptr(charge), _ptr(charge_B_A),
+            _vec3(scaler), _ptr(excluded_list_start), _ptr(excluded_list), _ptr(excluded_atom_numbers),
+            int(lj_pertubated), out.ctypes.data_as(_P), _stream(stream)), "Get_Cross_PME_Partial_H_Partial_Lambda")
+        return dict(reciprocal=float(out[0]), self=float(out[1]), correction=float(out[2]), direct=float(out[3]))
+
+    def lambda_windows(self, uint_crd, charge, charge_B_A, scaler, excluded_list_start, excluded_list,
+                       excluded_atom_numbers, lambda0, lambdas, stream=None):
+        """Reciprocal, self and correction energies at every window
+        (sponge_pme_lambda_windows): array (nwin, 3)."""
+        lams = np.ascontiguousarray(np.atleast_1d(lambdas), np.float64)
+        out = np.zeros((len(lams), 3))
+        _check(load_library().sponge_pme_lambda_windows(
+            self.h, _ptr(uint_crd), _ptr(charge), _ptr(charge_B_A), _vec3(scaler), _ptr(excluded_list_start),
+            _ptr(excluded_list), _ptr(excluded_atom_numbers), float(lambda0), len(lams), lams.ctypes.data_as(_P),
+            out.ctypes.data_as(_P), _stream(stream)), "PME lambda windows")
+        return out
+
+    def Seperate_Direct_Atom_Energy(self, nl, uint_crd, charge, subsys_division=None, stream=None):
+        """FEP_CORE::Seperate_Direct_Atom_Energy (FEP_core.cu:545-558) with this
+        module's cutoff and beta: (intersys, intrasys)."""
+        out = np.zeros(2)
+        _check(load_library().sponge_fep_separate_direct_energy(
+            self.h, nl.h, _ptr(uint_crd), _ptr(charge), _ptr(subsys_division), out.ctypes.data_as(_P),
+            _stream(stream)), "Seperate_Direct_Atom_Energy")
+        return float(out[0]), float(out[1])
 
     def Update_Box_Length(self, boxlength):
         _check(load_library().sponge_pme_update_box_length(self.h, _vec3(boxlength)),

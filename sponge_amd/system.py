@@ -195,6 +195,25 @@ def fep_system(n_water: int, lam: float, seed: int = 11, n_solute: int = 12,
                      charge_A=qA, charge_B=qB, lam=float(lam))
 
 
+def fep_mixed(fs: FepSystem, n_change: int = 5, n_vanish: int = 5, charge_scale: float = 0.5) -> FepSystem:
+    """A FepSystem whose solvent is perturbed too, so that every kind of pair
+    of LJ_soft_core.cu:1041-1047 occurs: the oxygens of the first `n_change`
+    waters take the first solute type in state B (ordinary pairs whose end
+    states differ), those of the next `n_vanish` waters the zero type (soft
+    pairs inside one mask), and the charges of all these waters are scaled by
+    `charge_scale` in state B (perturbed charges on ordinary pairs). The
+    charges at lambda are recomputed."""
+    tB = fs.lj_type_B.copy()
+    qB = fs.charge_B.copy()
+    zero = int(fs.lj_type_B.max())
+    for k in range(n_change + n_vanish):
+        tB[3 * k] = 2 if k < n_change else zero
+        qB[3 * k:3 * k + 3] = (fs.charge_A[3 * k:3 * k + 3].astype(np.float64) * charge_scale).astype(np.float32)
+    lam = fs.lam
+    q = ((1.0 - lam) * fs.charge_A.astype(np.float64) + lam * qB.astype(np.float64)).astype(np.float32)
+    return dataclasses.replace(fs, lj_type_B=tB, charge_B=qB, charge=q, name=fs.name + "_mixed")
+
+
 def fep_batch(seed: int = 11, window_waters: int = 1000, n_windows: int = 16,
               replica_atoms=(1500, 3000, 4500, 6000, 7500, 9000, 10500, 12000)):
     """The batched-FEP workload: `n_windows` lambda windows (lambda = k/(n-1))
