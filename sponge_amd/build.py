@@ -14,7 +14,9 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsponge_nb.so")
 DRIVER = os.path.join(HERE, "sponge_nb_driver")
+FE_DRIVER = os.path.join(HERE, "sponge_fe_driver")
 HOST_SOURCES = ["host/sponge_host.cpp", "host/nb_driver.cpp"]
+FE_HOST_SOURCES = ["host/sponge_host.cpp", "host/fe_driver.cpp"]
 SOURCES = ["abi.hip", "nl.hip", "pair.hip", "pme.hip", "batch.hip", "adjoint.hip", "md.hip", "fep.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -49,21 +51,23 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     _build_driver()
     if verbose:
-        print(f"built {LIB} and {DRIVER}")
+        print(f"built {LIB}, {DRIVER} and {FE_DRIVER}")
     return LIB
 
 
 def _build_driver() -> None:
-    """Host-only C++ (the SPONGE class mirror and the step-loop driver),
-    linked against libsponge_nb.so through an $ORIGIN rpath."""
-    tmp = DRIVER + ".tmp"
-    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
-           *[os.path.join(CSRC, s) for s in HOST_SOURCES], "-o", tmp,
-           "-L", HERE, "-lsponge_nb", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for the driver:\n{r.stdout}{r.stderr}")
-    os.replace(tmp, DRIVER)
+    """Host-only C++ (the SPONGE class mirror, the step-loop driver and the
+    FEP / TI frame driver), linked against libsponge_nb.so through an $ORIGIN
+    rpath."""
+    for exe, sources in ((DRIVER, HOST_SOURCES), (FE_DRIVER, FE_HOST_SOURCES)):
+        tmp = exe + ".tmp"
+        cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+               *[os.path.join(CSRC, s) for s in sources], "-o", tmp,
+               "-L", HERE, "-lsponge_nb", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {os.path.basename(exe)}:\n{r.stdout}{r.stderr}")
+        os.replace(tmp, exe)
 
 
 if __name__ == "__main__":

@@ -360,6 +360,168 @@ void LENNARD_JONES_INFORMATION::Clear() {
     is_initialized = 0;
 }
 
+// ------------------------------------------------------------ LJ_SOFT_CORE
+// Get_Beta_LJ (LJ_soft_core.cu:6-35): double the upper bound until
+// erfc(b rc)/rc <= tol, then (doublings + 50) bisection steps; FP64 here, as
+// the PME module's beta, so that both modules hold the same value.
+static float soft_core_beta(double rc, double tol) {
+    double high = 1.0;
+    int ihigh = 1;
+    while (std::erfc(high * rc) / rc > tol) {
+        high *= 2;
+        ihigh++;
+    }
+    ihigh += 50;
+    double low = 0.0, beta = 0.0;
+    for (int ilow = 1; ilow < ihigh; ilow++) {
+        beta = (low + high) / 2;
+        if (std::erfc(beta * rc) / rc >= tol) low = beta;
+        else high = beta;
+    }
+    return (float)beta;
+}
+
+void LJ_SOFT_CORE::Initial(CONTROLLER* controller, float cut, VECTOR box, const char* name) {
+    std::snprintf(module_name, sizeof(module_name), "%s", name ? name : "LJ_soft_core");
+    if (!controller->Command_Exist(module_name, "in_file")) return;  // not initialised, as the reference
+    if (!controller->Command_Exist("lambda_lj")) {
+        std::fprintf(stderr, "Error: FEP lambda of LJ must be given for the calculation of SOFT CORE.\n");
+        std::exit(1);
+    }
+    lambda = (float)std::atof(controller->Command("lambda_lj"));
+    if (controller->Command_Exist("soft_core_alpha")) alpha = (float)std::atof(controller->Command("soft_core_alpha"));
+    if (controller->Command_Exist("soft_core_powfer")) p = (float)std::atof(controller->Command("soft_core_powfer"));
+    if (controller->Command_Exist("soft_core_sigma")) sigma = (float)std::atof(controller->Command("soft_core_sigma"));
+    if (controller->Command_Exist("soft_core_sigma_min"))
+        sigma_min = (float)std::atof(controller->Command("soft_core_sigma_min"));
+    FILE* fp = std::fopen(controller->Command(module_name, "in_file"), "r");
+    if (!fp) {
+        std::fprintf(stderr, "Error: cannot open %s\n", controller->Command(module_name, "in_file"));
+        std::exit(1);
+    }
+    auto bad = [&](const char* what) {
+        std::fprintf(stderr, "Error: %s ends early (%s)\n", controller->Command(module_name, "in_file"), what);
+        std::exit(1);
+    };
+    if (std::fscanf(fp, "%d %d %d", &atom_numbers, &atom_type_numbers_A, &atom_type_numbers_B) != 3) bad("header");
+    const int PA = atom_type_numbers_A * (atom_type_numbers_A + 1) / 2;
+    const int PB = atom_type_numbers_B * (atom_type_numbers_B + 1) / 2;
+    std::vector<float> AA(PA), AB(PA), BA(PB), BB(PB);
+    std::vector<int> tA(atom_numbers), tB(atom_numbers);
+    for (int i = 0; i < PA; i++) {
+        if (std::fscanf(fp, "%f", &AA[i]) != 1) bad("AA");
+        AA[i] *= 12.0f;
+    }
+    for (int i = 0; i < PA; i++) {
+        if (std::fscanf(fp, "%f", &AB[i]) != 1) bad("AB");
+        AB[i] *= 6.0f;
+    }
+    for (int i = 0; i < PB; i++) {
+        if (std::fscanf(fp, "%f", &BA[i]) != 1) bad("BA");
+        BA[i] *= 12.0f;
+    }
+    for (int i = 0; i < PB; i++) {
+        if (std::fscanf(fp, "%f", &BB[i]) != 1) bad("BB");
+        BB[i] *= 6.0f;
+    }
+    for (int i = 0; i < atom_numbers; i++)
+        if (std::fscanf(fp, "%d %d", &tA[i], &tB[i]) != 2) bad("types");
+    std::fclose(fp);
+    h_subsys_division.assign(atom_numbers, 0);
+    if (controller->Command_Exist("subsys_division_in_file")) {
+        FILE* fm = std::fopen(controller->Command("subsys_division_in_file"), "r");
+        int n = 0;
+        if (!fm || std::fscanf(fm, "%d", &n) != 1 || n != atom_numbers) {
+            std::fprintf(stderr, "Error: subsys_division_in_file missing or atom_numbers differ\n");
+            std::exit(1);
+        }
+        for (int i = 0; i < n; i++)
+            if (std::fscanf(fm, "%d", &h_subsys_division[i]) != 1) {
+                std::fprintf(stderr, "Error: subsys_division_in_file ends early\n");
+                std::exit(1);
+            }
+        std::fclose(fm);
+    }
+    float tol = 1e-5f;
+    if (controller->Command_Exist("PME_Direct_Tolerance"))
+        tol = (float)std::atof(controller->Command("PME_Direct_Tolerance"));
+    pme_beta = soft_core_beta(cut, tol);
+    cutoff = cut;
+    float b[3] = {box.x, box.y, box.z};
+    sponge_host_check(sponge_lj_soft_create(atom_numbers, atom_type_numbers_A, atom_type_numbers_B, AA.data(),
+                                            AB.data(), BA.data(), BB.data(), tA.data(), tB.data(),
+                                            h_subsys_division.data(), lambda, alpha, p, sigma, sigma_min, cut, b,
+                                            &handle),
+                      "LJ_SOFT_CORE::Initial");
+    hip_check(hipMalloc((void**)&d_subsys_division, atom_numbers * sizeof(int)), "hipMalloc");
+    hip_check(hipMemcpy(d_subsys_division, h_subsys_division.data(), atom_numbers * sizeof(int),
+                        hipMemcpyHostToDevice),
+              "hipMemcpy");
+    controller->printf("    FEP lj lambda %f, soft core alpha %f powfer %f sigma %f sigma_min %f\n", lambda, alpha,
+                       p, sigma, sigma_min);
+    is_initialized = 1;
+}
+
+float LJ_SOFT_CORE::Get_Energy_With_Coulomb_Direct(const UNSIGNED_INT_VECTOR* uint_crd, const sponge_nl* nl,
+                                                   const float* charge, int is_download) {
+    if (!is_initialized) return NAN;
+    double e[4];
+    sponge_host_check(sponge_lj_soft_energy_with_coulomb_direct(handle, nl, reinterpret_cast<const unsigned*>(uint_crd),
+                                                                charge, pme_beta, e, nullptr),
+                      "Get_Energy_With_Coulomb_Direct");
+    h_LJ_energy_sum_intersys = (float)e[0];
+    h_LJ_energy_sum_intrasys = (float)e[1];
+    h_direct_ene_sum_intersys = (float)e[2];
+    h_direct_ene_sum_intrasys = (float)e[3];
+    h_LJ_energy_sum = (float)(e[0] + e[1]);
+    return is_download ? h_LJ_energy_sum : 0.f;
+}
+
+float LJ_SOFT_CORE::Get_Partial_H_Partial_Lambda_With_Columb_Direct(const UNSIGNED_INT_VECTOR* uint_crd,
+                                                                    const float* charge, const sponge_nl* nl,
+                                                                    const float* charge_B_A,
+                                                                    const int charge_pertubated, int is_download) {
+    if (!is_initialized) return NAN;
+    double d[2];
+    sponge_host_check(sponge_lj_soft_dhdl_with_coulomb_direct(handle, nl, reinterpret_cast<const unsigned*>(uint_crd),
+                                                              charge, charge_B_A, charge_pertubated > 0, pme_beta, d,
+                                                              nullptr),
+                      "Get_Partial_H_Partial_Lambda_With_Columb_Direct");
+    h_sigma_of_dH_dlambda_lj = (float)d[0];
+    h_sigma_of_dH_dlambda_direct = (float)d[1];
+    return is_download ? h_sigma_of_dH_dlambda_lj : 0.f;
+}
+
+float LJ_SOFT_CORE::Long_Range_Correction() {
+    if (!is_initialized) return NAN;
+    double v = 0;
+    sponge_host_check(sponge_lj_soft_long_range_correction(handle, &v, nullptr), "Long_Range_Correction");
+    return (float)v;
+}
+
+float LJ_SOFT_CORE::Partial_H_Partial_Lambda_Long_Range_Correction() {
+    if (!is_initialized) return NAN;
+    double v = 0;
+    sponge_host_check(sponge_lj_soft_long_range_correction(handle, nullptr, &v),
+                      "Partial_H_Partial_Lambda_Long_Range_Correction");
+    return (float)v;
+}
+
+void LJ_SOFT_CORE::Update_Volume(VECTOR box) {
+    if (!is_initialized) return;
+    float b[3] = {box.x, box.y, box.z};
+    sponge_host_check(sponge_lj_update_volume(handle, b), "LJ_SOFT_CORE Update_Volume");
+}
+
+void LJ_SOFT_CORE::Clear() {
+    if (!is_initialized) return;
+    sponge_lj_destroy(handle);
+    handle = nullptr;
+    (void)hipFree(d_subsys_division);
+    d_subsys_division = nullptr;
+    is_initialized = 0;
+}
+
 // ------------------------------------------------------ Particle_Mesh_Ewald
 void Particle_Mesh_Ewald::Initial(CONTROLLER* controller, int n, VECTOR box, float cut, const char* name) {
     std::snprintf(module_name, sizeof(module_name), "%s", name ? name : "PME");

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "../../../include/sponge_fep.h"
 #include "../../../include/sponge_md.h"
 #include "../../../include/sponge_nb.h"
 
@@ -132,6 +133,42 @@ struct Particle_Mesh_Ewald {
                      const int* excluded_atom_numbers, int which_part = 0, int is_download = 1);
     void Update_Volume(VECTOR box_length);
     void Update_Box_Length(VECTOR boxlength);
+    void Clear();
+};
+
+// LJ_SOFT_CORE (Lennard_Jones_force/LJ_soft_core.cuh:37-160): Initial reads
+// <module>_in_file (`N TA TB`, AA and AB of the TA(TA+1)/2 A pairs, BA and BB
+// of the B pairs before the x12 / x6 scaling, then per atom `typeA typeB`),
+// subsys_division_in_file, lambda_lj (required), soft_core_alpha (0.5),
+// soft_core_powfer (1), soft_core_sigma (3), soft_core_sigma_min (0) and
+// PME_Direct_Tolerance for its own beta (LJ_soft_core.cu:1420-1610).
+struct LJ_SOFT_CORE {
+    char module_name[256];
+    int is_initialized = 0;
+    int atom_numbers = 0;
+    int atom_type_numbers_A = 0, atom_type_numbers_B = 0;
+    float lambda = 0.f, alpha = 0.5f, p = 1.f, sigma = 3.f, sigma_min = 0.f;
+    float cutoff = 10.0f;
+    float pme_beta = 0.f;
+    float h_LJ_energy_sum = 0.f;
+    float h_LJ_energy_sum_intersys = 0.f, h_LJ_energy_sum_intrasys = 0.f;
+    float h_direct_ene_sum_intersys = 0.f, h_direct_ene_sum_intrasys = 0.f;
+    float h_sigma_of_dH_dlambda_lj = 0.f, h_sigma_of_dH_dlambda_direct = 0.f;
+    std::vector<int> h_subsys_division;
+    int* d_subsys_division = nullptr;
+    sponge_lj* handle = nullptr;
+
+    void Initial(CONTROLLER* controller, float cutoff, VECTOR box_length, const char* module_name = nullptr);
+    // LJ_soft_core.cu:1809-1852; returns the LJ total and fills the four split sums
+    float Get_Energy_With_Coulomb_Direct(const UNSIGNED_INT_VECTOR* uint_crd, const sponge_nl* nl,
+                                         const float* charge, int is_download = 1);
+    // LJ_soft_core.cu:1854-1894; returns the LJ sum, the direct one is kept
+    float Get_Partial_H_Partial_Lambda_With_Columb_Direct(const UNSIGNED_INT_VECTOR* uint_crd, const float* charge,
+                                                          const sponge_nl* nl, const float* charge_B_A,
+                                                          const int charge_pertubated, int is_download = 1);
+    float Long_Range_Correction();                                 // :1927-1937
+    float Partial_H_Partial_Lambda_Long_Range_Correction();        // :1939-1955
+    void Update_Volume(VECTOR box_length);                         // :1896-1902
     void Clear();
 };
 

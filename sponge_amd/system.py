@@ -532,3 +532,79 @@ def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
         for k, v in keys.items():
             f.write(f"{k} = {v}\n")
     return mdin
+
+
+def write_fe_inputs(fs: FepSystem, directory: str, frames, boxes=None, charge_perturbed: int = 1,
+                    extra: dict | None = None) -> str:
+    """Writes a FepSystem and a trajectory as the inputs of SPONGE_FEP /
+    SPONGE_TI (and of sponge_fe_driver) and returns the mdin path.
+
+    Files: LJ_soft_core_in_file (`N TA TB`, AA and AB of the A pairs and BA
+    and BB of the B pairs before the x12 / x6 scaling, then per atom
+    `typeA typeB`; LJ_soft_core.cu:1483-1518), subsys_division_in_file
+    (`N`, then N masks), charge_in_file (the charges at lambda),
+    chargeA_in_file / chargeB_in_file (TI_core.cu:431-463), exclude_in_file,
+    crd (float32 xyz frames, as mdcrd.dat) and box (`Lx Ly Lz 90 90 90` per
+    frame); `frames` is a sequence of (N, 3) arrays and `boxes` of 3-vectors
+    (default: the system's box for every frame)."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    p = lambda name: os.path.join(directory, name)
+    frames = [np.asarray(f, np.float32).reshape(-1, 3) for f in frames]
+    boxes = [fs.box] * len(frames) if boxes is None else boxes
+    with open(p("LJ_soft_core.txt"), "w") as f:
+        f.write(f"{fs.n} {fs.ntypes} {fs.ntypes_B}\n")
+        f.write(" ".join(f"{float(a) / 12.0:.17g}" for a in fs.lj_A) + "\n")
+        f.write(" ".join(f"{float(b) / 6.0:.17g}" for b in fs.lj_B) + "\n")
+        f.write(" ".join(f"{float(a) / 12.0:.17g}" for a in fs.lj_A_B) + "\n")
+        f.write(" ".join(f"{float(b) / 6.0:.17g}" for b in fs.lj_B_B) + "\n")
+        f.write("\n".join(f"{int(a)} {int(b)}" for a, b in zip(fs.lj_type, fs.lj_type_B)) + "\n")
+    for name, arr, fmt in (("subsys_division.txt", fs.mask, "{:d}"), ("charge.txt", fs.charge, "{:.9g}"),
+                           ("chargeA.txt", fs.charge_A, "{:.9g}"), ("chargeB.txt", fs.charge_B, "{:.9g}")):
+        with open(p(name), "w") as f:
+            f.write(f"{fs.n}\n")
+            f.write("\n".join(fmt.format(x.item()) for x in arr) + "\n")
+    with open(p("exclude.txt"), "w") as f:
+        f.write(f"{fs.n} {len(fs.excl_list)}\n")
+        for i in range(fs.n):
+            s, k = int(fs.excl_start[i]), int(fs.excl_numbers[i])
+            f.write(" ".join([str(k)] + [str(int(j)) for j in fs.excl_list[s:s + k]]) + "\n")
+    with open(p("mdcrd.dat"), "wb") as f:
+        for fr in frames:
+            f.write(fr.astype(np.float32).tobytes())
+    with open(p("box.txt"), "w") as f:
+        for b in boxes:
+            f.write(f"{float(b[0]):.9g} {float(b[1]):.9g} {float(b[2]):.9g} 90.0 90.0 90.0\n")
+    keys = {
+        "atom_numbers": fs.n,
+        "frame_numbers": len(frames),
+        "crd": p("mdcrd.dat"),
+        "box": p("box.txt"),
+        "charge_pertubated": charge_perturbed,
+        "charge_in_file": p("charge.txt"),
+        "chargeA_in_file": p("chargeA.txt"),
+        "chargeB_in_file": p("chargeB.txt"),
+        "subsys_division_in_file": p("subsys_division.txt"),
+        "exclude_in_file": p("exclude.txt"),
+        "LJ_soft_core_in_file": p("LJ_soft_core.txt"),
+        "lambda_lj": fs.lam,
+        "soft_core_alpha": fs.sc_alpha,
+        "soft_core_powfer": fs.sc_p,
+        "soft_core_sigma": fs.sc_sigma,
+        "soft_core_sigma_min": fs.sc_sigma_min,
+        "cutoff": fs.cutoff,
+        "skin": fs.skin,
+        "PME_Direct_Tolerance": fs.pme_tolerance,
+        "float32ene": p("float32ene.dat"),
+        "TI": p("TI.txt"),
+    }
+    for axis, k in zip("xyz", fs.fft):
+        if k > 0:
+            keys[f"PME_fft{axis}"] = k
+    keys.update(extra or {})
+    mdin = p("mdin.txt")
+    with open(mdin, "w") as f:
+        f.write(f"{fs.name}\n")
+        for k, v in keys.items():
+            f.write(f"{k} = {v}\n")
+    return mdin
