@@ -144,6 +144,84 @@ def fep_batch_bench(steps: int, warmup: int, refresh: int):
     }
 
 
+def fep_frames_bench(frames: int, nwin: int = 16):
+    """FEP / TI frame evaluation (main_fep.cu / main_ti.cu per frame, nonbonded
+    part) of one solvated 12-atom solute in ~23.5k atoms at `nwin` lambda
+    windows with charges and LJ perturbed, all inputs resident on the GPU:
+    (a) the reference's way, one SPONGE_FEP evaluation per window (a soft-core
+    handle and a charge set per window: Get_Energy_With_Coulomb_Direct plus
+    the PME correction, self and reciprocal energies of that charge set from
+    one pipeline), (b) every window in one
+    pass (sponge_lj_soft_lambda_windows + sponge_pme_lambda_windows), and
+    (c) one TI frame (dH/dlambda and the PME cross terms). Every frame
+    rebuilds the neighbor list, as main_fep.cu:170 does. Returns
+    frame-windows/s of (a) and (b) and frames/s of (c)."""
+    import numpy as np
+    import torch
+    from sponge_amd import nb, system
+
+    fs = system.fep_system(7850, 0.5, seed=11)
+    eng = nb.NonbondedEngine(fs, refresh_interval=0)
+    lams = np.linspace(0.0, 1.0, nwin)
+    lam0 = float(np.float32(fs.lam))
+    qA, qB = fs.charge_A.astype(np.float64), fs.charge_B.astype(np.float64)
+    dq = torch.as_tensor((qB - qA).astype(np.float32), device="cuda")
+    q_w = [torch.as_tensor(((1 - l) * qA + l * qB).astype(np.float32), device="cuda") for l in lams]
+    soft = [nb.LJ_SOFT_CORE(fs.n, fs.ntypes, fs.ntypes_B, fs.lj_A, fs.lj_B, fs.lj_A_B, fs.lj_B_B, fs.lj_type,
+                            fs.lj_type_B, fs.mask, float(l), fs.cutoff, eng.box, fs.sc_alpha, fs.sc_p,
+                            fs.sc_sigma, fs.sc_sigma_min) for l in lams]
+    P = eng.pme
+    ex = (eng.excl_start, eng.excl_list, eng.excl_numbers)
+
+    def rebuild():
+        eng.update_neighbor_list(forced=1)
+        eng.crd_to_uint()
+
+    def per_window():
+        rebuild()
+        for w in range(nwin):
+            soft[w].Get_Energy_With_Coulomb_Direct(eng.uint_crd, eng.nl, q_w[w], P.beta)
+            P.lambda_windows(eng.uint_crd, q_w[w], None, eng.scaler, *ex, lams[w], [lams[w]])
+
+    def all_windows():
+        rebuild()
+        eng.lj.lambda_windows(eng.nl, eng.uint_crd, eng.charge, dq, P.beta, lams)
+        P.lambda_windows(eng.uint_crd, eng.charge, dq, eng.scaler, *ex, lam0, lams)
+
+    def ti_frame():
+        rebuild()
+        eng.lj.Get_Partial_H_Partial_Lambda_With_Columb_Direct(eng.uint_crd, eng.charge, eng.nl, dq, 1, P.beta)
+        P.cross_dhdl(eng.nl, eng.uint_crd, eng.charge, dq, eng.scaler, *ex, lj_pertubated=1)
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / frames
+
+    t_ref = timed(per_window)
+    t_all = timed(all_windows)
+    t_ti = timed(ti_frame)
+    for s in soft:
+        s.Clear()
+    return {
+        "metric": "frame-windows/s, FEP partition energies (nonbonded) of a 23.5k-atom frame",
+        "workload": f"{fs.name}: {fs.n} atoms, {nwin} lambda windows, charges and LJ perturbed, list rebuilt "
+                    f"every frame, PME {P.fftx}x{P.ffty}x{P.fftz}",
+        "frames": frames,
+        "per_window_ms_per_frame": round(t_ref * 1e3, 3),
+        "all_windows_ms_per_frame": round(t_all * 1e3, 3),
+        "per_window_frame_windows_per_s": round(nwin / t_ref, 1),
+        "all_windows_frame_windows_per_s": round(nwin / t_all, 1),
+        "speedup_all_windows": round(t_ref / t_all, 2),
+        "ti_ms_per_frame": round(t_ti * 1e3, 3),
+    }
+
+
 def adjoint_bench(eng, reps: int):
     """Config-5 timing: one forward force step, one energy-gradient call and
     one force-matching-gradient call (target forces from charges perturbed by
@@ -211,6 +289,8 @@ def main():
                     help="launches per timing of the gradient calls (0 skips them)")
     ap.add_argument("--fep-steps", type=int, default=200,
                     help="steps of the batched-FEP comparison (0 skips it)")
+    ap.add_argument("--fep-frames", type=int, default=20,
+                    help="frames of the FEP / TI frame-evaluator comparison (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -350,6 +430,8 @@ def main():
         out["adjoint"] = adjoint_bench(eng, args.adjoint_reps)
     if args.fep_steps > 0:
         out["fep_batch"] = fep_batch_bench(args.fep_steps, 20, args.refresh)
+    if args.fep_frames > 0:
+        out["fep_frames"] = fep_frames_bench(args.fep_frames)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sys_, args.cpu_seconds, eng.pme.beta,
                                            (eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
