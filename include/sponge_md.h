@@ -84,6 +84,31 @@ int sponge_md_iteration(sponge_settle* s, int atom_numbers, float* d_vel, float*
                         const unsigned int* d_uint_crd, const float box_length[3], float dt,
                         float max_velocity, int need_pressure, float* d_pressure, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* Pressure and volume coupling. */
+
+/* MD_INFORMATION::system_information::Get_Pressure (MD_core/MD_core.cu:
+ * 1777-1795, kernels MD_Atom_Ek :241-250 and Calculate_Pressure_Cuda
+ * :183-186), the body of Calculate_Pressure_And_Potential_If_Needed
+ * (:1765-1775) when need_pressure: Ek = sum m v^2 / 2 of d_vel,
+ * *d_virial += sum of d_atom_virial (may be NULL), then
+ * *d_pressure = (2 Ek + *d_virial) / (3 volume) (assigned), in kcal/mol/A^3;
+ * *d_ek = Ek when d_ek is not NULL. FP64 sums, one launch. The constraint
+ * virial is added afterwards by sponge_settle_do / sponge_md_iteration. */
+int sponge_md_get_pressure(int atom_numbers, const float* d_vel, const float* d_mass, const float* d_atom_virial,
+                           float* d_virial, float volume, float* d_pressure, float* d_ek, void* stream);
+
+/* The same pressure from the FP64 parts of sponge_nb_force_step (no
+ * counterpart in the reference, whose force calls accumulate *d_virial):
+ * W = parts[LJ_VIRIAL] + 6 parts[LRC] + parts[DIRECT] + parts[CORRECTION]
+ *   + parts[SELF] + parts[RECIPROCAL]. */
+int sponge_md_pressure_from_parts(int atom_numbers, const float* d_vel, const float* d_mass, const double* d_parts,
+                                  float volume, float* d_pressure, float* d_ek, void* stream);
+
+/* Scale_List (common.cu): d_list[0..count) *= factor; the barostats' scaling
+ * of coordinates and velocities (main.cu:334-372). */
+int sponge_md_scale_list(int count, float* d_list, float factor, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

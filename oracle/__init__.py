@@ -426,3 +426,73 @@ def md_run(sys_, vel0, nsteps, dt, groups, crd0=None, beta=None, K=None, record=
         if record is not None:
             record(step, crd, vel)
     return crd, vel, log
+
+
+# ---------------------------------------------------------------------------
+# Pressure and barostat coupling.
+
+def virial(ref):
+    """The virial the reference accumulates for the pressure of one force step
+    (Lennard_Jones_force.cu:1178-1191, PME_force.cu:753-771, MD_core.cu:
+    1777-1795): sum(-f_LJ r^2) + 6 E_lrc + E_direct + E_correction + E_self +
+    E_reciprocal, from a full_reference dict."""
+    return (ref["w_lj"] + 6.0 * ref["e_lrc"] + ref["e_direct"] + ref["e_corr"] + ref["e_self"]
+            + ref["e_rec"])
+
+
+def pressure(vel, mass, W, box):
+    """Calculate_Pressure_Cuda (MD_core.cu:183-186) with MD_Atom_Ek (:241-250),
+    FP64: (2 Ek + W) / (3 V)."""
+    v = np.asarray(vel, np.float64)
+    ek = 0.5 * float((np.asarray(mass, np.float64)[:, None] * v * v).sum())
+    V = float(np.prod(np.asarray(box, np.float64)))
+    return (2.0 * ek + W) / (3.0 * V), ek
+
+
+def barostat_run(sys_, vel0, nsteps, dt_ps, groups, kind="berendsen", target_pressure_bar=1.0, tau=1.0,
+                 compressibility=4.5e-5, update_interval=10, dV_dt=0.0, beta=None, K=None):
+    """main.cu's NVE loop with a barostat, FP64 state. The pressure of a force
+    step plus SETTLE's constraint part (0.33333 W_c / V, SETTLE.cu:507-525)
+    drives, after the integration:
+    berendsen (main.cu:334-354, barostats/Berendsen_barostat.cu) at steps that
+      are multiples of update_interval (counted from 1):
+      mu = 1 - interval * compressibility * dt / tau / 3 * (p0 - p), dt and
+      tau in ps; coordinates scaled by mu;
+    andersen (main.cu:356-372, barostats/andersen_barostat.cu) every step:
+      dV/dt += (p - p0) V0 compressibility / tau^2 (tau in internal time),
+      V' = V + dV/dt dt, mu = cbrt(V' / V); coordinates scaled by mu,
+      velocities by 1 / mu;
+    then the box is scaled by mu. Returns (crd, vel, box, log)."""
+    import dataclasses
+    p0 = target_pressure_bar * 0.00001439506089041446
+    comp = compressibility * 6.946827162543585e4
+    crd = np.array(sys_.crd, np.float64)
+    vel = np.array(vel0, np.float64)
+    box = np.array(sys_.box, np.float64)
+    m = sys_.mass.astype(np.float64)
+    dt = dt_ps * 20.455
+    V0 = float(np.prod(box))
+    inv_w = V0 * comp / (tau * 20.455) ** 2
+    interval = 1 if kind == "andersen" else update_interval
+    log = []
+    for step in range(1, nsteps + 1):
+        s = dataclasses.replace(sys_, box=box.astype(np.float32))
+        ref = full_reference(s, crd=crd.astype(np.float32), beta=beta, K=K)
+        beta, K = ref["beta"], ref["K"]
+        need = step % interval == 0
+        if need:
+            p, ek = pressure(vel, m, virial(ref), s.box)
+        wc = md_iteration(crd, vel, ref["frc"], m, ref["uint"], s.box, dt, groups)
+        if need:
+            V = float(np.prod(s.box.astype(np.float64)))
+            p += 0.33333 * wc / V
+            if kind == "andersen":
+                dV_dt += (p - p0) * inv_w
+                mu = np.cbrt((V + dV_dt * dt) / V)
+                vel /= mu
+            else:
+                mu = 1.0 - update_interval * comp * dt_ps / tau / 3.0 * (p0 - p)
+            crd *= mu
+            box *= mu
+            log.append(dict(step=step, pressure=p, mu=mu))
+    return crd, vel, box, log

@@ -184,10 +184,31 @@ int main(int argc, char** argv) {
     // MD_INFORMATION::Read_Mode / Read_dt (MD_core.cu:494-551): this build
     // integrates NVE only
     const bool md = controller.Command_Exist("mode") || controller.Command_Exist("mass_in_file");
-    if (controller.Command_Exist("mode") && std::strcmp(controller.Command("mode"), "NVE") != 0) {
-        std::fprintf(stderr, "Error: mode %s is not provided here (NVE only)\n", controller.Command("mode"));
+    const bool npt = controller.Command_Exist("mode") && std::strcmp(controller.Command("mode"), "NPT") == 0;
+    if (controller.Command_Exist("mode") && std::strcmp(controller.Command("mode"), "NVE") != 0 && !npt) {
+        std::fprintf(stderr, "Error: mode %s is not provided here (NVE, or NPT with a barostat)\n",
+                     controller.Command("mode"));
         return 1;
     }
+    // NPT: a barostat (main.cu:115-126) over the NVE leap-frog; thermostats
+    // are not provided here
+    if (npt && controller.Command_Exist("thermostat")) {
+        std::fprintf(stderr, "Error: thermostat %s is not provided here (NPT runs the NVE leap-frog with the "
+                             "barostat)\n",
+                     controller.Command("thermostat"));
+        return 1;
+    }
+    const bool use_bd = npt && controller.Command_Choice("barostat", "berendsen_barostat");
+    const bool use_ad = npt && controller.Command_Choice("barostat", "andersen_barostat");
+    if (npt && !use_bd && !use_ad) {
+        std::fprintf(stderr, "Error: mode NPT needs barostat = berendsen_barostat or andersen_barostat\n");
+        return 1;
+    }
+    // MD_INFORMATION::system_information::Initial (MD_core.cu:268-271): bar -> kcal/mol/A^3
+    float target_pressure = 1.0f;
+    if (npt && controller.Command_Exist("target_pressure"))
+        target_pressure = (float)std::atof(controller.Command("target_pressure"));
+    target_pressure *= 0.00001439506089041446f;
     double dt_ps = 0.001;
     if (controller.Command_Exist("dt")) dt_ps = std::atof(controller.Command("dt"));
     const float dt = (float)dt_ps * 20.455f;
@@ -219,14 +240,17 @@ int main(int argc, char** argv) {
     int* d_el = to_device(in.ex_list);
     UNSIGNED_INT_VECTOR* uint_crd = nullptr;
     VECTOR* frc = nullptr;
-    float *d_atom_energy = nullptr, *d_atom_virial = nullptr, *d_scalars = nullptr;
+    float *d_atom_energy = nullptr, *d_atom_virial = nullptr, *d_scalars = nullptr, *d_pressure = nullptr;
     hip_ok(hipMalloc((void**)&uint_crd, n * sizeof(UNSIGNED_INT_VECTOR)), "hipMalloc");
     hip_ok(hipMalloc((void**)&frc, n * sizeof(VECTOR)), "hipMalloc");
     hip_ok(hipMalloc((void**)&d_atom_energy, n * sizeof(float)), "hipMalloc");
     hip_ok(hipMalloc((void**)&d_atom_virial, n * sizeof(float)), "hipMalloc");
     hip_ok(hipMalloc((void**)&d_scalars, 2 * sizeof(float)), "hipMalloc");  // d_virial, d_potential
-    const VECTOR scaler{in.box.x / 4294967296.0f, in.box.y / 4294967296.0f, in.box.z / 4294967296.0f};
-    const float box3[3] = {in.box.x, in.box.y, in.box.z};
+    hip_ok(hipMalloc((void**)&d_pressure, sizeof(float)), "hipMalloc");
+    hip_ok(hipMemset(d_pressure, 0, sizeof(float)), "hipMemset");
+    VECTOR box = in.box;
+    VECTOR scaler{box.x / 4294967296.0f, box.y / 4294967296.0f, box.z / 4294967296.0f};
+    float box3[3] = {box.x, box.y, box.z};
 
     NEIGHBOR_LIST neighbor_list;
     LENNARD_JONES_INFORMATION lj;
@@ -237,6 +261,36 @@ int main(int argc, char** argv) {
     pme.Initial(&controller, n, in.box, cutoff);
     CONSTRAIN constrain;
     SETTLE settle;
+    BERENDSEN_BAROSTAT_INFORMATION bd_baro;
+    ANDERSEN_BAROSTAT_INFORMATION ad_baro;
+    if (use_bd) bd_baro.Initial(&controller, target_pressure, box);
+    if (use_ad) ad_baro.Initial(&controller, target_pressure, box);
+    float h_pressure = 0.f;
+    // Main_Volume_Change (main.cu:438-448) after the coordinates were scaled,
+    // and Main_Volume_Change_Largely (:463-472) past a third of the volume
+    auto volume_change = [&](float factor, float* V0) {
+        box = VECTOR{factor * box.x, factor * box.y, factor * box.z};
+        scaler = VECTOR{box.x / 4294967296.0f, box.y / 4294967296.0f, box.z / 4294967296.0f};
+        box3[0] = box.x;
+        box3[1] = box.y;
+        box3[2] = box.z;
+        neighbor_list.Update_Volume(box);
+        neighbor_list.Neighbor_List_Update(crd, d_es, d_el, d_en, NEIGHBOR_LIST::CONDITIONAL_UPDATE,
+                                           NEIGHBOR_LIST::FORCED_CHECK);
+        lj.Update_Volume(box);
+        pme.Update_Volume(box);
+        const float V = box.x * box.y * box.z;
+        if (V > 1.331f * *V0 || V < 0.729f * *V0) {
+            controller.printf("Some modules are based on the meshing methods, and it is more precise to "
+                              "re-initialize these modules now for a long time or a large volume change.\n");
+            neighbor_list.Clear();
+            pme.Clear();
+            neighbor_list.Initial(&controller, n, box, cutoff, skin);
+            neighbor_list.Neighbor_List_Update(crd, d_es, d_el, d_en, NEIGHBOR_LIST::FORCED_UPDATE);
+            pme.Initial(&controller, n, box, cutoff);
+            *V0 = V;
+        }
+    };
     FILE* crd_traj = nullptr;
     FILE* box_traj = nullptr;
     if (md) {
@@ -262,7 +316,9 @@ int main(int argc, char** argv) {
     auto t0 = std::chrono::steady_clock::now();
     for (int step = 1; step <= step_limit; step++) {
         const int need_potential = step % mdout_interval == 0;
-        const int need_pressure = 0;
+        int need_pressure = 0;
+        bd_baro.Ask_For_Calculate_Pressure(step, &need_pressure);  // main.cu:177-178
+        if (ad_baro.is_initialized) need_pressure = 1;
         sponge_host_check(sponge_crd_to_uint(n, box3, reinterpret_cast<const float*>(crd),
                                              reinterpret_cast<unsigned*>(uint_crd), nullptr),
                           "Crd_To_Uint_Crd");
@@ -278,6 +334,8 @@ int main(int argc, char** argv) {
                                                 pme.d_correction_atom_energy);
         pme.PME_Reciprocal_Force_With_Energy_And_Virial(uint_crd, d_charge, frc, need_pressure, need_potential,
                                                         d_scalars, d_scalars + 1);
+        if (need_pressure)  // Calculate_Pressure_And_Potential_If_Needed (MD_core.cu:1765-1775)
+            Get_Pressure(n, vel, d_mass, d_atom_virial, d_scalars, box, d_pressure, 0);
         if (step == 1 && controller.Command_Exist("frc_out_file")) {
             std::vector<VECTOR> h(n);
             hip_ok(hipMemcpy(h.data(), frc, n * sizeof(VECTOR), hipMemcpyDeviceToHost), "hipMemcpy");
@@ -297,7 +355,31 @@ int main(int argc, char** argv) {
             // Main_Iteration (main.cu:282-320), NVE
             settle.Remember_Last_Coordinates(uint_crd, scaler);
             Leap_Frog(n, vel, crd, frc, acc, d_mass_inverse, dt, -1.0f);
-            settle.Do_SETTLE(d_mass, crd, in.box, vel, 0, nullptr);
+            settle.Do_SETTLE(d_mass, crd, box, vel, need_pressure, d_pressure);
+            if (bd_baro.is_initialized && step % bd_baro.update_interval == 0) {  // main.cu:334-354
+                hip_ok(hipMemcpy(&h_pressure, d_pressure, sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+                float vel_factor = 0.f;
+                const float mu = bd_baro.Scale_Factor(h_pressure, target_pressure, &vel_factor);
+                if (vel_factor != 0.f)
+                    sponge_host_check(sponge_md_scale_list(3 * n, reinterpret_cast<float*>(vel), vel_factor, nullptr),
+                                      "Scale_List");
+                sponge_host_check(sponge_md_scale_list(3 * n, reinterpret_cast<float*>(crd), mu, nullptr),
+                                  "Scale_List");
+                volume_change(mu, &bd_baro.V0);
+                bd_baro.newV = box.x * box.y * box.z;
+            }
+            if (ad_baro.is_initialized) {  // main.cu:356-372
+                hip_ok(hipMemcpy(&h_pressure, d_pressure, sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+                const float mu = ad_baro.Scale_Factor(h_pressure, target_pressure, box.x * box.y * box.z, dt);
+                sponge_host_check(sponge_md_scale_list(3 * n, reinterpret_cast<float*>(crd), mu, nullptr),
+                                  "Scale_List");
+                sponge_host_check(sponge_md_scale_list(3 * n, reinterpret_cast<float*>(vel), 1.0f / mu, nullptr),
+                                  "Scale_List");
+                volume_change(mu, &ad_baro.V0);
+            }
+            if (need_pressure && need_potential)
+                controller.printf("step %d pressure %.4f bar box %.6f %.6f %.6f\n", step, h_pressure * 6.946827162543585e4f,
+                                  box.x, box.y, box.z);
         }
         neighbor_list.Neighbor_List_Update(crd, d_es, d_el, d_en);
         if (md && info_interval > 0 && step % info_interval == 0) {
@@ -305,7 +387,7 @@ int main(int argc, char** argv) {
             std::vector<VECTOR> h;
             download(crd, h);
             std::fwrite(&h[0].x, sizeof(VECTOR), n, crd_traj);
-            std::fprintf(box_traj, "%f %f %f %.0f %.0f %.0f\n", in.box.x, in.box.y, in.box.z, 90.0f, 90.0f, 90.0f);
+            std::fprintf(box_traj, "%f %f %f %.0f %.0f %.0f\n", box.x, box.y, box.z, 90.0f, 90.0f, 90.0f);
         }
         if (md && restart_interval > 0 && step % restart_interval == 0) {
             // Export_Restart_File without an AMBER restart (MD_core.cu:1466-1487)
@@ -325,7 +407,7 @@ int main(int argc, char** argv) {
                 std::fprintf(fc, "%12.7f %12.7f %12.7f\n", hc[i].x, hc[i].y, hc[i].z);
                 std::fprintf(fv, "%12.7f %12.7f %12.7f\n", hv[i].x, hv[i].y, hv[i].z);
             }
-            std::fprintf(fc, "%12.7f %12.7f %12.7f %12.7f %12.7f %12.7f", in.box.x, in.box.y, in.box.z, 90.0f, 90.0f,
+            std::fprintf(fc, "%12.7f %12.7f %12.7f %12.7f %12.7f %12.7f", box.x, box.y, box.z, 90.0f, 90.0f,
                          90.0f);
             std::fclose(fc);
             std::fclose(fv);
@@ -346,7 +428,7 @@ int main(int argc, char** argv) {
     lj.Clear();
     neighbor_list.Clear();
     for (void* p : {(void*)vel, (void*)d_mass, (void*)d_mass_inverse, (void*)acc, (void*)crd, (void*)d_charge, (void*)d_es, (void*)d_en, (void*)d_el, (void*)uint_crd, (void*)frc,
-                    (void*)d_atom_energy, (void*)d_atom_virial, (void*)d_scalars})
+                    (void*)d_atom_energy, (void*)d_atom_virial, (void*)d_scalars, (void*)d_pressure})
         (void)hipFree(p);
     return overflow ? 2 : 0;
 }

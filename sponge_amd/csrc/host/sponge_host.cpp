@@ -693,3 +693,99 @@ void Leap_Frog(int atom_numbers, VECTOR* vel, VECTOR* crd, VECTOR* frc, VECTOR* 
                                           d_mass_inverse, dt, max_velocity, nullptr),
                       "Leap_Frog");
 }
+
+float Get_Pressure(int atom_numbers, const VECTOR* vel, const float* d_mass, const float* d_atom_virial,
+                   float* d_virial, VECTOR box, float* d_pressure, int is_download) {
+    const float volume = box.x * box.y * box.z;
+    sponge_host_check(sponge_md_get_pressure(atom_numbers, reinterpret_cast<const float*>(vel), d_mass, d_atom_virial,
+                                             d_virial, volume, d_pressure, nullptr, nullptr),
+                      "Get_Pressure");
+    if (!is_download) return 0.f;
+    float p = 0.f;
+    hip_check(hipMemcpy(&p, d_pressure, sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+    return p;
+}
+
+// ---------------------------------------------------------------- barostats
+static const float kPresConvertion = 6.946827162543585e4f;  // bar per kcal/mol/A^3 (common.cuh:24-25)
+
+void BERENDSEN_BAROSTAT_INFORMATION::Initial(CONTROLLER* controller, float target_pressure, VECTOR box,
+                                             const char* name) {
+    std::snprintf(module_name, sizeof(module_name), "%s", name ? name : "berendsen_barostat");
+    V0 = box.x * box.y * box.z;
+    newV = V0;
+    if (controller->Command_Exist("dt")) dt = (float)std::atof(controller->Command("dt"));
+    if (controller->Command_Exist(module_name, "tau")) taup = (float)std::atof(controller->Command(module_name, "tau"));
+    if (controller->Command_Exist(module_name, "compressibility"))
+        compressibility = (float)std::atof(controller->Command(module_name, "compressibility"));
+    compressibility *= kPresConvertion;
+    if (controller->Command_Exist(module_name, "stochastic_term"))
+        stochastic_term = (int)std::atof(controller->Command(module_name, "stochastic_term"));
+    update_interval = stochastic_term == 1 ? 1 : 10;
+    if (controller->Command_Exist(module_name, "update_interval"))
+        update_interval = std::atoi(controller->Command(module_name, "update_interval"));
+    if (stochastic_term && update_interval != 1) {
+        std::fprintf(stderr, "Error: the update interval should be 1 when using stochastic term\n");
+        std::exit(1);
+    }
+    if (controller->Command_Exist("target_temperature"))
+        target_temperature = (float)std::atof(controller->Command("target_temperature"));
+    rng_state = 1;
+    if (controller->Command_Exist(module_name, "random_seed"))
+        rng_state = (unsigned long long)std::atoll(controller->Command(module_name, "random_seed"));
+    controller->printf("    Berendsen barostat: target %.2f bar, tau %f ps, compressibility %e, interval %d\n",
+                       target_pressure * kPresConvertion, taup, compressibility / kPresConvertion, update_interval);
+    is_initialized = 1;
+}
+
+void BERENDSEN_BAROSTAT_INFORMATION::Ask_For_Calculate_Pressure(int steps, int* need_pressure) {
+    if (is_initialized && steps % update_interval == 0) *need_pressure += 1;
+}
+
+// Standard normal deviate from a 64-bit linear congruential generator and the
+// Box-Muller transform (the reference draws from std::normal_distribution).
+static double lcg_normal(unsigned long long* s) {
+    auto next = [s]() {
+        *s = *s * 6364136223846793005ULL + 1442695040888963407ULL;
+        return ((*s >> 11) + 0.5) / 9007199254740992.0;
+    };
+    const double u1 = next(), u2 = next();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+float BERENDSEN_BAROSTAT_INFORMATION::Scale_Factor(float p_now, float target_pressure, float* vel_factor) {
+    float mu = 1 - update_interval * compressibility * dt / taup / 3 * (target_pressure - p_now);
+    *vel_factor = 0.f;
+    if (stochastic_term) {
+        const float sd = update_interval * dt * 20.455f;
+        mu += std::sqrt(2 * 0.00198716f * target_temperature * compressibility / taup / newV) / 3 *
+              (float)(sd * lcg_normal(&rng_state));
+        *vel_factor = 1.0f / mu;
+    }
+    return mu;
+}
+
+void ANDERSEN_BAROSTAT_INFORMATION::Initial(CONTROLLER* controller, float target_pressure, VECTOR box,
+                                            const char* name) {
+    std::snprintf(module_name, sizeof(module_name), "%s", name ? name : "andersen_barostat");
+    V0 = box.x * box.y * box.z;
+    new_V = V0;
+    float taup = 1.0f, compressibility = 4.5e-5f;
+    if (controller->Command_Exist(module_name, "tau")) taup = (float)std::atof(controller->Command(module_name, "tau"));
+    if (controller->Command_Exist(module_name, "compressibility"))
+        compressibility = (float)std::atof(controller->Command(module_name, "compressibility"));
+    taup *= 20.455f;
+    compressibility *= kPresConvertion;
+    h_mass_inverse = V0 * compressibility / taup / taup;
+    dV_dt = 0.f;
+    if (controller->Command_Exist(module_name, "dV/dt")) dV_dt = (float)std::atof(controller->Command(module_name, "dV/dt"));
+    controller->printf("    Andersen barostat: target %.2f bar, tau %f ps, compressibility %e\n",
+                       target_pressure * kPresConvertion, taup / 20.455f, compressibility / kPresConvertion);
+    is_initialized = 1;
+}
+
+float ANDERSEN_BAROSTAT_INFORMATION::Scale_Factor(float p_now, float target_pressure, float volume, float dt) {
+    dV_dt += (p_now - target_pressure) * h_mass_inverse;
+    new_V = volume + dV_dt * dt;
+    return (float)std::cbrt((double)new_V / (double)volume);
+}

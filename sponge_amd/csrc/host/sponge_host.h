@@ -270,3 +270,40 @@ struct SETTLE {
 // on MD_INFORMATION's arrays.
 void Leap_Frog(int atom_numbers, VECTOR* vel, VECTOR* crd, VECTOR* frc, VECTOR* acc, const float* d_mass_inverse,
                float dt, float max_velocity);
+
+// MD_INFORMATION::system_information::Get_Pressure (MD_core/MD_core.cu:
+// 1777-1795) on MD_INFORMATION's arrays: *d_virial += sum of d_atom_virial,
+// *d_pressure = (2 Ek + *d_virial) / (3 V); returns the pressure when
+// is_download (kcal/mol/A^3).
+float Get_Pressure(int atom_numbers, const VECTOR* vel, const float* d_mass, const float* d_atom_virial,
+                   float* d_virial, VECTOR box_length, float* d_pressure, int is_download = 1);
+
+// BERENDSEN_BAROSTAT_INFORMATION (barostats/Berendsen_barostat.cuh): Initial
+// reads dt (ps), berendsen_barostat_tau (1 ps), _compressibility (4.5e-5
+// bar^-1), _stochastic_term (0), _update_interval (10, or 1 with the
+// stochastic term), _random_seed (Berendsen_barostat.cu:5-77).
+// Scale_Factor is the step of main.cu:336-344 for the pressure p_now;
+// vel_factor is 1 / mu with the stochastic term, else 0 (no velocity scaling).
+struct BERENDSEN_BAROSTAT_INFORMATION {
+    char module_name[256];
+    int is_initialized = 0;
+    float dt = 1e-3f, taup = 1.0f, compressibility = 4.5e-5f;
+    int stochastic_term = 0, update_interval = 10;
+    float V0 = 0.f, newV = 0.f;
+    float target_temperature = 300.f;
+    unsigned long long rng_state = 0;
+    void Initial(CONTROLLER* controller, float target_pressure, VECTOR box_length, const char* module_name = nullptr);
+    void Ask_For_Calculate_Pressure(int steps, int* need_pressure);
+    float Scale_Factor(float p_now, float target_pressure, float* vel_factor);
+};
+
+// ANDERSEN_BAROSTAT_INFORMATION (barostats/andersen_barostat.cuh): Initial
+// reads andersen_barostat_tau (1 ps), _compressibility (4.5e-5 bar^-1) and
+// _dV/dt (0) (andersen_barostat.cu:5-57); Scale_Factor is main.cu:358-362.
+struct ANDERSEN_BAROSTAT_INFORMATION {
+    char module_name[256];
+    int is_initialized = 0;
+    float V0 = 0.f, new_V = 0.f, h_mass_inverse = 0.f, dV_dt = 0.f;
+    void Initial(CONTROLLER* controller, float target_pressure, VECTOR box_length, const char* module_name = nullptr);
+    float Scale_Factor(float p_now, float target_pressure, float volume, float dt);
+};

@@ -182,6 +182,46 @@ __device__ __forceinline__ void add_pressure(float vir, int need_pressure, float
     if ((threadIdx.x & 63) == 0 && vir != 0.f) atomicAdd(pressure, vir * pfac);
 }
 
+// MD_INFORMATION::system_information::Get_Pressure (MD_core.cu:1777-1795):
+// MD_Atom_Ek (:241-250), Sum_Of_List of the kinetic energies, Add_Sum_List
+// of the per-atom virials into *virial, Calculate_Pressure_Cuda (:183-186),
+// one block, FP64 sums. With `parts` (the fused step's FP64 energy / virial
+// parts) the virial is taken from them instead:
+// W = sum(-f_LJ r^2) + 6 E_lrc + E_direct + E_correction + E_self + E_reciprocal.
+__global__ __launch_bounds__(1024) void pressure_kernel(int n, const float* __restrict__ vel,
+                                                        const float* __restrict__ mass,
+                                                        const float* __restrict__ atom_virial, float* virial,
+                                                        const double* __restrict__ parts, double inv_v,
+                                                        float* __restrict__ pressure, float* __restrict__ ek_out) {
+    __shared__ double red[16];
+    double ek = 0.0, av = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+        ek += 0.5 * (vx * vx + vy * vy + vz * vz) * (double)mass[i];
+        if (atom_virial) av += atom_virial[i];
+    }
+    ek = block_sum<1024>(ek, red);
+    av = block_sum<1024>(av, red);
+    if (threadIdx.x == 0) {
+        double w;
+        if (parts) {
+            w = parts[SPONGE_PART_LJ_VIRIAL] + 6.0 * parts[SPONGE_PART_LRC] + parts[SPONGE_PART_DIRECT] +
+                parts[SPONGE_PART_CORRECTION] + parts[SPONGE_PART_SELF] + parts[SPONGE_PART_RECIPROCAL];
+        } else {
+            w = (double)virial[0] + av;
+            virial[0] = (float)w;
+        }
+        pressure[0] = (float)((2.0 * ek + w) * (1.0 / 3.0) * inv_v);
+        if (ek_out) ek_out[0] = (float)ek;
+    }
+}
+
+// Scale_List (common.cu): x *= factor.
+__global__ void scale_list_kernel(int count, float* __restrict__ x, float factor) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) x[i] *= factor;
+}
+
 __global__ void leap_frog_kernel(int n, float* vel, float* crd, const float* frc, float* acc, const float* im,
                                  float dt, float vmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -343,6 +383,34 @@ extern "C" int sponge_md_leap_frog(int atom_numbers, float* d_vel, float* d_crd,
     if (atom_numbers == 0) return SPONGE_NB_OK;
     leap_frog_kernel<<<launch_blocks(atom_numbers), kMdThreads, 0, as_stream(stream)>>>(
         atom_numbers, d_vel, d_crd, d_frc, d_acc, d_inverse_mass, dt, max_velocity);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_md_get_pressure(int atom_numbers, const float* d_vel, const float* d_mass,
+                                      const float* d_atom_virial, float* d_virial, float volume, float* d_pressure,
+                                      float* d_ek, void* stream) {
+    SNB_ARG(atom_numbers > 0 && d_vel && d_mass && d_virial && d_pressure && volume > 0.f, "get_pressure");
+    pressure_kernel<<<1, 1024, 0, as_stream(stream)>>>(atom_numbers, d_vel, d_mass, d_atom_virial, d_virial, nullptr,
+                                                       1.0 / (double)volume, d_pressure, d_ek);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_md_pressure_from_parts(int atom_numbers, const float* d_vel, const float* d_mass,
+                                             const double* d_parts, float volume, float* d_pressure, float* d_ek,
+                                             void* stream) {
+    SNB_ARG(atom_numbers > 0 && d_vel && d_mass && d_parts && d_pressure && volume > 0.f, "pressure_from_parts");
+    pressure_kernel<<<1, 1024, 0, as_stream(stream)>>>(atom_numbers, d_vel, d_mass, nullptr, nullptr, d_parts,
+                                                       1.0 / (double)volume, d_pressure, d_ek);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_md_scale_list(int count, float* d_list, float factor, void* stream) {
+    SNB_ARG(count >= 0 && (count == 0 || d_list), "scale_list");
+    if (count == 0) return SPONGE_NB_OK;
+    scale_list_kernel<<<ceil_div(count, 256), 256, 0, as_stream(stream)>>>(count, d_list, factor);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }

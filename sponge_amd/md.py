@@ -88,6 +88,107 @@ def md_iteration(settle, atom_numbers, vel, crd, frc, acc, inverse_mass, mass, u
         int(need_pressure), _ptr(pressure), _stream(stream)), "sponge_md_iteration")
 
 
+def get_pressure(atom_numbers, vel, mass, atom_virial, virial, volume, pressure, ek=None, stream=None):
+    """MD_INFORMATION::system_information::Get_Pressure (MD_core/MD_core.cu:1777-1795)."""
+    _check(load_library().sponge_md_get_pressure(int(atom_numbers), _ptr(vel), _ptr(mass), _ptr(atom_virial),
+                                                 _ptr(virial), float(volume), _ptr(pressure), _ptr(ek),
+                                                 _stream(stream)), "Get_Pressure")
+
+
+def pressure_from_parts(atom_numbers, vel, mass, parts, volume, pressure, ek=None, stream=None):
+    """Get_Pressure from the fused step's FP64 parts (sponge_md_pressure_from_parts)."""
+    _check(load_library().sponge_md_pressure_from_parts(int(atom_numbers), _ptr(vel), _ptr(mass), _ptr(parts),
+                                                        float(volume), _ptr(pressure), _ptr(ek), _stream(stream)),
+           "pressure_from_parts")
+
+
+def scale_list(t, factor, stream=None):
+    """Scale_List (common.cu): t *= factor in place (float32 device tensor)."""
+    _check(load_library().sponge_md_scale_list(int(t.numel()), _ptr(t), float(factor), _stream(stream)),
+           "Scale_List")
+
+
+PRES_CONVERTION = 6.946827162543585e4          # bar per kcal/mol/A^3 (common.cuh:24-25)
+PRES_CONVERTION_INVERSE = 0.00001439506089041446
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+class BERENDSEN_BAROSTAT_INFORMATION:
+    """barostats/Berendsen_barostat.cu:5-88 and its step (main.cu:334-354):
+    every update_interval steps, with the pressure p of the step,
+    mu = 1 - update_interval * compressibility * dt / tau / 3 * (p0 - p)
+    (dt and tau in ps, compressibility in bar^-1 converted to
+    (kcal/mol/A^3)^-1), coordinates scaled by mu, then the volume change.
+    The stochastic term (stochastic_term = 1) adds
+    sqrt(2 kB T compressibility / tau / V) / 3 * N(0, dt_internal) to mu and
+    scales the velocities by 1 / mu."""
+
+    def __init__(self, target_pressure_bar, box_length, dt_ps=1e-3, tau=1.0, compressibility=4.5e-5,
+                 update_interval=None, stochastic_term=0, random_seed=0, target_temperature=300.0):
+        self.target_pressure = _f32(target_pressure_bar * PRES_CONVERTION_INVERSE)
+        b = np.asarray(box_length, np.float32)
+        self.V0 = _f32(b[0] * b[1] * b[2])
+        self.newV = self.V0
+        self.dt = _f32(dt_ps)
+        self.taup = _f32(tau)
+        self.compressibility = _f32(_f32(compressibility) * np.float32(PRES_CONVERTION))
+        self.stochastic_term = int(stochastic_term)
+        self.update_interval = int(update_interval) if update_interval else (1 if self.stochastic_term else 10)
+        if self.stochastic_term and self.update_interval != 1:
+            raise ValueError("the update interval should be 1 when using stochastic term")
+        self.target_temperature = float(target_temperature)
+        self.rng = np.random.default_rng(random_seed)
+        self.is_initialized = 1
+
+    def Ask_For_Calculate_Pressure(self, steps):
+        return self.is_initialized and steps % self.update_interval == 0
+
+    def scale_factor(self, p_now):
+        """crd_scale_factor of main.cu:337-344 (FP32 like the reference)."""
+        f = np.float32
+        mu = f(1) - f(self.update_interval) * f(self.compressibility) * f(self.dt) / f(self.taup) / f(3) * \
+            (f(self.target_pressure) - f(p_now))
+        vel_scale = None
+        if self.stochastic_term:
+            noise = f(self.rng.normal(0.0, self.update_interval * self.dt * 20.455))
+            mu = mu + f(np.sqrt(f(2) * f(system.KB) * f(self.target_temperature) * f(self.compressibility) /
+                                f(self.taup) / f(self.newV))) / f(3) * noise
+            vel_scale = float(f(1) / mu)
+        return float(mu), vel_scale
+
+
+class ANDERSEN_BAROSTAT_INFORMATION:
+    """barostats/andersen_barostat.cu:5-57 and its step (main.cu:356-372):
+    every step dV/dt += (p - p0) / W with 1/W = V0 compressibility / tau^2
+    (tau in internal time units), V' = V + dV/dt dt, mu = cbrt(V' / V);
+    coordinates scaled by mu, velocities by 1 / mu, then the volume change."""
+
+    def __init__(self, target_pressure_bar, box_length, tau=1.0, compressibility=4.5e-5, dV_dt=0.0):
+        self.target_pressure = _f32(target_pressure_bar * PRES_CONVERTION_INVERSE)
+        b = np.asarray(box_length, np.float32)
+        self.V0 = _f32(b[0] * b[1] * b[2])
+        self.new_V = self.V0
+        taup = _f32(_f32(tau) * np.float32(system.TIME_UNIT_PER_PS))
+        comp = _f32(_f32(compressibility) * np.float32(PRES_CONVERTION))
+        self.h_mass_inverse = _f32(np.float32(self.V0) * np.float32(comp) / np.float32(taup) / np.float32(taup))
+        self.dV_dt = _f32(dV_dt)
+        self.update_interval = 1
+        self.is_initialized = 1
+
+    def Ask_For_Calculate_Pressure(self, steps):
+        return bool(self.is_initialized)
+
+    def scale_factor(self, p_now, volume, dt_internal):
+        f = np.float32
+        self.dV_dt = float(f(self.dV_dt) + (f(p_now) - f(self.target_pressure)) * f(self.h_mass_inverse))
+        self.new_V = float(f(volume) + f(self.dV_dt) * f(dt_internal))
+        mu = float(np.cbrt(np.float64(self.new_V) / np.float64(volume)))
+        return mu, float(f(1.0) / f(mu))
+
+
 class MDEngine:
     """NVE molecular dynamics of a `system.System` with rigid waters.
 
@@ -122,6 +223,9 @@ class MDEngine:
         # True: the reciprocal pipeline runs on the caller's stream after the
         # pair kernel instead of concurrently on the PME handle's stream
         self.serial = False
+        self.barostat = None     # BERENDSEN_ / ANDERSEN_BAROSTAT_INFORMATION
+        self.h_pressure = 0.0
+        self.ek = torch.zeros(1, dtype=torch.float32, device=device)
 
     @property
     def crd(self):
@@ -140,11 +244,52 @@ class MDEngine:
                      self.mass, self.nb.uint_crd, self.nb.box, self.dt, self.max_velocity, need_pressure,
                      self.pressure, stream)
 
+    def volume(self):
+        b = np.asarray(self.nb.box, np.float32)
+        return float(np.float32(np.float32(b[0] * b[1]) * b[2]))
+
+    def calculate_pressure(self, stream=None):
+        """Calculate_Pressure_And_Potential_If_Needed's pressure
+        (MD_core.cu:1765-1795) after a force step with need_virial: assigns
+        self.pressure from the velocities and the step's FP64 parts."""
+        pressure_from_parts(self.n, self.vel, self.mass, self.nb.parts, self.volume(), self.pressure, self.ek,
+                            stream)
+
+    def volume_change(self, factor, stream=None):
+        """Main_Volume_Change (main.cu:438-448) after the caller scaled the
+        coordinates: box *= factor (FP32, MD_core.cu:1490-1497), every module
+        takes the new box, the list is checked (and rebuilt if needed) and the
+        uint coordinates follow."""
+        f = np.float32(factor)
+        box = [float(f * np.float32(b)) for b in self.nb.box]
+        self.nb.update_volume(box)
+        self.nb.update_neighbor_list(forced=0, forced_check=1, stream=stream)
+        self.nb.crd_to_uint(stream)
+
     def step(self, need_energy=0, stream=None, forced_update=None):
         """One MD step. forced_update None: the list's own interval / skin
-        rule; 1: rebuild now; 0: never (used for graph capture)."""
-        self.force(need_energy, 0, stream)
-        self.iterate(stream=stream)
+        rule; 1: rebuild now; 0: never (used for graph capture). With a
+        barostat the step also computes the pressure when the barostat asks
+        for it, and scales the system after the integration (main.cu:334-372);
+        such steps synchronise with the host."""
+        baro = self.barostat
+        need_pressure = bool(baro is not None and baro.Ask_For_Calculate_Pressure(self.steps + 1))
+        self.force(need_energy, 1 if need_pressure else 0, stream)
+        if need_pressure:
+            self.calculate_pressure(stream)
+        self.iterate(need_pressure=1 if need_pressure else 0, stream=stream)
+        if need_pressure:
+            import torch
+            torch.cuda.synchronize()
+            self.h_pressure = float(self.pressure.cpu()[0])
+            if isinstance(baro, ANDERSEN_BAROSTAT_INFORMATION):
+                mu, vs = baro.scale_factor(self.h_pressure, self.volume(), self.dt)
+            else:
+                mu, vs = baro.scale_factor(self.h_pressure)
+            scale_list(self.nb.crd, mu, stream)
+            if vs is not None:
+                scale_list(self.vel, vs, stream)
+            self.volume_change(mu, stream)
         if forced_update is None:
             self.nb.update_neighbor_list(stream=stream)
         elif forced_update:

@@ -80,6 +80,9 @@ _SIGS = {
     "sponge_settle_remember_last_coordinates": (_I, [_P, _P, _FP3, _P]),
     "sponge_settle_do": (_I, [_P, _P, _P, _FP3, _P, _I, _P, _P]),
     "sponge_md_iteration": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _FP3, _F, _F, _I, _P, _P]),
+    "sponge_md_get_pressure": (_I, [_I, _P, _P, _P, _P, _F, _P, _P, _P]),
+    "sponge_md_pressure_from_parts": (_I, [_I, _P, _P, _P, _F, _P, _P, _P]),
+    "sponge_md_scale_list": (_I, [_I, _P, _F, _P]),
     # include/sponge_fep.h
     "sponge_lj_soft_energy_with_coulomb_direct": (_I, [_P, _P, _P, _P, _F, _P, _P]),
     "sponge_lj_soft_dhdl_with_coulomb_direct": (_I, [_P, _P, _P, _P, _P, _I, _F, _P, _P]),
@@ -504,9 +507,21 @@ class NonbondedEngine:
         self.pme = Particle_Mesh_Ewald(n, self.box, sys_.cutoff, sys_.pme_tolerance, *sys_.fft)
         self.scaler = [b / 4294967296.0 for b in self.box]
 
-    def update_neighbor_list(self, forced=0, stream=None):
+    def update_neighbor_list(self, forced=0, stream=None, forced_check=0):
         self.nl.Neighbor_List_Update(self.crd, self.excl_start, self.excl_list, self.excl_numbers,
-                                     forced_update=forced, stream=stream)
+                                     forced_update=forced, forced_check=forced_check, stream=stream)
+
+    def update_volume(self, box):
+        """The module half of Main_Volume_Change / Main_Box_Length_Change
+        (main.cu:438-461): every module takes the new box (the PME module its
+        new box, not the stale one the reference's Update_Volume forwards,
+        PME_force.cu:996-999). Host-side: issue between steps, not inside a
+        captured graph."""
+        self.box = [float(x) for x in box]
+        self.nl.Update_Volume(self.box)
+        self.lj.Update_Volume(self.box)
+        self.pme.Update_Box_Length(self.box)
+        self.scaler = [b / 4294967296.0 for b in self.box]
 
     def crd_to_uint(self, stream=None):
         crd_to_uint(self.crd, self.box, self.uint_crd, stream)

@@ -94,12 +94,22 @@ def test_input_files_round_trip(tmp_path):
 
 
 @needs_driver
-def test_only_nve_is_accepted(tmp_path):
+def test_only_nve_and_npt_are_accepted(tmp_path):
     sys_ = system.config(0)
     mdin = system.write_sponge_inputs(sys_, str(tmp_path), velocities=np.zeros((sys_.n, 3)),
                                       extra={"mode": "NVT"})
     r = run(["-mdin", mdin])
-    assert r.returncode != 0 and "NVE only" in r.stderr
+    assert r.returncode != 0 and "NVE, or NPT" in r.stderr
+    # NPT needs one of the two barostats, and no thermostat
+    mdin = system.write_sponge_inputs(sys_, str(tmp_path), velocities=np.zeros((sys_.n, 3)),
+                                      extra={"mode": "NPT"})
+    r = run(["-mdin", mdin])
+    assert r.returncode != 0 and "barostat" in r.stderr
+    mdin = system.write_sponge_inputs(sys_, str(tmp_path), velocities=np.zeros((sys_.n, 3)),
+                                      extra={"mode": "NPT", "barostat": "berendsen_barostat",
+                                             "thermostat": "middle_langevin"})
+    r = run(["-mdin", mdin])
+    assert r.returncode != 0 and "thermostat" in r.stderr
 
 
 @needs_driver
@@ -208,3 +218,44 @@ def test_driver_nve_matches_oracle(tmp_path):
     x = crd.reshape(-1, 3, 3)
     oh = np.linalg.norm((x[:, 1:] - x[:, :1]) - L * np.round((x[:, 1:] - x[:, :1]) / L), axis=2)
     assert np.abs(oh - system.TIP3P_ROH).max() < 1e-4
+
+
+@pytest.mark.gpu
+@needs_driver
+@pytest.mark.parametrize("barostat,keys", [
+    ("berendsen_barostat", {"berendsen_barostat_tau": 0.05, "berendsen_barostat_update_interval": 5}),
+    ("andersen_barostat", {"andersen_barostat_tau": 0.2})])
+def test_driver_npt_matches_oracle(tmp_path, barostat, keys):
+    """mode NPT with the Berendsen or the Andersen barostat over the NVE
+    leap-frog, 10 steps of 2 fs on the 648-atom box: the pressure of the
+    coupled steps, Main_Volume_Change and its list check; the box and the
+    restart coordinates against the oracle's loop (box 1e-6 relative,
+    positions 1e-4 Angstrom RMS)."""
+    import oracle
+    sys_ = system.config(0)
+    v = system.rigid_velocities(sys_)
+    print(f"[driver] npt {barostat} water648 10 steps", flush=True)
+    extra = {"mode": "NPT", "barostat": barostat, "target_pressure": 1.0, "write_information_interval": 10,
+             "crd": str(tmp_path / "mdcrd.dat"), "box": str(tmp_path / "mdbox.txt"),
+             "rst": str(tmp_path / "restart")}
+    extra.update(keys)
+    mdin = system.write_sponge_inputs(sys_, str(tmp_path), step_limit=10, write_mdout_interval=5,
+                                      refresh_interval=0, velocities=v, dt_ps=0.002, extra=extra)
+    r = run(["-mdin", mdin], timeout=100)
+    sys.stdout.write(r.stdout[-800:])
+    assert r.returncode == 0, r.stderr
+    box = np.array([float(x) for x in (tmp_path / "mdbox.txt").read_text().split()[:3]])
+    crd = np.loadtxt(tmp_path / "restart_coordinate.txt", skiprows=1, max_rows=sys_.n)
+    groups = oracle.settle_groups(sys_.n, *system.water_constraints(sys_), sys_.mass)
+    kind = "andersen" if barostat.startswith("andersen") else "berendsen"
+    kw = dict(tau=keys[f"{barostat}_tau"])
+    if kind == "berendsen":
+        kw["update_interval"] = keys["berendsen_barostat_update_interval"]
+    c_ref, _, b_ref, log = oracle.barostat_run(sys_, v, 10, 0.002, groups, kind=kind, **kw)
+    d = crd - c_ref
+    d -= b_ref * np.round(d / b_ref)
+    rms = float(np.sqrt((d ** 2).mean()))
+    print(f"    box {box[0]:.6f} / {b_ref[0]:.6f}, RMS |dx| {rms:.3e}", flush=True)
+    assert abs(box[0] / b_ref[0] - 1.0) < 1e-6 + 1e-6 / b_ref[0]   # mdbox.txt prints 6 decimals
+    assert abs(b_ref[0] / float(sys_.box[0]) - 1.0) > 1e-5
+    assert rms < 1e-4
