@@ -303,6 +303,8 @@ int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float sc
 int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
                        int energy, int prep_done, hipStream_t s);
 int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s);
+int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s);
+int snb_pme_convolve(sponge_pme* pme, int energy, hipStream_t s);
 int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd, unsigned int* d_uint_crd,
                   const float* d_charge, int energy, hipStream_t s);
 int snb_pme_finalize(sponge_pme* pme, const float* d_direct_atom_energy, int need_energy, int need_virial,
