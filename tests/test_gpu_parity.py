@@ -51,6 +51,8 @@ def run_step(eng, energy=1):
 CASES = {
     "water648": lambda: system.config(0),
     "water23570": lambda: system.config(1),
+    # 96,120 atoms, PME 108^3 (radix 4 x 3 x 3 x 3 axes, 113 KB padded planes)
+    "water96120": lambda: system.config(2),
     "random_noncubic": lambda: system.small_random(900, (21.0, 24.5, 27.3), seed=3, cutoff=7.0,
                                                    skin=1.5),
     "random_smallbox": lambda: system.small_random(300, (12.0, 13.0, 14.0), seed=4, cutoff=4.5,
