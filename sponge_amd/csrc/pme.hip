@@ -222,32 +222,37 @@ __device__ __forceinline__ void wave_pass(float2* buf, int nlines, int ls, int k
 // Full multi-pass transform of the wave's lines; callers put a workgroup
 // barrier before (data written by other waves) and after (data read by other
 // waves).
-// R8 = true: every pass of the plan is radix 8 (K = 8, 64, 512), so only
-// that pass is compiled in; the general form holds the code of all six
-// radices, which costs the kernels ~90 VGPRs more and halves their waves
-// per SIMD.
-template <int SIGN, int NT, bool R8 = false>
+// Radix sets of the transform kernels: bit r set = radix-r passes compiled
+// in. Kernels are instantiated for a few sets (kRadix8: 64- and 512-point
+// axes; kRadix34: e.g. 108 = 4 x 3 x 3 x 3; kRadix248: other powers of two)
+// besides all six radices: without the code of the radices an axis does not
+// use, the kernels need ~120 instead of ~210 VGPRs, so twice the waves fit a
+// SIMD. The host picks the smallest instantiated set that holds every pass.
+constexpr int kRadixAll = (1 << 2) | (1 << 3) | (1 << 4) | (1 << 5) | (1 << 7) | (1 << 8);
+constexpr int kRadix8 = 1 << 8;
+constexpr int kRadix34 = (1 << 3) | (1 << 4);
+constexpr int kRadix248 = (1 << 2) | (1 << 4) | (1 << 8);
+
+template <int SIGN, int NT, int RM = kRadixAll>
 __device__ __forceinline__ void lds_fft(float2* buf, int nlines, int ls, int ks, const FftPlan& p, const float2* tw) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = NT / 64;
     int Ns = 1, kons = p.K;
-    if (R8) {
-        for (int ps = 0; ps < p.npass; ps++) {
-            wave_pass<8, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane);
-            kons /= 8;
-            Ns *= 8;
-        }
-        return;
-    }
     for (int ps = 0; ps < p.npass; ps++) {
         const int R = (int)((p.code >> (4 * ps)) & 15ull);
-        switch (R) {
-            case 8: wave_pass<8, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 8; break;
-            case 4: wave_pass<4, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 4; break;
-            case 2: wave_pass<2, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 2; break;
-            case 3: wave_pass<3, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 3; break;
-            case 5: wave_pass<5, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 5; break;
-            default: wave_pass<7, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane); kons /= 7; break;
+        if ((RM & (1 << 8)) && R == 8) {
+            wave_pass<8, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane);
+        } else if ((RM & (1 << 4)) && R == 4) {
+            wave_pass<4, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane);
+        } else if ((RM & (1 << 2)) && R == 2) {
+            wave_pass<2, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane);
+        } else if ((RM & (1 << 3)) && R == 3) {
+            wave_pass<3, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane);
+        } else if ((RM & (1 << 5)) && R == 5) {
+            wave_pass<5, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane);
+        } else if (RM & (1 << 7)) {
+            wave_pass<7, SIGN>(buf, nlines, ls, ks, p.K, Ns, kons, tw, wid, nw, lane);
         }
+        kons /= R;
         Ns *= R;
     }
 }
@@ -312,7 +317,7 @@ __device__ __forceinline__ float bsp_dw(float f, int k) {
 // force-matching gradient, q_i w_i . grad M_i(g): wv[i] = (w_x Kx/Lx,
 // w_y Ky/Ly, w_z Kz/Lz), with the fixed-point unit read from scale_ptr (set
 // on the device so that no contribution reaches 2^31).
-template <int MODE = 0, bool R8 = false>
+template <int MODE = 0, int RM = kRadixAll>
 __device__ __forceinline__ void spread_z_body(
     int x, int yb, const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
     const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz,
@@ -432,7 +437,7 @@ __device__ __forceinline__ void spread_z_body(
     if (nr & 1)
         for (int z = threadIdx.x; z < Kz; z += kZThreads) pkf[2 * ((nr >> 1) * zp + pd(z)) + 1] = 0.f;
     __syncthreads();
-    lds_fft<-1, kZThreads, R8>(pk, nrp, zp, 1, pz, tz);
+    lds_fft<-1, kZThreads, RM>(pk, nrp, zp, 1, pz, tz);
     __syncthreads();
     // Unpack the paired transforms Z = FFT(a + i b): A_k = (Z_k + conj Z_-k)/2,
     // B_k = (Z_k - conj Z_-k)/(2i); consecutive threads take consecutive rows
@@ -448,13 +453,13 @@ __device__ __forceinline__ void spread_z_body(
     }
 }
 
-template <bool R8>
+template <int RM>
 __global__ __launch_bounds__(kZThreads) void spread_z_kernel(
     const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
     const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, int nyb,
     FftPlan pz, const float2* __restrict__ twz, double* __restrict__ acc) {
     const int x = blockIdx.x / nyb;
-    spread_z_body<0, R8>(x, blockIdx.x - x * nyb, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, twz,
+    spread_z_body<0, RM>(x, blockIdx.x - x * nyb, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, twz,
                          acc);
 }
 
@@ -484,7 +489,7 @@ __global__ __launch_bounds__(kZThreads) void spread_z_batch(const BatchSys* __re
 // and columns), FFT along y then x, energy 1/2 sum m BC |F|^2 (FP64; m = 2 for
 // the kz that stand for their conjugates too), multiply by BC, inverse FFT
 // along x then y.
-template <bool R8 = false>
+template <int RM = kRadixAll>
 __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, const float* __restrict__ bc, int Kx,
                                             int Ky, int Kz, const FftPlan& px, const FftPlan& py,
                                             const float2* __restrict__ twx, const float2* __restrict__ twy, int energy,
@@ -529,9 +534,9 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
         }
     }
     __syncthreads();
-    lds_fft<-1, kConvThreads, R8>(sm, Kx, pitch, 1, py, ty);
+    lds_fft<-1, kConvThreads, RM>(sm, Kx, pitch, 1, py, ty);
     __syncthreads();
-    lds_fft<-1, kConvThreads, R8>(sm, Ky, 1, pitch, px, tx);
+    lds_fft<-1, kConvThreads, RM>(sm, Ky, 1, pitch, px, tx);
     __syncthreads();
     double e = 0.0;
     for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * kConvThreads) {
@@ -555,9 +560,9 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
         }
     }
     __syncthreads();
-    lds_fft<1, kConvThreads, R8>(sm, Ky, 1, pitch, px, tx);
+    lds_fft<1, kConvThreads, RM>(sm, Ky, 1, pitch, px, tx);
     __syncthreads();
-    lds_fft<1, kConvThreads, R8>(sm, Kx, pitch, 1, py, ty);
+    lds_fft<1, kConvThreads, RM>(sm, Kx, pitch, 1, py, ty);
     __syncthreads();
     for (int idx = threadIdx.x; idx < np; idx += kConvThreads) {
         const int xx = div_small(idx, rky), y = idx - xx * Ky;
@@ -570,13 +575,13 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
     }
 }
 
-template <bool R8>
+template <int RM>
 __global__ __launch_bounds__(kConvThreads) void conv2d_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
                                                               int Kx, int Ky, int Kz, FftPlan px, FftPlan py,
                                                               const float2* __restrict__ twx,
                                                               const float2* __restrict__ twy, int energy,
                                                               double* __restrict__ acc, int* __restrict__ bin_count) {
-    conv2d_body<R8>(blockIdx.x, FQ, bc, Kx, Ky, Kz, px, py, twx, twy, energy, acc, bin_count);
+    conv2d_body<RM>(blockIdx.x, FQ, bc, Kx, Ky, Kz, px, py, twx, twy, energy, acc, bin_count);
 }
 
 __global__ __launch_bounds__(kConvThreads) void conv2d_batch(const BatchSys* __restrict__ bs,
@@ -594,7 +599,7 @@ __global__ __launch_bounds__(kConvThreads) void conv2d_batch(const BatchSys* __r
 // each row extended by Hermitian symmetry (self-conjugate bins keep only their
 // real part, the C2R convention), transformed, and written to the real grid
 // R [x][y][z].
-template <bool R8 = false>
+template <int RM = kRadixAll>
 __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
                                            int Ky, int Kz, const FftPlan& pz, const float2* __restrict__ twz) {
     extern __shared__ float2 sm[];
@@ -641,7 +646,7 @@ __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restri
         pk[p * zp + pd(k)] = make_float2(A.x - B.y, A.y + B.x);
     }
     __syncthreads();
-    lds_fft<1, kZThreads, R8>(pk, nrp, zp, 1, pz, tz);
+    lds_fft<1, kZThreads, RM>(pk, nrp, zp, 1, pz, tz);
     __syncthreads();
     float* Rp = R + ((size_t)x * Ky + y0) * Kz;
     for (int idx = threadIdx.x; idx < nrp * Kz; idx += kZThreads) {
@@ -652,12 +657,12 @@ __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restri
     }
 }
 
-template <bool R8>
+template <int RM>
 __global__ __launch_bounds__(kZThreads) void inv_z_kernel(const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
                                                           int Ky, int Kz, int nyb, FftPlan pz,
                                                           const float2* __restrict__ twz) {
     const int x = blockIdx.x / nyb;
-    inv_z_body<R8>(x, blockIdx.x - x * nyb, FQ, R, Kx, Ky, Kz, pz, twz);
+    inv_z_body<RM>(x, blockIdx.x - x * nyb, FQ, R, Kx, Ky, Kz, pz, twz);
 }
 
 __global__ __launch_bounds__(kZThreads) void inv_z_batch(const BatchSys* __restrict__ bs,
@@ -880,11 +885,14 @@ size_t conv2d_lds(int Kx, int Ky) { return (size_t)(padded_len(Kx) * line_pitch(
 int raise_lds_limits(size_t bytes) {
     static size_t done = 64 * 1024;
     if (bytes <= done) return SPONGE_NB_OK;
-    const void* fns[] = {(const void*)spread_z_kernel<false>, (const void*)spread_z_kernel<true>,
-                         (const void*)spread_z_adjoint_kernel,  (const void*)spread_z_batch,
-                         (const void*)conv2d_kernel<false>,     (const void*)conv2d_kernel<true>,
-                         (const void*)conv2d_batch,             (const void*)inv_z_kernel<false>,
-                         (const void*)inv_z_kernel<true>,       (const void*)inv_z_batch};
+    const void* fns[] = {(const void*)spread_z_kernel<kRadixAll>, (const void*)spread_z_kernel<kRadix8>,
+                         (const void*)spread_z_kernel<kRadix34>,  (const void*)spread_z_kernel<kRadix248>,
+                         (const void*)spread_z_adjoint_kernel,    (const void*)spread_z_batch,
+                         (const void*)conv2d_kernel<kRadixAll>,   (const void*)conv2d_kernel<kRadix8>,
+                         (const void*)conv2d_kernel<kRadix34>,    (const void*)conv2d_kernel<kRadix248>,
+                         (const void*)conv2d_batch,               (const void*)inv_z_kernel<kRadixAll>,
+                         (const void*)inv_z_kernel<kRadix8>,      (const void*)inv_z_kernel<kRadix34>,
+                         (const void*)inv_z_kernel<kRadix248>,    (const void*)inv_z_batch};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
             set_error("cannot raise the LDS limit of the PME kernels to %zu B", bytes);
@@ -1095,12 +1103,28 @@ extern "C" int sponge_pme_update_box_length(sponge_pme* pme, const float box_len
     return SPONGE_NB_OK;
 }
 
-// True when every pass of the plan is radix 8 (the kernels' R8 form applies).
-static bool all_radix8(const FftPlan& p) {
-    for (int k = 0; k < p.npass; k++)
-        if (((p.code >> (4 * k)) & 15ull) != 8ull) return false;
-    return true;
+// Radix set of a plan (bit r per radix-r pass) and the smallest
+// instantiated set holding it.
+static int radix_mask(const FftPlan& p) {
+    int m = 0;
+    for (int k = 0; k < p.npass; k++) m |= 1 << (int)((p.code >> (4 * k)) & 15ull);
+    return m;
 }
+static int kernel_set(int m) {
+    for (int c : {kRadix8, kRadix34, kRadix248})
+        if ((m & ~c) == 0) return c;
+    return kRadixAll;
+}
+
+#define SNB_RADIX_DISPATCH(SET, KERNEL, GRID, BLOCK, LDS, STREAM, ...)                                  \
+    do {                                                                                                \
+        switch (SET) {                                                                                  \
+            case kRadix8: KERNEL<kRadix8><<<GRID, BLOCK, LDS, STREAM>>>(__VA_ARGS__); break;             \
+            case kRadix34: KERNEL<kRadix34><<<GRID, BLOCK, LDS, STREAM>>>(__VA_ARGS__); break;           \
+            case kRadix248: KERNEL<kRadix248><<<GRID, BLOCK, LDS, STREAM>>>(__VA_ARGS__); break;         \
+            default: KERNEL<kRadixAll><<<GRID, BLOCK, LDS, STREAM>>>(__VA_ARGS__); break;                \
+        }                                                                                               \
+    } while (0)
 
 // Stage 1 of the pipeline: spread + z transforms of the binned atoms.
 int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s) {
@@ -1109,19 +1133,14 @@ int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s) {
     plan_axis(Kz, &pz);
     const int nyb = row_blocks(Ky);
     double* acc = energy ? pme->acc : nullptr;
-    if (all_radix8(pz))
-        spread_z_kernel<true><<<Kx * nyb, kZThreads, spread_z_lds(Kz), s>>>(
-            pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb, pz,
-            pme->d_tw[2], acc);
-    else
-        spread_z_kernel<false><<<Kx * nyb, kZThreads, spread_z_lds(Kz), s>>>(
-            pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb, pz,
-            pme->d_tw[2], acc);
+    SNB_RADIX_DISPATCH(kernel_set(radix_mask(pz)), spread_z_kernel, Kx * nyb, kZThreads, spread_z_lds(Kz), s,
+                       pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb,
+                       pz, pme->d_tw[2], acc);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
 
-// conv2d_kernel then inv_z_kernel, each in its radix-8 form when its axes allow.
+// conv2d_kernel then inv_z_kernel, each in the smallest radix set of its axes.
 static int convolve_launch(sponge_pme* pme, int energy, hipStream_t s) {
     const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
     FftPlan px, py, pz;
@@ -1129,25 +1148,16 @@ static int convolve_launch(sponge_pme* pme, int energy, hipStream_t s) {
     plan_axis(Ky, &py);
     plan_axis(Kz, &pz);
     const int nyb = row_blocks(Ky);
-    if (all_radix8(px) && all_radix8(py))
-        conv2d_kernel<true><<<H, kConvThreads, conv2d_lds(Kx, Ky), s>>>(
-            pme->FQ, pme->d_bc, Kx, Ky, Kz, px, py, pme->d_tw[0], pme->d_tw[1], energy, pme->acc, pme->bin_count);
-    else
-        conv2d_kernel<false><<<H, kConvThreads, conv2d_lds(Kx, Ky), s>>>(
-            pme->FQ, pme->d_bc, Kx, Ky, Kz, px, py, pme->d_tw[0], pme->d_tw[1], energy, pme->acc, pme->bin_count);
+    SNB_RADIX_DISPATCH(kernel_set(radix_mask(px) | radix_mask(py)), conv2d_kernel, H, kConvThreads,
+                       conv2d_lds(Kx, Ky), s, pme->FQ, pme->d_bc, Kx, Ky, Kz, px, py, pme->d_tw[0], pme->d_tw[1],
+                       energy, pme->acc, pme->bin_count);
     SNB_CHECK_LAUNCH();
-    if (all_radix8(pz))
-        inv_z_kernel<true><<<Kx * nyb, kZThreads, inv_z_lds(Kz), s>>>(pme->FQ, pme->R, Kx, Ky, Kz, nyb, pz,
-                                                                     pme->d_tw[2]);
-    else
-        inv_z_kernel<false><<<Kx * nyb, kZThreads, inv_z_lds(Kz), s>>>(pme->FQ, pme->R, Kx, Ky, Kz, nyb, pz,
-                                                                      pme->d_tw[2]);
+    SNB_RADIX_DISPATCH(kernel_set(radix_mask(pz)), inv_z_kernel, Kx * nyb, kZThreads, inv_z_lds(Kz), s, pme->FQ,
+                       pme->R, Kx, Ky, Kz, nyb, pz, pme->d_tw[2]);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
 
-// Stage 2: the (x, y) transforms, BC convolution (and energy), inverse, and
-// the inverse z transforms into the convolved grid R.
 int snb_pme_convolve(sponge_pme* pme, int energy, hipStream_t s) { return convolve_launch(pme, energy, s); }
 
 // The reciprocal pipeline; forces added to d_frc (original atom order).
