@@ -308,20 +308,31 @@ void snb_fep_free(void* p) {
 
 namespace {
 
+int fep_scratch_alloc(FepScratch* f, int n) {
+    f->n = n;
+    const int nacc = kHard + SPONGE_FEP_MAX_WINDOWS * kSoftFields;
+    SNB_HIP(hipMalloc((void**)&f->acc, nacc * sizeof(double)));
+    SNB_HIP(hipMalloc((void**)&f->count, sizeof(int)));
+    SNB_HIP(hipMalloc((void**)&f->win, SPONGE_FEP_MAX_WINDOWS * kWinParams * sizeof(double)));
+    SNB_HIP(hipMalloc((void**)&f->zero_type, n * sizeof(int)));
+    SNB_HIP(hipMemset(f->zero_type, 0, n * sizeof(int)));
+    SNB_HIP(hipMalloc((void**)&f->soft, 2 * (size_t)kInitialSoftCap * sizeof(int4)));
+    f->soft_cap = kInitialSoftCap;
+    return SPONGE_NB_OK;
+}
+
+// The scratch is stored in *slot only once every buffer is allocated, so a
+// failed allocation leaves the handle as it was and the next call retries.
 int fep_scratch(void** slot, int n, FepScratch** out) {
     FepScratch* f = static_cast<FepScratch*>(*slot);
     if (!f) {
         f = new FepScratch();
+        const int rc = fep_scratch_alloc(f, n);
+        if (rc) {
+            snb_fep_free(f);
+            return rc;
+        }
         *slot = f;
-        f->n = n;
-        const int nacc = kHard + SPONGE_FEP_MAX_WINDOWS * kSoftFields;
-        SNB_HIP(hipMalloc((void**)&f->acc, nacc * sizeof(double)));
-        SNB_HIP(hipMalloc((void**)&f->count, sizeof(int)));
-        SNB_HIP(hipMalloc((void**)&f->win, SPONGE_FEP_MAX_WINDOWS * kWinParams * sizeof(double)));
-        SNB_HIP(hipMalloc((void**)&f->zero_type, n * sizeof(int)));
-        SNB_HIP(hipMemset(f->zero_type, 0, n * sizeof(int)));
-        f->soft_cap = kInitialSoftCap;
-        SNB_HIP(hipMalloc((void**)&f->soft, 2 * (size_t)f->soft_cap * sizeof(int4)));
     }
     *out = f;
     return SPONGE_NB_OK;
@@ -384,10 +395,14 @@ int fep_walk(FepScratch* f, const WalkArgs& A, int nwin, const double* h_win, do
             if (h_soft) memcpy(h_soft, h.data() + kHard, (size_t)nwin * kSoftFields * sizeof(double));
             return SPONGE_NB_OK;
         }
+        // the capacity stays 0 until the larger list is allocated, so a
+        // failed allocation leaves no pointer the kernels would write through
         SNB_HIP(hipFree(f->soft));
         f->soft = nullptr;
-        f->soft_cap = cnt + cnt / 4;
-        SNB_HIP(hipMalloc((void**)&f->soft, 2 * (size_t)f->soft_cap * sizeof(int4)));
+        f->soft_cap = 0;
+        const int cap = cnt + cnt / 4;
+        SNB_HIP(hipMalloc((void**)&f->soft, 2 * (size_t)cap * sizeof(int4)));
+        f->soft_cap = cap;
     }
     set_error("fep: soft-core pair list still overflowing");
     return SPONGE_NB_ECAPACITY;
