@@ -103,6 +103,15 @@ int sponge_nl_update_volume(sponge_nl* nl, const float box_length[3]);
  * flags (SPONGE_NL_OVERFLOW_*), and how many rebuilds have run. */
 int sponge_nl_status(sponge_nl* nl, long* pair_count, int* overflow_flags, int* rebuilds);
 
+/* Synchronises. Diagnostics of the fused step's inner list (no counterpart in
+ * the reference): with plain LJ, force-only fused steps read per-slot partner
+ * lists pruned to the inner radius; `slot_overflow` = 1 when a slot had more
+ * partners than those lists hold at a prune since the last rebuild (its
+ * cluster then reads the full list), and `slot_launches` counts the
+ * pair-kernel launches that read the per-slot form since the list was
+ * created. */
+int sponge_nl_inner_status(sponge_nl* nl, int* slot_overflow, int* slot_launches);
+
 /* Synchronises. The list as canonical atom pairs (i<j, original indices),
  * unsorted; at most `capacity` pairs are written to h_pairs (2 ints each).
  * Replaces reading d_nl[i].atom_serial on the host (neighbor_list.cu:490-503). */

@@ -104,6 +104,7 @@ extern "C" int sponge_batch_create(int nsys, sponge_nl* const* nl, sponge_lj* co
         b->lds_inv = std::max(b->lds_inv, snb_pme_lds_inv(pme[s]));
     }
     b->lds_pair = (size_t)b->max_nt * b->max_nt * sizeof(float2);
+    if (rc == SPONGE_NB_OK) rc = snb_pme_raise_lds(std::max({b->lds_fwd, b->lds_conv, b->lds_inv}));
     if (rc == SPONGE_NB_OK && hipMalloc((void**)&b->d, nsys * sizeof(BatchSys)) != hipSuccess) {
         set_error("batch_create: out of device memory");
         rc = SPONGE_NB_EHIP;

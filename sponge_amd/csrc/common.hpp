@@ -174,6 +174,14 @@ struct sponge_nl {
     float prune_r2;
     int* list_in_n;             // ncl_cap
     uint2* list_in;             // ncl_cap * list_cap
+    // the inner pairs per i slot (j slots within prune_r2) for plain LJ,
+    // which force-only fused steps read (list_in is soft-core LJ's form).
+    // flags[11] = 1 once a slot held more than atom_in_cap partners at a
+    // prune since the last rebuild (its cluster then reads the full list),
+    // flags[12] counts the launches that read the per-slot form
+    int atom_in_cap;
+    int* atom_in_n;             // slot_cap
+    int* atom_in;               // slot_cap * atom_in_cap
     // symmetric exclusion lists in original atom order (partners of atom a:
     // sym_list[sym_start[a] .. sym_start[a+1])), built on the host from the
     // exclusion CSR the first time an update passes those arrays
@@ -206,6 +214,7 @@ struct sponge_lj {
     double fep_lambda, fep_alpha, fep_p, fep_sig6, fep_sig6min;
     double lrc_A, lrc_B;         // long-range factors of state A (sum AB) and B (sum BB)
     void* fep;                   // frame-evaluator scratch, allocated by the first call
+    int conv_split;              // 2D transforms as line launches (large planes)
 };
 
 struct sponge_pme {
@@ -232,6 +241,7 @@ struct sponge_pme {
     hipStream_t side;            // reciprocal pipeline of the fused step runs here
     hipEvent_t fork, join;       // concurrently with the pair kernel
     void* fep;                   // frame-evaluator scratch, allocated by the first call
+    int conv_split;              // 2D transforms as line launches (large planes)
 };
 
 // ---- batched systems ------------------------------------------------------
@@ -273,10 +283,14 @@ struct sponge_batch {
 
 // Inner-list arguments of the pair kernel (interval 0: no inner list).
 struct PruneArgs {
-    int interval;
-    float r2;
-    int* in_n;
-    uint2* in_list;
+    int interval = 0;
+    float r2 = 0.f;
+    int* in_n = nullptr;
+    uint2* in_list = nullptr;
+    int* atom_n = nullptr;      // per-slot inner lists (NULL: cluster form only)
+    int* atom_list = nullptr;
+    int atom_cap = 0;
+    int* diag = nullptr;        // flags + 11: [0] per-slot overflow, [1] per-slot launches
 };
 
 // Excluded-pair correction folded into the pair kernel (start NULL: off):
@@ -314,6 +328,7 @@ void snb_pme_plans(const sponge_pme* pme, FftPlan* px, FftPlan* py, FftPlan* pz)
 int snb_pme_row_blocks(const sponge_pme* pme);
 size_t snb_pme_lds_fwd(const sponge_pme* pme);
 size_t snb_pme_lds_conv(const sponge_pme* pme);
+int snb_pme_raise_lds(size_t bytes);
 size_t snb_pme_lds_inv(const sponge_pme* pme);
 float snb_nl_rlm(const sponge_nl* nl);
 int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s);

@@ -73,6 +73,7 @@ __device__ __forceinline__ void nl_begin_body(int forced, int* flags, int* col_c
             flags[9] = 0;
             flags[2] += 1;
             flags[10] = 0;  // the next fused step prunes the new list
+            flags[11] = 0;  // and rewrites the per-slot inner lists
         }
     }
     __syncthreads();
@@ -614,11 +615,23 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
     A((void**)&nl->list, (size_t)nl->ncl_cap * nl->list_cap * sizeof(uint2));
     A((void**)&nl->list_in, (size_t)nl->ncl_cap * nl->list_cap * sizeof(uint2));
     A((void**)&nl->list_in_n, nl->ncl_cap * sizeof(int));
+    {
+        // partners of one slot within the list radius: the mean count at the
+        // system's density with a quarter and 64 to spare (a slot in a denser
+        // region sends its cluster to the full list); each slot's list starts
+        // on a 128-byte line of its own
+        const double rl3 = std::pow((double)cutoff + (double)skin, 3);
+        const int est = (int)std::ceil(1.25 * (4.0 / 3.0) * M_PI * rl3 * dens) + 64;
+        nl->atom_in_cap = (std::min(est, nl->slot_cap) + 31) & ~31;
+    }
+    A((void**)&nl->atom_in_n, nl->slot_cap * sizeof(int));
+    A((void**)&nl->atom_in, (size_t)nl->slot_cap * nl->atom_in_cap * sizeof(int));
     A((void**)&nl->pack, nl->slot_cap * sizeof(uint4));
     A((void**)&nl->flags, 16 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->flags, 0, 16 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->list_n, 0, nl->ncl_cap * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->list_in_n, 0, nl->ncl_cap * sizeof(int));
+    if (e == hipSuccess) e = hipMemset(nl->atom_in_n, 0, nl->slot_cap * sizeof(int));
     // Inner list of the fused step: in interval mode the reference trusts the
     // skin to cover refresh_interval steps of motion; prorating it, an inner
     // radius of rc + skin * P / refresh_interval covers P steps, and the list
@@ -644,7 +657,8 @@ extern "C" void sponge_nl_destroy(sponge_nl* nl) {
     if (!nl) return;
     void* ptrs[] = {nl->old_crd, nl->nl_uint, nl->col_count, nl->col_start, nl->col_atoms, nl->s2o,
                     nl->o2s,     nl->cl_lo,   nl->cl_hi,     nl->list_n,    nl->list,      nl->pack,
-                    nl->flags,   nl->list_in, nl->list_in_n, nl->sym_start, nl->sym_list};
+                    nl->flags,   nl->list_in, nl->list_in_n, nl->sym_start, nl->sym_list, nl->atom_in_n,
+                    nl->atom_in};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete nl;
@@ -780,6 +794,16 @@ extern "C" int sponge_nl_status(sponge_nl* nl, long* pair_count, int* overflow_f
     if (overflow_flags) *overflow_flags = f[1];
     if (rebuilds) *rebuilds = f[2];
     return f[1] ? SPONGE_NB_ECAPACITY : SPONGE_NB_OK;
+}
+
+extern "C" int sponge_nl_inner_status(sponge_nl* nl, int* slot_overflow, int* slot_launches) {
+    SNB_ARG(nl, "nl_inner_status");
+    int f[16];
+    SNB_HIP(hipDeviceSynchronize());
+    SNB_HIP(hipMemcpy(f, nl->flags, sizeof(f), hipMemcpyDeviceToHost));
+    if (slot_overflow) *slot_overflow = f[11];
+    if (slot_launches) *slot_launches = f[12];
+    return SPONGE_NB_OK;
 }
 
 extern "C" int sponge_nl_get_pairs(sponge_nl* nl, int* h_pairs, long capacity, long* pair_count) {

@@ -155,6 +155,171 @@ __device__ __forceinline__ void soft_pair_energy(double r2, double qq, double be
     vlj = -flj * r2;
 }
 
+// PME_Excluded_Force_With_Atom_Energy's force (PME_force.cu:620-695) on one
+// atom (slot record a) from its exclusion partners k0, k0 + kstep, ...: each
+// atom collects its own side of every excluded pair, so no other atom's force
+// is written.
+__device__ __forceinline__ void excluded_partners(const ExclArgs& ex, const uint4* __restrict__ pk, int at, int k0,
+                                                  int kstep, unsigned ax, unsigned ay, unsigned az, float qi,
+                                                  float cx, float cy, float cz, float beta, float& fix, float& fiy,
+                                                  float& fiz) {
+    const int r1 = ex.start[at + 1];
+    for (int k = ex.start[at] + k0; k < r1; k += kstep) {
+        const uint4 b = pk[ex.o2s[ex.list[k]]];
+        const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+        const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
+        const float dz = (float)(int)((b.z & ~3u) - az) * cz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r = sqrtf(r2);
+        const float br = beta * r;
+        const float qq = qi * __uint_as_float(b.w);
+        const float fa = -qq * ((kTwoOverSqrtPi * br * expf(-br * br) + erfcf(br) - 1.0f) / r2 / r);
+        fix += fa * dx;
+        fiy += fa * dy;
+        fiz += fa * dz;
+    }
+}
+
+// Per-slot inner lists, written on prune steps: the full list's pairs of
+// cluster ic within pa.r2 at this step's positions, per i slot in list order
+// (lanes ii, ii + 4, ... of a round in lane order). Returns the partner count
+// of slot lane >> 4 (the lane mapping of pair_atoms); a count above atom_cap
+// (partners past the cap dropped) sends the cluster to the full list until
+// the next prune. The list stores are complete and visible to the whole wave
+// on return.
+template <int TB>
+__device__ __forceinline__ int prune_slots(int ic, const int* __restrict__ list_n, const uint2* __restrict__ list,
+                                           int list_cap, const uint4* __restrict__ pk, float cx, float cy, float cz,
+                                           const PruneArgs& pa) {
+    const int lane = threadIdx.x & 63;
+    const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
+    const int islot = ic * kClusterSize + ii;
+    const uint4 a = pk[islot];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
+    const int n = list_n[ic];
+    const uint2* lst = list + (size_t)ic * list_cap;
+    int* out = pa.atom_list + (size_t)islot * pa.atom_cap;
+    const unsigned long long sel = 0x1111111111111111ull << ii, below = (1ull << lane) - 1ull;
+    int count = 0;
+    for (int base = 0; base < n; base += 64) {
+        const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
+        const int cnt = min(64, n - base);
+        // four rounds of 4 entries per pass, their j records loaded together
+        // (the loop is latency bound: little arithmetic per record)
+        for (int it = 0; it < cnt; it += 16) {
+            int jcs[4];
+            unsigned ms[4];
+            uint4 bs[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = it + 4 * u + g;
+                const int src = min(e, 63);
+                jcs[u] = __shfl((int)my.x, src);
+                const unsigned mm = (unsigned)__shfl((int)my.y, src);
+                ms[u] = e < cnt ? mm : 0u;
+                bs[u] = pk[jcs[u] * kClusterSize + jj];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint4 b = bs[u];
+                const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+                const float dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
+                const float dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
+                const bool keep = ((ms[u] >> bit) & 1u) && dr2_exact(dx, dy, dz) < pa.r2;
+                const unsigned long long mi = __ballot(keep) & sel;
+                const int pos = count + __popcll(mi & below);
+                if (keep && pos < pa.atom_cap) out[pos] = jcs[u] * kClusterSize + jj;
+                count += __popcll(mi);
+            }
+        }
+    }
+    if (lane < kClusterSize) {
+        pa.atom_n[islot] = count;
+        if (count > pa.atom_cap) atomicOr(pa.diag, 1);
+    }
+    // other lanes of the wave read these lists next: the wave's stores must
+    // have completed (one CU, so the workgroup scope suffices; the lists of
+    // different slots share no cache line)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    return __shfl(count, lane >> 4);
+}
+
+// Force-only steps of plain LJ from the per-slot inner lists: lane (i, k),
+// i = lane >> 4, takes partners k, k + 16, ... of atom i of the wave's
+// cluster (n of them; nmax the longest of the four lists), so every lane
+// holds a listed pair (the cluster form evaluates all 16 pairs of each listed
+// cluster pair, in cutoff or not). The pair terms are pair_body's, in FP32;
+// the partner index is fetched two rounds ahead and its record one round
+// ahead.
+template <int TB>
+__device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4* __restrict__ pk, const float2* sab,
+                                           int nt, float cx, float cy, float cz, float rc2, float beta,
+                                           const int* __restrict__ s2o, float* __restrict__ frc, int overwrite,
+                                           const PruneArgs& pa, const ExclArgs& ex) {
+    const int lane = threadIdx.x & 63, ii = lane >> 4, k = lane & 15;
+    const int islot = ic * kClusterSize + ii;
+    const uint4 a = pk[islot];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
+    const float qi = __uint_as_float(a.w);
+    const int ti = TB == 2 ? (int)(a.x & 3u) : unpack_type(a);
+    const float2* srow = sab + ti * nt;
+    const float mlog2e_b2 = -1.4426950408889634f * (beta * beta);
+    const int* __restrict__ lst = pa.atom_list + (size_t)islot * pa.atom_cap;
+    float fix = 0.f, fiy = 0.f, fiz = 0.f;
+    int j1 = k + 16 < n ? lst[k + 16] : islot;
+    uint4 b = pk[k < n ? lst[k] : islot];
+    for (int t = 0; t < nmax; t += 16) {
+        const int j2 = t + 32 + k < n ? lst[t + 32 + k] : islot;
+        const uint4 bn = pk[j1];
+        // with at most 4 types the y and z words carry no type bits
+        const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+        const float dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
+        const float dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
+        const float r2 = dr2_exact(dx, dy, dz);
+        const bool in = t + k < n && r2 < rc2;
+        // unused lanes (past the list's end: the atom itself) may give inf /
+        // NaN below; the final select discards them
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float rinv2 = rinv * rinv;
+        const float rinv6 = rinv2 * rinv2 * rinv2;
+        const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
+        const float2 AB = srow[tj];
+        const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
+        const float x = beta * (r2 * rinv);
+        const float ex_ = __builtin_amdgcn_exp2f(mlog2e_b2 * r2);
+        const float ecx = erfcx_poly(x);
+        const float qq = qi * __uint_as_float(b.w);
+        const float fc = qq * ex_ * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+        const float f = in ? flj - fc : 0.f;
+        fix += f * dx;
+        fiy += f * dy;
+        fiz += f * dz;
+        b = bn;
+        j1 = j2;
+    }
+    const int at = s2o[islot];
+    if (ex.start && at >= 0) excluded_partners(ex, pk, at, k, 16, ax, ay, az, qi, cx, cy, cz, beta, fix, fiy, fiz);
+    // i force: sum the 16 lanes of atom i
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        fix += __shfl_xor(fix, o);
+        fiy += __shfl_xor(fiy, o);
+        fiz += __shfl_xor(fiz, o);
+    }
+    if (k == 0 && at >= 0 && frc) {
+        // every atom has exactly one slot, so this is the only writer
+        if (overwrite) {
+            frc[3 * at] = fix;
+            frc[3 * at + 1] = fiy;
+            frc[3 * at + 2] = fiz;
+        } else {
+            frc[3 * at] += fix;
+            frc[3 * at + 1] += fiy;
+            frc[3 * at + 2] += fiz;
+        }
+    }
+}
+
 // Cutoff test shared with the oracle: FP32 dr2 = (dx*dx + dy*dy) + dz*dz with
 // every operation rounded (the neighbor list's expression) against rc*rc in
 // FP32, so both sides agree on every pair, including those within one ulp of
@@ -172,7 +337,7 @@ __device__ __forceinline__ void pair_body(
     int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
     float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
-    const float4* __restrict__ sc, PruneArgs pa = PruneArgs{0, 0.f, nullptr, nullptr},
+    const float4* __restrict__ sc, PruneArgs pa = PruneArgs{},
     ExclArgs ex = ExclArgs{nullptr, nullptr, nullptr}) {
     extern __shared__ float2 sab[];
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
@@ -181,15 +346,32 @@ __device__ __forceinline__ void pair_body(
     const int ic = lb * kPairWaves + w;
     if (lb < 0 || ic >= flags[4]) return;
     // Inner list (pa.interval > 0): on every pa.interval-th fused step since
-    // the last rebuild this launch reads the full list and writes the entries
-    // and bits within pa.r2 as the inner list; the other steps read the inner
-    // list. The counter was advanced by this step's prep kernel.
+    // the last rebuild this launch prunes the full list to the pairs within
+    // pa.r2; the other steps read the pruned list. The counter was advanced by
+    // this step's prep kernel. Plain LJ keeps the pruned pairs per i slot
+    // (force-only steps read them, prune steps included; energy steps read
+    // the full list), soft-core LJ as cluster pairs.
     bool use_inner = false, write_inner = false;
     if (pa.interval > 0) {
         const int c = flags[10];
         write_inner = c <= 1 || (c - 1) % pa.interval == 0;
         use_inner = !write_inner;
     }
+    const bool slots = !SOFT && pa.atom_list != nullptr && pa.interval > 0;
+    if (slots && !ENERGY) {
+        const int n = write_inner ? prune_slots<TB>(ic, list_n, list, list_cap, pk, cx, cy, cz, pa)
+                                  : pa.atom_n[ic * kClusterSize + (lane >> 4)];
+        const int nmax = max(max(__shfl(n, 0), __shfl(n, 16)), max(__shfl(n, 32), __shfl(n, 48)));
+        if (nmax <= pa.atom_cap) {
+            if (ic == 0 && lane == 0) atomicAdd(pa.diag + 1, 1);
+            pair_atoms<TB>(ic, n, nmax, pk, sab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite, pa, ex);
+            return;
+        }
+        write_inner = false;  // a slot overflowed: this cluster's forces from the full list
+    }
+    const bool write_atoms = slots && write_inner;      // energy steps that prune
+    const bool write_cluster = !slots && write_inner;
+    if (slots) use_inner = false;
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
     const int islot = ic * kClusterSize + ii;
     const uint4 a = pk[islot];
@@ -205,8 +387,8 @@ __device__ __forceinline__ void pair_body(
     double tlj = 0.0, tdir = 0.0, tvir = 0.0, tcnt = 0.0;
     const int n = use_inner ? pa.in_n[ic] : list_n[ic];
     const uint2* lst = (use_inner ? pa.in_list : list) + (size_t)ic * list_cap;
-    uint2* inner_out = write_inner ? pa.in_list + (size_t)ic * list_cap : nullptr;
-    int inner_count = 0;
+    uint2* inner_out = write_cluster ? pa.in_list + (size_t)ic * list_cap : nullptr;
+    int inner_count = 0, atom_count = 0;
     for (int base = 0; base < n; base += 64) {
         // 64 list entries (16 iterations of 4 j-clusters) held one per lane.
         const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
@@ -228,12 +410,22 @@ __device__ __forceinline__ void pair_body(
             if (write_inner) {
                 const bool keep = ((m >> bit) & 1u) && r2 < pa.r2;
                 const unsigned long long kb = __ballot(keep);
-                const unsigned mk = (unsigned)(kb >> (g * 16)) & 0xffffu;
-                const bool leader = (lane & 15) == 0 && mk != 0u;
-                const unsigned long long lb2 = __ballot(leader);
-                if (leader)
-                    inner_out[inner_count + __popcll(lb2 & ((1ull << lane) - 1ull))] = make_uint2((unsigned)jc, mk);
-                inner_count += __popcll(lb2);
+                if (write_cluster) {
+                    const unsigned mk = (unsigned)(kb >> (g * 16)) & 0xffffu;
+                    const bool leader = (lane & 15) == 0 && mk != 0u;
+                    const unsigned long long lb2 = __ballot(leader);
+                    if (leader)
+                        inner_out[inner_count + __popcll(lb2 & ((1ull << lane) - 1ull))] =
+                            make_uint2((unsigned)jc, mk);
+                    inner_count += __popcll(lb2);
+                } else {
+                    // lanes ii, ii + 4, ... append to slot ii's list in lane order
+                    const unsigned long long mi = kb & (0x1111111111111111ull << ii);
+                    const int pos = atom_count + __popcll(mi & ((1ull << lane) - 1ull));
+                    if (keep && pos < pa.atom_cap)
+                        pa.atom_list[(size_t)islot * pa.atom_cap + pos] = jc * kClusterSize + jj;
+                    atom_count += __popcll(mi);
+                }
             }
             // masked lanes (padding, the atom itself) may give inf / NaN
             // below; the final select discards them
@@ -300,31 +492,16 @@ __device__ __forceinline__ void pair_body(
             jc = jcn;
         }
     }
-    if (write_inner && lane == 0) pa.in_n[ic] = inner_count;
+    if (write_cluster && lane == 0) pa.in_n[ic] = inner_count;
+    if (write_atoms && lane < kClusterSize) {
+        pa.atom_n[islot] = atom_count;
+        if (atom_count > pa.atom_cap) atomicOr(pa.diag, 1);
+    }
     if (ex.start) {
-        // PME_Excluded_Force_With_Atom_Energy's force (PME_force.cu:620-695)
-        // for this cluster's atoms: lane (ii, k) takes the k-th exclusion
-        // partner of atom ii (16 per pass) from the step's slot-ordered
-        // records; each atom collects its own side of every excluded pair, so
-        // no other atom's force is written.
+        // this cluster's atoms: lane (ii, k) takes the k-th exclusion partner
+        // of atom ii, 16 per pass
         const int at = s2o[islot];
-        if (at >= 0) {
-            const int r1 = ex.start[at + 1];
-            for (int k = ex.start[at] + (lane >> 2); k < r1; k += 16) {
-                const uint4 b = pk[ex.o2s[ex.list[k]]];
-                const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
-                const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
-                const float dz = (float)(int)((b.z & ~3u) - az) * cz;
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const float r = sqrtf(r2);
-                const float br = beta * r;
-                const float qq = qi * __uint_as_float(b.w);
-                const float fa = -qq * ((kTwoOverSqrtPi * br * expf(-br * br) + erfcf(br) - 1.0f) / r2 / r);
-                fix += fa * dx;
-                fiy += fa * dy;
-                fiz += fa * dz;
-            }
-        }
+        if (at >= 0) excluded_partners(ex, pk, at, lane >> 2, 16, ax, ay, az, qi, cx, cy, cz, beta, fix, fiy, fiz);
     }
     // i force: sum the 16 lanes sharing ii (lanes ii, ii+4, ..., ii+60).
 #pragma unroll
@@ -722,8 +899,17 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered,
               int overwrite, const float4* sc, int inner, ExclArgs ex) {
     inner = inner && nl->prune_interval > 0;
-    const PruneArgs pa = inner ? PruneArgs{nl->prune_interval, nl->prune_r2, nl->list_in_n, nl->list_in}
-                               : PruneArgs{0, 0.f, nullptr, nullptr};
+    PruneArgs pa{};
+    if (inner) {
+        pa.interval = nl->prune_interval;
+        pa.r2 = nl->prune_r2;
+        pa.in_n = nl->list_in_n;
+        pa.in_list = nl->list_in;
+        pa.atom_n = nl->atom_in_n;
+        pa.atom_list = nl->atom_in;
+        pa.atom_cap = nl->atom_in_cap;
+        pa.diag = nl->flags + 11;
+    }
     const int* list_n = nl->list_n;
     const uint2* list = nl->list;
     int rc = scratch_ensure(ps, nl);

@@ -24,6 +24,7 @@
 
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -594,6 +595,131 @@ __global__ __launch_bounds__(kConvThreads) void conv2d_batch(const BatchSys* __r
                 P.bin_count);
 }
 
+// Split convolution for large planes: one workgroup per plane leaves most of
+// the chip idle (55 planes of 108 x 108 at a 108^3 grid) and cannot hold
+// planes past ~128 x 128 in LDS, so the same steps run as three launches of
+// kLines lines per workgroup: y-lines forward, x-lines forward + energy + BC +
+// inverse, y-lines inverse. Each line is read and written once per launch.
+constexpr int kLines = 16;
+constexpr int kLineThreads = 256;
+// largest Kx * Ky that runs as one workgroup per plane by default
+constexpr int kConvPlaneMax = 64 * 64;
+
+// kLines consecutive x rows of one kz plane (contiguous in FQ), transformed
+// along y in place.
+template <int SIGN, int RM>
+__device__ __forceinline__ void conv_y_body(float2* __restrict__ FQ, int Kx, int Ky, const FftPlan& py,
+                                            const float2* __restrict__ twy) {
+    extern __shared__ float2 sm[];
+    const int pitch = line_pitch(Ky);
+    float2* ty = sm + kLines * pitch;
+    const int x0 = blockIdx.x * kLines, nl = min(kLines, Kx - x0);
+    float2* rows = FQ + ((size_t)blockIdx.y * Kx + x0) * Ky;
+    load_twiddles<kLineThreads>(ty, twy, Ky);
+    const int np = nl * Ky;
+    const float rky = 1.0f / (float)Ky;
+    for (int idx = threadIdx.x; idx < np; idx += kLineThreads) {
+        const int r = div_small(idx, rky), y = idx - r * Ky;
+        sm[r * pitch + pd(y)] = rows[idx];
+    }
+    __syncthreads();
+    lds_fft<SIGN, kLineThreads, RM>(sm, nl, pitch, 1, py, ty);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < np; idx += kLineThreads) {
+        const int r = div_small(idx, rky), y = idx - r * Ky;
+        rows[idx] = sm[r * pitch + pd(y)];
+    }
+}
+
+template <int RM>
+__global__ __launch_bounds__(kLineThreads) void conv_y_forward_kernel(float2* __restrict__ FQ, int Kx, int Ky,
+                                                                      FftPlan py, const float2* __restrict__ twy) {
+    conv_y_body<-1, RM>(FQ, Kx, Ky, py, twy);
+}
+
+template <int RM>
+__global__ __launch_bounds__(kLineThreads) void conv_y_inverse_kernel(float2* __restrict__ FQ, int Kx, int Ky,
+                                                                      FftPlan py, const float2* __restrict__ twy) {
+    conv_y_body<1, RM>(FQ, Kx, Ky, py, twy);
+}
+
+// kLines consecutive y columns of one kz plane: forward x transform, energy
+// 1/2 sum m BC |F|^2 (FP64, as conv2d_body), multiply by BC, inverse x
+// transform. Column c of the block is LDS line c (x contiguous).
+template <int RM>
+__global__ __launch_bounds__(kLineThreads) void conv_x_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
+                                                              int Kx, int Ky, int Kz, FftPlan px,
+                                                              const float2* __restrict__ twx, int energy,
+                                                              double* __restrict__ acc, int* __restrict__ bin_count) {
+    extern __shared__ float2 sm[];
+    __shared__ double red[kLineThreads / 64];
+    const int pitch = line_pitch(Kx);
+    float2* tx = sm + kLines * pitch;
+    const int kz = blockIdx.y, y0 = blockIdx.x * kLines, nc = min(kLines, Ky - y0);
+    // the spread has consumed the x-plane bins: clear them for the next prep
+    if (kz == 0 && blockIdx.x == 0)
+        for (int b = threadIdx.x; b < Kx; b += kLineThreads) bin_count[b] = 0;
+    load_twiddles<kLineThreads>(tx, twx, Kx);
+    float2* cols = FQ + (size_t)kz * Kx * Ky + y0;
+    const float* bcp = bc + (size_t)kz * Kx * Ky + y0;
+    const int np = Kx * kLines;
+    // BC values of up to kConvBatch points per thread wait in registers
+    // through the forward transform
+    float bv[kConvBatch];
+    const bool onepass = np <= kConvBatch * kLineThreads;
+    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * kLineThreads) {
+        float2 v[kConvBatch];
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * kLineThreads, x = idx / kLines, c = idx % kLines;
+            if (idx < np && c < nc) {
+                v[s] = cols[(size_t)x * Ky + c];
+                if (onepass) bv[s] = bcp[(size_t)x * Ky + c];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * kLineThreads, x = idx / kLines, c = idx % kLines;
+            if (idx < np && c < nc) sm[c * pitch + pd(x)] = v[s];
+        }
+    }
+    __syncthreads();
+    lds_fft<-1, kLineThreads, RM>(sm, nc, pitch, 1, px, tx);
+    __syncthreads();
+    double e = 0.0;
+    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * kLineThreads) {
+        if (!onepass) {
+#pragma unroll
+            for (int s = 0; s < kConvBatch; s++) {
+                const int idx = s0 + s * kLineThreads, x = idx / kLines, c = idx % kLines;
+                if (idx < np && c < nc) bv[s] = bcp[(size_t)x * Ky + c];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * kLineThreads, x = idx / kLines, c = idx % kLines;
+            if (idx < np && c < nc) {
+                const float b = bv[s];
+                const float2 w = sm[c * pitch + pd(x)];
+                if (energy) e += (double)b * ((double)w.x * w.x + (double)w.y * w.y);
+                sm[c * pitch + pd(x)] = make_float2(w.x * b, w.y * b);
+            }
+        }
+    }
+    __syncthreads();
+    lds_fft<1, kLineThreads, RM>(sm, nc, pitch, 1, px, tx);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < np; idx += kLineThreads) {
+        const int x = idx / kLines, c = idx % kLines;
+        if (c < nc) cols[(size_t)x * Ky + c] = sm[c * pitch + pd(x)];
+    }
+    if (energy) {
+        const double t = block_sum<kLineThreads>(e, red);
+        const double m = (kz == 0 || 2 * kz == Kz) ? 1.0 : 2.0;
+        if (threadIdx.x == 0) atomicAdd(acc + SPONGE_PART_RECIPROCAL, 0.5 * m * t);
+    }
+}
+
 // Inverse z transform (C2R) of one block of kRowBlock y rows of one x plane:
 // rows (2p, 2p+1) are repacked into one complex line Z = A + i B over all kz,
 // each row extended by Hermitian symmetry (self-conjugate bins keep only their
@@ -879,6 +1005,7 @@ size_t inv_z_lds(int Kz) {
     return (size_t)(kRowBlock * (Kz / 2 + 1) + (kRowBlock / 2) * line_pitch(Kz) + Kz) * sizeof(float2);
 }
 size_t conv2d_lds(int Kx, int Ky) { return (size_t)(padded_len(Kx) * line_pitch(Ky) + Kx + Ky) * sizeof(float2); }
+size_t lines_lds(int K) { return (size_t)(kLines * line_pitch(K) + K) * sizeof(float2); }
 
 // Lets every transform kernel use up to `bytes` of dynamic LDS (once per
 // size; the attribute is per function).
@@ -892,7 +1019,13 @@ int raise_lds_limits(size_t bytes) {
                          (const void*)conv2d_kernel<kRadix34>,    (const void*)conv2d_kernel<kRadix248>,
                          (const void*)conv2d_batch,               (const void*)inv_z_kernel<kRadixAll>,
                          (const void*)inv_z_kernel<kRadix8>,      (const void*)inv_z_kernel<kRadix34>,
-                         (const void*)inv_z_kernel<kRadix248>,    (const void*)inv_z_batch};
+                         (const void*)inv_z_kernel<kRadix248>,    (const void*)inv_z_batch,
+                         (const void*)conv_x_kernel<kRadixAll>,   (const void*)conv_x_kernel<kRadix8>,
+                         (const void*)conv_x_kernel<kRadix34>,    (const void*)conv_x_kernel<kRadix248>,
+                         (const void*)conv_y_forward_kernel<kRadixAll>, (const void*)conv_y_forward_kernel<kRadix8>,
+                         (const void*)conv_y_forward_kernel<kRadix34>,  (const void*)conv_y_forward_kernel<kRadix248>,
+                         (const void*)conv_y_inverse_kernel<kRadixAll>, (const void*)conv_y_inverse_kernel<kRadix8>,
+                         (const void*)conv_y_inverse_kernel<kRadix34>,  (const void*)conv_y_inverse_kernel<kRadix248>};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
             set_error("cannot raise the LDS limit of the PME kernels to %zu B", bytes);
@@ -1003,7 +1136,15 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
         max_lds = 64 * 1024;
-    const size_t lds_need = std::max({spread_z_lds(Kz), inv_z_lds(Kz), conv2d_lds(Kx, Ky)});
+    // planes beyond 64 x 64 (or beyond the LDS) run the split convolution;
+    // SPONGE_PME_SPLIT_CONV=0/1 forces either form where it fits
+    pme->conv_split = (Kx * Ky > kConvPlaneMax || conv2d_lds(Kx, Ky) > (size_t)max_lds) ? 1 : 0;
+    if (const char* env = getenv("SPONGE_PME_SPLIT_CONV")) {
+        if (env[0] == '1') pme->conv_split = 1;
+        if (env[0] == '0' && conv2d_lds(Kx, Ky) <= (size_t)max_lds) pme->conv_split = 0;
+    }
+    const size_t conv_lds = pme->conv_split ? std::max(lines_lds(Kx), lines_lds(Ky)) : conv2d_lds(Kx, Ky);
+    const size_t lds_need = std::max({spread_z_lds(Kz), inv_z_lds(Kz), conv_lds});
     if (lds_need > (size_t)max_lds || Kx > 512 || Ky > 512 || Kz > 512) {
         set_error("PME grid %dx%dx%d too large for the in-LDS transforms (%zu B of LDS, %d available)", Kx, Ky, Kz,
                   lds_need, max_lds);
@@ -1148,10 +1289,24 @@ static int convolve_launch(sponge_pme* pme, int energy, hipStream_t s) {
     plan_axis(Ky, &py);
     plan_axis(Kz, &pz);
     const int nyb = row_blocks(Ky);
-    SNB_RADIX_DISPATCH(kernel_set(radix_mask(px) | radix_mask(py)), conv2d_kernel, H, kConvThreads,
-                       conv2d_lds(Kx, Ky), s, pme->FQ, pme->d_bc, Kx, Ky, Kz, px, py, pme->d_tw[0], pme->d_tw[1],
-                       energy, pme->acc, pme->bin_count);
-    SNB_CHECK_LAUNCH();
+    if (pme->conv_split) {
+        const dim3 gy(ceil_div(Kx, kLines), H), gx(ceil_div(Ky, kLines), H);
+        const int sy = kernel_set(radix_mask(py)), sx = kernel_set(radix_mask(px));
+        SNB_RADIX_DISPATCH(sy, conv_y_forward_kernel, gy, kLineThreads, lines_lds(Ky), s, pme->FQ, Kx, Ky, py,
+                           pme->d_tw[1]);
+        SNB_CHECK_LAUNCH();
+        SNB_RADIX_DISPATCH(sx, conv_x_kernel, gx, kLineThreads, lines_lds(Kx), s, pme->FQ, pme->d_bc, Kx, Ky, Kz, px,
+                           pme->d_tw[0], energy, pme->acc, pme->bin_count);
+        SNB_CHECK_LAUNCH();
+        SNB_RADIX_DISPATCH(sy, conv_y_inverse_kernel, gy, kLineThreads, lines_lds(Ky), s, pme->FQ, Kx, Ky, py,
+                           pme->d_tw[1]);
+        SNB_CHECK_LAUNCH();
+    } else {
+        SNB_RADIX_DISPATCH(kernel_set(radix_mask(px) | radix_mask(py)), conv2d_kernel, H, kConvThreads,
+                           conv2d_lds(Kx, Ky), s, pme->FQ, pme->d_bc, Kx, Ky, Kz, px, py, pme->d_tw[0],
+                           pme->d_tw[1], energy, pme->acc, pme->bin_count);
+        SNB_CHECK_LAUNCH();
+    }
     SNB_RADIX_DISPATCH(kernel_set(radix_mask(pz)), inv_z_kernel, Kx * nyb, kZThreads, inv_z_lds(Kz), s, pme->FQ,
                        pme->R, Kx, Ky, Kz, nyb, pz, pme->d_tw[2]);
     SNB_CHECK_LAUNCH();
@@ -1226,6 +1381,19 @@ int snb_pme_row_blocks(const sponge_pme* pme) { return row_blocks(pme->K[1]); }
 size_t snb_pme_lds_fwd(const sponge_pme* pme) { return spread_z_lds(pme->K[2]); }
 
 size_t snb_pme_lds_conv(const sponge_pme* pme) { return conv2d_lds(pme->K[0], pme->K[1]); }
+
+// The batched kernels hold a whole plane per workgroup also where the
+// system's own pipeline runs the split convolution.
+int snb_pme_raise_lds(size_t bytes) {
+    int dev = 0, max_lds = 0;
+    SNB_HIP(hipGetDevice(&dev));
+    SNB_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (bytes > (size_t)max_lds) {
+        set_error("batched PME transforms need %zu B of LDS per workgroup, %d available", bytes, max_lds);
+        return SPONGE_NB_EARG;
+    }
+    return raise_lds_limits(bytes);
+}
 
 size_t snb_pme_lds_inv(const sponge_pme* pme) { return inv_z_lds(pme->K[2]); }
 

@@ -41,6 +41,7 @@ _SIGS = {
     "sponge_nl_update": (_I, [_P, _P, _P, _P, _P, _I, _I, _P]),
     "sponge_nl_update_volume": (_I, [_P, _FP3]),
     "sponge_nl_set_prune": (_I, [_P, _I, _F]),
+    "sponge_nl_inner_status": (_I, [_P, _P, _P]),
     "sponge_nl_status": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "sponge_nl_get_pairs": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
     "sponge_lj_create": (_I, [_I, _I, _P, _P, _P, _F, _FP3, ctypes.POINTER(_P)]),
@@ -193,6 +194,14 @@ class NEIGHBOR_LIST:
         pc, fl, rb = _L(0), _I(0), _I(0)
         rc = load_library().sponge_nl_status(self.h, ctypes.byref(pc), ctypes.byref(fl), ctypes.byref(rb))
         return dict(pairs=pc.value, overflow=fl.value, rebuilds=rb.value, ok=rc == 0)
+
+    def inner_status(self):
+        """Per-slot inner lists of the fused step: overflow at the last prune
+        and the number of pair-kernel launches that read them."""
+        ov, nl_ = _I(0), _I(0)
+        _check(load_library().sponge_nl_inner_status(self.h, ctypes.byref(ov), ctypes.byref(nl_)),
+               "sponge_nl_inner_status")
+        return dict(slot_overflow=ov.value, slot_launches=nl_.value)
 
     def pairs(self):
         """(P,2) int32 canonical pairs (i<j), sorted lexicographically."""

@@ -64,13 +64,36 @@ CASES = {
     # soft-core FEP window: 12-atom solute decoupled at lambda 0.35 (LJ_SOFT_CORE)
     "fep_window": lambda: system.fep_system(700, 0.35, seed=3),
 }
+# The same systems with the 2D transforms of the convolution forced into the
+# split form (line launches of 16 lines; planes above 64 x 64 use it by
+# default): partial line blocks (12 x 14, 16 x 108 planes) and the 64 x 64
+# planes of the 23,570-atom box.
+SPLIT = "_split"
+for _name in ("random_biggrid", "random_noncubic", "random_smallbox", "water23570"):
+    CASES[_name + SPLIT] = CASES[_name]
+
+
+def make_case_engine(name, sys_):
+    import os
+    if not name.endswith(SPLIT):
+        return make_engine(sys_)
+    key = "SPONGE_PME_SPLIT_CONV"
+    old = os.environ.get(key)
+    os.environ[key] = "1"
+    try:
+        return make_engine(sys_)
+    finally:
+        if old is None:
+            del os.environ[key]
+        else:
+            os.environ[key] = old
 
 
 @pytest.fixture(scope="module", params=sorted(CASES))
 def case(request):
     sys_ = CASES[request.param]()
     say(f"[case] {request.param} n={sys_.n} box={sys_.box.tolist()}")
-    eng = make_engine(sys_)
+    eng = make_case_engine(request.param, sys_)
     run_step(eng)
     ref = oracle.full_reference(sys_, beta=eng.pme.beta,
                                 K=(eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
@@ -258,7 +281,9 @@ def test_graph_replay_matches_eager(case):
 def test_pruned_inner_list(case):
     """Interval mode prunes the fused step's list to an inner radius (default
     rc + skin / 4, every refresh / 4 steps); forces stay those of the full list,
-    also when pruned to exactly rc every step."""
+    also when pruned to exactly rc every step. The force-only steps of plain
+    LJ read the per-slot inner lists (all three steps here, the pruning one
+    included; soft-core systems keep the cluster form)."""
     import torch
     from sponge_amd import nb
     name, sys_, eng, ref = case
@@ -267,11 +292,18 @@ def test_pruned_inner_list(case):
         if interval is not None:
             e2.nl.set_prune(interval, radius)
         e2.update_neighbor_list(forced=1)
+        before = e2.nl.inner_status()
         for _ in range(3):
             e2.force_step(0, 0, from_crd=True, overwrite=True)
         torch.cuda.synchronize()
+        st = e2.nl.inner_status()
         f = e2.frc.cpu().numpy()
+        say(f"[{name}] prune {interval} {radius}: per-slot launches {st['slot_launches'] - before['slot_launches']}, "
+            f"overflow {st['slot_overflow']}, force rel rms {rel_rms(f, ref['frc']):.3e}")
         assert rel_rms(f, ref["frc"]) < FORCE_TOL, (interval, radius)
+        assert st["slot_overflow"] == 0
+        expect = 0 if name == "fep_window" else 3
+        assert st["slot_launches"] - before["slot_launches"] == expect
         e2.force_step(1, 1, from_crd=True, overwrite=True)
         p = e2.energy_parts()
         assert p["pairs_in"] == ref["n_in"], (interval, radius)
