@@ -43,11 +43,17 @@ constexpr int kSpreadPerThread = 8;
 // Fixed-point unit of the spread grid: one contribution q*Mx*My*Mz must stay
 // below 2^31 units, i.e. |q| < 128 / (2/3)^3 = 432 (18.2223 x 23.7 e).
 constexpr float kSpreadScale = 16777216.0f;
-// one block of the convolution kernel holds a whole Kx x Ky plane
-constexpr int kConvThreads = 512;
-// global loads in flight per thread in the convolution kernel (a 64 x 64
-// plane in one round trip), and spectrum
-// points per thread of the inverse z kernel (16 rows x (Kz/2 + 1), Kz <= 512)
+// one block of the convolution kernel holds a whole Kx x Ky plane; four
+// waves, one per SIMD, so that a block finds room on CUs the concurrent pair
+// kernel already fills
+constexpr int kConvThreads = 256;
+// Issue priority of the reciprocal pipeline's waves (s_setprio): the pipeline
+// is the fused step's critical path and shares the CUs with the pair kernel's
+// waves, which have slack
+constexpr int kPmeWavePriority = 3;
+// global loads in flight per thread in the convolution kernels (a 64 x 64
+// plane in two round trips), and spectrum points per thread of the inverse z
+// kernel (16 rows x (Kz/2 + 1), Kz <= 512)
 constexpr int kConvBatch = 8;
 constexpr int kZPer = 17;
 constexpr int kGatherThreads = 256;
@@ -324,6 +330,7 @@ __device__ __forceinline__ void spread_z_body(
     const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz,
     const FftPlan& pz, const float2* __restrict__ twz, double* __restrict__ acc, const float4* __restrict__ wv = nullptr,
     const float* __restrict__ scale_ptr = nullptr) {
+    __builtin_amdgcn_s_setprio(kPmeWavePriority);
     extern __shared__ float2 sm[];
     const float unit = MODE ? scale_ptr[0] : kSpreadScale;
     const int H = Kz / 2 + 1, zp = line_pitch(Kz);
@@ -495,6 +502,7 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
                                             int Ky, int Kz, const FftPlan& px, const FftPlan& py,
                                             const float2* __restrict__ twx, const float2* __restrict__ twy, int energy,
                                             double* __restrict__ acc, int* __restrict__ bin_count) {
+    __builtin_amdgcn_s_setprio(kPmeWavePriority);
     extern __shared__ float2 sm[];
     __shared__ double red[kConvThreads / 64];
     const int pitch = line_pitch(Ky);
@@ -610,6 +618,7 @@ constexpr int kConvPlaneMax = 64 * 64;
 template <int SIGN, int RM>
 __device__ __forceinline__ void conv_y_body(float2* __restrict__ FQ, int Kx, int Ky, const FftPlan& py,
                                             const float2* __restrict__ twy) {
+    __builtin_amdgcn_s_setprio(kPmeWavePriority);
     extern __shared__ float2 sm[];
     const int pitch = line_pitch(Ky);
     float2* ty = sm + kLines * pitch;
@@ -651,6 +660,7 @@ __global__ __launch_bounds__(kLineThreads) void conv_x_kernel(float2* __restrict
                                                               int Kx, int Ky, int Kz, FftPlan px,
                                                               const float2* __restrict__ twx, int energy,
                                                               double* __restrict__ acc, int* __restrict__ bin_count) {
+    __builtin_amdgcn_s_setprio(kPmeWavePriority);
     extern __shared__ float2 sm[];
     __shared__ double red[kLineThreads / 64];
     const int pitch = line_pitch(Kx);
@@ -728,6 +738,7 @@ __global__ __launch_bounds__(kLineThreads) void conv_x_kernel(float2* __restrict
 template <int RM = kRadixAll>
 __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
                                            int Ky, int Kz, const FftPlan& pz, const float2* __restrict__ twz) {
+    __builtin_amdgcn_s_setprio(kPmeWavePriority);
     extern __shared__ float2 sm[];
     const int H = Kz / 2 + 1, zp = line_pitch(Kz);
     const int y0 = yb * kRowBlock, nr = min(kRowBlock, Ky - y0), nrp = (nr + 1) / 2;
