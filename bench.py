@@ -32,7 +32,7 @@ PAIR_FLOPS = 50              # per in-cutoff pair, force-only variant
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same launch, each scaled by
 # what the counter reads for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0);
 # the runs and their commands are kept under profiles/.
-PAIR_TRAFFIC_BYTES = 18.28e6
+PAIR_TRAFFIC_BYTES = 65.47e6
 
 
 def cpu_baseline(sys_, seconds: float, beta, K):
