@@ -171,23 +171,40 @@ __device__ __forceinline__ void wave_pass(float2* buf, int nlines, int ls, int k
         const int jn = div_small(j, rns);
         const int k = j - jn * Ns;
         const int d = jn * Ns * R + k;
-        for (int i0 = 0; wid + nw * i0 < nlines; i0 += G) {
-            const int l = wid + nw * (i0 + sub);
-            if (sub < G && l < nlines) {
-                float2 v[R];
+        // Two groups of G lines per round: every LDS read of both groups
+        // (elements and twiddles) is issued before the first butterfly, so
+        // one read latency is exposed per two groups. A lane without a line
+        // in a group reads line 0 and discards it.
+        float2 t[R > 1 ? R - 1 : 1];
 #pragma unroll
-                for (int q = 0; q < R; q++) {
-                    float2 x = buf[pd(l) * ls + pd(j + q * per) * ks];
-                    if (q > 0) {
-                        float2 t = tw[q * k * tws];
-                        if (SIGN > 0) t.y = -t.y;
-                        x = cmul(x, t);
-                    }
-                    v[q] = x;
-                }
-                dft<R, SIGN>(v);
+        for (int q = 1; q < R; q++) {
+            t[q - 1] = tw[q * k * tws];
+            if (SIGN > 0) t[q - 1].y = -t[q - 1].y;
+        }
+        for (int i0 = 0; wid + nw * i0 < nlines; i0 += 2 * G) {
+            const int la = wid + nw * (i0 + sub), lb = la + nw * G;
+            const bool oka = sub < G && la < nlines, okb = sub < G && lb < nlines;
+            const int ra = pd(oka ? la : 0) * ls, rb = pd(okb ? lb : 0) * ls;
+            float2 va[R], vb[R];
 #pragma unroll
-                for (int q = 0; q < R; q++) buf[pd(l) * ls + pd(d + q * Ns) * ks] = v[q];
+            for (int q = 0; q < R; q++) {
+                va[q] = buf[ra + pd(j + q * per) * ks];
+                vb[q] = buf[rb + pd(j + q * per) * ks];
+            }
+#pragma unroll
+            for (int q = 1; q < R; q++) {
+                va[q] = cmul(va[q], t[q - 1]);
+                vb[q] = cmul(vb[q], t[q - 1]);
+            }
+            dft<R, SIGN>(va);
+            dft<R, SIGN>(vb);
+            if (oka) {
+#pragma unroll
+                for (int q = 0; q < R; q++) buf[ra + pd(d + q * Ns) * ks] = va[q];
+            }
+            if (okb) {
+#pragma unroll
+                for (int q = 0; q < R; q++) buf[rb + pd(d + q * Ns) * ks] = vb[q];
             }
         }
     } else {
