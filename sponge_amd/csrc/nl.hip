@@ -419,30 +419,40 @@ __device__ __forceinline__ void nl_pairs_body(
     };
     int count = 0;
     long npairs = 0;
-    // the next group's j record is loaded while this group is tested
-    int jc = g < ncand ? cand[w][g] : -1;
-    uint4 pj = jc >= 0 ? pack[jc * kClusterSize + jj] : make_uint4(0u, 0u, 0u, 0u);
-    for (int cb = 0; cb < ncand; cb += 4) {
-        const int jcn = cb + 4 + g < ncand ? cand[w][cb + 4 + g] : -1;
-        const uint4 pjn = jcn >= 0 ? pack[jcn * kClusterSize + jj] : make_uint4(0u, 0u, 0u, 0u);
-        bool bit = false;
-        if (jc >= 0) {
-            int aj = (int)pj.w;
-            bool valid = ai >= 0 && aj >= 0 && (jc != ic || jj != ii);
-            if (valid && nl_dr2(pi, pj, crx, cry, crz) < rl2) bit = !is_excluded(aj);
+    // four groups of 4 candidates per pass: their j records are loaded
+    // together, then tested and appended in candidate order (the loop is
+    // latency bound; the list stores of a pass no longer stall the next
+    // group's loads)
+    for (int cb = 0; cb < ncand; cb += 16) {
+        int jcs[4];
+        uint4 pjs[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int c = cb + 4 * u + g;
+            jcs[u] = c < ncand ? cand[w][c] : -1;
+            pjs[u] = jcs[u] >= 0 ? pack[jcs[u] * kClusterSize + jj] : make_uint4(0u, 0u, 0u, 0u);
         }
-        unsigned long long bal = __ballot(bit);
-        unsigned m = (unsigned)(bal >> (g * 16)) & 0xffffu;
-        bool leader = (lane & 15) == 0 && m != 0;
-        unsigned long long lb2 = __ballot(leader);
-        int slot = count + __popcll(lb2 & ((1ull << lane) - 1ull));
-        if (leader) {
-            if (slot < list_cap) list[(size_t)ic * list_cap + slot] = make_uint2((unsigned)jc, m);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int jc = jcs[u];
+            const uint4 pj = pjs[u];
+            bool bit = false;
+            if (jc >= 0) {
+                int aj = (int)pj.w;
+                bool valid = ai >= 0 && aj >= 0 && (jc != ic || jj != ii);
+                if (valid && nl_dr2(pi, pj, crx, cry, crz) < rl2) bit = !is_excluded(aj);
+            }
+            unsigned long long bal = __ballot(bit);
+            unsigned m = (unsigned)(bal >> (g * 16)) & 0xffffu;
+            bool leader = (lane & 15) == 0 && m != 0;
+            unsigned long long lb2 = __ballot(leader);
+            int slot = count + __popcll(lb2 & ((1ull << lane) - 1ull));
+            if (leader) {
+                if (slot < list_cap) list[(size_t)ic * list_cap + slot] = make_uint2((unsigned)jc, m);
+            }
+            count += __popcll(lb2);
+            npairs += __popcll(bal);
         }
-        count += __popcll(lb2);
-        npairs += __popcll(bal);
-        jc = jcn;
-        pj = pjn;
     }
     if (lane == 0) {
         if (count > list_cap) {
