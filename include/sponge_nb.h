@@ -183,9 +183,11 @@ int sponge_lj_soft_create(int atom_numbers, int atom_type_numbers_A, int atom_ty
                           const float box_length[3], sponge_lj** out);
 
 /* Measurement hook (no counterpart in the reference): when set, every later
- * pair-kernel launch of this module is bracketed by hipEventRecord of the
- * two hipEvent_t on the launch stream, so a caller can time the dominant
- * kernel inside a timed region. NULL events switch the hook off. */
+ * pair-kernel launch of this module records the two hipEvent_t as its start
+ * and end (hipExtLaunchKernel on the launch stream: the kernel's own times,
+ * without the latency of marker packets around it), so a caller can time the
+ * dominant kernel inside a timed region. Both events or neither; NULL events
+ * switch the hook off. Not for graph capture. */
 int sponge_lj_set_pair_events(sponge_lj* lj, void* start_event, void* stop_event);
 
 /* ------------------------------------------------------------------ */

@@ -20,6 +20,8 @@
 #include <tuple>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "common.hpp"
 #include "pair_math.hpp"
 #include "pme_prep.hpp"
@@ -923,17 +925,29 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     const int nblk = xcd_grid(ceil_div(nl->ncl_cap, kPairWaves));
     const float rc2 = cutoff * cutoff;
     const size_t lds = (size_t)nt * nt * sizeof(float2);
-    if (ps->ev_start) SNB_HIP(hipEventRecord(ps->ev_start, s));
+    // With timing events set, the launch itself records them (hipExtLaunchKernel
+    // takes the kernel's own start and end times; events recorded around a
+    // launch add the marker packets' latency, ~8 us per launch here)
+    const bool timed = ps->ev_start && ps->ev_stop;
 #define SNB_PAIR(E, S)                      \
     do {                                    \
         if (nt <= 4) SNB_PAIR_TB(E, S, 2);  \
         else SNB_PAIR_TB(E, S, 6);          \
     } while (0)
-#define SNB_PAIR_TB(E, S, TB)                                                                                   \
-    pair_kernel<E, S, TB><<<nblk, kPairWaves * 64, lds, s>>>(nl->flags, list_n, list, nl->list_cap, ps->pk, ab,   \
-                                                         nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,         \
-                                                         E ? atom_e : nullptr, E ? atom_v : nullptr,                    \
-                                                         E ? atom_d : nullptr, ps->acc, overwrite, sc, pa, ex)
+#define SNB_PAIR_TB(E, S, TB)                                                                                        \
+    do {                                                                                                            \
+        if (timed)                                                                                                  \
+            hipExtLaunchKernelGGL(pair_kernel<E, S, TB>, dim3(nblk), dim3(kPairWaves * 64), (std::uint32_t)lds, s,   \
+                                  ps->ev_start, ps->ev_stop, 0u, nl->flags, list_n, list, nl->list_cap,              \
+                                  (const uint4*)ps->pk, ab, nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,   \
+                                  E ? atom_e : nullptr, E ? atom_v : nullptr, E ? atom_d : nullptr, ps->acc,         \
+                                  overwrite, sc, pa, ex);                                                            \
+        else                                                                                                        \
+            pair_kernel<E, S, TB><<<nblk, kPairWaves * 64, lds, s>>>(                                               \
+                nl->flags, list_n, list, nl->list_cap, ps->pk, ab, nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o,  \
+                d_frc, E ? atom_e : nullptr, E ? atom_v : nullptr, E ? atom_d : nullptr, ps->acc, overwrite, sc,    \
+                pa, ex);                                                                                            \
+    } while (0)
     if (energy) {
         if (sc) SNB_PAIR(true, true);
         else SNB_PAIR(true, false);
@@ -944,7 +958,6 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
 #undef SNB_PAIR
 #undef SNB_PAIR_TB
     SNB_CHECK_LAUNCH();
-    if (ps->ev_stop) SNB_HIP(hipEventRecord(ps->ev_stop, s));
     return SPONGE_NB_OK;
 }
 
@@ -974,6 +987,7 @@ int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const flo
 
 extern "C" int sponge_lj_set_pair_events(sponge_lj* lj, void* start_event, void* stop_event) {
     SNB_ARG(lj, "lj_set_pair_events");
+    SNB_ARG((start_event == nullptr) == (stop_event == nullptr), "both timing events or neither");
     lj->ps.ev_start = reinterpret_cast<hipEvent_t>(start_event);
     lj->ps.ev_stop = reinterpret_cast<hipEvent_t>(stop_event);
     return SPONGE_NB_OK;
