@@ -91,7 +91,8 @@ __global__ void nl_begin_kernel(int forced, int* flags, int* col_count, int ncol
 __device__ __forceinline__ void nl_bin_body(int blk, int n, float* __restrict__ crd, float* __restrict__ old_crd,
                                             unsigned int* __restrict__ nl_uint, float bx, float by, float bz,
                                             float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap,
-                                            int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags) {
+                                            int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags,
+                                            int* __restrict__ o2s) {
     if (!flags[0]) return;
     int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -116,16 +117,23 @@ __device__ __forceinline__ void nl_bin_body(int blk, int n, float* __restrict__ 
     int cy = (int)(((unsigned long long)u[1] * (unsigned)ncy) >> 32);
     int c = cx * ncy + cy;
     int s = atomicAdd(col_count + c, 1);
-    if (s < col_cap) col_atoms[(size_t)c * col_cap + s] = i;
-    else atomicOr(flags + 1, SPONGE_NL_OVERFLOW_COLUMN);
+    if (s < col_cap) {
+        col_atoms[(size_t)c * col_cap + s] = i;
+    } else {
+        // left out of this list (reported); no slot, which the slot map's
+        // readers test (the sort writes the slots of the others)
+        atomicOr(flags + 1, SPONGE_NL_OVERFLOW_COLUMN);
+        o2s[i] = -1;
+    }
 }
 
 __global__ void nl_bin_kernel(int n, float* __restrict__ crd, float* __restrict__ old_crd,
                               unsigned int* __restrict__ nl_uint, float bx, float by, float bz,
                               float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap,
-                              int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags) {
+                              int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags,
+                              int* __restrict__ o2s) {
     nl_bin_body(blockIdx.x, n, crd, old_crd, nl_uint, bx, by, bz, cqx, cqy, cqz, ncx, ncy, col_cap, col_count,
-                col_atoms, flags);
+                col_atoms, flags, o2s);
 }
 
 // Exclusive scan of the padded (multiple of 4) column sizes; one block.
@@ -496,7 +504,7 @@ __global__ void nl_bin_batch(const BatchSys* __restrict__ bs, const int2* __rest
     const BatchSys& b = bs[m.x];
     const sponge_nl& L = b.nl;
     nl_bin_body(m.y, L.n, b.io.crd, L.old_crd, L.nl_uint, L.box[0], L.box[1], L.box[2], L.cof_q[0], L.cof_q[1],
-                L.cof_q[2], L.ncx, L.ncy, L.col_cap, L.col_count, L.col_atoms, L.flags);
+                L.cof_q[2], L.ncx, L.ncy, L.col_cap, L.col_count, L.col_atoms, L.flags, L.o2s);
 }
 
 __global__ void nl_scan_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map) {
@@ -687,7 +695,8 @@ static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el,
     SNB_CHECK_LAUNCH();
     nl_bin_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_crd, nl->old_crd, nl->nl_uint, nl->box[0], nl->box[1],
                                                    nl->box[2], nl->cof_q[0], nl->cof_q[1], nl->cof_q[2], nl->ncx,
-                                                   nl->ncy, nl->col_cap, nl->col_count, nl->col_atoms, nl->flags);
+                                                   nl->ncy, nl->col_cap, nl->col_count, nl->col_atoms, nl->flags,
+                                                   nl->o2s);
     SNB_CHECK_LAUNCH();
     nl_scan_kernel<<<1, 1024, 0, s>>>(nl->ncol, nl->col_cap, nl->col_count, nl->col_start, nl->flags);
     SNB_CHECK_LAUNCH();

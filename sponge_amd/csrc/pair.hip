@@ -167,7 +167,10 @@ __device__ __forceinline__ void excluded_partners(const ExclArgs& ex, const uint
                                                   float& fiz) {
     const int r1 = ex.start[at + 1];
     for (int k = ex.start[at] + k0; k < r1; k += kstep) {
-        const uint4 b = pk[ex.o2s[ex.list[k]]];
+        // a partner left out of an overflowing list has no slot
+        const int js = ex.o2s[ex.list[k]];
+        if (js < 0) continue;
+        const uint4 b = pk[js];
         const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
         const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
         const float dz = (float)(int)((b.z & ~3u) - az) * cz;
@@ -364,8 +367,8 @@ __device__ __forceinline__ void pair_body(
         const int n = write_inner ? prune_slots<TB>(ic, list_n, list, list_cap, pk, cx, cy, cz, pa)
                                   : pa.atom_n[ic * kClusterSize + (lane >> 4)];
         const int nmax = max(max(__shfl(n, 0), __shfl(n, 16)), max(__shfl(n, 32), __shfl(n, 48)));
+        if (ic == 0 && lane == 0) atomicAdd(pa.diag + 1, 1);
         if (nmax <= pa.atom_cap) {
-            if (ic == 0 && lane == 0) atomicAdd(pa.diag + 1, 1);
             pair_atoms<TB>(ic, n, nmax, pk, sab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite, pa, ex);
             return;
         }

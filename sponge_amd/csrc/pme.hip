@@ -1217,6 +1217,11 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     if (e == hipSuccess) e = hipMemset(pme->corr_atom, 0, (size_t)atom_numbers * sizeof(float));
     if (e == hipSuccess) e = hipMemset(pme->bin_count, 0, (size_t)Kx * sizeof(int));
     if (e == hipSuccess) e = hipMemset(pme->acc, 0, SPONGE_NB_NPARTS * sizeof(double));
+    // an atom that an overflowing neighbor list leaves without a slot keeps
+    // these (the fused step's prep writes slotted atoms only): grid point 0
+    // rather than an undefined index for the gather
+    if (e == hipSuccess) e = hipMemset(pme->base, 0, (size_t)atom_numbers * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(pme->frac, 0, (size_t)atom_numbers * sizeof(float4));
     // default-priority side stream and fork / join events of the fused step
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&pme->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pme->fork, hipEventDisableTiming);

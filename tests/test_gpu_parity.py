@@ -311,3 +311,32 @@ def test_pruned_inner_list(case):
     e2.force_step(0, 0, from_crd=True, overwrite=True)
     torch.cuda.synchronize()
     assert rel_rms(e2.frc.cpu().numpy(), ref["frc"]) < FORCE_TOL
+
+
+def test_per_slot_overflow_falls_back_to_the_full_list():
+    """The 648-atom water box as a droplet in a box 1.5 times as long: the
+    per-slot lists are sized from the mean density (0.30 of the droplet's:
+    224 partners per slot), so 54 slots inside the droplet (up to 263 partners
+    within 8.5 A) overflow at the prune and their clusters read the full list;
+    the forces stay those of the oracle. (At twice the box length a cell
+    column would hold 130 atoms for 128 places: that list overflow is a
+    reported error, not this case.)"""
+    import dataclasses
+    import torch
+    from sponge_amd import nb
+    base = system.config(0)
+    sys_ = dataclasses.replace(base, box=(base.box.astype(np.float64) * 1.5).astype(np.float32))
+    say(f"[case] droplet n={sys_.n} box={sys_.box.tolist()}")
+    eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    eng.update_neighbor_list(forced=1)
+    for _ in range(3):
+        eng.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+    st = eng.nl.inner_status()
+    ref = oracle.full_reference(sys_, beta=eng.pme.beta, K=(eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
+    e = rel_rms(eng.frc.cpu().numpy(), ref["frc"])
+    say(f"    overflow {st['slot_overflow']}, per-slot launches {st['slot_launches']}, force rel rms {e:.3e}")
+    assert eng.nl.status()["overflow"] == 0
+    assert st["slot_overflow"] == 1
+    assert st["slot_launches"] == 3
+    assert e < FORCE_TOL
