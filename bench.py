@@ -295,8 +295,8 @@ def main():
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path would shard by spatial decomposition, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported (it would shard "
+              "by spatial decomposition: bricks of cell columns with a cutoff + skin halo)", file=sys.stderr)
         sys.exit(2)
 
     import torch
