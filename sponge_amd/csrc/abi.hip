@@ -101,22 +101,19 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     const int energy = (need_energy > 0 || need_virial > 0) ? 1 : 0;
     int rc = snb_step_prep(nl, lj, pme, d_crd, d_uint_crd, d_charge, energy, s);
     if (rc) return rc;
-    // Fork: the reciprocal pipeline (spread, FFTs, convolution) runs on a
-    // stream of the PME handle while the pair and exclusion kernels stay on s;
+    // Fork: the reciprocal pipeline (spread, FFTs, convolution) stays on s
+    // while the pair and exclusion kernels run on a stream of the PME handle;
     // both read only what step_prep wrote. Join before the gather, which adds
-    // into d_frc. The pipeline is the longer branch and is enqueued first: a
-    // captured graph keeps the first branch after a fork on the queue of the
-    // node before it, whatever stream it was captured on (no cross-queue wait
-    // ahead of the spread), and gives the other branch a second queue.
+    // into d_frc. The pipeline is the longer branch: eager, it starts without
+    // a cross-stream wait; captured, it is the first branch after the fork,
+    // which a graph keeps on the queue of the node before it (the other
+    // branch gets a second queue, whose start costs ~10 us).
     const bool serial = (flags & SPONGE_STEP_SERIAL) != 0;
     hipStream_t ps = serial ? s : pme->side;
-    if (!serial) {
-        SNB_HIP(hipEventRecord(pme->fork, s));
-        SNB_HIP(hipStreamWaitEvent(ps, pme->fork, 0));
-    }
-    rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, energy, 1, ps);
+    if (!serial) SNB_HIP(hipEventRecord(pme->fork, s));
+    rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, energy, 1, s);
     if (rc) return rc;
-    if (!serial) SNB_HIP(hipEventRecord(pme->join, ps));
+    if (!serial) SNB_HIP(hipStreamWaitEvent(ps, pme->fork, 0));
     // Force-only steps fold the excluded-pair correction into the pair kernel
     // when the neighbor list holds the symmetric form of this exclusion CSR;
     // energy steps keep the separate kernel, which also fills the per-atom
@@ -126,15 +123,18 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     const ExclArgs ex = fold_excl ? ExclArgs{nl->sym_start, nl->sym_list, nl->o2s} : ExclArgs{nullptr, nullptr, nullptr};
     rc = snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme->beta,
                    energy, &lj->ps, d_frc, need_energy > 0 ? d_atom_energy : nullptr,
-                   need_virial > 0 ? d_atom_virial : nullptr, nullptr, s, 1,
+                   need_virial > 0 ? d_atom_virial : nullptr, nullptr, ps, 1,
                    (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, lj->d_sc, 1, ex);
     if (rc) return rc;
     if (!fold_excl) {
         rc = snb_excluded(pme, d_uint_crd, lj->cof_dr, d_charge, d_excluded_list_start, d_excluded_list,
-                          d_excluded_numbers, d_frc, nullptr, s);
+                          d_excluded_numbers, d_frc, nullptr, ps);
         if (rc) return rc;
     }
-    if (!serial) SNB_HIP(hipStreamWaitEvent(s, pme->join, 0));
+    if (!serial) {
+        SNB_HIP(hipEventRecord(pme->join, ps));
+        SNB_HIP(hipStreamWaitEvent(s, pme->join, 0));
+    }
     rc = snb_pme_gather(pme, d_frc, s);
     if (rc) return rc;
     if (energy) {
