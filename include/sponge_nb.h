@@ -249,9 +249,9 @@ int sponge_pme_update_box_length(sponge_pme* pme, const float box_length[3]);
  * not zero it). When need_energy / need_virial is set, the FP64 parts
  * (SPONGE_PART_*) are written to d_parts (device, 8 doubles) and the per-atom
  * arrays are filled like the separate calls fill them.
- * The reciprocal spread / FFT / convolution runs on a stream of the PME
- * handle, forked from and joined back into `stream` with events (so the call
- * stays capturable into a graph), concurrently with the pair kernel;
+ * The pair kernel runs on a stream of the PME handle, forked from and joined
+ * back into `stream` with events (so the call stays capturable into a graph),
+ * concurrently with the reciprocal spread / FFT / convolution on `stream`;
  * SPONGE_STEP_SERIAL keeps every launch on `stream`. */
 #define SPONGE_STEP_OVERWRITE_FORCE 1
 #define SPONGE_STEP_SERIAL 2
