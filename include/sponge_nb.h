@@ -10,9 +10,10 @@
  * Outputs follow the reference's accumulate-into (+=) semantics.
  *
  * `stream` is a hipStream_t (NULL = the default stream). No entry point that
- * takes a stream allocates, frees or synchronises, so a whole MD step can be
- * captured into one hipGraph; functions that return host values synchronise
- * and say so.
+ * takes a stream allocates, frees or synchronises in steady state (scratch is
+ * sized on the first call, and sponge_nl_update reads a new exclusion CSR
+ * once), so a whole MD step can be captured into one hipGraph after one eager
+ * step; functions that return host values synchronise and say so.
  *
  * Return value: 0 on success, a negative SPONGE_NB_E* code on failure;
  * sponge_nb_last_error() describes the last failure on the calling thread.
@@ -29,6 +30,7 @@ extern "C" {
 #define SPONGE_NB_EHIP (-2)      /* HIP runtime error */
 #define SPONGE_NB_ECAPACITY (-3) /* a fixed capacity was exceeded (see overflow flags) */
 #define SPONGE_NB_ENODEV (-4)    /* no usable gfx950 device */
+#define SPONGE_NB_EINTERNAL (-5) /* an internal consistency check failed */
 
 /* Overflow flags reported by sponge_nl_status (bit set = overflowed). */
 #define SPONGE_NL_OVERFLOW_COLUMN 1  /* too many atoms in one cell column */
@@ -85,6 +87,16 @@ int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_excluded_list_sta
                      const int* d_excluded_list, const int* d_excluded_numbers,
                      int forced_update, int forced_check, void* stream);
 
+/* No counterpart in the reference (its exclusion deletion reads the CSR on
+ * every rebuild). Synchronises. The list keeps a symmetric copy of the
+ * exclusion CSR (the partners of every atom, both directions) for its
+ * kernels; sponge_nl_update builds it the first time it sees a set of CSR
+ * pointers (synchronising once; refused inside a stream capture) and reuses
+ * it while the same pointers come back. A caller that rewrites the CSR in
+ * place, or reuses the addresses for new arrays, calls this to rebuild it. */
+int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_list_start, const int* d_excluded_list,
+                             const int* d_excluded_numbers);
+
 /* Inner list of the fused step (no counterpart in the reference): every
  * prune_interval fused steps, sponge_nb_force_step keeps only the list
  * entries and pair bits within prune_radius (>= cutoff) of the current
@@ -113,8 +125,10 @@ int sponge_nl_status(sponge_nl* nl, long* pair_count, int* overflow_flags, int* 
 int sponge_nl_inner_status(sponge_nl* nl, int* slot_overflow, int* slot_launches);
 
 /* Synchronises. The list as canonical atom pairs (i<j, original indices),
- * unsorted; at most `capacity` pairs are written to h_pairs (2 ints each).
- * Replaces reading d_nl[i].atom_serial on the host (neighbor_list.cu:490-503). */
+ * sorted; at most `capacity` pairs are written to h_pairs (2 ints each).
+ * Replaces reading d_nl[i].atom_serial on the host (neighbor_list.cu:490-503).
+ * Also checks that every pair is stored from both of its atoms (the layout
+ * the pair kernel relies on): SPONGE_NB_EINTERNAL if not. */
 int sponge_nl_get_pairs(sponge_nl* nl, int* h_pairs, long capacity, long* pair_count);
 
 /* ------------------------------------------------------------------ */

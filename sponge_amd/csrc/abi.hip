@@ -99,7 +99,8 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
             "LJ and PME boxes differ");
     hipStream_t s = as_stream(stream);
     const int energy = (need_energy > 0 || need_virial > 0) ? 1 : 0;
-    int rc = snb_step_prep(nl, lj, pme, d_crd, d_uint_crd, d_charge, energy, s);
+    const bool overwrite = (flags & SPONGE_STEP_OVERWRITE_FORCE) != 0;
+    int rc = snb_step_prep(nl, lj, pme, d_crd, d_uint_crd, d_charge, energy, overwrite ? d_frc : nullptr, s);
     if (rc) return rc;
     // Fork: the reciprocal pipeline (spread, FFTs, convolution) stays on s
     // while the pair and exclusion kernels run on a stream of the PME handle;

@@ -150,7 +150,8 @@ extern "C" int sponge_batch_force_step(sponge_batch* b, int flags, int need_ener
     SNB_ARG(b, "batch_force_step");
     hipStream_t s = as_stream(stream);
     const int energy = (need_energy > 0 || need_virial > 0) ? 1 : 0;
-    int rc = snb_batch_prep(b, energy, (flags & SPONGE_STEP_FROM_CRD) ? 1 : 0, s);
+    int rc = snb_batch_prep(b, energy, (flags & SPONGE_STEP_FROM_CRD) ? 1 : 0,
+                            (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, s);
     if (!rc) rc = snb_batch_pairs(b, energy, (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, s);
     if (!rc) rc = snb_batch_excluded(b, s);
     if (!rc) rc = snb_batch_reciprocal(b, energy, s);

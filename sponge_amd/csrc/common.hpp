@@ -37,6 +37,9 @@ constexpr int kWave = 64;
 constexpr int kClusterSize = 4;          // atoms per cluster (4x4 cluster pairs, 4 per wave64)
 constexpr int kMaxLjTypes = 64;          // LDS pair table holds kMaxLjTypes^2 (A,B)
 constexpr int kNumXcd = 8;               // MI355X: 8 XCDs, workgroups dealt round-robin
+// Largest PME grid axis: sponge_pme_create rejects larger axes, and the prep
+// kernels' per-x-plane LDS histograms hold this many planes
+constexpr int kMaxGridAxis = 512;
 
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
@@ -156,14 +159,16 @@ struct sponge_nl {
     int* col_start;             // ncol+1 (slots)
     int* col_atoms;             // ncol * col_cap
     int* s2o;                   // slot_cap: slot -> atom (-1 = padding)
-    int* o2s;                   // n: atom -> slot
+    int* o2s;                   // n: atom -> slot (-1: left out by a column overflow)
+    int* dropped;               // n: atoms left out by a column overflow, flags[5] of them
     float4* cl_lo;              // ncl_cap bounding boxes (Angstrom)
     float4* cl_hi;
     int* list_n;                // ncl_cap
     uint2* list;                // ncl_cap * list_cap : (j cluster, 16-bit pair mask)
     uint4* pack;                // slot_cap sorted (ux, uy, uz, atom) at the last rebuild
     int* flags;                 // [0] rebuild flag, [1] overflow bits, [2] rebuild count,
-                                // [3] slot count, [4] cluster count, [7] skin-check vote,
+                                // [3] slot count, [4] cluster count, [5] atoms left out
+                                // (column overflow), [7] skin-check vote,
                                 // [8..9] listed pair count (64-bit), [10] fused steps since
                                 // the last rebuild
     int device;
@@ -320,7 +325,7 @@ int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s);
 int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s);
 int snb_pme_convolve(sponge_pme* pme, int energy, hipStream_t s);
 int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd, unsigned int* d_uint_crd,
-                  const float* d_charge, int energy, hipStream_t s);
+                  const float* d_charge, int energy, float* d_frc_assigned, hipStream_t s);
 int snb_pme_finalize(sponge_pme* pme, const float* d_direct_atom_energy, int need_energy, int need_virial,
                      float* d_potential, float* d_virial, hipStream_t s);
 int snb_fill_zero_f64(double* p, int n, hipStream_t s);
@@ -331,7 +336,7 @@ size_t snb_pme_lds_conv(const sponge_pme* pme);
 int snb_pme_raise_lds(size_t bytes);
 size_t snb_pme_lds_inv(const sponge_pme* pme);
 float snb_nl_rlm(const sponge_nl* nl);
-int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s);
+int snb_batch_prep(sponge_batch* b, int energy, int from_crd, int overwrite, hipStream_t s);
 int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s);
 int snb_batch_excluded(sponge_batch* b, hipStream_t s);
 int snb_lj_scratch(sponge_lj* lj, const sponge_nl* nl);

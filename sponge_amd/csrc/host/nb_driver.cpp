@@ -382,6 +382,15 @@ int main(int argc, char** argv) {
                                   box.x, box.y, box.z);
         }
         neighbor_list.Neighbor_List_Update(crd, d_es, d_el, d_en);
+        if (need_potential) {
+            // print steps synchronise anyway: stop on a list that left atoms
+            // or pairs out (the reference writes past its buckets silently)
+            int ovf = 0;
+            if (sponge_nl_status(neighbor_list.d_nl, nullptr, &ovf, nullptr) != SPONGE_NB_OK) {
+                std::fprintf(stderr, "step %d: neighbor list capacity overflow (flags 0x%x); stopping\n", step, ovf);
+                return 2;
+            }
+        }
         if (md && info_interval > 0 && step % info_interval == 0) {
             // Main_Print's trajectory frames (MD_core.cu:1349-1360, 1402-1412)
             std::vector<VECTOR> h;

@@ -69,6 +69,7 @@ __device__ __forceinline__ void nl_begin_body(int forced, int* flags, int* col_c
         go = f;
         if (f) {
             flags[1] = 0;
+            flags[5] = 0;
             flags[8] = 0;
             flags[9] = 0;
             flags[2] += 1;
@@ -92,7 +93,7 @@ __device__ __forceinline__ void nl_bin_body(int blk, int n, float* __restrict__ 
                                             unsigned int* __restrict__ nl_uint, float bx, float by, float bz,
                                             float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap,
                                             int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags,
-                                            int* __restrict__ o2s) {
+                                            int* __restrict__ o2s, int* __restrict__ dropped) {
     if (!flags[0]) return;
     int i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -120,10 +121,14 @@ __device__ __forceinline__ void nl_bin_body(int blk, int n, float* __restrict__ 
     if (s < col_cap) {
         col_atoms[(size_t)c * col_cap + s] = i;
     } else {
-        // left out of this list (reported); no slot, which the slot map's
-        // readers test (the sort writes the slots of the others)
+        // left out of this list (reported): no slot, which the slot map's
+        // readers test (the sort writes the slots of the others); the fused
+        // step's prep finds the atom in `dropped` and gives it a defined
+        // state (fresh uint coordinates, zero charge on the PME grid, zero
+        // force when forces are assigned)
         atomicOr(flags + 1, SPONGE_NL_OVERFLOW_COLUMN);
         o2s[i] = -1;
+        dropped[atomicAdd(flags + 5, 1)] = i;
     }
 }
 
@@ -131,9 +136,9 @@ __global__ void nl_bin_kernel(int n, float* __restrict__ crd, float* __restrict_
                               unsigned int* __restrict__ nl_uint, float bx, float by, float bz,
                               float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap,
                               int* __restrict__ col_count, int* __restrict__ col_atoms, int* flags,
-                              int* __restrict__ o2s) {
+                              int* __restrict__ o2s, int* __restrict__ dropped) {
     nl_bin_body(blockIdx.x, n, crd, old_crd, nl_uint, bx, by, bz, cqx, cqy, cqz, ncx, ncy, col_cap, col_count,
-                col_atoms, flags, o2s);
+                col_atoms, flags, o2s, dropped);
 }
 
 // Exclusive scan of the padded (multiple of 4) column sizes; one block.
@@ -504,7 +509,7 @@ __global__ void nl_bin_batch(const BatchSys* __restrict__ bs, const int2* __rest
     const BatchSys& b = bs[m.x];
     const sponge_nl& L = b.nl;
     nl_bin_body(m.y, L.n, b.io.crd, L.old_crd, L.nl_uint, L.box[0], L.box[1], L.box[2], L.cof_q[0], L.cof_q[1],
-                L.cof_q[2], L.ncx, L.ncy, L.col_cap, L.col_count, L.col_atoms, L.flags, L.o2s);
+                L.cof_q[2], L.ncx, L.ncy, L.col_cap, L.col_count, L.col_atoms, L.flags, L.o2s, L.dropped);
 }
 
 __global__ void nl_scan_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map) {
@@ -627,6 +632,7 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
     A((void**)&nl->col_atoms, (size_t)nl->ncol * nl->col_cap * sizeof(int));
     A((void**)&nl->s2o, nl->slot_cap * sizeof(int));
     A((void**)&nl->o2s, n * sizeof(int));
+    A((void**)&nl->dropped, n * sizeof(int));
     A((void**)&nl->cl_lo, nl->ncl_cap * sizeof(float4));
     A((void**)&nl->cl_hi, nl->ncl_cap * sizeof(float4));
     A((void**)&nl->list_n, nl->ncl_cap * sizeof(int));
@@ -676,7 +682,7 @@ extern "C" void sponge_nl_destroy(sponge_nl* nl) {
     void* ptrs[] = {nl->old_crd, nl->nl_uint, nl->col_count, nl->col_start, nl->col_atoms, nl->s2o,
                     nl->o2s,     nl->cl_lo,   nl->cl_hi,     nl->list_n,    nl->list,      nl->pack,
                     nl->flags,   nl->list_in, nl->list_in_n, nl->sym_start, nl->sym_list, nl->atom_in_n,
-                    nl->atom_in};
+                    nl->atom_in, nl->dropped};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete nl;
@@ -696,7 +702,7 @@ static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el,
     nl_bin_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_crd, nl->old_crd, nl->nl_uint, nl->box[0], nl->box[1],
                                                    nl->box[2], nl->cof_q[0], nl->cof_q[1], nl->cof_q[2], nl->ncx,
                                                    nl->ncy, nl->col_cap, nl->col_count, nl->col_atoms, nl->flags,
-                                                   nl->o2s);
+                                                   nl->o2s, nl->dropped);
     SNB_CHECK_LAUNCH();
     nl_scan_kernel<<<1, 1024, 0, s>>>(nl->ncol, nl->col_cap, nl->col_count, nl->col_start, nl->flags);
     SNB_CHECK_LAUNCH();
@@ -717,9 +723,20 @@ static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el,
 // Symmetric exclusion lists from the reference's CSR (each excluded pair
 // listed once, in the row of one of its atoms; MD_core/MD_core.cu:284-483):
 // built on the host the first time these CSR arrays are passed (one
-// synchronising copy), reused while the same arrays come back.
-static int nl_sym_exclusions(sponge_nl* nl, const int* es, const int* el, const int* en) {
+// synchronising copy), reused while the same arrays come back (the cache is
+// keyed on the three device pointers: a caller that rewrites the CSR in
+// place calls sponge_nl_set_exclusions). A first pass inside a stream
+// capture is refused, since the copy would synchronise the capture.
+static int nl_sym_exclusions(sponge_nl* nl, const int* es, const int* el, const int* en, hipStream_t s) {
     if (es == nl->ex_es && el == nl->ex_el && en == nl->ex_en && nl->sym_start) return SPONGE_NB_OK;
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            set_error("Neighbor_List_Update: new exclusion arrays inside a stream capture; pass them once "
+                      "(or call sponge_nl_set_exclusions) before capturing");
+            return SPONGE_NB_EARG;
+        }
+    }
     const int n = nl->n;
     std::vector<int> hs(n), hn(n);
     SNB_HIP(hipDeviceSynchronize());
@@ -767,7 +784,7 @@ extern "C" int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_exclud
     SNB_ARG(nl && d_crd && d_excluded_list_start && d_excluded_list && d_excluded_numbers, "nl_update");
     hipStream_t s = as_stream(stream);
     {
-        const int rc = nl_sym_exclusions(nl, d_excluded_list_start, d_excluded_list, d_excluded_numbers);
+        const int rc = nl_sym_exclusions(nl, d_excluded_list_start, d_excluded_list, d_excluded_numbers, s);
         if (rc) return rc;
     }
     if (forced_update) {
@@ -785,6 +802,13 @@ extern "C" int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_exclud
                                                          nl->box[2], thresh, nl->flags);
     SNB_CHECK_LAUNCH();
     return nl_rebuild(nl, d_crd, d_excluded_list_start, d_excluded_list, d_excluded_numbers, 0, s);
+}
+
+extern "C" int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_list_start, const int* d_excluded_list,
+                                        const int* d_excluded_numbers) {
+    SNB_ARG(nl && d_excluded_list_start && d_excluded_list && d_excluded_numbers, "nl_set_exclusions");
+    nl->ex_es = nl->ex_el = nl->ex_en = nullptr;  // rebuild even for the same pointers
+    return nl_sym_exclusions(nl, d_excluded_list_start, d_excluded_list, d_excluded_numbers, nullptr);
 }
 
 extern "C" int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius) {
@@ -838,7 +862,11 @@ extern "C" int sponge_nl_get_pairs(sponge_nl* nl, int* h_pairs, long capacity, l
         SNB_HIP(hipMemcpy(s2o.data(), nl->s2o, nslot * sizeof(int), hipMemcpyDeviceToHost));
         SNB_HIP(hipMemcpy(lst.data(), nl->list, lst.size() * sizeof(uint2), hipMemcpyDeviceToHost));
     }
-    long m = 0;
+    // Every unordered pair must be listed from both of its atoms (the pair
+    // kernel accumulates only its own atoms' forces): the pairs seen from
+    // the lower atom and from the higher one are collected separately and
+    // must be the same set.
+    std::vector<unsigned long long> lo_side, hi_side;
     for (int ic = 0; ic < ncl; ic++)
         for (int e = 0; e < ln[ic]; e++) {
             uint2 ent = lst[(size_t)ic * nl->list_cap + e];
@@ -846,16 +874,27 @@ extern "C" int sponge_nl_get_pairs(sponge_nl* nl, int* h_pairs, long capacity, l
                 if (ent.y >> b & 1u) {
                     int ii = b & 3, jj = b >> 2;
                     int a = s2o[ic * kClusterSize + ii], c = s2o[ent.x * kClusterSize + jj];
-                    if (a > c) continue;  // each unordered pair is listed from both atoms
-                    if (h_pairs && m < capacity) {
-                        h_pairs[2 * m] = std::min(a, c);
-                        h_pairs[2 * m + 1] = std::max(a, c);
-                    }
-                    m++;
+                    const unsigned long long key =
+                        ((unsigned long long)(unsigned)std::min(a, c) << 32) | (unsigned)std::max(a, c);
+                    (a < c ? lo_side : hi_side).push_back(key);
                 }
         }
+    std::sort(lo_side.begin(), lo_side.end());
+    std::sort(hi_side.begin(), hi_side.end());
+    const long m = (long)lo_side.size();
+    if (h_pairs)
+        for (long k = 0; k < std::min(m, capacity); k++) {
+            h_pairs[2 * k] = (int)(lo_side[k] >> 32);
+            h_pairs[2 * k + 1] = (int)(lo_side[k] & 0xffffffffu);
+        }
     if (pair_count) *pair_count = m;
-    return f[1] ? SPONGE_NB_ECAPACITY : SPONGE_NB_OK;
+    if (f[1]) return SPONGE_NB_ECAPACITY;
+    if (lo_side != hi_side) {
+        set_error("neighbor list is not symmetric: %zu pairs listed from the lower atom, %zu from the higher",
+                  lo_side.size(), hi_side.size());
+        return SPONGE_NB_EINTERNAL;
+    }
+    return SPONGE_NB_OK;
 }
 
 // Batched Neighbor_List_Update: forced = rebuild every system; otherwise the

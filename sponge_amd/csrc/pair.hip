@@ -58,13 +58,18 @@ __global__ void gather_kernel(const int* __restrict__ flags, const int* __restri
 // Fused per-step prep of the whole force path, one thread per cluster slot:
 // optional MD_Information_Crd_To_Uint_Crd (MD_core.cu:1528-1532, written back
 // to the caller's uint array), the pair-kernel record, and the PME base,
-// fraction and x-plane bin of the slot's atom.
+// fraction and x-plane bin of the slot's atom. The threads past the slots
+// take the atoms a column overflow left out of the list (flags[5] of them;
+// slots + those <= slot capacity): fresh uint coordinates, zero charge on
+// the PME grid (so no reciprocal force either), zero force when the step
+// assigns forces (zero_frc) -- the overflow is reported by the list status.
 __device__ __forceinline__ void step_prep_body(int blk, const int* __restrict__ flags, const int* __restrict__ s2o,
                                                const float* __restrict__ crd, float cqx, float cqy, float cqz,
                                                unsigned int* __restrict__ uc, const float* __restrict__ q,
                                                const int* __restrict__ type, uint4* __restrict__ pk,
                                                const PmePrep& P, int energy, double* __restrict__ acc,
-                                               int* __restrict__ step_counter = nullptr) {
+                                               int* __restrict__ step_counter, const int* __restrict__ dropped,
+                                               float* __restrict__ zero_frc) {
     const int p = blk * blockDim.x + threadIdx.x;
     // fused steps since the last list rebuild, this one included (read by
     // this step's pair kernel to decide whether it re-prunes the inner list)
@@ -73,9 +78,32 @@ __device__ __forceinline__ void step_prep_body(int blk, const int* __restrict__ 
     if (energy && p == 3) acc[SPONGE_PART_PAIRS_IN] = 0.0;
     const int nslot = flags[3];
     int a = p < nslot ? s2o[p] : -1;
+    bool left_out = false;
+    if (p >= nslot && p < nslot + flags[5]) {
+        a = dropped[p - nslot];
+        left_out = true;
+    }
     unsigned u0 = 0u, u1 = 0u, u2 = 0u;
     float qa = 0.f;
-    if (a >= 0) {
+    if (left_out) {
+        if (crd) {
+            u0 = ((unsigned)(int)__fmul_rn(crd[3 * a], cqx)) << 2;
+            u1 = ((unsigned)(int)__fmul_rn(crd[3 * a + 1], cqy)) << 2;
+            u2 = ((unsigned)(int)__fmul_rn(crd[3 * a + 2], cqz)) << 2;
+            uc[3 * a] = u0;
+            uc[3 * a + 1] = u1;
+            uc[3 * a + 2] = u2;
+        } else {
+            u0 = uc[3 * a];
+            u1 = uc[3 * a + 1];
+            u2 = uc[3 * a + 2];
+        }
+        if (zero_frc) {
+            zero_frc[3 * a] = 0.f;
+            zero_frc[3 * a + 1] = 0.f;
+            zero_frc[3 * a + 2] = 0.f;
+        }
+    } else if (a >= 0) {
         if (crd) {
             u0 = ((unsigned)(int)__fmul_rn(crd[3 * a], cqx)) << 2;
             u1 = ((unsigned)(int)__fmul_rn(crd[3 * a + 1], cqy)) << 2;
@@ -103,18 +131,21 @@ __global__ __launch_bounds__(256) void step_prep_kernel(const int* __restrict__ 
                                                         unsigned int* __restrict__ uc, const float* __restrict__ q,
                                                         const int* __restrict__ type, uint4* __restrict__ pk,
                                                         PmePrep P, int energy, double* __restrict__ acc,
-                                                        int* __restrict__ step_counter) {
-    step_prep_body(blockIdx.x, flags, s2o, crd, cqx, cqy, cqz, uc, q, type, pk, P, energy, acc, step_counter);
+                                                        int* __restrict__ step_counter, const int* __restrict__ dropped,
+                                                        float* __restrict__ zero_frc) {
+    step_prep_body(blockIdx.x, flags, s2o, crd, cqx, cqy, cqz, uc, q, type, pk, P, energy, acc, step_counter, dropped,
+                   zero_frc);
 }
 
 __global__ __launch_bounds__(256) void step_prep_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map,
-                                                       int energy, int from_crd) {
+                                                       int energy, int from_crd, int overwrite) {
     const int2 m = map[blockIdx.x];
     if (m.x < 0) return;
     const BatchSys& b = bs[m.x];
     const PmePrep P = pme_prep_args(&b.pme);
     step_prep_body(m.y, b.nl.flags, b.nl.s2o, from_crd ? b.io.crd : nullptr, b.cq[0], b.cq[1], b.cq[2],
-                   b.io.uint_crd, b.io.charge, b.lj.d_type, b.lj.ps.pk, P, energy, b.lj.ps.acc);
+                   b.io.uint_crd, b.io.charge, b.lj.d_type, b.lj.ps.pk, P, energy, b.lj.ps.acc, nullptr, b.nl.dropped,
+                   overwrite ? b.io.frc : nullptr);
 }
 
 
@@ -975,7 +1006,7 @@ int snb_gather(const sponge_nl* nl, const unsigned int* d_uint_crd, const float*
 }
 
 int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd, unsigned int* d_uint_crd,
-                  const float* d_charge, int energy, hipStream_t s) {
+                  const float* d_charge, int energy, float* d_frc_assigned, hipStream_t s) {
     int rc = scratch_ensure(&lj->ps, nl);
     if (rc) return rc;
     float cq[3];
@@ -983,7 +1014,8 @@ int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const flo
     const int cap = lj->ps.cap;
     step_prep_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(nl->flags, nl->s2o, d_crd, cq[0], cq[1], cq[2], d_uint_crd,
                                                         d_charge, lj->d_type, lj->ps.pk, pme_prep_args(pme), energy,
-                                                        lj->ps.acc, nl->prune_interval > 0 ? nl->flags + 10 : nullptr);
+                                                        lj->ps.acc, nl->prune_interval > 0 ? nl->flags + 10 : nullptr,
+                                                        nl->dropped, d_frc_assigned);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
@@ -1070,8 +1102,8 @@ extern "C" int sponge_pme_excluded_force(sponge_pme* pme, const unsigned int* d_
 
 // ---------------------------------------------------------------------------
 // Batched launches (one per stage over every system of a batch).
-int snb_batch_prep(sponge_batch* b, int energy, int from_crd, hipStream_t s) {
-    step_prep_batch<<<b->nblk[kStPrep], 256, 0, s>>>(b->d, b->map[kStPrep], energy, from_crd);
+int snb_batch_prep(sponge_batch* b, int energy, int from_crd, int overwrite, hipStream_t s) {
+    step_prep_batch<<<b->nblk[kStPrep], 256, 0, s>>>(b->d, b->map[kStPrep], energy, from_crd, overwrite);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }

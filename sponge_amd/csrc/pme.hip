@@ -1173,7 +1173,7 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     }
     const size_t conv_lds = pme->conv_split ? std::max(lines_lds(Kx), lines_lds(Ky)) : conv2d_lds(Kx, Ky);
     const size_t lds_need = std::max({spread_z_lds(Kz), inv_z_lds(Kz), conv_lds});
-    if (lds_need > (size_t)max_lds || Kx > 512 || Ky > 512 || Kz > 512) {
+    if (lds_need > (size_t)max_lds || Kx > kMaxGridAxis || Ky > kMaxGridAxis || Kz > kMaxGridAxis) {
         set_error("PME grid %dx%dx%d too large for the in-LDS transforms (%zu B of LDS, %d available)", Kx, Ky, Kz,
                   lds_need, max_lds);
         delete pme;

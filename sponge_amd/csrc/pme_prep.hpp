@@ -42,8 +42,8 @@ __host__ __device__ static inline PmePrep pme_prep_args(const sponge_pme* pme) {
 // sum q^2 and sum q added to acc in FP64.
 __device__ __forceinline__ void pme_prep_block(int i, unsigned ux, unsigned uy, unsigned uz, float q,
                                                const PmePrep& P, int energy) {
-    __shared__ int hist[256];
-    __shared__ int boff[256];
+    __shared__ int hist[kMaxGridAxis];
+    __shared__ int boff[kMaxGridAxis];
     for (int b = threadIdx.x; b < P.Kx; b += blockDim.x) hist[b] = 0;
     __syncthreads();
     double q2 = 0.0, q1 = 0.0;

@@ -232,8 +232,10 @@ class MDEngine:
         return self.nb.crd
 
     def prepare(self):
-        """Main_Initial's forced list build."""
+        """Main_Initial's forced list build; raises if it overflowed a
+        capacity."""
         self.nb.update_neighbor_list(forced=1)
+        self.nb.nl.check()
 
     def force(self, need_energy=0, need_virial=0, stream=None):
         self.nb.force_step(need_energy, need_virial, from_crd=True, overwrite=True, serial=self.serial,
@@ -295,6 +297,12 @@ class MDEngine:
         elif forced_update:
             self.nb.update_neighbor_list(forced=1, stream=stream)
         self.steps += 1
+        if need_energy or need_pressure:
+            # steps that synchronise anyway report a list that left atoms or
+            # pairs out (checked again after this step's rebuild, if any)
+            import torch
+            if not torch.cuda.is_current_stream_capturing():
+                self.nb.nl.check()
 
     def kinetic_energy(self):
         """Sum of m v^2 / 2 of the (half-step) velocities, FP64 on the host."""

@@ -23,6 +23,10 @@ NPARTS = 8
 OVERWRITE_FORCE = 1
 SERIAL = 2
 FROM_CRD = 4
+# sponge_nl_status overflow bits
+OVERFLOW_COLUMN = 1
+OVERFLOW_LIST = 2
+OVERFLOW_CANDIDATES = 4
 PART_NAMES = ["lj", "direct", "lj_virial", "correction", "self", "reciprocal", "lrc", "pairs_in"]
 
 _P = ctypes.c_void_p
@@ -43,6 +47,7 @@ _SIGS = {
     "sponge_nl_set_prune": (_I, [_P, _I, _F]),
     "sponge_nl_inner_status": (_I, [_P, _P, _P]),
     "sponge_nl_status": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "sponge_nl_set_exclusions": (_I, [_P, _P, _P, _P]),
     "sponge_nl_get_pairs": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
     "sponge_lj_create": (_I, [_I, _I, _P, _P, _P, _F, _FP3, ctypes.POINTER(_P)]),
     "sponge_lj_destroy": (None, [_P]),
@@ -182,6 +187,12 @@ class NEIGHBOR_LIST:
                                                int(forced_update), int(forced_check), _stream(stream)),
                "Neighbor_List_Update")
 
+    def set_exclusions(self, excluded_list_start, excluded_list, excluded_numbers):
+        """Rebuilds the list's symmetric copy of the exclusion CSR (after the
+        caller rewrote the arrays in place). Synchronises."""
+        _check(load_library().sponge_nl_set_exclusions(self.h, _ptr(excluded_list_start), _ptr(excluded_list),
+                                                       _ptr(excluded_numbers)), "sponge_nl_set_exclusions")
+
     def set_prune(self, interval, radius=0.0):
         """Inner list of the fused step: re-pruned every `interval` fused
         steps to `radius` (>= cutoff); interval 0 turns it off."""
@@ -194,6 +205,17 @@ class NEIGHBOR_LIST:
         pc, fl, rb = _L(0), _I(0), _I(0)
         rc = load_library().sponge_nl_status(self.h, ctypes.byref(pc), ctypes.byref(fl), ctypes.byref(rb))
         return dict(pairs=pc.value, overflow=fl.value, rebuilds=rb.value, ok=rc == 0)
+
+    def check(self):
+        """Raises SpongeError when the last rebuild overflowed a capacity
+        (SPONGE_NL_OVERFLOW_*: atoms or pairs were left out of the list).
+        Synchronises with the device: call it between steps, outside a graph
+        capture."""
+        st = self.status()
+        if not st["ok"]:
+            raise SpongeError(f"neighbor list capacity overflow (flags {st['overflow']:#x}): "
+                              "atoms or pairs were left out of the last rebuild")
+        return st
 
     def inner_status(self):
         """Per-slot inner lists of the fused step: overflow at the last prune
