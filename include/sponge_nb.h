@@ -156,6 +156,22 @@ int sponge_lj_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, const unsigne
                                float* d_atom_lj_virial, float* d_atom_direct_pme_energy,
                                void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, results and kernels as
+ * sponge_lj_pme_direct_force, but its scratch (cluster-order records and FP64
+ * totals) is a static device array of the library (2 MiB per GPU, held until
+ * the process ends) instead of the module's lazily allocated buffers: it
+ * calls no hipMalloc / hipFree. Lists of more than 131,072 slots (about
+ * 125,000 atoms) are refused with SPONGE_NB_ECAPACITY. Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialised by the caller. */
+int sponge_lj_pme_direct_force_noalloc(sponge_lj* lj, const sponge_nl* nl, const unsigned int* d_uint_crd,
+                                       const float* d_charge, float* d_frc, float pme_beta,
+                                       int need_atom_energy, float* d_atom_energy, int need_virial,
+                                       float* d_atom_lj_virial, float* d_atom_direct_pme_energy,
+                                       void* stream);
+
 /* LENNARD_JONES_INFORMATION::Long_Range_Correction (Lennard_Jones_force.cu:1178-1191). */
 int sponge_lj_long_range_correction(sponge_lj* lj, int need_pressure, float* d_virial,
                                     int need_potential, float* d_potential, void* stream);

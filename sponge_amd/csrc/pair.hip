@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -688,6 +689,73 @@ __global__ __launch_bounds__(256) void excluded_batch(const BatchSys* __restrict
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Allocation-free direct force: the cluster-order records and FP64 totals of
+// sponge_lj_pme_direct_force live in this file-scope device array instead of
+// the module's scratch (allocated on a handle's first call). One copy per
+// GPU, held until the process ends; every call on a GPU shares it.
+constexpr int kNoAllocSlots = 131072;  // list slots (atoms + column padding)
+constexpr size_t kNoAllocPkBytes = (size_t)kNoAllocSlots * sizeof(uint4);
+constexpr size_t kNoAllocBytes = kNoAllocPkBytes + SPONGE_NB_NPARTS * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (the array's host-side name is not a device address).
+constexpr int kNoAllocMaxDevices = 64;
+static void* g_scratch_ptr[kNoAllocMaxDevices];
+static hipError_t g_scratch_err[kNoAllocMaxDevices];
+static std::once_flag g_scratch_once[kNoAllocMaxDevices];
+
+static int noalloc_scratch(void** out) {
+    int dev = 0;
+    SNB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        set_error("allocation-free entry points support devices 0..%d (current %d)", kNoAllocMaxDevices - 1, dev);
+        return SPONGE_NB_ENODEV;
+    }
+    std::call_once(g_scratch_once[dev], [dev] {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = p;
+    });
+    if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev]) {
+        set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(g_scratch_err[dev]));
+        return SPONGE_NB_EHIP;
+    }
+    *out = g_scratch_ptr[dev];
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_lj_pme_direct_force_noalloc(sponge_lj* lj, const sponge_nl* nl,
+                                                  const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
+                                                  float pme_beta, int need_atom_energy, float* d_atom_energy,
+                                                  int need_virial, float* d_atom_lj_virial,
+                                                  float* d_atom_direct_pme_energy, void* stream) {
+    SNB_ARG(lj && nl && d_uint_crd && d_charge && d_frc, "lj_pme_direct_force_noalloc");
+    SNB_ARG(nl->n == lj->n, "atom count of LJ and neighbor list differ");
+    if (nl->slot_cap > kNoAllocSlots) {
+        set_error("lj_pme_direct_force_noalloc: the list has %d slots; the static scratch holds %d "
+                  "(about 125,000 atoms); use sponge_lj_pme_direct_force",
+                  nl->slot_cap, kNoAllocSlots);
+        return SPONGE_NB_ECAPACITY;
+    }
+    void* base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    // The gather kernel rewrites every record the pair kernel reads and, with
+    // energy, zeroes the totals first: nothing of an earlier call is read.
+    PairScratch ps{};
+    ps.cap = kNoAllocSlots;
+    ps.pk = reinterpret_cast<uint4*>(base);
+    ps.acc = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(base) + kNoAllocPkBytes);
+    hipStream_t s = as_stream(stream);
+    const int energy = (need_atom_energy > 0 || need_virial > 0) ? 1 : 0;
+    return snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme_beta,
+                     energy, &ps, d_frc, need_atom_energy > 0 ? d_atom_energy : nullptr,
+                     need_virial > 0 ? d_atom_lj_virial : nullptr, energy ? d_atom_direct_pme_energy : nullptr, s, 0,
+                     0, lj->d_sc);
+}
 
 // ---------------------------------------------------------------------------
 extern "C" int sponge_lj_create(int atom_numbers, int atom_type_numbers, const float* h_LJ_A, const float* h_LJ_B,

@@ -52,6 +52,7 @@ _SIGS = {
     "sponge_lj_create": (_I, [_I, _I, _P, _P, _P, _F, _FP3, ctypes.POINTER(_P)]),
     "sponge_lj_destroy": (None, [_P]),
     "sponge_lj_pme_direct_force": (_I, [_P, _P, _P, _P, _P, _F, _I, _P, _I, _P, _P, _P]),
+    "sponge_lj_pme_direct_force_noalloc": (_I, [_P, _P, _P, _P, _P, _F, _I, _P, _I, _P, _P, _P]),
     "sponge_lj_long_range_correction": (_I, [_P, _I, _P, _I, _P, _P]),
     "sponge_lj_get_energy": (_I, [_P, _P, _P, ctypes.POINTER(_D), _P]),
     "sponge_lj_update_volume": (_I, [_P, _FP3]),
