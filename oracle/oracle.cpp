@@ -25,6 +25,8 @@
 #include <omp.h>
 #endif
 
+#include "settle_math.hpp"
+
 typedef std::complex<double> cplx;
 static const double PI_D = 3.14159265358979323846;
 
@@ -1134,23 +1136,9 @@ double orc_ewald_exact(int n, const double* xyz, const double* q, const float* b
 // ---------------------------------------------------------------------------
 // Integrator and SETTLE (next to the force path).
 
-// Get_Rabcde_From_SSS (constrain/SETTLE.cu:8-47) in FP64: a rigid triangle
-// with sides |AB|, |AC|, |BC| placed with its centre of mass at the origin,
-// A at (0, ra), B at (rc, rb), C at (rd, re).
+// Get_Rabcde_From_SSS (constrain/SETTLE.cu:8-47) in FP64 (settle_math.hpp).
 void orc_settle_geometry(double rAB, double rAC, double rBC, double mA, double mB, double mC, double* g) {
-    double mT = mA + mB + mC;
-    double Ax = 0, Ay = 0, Bx = -rAB, By = 0;
-    double c = (rBC * rBC - rAC * rAC - rAB * rAB) / (2 * rAC * rAB);
-    double Cx = rAC * c, Cy = rAC * std::sqrt(1.0 - c * c);
-    double Ox = (Bx * mB + Cx * mC) / mT, Oy = Cy * mC / mT;
-    Ax -= Ox; Ay -= Oy; Bx -= Ox; By -= Oy; Cx -= Ox; Cy -= Oy;
-    c = 1.0 / std::sqrt(1.0 + Ax * Ax / Ay / Ay);
-    double s = c * Ax / Ay;
-    double ra = Ax * s + Ay * c;
-    double rc = Bx * c - By * s, rb = Bx * s + By * c;
-    double rd = Cx * c - Cy * s, re = Cx * s + Cy * c;
-    if (ra < 0) { ra = -ra; rb = -rb; re = -re; }
-    g[0] = ra; g[1] = rb; g[2] = rc; g[3] = rd; g[4] = re;
+    orc::settle_geometry<double>(rAB, rAC, rBC, mA, mB, mC, g);
 }
 
 // SETTLE::Initial group discovery (constrain/SETTLE.cu:209-331, pairs' k at
@@ -1205,65 +1193,9 @@ void orc_settle_groups(int n, int np, const int* ai, const int* aj, const float*
     counts[0] = nt; counts[1] = npr; counts[2] = rest;
 }
 
-static inline void v_sub(const double* a, const double* b, double* o) { for (int d = 0; d < 3; d++) o[d] = a[d] - b[d]; }
-static inline double v_dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-static inline void v_cross(const double* a, const double* b, double* o) {
-    o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
-}
-static inline void v_unit(double* a) { double s = 1.0 / std::sqrt(v_dot(a, a)); for (int d = 0; d < 3; d++) a[d] *= s; }
-// Get_Periodic_Displacement of float coordinates (common.cu:99-107)
-static inline void v_pbc(const double* a, const double* b, const float* L, double* o) {
-    for (int d = 0; d < 3; d++) { double x = a[d] - b[d]; o[d] = x - std::floor(x / L[d] + 0.5) * L[d]; }
-}
-
-// SETTLE_DO_TRIANGLE (constrain/SETTLE.cu:100-188) in FP64. rB0, rC0: old
-// B-A, C-A; rA1..rC1: new positions relative to the new centre of mass,
-// replaced by the constrained ones; v: the three velocities, corrected.
-// Returns the triangle's virial F_B . rB0 + F_C . rC0.
-static double settle_do_triangle(const double* rB0, const double* rC0, double* rA1, double* rB1, double* rC1,
-                                 const double* g, double mB, double mC, double dt, double vfac, double xfac,
-                                 double* vA, double* vB, double* vC) {
-    double ra = g[0], rb = g[1], rc = g[2], rd = g[3], re = g[4];
-    double ez[3], ex[3], ey[3];
-    v_cross(rB0, rC0, ez);
-    v_cross(rA1, ez, ex);
-    v_cross(ez, ex, ey);
-    v_unit(ex); v_unit(ey); v_unit(ez);
-    double B0x = v_dot(ex, rB0), B0y = v_dot(ey, rB0), C0x = v_dot(ex, rC0), C0y = v_dot(ey, rC0);
-    double A1z = v_dot(ez, rA1);
-    double B1x = v_dot(ex, rB1), B1y = v_dot(ey, rB1), B1z = v_dot(ez, rB1);
-    double C1x = v_dot(ex, rC1), C1y = v_dot(ey, rC1), C1z = v_dot(ez, rC1);
-    double sinphi = A1z / ra, cosphi = std::sqrt(1.0 - sinphi * sinphi);
-    double sinpsi = (B1z - C1z - (rb - re) * sinphi) / ((rd - rc) * cosphi);
-    double cospsi = std::sqrt(1.0 - sinpsi * sinpsi);
-    double A2y = ra * cosphi;
-    double B2x = rc * cospsi, B2y = rb * cosphi + rc * sinpsi * sinphi;
-    double C2x = rd * cospsi, C2y = re * cosphi + rd * sinpsi * sinphi;
-    double alpha = B2x * B0x + C2x * C0x + B2y * B0y + C2y * C0y;
-    double beta = -B2x * B0y - C2x * C0y + B2y * B0x + C2y * C0x;
-    double gamma = B1y * B0x - B1x * B0y + C1y * C0x - C1x * C0y;
-    double t = alpha * alpha + beta * beta;
-    double sinth = (alpha * gamma - beta * std::sqrt(t - gamma * gamma)) / t;
-    double costh = std::sqrt(1.0 - sinth * sinth);
-    double A3[3] = {-A2y * sinth, A2y * costh, A1z};
-    double B3[3] = {B2x * costh - B2y * sinth, B2x * sinth + B2y * costh, B1z};
-    double C3[3] = {C2x * costh - C2y * sinth, C2x * sinth + C2y * costh, C1z};
-    double a3[3], b3[3], c3[3];
-    for (int d = 0; d < 3; d++) {
-        a3[d] = A3[0] * ex[d] + A3[1] * ey[d] + A3[2] * ez[d];
-        b3[d] = B3[0] * ex[d] + B3[1] * ey[d] + B3[2] * ez[d];
-        c3[d] = C3[0] * ex[d] + C3[1] * ey[d] + C3[2] * ez[d];
-    }
-    double kv = vfac / dt / xfac, kf = 1.0 / dt / dt / xfac, vir = 0;
-    for (int d = 0; d < 3; d++) {
-        vA[d] += kv * (a3[d] - rA1[d]);
-        vB[d] += kv * (b3[d] - rB1[d]);
-        vC[d] += kv * (c3[d] - rC1[d]);
-        vir += kf * mB * (b3[d] - rB1[d]) * rB0[d] + kf * mC * (c3[d] - rC1[d]) * rC0[d];
-        rA1[d] = a3[d]; rB1[d] = b3[d]; rC1[d] = c3[d];
-    }
-    return vir;
-}
+using orc::settle_do_triangle;
+using orc::v_dot;
+using orc::v_pbc;
 
 // One NVE step of main.cu:282-320 in FP64: SETTLE::Remember_Last_Coordinates
 // from the step's uint coordinates (SETTLE.cu:385-410, scaler L / 2^32),
@@ -1324,7 +1256,7 @@ double orc_md_iteration(int n, double* crd, double* vel, const double* frc, cons
             rB[d] += rA[d];
             rC[d] += rA[d];
         }
-        vir += settle_do_triangle(&last[6 * k], &last[6 * k + 3], rA, rB, rC, geo + 5 * k, mB, mC, dt, vfac, xfac,
+        vir += settle_do_triangle<double>(&last[6 * k], &last[6 * k + 3], rA, rB, rC, geo + 5 * k, mB, mC, dt, vfac, xfac,
                                   &vel[3 * a], &vel[3 * b], &vel[3 * c]);
         for (int d = 0; d < 3; d++) {
             crd[3 * a + d] = rA[d] + rO[d];
