@@ -8,7 +8,8 @@ pair force, excluded-pair correction, PME reciprocal (spread, R2C FFT, BC
 convolution, C2R FFT, gather), leap-frog with SETTLE rigid waters, and the
 neighbor-list refresh every 20 steps, all on the GPU through libsponge_nb.so.
 The trajectory starts from the seeded lattice with rigid-body velocities at
-300 K. Inputs are resident in HBM before timing.
+300 K, relaxed by untimed MD steps with velocities rescaled to 300 K.
+Inputs are resident in HBM before timing.
 
 Prints one JSON line (see the keys below). `--gpus N` with N > 1 exits:
 this build runs on one GPU.
@@ -21,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -28,43 +31,160 @@ if ROOT not in sys.path:
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 (vector) peak
 PEAK_HBM_GBS = 8000.0
 PAIR_FLOPS = 50              # per in-cutoff pair, force-only variant
-# HBM bytes per launch of the force-only pair kernel on the 23,570-atom box:
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of the same launch, each scaled by
-# what the counter reads for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0);
-# the runs and their commands are kept under profiles/.
-PAIR_TRAFFIC_BYTES = 65.47e6
+# HBM bytes per launch of the force-only pair kernel, by workload: rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE of the same launches (KiB per dispatch,
+# averaged over the run's pair launches), each scaled by what the counter
+# reads for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0). Profile
+# constants: a kernel change needs a new counter run.
+PAIR_TRAFFIC_BYTES = {"water23570": 65.47e6, "water96120": 277.0e6}
+# Algorithmic HBM bytes: pair kernel 44 per atom (16-byte record read, force
+# 12 written, 16 of exclusion partners and list bookkeeping) + 4 per listed
+# atom pair; spread 24 per atom (base, fraction + charge, bin entry) + 8 per
+# half-spectrum point written; gather 44 per atom (base, fraction + charge,
+# force read and written) + 4 per real grid point read.
 
 
-def cpu_baseline(sys_, seconds: float, beta, K):
-    """The FP64 oracle (same formulas, OpenMP) on a bounded sample."""
-    import numpy as np
+def cpu_baseline(sys_, crd, vel, dt, refresh, beta, K, steps=200, warmup=20):
+    """The OpenMP FP32 CPU engine of the same step (oracle/cpu_md.cpp: the
+    reference's cell list, pair and PME formulas, its own FFT, leap-frog +
+    SETTLE) on the host cores, from the GPU run's start state: `warmup`
+    untimed steps, then `steps` timed."""
+    import dataclasses
     import oracle
-    from oracle import direct, excluded, pme_reciprocal, nl_brute, crd_to_uint, periodic_map
 
-    crd = periodic_map(sys_.crd, sys_.box)
-    u = crd_to_uint(crd, sys_.box)
+    s2 = dataclasses.replace(sys_, crd=crd)
+    md = oracle.CpuMd(s2, vel, dt, refresh=refresh, beta=beta, K=K)
+    md.run(warmup)
     t0 = time.perf_counter()
-    pairs = nl_brute(sys_, u)
-    t_nl = time.perf_counter() - t0
-    n_force = 0
-    t0 = time.perf_counter()
-    while True:
-        direct(sys_, u, pairs, beta)
-        excluded(sys_, u, beta)
-        pme_reciprocal(sys_, u, K, beta)
-        n_force += 1
-        if time.perf_counter() - t0 > seconds:
-            break
-    t_force = (time.perf_counter() - t0) / n_force
-    t_step = t_force + t_nl / 20.0
+    md.run(steps)
+    t_step = (time.perf_counter() - t0) / steps
     return {
         "value": round(0.1728 / t_step, 4),
         "unit": "ns/day",
-        "cores": oracle.num_threads(),
+        "cores": oracle.CpuMd.threads(),
         "kind": "port",
-        "sample": (f"{n_force} FP64 oracle force evaluations (direct + exclusion + PME "
-                   f"{K[0]}x{K[1]}x{K[2]}) + 1 brute-force neighbor list / 20 on the "
-                   f"{sys_.n}-atom box; {t_step * 1e3:.1f} ms/step"),
+        "sample": (f"{steps} NVE steps after {warmup} warmup of the {sys_.n}-atom box (FP32 OpenMP engine: cell "
+                   f"list every {refresh} steps, LJ + PME direct, exclusion correction, PME "
+                   f"{K[0]}x{K[1]}x{K[2]} with its own FFT, leap-frog + SETTLE); {t_step * 1e3:.2f} ms/step"),
+    }
+
+
+def time_md(mde, steps, warmup, refresh, pair_events=True, pme_events=False):
+    """Times `steps` MD steps three ways: eager launches, one hipGraph per
+    kind of step (plain / with the list rebuild), and one hipGraph holding a
+    whole refresh period (refresh - 1 plain steps and the rebuilding one;
+    steps beyond whole periods from the per-step graphs). Returns per-step
+    milliseconds of each and the launch-recorded kernel times."""
+    import torch
+    eng = mde.nb
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        mde.step()
+    K = steps
+    mk = lambda: [torch.cuda.Event(enable_timing=True) for _ in range(K)]
+    ev = {"pair": (mk(), mk())}
+    if pme_events:
+        ev["spread"] = (mk(), mk())
+        ev["gather"] = (mk(), mk())
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    for a, b in ev.values():   # torch creates the hipEvent on first record
+        for e in a + b:
+            e.record(stream)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(K):
+        if pair_events:
+            eng.lj.set_pair_events(ev["pair"][0][k], ev["pair"][1][k])
+        if pme_events:
+            eng.pme.set_events((ev["spread"][0][k], ev["spread"][1][k]), (ev["gather"][0][k], ev["gather"][1][k]))
+        mde.step()
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    eng.lj.set_pair_events(None, None)
+    if pme_events:
+        eng.pme.set_events(None, None)
+    out = {"eager": t_start.elapsed_time(t_end) / K}
+    for name, (a, b) in ev.items():
+        if name == "pair" and not pair_events:
+            continue
+        out[name + "_kernel"] = sum(x.elapsed_time(y) for x, y in zip(a, b)) / K
+
+    def replay(fn, n):
+        torch.cuda.synchronize()
+        t_start.record(stream)
+        for k in range(n):
+            fn(k)
+        t_end.record(stream)
+        torch.cuda.synchronize()
+        return t_start.elapsed_time(t_end)
+
+    g_plain, g_rebuild = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g_plain):
+        mde.step(forced_update=0, stream=torch.cuda.current_stream())
+    with torch.cuda.graph(g_rebuild):
+        mde.step(forced_update=1, stream=torch.cuda.current_stream())
+    per_step = lambda k: (g_rebuild if (k + 1) % refresh == 0 else g_plain).replay()
+    for k in range(max(warmup, refresh)):
+        per_step(k)
+    out["graph"] = replay(per_step, K) / K
+    # the refresh-period graph, warmed up by two replays of its own
+    g_block = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g_block):
+        for j in range(refresh):
+            mde.step(forced_update=1 if (j + 1) % refresh == 0 else 0, stream=torch.cuda.current_stream())
+    for _ in range(2):
+        g_block.replay()
+    periods, rest = divmod(K, refresh)
+    out["refresh_graph"] = replay(lambda k: g_block.replay() if k < periods else per_step(k - periods),
+                                  periods + rest) / K
+    out["md_steps_run"] = warmup + 2 * K + max(warmup, refresh) + 3 * refresh
+    return out
+
+
+def scale_bench(steps: int, warmup: int, refresh: int, equilibrate: int):
+    """The 96,120-atom box (32,000 waters + 60 Na+ + 60 Cl-, 98.6 A, PME
+    108^3): per-step time and the achieved bandwidth of its pair, spread and
+    gather kernels (algorithmic bytes over launch-recorded kernel times)."""
+    import torch
+    from sponge_amd import md, system
+    sys_ = system.config(2)
+    mde = md.MDEngine(sys_, refresh_interval=refresh, temperature=300.0, seed=5)
+    mde.prepare()
+    mde.equilibrate(equilibrate)
+    eng = mde.nb
+    mde.force(need_energy=1, need_virial=1)
+    pairs_in = eng.energy_parts()["pairs_in"]
+    listed = eng.nl.status()["pairs"]
+    t = time_md(mde, steps, warmup, refresh, pme_events=True)
+    ms = min(t["eager"], t["graph"], t["refresh_graph"])
+    n = sys_.n
+    Kx, Ky, Kz = eng.pme.fftx, eng.pme.ffty, eng.pme.fftz
+    b_pair = 44.0 * n + 4.0 * 2 * listed
+    b_spread = 24.0 * n + 8.0 * Kx * Ky * (Kz // 2 + 1)
+    b_gather = 44.0 * n + 4.0 * Kx * Ky * Kz
+    gbs = lambda b, ms_: round(b / (ms_ * 1e-3) / 1e9, 1)
+    temperature = mde.temperature()
+    del mde, eng
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{sys_.name}: {n} atoms, box {float(sys_.box[0])} A, rc 10 skin 2, PME {Kx}x{Ky}x{Kz}, "
+                    f"NVE 2 fs with SETTLE, list every {refresh} steps",
+        "steps": steps,
+        "us_per_step": round(ms * 1e3, 2),
+        "ns_per_day": round(0.1728 / (ms * 1e-3), 1),
+        "eager_us_per_step": round(t["eager"] * 1e3, 2),
+        "graph_us_per_step": round(t["graph"] * 1e3, 2),
+        "refresh_graph_us_per_step": round(t["refresh_graph"] * 1e3, 2),
+        "pairs_in_cutoff": int(pairs_in),
+        "pair_kernel_us": round(t["pair_kernel"] * 1e3, 2),
+        "pair_gbs": gbs(b_pair, t["pair_kernel"]),
+        "pair_tflops": round(PAIR_FLOPS * pairs_in / (t["pair_kernel"] * 1e-3) / 1e12, 2),
+        "spread_kernel_us": round(t["spread_kernel"] * 1e3, 2),
+        "spread_gbs": gbs(b_spread, t["spread_kernel"]),
+        "gather_kernel_us": round(t["gather_kernel"] * 1e3, 2),
+        "gather_gbs": gbs(b_gather, t["gather_kernel"]),
+        "final_temperature_K": round(temperature, 1),
     }
 
 
@@ -280,17 +400,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--config", type=int, default=1, help="0: 648 atoms, 1: 23,570, 2: 96,120")
     ap.add_argument("--refresh", type=int, default=20)
+    ap.add_argument("--equilibrate", type=int, default=2000,
+                    help="untimed MD steps with velocities rescaled to 300 K before timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-graph", action="store_true", help="time eager launches instead of graph replays")
     ap.add_argument("--serial", action="store_true",
                     help="run the reciprocal pipeline after the pair kernel on one stream")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--adjoint-reps", type=int, default=50,
                     help="launches per timing of the gradient calls (0 skips them)")
     ap.add_argument("--fep-steps", type=int, default=200,
                     help="steps of the batched-FEP comparison (0 skips it)")
     ap.add_argument("--fep-frames", type=int, default=20,
                     help="frames of the FEP / TI frame-evaluator comparison (0 skips it)")
+    ap.add_argument("--scale-steps", type=int, default=100,
+                    help="steps of the 96,120-atom measurement (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -315,103 +438,46 @@ def main():
     mde = md.MDEngine(sys_, refresh_interval=args.refresh, temperature=300.0, seed=5)
     mde.serial = args.serial
     eng = mde.nb
-    stream = torch.cuda.current_stream()
 
-    # Main_Initial: forced rebuild; count in-cutoff pairs of the start
+    # Main_Initial: forced rebuild; then the synthetic lattice start is
+    # relaxed (velocities rescaled to 300 K every step) so that the timed
+    # trajectory is liquid water near 300 K
     mde.prepare()
+    mde.equilibrate(args.equilibrate)
+    t_start_K = mde.temperature()
+    crd0 = eng.crd.cpu().numpy().copy()
+    vel0 = mde.vel.cpu().numpy().copy()
     mde.force(need_energy=1, need_virial=1)
     pairs_in0 = eng.energy_parts()["pairs_in"]
     nl_pairs = eng.nl.status()["pairs"]
 
-    def step():
-        # main.cu's NVE step: fused force (uint conversion fused, forces
-        # assigned), fused leap-frog + SETTLE, neighbor-list refresh
-        mde.step()
-
-    for _ in range(args.warmup):
-        step()
-    K = args.steps
-    ev_s = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    ev_e = [torch.cuda.Event(enable_timing=True) for _ in range(K)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    for ev in ev_s + ev_e:   # torch creates the hipEvent on first record
-        ev.record(stream)
-    torch.cuda.synchronize()
-    t_start.record(stream)
-    for k in range(K):
-        eng.lj.set_pair_events(ev_s[k], ev_e[k])
-        step()
-    t_end.record(stream)
-    torch.cuda.synchronize()
-    eng.lj.set_pair_events(None, None)
-    eager_ms = t_start.elapsed_time(t_end) / K
-    pair_ms = sum(a.elapsed_time(b) for a, b in zip(ev_s, ev_e)) / K
-
-    # The same MD steps replayed from hipGraphs, one per kind of step: force +
-    # integration, and force + integration + list rebuild (every `refresh`
-    # steps). Host launch overhead leaves the critical path.
-    graph_ms = None
-    if not args.no_graph:
-        g_plain, g_rebuild = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g_plain):
-            mde.step(forced_update=0, stream=torch.cuda.current_stream())
-        with torch.cuda.graph(g_rebuild):
-            mde.step(forced_update=1, stream=torch.cuda.current_stream())
-        for k in range(args.warmup):
-            (g_rebuild if (k + 1) % args.refresh == 0 else g_plain).replay()
-        torch.cuda.synchronize()
-        t_start.record(stream)
-        for k in range(K):
-            (g_rebuild if (k + 1) % args.refresh == 0 else g_plain).replay()
-        t_end.record(stream)
-        torch.cuda.synchronize()
-        graph_ms = t_start.elapsed_time(t_end) / K
-    # One refresh period (refresh - 1 plain steps and the rebuilding one)
-    # captured as a single graph: no replay boundary between its steps.
-    block_ms = None
-    if not args.no_graph and K % args.refresh == 0 and args.warmup % args.refresh == 0:
-        g_block = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g_block):
-            for j in range(args.refresh):
-                mde.step(forced_update=1 if (j + 1) % args.refresh == 0 else 0, stream=torch.cuda.current_stream())
-        for _ in range(args.warmup // args.refresh):
-            g_block.replay()
-        torch.cuda.synchronize()
-        t_start.record(stream)
-        for _ in range(K // args.refresh):
-            g_block.replay()
-        t_end.record(stream)
-        torch.cuda.synchronize()
-        block_ms = t_start.elapsed_time(t_end) / K
+    t = time_md(mde, args.steps, args.warmup, args.refresh)
+    eager_ms, graph_ms, block_ms, pair_ms = t["eager"], t["graph"], t["refresh_graph"], t["pair_kernel"]
     # the fastest of the ways of issuing the same steps
-    ms = min(x for x in (eager_ms, graph_ms, block_ms) if x is not None)
+    ms = min(eager_ms, graph_ms, block_ms)
 
-    status = eng.nl.status()
-    if status["overflow"]:
-        print(f"bench.py: neighbor-list capacity overflow {status}", file=sys.stderr)
-        sys.exit(1)
+    eng.nl.check()
     eng.update_neighbor_list(forced=1)
     mde.force(need_energy=1, need_virial=1)
     pairs_in1 = eng.energy_parts()["pairs_in"]
     pairs_in = 0.5 * (pairs_in0 + pairs_in1)
-    n_md = (args.warmup + K) * (1 + (graph_ms is not None) + (block_ms is not None))
-    temperature = 2 * mde.kinetic_energy() / ((6 * sys_.waters + 3 * (sys_.n - 3 * sys_.waters)) * system.KB)
+    temperature = mde.temperature()
 
     flops = PAIR_FLOPS * pairs_in
     achieved = flops / (pair_ms * 1e-3) / 1e12
+    traffic = PAIR_TRAFFIC_BYTES.get(sys_.name)
     out = {
         "metric": "ns/day (24k-atom PME water, 2 fs, NVE MD step with SETTLE)",
         "value": round(0.1728 / (ms * 1e-3), 3),
         "unit": "ns/day",
         "n_gpus": 1,
-        "steps": K,
+        "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 6),
         "eager_ms_per_step": round(eager_ms, 6),
-        "graph_ms_per_step": None if graph_ms is None else round(graph_ms, 6),
-        "refresh_graph_ms_per_step": None if block_ms is None else round(block_ms, 6),
-        "timed_as": ("eager" if ms == eager_ms else "graph" if ms == graph_ms else
+        "graph_ms_per_step": round(graph_ms, 6),
+        "refresh_graph_ms_per_step": round(block_ms, 6),
+        "timed_as": ("eager" if ms == eager_ms else "graph per step" if ms == graph_ms else
                      f"graph of {args.refresh} steps") + (", one stream" if args.serial else
                                                           ", PME on a second stream"),
         "higher_is_better": True,
@@ -427,33 +493,45 @@ def main():
             "atoms": sys_.n,
             "listed_pairs": int(nl_pairs),
             "pairs_in_cutoff": int(pairs_in),
-            "md_steps_run": n_md,
+            "equilibration_steps": args.equilibrate,
+            "start_temperature_K": round(t_start_K, 1),
+            "md_steps_run": t["md_steps_run"],
             "final_temperature_K": round(temperature, 1),
             "parallelism": "single GPU",
         },
         "roofline": {
             "kernel": "pair_kernel<false> (LJ + PME direct)",
-            "bound": "mfma",
-            "note": "FP32 VALU bound (no MFMA); peak = MI355X FP32 157.3 TFLOP/s",
+            "bound": "valu",
+            "note": "FP32 VALU bound (no MFMA shape); peak = MI355X FP32 vector 157.3 TFLOP/s",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": PAIR_TRAFFIC_BYTES if args.config == 1 else None,
+            "traffic": traffic,
             "traffic_unit": "bytes per launch",
+            "traffic_source": ("profile constant: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, "
+                               "scaled by a 256 MiB copy" if traffic else None),
             "kernel_ms": round(pair_ms, 6),
             "flops_per_launch": flops,
         },
     }
+    del mde, eng
+    torch.cuda.empty_cache()
+    if args.scale_steps > 0:
+        out["scale"] = scale_bench(args.scale_steps, 20, args.refresh, max(500, args.equilibrate // 2))
     if args.adjoint_reps > 0:
-        out["adjoint"] = adjoint_bench(eng, args.adjoint_reps)
+        from sponge_amd import nb
+        out["adjoint"] = adjoint_bench(nb.NonbondedEngine(sys_, refresh_interval=args.refresh), args.adjoint_reps)
     if args.fep_steps > 0:
         out["fep_batch"] = fep_batch_bench(args.fep_steps, 20, args.refresh)
     if args.fep_frames > 0:
         out["fep_frames"] = fep_frames_bench(args.fep_frames)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sys_, args.cpu_seconds, eng.pme.beta,
-                                           (eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
+        import oracle
+        beta = float(np.float32(oracle.pme_beta(sys_.cutoff, sys_.pme_tolerance)))
+        K = tuple(sys_.fft[d] if sys_.fft[d] > 0 else system.fft_size(sys_.box[d]) for d in range(3))
+        out["cpu_baseline"] = cpu_baseline(sys_, crd0, vel0, 0.002 * system.TIME_UNIT_PER_PS, args.refresh, beta,
+                                           K, steps=args.cpu_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -100,13 +100,18 @@ int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_list_start, co
 /* Inner list of the fused step (no counterpart in the reference): every
  * prune_interval fused steps, sponge_nb_force_step keeps only the list
  * entries and pair bits within prune_radius (>= cutoff) of the current
- * coordinates and runs the pair kernel over them; 0 turns it off. The caller
- * vouches that no atom moves more than (prune_radius - cutoff) / 2 between
- * prunes. Default for refresh_interval > 0: interval refresh_interval / 4 and
- * radius cutoff + skin * interval / refresh_interval (the skin prorated, the
- * reference's own assumption for interval updates); off otherwise. The other
+ * coordinates and runs the pair kernel over them; 0 turns it off. Between
+ * scheduled prunes, a step at which some atom has moved more than
+ * (prune_radius - cutoff) / 2 since the last prune prunes at once (decided on
+ * the device), so the inner list never misses a pair inside the cutoff that
+ * the full list holds. Default for refresh_interval > 0: interval
+ * refresh_interval / 5 and radius cutoff + skin / 2; off otherwise. The other
  * entry points always use the full list. */
 int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius);
+
+/* Synchronises. Number of inner-list prunes that the displacement test above
+ * caused (beyond the scheduled ones) since the list was created. */
+int sponge_nl_prune_votes(sponge_nl* nl, int* voted_prunes);
 
 /* NEIGHBOR_LIST::Update_Volume (neighbor_list/neighbor_list.cu:645-656). */
 int sponge_nl_update_volume(sponge_nl* nl, const float box_length[3]);
@@ -229,6 +234,15 @@ int sponge_lj_set_pair_events(sponge_lj* lj, void* start_event, void* stop_event
 int sponge_pme_create(int atom_numbers, const float box_length[3], float cutoff,
                       float direct_tolerance, int fftx, int ffty, int fftz, sponge_pme** out);
 void sponge_pme_destroy(sponge_pme* pme);
+
+/* Measurement hook (no counterpart in the reference), like
+ * sponge_lj_set_pair_events: when set, every later launch of the charge
+ * spread (with its z transforms) records spread_start / spread_stop and every
+ * force gather records gather_start / gather_stop (hipEvent_t) at the
+ * kernel's own start and end. NULL pairs switch them off. Not for graph
+ * capture. */
+int sponge_pme_set_events(sponge_pme* pme, void* spread_start, void* spread_stop, void* gather_start,
+                          void* gather_stop);
 
 /* Public members the reference's callers read: beta (PME_force.cuh:62) and
  * fftx/ffty/fftz (:30-32). */

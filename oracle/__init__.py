@@ -496,3 +496,82 @@ def barostat_run(sys_, vel0, nsteps, dt_ps, groups, kind="berendsen", target_pre
             box *= mu
             log.append(dict(step=step, pressure=p, mu=mu))
     return crd, vel, box, log
+
+
+# ---------------------------------------------------------------------------
+# OpenMP FP32 CPU baseline of the MD step (cpu_md.cpp): bench.py's
+# cpu_baseline leg and its tests.
+
+_md_lib = None
+
+
+def _cpu_md_lib():
+    global _md_lib
+    if _md_lib is None:
+        path = os.path.join(_HERE, "_build", "libcpumd.so")
+        if not os.path.exists(path):
+            build(force=True)
+        L = ctypes.CDLL(path)
+        P, I, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        L.cpu_md_create.restype = P
+        L.cpu_md_create.argtypes = [I, P, P, P, P, P, P, I, P, P, P, P, P, F, F, I, I, I, I, F, P, F, I, F, F]
+        L.cpu_md_destroy.argtypes = [P]
+        L.cpu_md_forces.argtypes = [P, P]
+        L.cpu_md_run.restype = I
+        L.cpu_md_run.argtypes = [P, I]
+        L.cpu_md_state.argtypes = [P, P, P]
+        L.cpu_md_threads.restype = I
+        _md_lib = L
+    return _md_lib
+
+
+class CpuMd:
+    """The whole NVE step (list every `refresh` steps, LJ + PME direct,
+    exclusion correction, SPME reciprocal with its own FFT, leap-frog +
+    SETTLE) in FP32 with OpenMP on the host cores."""
+
+    def __init__(self, sys_, vel, dt, refresh=20, beta=None, K=None):
+        from sponge_amd.system import fft_size, TIP3P_ROH, TIP3P_HOH_DEG
+        import math
+        if beta is None:
+            beta = float(np.float32(pme_beta(sys_.cutoff, sys_.pme_tolerance)))
+        if K is None:
+            K = tuple(sys_.fft[d] if sys_.fft[d] > 0 else fft_size(sys_.box[d]) for d in range(3))
+        self.K, self.beta, self.n = tuple(int(k) for k in K), float(beta), sys_.n
+        bc = pme_bc_full(self.K, sys_.box, beta)
+        es, en, el = _excl(sys_)
+        self._keep = [_c(a, dt_) for a, dt_ in (
+            (sys_.box, np.float32), (sys_.crd, np.float32), (vel, np.float32), (sys_.mass, np.float32),
+            (sys_.charge, np.float32), (sys_.lj_type, np.int32), (sys_.lj_A, np.float32),
+            (sys_.lj_B, np.float32))] + [es, en, el, _c(bc, np.float64)]
+        k = self._keep
+        r_hh = 2.0 * TIP3P_ROH * math.sin(math.radians(TIP3P_HOH_DEG) / 2)
+        self.h = _cpu_md_lib().cpu_md_create(
+            sys_.n, _p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]), _p(k[5]), sys_.ntypes, _p(k[6]), _p(k[7]),
+            _p(k[8]), _p(k[9]), _p(k[10]), sys_.cutoff, sys_.skin, int(refresh), *self.K, self.beta, _p(k[11]),
+            float(dt), int(sys_.waters), TIP3P_ROH, r_hh)
+
+    def forces(self):
+        out = np.zeros((self.n, 3), np.float32)
+        _cpu_md_lib().cpu_md_forces(self.h, _p(out))
+        return out
+
+    def run(self, steps):
+        """`steps` NVE steps; raises if a neighbor list overflowed."""
+        if _cpu_md_lib().cpu_md_run(self.h, int(steps)):
+            raise RuntimeError("CPU baseline: neighbor list capacity (1024 per atom) exceeded")
+
+    def state(self):
+        c = np.zeros((self.n, 3), np.float32)
+        v = np.zeros((self.n, 3), np.float32)
+        _cpu_md_lib().cpu_md_state(self.h, _p(c), _p(v))
+        return c, v
+
+    @staticmethod
+    def threads():
+        return _cpu_md_lib().cpu_md_threads()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _cpu_md_lib().cpu_md_destroy(self.h)
+            self.h = None

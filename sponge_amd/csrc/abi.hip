@@ -136,7 +136,7 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
         SNB_HIP(hipEventRecord(pme->join, ps));
         SNB_HIP(hipStreamWaitEvent(s, pme->join, 0));
     }
-    rc = snb_pme_gather(pme, d_frc, s);
+    rc = snb_pme_gather(pme, d_frc, s, nl->prune_interval > 0 ? nl->flags + 10 : nullptr);
     if (rc) return rc;
     if (energy) {
         rc = snb_pme_finalize(pme, nullptr, 0, 0, nullptr, nullptr, s);

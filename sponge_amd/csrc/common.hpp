@@ -169,14 +169,23 @@ struct sponge_nl {
     int* flags;                 // [0] rebuild flag, [1] overflow bits, [2] rebuild count,
                                 // [3] slot count, [4] cluster count, [5] atoms left out
                                 // (column overflow), [7] skin-check vote,
-                                // [8..9] listed pair count (64-bit), [10] fused steps since
-                                // the last rebuild
+                                // [8..9] listed pair count (64-bit), [10] fused steps
+                                // completed since the last rebuild
     int device;
     // inner list of the fused step: the entries and masks of `list` whose
     // pairs lie within prune_r2, recomputed every prune_interval fused steps
     // (0 = off) from the step's coordinates
     int prune_interval;
     float prune_r2;
+    // displacement vote: the fused step's prep kernel writes its step number
+    // (flags[10] + 1) into flags[13] when an atom moved more than half the
+    // prune margin, (prune radius - cutoff) / 2, since the last prune; the
+    // pair kernel of that step then prunes at once (a pair left out at the
+    // prune was at least the prune radius apart, so it cannot be inside the
+    // cutoff before some atom moved that far). flags[15] counts the prunes
+    // the vote caused.
+    float prune_vote_r2;
+    uint4* prune_ref;           // slot_cap: slot records at the last prune
     int* list_in_n;             // ncl_cap
     uint2* list_in;             // ncl_cap * list_cap
     // the inner pairs per i slot (j slots within prune_r2) for plain LJ,
@@ -247,6 +256,7 @@ struct sponge_pme {
     hipEvent_t fork, join;       // concurrently with the pair kernel
     void* fep;                   // frame-evaluator scratch, allocated by the first call
     int conv_split;              // 2D transforms as line launches (large planes)
+    hipEvent_t ev[4];            // optional: recorded by the spread and gather launches
 };
 
 // ---- batched systems ------------------------------------------------------
@@ -295,7 +305,9 @@ struct PruneArgs {
     int* atom_n = nullptr;      // per-slot inner lists (NULL: cluster form only)
     int* atom_list = nullptr;
     int atom_cap = 0;
-    int* diag = nullptr;        // flags + 11: [0] per-slot overflow, [1] per-slot launches
+    int* diag = nullptr;        // flags + 11: [0] per-slot overflow, [1] per-slot launches,
+                                // [2] step of the last displacement vote, [4] voted prunes
+    uint4* ref = nullptr;       // slot records at the last prune
 };
 
 // Excluded-pair correction folded into the pair kernel (start NULL: off):
@@ -321,7 +333,7 @@ int snb_excluded(sponge_pme* pme, const unsigned int* d_uint_crd, const float sc
                  const int* es, const int* el, const int* en, float* d_frc, float* d_atom_energy, hipStream_t s);
 int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
                        int energy, int prep_done, hipStream_t s);
-int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s);
+int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s, int* step_counter = nullptr);
 int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s);
 int snb_pme_convolve(sponge_pme* pme, int energy, hipStream_t s);
 int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd, unsigned int* d_uint_crd,
@@ -336,6 +348,7 @@ size_t snb_pme_lds_conv(const sponge_pme* pme);
 int snb_pme_raise_lds(size_t bytes);
 size_t snb_pme_lds_inv(const sponge_pme* pme);
 float snb_nl_rlm(const sponge_nl* nl);
+float snb_prune_vote_r2(float cutoff, float prune_radius);
 int snb_batch_prep(sponge_batch* b, int energy, int from_crd, int overwrite, hipStream_t s);
 int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s);
 int snb_batch_excluded(sponge_batch* b, hipStream_t s);

@@ -74,6 +74,7 @@ __device__ __forceinline__ void nl_begin_body(int forced, int* flags, int* col_c
             flags[9] = 0;
             flags[2] += 1;
             flags[10] = 0;  // the next fused step prunes the new list
+            flags[13] = 0;  // no displacement vote on the new list yet
             flags[11] = 0;  // and rewrites the per-slot inner lists
         }
     }
@@ -656,15 +657,20 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
     if (e == hipSuccess) e = hipMemset(nl->list_n, 0, nl->ncl_cap * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->list_in_n, 0, nl->ncl_cap * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->atom_in_n, 0, nl->slot_cap * sizeof(int));
-    // Inner list of the fused step: in interval mode the reference trusts the
-    // skin to cover refresh_interval steps of motion; prorating it, an inner
-    // radius of rc + skin * P / refresh_interval covers P steps, and the list
-    // is pruned to it every P = refresh_interval / 4 fused steps.
+    // Inner list of the fused step (interval mode): pruned every
+    // P = refresh_interval / 5 fused steps to rc + skin / 2, and at once on a
+    // step where an atom has moved more than skin / 4 since the last prune.
+    // Measured on 300 K water (23,570 and 96,120 atoms, refresh 20): P = 4
+    // with a 1 A margin needs no displacement prunes and was the fastest of
+    // P = 4..20 with margins 0.5..2 A.
     if (refresh_interval > 0) {
-        nl->prune_interval = std::max(1, refresh_interval / 4);
-        const float rp = cutoff + skin * (float)nl->prune_interval / (float)refresh_interval;
+        nl->prune_interval = std::max(1, refresh_interval / 5);
+        const float rp = cutoff + 0.5f * skin;
         nl->prune_r2 = rp * rp;
+        nl->prune_vote_r2 = snb_prune_vote_r2(cutoff, rp);
     }
+    A((void**)&nl->prune_ref, nl->slot_cap * sizeof(uint4));
+    if (e == hipSuccess) e = hipMemset(nl->prune_ref, 0, nl->slot_cap * sizeof(uint4));
     if (e == hipSuccess) e = hipMemset(nl->s2o, 0xff, nl->slot_cap * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->col_start, 0, (nl->ncol + 1) * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->old_crd, 0, 3 * n * sizeof(float));
@@ -682,10 +688,18 @@ extern "C" void sponge_nl_destroy(sponge_nl* nl) {
     void* ptrs[] = {nl->old_crd, nl->nl_uint, nl->col_count, nl->col_start, nl->col_atoms, nl->s2o,
                     nl->o2s,     nl->cl_lo,   nl->cl_hi,     nl->list_n,    nl->list,      nl->pack,
                     nl->flags,   nl->list_in, nl->list_in_n, nl->sym_start, nl->sym_list, nl->atom_in_n,
-                    nl->atom_in, nl->dropped};
+                    nl->atom_in, nl->dropped, nl->prune_ref};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete nl;
+}
+
+// Squared displacement past which the fused step re-prunes its inner list:
+// half the prune margin, less 1 % for the FP32 rounding of the distances
+// compared at the prune.
+float snb_prune_vote_r2(float cutoff, float prune_radius) {
+    const float h = 0.5f * (prune_radius - cutoff);
+    return 0.98f * h * h;
 }
 
 // Bounding-box search radius: rc + skin with a small margin for the FP32
@@ -816,6 +830,7 @@ extern "C" int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prun
     SNB_ARG(prune_interval == 0 || prune_radius >= nl->cutoff, "prune radius below the cutoff");
     nl->prune_interval = prune_interval;
     nl->prune_r2 = prune_radius * prune_radius;
+    nl->prune_vote_r2 = snb_prune_vote_r2(nl->cutoff, prune_radius);
     return SPONGE_NB_OK;
 }
 
@@ -846,6 +861,15 @@ extern "C" int sponge_nl_inner_status(sponge_nl* nl, int* slot_overflow, int* sl
     SNB_HIP(hipMemcpy(f, nl->flags, sizeof(f), hipMemcpyDeviceToHost));
     if (slot_overflow) *slot_overflow = f[11];
     if (slot_launches) *slot_launches = f[12];
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_nl_prune_votes(sponge_nl* nl, int* voted_prunes) {
+    SNB_ARG(nl && voted_prunes, "nl_prune_votes");
+    int f[16];
+    SNB_HIP(hipDeviceSynchronize());
+    SNB_HIP(hipMemcpy(f, nl->flags, sizeof(f), hipMemcpyDeviceToHost));
+    *voted_prunes = f[15];
     return SPONGE_NB_OK;
 }
 

@@ -70,11 +70,10 @@ __device__ __forceinline__ void step_prep_body(int blk, const int* __restrict__ 
                                                const int* __restrict__ type, uint4* __restrict__ pk,
                                                const PmePrep& P, int energy, double* __restrict__ acc,
                                                int* __restrict__ step_counter, const int* __restrict__ dropped,
-                                               float* __restrict__ zero_frc) {
+                                               float* __restrict__ zero_frc, const uint4* __restrict__ prune_ref = nullptr,
+                                               float vote_r2 = 0.f, float cx = 0.f, float cy = 0.f, float cz = 0.f,
+                                               int* __restrict__ vote = nullptr) {
     const int p = blk * blockDim.x + threadIdx.x;
-    // fused steps since the last list rebuild, this one included (read by
-    // this step's pair kernel to decide whether it re-prunes the inner list)
-    if (step_counter && p == 0) atomicAdd(step_counter, 1);
     if (energy && p < 3) acc[p] = 0.0;  // SPONGE_PART_LJ, _DIRECT, _LJ_VIRIAL
     if (energy && p == 3) acc[SPONGE_PART_PAIRS_IN] = 0.0;
     const int nslot = flags[3];
@@ -124,6 +123,20 @@ __device__ __forceinline__ void step_prep_body(int blk, const int* __restrict__ 
     } else if (p < nslot) {
         pk[p] = make_uint4(0u, 0u, 0u, 0u);
     }
+    if (vote) {
+        // displacement of the slot's atom since the inner list's last prune
+        bool moved = false;
+        if (a >= 0 && !left_out) {
+            const uint4 r = prune_ref[p];
+            const float dx = (float)(int)(u0 - (r.x & ~3u)) * cx;
+            const float dy = (float)(int)(u1 - (r.y & ~3u)) * cy;
+            const float dz = (float)(int)(u2 - (r.z & ~3u)) * cz;
+            moved = dx * dx + dy * dy + dz * dz > vote_r2;
+        }
+        // this step's number since the last rebuild (the count of completed
+        // steps is advanced by the step's last kernel, so it is fixed here)
+        if (__ballot(moved) && (threadIdx.x & 63) == 0) atomicMax(vote, *step_counter + 1);
+    }
     pme_prep_block(a, u0, u1, u2, qa, P, energy);
 }
 
@@ -133,9 +146,11 @@ __global__ __launch_bounds__(256) void step_prep_kernel(const int* __restrict__ 
                                                         const int* __restrict__ type, uint4* __restrict__ pk,
                                                         PmePrep P, int energy, double* __restrict__ acc,
                                                         int* __restrict__ step_counter, const int* __restrict__ dropped,
-                                                        float* __restrict__ zero_frc) {
+                                                        float* __restrict__ zero_frc, const uint4* __restrict__ prune_ref,
+                                                        float vote_r2, float cx, float cy, float cz,
+                                                        int* __restrict__ vote) {
     step_prep_body(blockIdx.x, flags, s2o, crd, cqx, cqy, cqz, uc, q, type, pk, P, energy, acc, step_counter, dropped,
-                   zero_frc);
+                   zero_frc, prune_ref, vote_r2, cx, cy, cz, vote);
 }
 
 __global__ __launch_bounds__(256) void step_prep_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map,
@@ -383,16 +398,24 @@ __device__ __forceinline__ void pair_body(
     const int ic = lb * kPairWaves + w;
     if (lb < 0 || ic >= flags[4]) return;
     // Inner list (pa.interval > 0): on every pa.interval-th fused step since
-    // the last rebuild this launch prunes the full list to the pairs within
-    // pa.r2; the other steps read the pruned list. The counter was advanced by
-    // this step's prep kernel. Plain LJ keeps the pruned pairs per i slot
-    // (force-only steps read them, prune steps included; energy steps read
-    // the full list), soft-core LJ as cluster pairs.
+    // the last rebuild, and on a step whose prep kernel voted (an atom moved
+    // more than half the prune margin since the last prune), this launch
+    // prunes the full list to the pairs within pa.r2 and keeps the slot
+    // records it pruned at; the other steps read the pruned list. c = this
+    // step's number since the rebuild (the count of completed steps is
+    // advanced by the step's gather, after this kernel). Plain LJ keeps the
+    // pruned pairs per i slot (force-only steps read them, prune steps
+    // included; energy steps read the full list), soft-core LJ as cluster
+    // pairs.
     bool use_inner = false, write_inner = false;
     if (pa.interval > 0) {
-        const int c = flags[10];
-        write_inner = c <= 1 || (c - 1) % pa.interval == 0;
+        const int c = flags[10] + 1;
+        const bool scheduled = c <= 1 || (c - 1) % pa.interval == 0;
+        const bool voted = pa.diag[2] == c;
+        write_inner = scheduled || voted;
         use_inner = !write_inner;
+        if (write_inner && lane < kClusterSize) pa.ref[ic * kClusterSize + lane] = pk[ic * kClusterSize + lane];
+        if (voted && !scheduled && ic == 0 && lane == 0) atomicAdd(pa.diag + 4, 1);
     }
     const bool slots = !SOFT && pa.atom_list != nullptr && pa.interval > 0;
     if (slots && !ENERGY) {
@@ -1013,6 +1036,7 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
         pa.atom_list = nl->atom_in;
         pa.atom_cap = nl->atom_in_cap;
         pa.diag = nl->flags + 11;
+        pa.ref = nl->prune_ref;
     }
     const int* list_n = nl->list_n;
     const uint2* list = nl->list;
@@ -1080,10 +1104,13 @@ int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const flo
     float cq[3];
     for (int d = 0; d < 3; d++) cq[d] = 0.25f * (4294967296.0f / lj->box[d]);
     const int cap = lj->ps.cap;
+    const bool prune = nl->prune_interval > 0;
     step_prep_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(nl->flags, nl->s2o, d_crd, cq[0], cq[1], cq[2], d_uint_crd,
                                                         d_charge, lj->d_type, lj->ps.pk, pme_prep_args(pme), energy,
-                                                        lj->ps.acc, nl->prune_interval > 0 ? nl->flags + 10 : nullptr,
-                                                        nl->dropped, d_frc_assigned);
+                                                        lj->ps.acc, prune ? nl->flags + 10 : nullptr, nl->dropped,
+                                                        d_frc_assigned, nl->prune_ref, nl->prune_vote_r2,
+                                                        lj->cof_dr[0], lj->cof_dr[1], lj->cof_dr[2],
+                                                        prune ? nl->flags + 13 : nullptr);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }

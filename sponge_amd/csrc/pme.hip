@@ -21,6 +21,7 @@
 // All FFT passes are mixed-radix (8,4,2,3,5,7) Stockham steps over lines held
 // in LDS, each line owned by one wave for all of its passes.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <complex>
@@ -921,11 +922,15 @@ __global__ __launch_bounds__(kGatherThreads) void pme_adj_gather_kernel(
     if (i < n && sub == 0) out[i] += sign * acc;
 }
 
+// step_counter (the fused step's count of completed steps since the last
+// list rebuild, or NULL) is advanced here: the gather is the step's last
+// kernel, so the count stays fixed while the prep and pair kernels read it.
 __global__ __launch_bounds__(kGatherThreads) void pme_gather_kernel(int n, const unsigned* __restrict__ base,
                                                                     const float4* __restrict__ frac,
                                                                     const float* __restrict__ R, int Kx, int Ky,
                                                                     int Kz, float ibx, float iby, float ibz,
-                                                                    float* __restrict__ frc) {
+                                                                    float* __restrict__ frc, int* step_counter) {
+    if (step_counter && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(step_counter, 1);
     pme_gather_body(blockIdx.x, n, base, frac, R, Kx, Ky, Kz, ibx, iby, ibz, frc);
 }
 
@@ -1300,6 +1305,30 @@ static int kernel_set(int m) {
         }                                                                                               \
     } while (0)
 
+// The same dispatch through hipExtLaunchKernelGGL, whose launch records the
+// two events at the kernel's own start and end (measurement hook).
+#define SNB_RADIX_DISPATCH_EV(SET, KERNEL, GRID, BLOCK, LDS, STREAM, E0, E1, ...)                          \
+    do {                                                                                                   \
+        switch (SET) {                                                                                     \
+            case kRadix8:                                                                                  \
+                hipExtLaunchKernelGGL(KERNEL<kRadix8>, dim3(GRID), dim3(BLOCK), (std::uint32_t)(LDS), STREAM, \
+                                      E0, E1, 0u, __VA_ARGS__);                                            \
+                break;                                                                                     \
+            case kRadix34:                                                                                 \
+                hipExtLaunchKernelGGL(KERNEL<kRadix34>, dim3(GRID), dim3(BLOCK), (std::uint32_t)(LDS),      \
+                                      STREAM, E0, E1, 0u, __VA_ARGS__);                                    \
+                break;                                                                                     \
+            case kRadix248:                                                                                \
+                hipExtLaunchKernelGGL(KERNEL<kRadix248>, dim3(GRID), dim3(BLOCK), (std::uint32_t)(LDS),     \
+                                      STREAM, E0, E1, 0u, __VA_ARGS__);                                    \
+                break;                                                                                     \
+            default:                                                                                       \
+                hipExtLaunchKernelGGL(KERNEL<kRadixAll>, dim3(GRID), dim3(BLOCK), (std::uint32_t)(LDS),     \
+                                      STREAM, E0, E1, 0u, __VA_ARGS__);                                    \
+                break;                                                                                     \
+        }                                                                                                  \
+    } while (0)
+
 // Stage 1 of the pipeline: spread + z transforms of the binned atoms.
 int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s) {
     const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2];
@@ -1307,9 +1336,15 @@ int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s) {
     plan_axis(Kz, &pz);
     const int nyb = row_blocks(Ky);
     double* acc = energy ? pme->acc : nullptr;
-    SNB_RADIX_DISPATCH(kernel_set(radix_mask(pz)), spread_z_kernel, Kx * nyb, kZThreads, spread_z_lds(Kz), s,
-                       pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb,
-                       pz, pme->d_tw[2], acc);
+    if (pme->ev[0]) {
+        SNB_RADIX_DISPATCH_EV(kernel_set(radix_mask(pz)), spread_z_kernel, Kx * nyb, kZThreads, spread_z_lds(Kz), s,
+                              pme->ev[0], pme->ev[1], pme->base, pme->frac, pme->bin_count, pme->bin_atoms,
+                              pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb, pz, pme->d_tw[2], acc);
+    } else {
+        SNB_RADIX_DISPATCH(kernel_set(radix_mask(pz)), spread_z_kernel, Kx * nyb, kZThreads, spread_z_lds(Kz), s,
+                           pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz,
+                           nyb, pz, pme->d_tw[2], acc);
+    }
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
@@ -1364,12 +1399,31 @@ int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const fl
 }
 
 // Reciprocal forces from the convolved grid R, added to d_frc.
-int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s) {
+int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s, int* step_counter) {
     const int n = pme->n, Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2];
-    pme_gather_kernel<<<ceil_div((long)n * 16, kGatherThreads), kGatherThreads, 0, s>>>(
-        n, pme->base, pme->frac, pme->R, Kx, Ky, Kz, (float)Kx / pme->box[0], (float)Ky / pme->box[1],
-        (float)Kz / pme->box[2], d_frc);
+    const int grid = ceil_div((long)n * 16, kGatherThreads);
+    if (pme->ev[2])
+        hipExtLaunchKernelGGL(pme_gather_kernel, dim3(grid), dim3(kGatherThreads), 0u, s, pme->ev[2], pme->ev[3], 0u,
+                              n, pme->base, pme->frac, pme->R, Kx, Ky, Kz, (float)Kx / pme->box[0],
+                              (float)Ky / pme->box[1], (float)Kz / pme->box[2], d_frc, step_counter);
+    else
+        pme_gather_kernel<<<grid, kGatherThreads, 0, s>>>(n, pme->base, pme->frac, pme->R, Kx, Ky, Kz,
+                                                           (float)Kx / pme->box[0], (float)Ky / pme->box[1],
+                                                           (float)Kz / pme->box[2], d_frc, step_counter);
     SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_pme_set_events(sponge_pme* pme, void* spread_start, void* spread_stop, void* gather_start,
+                                     void* gather_stop) {
+    SNB_ARG(pme, "pme_set_events");
+    SNB_ARG((spread_start == nullptr) == (spread_stop == nullptr) &&
+                (gather_start == nullptr) == (gather_stop == nullptr),
+            "both events of a kernel or neither");
+    pme->ev[0] = reinterpret_cast<hipEvent_t>(spread_start);
+    pme->ev[1] = reinterpret_cast<hipEvent_t>(spread_stop);
+    pme->ev[2] = reinterpret_cast<hipEvent_t>(gather_start);
+    pme->ev[3] = reinterpret_cast<hipEvent_t>(gather_stop);
     return SPONGE_NB_OK;
 }
 

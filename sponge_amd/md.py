@@ -304,6 +304,28 @@ class MDEngine:
             if not torch.cuda.is_current_stream_capturing():
                 self.nb.nl.check()
 
+    def degrees_of_freedom(self):
+        """6 per rigid water, 3 per other atom."""
+        return 6 * self.sys.waters + 3 * (self.n - 3 * self.sys.waters)
+
+    def equilibrate(self, steps, temperature=300.0, stream=None):
+        """Prepares a synthetic start: `steps` MD steps, each followed by a
+        rescale of every velocity to `temperature` (the lattice start releases
+        its strain energy as heat, which the rescale removes). This is input
+        preparation, not a SPONGE thermostat; the rescale factor is computed
+        on the device (no host round trip)."""
+        import torch
+        m = self.mass[:, None].double()
+        target = 0.5 * self.degrees_of_freedom() * system.KB * float(temperature)
+        for _ in range(int(steps)):
+            self.step(stream=stream)
+            ke = 0.5 * (m * self.vel.double() ** 2).sum()
+            self.vel.mul_(torch.sqrt(target / torch.clamp(ke, min=1e-30)).float())
+        torch.cuda.synchronize()
+
+    def temperature(self):
+        return 2.0 * self.kinetic_energy() / (self.degrees_of_freedom() * system.KB)
+
     def kinetic_energy(self):
         """Sum of m v^2 / 2 of the (half-step) velocities, FP64 on the host."""
         import torch

@@ -46,6 +46,7 @@ _SIGS = {
     "sponge_nl_update_volume": (_I, [_P, _FP3]),
     "sponge_nl_set_prune": (_I, [_P, _I, _F]),
     "sponge_nl_inner_status": (_I, [_P, _P, _P]),
+    "sponge_nl_prune_votes": (_I, [_P, ctypes.POINTER(_I)]),
     "sponge_nl_status": (_I, [_P, ctypes.POINTER(_L), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "sponge_nl_set_exclusions": (_I, [_P, _P, _P, _P]),
     "sponge_nl_get_pairs": (_I, [_P, _P, _L, ctypes.POINTER(_L)]),
@@ -67,6 +68,7 @@ _SIGS = {
     "sponge_pme_reciprocal_force": (_I, [_P, _P, _P, _P, _I, _I, _P, _P, _P, _P]),
     "sponge_pme_get_energy": (_I, [_P, _P, _P, _P, _FP3, _P, _P, _P, _I, ctypes.POINTER(_D), _P]),
     "sponge_pme_update_box_length": (_I, [_P, _FP3]),
+    "sponge_pme_set_events": (_I, [_P, _P, _P, _P, _P]),
     "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "sponge_sum_list": (_I, [_I, _P, _P, _P]),
     "sponge_lj_set_tables": (_I, [_P, _P, _P]),
@@ -225,6 +227,13 @@ class NEIGHBOR_LIST:
         _check(load_library().sponge_nl_inner_status(self.h, ctypes.byref(ov), ctypes.byref(nl_)),
                "sponge_nl_inner_status")
         return dict(slot_overflow=ov.value, slot_launches=nl_.value)
+
+    def prune_votes(self):
+        """Inner-list prunes caused by the displacement test (beyond the
+        scheduled ones) since the list was created."""
+        v = _I(0)
+        _check(load_library().sponge_nl_prune_votes(self.h, ctypes.byref(v)), "sponge_nl_prune_votes")
+        return v.value
 
     def pairs(self):
         """(P,2) int32 canonical pairs (i<j), sorted lexicographically."""
@@ -470,6 +479,16 @@ class Particle_Mesh_Ewald:
             _stream(stream)), "Seperate_Direct_Atom_Energy")
         return float(out[0]), float(out[1])
 
+    def set_events(self, spread=None, gather=None):
+        """Measurement hook: (start, stop) torch.cuda.Event pairs recorded by
+        the spread and gather launches themselves; None switches one off."""
+        def ev(e):
+            return None if e is None else ctypes.c_void_p(e.cuda_event)
+        sp = spread or (None, None)
+        ga = gather or (None, None)
+        _check(load_library().sponge_pme_set_events(self.h, ev(sp[0]), ev(sp[1]), ev(ga[0]), ev(ga[1])),
+               "sponge_pme_set_events")
+
     def Update_Box_Length(self, boxlength):
         _check(load_library().sponge_pme_update_box_length(self.h, _vec3(boxlength)),
                "PME Update_Box_Length")
@@ -528,6 +547,8 @@ class NonbondedEngine:
         el = sys_.excl_list if len(sys_.excl_list) else np.zeros(1, np.int32)
         self.excl_list = t(el, torch.int32)
         self.nl = NEIGHBOR_LIST(n, self.box, sys_.cutoff, sys_.skin, refresh_interval=refresh_interval)
+        self.refresh_interval = int(refresh_interval)
+        self._interval_updates = 0
         if getattr(sys_, "lj_type_B", None) is not None:
             self.lj = LJ_SOFT_CORE(n, sys_.ntypes, sys_.ntypes_B, sys_.lj_A, sys_.lj_B, sys_.lj_A_B,
                                    sys_.lj_B_B, sys_.lj_type, sys_.lj_type_B, sys_.mask, sys_.lam,
@@ -542,6 +563,14 @@ class NonbondedEngine:
     def update_neighbor_list(self, forced=0, stream=None, forced_check=0):
         self.nl.Neighbor_List_Update(self.crd, self.excl_start, self.excl_list, self.excl_numbers,
                                      forced_update=forced, forced_check=forced_check, stream=stream)
+        if not forced and not forced_check and self.refresh_interval > 0:
+            self._interval_updates += 1
+
+    def steps_to_refresh(self):
+        """Interval mode: force steps left before the update that rebuilds
+        the list (1 = the next force step is the last on this list)."""
+        r = self.refresh_interval
+        return (r - self._interval_updates % r) % r + 1
 
     def update_volume(self, box):
         """The module half of Main_Volume_Change / Main_Box_Length_Change

@@ -239,3 +239,105 @@ def test_grid_axis_above_256():
     assert abs(p["reciprocal"] - ref["e_rec"]) <= 1e-6 * abs(ref["e_rec"])
     with pytest.raises(nb.SpongeError):
         nb.Particle_Mesh_Ewald(sys_.n, eng.box, sys_.cutoff, 1e-5, 600, 16, 16)
+
+
+TOL_ABS = 5e-4   # kcal/mol/A: per-atom FP32 force error bound of this box
+
+
+def test_prune_displacement_vote():
+    """The inner list is pruned to rc + 0.5 A with a 20-step schedule. A pair
+    8.5-8.6 A apart at the prune is not in it; one of its atoms then moves
+    0.6 A towards the other (past half the prune margin, 0.25 A), bringing the
+    pair inside rc = 8 A. The next fused step must re-prune on its own (the
+    prep kernel's vote) and include the pair: forces equal the oracle's at the
+    new coordinates, while leaving the pair out would move that atom's force
+    by more than the tolerance. A move below the margin leaves the list."""
+    import torch
+    from sponge_amd import nb
+    sys_ = system.config(0)
+    say(f"[case] prune vote, n={sys_.n}")
+    eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    rp = sys_.cutoff + 0.5
+    eng.nl.set_prune(20, rp)
+    eng.update_neighbor_list(forced=1)
+    eng.force_step(0, 0, from_crd=True, overwrite=True)   # prunes at these coordinates
+    torch.cuda.synchronize()
+    crd = eng.crd.cpu().numpy().astype(np.float64)
+    L = sys_.box.astype(np.float64)
+    oxy = np.arange(0, 3 * sys_.waters, 3)
+    d = crd[oxy][:, None, :] - crd[oxy][None, :, :]
+    d -= L * np.round(d / L)
+    r = np.linalg.norm(d, axis=2)
+    ii, jj = np.where((r > rp + 0.01) & (r < rp + 0.1))
+    assert len(ii), "no oxygen pair just outside the prune radius"
+    i, j = int(oxy[ii[0]]), int(oxy[jj[0]])
+    v = crd[j] - crd[i]
+    v -= L * np.round(v / L)
+    u = v / np.linalg.norm(v)
+    votes0 = eng.nl.prune_votes()
+    # a move below the margin: no re-prune, forces still the oracle's
+    moved = crd.copy()
+    moved[i] = np.mod(crd[i] + 0.2 * u, L)
+    eng.crd.copy_(torch.from_numpy(moved.astype(np.float32)))
+    eng.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+    assert eng.nl.prune_votes() == votes0
+    ref = oracle.full_reference(dataclasses.replace(sys_, crd=moved.astype(np.float32)), beta=eng.pme.beta,
+                                K=(eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
+    assert np.abs(eng.frc.cpu().numpy() - ref["frc"]).max() < TOL_ABS
+    # past the margin: the pair comes inside the cutoff
+    moved[i] = np.mod(crd[i] + 0.6 * u, L)
+    s2 = dataclasses.replace(sys_, crd=moved.astype(np.float32))
+    eng.crd.copy_(torch.from_numpy(moved.astype(np.float32)))
+    eng.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+    assert eng.nl.prune_votes() == votes0 + 1
+    ref = oracle.full_reference(s2, beta=eng.pme.beta, K=(eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
+    f = eng.frc.cpu().numpy()
+    err = np.abs(f - ref["frc"]).max()
+    # what leaving the pair out would change on atom i
+    pairs = ref["pairs"]
+    keep = ~(((pairs[:, 0] == min(i, j)) & (pairs[:, 1] == max(i, j))))
+    assert keep.sum() == len(pairs) - 1
+    full = oracle.direct(s2, ref["uint"], pairs, ref["beta"])
+    without = oracle.direct(s2, ref["uint"], pairs[keep], ref["beta"])
+    miss = float(np.abs(full["frc"][i] - without["frc"][i]).max())
+    say(f"    pair ({i}, {j}) at {np.linalg.norm(v):.3f} -> {np.linalg.norm(v) - 0.6:.3f} A: max force error "
+        f"{err:.2e}, the pair's force on atom {i} {miss:.2e}")
+    assert miss > 3 * TOL_ABS
+    assert err < TOL_ABS
+
+
+def test_pruned_list_under_motion():
+    """The 23,570-atom box relaxed to ~300 K, then MD steps with the default
+    prune (every 5 steps to rc + 0.5 A plus displacement votes): at the last
+    step before a refresh the fused step's forces equal a prune-off
+    evaluation, after a forced rebuild, at the same coordinates."""
+    import torch
+    from sponge_amd import md, nb
+    sys_ = system.config(1)
+    say(f"[case] pruned list under motion, n={sys_.n}")
+    mde = md.MDEngine(sys_, refresh_interval=20, temperature=300.0, seed=5)
+    mde.prepare()
+    mde.equilibrate(600)
+    for _ in range(20):
+        mde.step()
+    while mde.nb.steps_to_refresh() != 1:
+        mde.step()
+    mde.force()   # the 20th force step on this list
+    torch.cuda.synchronize()
+    f = mde.nb.frc.cpu().numpy()
+    T = mde.temperature()
+    votes = mde.nb.nl.prune_votes()
+    crd = mde.nb.crd.cpu().numpy()
+    e2 = nb.NonbondedEngine(dataclasses.replace(sys_, crd=crd), refresh_interval=20)
+    e2.nl.set_prune(0)
+    e2.update_neighbor_list(forced=1)
+    e2.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+    g = e2.frc.cpu().numpy()
+    e = rel_rms(f, g)
+    say(f"    T {T:.1f} K, voted prunes {votes}, force rel rms vs prune-off {e:.2e}, "
+        f"max abs {np.abs(f - g).max():.2e}")
+    assert 200.0 < T < 450.0
+    assert e < 1e-5

@@ -280,7 +280,7 @@ def test_graph_replay_matches_eager(case):
 
 def test_pruned_inner_list(case):
     """Interval mode prunes the fused step's list to an inner radius (default
-    rc + skin / 4, every refresh / 4 steps); forces stay those of the full list,
+    rc + skin / 2, every refresh / 5 steps); forces stay those of the full list,
     also when pruned to exactly rc every step. The force-only steps of plain
     LJ read the per-slot inner lists (all three steps here, the pruning one
     included; soft-core systems keep the cluster form)."""
@@ -316,8 +316,9 @@ def test_pruned_inner_list(case):
 def test_per_slot_overflow_falls_back_to_the_full_list():
     """The 648-atom water box as a droplet in a box 1.5 times as long: the
     per-slot lists are sized from the mean density (0.30 of the droplet's:
-    224 partners per slot), so 54 slots inside the droplet (up to 263 partners
-    within 8.5 A) overflow at the prune and their clusters read the full list;
+    224 partners per slot), so the slots inside the droplet (up to ~300
+    partners within the 9 A prune radius) overflow at the prune and their
+    clusters read the full list;
     the forces stay those of the oracle. (At twice the box length a cell
     column would hold 130 atoms for 128 places: that list overflow is a
     reported error, not this case.)"""
