@@ -69,45 +69,52 @@ def cpu_baseline(sys_, crd, vel, dt, refresh, beta, K, steps=200, warmup=20):
     }
 
 
-def time_md(mde, steps, warmup, refresh, pair_events=True, pme_events=False):
+def time_md(mde, steps, warmup, refresh, pme_events=False, compare_forms=False):
     """Times `steps` MD steps three ways: eager launches, one hipGraph per
     kind of step (plain / with the list rebuild), and one hipGraph holding a
     whole refresh period (refresh - 1 plain steps and the rebuilding one;
-    steps beyond whole periods from the per-step graphs). Returns per-step
-    milliseconds of each and the launch-recorded kernel times."""
+    steps beyond whole periods from the per-step graphs). The kernels are
+    timed in a separate eager pass with every kernel launched on its own
+    (serial form: the fused step's pair blocks ride in other launches), by
+    events the launches record themselves. Returns per-step milliseconds."""
     import torch
     eng = mde.nb
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
         mde.step()
     K = steps
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for k in range(K):
+        mde.step()
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    out = {"eager": t_start.elapsed_time(t_end) / K}
+
+    # kernel times, serial form
     mk = lambda: [torch.cuda.Event(enable_timing=True) for _ in range(K)]
     ev = {"pair": (mk(), mk())}
     if pme_events:
         ev["spread"] = (mk(), mk())
         ev["gather"] = (mk(), mk())
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
     for a, b in ev.values():   # torch creates the hipEvent on first record
         for e in a + b:
             e.record(stream)
+    mde.serial = True
     torch.cuda.synchronize()
-    t_start.record(stream)
     for k in range(K):
-        if pair_events:
-            eng.lj.set_pair_events(ev["pair"][0][k], ev["pair"][1][k])
+        eng.lj.set_pair_events(ev["pair"][0][k], ev["pair"][1][k])
         if pme_events:
             eng.pme.set_events((ev["spread"][0][k], ev["spread"][1][k]), (ev["gather"][0][k], ev["gather"][1][k]))
         mde.step()
-    t_end.record(stream)
     torch.cuda.synchronize()
+    mde.serial = False
     eng.lj.set_pair_events(None, None)
     if pme_events:
         eng.pme.set_events(None, None)
-    out = {"eager": t_start.elapsed_time(t_end) / K}
     for name, (a, b) in ev.items():
-        if name == "pair" and not pair_events:
-            continue
         out[name + "_kernel"] = sum(x.elapsed_time(y) for x, y in zip(a, b)) / K
 
     def replay(fn, n):
@@ -119,26 +126,39 @@ def time_md(mde, steps, warmup, refresh, pair_events=True, pme_events=False):
         torch.cuda.synchronize()
         return t_start.elapsed_time(t_end)
 
-    g_plain, g_rebuild = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g_plain):
-        mde.step(forced_update=0, stream=torch.cuda.current_stream())
-    with torch.cuda.graph(g_rebuild):
-        mde.step(forced_update=1, stream=torch.cuda.current_stream())
-    per_step = lambda k: (g_rebuild if (k + 1) % refresh == 0 else g_plain).replay()
-    for k in range(max(warmup, refresh)):
-        per_step(k)
-    out["graph"] = replay(per_step, K) / K
-    # the refresh-period graph, warmed up by two replays of its own
-    g_block = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g_block):
-        for j in range(refresh):
-            mde.step(forced_update=1 if (j + 1) % refresh == 0 else 0, stream=torch.cuda.current_stream())
-    for _ in range(2):
-        g_block.replay()
-    periods, rest = divmod(K, refresh)
-    out["refresh_graph"] = replay(lambda k: g_block.replay() if k < periods else per_step(k - periods),
-                                  periods + rest) / K
-    out["md_steps_run"] = warmup + 2 * K + max(warmup, refresh) + 3 * refresh
+    def graphs():
+        g_plain, g_rebuild = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g_plain):
+            mde.step(forced_update=0, stream=torch.cuda.current_stream())
+        with torch.cuda.graph(g_rebuild):
+            mde.step(forced_update=1, stream=torch.cuda.current_stream())
+        g_block = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g_block):
+            for j in range(refresh):
+                mde.step(forced_update=1 if (j + 1) % refresh == 0 else 0, stream=torch.cuda.current_stream())
+        return g_plain, g_rebuild, g_block
+
+    def time_graphs(tag):
+        g_plain, g_rebuild, g_block = graphs()
+        per_step = lambda k: (g_rebuild if (k + 1) % refresh == 0 else g_plain).replay()
+        for k in range(max(warmup, refresh)):
+            per_step(k)
+        out["graph" + tag] = replay(per_step, K) / K
+        # the refresh-period graph, warmed up by two replays of its own
+        for _ in range(2):
+            g_block.replay()
+        periods, rest = divmod(K, refresh)
+        out["refresh_graph" + tag] = replay(lambda k: g_block.replay() if k < periods else per_step(k - periods),
+                                            periods + rest) / K
+
+    time_graphs("")
+    n_graph_steps = 2 * K + max(warmup, refresh) + 3 * refresh
+    if compare_forms:
+        mde.pair_in_pipeline = True
+        time_graphs("_pair_in_pipeline")
+        mde.pair_in_pipeline = False
+        n_graph_steps *= 2
+    out["md_steps_run"] = warmup + 2 * K + n_graph_steps
     return out
 
 
@@ -451,7 +471,7 @@ def main():
     pairs_in0 = eng.energy_parts()["pairs_in"]
     nl_pairs = eng.nl.status()["pairs"]
 
-    t = time_md(mde, args.steps, args.warmup, args.refresh)
+    t = time_md(mde, args.steps, args.warmup, args.refresh, compare_forms=True)
     eager_ms, graph_ms, block_ms, pair_ms = t["eager"], t["graph"], t["refresh_graph"], t["pair_kernel"]
     # the fastest of the ways of issuing the same steps
     ms = min(eager_ms, graph_ms, block_ms)
@@ -477,9 +497,10 @@ def main():
         "eager_ms_per_step": round(eager_ms, 6),
         "graph_ms_per_step": round(graph_ms, 6),
         "refresh_graph_ms_per_step": round(block_ms, 6),
+        "pair_in_pipeline_refresh_graph_ms_per_step": round(t["refresh_graph_pair_in_pipeline"], 6),
         "timed_as": ("eager" if ms == eager_ms else "graph per step" if ms == graph_ms else
-                     f"graph of {args.refresh} steps") + (", one stream" if args.serial else
-                                                          ", PME on a second stream"),
+                     f"graph of {args.refresh} steps") + (", every kernel on its own" if args.serial else
+                                                          ", pair kernel on a second stream"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -500,7 +521,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "pair_kernel<false> (LJ + PME direct)",
+            "kernel": "pair_kernel<false> (LJ + PME direct), timed launched on its own",
             "bound": "valu",
             "note": "FP32 VALU bound (no MFMA shape); peak = MI355X FP32 vector 157.3 TFLOP/s",
             "achieved": round(achieved, 3),

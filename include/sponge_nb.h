@@ -244,6 +244,19 @@ void sponge_pme_destroy(sponge_pme* pme);
 int sponge_pme_set_events(sponge_pme* pme, void* spread_start, void* spread_stop, void* gather_start,
                           void* gather_stop);
 
+/* Launch-shape tuning (no counterpart in the reference; -1 / NULL keep a
+ * setting). step_form: how sponge_nb_force_step issues a force-only step
+ * with a plain LJ handle when its flags do not say -- 0 the pair kernel on a
+ * second stream (default), 1 the pair kernel's blocks appended to the
+ * reciprocal pipeline's launches on one stream, 2 every kernel on its own.
+ * conv_split 0 / 1 selects the one-workgroup-per-plane convolution or the
+ * three line launches (0 is refused for planes beyond the LDS). pair_share:
+ * form 1 deals the pair blocks over the pipeline's launches in these
+ * proportions -- 3 values (spread, convolution, inverse z) for the
+ * one-workgroup convolution, 5 (spread, y forward, x, y inverse, inverse z)
+ * for the split one; they apply to the convolution in use after conv_split. */
+int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_split, const float* pair_share, int n_share);
+
 /* Public members the reference's callers read: beta (PME_force.cuh:62) and
  * fftx/ffty/fftz (:30-32). */
 int sponge_pme_info(const sponge_pme* pme, float* beta, int fft[3]);
@@ -295,10 +308,16 @@ int sponge_pme_update_box_length(sponge_pme* pme, const float box_length[3]);
  * arrays are filled like the separate calls fill them.
  * The pair kernel runs on a stream of the PME handle, forked from and joined
  * back into `stream` with events (so the call stays capturable into a graph),
- * concurrently with the reciprocal spread / FFT / convolution on `stream`;
- * SPONGE_STEP_SERIAL keeps every launch on `stream`. */
+ * concurrently with the reciprocal spread / FFT / convolution on `stream`.
+ * Force-only steps with a plain LJ handle can instead append the pair
+ * kernel's blocks to the pipeline's launches on `stream`
+ * (SPONGE_STEP_PAIR_IN_PIPELINE) or run every kernel on its own
+ * (SPONGE_STEP_SERIAL); without these flags (or SPONGE_STEP_TWO_STREAMS) the
+ * form set by sponge_pme_set_tuning applies. */
 #define SPONGE_STEP_OVERWRITE_FORCE 1
 #define SPONGE_STEP_SERIAL 2
+#define SPONGE_STEP_TWO_STREAMS 8
+#define SPONGE_STEP_PAIR_IN_PIPELINE 16
 int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd,
                          unsigned int* d_uint_crd, const float* d_charge,
                          const int* d_excluded_list_start, const int* d_excluded_list,

@@ -1,0 +1,506 @@
+// Device side of the LJ + PME direct pair kernel (LENNARD_JONES_INFORMATION,
+// Lennard_Jones_force/Lennard_Jones_force.cu:424-562; soft-core terms of
+// LJ_soft_core.cu:254-300; the folded exclusion correction of
+// PME_force.cu:620-695), shared by the pair kernels of pair.hip and the fused
+// step kernels of pme.hip, which append pair-kernel blocks to the reciprocal
+// pipeline's launches.
+//
+// Layout: one wave64 per 4-atom i-cluster; each iteration takes four
+// j-clusters from the cluster-pair list, lane = (g:2, jj:2, ii:2) holds the
+// pair (ii, jj) of j-cluster g. Coordinates, charges and LJ types are read as
+// one 16-byte record per slot. The list names every neighbouring cluster
+// from both sides, so each wave only accumulates the force on its own four
+// atoms and writes it with plain read-modify-writes: no atomics.
+#pragma once
+#include "common.hpp"
+#include "pair_math.hpp"
+
+// Pair-kernel blocks appended to another kernel's launch (the fused step):
+// the launch's blocks from `first` on run logical pair blocks c0 .. c0 +
+// nlog - 1, dealt so that each XCD (workgroup b runs on XCD b % 8) gets one
+// contiguous range of i-clusters, as the standalone pair kernel does.
+struct PairLaunch {
+    const int* flags;
+    const int* list_n;
+    const uint2* list;
+    int list_cap;
+    const uint4* pk;
+    const float2* ab;
+    int nt;
+    float cx, cy, cz, rc2, beta;
+    const int* s2o;
+    float* frc;
+    double* acc;
+    int overwrite;
+    PruneArgs pa;
+    ExclArgs ex;
+    int c0, nlog;
+};
+
+namespace snb {
+
+constexpr int kPairWaves = 4;
+
+// Soft-core pair terms (LJ_soft_core.cu:254-300): with s^6 = r^6 + c per end
+// state, F = (f_LJ - f_C) dr where
+//   f_LJ = r^4 [(l_AB - l_AA s_A^-6) s_A^-12 + (lB_B - lB_A s_B^-6) s_B^-12]
+//   f_C  = q_i q_j r^4 [l_ (2b/sqrt(pi) e^{-b^2 s_A^2} + erfc(b s_A)/s_A) s_A^-6 + (B alike)]
+// s0 = {l_ AA, l_ AB, lam BA, lam BB} (l_ = 1 - lambda), s1 = {c_A, c_B, l_, lam}.
+// s = (s^6)^(1/6) is taken as exp2(log2(s^6) / 6).
+__device__ __forceinline__ void soft_pair(float r2, float qq, float beta, float mlog2e_b2, float4 s0, float4 s1,
+                                          float& flj, float& fc) {
+    const float r4 = r2 * r2, r6 = r4 * r2;
+    const float SA6 = r6 + s1.x, SB6 = r6 + s1.y;
+    const float iA6 = __builtin_amdgcn_rcpf(SA6), iB6 = __builtin_amdgcn_rcpf(SB6);
+    flj = r4 * ((s0.y - s0.x * iA6) * iA6 * iA6 + (s0.w - s0.z * iB6) * iB6 * iB6);
+    const float sA = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(SA6) * (1.0f / 6.0f));
+    const float sB = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(SB6) * (1.0f / 6.0f));
+    const float isA = __builtin_amdgcn_rcpf(sA), isB = __builtin_amdgcn_rcpf(sB);
+    const float exA = __builtin_amdgcn_exp2f(mlog2e_b2 * (sA * sA));
+    const float exB = __builtin_amdgcn_exp2f(mlog2e_b2 * (sB * sB));
+    const float tA = fmaf(exA * erfcx_poly(beta * sA), isA, kTwoOverSqrtPi * beta * exA);
+    const float tB = fmaf(exB * erfcx_poly(beta * sB), isB, kTwoOverSqrtPi * beta * exB);
+    fc = qq * r4 * (s1.z * tA * iA6 + s1.w * tB * iB6);
+}
+
+// FP64 energy-step terms of one soft-core pair from the exact displacement:
+// E_LJ, E_C and the LJ virial -f_LJ r^2 (LJ_soft_core.cu:434-473, 598-641).
+__device__ __forceinline__ void soft_pair_energy(double r2, double qq, double beta, float4 s0, float4 s1,
+                                                 double& elj, double& edir, double& vlj) {
+    const double r4 = r2 * r2, r6 = r4 * r2;
+    const double SA6 = r6 + (double)s1.x, SB6 = r6 + (double)s1.y;
+    const double iA6 = 1.0 / SA6, iB6 = 1.0 / SB6;
+    const double sA = cbrt(sqrt(SA6)), sB = cbrt(sqrt(SB6));
+    elj = ((double)s0.x * (1.0 / 12.0) * iA6 - (double)s0.y * (1.0 / 6.0)) * iA6 +
+          ((double)s0.z * (1.0 / 12.0) * iB6 - (double)s0.w * (1.0 / 6.0)) * iB6;
+    edir = qq * ((double)s1.z * erfc(beta * sA) / sA + (double)s1.w * erfc(beta * sB) / sB);
+    const double flj = r4 * (((double)s0.y - (double)s0.x * iA6) * iA6 * iA6 +
+                             ((double)s0.w - (double)s0.z * iB6) * iB6 * iB6);
+    vlj = -flj * r2;
+}
+
+// PME_Excluded_Force_With_Atom_Energy's force (PME_force.cu:620-695) on one
+// atom (slot record a) from its exclusion partners k0, k0 + kstep, ...: each
+// atom collects its own side of every excluded pair, so no other atom's force
+// is written.
+__device__ __forceinline__ void excluded_partners(const ExclArgs& ex, const uint4* __restrict__ pk, int at, int k0,
+                                                  int kstep, unsigned ax, unsigned ay, unsigned az, float qi,
+                                                  float cx, float cy, float cz, float beta, float& fix, float& fiy,
+                                                  float& fiz) {
+    const int r1 = ex.start[at + 1];
+    for (int k = ex.start[at] + k0; k < r1; k += kstep) {
+        // a partner left out of an overflowing list has no slot
+        const int js = ex.o2s[ex.list[k]];
+        if (js < 0) continue;
+        const uint4 b = pk[js];
+        const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+        const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
+        const float dz = (float)(int)((b.z & ~3u) - az) * cz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float r = sqrtf(r2);
+        const float br = beta * r;
+        const float qq = qi * __uint_as_float(b.w);
+        const float fa = -qq * ((kTwoOverSqrtPi * br * expf(-br * br) + erfcf(br) - 1.0f) / r2 / r);
+        fix += fa * dx;
+        fiy += fa * dy;
+        fiz += fa * dz;
+    }
+}
+
+// Per-slot inner lists, written on prune steps: the full list's pairs of
+// cluster ic within pa.r2 at this step's positions, per i slot in list order
+// (lanes ii, ii + 4, ... of a round in lane order). Returns the partner count
+// of slot lane >> 4 (the lane mapping of pair_atoms); a count above atom_cap
+// (partners past the cap dropped) sends the cluster to the full list until
+// the next prune. The list stores are complete and visible to the whole wave
+// on return.
+template <int TB>
+__device__ __forceinline__ int prune_slots(int ic, const int* __restrict__ list_n, const uint2* __restrict__ list,
+                                           int list_cap, const uint4* __restrict__ pk, float cx, float cy, float cz,
+                                           const PruneArgs& pa) {
+    const int lane = threadIdx.x & 63;
+    const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
+    const int islot = ic * kClusterSize + ii;
+    const uint4 a = pk[islot];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
+    const int n = list_n[ic];
+    const uint2* lst = list + (size_t)ic * list_cap;
+    int* out = pa.atom_list + (size_t)islot * pa.atom_cap;
+    const unsigned long long sel = 0x1111111111111111ull << ii, below = (1ull << lane) - 1ull;
+    int count = 0;
+    for (int base = 0; base < n; base += 64) {
+        const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
+        const int cnt = min(64, n - base);
+        // four rounds of 4 entries per pass, their j records loaded together
+        // (the loop is latency bound: little arithmetic per record)
+        for (int it = 0; it < cnt; it += 16) {
+            int jcs[4];
+            unsigned ms[4];
+            uint4 bs[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = it + 4 * u + g;
+                const int src = min(e, 63);
+                jcs[u] = __shfl((int)my.x, src);
+                const unsigned mm = (unsigned)__shfl((int)my.y, src);
+                ms[u] = e < cnt ? mm : 0u;
+                bs[u] = pk[jcs[u] * kClusterSize + jj];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint4 b = bs[u];
+                const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+                const float dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
+                const float dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
+                const bool keep = ((ms[u] >> bit) & 1u) && dr2_exact(dx, dy, dz) < pa.r2;
+                const unsigned long long mi = __ballot(keep) & sel;
+                const int pos = count + __popcll(mi & below);
+                if (keep && pos < pa.atom_cap) out[pos] = jcs[u] * kClusterSize + jj;
+                count += __popcll(mi);
+            }
+        }
+    }
+    if (lane < kClusterSize) {
+        pa.atom_n[islot] = count;
+        if (count > pa.atom_cap) atomicOr(pa.diag, 1);
+    }
+    // other lanes of the wave read these lists next: the wave's stores must
+    // have completed (one CU, so the workgroup scope suffices; the lists of
+    // different slots share no cache line)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    return __shfl(count, lane >> 4);
+}
+
+// Force-only steps of plain LJ from the per-slot inner lists: lane (i, k),
+// i = lane >> 4, takes partners k, k + 16, ... of atom i of the wave's
+// cluster (n of them; nmax the longest of the four lists), so every lane
+// holds a listed pair (the cluster form evaluates all 16 pairs of each listed
+// cluster pair, in cutoff or not). The pair terms are pair_body's, in FP32;
+// the partner index is fetched two rounds ahead and its record one round
+// ahead.
+template <int TB>
+__device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4* __restrict__ pk, const float2* sab,
+                                           int nt, float cx, float cy, float cz, float rc2, float beta,
+                                           const int* __restrict__ s2o, float* __restrict__ frc, int overwrite,
+                                           const PruneArgs& pa, const ExclArgs& ex) {
+    const int lane = threadIdx.x & 63, ii = lane >> 4, k = lane & 15;
+    const int islot = ic * kClusterSize + ii;
+    const uint4 a = pk[islot];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
+    const float qi = __uint_as_float(a.w);
+    const int ti = TB == 2 ? (int)(a.x & 3u) : unpack_type(a);
+    const float2* srow = sab + ti * nt;
+    const float mlog2e_b2 = -1.4426950408889634f * (beta * beta);
+    const int* __restrict__ lst = pa.atom_list + (size_t)islot * pa.atom_cap;
+    float fix = 0.f, fiy = 0.f, fiz = 0.f;
+    int j1 = k + 16 < n ? lst[k + 16] : islot;
+    uint4 b = pk[k < n ? lst[k] : islot];
+    for (int t = 0; t < nmax; t += 16) {
+        const int j2 = t + 32 + k < n ? lst[t + 32 + k] : islot;
+        const uint4 bn = pk[j1];
+        // with at most 4 types the y and z words carry no type bits
+        const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+        const float dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
+        const float dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
+        const float r2 = dr2_exact(dx, dy, dz);
+        const bool in = t + k < n && r2 < rc2;
+        // unused lanes (past the list's end: the atom itself) may give inf /
+        // NaN below; the final select discards them
+        const float rinv = __builtin_amdgcn_rsqf(r2);
+        const float rinv2 = rinv * rinv;
+        const float rinv6 = rinv2 * rinv2 * rinv2;
+        const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
+        const float2 AB = srow[tj];
+        const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
+        const float x = beta * (r2 * rinv);
+        const float ex_ = __builtin_amdgcn_exp2f(mlog2e_b2 * r2);
+        const float ecx = erfcx_poly(x);
+        const float qq = qi * __uint_as_float(b.w);
+        const float fc = qq * ex_ * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+        const float f = in ? flj - fc : 0.f;
+        fix += f * dx;
+        fiy += f * dy;
+        fiz += f * dz;
+        b = bn;
+        j1 = j2;
+    }
+    const int at = s2o[islot];
+    if (ex.start && at >= 0) excluded_partners(ex, pk, at, k, 16, ax, ay, az, qi, cx, cy, cz, beta, fix, fiy, fiz);
+    // i force: sum the 16 lanes of atom i
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        fix += __shfl_xor(fix, o);
+        fiy += __shfl_xor(fiy, o);
+        fiz += __shfl_xor(fiz, o);
+    }
+    if (k == 0 && at >= 0 && frc) {
+        // every atom has exactly one slot, so this is the only writer
+        if (overwrite) {
+            frc[3 * at] = fix;
+            frc[3 * at + 1] = fiy;
+            frc[3 * at + 2] = fiz;
+        } else {
+            frc[3 * at] += fix;
+            frc[3 * at + 1] += fiy;
+            frc[3 * at + 2] += fiz;
+        }
+    }
+}
+
+// Cutoff test shared with the oracle: FP32 dr2 = (dx*dx + dy*dy) + dz*dz with
+// every operation rounded (the neighbor list's expression) against rc*rc in
+// FP32, so both sides agree on every pair, including those within one ulp of
+// the cutoff.
+// SOFT: the type index is the combined (type A, type B, mask) index of a
+// soft-core handle; `ab` holds the lambda-mixed (A, B) of ordinary pair
+// types and A < 0 for soft-core pair types, whose records are sc[t_i nt + t_j]
+// (s0) and sc[nt^2 + t_i nt + t_j] (s1), read from global memory only by
+// waves that meet such a pair.
+// TB = 2: every type index is below 4 and sits in the x coordinate's low bits
+// alone (two instructions fewer per pair); TB = 6: the general 6-bit index.
+template <bool ENERGY, bool SOFT, int TB>
+__device__ __forceinline__ void pair_body(
+    int lb, const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list,
+    int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
+    float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
+    float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
+    const float4* __restrict__ sc, PruneArgs pa = PruneArgs{},
+    ExclArgs ex = ExclArgs{nullptr, nullptr, nullptr}) {
+    extern __shared__ float2 sab[];
+    for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int ic = lb * kPairWaves + w;
+    if (lb < 0 || ic >= flags[4]) return;
+    // Inner list (pa.interval > 0): on every pa.interval-th fused step since
+    // the last rebuild, and on a step whose prep kernel voted (an atom moved
+    // more than half the prune margin since the last prune), this launch
+    // prunes the full list to the pairs within pa.r2 and keeps the slot
+    // records it pruned at; the other steps read the pruned list. c = this
+    // step's number since the rebuild (the count of completed steps is
+    // advanced by the step's gather, after this kernel). Plain LJ keeps the
+    // pruned pairs per i slot (force-only steps read them, prune steps
+    // included; energy steps read the full list), soft-core LJ as cluster
+    // pairs.
+    bool use_inner = false, write_inner = false;
+    if (pa.interval > 0) {
+        const int c = flags[10] + 1;
+        const bool scheduled = c <= 1 || (c - 1) % pa.interval == 0;
+        const bool voted = pa.diag[2] == c;
+        write_inner = scheduled || voted;
+        use_inner = !write_inner;
+        if (write_inner && lane < kClusterSize) pa.ref[ic * kClusterSize + lane] = pk[ic * kClusterSize + lane];
+        if (voted && !scheduled && ic == 0 && lane == 0) atomicAdd(pa.diag + 4, 1);
+    }
+    const bool slots = !SOFT && pa.atom_list != nullptr && pa.interval > 0;
+    if (slots && !ENERGY) {
+        const int n = write_inner ? prune_slots<TB>(ic, list_n, list, list_cap, pk, cx, cy, cz, pa)
+                                  : pa.atom_n[ic * kClusterSize + (lane >> 4)];
+        const int nmax = max(max(__shfl(n, 0), __shfl(n, 16)), max(__shfl(n, 32), __shfl(n, 48)));
+        if (ic == 0 && lane == 0) atomicAdd(pa.diag + 1, 1);
+        if (nmax <= pa.atom_cap) {
+            pair_atoms<TB>(ic, n, nmax, pk, sab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite, pa, ex);
+            return;
+        }
+        write_inner = false;  // a slot overflowed: this cluster's forces from the full list
+    }
+    const bool write_atoms = slots && write_inner;      // energy steps that prune
+    const bool write_cluster = !slots && write_inner;
+    if (slots) use_inner = false;
+    const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
+    const int islot = ic * kClusterSize + ii;
+    const uint4 a = pk[islot];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
+    const float qi = __uint_as_float(a.w);
+    const int ti = TB == 2 ? (int)(a.x & 3u) : unpack_type(a);
+    const float2* srow = sab + ti * nt;
+    const float mlog2e_b2 = -1.4426950408889634f * (beta * beta);
+    int ai = 0;
+    if (ENERGY) ai = s2o[islot];
+    float fix = 0.f, fiy = 0.f, fiz = 0.f;
+    float eie = 0.f, eiv = 0.f, eid = 0.f;
+    double tlj = 0.0, tdir = 0.0, tvir = 0.0, tcnt = 0.0;
+    const int n = use_inner ? pa.in_n[ic] : list_n[ic];
+    const uint2* lst = (use_inner ? pa.in_list : list) + (size_t)ic * list_cap;
+    uint2* inner_out = write_cluster ? pa.in_list + (size_t)ic * list_cap : nullptr;
+    int inner_count = 0, atom_count = 0;
+    for (int base = 0; base < n; base += 64) {
+        // 64 list entries (16 iterations of 4 j-clusters) held one per lane.
+        const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
+        const int cnt = min(64, n - base);
+        int jc = __shfl((int)my.x, g);
+        unsigned m = (unsigned)__shfl((int)my.y, g);
+        uint4 b = pk[jc * kClusterSize + jj];
+        for (int it = 0; it < cnt; it += 4) {
+            // prefetch the next iteration's j record
+            const int src = min(it + 4 + g, 63);
+            const int jcn = __shfl((int)my.x, src);
+            const unsigned mn = (unsigned)__shfl((int)my.y, src);
+            const uint4 bn = pk[jcn * kClusterSize + jj];
+            const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+            const float dy = (float)(int)((b.y & ~3u) - ay) * cy;
+            const float dz = (float)(int)((b.z & ~3u) - az) * cz;
+            const float r2 = dr2_exact(dx, dy, dz);
+            const bool in = ((m >> bit) & 1u) && r2 < rc2;
+            if (write_inner) {
+                const bool keep = ((m >> bit) & 1u) && r2 < pa.r2;
+                const unsigned long long kb = __ballot(keep);
+                if (write_cluster) {
+                    const unsigned mk = (unsigned)(kb >> (g * 16)) & 0xffffu;
+                    const bool leader = (lane & 15) == 0 && mk != 0u;
+                    const unsigned long long lb2 = __ballot(leader);
+                    if (leader)
+                        inner_out[inner_count + __popcll(lb2 & ((1ull << lane) - 1ull))] =
+                            make_uint2((unsigned)jc, mk);
+                    inner_count += __popcll(lb2);
+                } else {
+                    // lanes ii, ii + 4, ... append to slot ii's list in lane order
+                    const unsigned long long mi = kb & (0x1111111111111111ull << ii);
+                    const int pos = atom_count + __popcll(mi & ((1ull << lane) - 1ull));
+                    if (keep && pos < pa.atom_cap)
+                        pa.atom_list[(size_t)islot * pa.atom_cap + pos] = jc * kClusterSize + jj;
+                    atom_count += __popcll(mi);
+                }
+            }
+            // masked lanes (padding, the atom itself) may give inf / NaN
+            // below; the final select discards them
+            const float r2s = r2;
+            const float rinv = __builtin_amdgcn_rsqf(r2s);
+            const float rinv2 = rinv * rinv;
+            const float rinv6 = rinv2 * rinv2 * rinv2;
+            const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
+            const float2 AB = srow[tj];
+            float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
+            const float x = beta * (r2s * rinv);
+            const float ex = __builtin_amdgcn_exp2f(mlog2e_b2 * r2s);
+            const float ecx = erfcx_poly(x);
+            const float qq = qi * __uint_as_float(b.w);
+            float fc = qq * ex * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+            bool sp = false;
+            float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0;
+            if (SOFT) {
+                sp = in && AB.x < 0.f;
+                if (sp) {
+                    s0 = sc[ti * nt + tj];
+                    s1 = sc[nt * nt + ti * nt + tj];
+                    soft_pair(r2s, qq, beta, mlog2e_b2, s0, s1, flj, fc);
+                }
+            }
+            const float f = in ? flj - fc : 0.f;
+            fix += f * dx;
+            fiy += f * dy;
+            fiz += f * dz;
+            if (ENERGY) {
+                // Energy steps evaluate the pair energies in FP64 from the exact
+                // integer displacement: the totals are nets of large cancelling
+                // terms, which FP32 terms cannot hold to 1e-6. Each unordered
+                // pair is credited once, to its lower-index atom, as the
+                // reference's half list does.
+                const int aj = s2o[jc * kClusterSize + jj];
+                if (in && ai < aj) {
+                    const double ddx = (double)(int)((b.x & ~3u) - ax) * (double)cx;
+                    const double ddy = (double)(int)((b.y & ~3u) - ay) * (double)cy;
+                    const double ddz = (double)(int)((b.z & ~3u) - az) * (double)cz;
+                    const double dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    double elj, edir, vlj;
+                    if (SOFT && sp) {
+                        soft_pair_energy(dr2, (double)qi * (double)__uint_as_float(b.w), (double)beta, s0, s1, elj,
+                                         edir, vlj);
+                    } else {
+                        const double dr = sqrt(dr2);
+                        const double ir2 = 1.0 / dr2, ir6 = ir2 * ir2 * ir2;
+                        elj = ((double)AB.x * (1.0 / 12.0) * ir6 - (double)AB.y * (1.0 / 6.0)) * ir6;
+                        edir = (double)qi * (double)__uint_as_float(b.w) * erfc((double)beta * dr) / dr;
+                        vlj = ((double)AB.x * ir6 - (double)AB.y) * ir6;
+                    }
+                    tlj += elj;
+                    tdir += edir;
+                    tvir += vlj;
+                    tcnt += 1.0;
+                    eie += (float)(elj + edir);
+                    eiv += (float)vlj;
+                    eid += (float)edir;
+                }
+            }
+            b = bn;
+            m = mn;
+            jc = jcn;
+        }
+    }
+    if (write_cluster && lane == 0) pa.in_n[ic] = inner_count;
+    if (write_atoms && lane < kClusterSize) {
+        pa.atom_n[islot] = atom_count;
+        if (atom_count > pa.atom_cap) atomicOr(pa.diag, 1);
+    }
+    if (ex.start) {
+        // this cluster's atoms: lane (ii, k) takes the k-th exclusion partner
+        // of atom ii, 16 per pass
+        const int at = s2o[islot];
+        if (at >= 0) excluded_partners(ex, pk, at, lane >> 2, 16, ax, ay, az, qi, cx, cy, cz, beta, fix, fiy, fiz);
+    }
+    // i force: sum the 16 lanes sharing ii (lanes ii, ii+4, ..., ii+60).
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) {
+        fix += __shfl_xor(fix, o);
+        fiy += __shfl_xor(fiy, o);
+        fiz += __shfl_xor(fiz, o);
+        if (ENERGY) {
+            eie += __shfl_xor(eie, o);
+            eiv += __shfl_xor(eiv, o);
+            eid += __shfl_xor(eid, o);
+        }
+    }
+    if (lane < 4) {
+        const int at = s2o[islot];
+        if (at >= 0) {
+            if (frc) {
+                // every atom has exactly one slot, so this is the only writer
+                if (overwrite) {
+                    frc[3 * at] = fix;
+                    frc[3 * at + 1] = fiy;
+                    frc[3 * at + 2] = fiz;
+                } else {
+                    frc[3 * at] += fix;
+                    frc[3 * at + 1] += fiy;
+                    frc[3 * at + 2] += fiz;
+                }
+            }
+            if (ENERGY) {
+                if (atom_e) atom_e[at] += eie;
+                if (atom_v) atom_v[at] += eiv;
+                if (atom_d) atom_d[at] = eid;
+            }
+        }
+    }
+    if (ENERGY) {
+        tlj = wave_sum(tlj);
+        tdir = wave_sum(tdir);
+        tvir = wave_sum(tvir);
+        tcnt = wave_sum(tcnt);
+        if (lane == 0) {
+            atomicAdd(acc + SPONGE_PART_LJ, tlj);
+            atomicAdd(acc + SPONGE_PART_DIRECT, tdir);
+            atomicAdd(acc + SPONGE_PART_LJ_VIRIAL, tvir);
+            atomicAdd(acc + SPONGE_PART_PAIRS_IN, tcnt);
+        }
+    }
+}
+
+
+template <int TB>
+__device__ __forceinline__ void pair_chunk(const PairLaunch& L, int first) {
+    const int k = (int)blockIdx.x - first;
+    const int per = (L.nlog + kNumXcd - 1) / kNumXcd;
+    const int j = k / kNumXcd, x = (int)blockIdx.x % kNumXcd;
+    const int l = x * per + j;
+    pair_body<false, false, TB>(l < L.nlog ? L.c0 + l : -1, L.flags, L.list_n, L.list, L.list_cap, L.pk, L.ab, L.nt,
+                                L.cx, L.cy, L.cz, L.rc2, L.beta, L.s2o, L.frc, nullptr, nullptr, nullptr, L.acc,
+                                L.overwrite, nullptr, L.pa, L.ex);
+}
+
+// Number of launch blocks that carry nlog logical pair blocks.
+__host__ __device__ inline int pair_chunk_blocks(int nlog) {
+    return nlog <= 0 ? 0 : ((nlog + kNumXcd - 1) / kNumXcd) * kNumXcd;
+}
+
+}  // namespace snb

@@ -23,6 +23,8 @@ NPARTS = 8
 OVERWRITE_FORCE = 1
 SERIAL = 2
 FROM_CRD = 4
+TWO_STREAMS = 8
+PAIR_IN_PIPELINE = 16
 # sponge_nl_status overflow bits
 OVERFLOW_COLUMN = 1
 OVERFLOW_LIST = 2
@@ -69,6 +71,7 @@ _SIGS = {
     "sponge_pme_get_energy": (_I, [_P, _P, _P, _P, _FP3, _P, _P, _P, _I, ctypes.POINTER(_D), _P]),
     "sponge_pme_update_box_length": (_I, [_P, _FP3]),
     "sponge_pme_set_events": (_I, [_P, _P, _P, _P, _P]),
+    "sponge_pme_set_tuning": (_I, [_P, _I, _I, _P, _I]),
     "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "sponge_sum_list": (_I, [_I, _P, _P, _P]),
     "sponge_lj_set_tables": (_I, [_P, _P, _P]),
@@ -479,6 +482,16 @@ class Particle_Mesh_Ewald:
             _stream(stream)), "Seperate_Direct_Atom_Energy")
         return float(out[0]), float(out[1])
 
+    def set_tuning(self, step_form=-1, conv_split=-1, pair_share=None):
+        """Force-only step form (0: pair kernel on a second stream, 1: pair
+        blocks in the pipeline's launches, 2: every kernel on its own),
+        convolution form (0: one workgroup per plane, 1: line launches) and
+        form 1's pair-block shares per launch; -1 / None keep a setting."""
+        arr = None if pair_share is None else np.ascontiguousarray(pair_share, np.float32)
+        _check(load_library().sponge_pme_set_tuning(self.h, int(step_form), int(conv_split),
+                                                    None if arr is None else arr.ctypes.data_as(_P),
+                                                    0 if arr is None else len(arr)), "sponge_pme_set_tuning")
+
     def set_events(self, spread=None, gather=None):
         """Measurement hook: (start, stop) torch.cuda.Event pairs recorded by
         the spread and gather launches themselves; None switches one off."""
@@ -588,12 +601,16 @@ class NonbondedEngine:
         crd_to_uint(self.crd, self.box, self.uint_crd, stream)
 
     def force_step(self, need_energy=0, need_virial=0, from_crd=False, overwrite=False, serial=False,
-                   stream=None):
+                   stream=None, two_streams=False, pair_in_pipeline=False):
         """Fused nonbonded force of one step into self.frc. from_crd: compute the
         uint coordinates from self.crd first; overwrite: assign self.frc
-        instead of adding to it; serial: keep the reciprocal pipeline on the
-        caller's stream instead of overlapping it with the pair kernel."""
-        flags = (OVERWRITE_FORCE if overwrite else 0) | (SERIAL if serial else 0)
+        instead of adding to it; serial: every kernel on its own, one after
+        the other; two_streams: the pair kernel on a second stream;
+        pair_in_pipeline: the pair kernel's blocks in the reciprocal
+        pipeline's launches (force-only steps); neither: the PME handle's
+        tuned form."""
+        flags = ((OVERWRITE_FORCE if overwrite else 0) | (SERIAL if serial else 0) |
+                 (TWO_STREAMS if two_streams else 0) | (PAIR_IN_PIPELINE if pair_in_pipeline else 0))
         _check(load_library().sponge_nb_force_step(
             self.nl.h, self.lj.h, self.pme.h, _ptr(self.crd) if from_crd else None, _ptr(self.uint_crd),
             _ptr(self.charge), _ptr(self.excl_start), _ptr(self.excl_list), _ptr(self.excl_numbers),

@@ -229,6 +229,25 @@ def test_overlapped_step_matches_serial(case):
         assert abs(p[k] - p_serial[k]) <= 1e-9 * max(1.0, abs(p_serial[k])), k
 
 
+def test_force_only_step_forms_agree(case):
+    """Force-only steps in the three launch forms -- pair blocks riding in
+    the reciprocal pipeline's launches, the pair kernel on a second stream
+    (default), every kernel on its own -- give the same forces, also over
+    back-to-back steps without host synchronisation."""
+    import torch
+    name, sys_, eng, ref = case
+    out = {}
+    for form, kw in (("serial", dict(serial=True)), ("two_streams", dict(two_streams=True)),
+                     ("fused", dict(pair_in_pipeline=True))):
+        for _ in range(3):
+            eng.force_step(0, 0, from_crd=True, overwrite=True, **kw)
+        torch.cuda.synchronize()
+        out[form] = eng.frc.cpu().numpy().copy()
+    for form in ("two_streams", "fused"):
+        assert rel_rms(out[form], out["serial"]) < 1e-6, form
+    assert rel_rms(out["fused"], ref["frc"]) < FORCE_TOL
+
+
 def test_get_energy_queries(case):
     name, sys_, eng, ref = case
     args = (eng.uint_crd, eng.charge, eng.nl, eng.scaler, eng.excl_start, eng.excl_list,
