@@ -69,7 +69,7 @@ def cpu_baseline(sys_, crd, vel, dt, refresh, beta, K, steps=200, warmup=20):
     }
 
 
-def time_md(mde, steps, warmup, refresh, pme_events=False, compare_forms=False):
+def time_md(mde, steps, warmup, refresh, pme_events=False):
     """Times `steps` MD steps three ways: eager launches, one hipGraph per
     kind of step (plain / with the list rebuild), and one hipGraph holding a
     whole refresh period (refresh - 1 plain steps and the rebuilding one;
@@ -153,11 +153,6 @@ def time_md(mde, steps, warmup, refresh, pme_events=False, compare_forms=False):
 
     time_graphs("")
     n_graph_steps = 2 * K + max(warmup, refresh) + 3 * refresh
-    if compare_forms:
-        mde.pair_in_pipeline = True
-        time_graphs("_pair_in_pipeline")
-        mde.pair_in_pipeline = False
-        n_graph_steps *= 2
     out["md_steps_run"] = warmup + 2 * K + n_graph_steps
     return out
 
@@ -471,7 +466,7 @@ def main():
     pairs_in0 = eng.energy_parts()["pairs_in"]
     nl_pairs = eng.nl.status()["pairs"]
 
-    t = time_md(mde, args.steps, args.warmup, args.refresh, compare_forms=True)
+    t = time_md(mde, args.steps, args.warmup, args.refresh)
     eager_ms, graph_ms, block_ms, pair_ms = t["eager"], t["graph"], t["refresh_graph"], t["pair_kernel"]
     # the fastest of the ways of issuing the same steps
     ms = min(eager_ms, graph_ms, block_ms)
@@ -497,7 +492,6 @@ def main():
         "eager_ms_per_step": round(eager_ms, 6),
         "graph_ms_per_step": round(graph_ms, 6),
         "refresh_graph_ms_per_step": round(block_ms, 6),
-        "pair_in_pipeline_refresh_graph_ms_per_step": round(t["refresh_graph_pair_in_pipeline"], 6),
         "timed_as": ("eager" if ms == eager_ms else "graph per step" if ms == graph_ms else
                      f"graph of {args.refresh} steps") + (", every kernel on its own" if args.serial else
                                                           ", pair kernel on a second stream"),

@@ -161,6 +161,26 @@ int sponge_lj_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, const unsigne
                                float* d_atom_lj_virial, float* d_atom_direct_pme_energy,
                                void* stream);
 
+/* LENNARD_JONES_INFORMATION::LJ_Force_With_PME_Direct_Force, the overload of
+ * Lennard_Jones_force.cuh:124 (uint coordinates and charges, the module's
+ * cutoff and box): the LJ + PME direct force alone, added to d_frc (what
+ * LJ_PME_Direct_Force_With_Atom_Energy_And_Virial does without energy or
+ * virial, Lennard_Jones_force.cu:1318-1328). */
+int sponge_lj_force_with_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, const unsigned int* d_uint_crd,
+                                          const float* d_charge, float* d_frc, float pme_beta, void* stream);
+
+/* LENNARD_JONES_INFORMATION::LJ_Force_With_PME_Direct_Force, the overload of
+ * Lennard_Jones_force.cuh:121 (.cu:1297-1306): the same force from the
+ * caller's UINT_VECTOR_LJ_TYPE records (uint x, y, z, LJ type, charge; 20
+ * bytes per atom, Lennard_Jones_force.cuh:26-33) with an explicit scaler
+ * (uint to Angstrom per axis) and cutoff. atom_numbers must be the module's
+ * atom count: the reference lets it be smaller to evaluate part of the
+ * system for SITS, which is not provided here (SPONGE_NB_EARG). Plain LJ
+ * handles only. */
+int sponge_lj_force_with_pme_direct_force_records(sponge_lj* lj, const sponge_nl* nl, int atom_numbers,
+                                                 const void* d_uint_crd_with_LJ, const float scaler[3], float* d_frc,
+                                                 float cutoff, float pme_beta, void* stream);
+
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under hipGraph capture, tight serving loops, harnesses that
  * forbid hipMalloc). Same arguments, results and kernels as
@@ -244,18 +264,13 @@ void sponge_pme_destroy(sponge_pme* pme);
 int sponge_pme_set_events(sponge_pme* pme, void* spread_start, void* spread_stop, void* gather_start,
                           void* gather_stop);
 
-/* Launch-shape tuning (no counterpart in the reference; -1 / NULL keep a
- * setting). step_form: how sponge_nb_force_step issues a force-only step
- * with a plain LJ handle when its flags do not say -- 0 the pair kernel on a
- * second stream (default), 1 the pair kernel's blocks appended to the
- * reciprocal pipeline's launches on one stream, 2 every kernel on its own.
- * conv_split 0 / 1 selects the one-workgroup-per-plane convolution or the
- * three line launches (0 is refused for planes beyond the LDS). pair_share:
- * form 1 deals the pair blocks over the pipeline's launches in these
- * proportions -- 3 values (spread, convolution, inverse z) for the
- * one-workgroup convolution, 5 (spread, y forward, x, y inverse, inverse z)
- * for the split one; they apply to the convolution in use after conv_split. */
-int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_split, const float* pair_share, int n_share);
+/* Launch-shape tuning (no counterpart in the reference; -1 keeps a
+ * setting). step_form: how sponge_nb_force_step issues a step when its flags
+ * do not say -- 0 the pair kernel on a second stream, concurrent with the
+ * reciprocal pipeline (default), 1 every kernel on its own. conv_split 0 / 1
+ * selects the one-workgroup-per-plane convolution or the three line launches
+ * (0 is refused for planes beyond the LDS). */
+int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_split);
 
 /* Public members the reference's callers read: beta (PME_force.cuh:62) and
  * fftx/ffty/fftz (:30-32). */
@@ -308,16 +323,13 @@ int sponge_pme_update_box_length(sponge_pme* pme, const float box_length[3]);
  * arrays are filled like the separate calls fill them.
  * The pair kernel runs on a stream of the PME handle, forked from and joined
  * back into `stream` with events (so the call stays capturable into a graph),
- * concurrently with the reciprocal spread / FFT / convolution on `stream`.
- * Force-only steps with a plain LJ handle can instead append the pair
- * kernel's blocks to the pipeline's launches on `stream`
- * (SPONGE_STEP_PAIR_IN_PIPELINE) or run every kernel on its own
- * (SPONGE_STEP_SERIAL); without these flags (or SPONGE_STEP_TWO_STREAMS) the
- * form set by sponge_pme_set_tuning applies. */
+ * concurrently with the reciprocal spread / FFT / convolution on `stream`;
+ * SPONGE_STEP_SERIAL keeps every launch on `stream`, SPONGE_STEP_TWO_STREAMS
+ * forces the concurrent form; without either the form set by
+ * sponge_pme_set_tuning applies. */
 #define SPONGE_STEP_OVERWRITE_FORCE 1
 #define SPONGE_STEP_SERIAL 2
 #define SPONGE_STEP_TWO_STREAMS 8
-#define SPONGE_STEP_PAIR_IN_PIPELINE 16
 int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const float* d_crd,
                          unsigned int* d_uint_crd, const float* d_charge,
                          const int* d_excluded_list_start, const int* d_excluded_list,

@@ -8,7 +8,6 @@
 #include <vector>
 
 #include "common.hpp"
-#include "pair_body.hpp"
 
 using namespace snb;
 
@@ -111,37 +110,16 @@ extern "C" int sponge_nb_force_step(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
                            nl->ex_el == d_excluded_list && nl->ex_en == d_excluded_numbers;
     const ExclArgs ex = fold_excl ? ExclArgs{nl->sym_start, nl->sym_list, nl->o2s} : ExclArgs{nullptr, nullptr, nullptr};
     // launch form: forced by the flags, else the PME handle's (tuned) form
-    int form = pme->step_form;
-    if (flags & SPONGE_STEP_TWO_STREAMS) form = 0;
-    if (flags & SPONGE_STEP_PAIR_IN_PIPELINE) form = 1;
-    if (flags & SPONGE_STEP_SERIAL) form = 2;
-    const bool serial = form == 2;
+    const bool serial = (flags & SPONGE_STEP_SERIAL) ||
+                        (!(flags & SPONGE_STEP_TWO_STREAMS) && pme->step_form == 1);
     int* step_count = nl->prune_interval > 0 ? nl->flags + 10 : nullptr;
-    if (form == 1 && !energy && !lj->d_sc) {
-        // One queue: the pair kernel's blocks ride in the reciprocal
-        // pipeline's launches (the pipeline's own blocks first), so the
-        // chip stays full without a second hardware queue.
-        PairLaunch pl;
-        int nlog = 0, tb = 2;
-        size_t plds = 0;
-        rc = snb_pair_launch(nl, lj, pme->beta, d_frc, overwrite ? 1 : 0, ex, &pl, &nlog, &tb, &plds);
-        if (rc) return rc;
-        rc = snb_pme_pipeline_with_pairs(pme, pl, nlog, tb, plds, 0, s);
-        if (rc) return rc;
-        if (!fold_excl) {
-            rc = snb_excluded(pme, d_uint_crd, lj->cof_dr, d_charge, d_excluded_list_start, d_excluded_list,
-                              d_excluded_numbers, d_frc, nullptr, s);
-            if (rc) return rc;
-        }
-        return snb_pme_gather(pme, d_frc, s, step_count);
-    }
-    // Fork (two-stream form, energy steps, soft-core handles): the reciprocal
-    // pipeline (spread, FFTs, convolution) stays on s while the pair and
-    // exclusion kernels run on a stream of the PME handle; both read only
-    // what step_prep wrote. Join before the gather, which adds into d_frc.
-    // The pipeline is the longer branch: eager, it starts without a
-    // cross-stream wait; captured, it is the first branch after the fork,
-    // which a graph keeps on the queue of the node before it.
+    // Fork: the reciprocal pipeline (spread, FFTs, convolution) stays on s
+    // while the pair and exclusion kernels run on a stream of the PME handle;
+    // both read only what step_prep wrote. Join before the gather, which
+    // adds into d_frc. The pipeline is the longer branch: eager, it starts
+    // without a cross-stream wait; captured, it is the first branch after the
+    // fork, which a graph keeps on the queue of the node before it (the other
+    // branch gets a second queue, whose start costs 10-40 us).
     hipStream_t ps = serial ? s : pme->side;
     if (!serial) SNB_HIP(hipEventRecord(pme->fork, s));
     rc = snb_pme_reciprocal(pme, d_uint_crd, d_charge, nullptr, energy, 1, s);

@@ -257,9 +257,8 @@ struct sponge_pme {
     void* fep;                   // frame-evaluator scratch, allocated by the first call
     int conv_split;              // 2D transforms as line launches (large planes)
     hipEvent_t ev[4];            // optional: recorded by the spread and gather launches
-    float pair_share[2][5];      // fused step: pair blocks per launch, [conv_split][stage]
-    int step_form;               // force-only fused step: 0 two streams, 1 pair blocks in the
-                                 // pipeline's launches, 2 every kernel on its own
+    int step_form;               // fused step: 0 pair kernel on a second stream, 1 every kernel
+                                 // on its own
 };
 
 // ---- batched systems ------------------------------------------------------
@@ -361,11 +360,7 @@ int snb_batch_reciprocal(sponge_batch* b, int energy, hipStream_t s);
 int snb_batch_gather(sponge_batch* b, hipStream_t s);
 int snb_batch_finalize(sponge_batch* b, hipStream_t s);
 int snb_pme_prep(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, hipStream_t s);
-struct PairLaunch;
-int snb_pme_pipeline_with_pairs(sponge_pme* pme, PairLaunch pl, int nlog, int tb, size_t pair_lds, int energy,
-                                hipStream_t s);
-int snb_pair_launch(const sponge_nl* nl, sponge_lj* lj, float beta, float* d_frc, int overwrite, ExclArgs ex,
-                    PairLaunch* out, int* nlog, int* tb, size_t* lds);
+
 int snb_pme_adjoint_grid(sponge_pme* pme, const float4* wv, const float* scale, hipStream_t s);
 int snb_pme_adj_gather(sponge_pme* pme, int mode, const float4* wv, float sign, float* out, hipStream_t s);
 int snb_batch_nl(sponge_batch* b, int forced, hipStream_t s);

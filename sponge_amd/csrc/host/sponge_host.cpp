@@ -325,6 +325,26 @@ void LENNARD_JONES_INFORMATION::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(
                       "LJ_PME_Direct_Force_With_Atom_Energy_And_Virial");
 }
 
+void LENNARD_JONES_INFORMATION::LJ_Force_With_PME_Direct_Force(const int n, const UINT_VECTOR_LJ_TYPE* uint_crd,
+                                                               const VECTOR scaler, VECTOR* frc, const sponge_nl* nl,
+                                                               const float cutoff_, const float pme_beta) {
+    if (!is_initialized) return;
+    const float sc[3] = {scaler.x, scaler.y, scaler.z};
+    sponge_host_check(sponge_lj_force_with_pme_direct_force_records(handle, nl, n, uint_crd, sc,
+                                                                    reinterpret_cast<float*>(frc), cutoff_, pme_beta,
+                                                                    nullptr),
+                      "LJ_Force_With_PME_Direct_Force");
+}
+
+void LENNARD_JONES_INFORMATION::LJ_Force_With_PME_Direct_Force(const UNSIGNED_INT_VECTOR* uint_crd,
+                                                               const float* charge, VECTOR* frc, const sponge_nl* nl,
+                                                               const float pme_beta) {
+    if (!is_initialized) return;
+    sponge_host_check(sponge_lj_force_with_pme_direct_force(handle, nl, reinterpret_cast<const unsigned*>(uint_crd),
+                                                            charge, reinterpret_cast<float*>(frc), pme_beta, nullptr),
+                      "LJ_Force_With_PME_Direct_Force");
+}
+
 void LENNARD_JONES_INFORMATION::Long_Range_Correction(int need_pressure, float* d_virial, int need_potential,
                                                       float* d_potential) {
     if (!is_initialized) return;

@@ -24,6 +24,12 @@ struct VECTOR {
 struct UNSIGNED_INT_VECTOR {
     unsigned int uint_x, uint_y, uint_z;
 };
+// Lennard_Jones_force.cuh:26-33
+struct UINT_VECTOR_LJ_TYPE {
+    unsigned int uint_x, uint_y, uint_z;
+    int LJ_type;
+    float charge;
+};
 
 // control.cu:52-164, 176-407 (mdin file: first line = task name; `key =
 // value` entries separated by newlines or commas; `prefix { ... }` blocks
@@ -97,6 +103,12 @@ struct LENNARD_JONES_INFORMATION {
                                                          const float pme_beta, const int need_atom_energy,
                                                          float* atom_energy, const int need_virial,
                                                          float* atom_lj_virial, float* atom_direct_pme_energy);
+    // Lennard_Jones_force.cuh:121-124, both overloads (force only)
+    void LJ_Force_With_PME_Direct_Force(const int atom_numbers, const UINT_VECTOR_LJ_TYPE* uint_crd,
+                                        const VECTOR scaler, VECTOR* frc, const sponge_nl* nl, const float cutoff,
+                                        const float pme_beta);
+    void LJ_Force_With_PME_Direct_Force(const UNSIGNED_INT_VECTOR* uint_crd, const float* charge, VECTOR* frc,
+                                        const sponge_nl* nl, const float pme_beta);
     void Long_Range_Correction(int need_pressure, float* d_virial, int need_potential, float* d_potential);
     float Get_Energy(const UNSIGNED_INT_VECTOR* uint_crd, const sponge_nl* nl, int is_download = 1);
     void Update_Volume(VECTOR box_length);

@@ -55,6 +55,23 @@ __global__ void gather_kernel(const int* __restrict__ flags, const int* __restri
     }
 }
 
+// Slot records from the caller's UINT_VECTOR_LJ_TYPE array (uint x, y, z, LJ
+// type, charge: 5 words per atom, Lennard_Jones_force.cuh:26-33).
+__global__ void gather_records_kernel(const int* __restrict__ flags, const int* __restrict__ s2o,
+                                      const unsigned int* __restrict__ rec, uint4* __restrict__ pk) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= flags[3]) return;
+    const int a = s2o[p];
+    if (a >= 0) {
+        const unsigned* r = rec + 5 * (size_t)a;
+        const unsigned t = r[3];
+        pk[p] = make_uint4((r[0] & ~3u) | (t & 3u), (r[1] & ~3u) | ((t >> 2) & 3u), (r[2] & ~3u) | ((t >> 4) & 3u),
+                           r[4]);
+    } else {
+        pk[p] = make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
 // Fused per-step prep of the whole force path, one thread per cluster slot:
 // optional MD_Information_Crd_To_Uint_Crd (MD_core.cu:1528-1532, written back
 // to the caller's uint array), the pair-kernel record, and the PME base,
@@ -641,51 +658,6 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     return SPONGE_NB_OK;
 }
 
-// Arguments of the force-only plain-LJ pair blocks of a fused step (records
-// already written by the step's prep kernel; inner list per the list's prune
-// settings): nlog logical blocks of kPairWaves i-clusters, tb = type-bit
-// width, lds = dynamic LDS of a pair block.
-int snb_pair_launch(const sponge_nl* nl, sponge_lj* lj, float beta, float* d_frc, int overwrite, ExclArgs ex,
-                    PairLaunch* out, int* nlog, int* tb, size_t* lds) {
-    SNB_ARG(!lj->d_sc, "the fused pair blocks take plain LJ handles");
-    int rc = scratch_ensure(&lj->ps, nl);
-    if (rc) return rc;
-    PairLaunch L{};
-    L.flags = nl->flags;
-    L.list_n = nl->list_n;
-    L.list = nl->list;
-    L.list_cap = nl->list_cap;
-    L.pk = lj->ps.pk;
-    L.ab = lj->d_ab;
-    L.nt = lj->ntypes;
-    L.cx = lj->cof_dr[0];
-    L.cy = lj->cof_dr[1];
-    L.cz = lj->cof_dr[2];
-    L.rc2 = lj->cutoff * lj->cutoff;
-    L.beta = beta;
-    L.s2o = nl->s2o;
-    L.frc = d_frc;
-    L.acc = lj->ps.acc;
-    L.overwrite = overwrite;
-    if (nl->prune_interval > 0) {
-        L.pa.interval = nl->prune_interval;
-        L.pa.r2 = nl->prune_r2;
-        L.pa.in_n = nl->list_in_n;
-        L.pa.in_list = nl->list_in;
-        L.pa.atom_n = nl->atom_in_n;
-        L.pa.atom_list = nl->atom_in;
-        L.pa.atom_cap = nl->atom_in_cap;
-        L.pa.diag = nl->flags + 11;
-        L.pa.ref = nl->prune_ref;
-    }
-    L.ex = ex;
-    *out = L;
-    *nlog = ceil_div(nl->ncl_cap, kPairWaves);
-    *tb = lj->ntypes <= 4 ? 2 : 6;
-    *lds = (size_t)lj->ntypes * lj->ntypes * sizeof(float2);
-    return SPONGE_NB_OK;
-}
-
 int snb_gather(const sponge_nl* nl, const unsigned int* d_uint_crd, const float* d_charge, const int* d_type,
                PairScratch* ps, hipStream_t s) {
     int rc = scratch_ensure(ps, nl);
@@ -735,6 +707,34 @@ extern "C" int sponge_lj_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, co
                      energy, &lj->ps, d_frc, need_atom_energy > 0 ? d_atom_energy : nullptr,
                      need_virial > 0 ? d_atom_lj_virial : nullptr, energy ? d_atom_direct_pme_energy : nullptr, s, 0,
                      0, lj->d_sc);
+}
+
+extern "C" int sponge_lj_force_with_pme_direct_force(sponge_lj* lj, const sponge_nl* nl, const unsigned int* d_uint_crd,
+                                                    const float* d_charge, float* d_frc, float pme_beta, void* stream) {
+    SNB_ARG(lj && nl && d_uint_crd && d_charge && d_frc, "lj_force_with_pme_direct_force");
+    SNB_ARG(nl->n == lj->n, "atom count of LJ and neighbor list differ");
+    return snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, lj->cutoff, lj->cof_dr, d_uint_crd, d_charge, pme_beta, 0,
+                     &lj->ps, d_frc, nullptr, nullptr, nullptr, as_stream(stream), 0, 0, lj->d_sc);
+}
+
+extern "C" int sponge_lj_force_with_pme_direct_force_records(sponge_lj* lj, const sponge_nl* nl, int atom_numbers,
+                                                            const void* d_uint_crd_with_LJ, const float scaler[3],
+                                                            float* d_frc, float cutoff, float pme_beta,
+                                                            void* stream) {
+    SNB_ARG(lj && nl && d_uint_crd_with_LJ && scaler && d_frc, "lj_force_with_pme_direct_force_records");
+    SNB_ARG(nl->n == lj->n, "atom count of LJ and neighbor list differ");
+    SNB_ARG(atom_numbers == lj->n,
+            "atom_numbers must be the module's atom count (partial ranges are used by SITS, not provided here)");
+    SNB_ARG(!lj->d_sc, "records carry plain LJ types; soft-core handles take sponge_lj_pme_direct_force");
+    SNB_ARG(cutoff > 0, "cutoff");
+    hipStream_t s = as_stream(stream);
+    int rc = scratch_ensure(&lj->ps, nl);
+    if (rc) return rc;
+    gather_records_kernel<<<ceil_div(lj->ps.cap, 256), 256, 0, s>>>(
+        nl->flags, nl->s2o, reinterpret_cast<const unsigned int*>(d_uint_crd_with_LJ), lj->ps.pk);
+    SNB_CHECK_LAUNCH();
+    return snb_pairs(nl, lj->d_ab, lj->ntypes, lj->d_type, cutoff, scaler, nullptr, nullptr, pme_beta, 0, &lj->ps,
+                     d_frc, nullptr, nullptr, nullptr, s, 1, 0, nullptr);
 }
 
 extern "C" int sponge_lj_long_range_correction(sponge_lj* lj, int need_pressure, float* d_virial, int need_potential,

@@ -1,9 +1,8 @@
 // Device side of the LJ + PME direct pair kernel (LENNARD_JONES_INFORMATION,
 // Lennard_Jones_force/Lennard_Jones_force.cu:424-562; soft-core terms of
 // LJ_soft_core.cu:254-300; the folded exclusion correction of
-// PME_force.cu:620-695), shared by the pair kernels of pair.hip and the fused
-// step kernels of pme.hip, which append pair-kernel blocks to the reciprocal
-// pipeline's launches.
+// PME_force.cu:620-695): one i-cluster per wave (pair_wave), the kernels'
+// blocks (pair_body).
 //
 // Layout: one wave64 per 4-atom i-cluster; each iteration takes four
 // j-clusters from the cluster-pair list, lane = (g:2, jj:2, ii:2) holds the
@@ -14,28 +13,6 @@
 #pragma once
 #include "common.hpp"
 #include "pair_math.hpp"
-
-// Pair-kernel blocks appended to another kernel's launch (the fused step):
-// the launch's blocks from `first` on run logical pair blocks c0 .. c0 +
-// nlog - 1, dealt so that each XCD (workgroup b runs on XCD b % 8) gets one
-// contiguous range of i-clusters, as the standalone pair kernel does.
-struct PairLaunch {
-    const int* flags;
-    const int* list_n;
-    const uint2* list;
-    int list_cap;
-    const uint4* pk;
-    const float2* ab;
-    int nt;
-    float cx, cy, cz, rc2, beta;
-    const int* s2o;
-    float* frc;
-    double* acc;
-    int overwrite;
-    PruneArgs pa;
-    ExclArgs ex;
-    int c0, nlog;
-};
 
 namespace snb {
 
@@ -259,19 +236,13 @@ __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4*
 // TB = 2: every type index is below 4 and sits in the x coordinate's low bits
 // alone (two instructions fewer per pair); TB = 6: the general 6-bit index.
 template <bool ENERGY, bool SOFT, int TB>
-__device__ __forceinline__ void pair_body(
-    int lb, const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list,
-    int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
+__device__ __forceinline__ void pair_wave(
+    int ic, const float2* sab, const int* __restrict__ flags, const int* __restrict__ list_n,
+    const uint2* __restrict__ list, int list_cap, const uint4* __restrict__ pk, int nt, float cx, float cy, float cz,
     float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
-    const float4* __restrict__ sc, PruneArgs pa = PruneArgs{},
-    ExclArgs ex = ExclArgs{nullptr, nullptr, nullptr}) {
-    extern __shared__ float2 sab[];
-    for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
-    __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int ic = lb * kPairWaves + w;
-    if (lb < 0 || ic >= flags[4]) return;
+    const float4* __restrict__ sc, const PruneArgs& pa, const ExclArgs& ex) {
+    const int lane = threadIdx.x & 63;
     // Inner list (pa.interval > 0): on every pa.interval-th fused step since
     // the last rebuild, and on a step whose prep kernel voted (an atom moved
     // more than half the prune margin since the last prune), this launch
@@ -487,20 +458,23 @@ __device__ __forceinline__ void pair_body(
 }
 
 
-template <int TB>
-__device__ __forceinline__ void pair_chunk(const PairLaunch& L, int first) {
-    const int k = (int)blockIdx.x - first;
-    const int per = (L.nlog + kNumXcd - 1) / kNumXcd;
-    const int j = k / kNumXcd, x = (int)blockIdx.x % kNumXcd;
-    const int l = x * per + j;
-    pair_body<false, false, TB>(l < L.nlog ? L.c0 + l : -1, L.flags, L.list_n, L.list, L.list_cap, L.pk, L.ab, L.nt,
-                                L.cx, L.cy, L.cz, L.rc2, L.beta, L.s2o, L.frc, nullptr, nullptr, nullptr, L.acc,
-                                L.overwrite, nullptr, L.pa, L.ex);
-}
-
-// Number of launch blocks that carry nlog logical pair blocks.
-__host__ __device__ inline int pair_chunk_blocks(int nlog) {
-    return nlog <= 0 ? 0 : ((nlog + kNumXcd - 1) / kNumXcd) * kNumXcd;
+// One block of the pair kernel: the LJ (A, B) table into LDS, then one
+// i-cluster per wave (logical block lb covers clusters kPairWaves lb ..).
+template <bool ENERGY, bool SOFT, int TB>
+__device__ __forceinline__ void pair_body(
+    int lb, const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list,
+    int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
+    float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
+    float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
+    const float4* __restrict__ sc, PruneArgs pa = PruneArgs{},
+    ExclArgs ex = ExclArgs{nullptr, nullptr, nullptr}) {
+    extern __shared__ float2 sab[];
+    for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
+    __syncthreads();
+    const int ic = lb * kPairWaves + (int)(threadIdx.x >> 6);
+    if (lb < 0 || ic >= flags[4]) return;
+    pair_wave<ENERGY, SOFT, TB>(ic, sab, flags, list_n, list, list_cap, pk, nt, cx, cy, cz, rc2, beta, s2o, frc,
+                                atom_e, atom_v, atom_d, acc, overwrite, sc, pa, ex);
 }
 
 }  // namespace snb

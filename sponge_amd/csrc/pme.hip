@@ -32,7 +32,6 @@
 
 #include "common.hpp"
 #include "pme_prep.hpp"
-#include "pair_body.hpp"
 
 using namespace snb;
 
@@ -840,84 +839,6 @@ __global__ __launch_bounds__(kZThreads) void inv_z_batch(const BatchSys* __restr
     inv_z_body(x, m.y - x * nyb, P.FQ, P.R, P.K[0], P.K[1], P.K[2], b.pz, P.d_tw[2]);
 }
 
-// ---- fused step: pair-kernel blocks appended to the pipeline's launches ----
-// A force-only step needs the pair kernel and the reciprocal pipeline, which
-// read the same step records and write different things. The pipeline's
-// launches use a fraction of the chip each (33 workgroups for the convolution
-// of a 64^3 grid); instead of a second stream (whose hardware queue starts
-// late and unevenly), each launch carries, after its own blocks, a share of
-// the pair kernel's blocks, so one queue keeps the chip full. Block b <
-// npme does the stage's work with the stage's block index; the others are
-// pair blocks (pair_chunk).
-template <int RM, int TB>
-__global__ __launch_bounds__(256) void spread_z_pair_kernel(
-    const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
-    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, int nyb,
-    FftPlan pz, const float2* __restrict__ twz, double* __restrict__ acc, int npme, PairLaunch pl) {
-    if ((int)blockIdx.x < npme) {
-        const int x = blockIdx.x / nyb;
-        spread_z_body<0, RM>(x, blockIdx.x - x * nyb, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz,
-                             twz, acc);
-    } else {
-        pair_chunk<TB>(pl, npme);
-    }
-}
-
-template <int RM, int TB>
-__global__ __launch_bounds__(256) void conv2d_pair_kernel(float2* __restrict__ FQ, const float* __restrict__ bc, int Kx,
-                                                          int Ky, int Kz, FftPlan px, FftPlan py,
-                                                          const float2* __restrict__ twx, const float2* __restrict__ twy,
-                                                          int energy, double* __restrict__ acc,
-                                                          int* __restrict__ bin_count, int npme, PairLaunch pl) {
-    if ((int)blockIdx.x < npme) conv2d_body<RM>(blockIdx.x, FQ, bc, Kx, Ky, Kz, px, py, twx, twy, energy, acc, bin_count);
-    else pair_chunk<TB>(pl, npme);
-}
-
-// conv_y blocks: (x block, kz) = (b % nbx, b / nbx)
-template <int SIGN, int RM, int TB>
-__device__ __forceinline__ void conv_y_pair(float2* __restrict__ FQ, int Kx, int Ky, const FftPlan& py,
-                                            const float2* __restrict__ twy, int nbx, int npme, const PairLaunch& pl) {
-    const int b = blockIdx.x;
-    if (b < npme) conv_y_body<SIGN, RM>(b % nbx, b / nbx, FQ, Kx, Ky, py, twy);
-    else pair_chunk<TB>(pl, npme);
-}
-template <int RM, int TB>
-__global__ __launch_bounds__(256) void conv_y_forward_pair_kernel(float2* __restrict__ FQ, int Kx, int Ky, FftPlan py,
-                                                                  const float2* __restrict__ twy, int nbx, int npme,
-                                                                  PairLaunch pl) {
-    conv_y_pair<-1, RM, TB>(FQ, Kx, Ky, py, twy, nbx, npme, pl);
-}
-template <int RM, int TB>
-__global__ __launch_bounds__(256) void conv_y_inverse_pair_kernel(float2* __restrict__ FQ, int Kx, int Ky, FftPlan py,
-                                                                  const float2* __restrict__ twy, int nbx, int npme,
-                                                                  PairLaunch pl) {
-    conv_y_pair<1, RM, TB>(FQ, Kx, Ky, py, twy, nbx, npme, pl);
-}
-
-// conv_x blocks: (y block, kz) = (b % nby, b / nby)
-template <int RM, int TB>
-__global__ __launch_bounds__(256) void conv_x_pair_kernel(float2* __restrict__ FQ, const float* __restrict__ bc, int Kx,
-                                                          int Ky, int Kz, FftPlan px, const float2* __restrict__ twx,
-                                                          int energy, double* __restrict__ acc,
-                                                          int* __restrict__ bin_count, int nby, int npme,
-                                                          PairLaunch pl) {
-    const int b = blockIdx.x;
-    if (b < npme) conv_x_body<RM>(b % nby, b / nby, FQ, bc, Kx, Ky, Kz, px, twx, energy, acc, bin_count);
-    else pair_chunk<TB>(pl, npme);
-}
-
-template <int RM, int TB>
-__global__ __launch_bounds__(256) void inv_z_pair_kernel(const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
-                                                         int Ky, int Kz, int nyb, FftPlan pz,
-                                                         const float2* __restrict__ twz, int npme, PairLaunch pl) {
-    if ((int)blockIdx.x < npme) {
-        const int x = blockIdx.x / nyb;
-        inv_z_body<RM>(x, blockIdx.x - x * nyb, FQ, R, Kx, Ky, Kz, pz, twz);
-    } else {
-        pair_chunk<TB>(pl, npme);
-    }
-}
-
 // PME_Final (PME_force.cu:485-550): 16 lanes per atom (lane = kx, ky), each
 // walking the 4 kz points; force += -q sum R dtheta (K/L).
 __device__ __forceinline__ void pme_gather_body(int blk, int n, const unsigned* __restrict__ base,
@@ -1145,15 +1066,7 @@ int raise_lds_limits(size_t bytes) {
                          (const void*)conv_y_forward_kernel<kRadixAll>, (const void*)conv_y_forward_kernel<kRadix8>,
                          (const void*)conv_y_forward_kernel<kRadix34>,  (const void*)conv_y_forward_kernel<kRadix248>,
                          (const void*)conv_y_inverse_kernel<kRadixAll>, (const void*)conv_y_inverse_kernel<kRadix8>,
-                         (const void*)conv_y_inverse_kernel<kRadix34>,  (const void*)conv_y_inverse_kernel<kRadix248>,
-#define SNB_FUSED_FNS(K)                                                                                       \
-    (const void*)K<kRadixAll, 2>, (const void*)K<kRadix8, 2>, (const void*)K<kRadix34, 2>,                       \
-        (const void*)K<kRadix248, 2>, (const void*)K<kRadixAll, 6>, (const void*)K<kRadix8, 6>,                  \
-        (const void*)K<kRadix34, 6>, (const void*)K<kRadix248, 6>
-                         SNB_FUSED_FNS(spread_z_pair_kernel), SNB_FUSED_FNS(conv2d_pair_kernel),
-                         SNB_FUSED_FNS(conv_y_forward_pair_kernel), SNB_FUSED_FNS(conv_y_inverse_pair_kernel),
-                         SNB_FUSED_FNS(conv_x_pair_kernel), SNB_FUSED_FNS(inv_z_pair_kernel)};
-#undef SNB_FUSED_FNS
+                         (const void*)conv_y_inverse_kernel<kRadix34>,  (const void*)conv_y_inverse_kernel<kRadix248>};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
             set_error("cannot raise the LDS limit of the PME kernels to %zu B", bytes);
@@ -1287,22 +1200,6 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
         }
     }
     pme->nfft = Kx * Ky * H;
-    // shares of the fused step's pair blocks per pipeline launch (spread,
-    // convolution, inverse z / spread, y forward, x, y inverse, inverse z);
-    // SPONGE_PME_PAIR_SHARE="a,b,c[,d,e]" overrides the form in use
-    {
-        const float d0[5] = {0.3f, 0.55f, 0.15f, 0.f, 0.f}, d1[5] = {0.25f, 0.2f, 0.25f, 0.2f, 0.1f};
-        for (int k = 0; k < 5; k++) {
-            pme->pair_share[0][k] = d0[k];
-            pme->pair_share[1][k] = d1[k];
-        }
-        if (const char* env = getenv("SPONGE_PME_PAIR_SHARE")) {
-            float v[5] = {0, 0, 0, 0, 0};
-            const int got = sscanf(env, "%f,%f,%f,%f,%f", v, v + 1, v + 2, v + 3, v + 4);
-            if (got == (pme->conv_split ? 5 : 3))
-                for (int k = 0; k < got; k++) pme->pair_share[pme->conv_split][k] = v[k];
-        }
-    }
     std::vector<float> hbc;
     pme_fill_bc(pme, hbc);
     size_t NG = (size_t)Kx * Ky * Kz;
@@ -1494,103 +1391,6 @@ static int convolve_launch(sponge_pme* pme, int energy, hipStream_t s) {
 
 int snb_pme_convolve(sponge_pme* pme, int energy, hipStream_t s) { return convolve_launch(pme, energy, s); }
 
-// Fused launches: radix set x type-bit width of the pair blocks.
-#define SNB_RT_DISPATCH(SET, TB, KERNEL, GRID, LDS, STREAM, ...)                                                \
-    do {                                                                                                        \
-        switch (SET) {                                                                                          \
-            case kRadix8:                                                                                       \
-                if (TB == 2) KERNEL<kRadix8, 2><<<GRID, 256, LDS, STREAM>>>(__VA_ARGS__);                       \
-                else KERNEL<kRadix8, 6><<<GRID, 256, LDS, STREAM>>>(__VA_ARGS__);                               \
-                break;                                                                                          \
-            case kRadix34:                                                                                      \
-                if (TB == 2) KERNEL<kRadix34, 2><<<GRID, 256, LDS, STREAM>>>(__VA_ARGS__);                      \
-                else KERNEL<kRadix34, 6><<<GRID, 256, LDS, STREAM>>>(__VA_ARGS__);                              \
-                break;                                                                                          \
-            case kRadix248:                                                                                     \
-                if (TB == 2) KERNEL<kRadix248, 2><<<GRID, 256, LDS, STREAM>>>(__VA_ARGS__);                     \
-                else KERNEL<kRadix248, 6><<<GRID, 256, LDS, STREAM>>>(__VA_ARGS__);                             \
-                break;                                                                                          \
-            default:                                                                                            \
-                if (TB == 2) KERNEL<kRadixAll, 2><<<GRID, 256, LDS, STREAM>>>(__VA_ARGS__);                     \
-                else KERNEL<kRadixAll, 6><<<GRID, 256, LDS, STREAM>>>(__VA_ARGS__);                             \
-                break;                                                                                          \
-        }                                                                                                       \
-    } while (0)
-
-// The reciprocal pipeline of a force-only fused step (prep done) with the
-// pair kernel's nlog logical blocks dealt over its launches in proportion to
-// pme->pair_share (spread, convolution, inverse z; with the split
-// convolution: spread, y forward, x, y inverse, inverse z). tb: type-bit
-// width of the pair blocks (2: at most 4 LJ types, 6: up to 64); pair_lds:
-// their dynamic LDS (the type table).
-int snb_pme_pipeline_with_pairs(sponge_pme* pme, PairLaunch pl, int nlog, int tb, size_t pair_lds, int energy,
-                                hipStream_t s) {
-    const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
-    FftPlan px, py, pz;
-    plan_axis(Kx, &px);
-    plan_axis(Ky, &py);
-    plan_axis(Kz, &pz);
-    const int nyb = row_blocks(Ky);
-    const int nst = pme->conv_split ? 5 : 3;
-    double wsum = 0.0;
-    for (int k = 0; k < nst; k++) wsum += pme->pair_share[pme->conv_split][k];
-    int cnt[5], used = 0;
-    for (int k = 0; k < nst; k++) {
-        cnt[k] = k == nst - 1 ? nlog - used
-                              : (int)((double)nlog * pme->pair_share[pme->conv_split][k] / wsum + 0.5);
-        cnt[k] = std::max(0, std::min(cnt[k], nlog - used));
-        used += cnt[k];
-    }
-    double* acc = energy ? pme->acc : nullptr;
-    auto chunk = [&](int k) {
-        PairLaunch c = pl;
-        c.c0 = 0;
-        for (int j = 0; j < k; j++) c.c0 += cnt[j];
-        c.nlog = cnt[k];
-        return c;
-    };
-    auto lds = [&](size_t stage) { return std::max(stage, pair_lds); };
-    int st = 0;
-    {
-        const int npme = Kx * nyb;
-        SNB_RT_DISPATCH(kernel_set(radix_mask(pz)), tb, spread_z_pair_kernel, npme + pair_chunk_blocks(cnt[st]),
-                        lds(spread_z_lds(Kz)), s, pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap,
-                        pme->FQ, Kx, Ky, Kz, nyb, pz, pme->d_tw[2], acc, npme, chunk(st));
-        SNB_CHECK_LAUNCH();
-        st++;
-    }
-    if (pme->conv_split) {
-        const int nbx = ceil_div(Kx, kLines), nby = ceil_div(Ky, kLines);
-        const int sy = kernel_set(radix_mask(py)), sx = kernel_set(radix_mask(px));
-        SNB_RT_DISPATCH(sy, tb, conv_y_forward_pair_kernel, nbx * H + pair_chunk_blocks(cnt[st]), lds(lines_lds(Ky)),
-                        s, pme->FQ, Kx, Ky, py, pme->d_tw[1], nbx, nbx * H, chunk(st));
-        SNB_CHECK_LAUNCH();
-        st++;
-        SNB_RT_DISPATCH(sx, tb, conv_x_pair_kernel, nby * H + pair_chunk_blocks(cnt[st]), lds(lines_lds(Kx)), s,
-                        pme->FQ, pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count, nby,
-                        nby * H, chunk(st));
-        SNB_CHECK_LAUNCH();
-        st++;
-        SNB_RT_DISPATCH(sy, tb, conv_y_inverse_pair_kernel, nbx * H + pair_chunk_blocks(cnt[st]), lds(lines_lds(Ky)),
-                        s, pme->FQ, Kx, Ky, py, pme->d_tw[1], nbx, nbx * H, chunk(st));
-        SNB_CHECK_LAUNCH();
-        st++;
-    } else {
-        SNB_RT_DISPATCH(kernel_set(radix_mask(px) | radix_mask(py)), tb, conv2d_pair_kernel,
-                        H + pair_chunk_blocks(cnt[st]), lds(conv2d_lds(Kx, Ky)), s, pme->FQ, pme->d_bc, Kx, Ky, Kz,
-                        px, py, pme->d_tw[0], pme->d_tw[1], energy, pme->acc, pme->bin_count, H, chunk(st));
-        SNB_CHECK_LAUNCH();
-        st++;
-    }
-    {
-        const int npme = Kx * nyb;
-        SNB_RT_DISPATCH(kernel_set(radix_mask(pz)), tb, inv_z_pair_kernel, npme + pair_chunk_blocks(cnt[st]),
-                        lds(inv_z_lds(Kz)), s, pme->FQ, pme->R, Kx, Ky, Kz, nyb, pz, pme->d_tw[2], npme, chunk(st));
-        SNB_CHECK_LAUNCH();
-    }
-    return SPONGE_NB_OK;
-}
-
 // The reciprocal pipeline; forces added to d_frc (original atom order).
 int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
                        int energy, int prep_done, hipStream_t s) {
@@ -1622,10 +1422,9 @@ int snb_pme_gather(sponge_pme* pme, float* d_frc, hipStream_t s, int* step_count
     return SPONGE_NB_OK;
 }
 
-extern "C" int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_split, const float* pair_share,
-                                     int n_share) {
+extern "C" int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_split) {
     SNB_ARG(pme, "pme_set_tuning");
-    SNB_ARG(step_form >= -1 && step_form <= 2, "step_form must be -1, 0, 1 or 2");
+    SNB_ARG(step_form >= -1 && step_form <= 1, "step_form must be -1, 0 or 1");
     if (step_form >= 0) pme->step_form = step_form;
     if (conv_split == 0 || conv_split == 1) {
         int dev = 0, max_lds = 64 * 1024;
@@ -1638,17 +1437,6 @@ extern "C" int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_sp
         const int rc = raise_lds_limits(need);
         if (rc) return rc;
         pme->conv_split = conv_split;
-    }
-    if (pair_share) {
-        const int form = pme->conv_split;
-        SNB_ARG(n_share == (form ? 5 : 3), "pair_share: 3 shares with the one-workgroup convolution, 5 when split");
-        float sum = 0.f;
-        for (int k = 0; k < n_share; k++) {
-            SNB_ARG(pair_share[k] >= 0.f, "pair_share must be non-negative");
-            sum += pair_share[k];
-        }
-        SNB_ARG(sum > 0.f, "pair_share sums to zero");
-        for (int k = 0; k < n_share; k++) pme->pair_share[form][k] = pair_share[k];
     }
     return SPONGE_NB_OK;
 }

@@ -223,11 +223,6 @@ class MDEngine:
         # True: the reciprocal pipeline runs on the caller's stream after the
         # pair kernel instead of concurrently on the PME handle's stream
         self.serial = False
-        # force the launch form of force-only steps (else the PME handle's
-        # tuned form): the pair kernel on a second stream, or its blocks
-        # riding in the reciprocal pipeline's launches
-        self.two_streams = False
-        self.pair_in_pipeline = False
         self.barostat = None     # BERENDSEN_ / ANDERSEN_BAROSTAT_INFORMATION
         self.h_pressure = 0.0
         self.ek = torch.zeros(1, dtype=torch.float32, device=device)
@@ -244,7 +239,7 @@ class MDEngine:
 
     def force(self, need_energy=0, need_virial=0, stream=None):
         self.nb.force_step(need_energy, need_virial, from_crd=True, overwrite=True, serial=self.serial,
-                           stream=stream, two_streams=self.two_streams, pair_in_pipeline=self.pair_in_pipeline)
+                           stream=stream)
 
     def iterate(self, need_pressure=0, stream=None):
         md_iteration(self.settle, self.n, self.vel, self.nb.crd, self.nb.frc, self.acc, self.inverse_mass,

@@ -24,7 +24,6 @@ OVERWRITE_FORCE = 1
 SERIAL = 2
 FROM_CRD = 4
 TWO_STREAMS = 8
-PAIR_IN_PIPELINE = 16
 # sponge_nl_status overflow bits
 OVERFLOW_COLUMN = 1
 OVERFLOW_LIST = 2
@@ -56,6 +55,8 @@ _SIGS = {
     "sponge_lj_destroy": (None, [_P]),
     "sponge_lj_pme_direct_force": (_I, [_P, _P, _P, _P, _P, _F, _I, _P, _I, _P, _P, _P]),
     "sponge_lj_pme_direct_force_noalloc": (_I, [_P, _P, _P, _P, _P, _F, _I, _P, _I, _P, _P, _P]),
+    "sponge_lj_force_with_pme_direct_force": (_I, [_P, _P, _P, _P, _P, _F, _P]),
+    "sponge_lj_force_with_pme_direct_force_records": (_I, [_P, _P, _I, _P, _FP3, _P, _F, _F, _P]),
     "sponge_lj_long_range_correction": (_I, [_P, _I, _P, _I, _P, _P]),
     "sponge_lj_get_energy": (_I, [_P, _P, _P, ctypes.POINTER(_D), _P]),
     "sponge_lj_update_volume": (_I, [_P, _FP3]),
@@ -71,7 +72,7 @@ _SIGS = {
     "sponge_pme_get_energy": (_I, [_P, _P, _P, _P, _FP3, _P, _P, _P, _I, ctypes.POINTER(_D), _P]),
     "sponge_pme_update_box_length": (_I, [_P, _FP3]),
     "sponge_pme_set_events": (_I, [_P, _P, _P, _P, _P]),
-    "sponge_pme_set_tuning": (_I, [_P, _I, _I, _P, _I]),
+    "sponge_pme_set_tuning": (_I, [_P, _I, _I]),
     "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "sponge_sum_list": (_I, [_I, _P, _P, _P]),
     "sponge_lj_set_tables": (_I, [_P, _P, _P]),
@@ -288,6 +289,27 @@ class LENNARD_JONES_INFORMATION:
             _ptr(atom_energy), int(need_virial), _ptr(atom_lj_virial), _ptr(atom_direct_pme_energy),
             _stream(stream)), "LJ_PME_Direct_Force_With_Atom_Energy_And_Virial")
 
+    def LJ_Force_With_PME_Direct_Force(self, *args, stream=None):
+        """Lennard_Jones_force.cuh:121-124, both overloads:
+        (uint_crd, charge, frc, nl, pme_beta) or
+        (atom_numbers, uint_crd_with_LJ, scaler, frc, nl, cutoff, pme_beta),
+        the records an (N, 5) int32 device tensor (uint x, y, z, LJ type,
+        charge bits)."""
+        lib = load_library()
+        if len(args) == 5:
+            uint_crd, charge, frc, nl, pme_beta = args
+            _check(lib.sponge_lj_force_with_pme_direct_force(self.h, nl.h, _ptr(uint_crd), _ptr(charge), _ptr(frc),
+                                                            float(pme_beta), _stream(stream)),
+                   "LJ_Force_With_PME_Direct_Force")
+        elif len(args) == 7:
+            atom_numbers, records, scaler, frc, nl, cutoff, pme_beta = args
+            _check(lib.sponge_lj_force_with_pme_direct_force_records(self.h, nl.h, int(atom_numbers), _ptr(records),
+                                                                    _vec3(scaler), _ptr(frc), float(cutoff),
+                                                                    float(pme_beta), _stream(stream)),
+                   "LJ_Force_With_PME_Direct_Force")
+        else:
+            raise TypeError("LJ_Force_With_PME_Direct_Force takes 5 or 7 arguments")
+
     def Long_Range_Correction(self, need_pressure, d_virial, need_potential, d_potential, stream=None):
         _check(load_library().sponge_lj_long_range_correction(
             self.h, int(need_pressure), _ptr(d_virial), int(need_potential), _ptr(d_potential),
@@ -482,15 +504,12 @@ class Particle_Mesh_Ewald:
             _stream(stream)), "Seperate_Direct_Atom_Energy")
         return float(out[0]), float(out[1])
 
-    def set_tuning(self, step_form=-1, conv_split=-1, pair_share=None):
-        """Force-only step form (0: pair kernel on a second stream, 1: pair
-        blocks in the pipeline's launches, 2: every kernel on its own),
-        convolution form (0: one workgroup per plane, 1: line launches) and
-        form 1's pair-block shares per launch; -1 / None keep a setting."""
-        arr = None if pair_share is None else np.ascontiguousarray(pair_share, np.float32)
-        _check(load_library().sponge_pme_set_tuning(self.h, int(step_form), int(conv_split),
-                                                    None if arr is None else arr.ctypes.data_as(_P),
-                                                    0 if arr is None else len(arr)), "sponge_pme_set_tuning")
+    def set_tuning(self, step_form=-1, conv_split=-1):
+        """Step form (0: pair kernel on a second stream, 1: every kernel on
+        its own) and convolution form (0: one workgroup per plane, 1: line
+        launches); -1 keeps a setting."""
+        _check(load_library().sponge_pme_set_tuning(self.h, int(step_form), int(conv_split)),
+               "sponge_pme_set_tuning")
 
     def set_events(self, spread=None, gather=None):
         """Measurement hook: (start, stop) torch.cuda.Event pairs recorded by
@@ -601,16 +620,14 @@ class NonbondedEngine:
         crd_to_uint(self.crd, self.box, self.uint_crd, stream)
 
     def force_step(self, need_energy=0, need_virial=0, from_crd=False, overwrite=False, serial=False,
-                   stream=None, two_streams=False, pair_in_pipeline=False):
+                   stream=None, two_streams=False):
         """Fused nonbonded force of one step into self.frc. from_crd: compute the
         uint coordinates from self.crd first; overwrite: assign self.frc
         instead of adding to it; serial: every kernel on its own, one after
-        the other; two_streams: the pair kernel on a second stream;
-        pair_in_pipeline: the pair kernel's blocks in the reciprocal
-        pipeline's launches (force-only steps); neither: the PME handle's
-        tuned form."""
+        the other; two_streams: the pair kernel on a second stream; neither:
+        the PME handle's tuned form."""
         flags = ((OVERWRITE_FORCE if overwrite else 0) | (SERIAL if serial else 0) |
-                 (TWO_STREAMS if two_streams else 0) | (PAIR_IN_PIPELINE if pair_in_pipeline else 0))
+                 (TWO_STREAMS if two_streams else 0))
         _check(load_library().sponge_nb_force_step(
             self.nl.h, self.lj.h, self.pme.h, _ptr(self.crd) if from_crd else None, _ptr(self.uint_crd),
             _ptr(self.charge), _ptr(self.excl_start), _ptr(self.excl_list), _ptr(self.excl_numbers),

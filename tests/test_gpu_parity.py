@@ -230,22 +230,24 @@ def test_overlapped_step_matches_serial(case):
 
 
 def test_force_only_step_forms_agree(case):
-    """Force-only steps in the three launch forms -- pair blocks riding in
-    the reciprocal pipeline's launches, the pair kernel on a second stream
-    (default), every kernel on its own -- give the same forces, also over
-    back-to-back steps without host synchronisation."""
+    """Force-only steps in both launch forms -- the pair kernel on a second
+    stream (default), every kernel on its own, chosen per call or per handle
+    -- give the same forces, also over back-to-back steps without host
+    synchronisation."""
     import torch
     name, sys_, eng, ref = case
     out = {}
-    for form, kw in (("serial", dict(serial=True)), ("two_streams", dict(two_streams=True)),
-                     ("fused", dict(pair_in_pipeline=True))):
+    for form, kw, tuned in (("serial", dict(serial=True), -1), ("two_streams", dict(two_streams=True), -1),
+                            ("tuned_serial", {}, 1), ("tuned_two_streams", {}, 0)):
+        if tuned >= 0:
+            eng.pme.set_tuning(tuned)
         for _ in range(3):
             eng.force_step(0, 0, from_crd=True, overwrite=True, **kw)
         torch.cuda.synchronize()
         out[form] = eng.frc.cpu().numpy().copy()
-    for form in ("two_streams", "fused"):
+    for form in ("two_streams", "tuned_serial", "tuned_two_streams"):
         assert rel_rms(out[form], out["serial"]) < 1e-6, form
-    assert rel_rms(out["fused"], ref["frc"]) < FORCE_TOL
+    assert rel_rms(out["two_streams"], ref["frc"]) < FORCE_TOL
 
 
 def test_get_energy_queries(case):
