@@ -8,17 +8,22 @@
 // updates the neighbor list as Main_Iteration does and writes the outputs of
 // Main_Print. Without `mode` or `mass_in_file` the coordinates stay fixed.
 //
-// Inputs (mdin keys, as MD_core.cu:257-298, 308-345, 494-551, 864-1049,
-// 1053-1080, 1212-1256 read them): coordinate_in_file, velocity_in_file,
-// mass_in_file, charge_in_file, exclude_in_file, LJ_in_file,
+// Inputs (mdin keys, as MD_core.cu:257-298, 308-345, 494-551, 736-826,
+// 860-1049, 1053-1080, 1212-1347 read them): coordinate_in_file and
+// velocity_in_file, or amber_rst7 with amber_irest; mass_in_file,
+// charge_in_file, exclude_in_file, residue_in_file, LJ_in_file,
 // constrain_in_file (count, then `i j r`; constrain.cu:27-64), mode (NVE),
 // dt (ps), cutoff, skin, step_limit, write_mdout_interval,
-// write_information_interval, write_restart_file_interval, crd, box, rst,
-// neighbor_list_*, PME_*. Outputs: mdcrd.dat (float32 xyz frames,
-// MD_core.cu:1349-1360), mdbox.txt (MD_core.cu:1402-1412) and
-// <rst>_coordinate.txt / <rst>_velocity.txt (MD_core.cu:1466-1487). Extra key
-// of this driver: frc_out_file (forces after the first step: a line `N`,
-// then N lines `fx fy fz`).
+// write_information_interval, write_restart_file_interval, mdout, crd, box,
+// rst, neighbor_list_*, PME_*. Outputs: the mdout columns (control.cu:490-573:
+// step, time, temperature, potential, LJ, PME, and density, pressure, dV/dt
+// with a barostat), mdcrd.dat (float32 xyz frames, MD_core.cu:1349-1360),
+// mdbox.txt (MD_core.cu:1402-1412) and the restart: an AMBER rst7 named
+// <rst> after an rst7 input, else <rst>_coordinate.txt / <rst>_velocity.txt
+// (MD_core.cu:1414-1487). Extra keys of this driver: frc_out_file (forces
+// after the first step: a line `N`, then N lines `fx fy fz`); check_input
+// (parse, print what was read, write the restart of the state as read when
+// `rst` is given, and stop before any device call).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -49,9 +54,11 @@ struct MdInputs {
     int atom_numbers = 0;
     VECTOR box{0, 0, 0};
     double start_time = 0.0;
+    int amber_irest = -1;  // >= 0: restarts are AMBER rst7 files (MD_core.cu:1425)
     std::vector<VECTOR> crd, vel;
     std::vector<float> charge, mass;
     std::vector<int> ex_start, ex_numbers, ex_list;
+    std::vector<int> res_start, res_end;
 };
 
 static void check_count(MdInputs& in, int n, const char* what) {
@@ -66,12 +73,123 @@ static void check_count(MdInputs& in, int n, const char* what) {
     in.atom_numbers = n;
 }
 
-static void read_inputs(CONTROLLER& c, MdInputs& in) {
-    if (!c.Command_Exist("coordinate_in_file")) {
-        std::fprintf(stderr, "Error: coordinate_in_file is required\n");
-        std::exit(1);
+// The wrap of Crd_Periodic_Map (common.cu:299-345): a coordinate below 0
+// gains one box length, one at or past the box loses one.
+static float periodic_map(float x, float L) {
+    if (x < 0.f) return x + L;
+    return x < L ? x : x - L;
+}
+
+// MD_INFORMATION::Read_Rst7 (MD_core.cu:1258-1347): a title line; `natom`
+// or `natom time` (a time means the file holds velocities); natom
+// coordinates; natom velocities if it holds them; the box lengths. The
+// velocities are zero when the file holds none or amber_irest is 0. Ten
+// passes of the periodic map bring badly wrapped coordinates into the box.
+static void read_rst7(const char* path, int irest, MdInputs& in) {
+    FILE* fp = open_or_die(path, "r");
+    char line[1024];
+    int n = 0, fields = 0;
+    if (std::fgets(line, sizeof(line), fp) && std::fgets(line, sizeof(line), fp))
+        fields = std::sscanf(line, "%d %lf", &n, &in.start_time);
+    if (fields < 1) n = 0;
+    check_count(in, n, "amber_rst7");
+    const bool has_vel = fields == 2;
+    if (!has_vel) in.start_time = 0.0;
+    std::printf("    Start reading AMBER rst7:\n        atom_numbers is %d\n        system start time is %lf\n", n,
+                in.start_time);
+    if (!has_vel || irest == 0) std::printf("        All velocity will be set to 0\n");
+    in.crd.resize(n);
+    in.vel.assign(n, VECTOR{0.f, 0.f, 0.f});
+    auto read3 = [&](VECTOR& v, const char* what) {
+        if (std::fscanf(fp, "%f %f %f", &v.x, &v.y, &v.z) != 3) {
+            std::fprintf(stderr, "Error: amber_rst7 ends early in the %s\n", what);
+            std::exit(1);
+        }
+    };
+    for (int i = 0; i < n; i++) read3(in.crd[i], "coordinates");
+    if (has_vel)
+        for (int i = 0; i < n; i++) read3(in.vel[i], "velocities");
+    if (!has_vel || irest == 0) in.vel.assign(n, VECTOR{0.f, 0.f, 0.f});
+    read3(in.box, "box");
+    std::fclose(fp);
+    std::printf("        system size is %f %f %f\n    End reading AMBER rst7\n\n", in.box.x, in.box.y, in.box.z);
+    for (int pass = 0; pass < 10; pass++)
+        for (VECTOR& v : in.crd) {
+            v.x = periodic_map(v.x, in.box.x);
+            v.y = periodic_map(v.y, in.box.y);
+            v.z = periodic_map(v.z, in.box.z);
+        }
+}
+
+// MD_INFORMATION::trajectory_output::Export_Restart_File with amber_irest
+// >= 0 (MD_core.cu:1425-1463): title, `%8d %.3lf` atoms and time, then the
+// coordinates and the velocities as `%12.7f` triples two atoms a line, then
+// the box lengths and three 90 degree angles.
+static void write_rst7(const char* path, const std::string& title, double now, const std::vector<VECTOR>& crd,
+                       const std::vector<VECTOR>& vel, VECTOR box) {
+    FILE* fp = open_or_die(path, "w");
+    std::fprintf(fp, "%s\n", title.c_str());
+    std::fprintf(fp, "%8d %.3lf\n", (int)crd.size(), now);
+    for (const std::vector<VECTOR>* list : {&crd, &vel}) {
+        int s = 0;
+        for (const VECTOR& v : *list) {
+            std::fprintf(fp, "%12.7f%12.7f%12.7f", v.x, v.y, v.z);
+            if (++s == 2) {
+                s = 0;
+                std::fprintf(fp, "\n");
+            }
+        }
+        if (s == 1) std::fprintf(fp, "\n");
     }
-    {
+    std::fprintf(fp, "%12.7f%12.7f%12.7f", box.x, box.y, box.z);
+    std::fprintf(fp, "%12.7f%12.7f%12.7f", 90.0f, 90.0f, 90.0f);
+    std::fclose(fp);
+}
+
+// MD_INFORMATION::residue_information::Initial (MD_core.cu:736-826):
+// residue_in_file holds `natom nres` and the atom count of each residue in
+// order; without it every atom is its own residue.
+static void read_residues(CONTROLLER& c, MdInputs& in) {
+    const int n = in.atom_numbers;
+    in.res_start.clear();
+    in.res_end.clear();
+    if (!c.Command_Exist("residue_in_file")) {
+        for (int i = 0; i < n; i++) {
+            in.res_start.push_back(i);
+            in.res_end.push_back(i + 1);
+        }
+        return;
+    }
+    FILE* fp = open_or_die(c.Command("residue_in_file"), "r");
+    int atoms = 0, nres = 0;
+    if (std::fscanf(fp, "%d %d", &atoms, &nres) != 2) atoms = 0;
+    check_count(in, atoms, "residue_in_file");
+    int count = 0;
+    for (int r = 0; r < nres; r++) {
+        int k = -1;
+        if (std::fscanf(fp, "%d", &k) != 1 || k < 0 || count + k > n) {
+            std::fprintf(stderr, "Error: malformed residue_in_file at residue %d\n", r);
+            std::exit(1);
+        }
+        in.res_start.push_back(count);
+        count += k;
+        in.res_end.push_back(count);
+    }
+    std::fclose(fp);
+}
+
+static void read_inputs(CONTROLLER& c, MdInputs& in) {
+    // Read_Coordinate_And_Velocity (MD_core.cu:860-920): coordinate_in_file
+    // (with velocity_in_file) first, else amber_rst7
+    if (!c.Command_Exist("coordinate_in_file")) {
+        if (!c.Command_Exist("amber_rst7")) {
+            std::fprintf(stderr, "Error: coordinate_in_file or amber_rst7 is required\n");
+            std::exit(1);
+        }
+        in.amber_irest = 1;
+        if (c.Command_Exist("amber_irest")) in.amber_irest = std::atoi(c.Command("amber_irest"));
+        read_rst7(c.Command("amber_rst7"), in.amber_irest, in);
+    } else {
         FILE* fp = open_or_die(c.Command("coordinate_in_file"), "r");
         char line[1024];
         int n = 0;
@@ -88,9 +206,9 @@ static void read_inputs(CONTROLLER& c, MdInputs& in) {
             std::exit(1);
         }
         std::fclose(fp);
+        in.vel.assign(in.atom_numbers, VECTOR{0.f, 0.f, 0.f});
     }
-    in.vel.assign(in.atom_numbers, VECTOR{0.f, 0.f, 0.f});
-    if (c.Command_Exist("velocity_in_file")) {
+    if (c.Command_Exist("coordinate_in_file") && c.Command_Exist("velocity_in_file")) {
         FILE* fp = open_or_die(c.Command("velocity_in_file"), "r");
         char line[1024];
         int n = 0;
@@ -154,6 +272,7 @@ static void read_inputs(CONTROLLER& c, MdInputs& in) {
         }
         std::fclose(fp);
     }
+    read_residues(c, in);
 }
 
 template <class T>
@@ -163,6 +282,51 @@ static T* to_device(const std::vector<T>& h) {
     hip_ok(hipMalloc((void**)&d, bytes), "hipMalloc");
     if (!h.empty()) hip_ok(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
     return d;
+}
+
+// Export_Restart_File (MD_core.cu:1414-1487): an AMBER rst7 named by `rst`
+// (default `restart`) when the input was one (amber_irest >= 0), else
+// <rst>_coordinate.txt and <rst>_velocity.txt.
+static void export_restart(CONTROLLER& c, const MdInputs& in, const std::vector<VECTOR>& crd,
+                           const std::vector<VECTOR>& vel, VECTOR box, double now) {
+    const char* rst = c.Command_Exist("rst") ? c.Command("rst") : "restart";
+    if (in.amber_irest >= 0) {
+        write_rst7(rst, c.md_name, now, crd, vel, box);
+        return;
+    }
+    const int n = (int)crd.size();
+    char path[1100];
+    std::snprintf(path, sizeof(path), "%s_coordinate.txt", rst);
+    FILE* fc = open_or_die(path, "w");
+    std::snprintf(path, sizeof(path), "%s_velocity.txt", rst);
+    FILE* fv = open_or_die(path, "w");
+    std::fprintf(fc, "%d %.3lf\n", n, now);
+    std::fprintf(fv, "%d %.3lf\n", n, now);
+    for (int i = 0; i < n; i++) {
+        std::fprintf(fc, "%12.7f %12.7f %12.7f\n", crd[i].x, crd[i].y, crd[i].z);
+        std::fprintf(fv, "%12.7f %12.7f %12.7f\n", vel[i].x, vel[i].y, vel[i].z);
+    }
+    std::fprintf(fc, "%12.7f %12.7f %12.7f %12.7f %12.7f %12.7f", box.x, box.y, box.z, 90.0f, 90.0f, 90.0f);
+    std::fclose(fc);
+    std::fclose(fv);
+}
+
+// Get_Residue_Temperature (MD_core.cu:1721-1742, kernel :216-239): the
+// kinetic energy of each residue's centre-of-mass motion, T = 2 Ek / (3 kB
+// nres) with kB = 0.00198716 kcal/mol/K.
+static float residue_temperature(const MdInputs& in, const std::vector<VECTOR>& vel) {
+    double ek = 0.0;
+    for (size_t r = 0; r < in.res_start.size(); r++) {
+        float px = 0.f, py = 0.f, pz = 0.f, m = 0.f;
+        for (int i = in.res_start[r]; i < in.res_end[r]; i++) {
+            px += in.mass[i] * vel[i].x;
+            py += in.mass[i] * vel[i].y;
+            pz += in.mass[i] * vel[i].z;
+            m += in.mass[i];
+        }
+        ek += 0.5f * (px * px + py * py + pz * pz) / m;
+    }
+    return (float)(ek * 2.0 / 0.00198716f / (double)in.res_start.size() / 3.0);
 }
 
 int main(int argc, char** argv) {
@@ -224,6 +388,10 @@ int main(int argc, char** argv) {
         for (float q : in.charge) qsum += q;
         controller.printf("atoms %d box %g %g %g excluded %zu charge_sum %.6f cutoff %g skin %g steps %d\n", n,
                           in.box.x, in.box.y, in.box.z, in.ex_list.size(), qsum, cutoff, skin, step_limit);
+        controller.printf("residues %zu start_time %.3f amber_irest %d\n", in.res_start.size(), in.start_time,
+                          in.amber_irest);
+        // with `rst`, the restart file of the state as read (time start_time)
+        if (controller.Command_Exist("rst")) export_restart(controller, in, in.crd, in.vel, in.box, in.start_time);
         return 0;
     }
 
@@ -252,6 +420,13 @@ int main(int argc, char** argv) {
     VECTOR scaler{box.x / 4294967296.0f, box.y / 4294967296.0f, box.z / 4294967296.0f};
     float box3[3] = {box.x, box.y, box.z};
 
+    // MD_INFORMATION's mdout columns (MD_core.cu:273-276), ahead of the modules'
+    controller.Step_Print_Initial("step", "%d");
+    controller.Step_Print_Initial("time", "%.3lf");
+    controller.Step_Print_Initial("temperature", "%.2f");
+    controller.Step_Print_Initial("potential", "%.2f");
+    double total_mass = 0.0;
+    for (float m : in.mass) total_mass += m;
     NEIGHBOR_LIST neighbor_list;
     LENNARD_JONES_INFORMATION lj;
     Particle_Mesh_Ewald pme;
@@ -312,6 +487,8 @@ int main(int argc, char** argv) {
         h.resize(n);
         hip_ok(hipMemcpy(h.data(), d, n * sizeof(VECTOR), hipMemcpyDeviceToHost), "hipMemcpy");
     };
+    controller.Print_First_Line_To_Mdout();  // main.cu:132
+    std::vector<float> h_atom_energy(n);
 
     auto t0 = std::chrono::steady_clock::now();
     for (int step = 1; step <= step_limit; step++) {
@@ -344,9 +521,20 @@ int main(int argc, char** argv) {
             for (int i = 0; i < n; i++) std::fprintf(fp, "%.9g %.9g %.9g\n", h[i].x, h[i].y, h[i].z);
             std::fclose(fp);
         }
+        float e_pme = 0.f, e_lj = 0.f, potential = 0.f;
         if (need_potential) {
-            float e_pme = pme.Get_Energy(uint_crd, d_charge, neighbor_list.d_nl, scaler, d_es, d_el, d_en);
-            float e_lj = lj.Get_Energy(uint_crd, neighbor_list.d_nl);
+            // Get_Potential (MD_core.cu:1800-1812): the atom energies (LJ +
+            // direct Coulomb) plus the scalar terms (long-range correction,
+            // PME correction + self + reciprocal)
+            float scalar_potential = 0.f;
+            hip_ok(hipMemcpy(h_atom_energy.data(), d_atom_energy, n * sizeof(float), hipMemcpyDeviceToHost),
+                   "hipMemcpy");
+            hip_ok(hipMemcpy(&scalar_potential, d_scalars + 1, sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
+            double sum = scalar_potential;
+            for (float e : h_atom_energy) sum += e;
+            potential = (float)sum;
+            e_pme = pme.Get_Energy(uint_crd, d_charge, neighbor_list.d_nl, scaler, d_es, d_el, d_en);
+            e_lj = lj.Get_Energy(uint_crd, neighbor_list.d_nl);
             controller.printf("step %d LJ %.6f PME %.6f reciprocal %.6f self %.6f direct %.6f correction %.6f\n",
                               step, e_lj, e_pme, pme.reciprocal_ene, pme.self_ene, pme.direct_ene,
                               pme.correction_ene);
@@ -398,28 +586,27 @@ int main(int argc, char** argv) {
             std::fwrite(&h[0].x, sizeof(VECTOR), n, crd_traj);
             std::fprintf(box_traj, "%f %f %f %.0f %.0f %.0f\n", box.x, box.y, box.z, 90.0f, 90.0f, 90.0f);
         }
+        const double now = in.start_time + dt_ps * step;
+        if (need_potential) {  // Main_Print's mdout row (main.cu:382-409)
+            std::vector<VECTOR> hv;
+            download(vel, hv);
+            controller.Step_Print("step", step);
+            controller.Step_Print("time", now);
+            controller.Step_Print("temperature", residue_temperature(in, hv));
+            controller.Step_Print("potential", potential);
+            controller.Step_Print("PME", e_pme);
+            controller.Step_Print("LJ", e_lj);
+            const float V = box.x * box.y * box.z;  // Get_Density (MD_core.cu:1510-1514)
+            controller.Step_Print("density", (float)total_mass * 1e24f / 6.023e23f / V);
+            controller.Step_Print("pressure", h_pressure * 6.946827162543585e4f);
+            controller.Step_Print("dV/dt", ad_baro.dV_dt);
+            controller.Print_To_Screen_And_Mdout();
+        }
         if (md && restart_interval > 0 && step % restart_interval == 0) {
-            // Export_Restart_File without an AMBER restart (MD_core.cu:1466-1487)
             std::vector<VECTOR> hc, hv;
             download(crd, hc);
             download(vel, hv);
-            const char* rst = controller.Command_Exist("rst") ? controller.Command("rst") : "restart";
-            const double now = in.start_time + dt_ps * step;
-            char path[1100];
-            std::snprintf(path, sizeof(path), "%s_coordinate.txt", rst);
-            FILE* fc = open_or_die(path, "w");
-            std::snprintf(path, sizeof(path), "%s_velocity.txt", rst);
-            FILE* fv = open_or_die(path, "w");
-            std::fprintf(fc, "%d %.3lf\n", n, now);
-            std::fprintf(fv, "%d %.3lf\n", n, now);
-            for (int i = 0; i < n; i++) {
-                std::fprintf(fc, "%12.7f %12.7f %12.7f\n", hc[i].x, hc[i].y, hc[i].z);
-                std::fprintf(fv, "%12.7f %12.7f %12.7f\n", hv[i].x, hv[i].y, hv[i].z);
-            }
-            std::fprintf(fc, "%12.7f %12.7f %12.7f %12.7f %12.7f %12.7f", box.x, box.y, box.z, 90.0f, 90.0f,
-                         90.0f);
-            std::fclose(fc);
-            std::fclose(fv);
+            export_restart(controller, in, hc, hv, box, now);
         }
     }
     hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -432,6 +619,7 @@ int main(int argc, char** argv) {
     controller.printf("neighbor pairs %ld overflow %d rebuilds %d\n", pairs, overflow, rebuilds);
     controller.printf("Core Run Wall Time: %f second(s), %f steps/s\n", secs, step_limit / secs);
 
+    controller.Clear();
     settle.Clear();
     pme.Clear();
     lj.Clear();

@@ -165,6 +165,64 @@ void CONTROLLER::printf(const char* fmt, ...) const {
     va_end(ap);
 }
 
+void CONTROLLER::Step_Print_Initial(const char* head, const char* format) {
+    if (outputs_format.count(head)) return;
+    outputs_format[head] = format;
+    outputs_content[head] = "****";
+    outputs_key.push_back(head);
+}
+template <class T>
+static void step_print(CONTROLLER& c, const char* head, T value) {
+    auto it = c.outputs_format.find(head);
+    if (it == c.outputs_format.end()) return;  // a column nobody registered is ignored
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), it->second.c_str(), value);
+    c.outputs_content[head] = buf;
+}
+void CONTROLLER::Step_Print(const char* head, float value) { step_print(*this, head, (double)value); }
+void CONTROLLER::Step_Print(const char* head, double value) { step_print(*this, head, value); }
+void CONTROLLER::Step_Print(const char* head, int value) { step_print(*this, head, value); }
+void CONTROLLER::Print_First_Line_To_Mdout(FILE* fp) {
+    if (!fp) {
+        if (!mdout) {
+            const char* path = Command_Exist("mdout") ? Command("mdout") : "mdout.txt";
+            mdout = std::fopen(path, "w");
+            if (!mdout) {
+                std::fprintf(stderr, "Error: cannot open %s\n", path);
+                std::exit(1);
+            }
+        }
+        fp = mdout;
+    }
+    for (const std::string& k : outputs_key) std::fprintf(fp, "%12s ", k.c_str());
+    std::fprintf(fp, "\n");
+    std::printf("---------------------------------------------------------------------------------------\n");
+}
+void CONTROLLER::Print_To_Screen_And_Mdout(FILE* fp) {
+    if (!fp) fp = mdout;
+    int n = 0;
+    for (const std::string& k : outputs_key) {
+        std::string& v = outputs_content[k];
+        std::printf("%12s = %12s, ", k.c_str(), v.c_str());
+        if (fp) std::fprintf(fp, "%12s ", v.c_str());
+        v = "****";
+        if (++n % 3 == 0) std::printf("\n");
+    }
+    if (n % 3 != 0) std::printf("\n");
+    std::printf("---------------------------------------------------------------------------------------\n");
+    if (fp) {
+        std::fprintf(fp, "\n");
+        std::fflush(fp);
+    }
+}
+void CONTROLLER::Clear() {
+    if (mdout) std::fclose(mdout);
+    mdout = nullptr;
+    outputs_key.clear();
+    outputs_format.clear();
+    outputs_content.clear();
+}
+
 // ------------------------------------------------------------- NEIGHBOR_LIST
 void NEIGHBOR_LIST::Initial(CONTROLLER* controller, int md_atom_numbers, VECTOR box, float cut, float sk,
                             const char* name) {
@@ -310,6 +368,10 @@ void LENNARD_JONES_INFORMATION::Initial(CONTROLLER* controller, float cut, VECTO
         }
     long_range_factor = (float)(s * (-2.0 / 3.0) * M_PI / ((double)cut * cut * cut) / 6.0);
     is_initialized = 1;
+    if (!is_controller_printf_initialized) {  // Lennard_Jones_force.cu:1132-1136
+        controller->Step_Print_Initial(module_name, "%.2f");
+        is_controller_printf_initialized = 1;
+    }
 }
 
 void LENNARD_JONES_INFORMATION::LJ_PME_Direct_Force_With_Atom_Energy_And_Virial(
@@ -568,6 +630,10 @@ void Particle_Mesh_Ewald::Initial(CONTROLLER* controller, int n, VECTOR box, flo
     hip_check(hipMemset(d_correction_atom_energy, 0, n * sizeof(float)), "hipMemset");
     controller->printf("    PME beta %f, fft %d %d %d\n", beta, fftx, ffty, fftz);
     is_initialized = 1;
+    if (!is_controller_printf_initialized) {  // PME_force.cu:298-303
+        controller->Step_Print_Initial(module_name, "%.2f");
+        is_controller_printf_initialized = 1;
+    }
 }
 
 void Particle_Mesh_Ewald::PME_Excluded_Force_With_Atom_Energy(const UNSIGNED_INT_VECTOR* uint_crd, const VECTOR sacler,
@@ -756,6 +822,11 @@ void BERENDSEN_BAROSTAT_INFORMATION::Initial(CONTROLLER* controller, float targe
     controller->printf("    Berendsen barostat: target %.2f bar, tau %f ps, compressibility %e, interval %d\n",
                        target_pressure * kPresConvertion, taup, compressibility / kPresConvertion, update_interval);
     is_initialized = 1;
+    if (!is_controller_printf_initialized) {  // Berendsen_barostat.cu:70-76
+        controller->Step_Print_Initial("density", "%.4f");
+        controller->Step_Print_Initial("pressure", "%.2f");
+        is_controller_printf_initialized = 1;
+    }
 }
 
 void BERENDSEN_BAROSTAT_INFORMATION::Ask_For_Calculate_Pressure(int steps, int* need_pressure) {
@@ -802,6 +873,12 @@ void ANDERSEN_BAROSTAT_INFORMATION::Initial(CONTROLLER* controller, float target
     controller->printf("    Andersen barostat: target %.2f bar, tau %f ps, compressibility %e\n",
                        target_pressure * kPresConvertion, taup / 20.455f, compressibility / kPresConvertion);
     is_initialized = 1;
+    if (!is_controller_printf_initialized) {  // andersen_barostat.cu:46-53
+        controller->Step_Print_Initial("density", "%.4f");
+        controller->Step_Print_Initial("pressure", "%.2f");
+        controller->Step_Print_Initial("dV/dt", "%.2f");
+        is_controller_printf_initialized = 1;
+    }
 }
 
 float ANDERSEN_BAROSTAT_INFORMATION::Scale_Factor(float p_now, float target_pressure, float volume, float dt) {

@@ -54,6 +54,24 @@ class CONTROLLER {
     const char* Command(const char* prefix, const char* key) const;
     bool Command_Choice(const char* key, const char* value) const;
     void printf(const char* fmt, ...) const;
+
+    // mdout columns (control.cu:490-573): a module registers a column with
+    // its printf format once (Step_Print_Initial); Step_Print fills it on a
+    // print step; Print_To_Screen_And_Mdout writes the row (`%12s ` per
+    // column to the mdout file, `%12s = %12s, ` three a line to the screen)
+    // and resets every column to `****`. The mdout file (key `mdout`,
+    // default mdout.txt) opens with the header line, so a run that only
+    // parses its input writes no file.
+    std::vector<std::string> outputs_key;
+    std::map<std::string, std::string> outputs_format, outputs_content;
+    FILE* mdout = nullptr;
+    void Step_Print_Initial(const char* head, const char* format);
+    void Step_Print(const char* head, float value);
+    void Step_Print(const char* head, double value);
+    void Step_Print(const char* head, int value);
+    void Print_First_Line_To_Mdout(FILE* fp = nullptr);
+    void Print_To_Screen_And_Mdout(FILE* fp = nullptr);
+    void Clear();
 };
 
 struct NEIGHBOR_LIST {
@@ -94,6 +112,7 @@ struct LENNARD_JONES_INFORMATION {
     float long_range_factor = 0.0f;
     float h_LJ_energy_sum = 0.0f;
     sponge_lj* handle = nullptr;
+    int is_controller_printf_initialized = 0;  // its mdout columns are registered
 
     // Lennard_Jones_force.cu:1067-1139: LJ_in_file (`N ntypes`, A x T(T+1)/2,
     // B x T(T+1)/2, N types).
@@ -129,6 +148,7 @@ struct Particle_Mesh_Ewald {
     float* d_correction_atom_energy = nullptr;  // device
     float reciprocal_ene = 0, self_ene = 0, direct_ene = 0, correction_ene = 0, ee_ene = 0;
     sponge_pme* handle = nullptr;
+    int is_controller_printf_initialized = 0;  // its mdout columns are registered
 
     // PME_force.cu:108-306; mdin keys <module>_Direct_Tolerance, _fftx/y/z.
     void Initial(CONTROLLER* controller, int atom_numbers, VECTOR boxlength, float cutoff,
@@ -304,6 +324,7 @@ struct BERENDSEN_BAROSTAT_INFORMATION {
     float V0 = 0.f, newV = 0.f;
     float target_temperature = 300.f;
     unsigned long long rng_state = 0;
+    int is_controller_printf_initialized = 0;  // its mdout columns are registered
     void Initial(CONTROLLER* controller, float target_pressure, VECTOR box_length, const char* module_name = nullptr);
     void Ask_For_Calculate_Pressure(int steps, int* need_pressure);
     float Scale_Factor(float p_now, float target_pressure, float* vel_factor);
@@ -316,6 +337,7 @@ struct ANDERSEN_BAROSTAT_INFORMATION {
     char module_name[256];
     int is_initialized = 0;
     float V0 = 0.f, new_V = 0.f, h_mass_inverse = 0.f, dV_dt = 0.f;
+    int is_controller_printf_initialized = 0;  // its mdout columns are registered
     void Initial(CONTROLLER* controller, float target_pressure, VECTOR box_length, const char* module_name = nullptr);
     float Scale_Factor(float p_now, float target_pressure, float volume, float dt);
 };

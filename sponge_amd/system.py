@@ -452,10 +452,46 @@ def fft_size(length: float) -> int:
 # SPONGE input files
 # ---------------------------------------------------------------------------
 
+def write_rst7(path: str, title: str, crd, box, vel=None, time: float | None = None) -> None:
+    """Writes an AMBER rst7 restart as SPONGE writes one (MD_core.cu:1425-1463):
+    the title, `%8d %.3f` atoms and time (only `%8d` when there are no
+    velocities), coordinates then velocities as `%12.7f` triples two atoms a
+    line, then the box lengths and three 90 degree angles."""
+    crd = np.asarray(crd, np.float32).reshape(-1, 3)
+    lists = [crd] if vel is None else [crd, np.asarray(vel, np.float32).reshape(-1, 3)]
+    with open(path, "w") as f:
+        f.write(f"{title}\n")
+        f.write(f"{len(crd):8d}" + ("" if vel is None else f" {0.0 if time is None else time:.3f}") + "\n")
+        for arr in lists:
+            for i in range(0, len(arr), 2):
+                f.write("".join(f"{float(x):12.7f}" for x in arr[i:i + 2].reshape(-1)) + "\n")
+        b = np.asarray(box, np.float32)
+        f.write("".join(f"{float(x):12.7f}" for x in (*b, 90.0, 90.0, 90.0)) + "\n")
+
+
+def read_rst7(path: str):
+    """Reads an AMBER rst7 (fixed 12-column fields, six a line). Returns
+    (title, n, time or None, crd, vel or None, box)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    title = lines[0]
+    head = lines[1].split()
+    n = int(head[0])
+    time = float(head[1]) if len(head) > 1 else None
+    vals = []
+    for line in lines[2:]:
+        vals.extend(float(line[i:i + 12]) for i in range(0, len(line.rstrip()), 12))
+    vals = np.asarray(vals, np.float64)
+    crd = vals[:3 * n].reshape(n, 3)
+    vel = vals[3 * n:6 * n].reshape(n, 3) if time is not None else None
+    rest = vals[6 * n:] if time is not None else vals[3 * n:]
+    return title, n, time, crd, vel, rest[:3]
+
+
 def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
                         write_mdout_interval: int = 1, refresh_interval: int = 0,
                         extra: dict | None = None, velocities: np.ndarray | None = None,
-                        dt_ps: float | None = None) -> str:
+                        dt_ps: float | None = None, rst7: bool = False) -> str:
     """Writes `sys_` as SPONGE text inputs plus an mdin and returns the mdin path.
 
     Formats are the ones MD_core.cu reads: coordinate_in_file (`N start_time`,
@@ -469,6 +505,9 @@ def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
     (`N`, then N lines; MD_core.cu:864-890), mass_in_file (`N`, then N masses;
     :922-949), constrain_in_file (the rigid waters' constraint pairs: a count,
     then `i j r`; constrain/constrain.cu:27-64), mode NVE and dt (ps).
+
+    With `rst7` the coordinates (and the velocities, when given) go into one
+    AMBER rst7 file named by amber_rst7 (MD_core.cu:1258-1347) instead.
     """
     import os
     os.makedirs(directory, exist_ok=True)
@@ -509,8 +548,14 @@ def write_sponge_inputs(sys_: System, directory: str, step_limit: int = 1,
         md_keys = {"velocity_in_file": p("velocity.txt"), "mass_in_file": p("mass.txt"),
                    "constrain_in_file": p("constrain.txt"), "mode": "NVE",
                    "dt": 0.001 if dt_ps is None else dt_ps}
+    crd_keys = {"coordinate_in_file": p("coordinate.txt")}
+    if rst7:
+        write_rst7(p("system.rst7"), sys_.name, sys_.crd, sys_.box, velocities,
+                   None if velocities is None else 0.0)
+        crd_keys = {"amber_rst7": p("system.rst7")}
+        md_keys.pop("velocity_in_file", None)
     keys = {
-        "coordinate_in_file": p("coordinate.txt"),
+        **crd_keys,
         "charge_in_file": p("charge.txt"),
         "exclude_in_file": p("exclude.txt"),
         "LJ_in_file": p("LJ.txt"),

@@ -138,6 +138,107 @@ def test_malformed_exclusions_are_rejected(tmp_path):
     assert r.returncode != 0 and "malformed exclude_in_file" in r.stderr
 
 
+def _rst7_case(tmp_path, irest=None, with_vel=True, extra=None):
+    """A 300-atom system written as an rst7 with a few atoms placed outside
+    the box, read by check_input and exported again as an rst7."""
+    sys_ = system.small_random(300, (12.0, 13.0, 14.0), seed=4, cutoff=4.5, skin=1.0, fft=(12, 14, 15))
+    L = sys_.box.astype(np.float64)
+    crd = sys_.crd.astype(np.float64).copy()
+    crd[0, 0] = -1.5                       # one box length below: +L
+    crd[1, 1] = L[1] + 0.75                # past the box: -L
+    crd[2, 2] = -2.0 * L[2] + 0.5          # two box lengths below: two passes
+    crd[3, 0] = L[0]                       # exactly L maps to 0
+    vel = np.random.default_rng(7).normal(0.0, 0.01, (sys_.n, 3)) if with_vel else None
+    sys_.crd = crd.astype(np.float32)
+    keys = {"check_input": 1, "rst": str(tmp_path / "out.rst7")}
+    if irest is not None:
+        keys["amber_irest"] = irest
+    keys.update(extra or {})
+    mdin = system.write_sponge_inputs(sys_, str(tmp_path), extra=keys, velocities=vel, rst7=True)
+    if vel is not None:                    # write_sponge_inputs writes the time as 0.0: give it one
+        system.write_rst7(str(tmp_path / "system.rst7"), sys_.name, sys_.crd, sys_.box, vel, 12.5)
+    r = run(["-mdin", mdin])
+    return sys_, vel, r
+
+
+@needs_driver
+def test_rst7_read_and_export_round_trip(tmp_path):
+    """amber_rst7 (MD_core.cu:1258-1347) read with its velocities and time,
+    every coordinate wrapped into [0, L) by the periodic map, then written
+    back in the rst7 layout of Export_Restart_File (MD_core.cu:1425-1463):
+    title, `%8d %.3lf`, two atoms a line of %12.7f, box and angles."""
+    sys_, vel, r = _rst7_case(tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "residues 300 start_time 12.500 amber_irest 1" in r.stdout
+    title, n, time, crd, v, box = system.read_rst7(str(tmp_path / "out.rst7"))
+    assert title == sys_.name and n == sys_.n and time == 12.5
+    L = sys_.box.astype(np.float64)
+    _, _, _, crd_in, _, _ = system.read_rst7(str(tmp_path / "system.rst7"))
+    want = crd_in.copy()
+    for _ in range(10):                     # Crd_Periodic_Map, ten passes
+        want = np.where(want < 0, want + L, np.where(want < L, want, want - L))
+    assert (crd >= 0).all() and (crd < L).all()
+    np.testing.assert_allclose(crd, want, atol=2e-6)
+    assert abs(crd[0, 0] - (L[0] - 1.5)) < 1e-5 and abs(crd[1, 1] - 0.75) < 1e-5
+    assert abs(crd[2, 2] - 0.5) < 1e-5 and crd[3, 0] == 0.0
+    np.testing.assert_allclose(v, vel, atol=6e-8)
+    np.testing.assert_allclose(box, L, atol=1e-6)
+    lines = (tmp_path / "out.rst7").read_text().splitlines()
+    assert lines[1] == f"{sys_.n:8d} 12.500"
+    assert len(lines) == 2 + 2 * (sys_.n // 2) + 1
+    assert len(lines[2]) == 72 and lines[-1].endswith("  90.0000000  90.0000000  90.0000000")
+
+
+@needs_driver
+@pytest.mark.parametrize("irest,with_vel", [(0, True), (1, False)])
+def test_rst7_zero_velocities(tmp_path, irest, with_vel):
+    """amber_irest 0, or an rst7 with no time (so no velocities): every
+    velocity is zero and, without velocities, the start time too."""
+    sys_, _, r = _rst7_case(tmp_path, irest=irest, with_vel=with_vel)
+    assert r.returncode == 0, r.stderr
+    assert "All velocity will be set to 0" in r.stdout
+    _, n, time, _, v, _ = system.read_rst7(str(tmp_path / "out.rst7"))
+    assert (v == 0).all() and v.shape == (sys_.n, 3)
+    assert time == (12.5 if with_vel else 0.0)
+
+
+@needs_driver
+def test_rst7_negative_irest_exports_text(tmp_path):
+    """amber_irest -1 keeps the velocities but writes the text restart pair
+    (MD_core.cu:1464-1487)."""
+    sys_, vel, r = _rst7_case(tmp_path, irest=-1)
+    assert r.returncode == 0, r.stderr
+    rst = str(tmp_path / "out.rst7")
+    head = open(rst + "_coordinate.txt").readline().split()
+    assert head == [str(sys_.n), "12.500"]
+    np.testing.assert_allclose(np.loadtxt(rst + "_velocity.txt", skiprows=1), vel, atol=6e-8)
+
+
+@needs_driver
+def test_rst7_truncated_is_an_error(tmp_path):
+    sys_, _, r = _rst7_case(tmp_path)
+    lines = (tmp_path / "system.rst7").read_text().splitlines()
+    (tmp_path / "system.rst7").write_text("\n".join(lines[:100]) + "\n")
+    r = run(["-mdin", str(tmp_path / "mdin.txt")])
+    assert r.returncode != 0 and "amber_rst7 ends early" in r.stderr
+
+
+@needs_driver
+def test_residue_file(tmp_path):
+    """residue_in_file (MD_core.cu:784-826): `natom nres`, then each
+    residue's atom count; counts past natom are refused."""
+    sys_ = system.config(0)
+    (tmp_path / "res.txt").write_text(f"{sys_.n} {sys_.waters}\n" + "3\n" * sys_.waters)
+    mdin = system.write_sponge_inputs(sys_, str(tmp_path), extra={"check_input": 1,
+                                                                  "residue_in_file": str(tmp_path / "res.txt")})
+    r = run(["-mdin", mdin])
+    assert r.returncode == 0, r.stderr
+    assert f"residues {sys_.waters} " in r.stdout
+    (tmp_path / "res.txt").write_text(f"{sys_.n} 2\n600\n100\n")
+    r = run(["-mdin", mdin])
+    assert r.returncode != 0 and "malformed residue_in_file" in r.stderr
+
+
 DRIVER_CASES = {
     "water648": lambda: system.config(0),
     "random_noncubic": lambda: system.small_random(900, (21.0, 24.5, 27.3), seed=3, cutoff=7.0,
@@ -153,7 +254,8 @@ def test_driver_step_loop_matches_oracle(tmp_path, name):
     sys_ = DRIVER_CASES[name]()
     print(f"[driver] {name} n={sys_.n}", flush=True)
     mdin = system.write_sponge_inputs(sys_, str(tmp_path), step_limit=3, write_mdout_interval=1,
-                                      extra={"frc_out_file": str(tmp_path / "frc.txt")})
+                                      extra={"frc_out_file": str(tmp_path / "frc.txt"),
+                                             "mdout": str(tmp_path / "mdout.txt")})
     r = run(["-mdin", mdin], timeout=100)
     sys.stdout.write(r.stdout[-2000:])
     assert r.returncode == 0, r.stderr
@@ -174,26 +276,54 @@ def test_driver_step_loop_matches_oracle(tmp_path, name):
         assert abs(e_rec - ref["e_rec"]) < 1e-5 * scale
         assert abs(e_lj - (ref["e_lj"] + ref["e_lrc"])) < 1e-5 * abs(ref["e_lj"]) + 1e-5
     assert "overflow 0" in r.stdout
+    # mdout (control.cu:538-573): the header, then one row per print step;
+    # potential = atom energies (LJ + direct) + LRC + PME correction, self
+    # and reciprocal (MD_core.cu:1800-1812), i.e. LJ + PME
+    head, rows = read_mdout(tmp_path / "mdout.txt")
+    assert head == ["step", "time", "temperature", "potential", "LJ", "PME"]
+    assert [int(r_["step"]) for r_ in rows] == [1, 2, 3]
+    total = ref["e_lj"] + ref["e_lrc"] + ref["e_rec"] + ref["e_self"] + ref["e_direct"] + ref["e_corr"]
+    for r_, st in zip(rows, steps):
+        assert r_["time"] == f"{0.001 * int(r_['step']):.3f}" and r_["temperature"] == "0.00"
+        assert r_["LJ"] == f"{float(st[1]):.2f}" and r_["PME"] == f"{float(st[2]):.2f}"
+        assert abs(float(r_["potential"]) - total) < 0.006 + 1e-5 * abs(total), (r_["potential"], total)
+    assert re.search(r"^ +step = +1,  +time = +0.001,  temperature = +0.00, $", r.stdout, flags=re.M)
+
+
+def read_mdout(path):
+    lines = open(path).read().splitlines()
+    head = lines[0].split()
+    return head, [dict(zip(head, line.split())) for line in lines[1:]]
 
 
 @pytest.mark.gpu
 @needs_driver
-def test_driver_nve_matches_oracle(tmp_path):
+@pytest.mark.parametrize("fmt", ["txt", "rst7"])
+def test_driver_nve_matches_oracle(tmp_path, fmt):
     """SPONGE's NVE loop on the 648-atom rigid-water box, 10 steps of 2 fs:
     the restart coordinates after step 10 against the oracle's FP64
-    trajectory from the same velocity file (1e-4 Angstrom RMS, the bound of
+    trajectory from the same velocities (1e-4 Angstrom RMS, the bound of
     the chaotic growth of FP32 rounding over 10 steps), every water rigid,
-    one trajectory frame per write_information_interval."""
+    one trajectory frame per write_information_interval. The inputs are the
+    text files or one AMBER rst7 (then the restart is an rst7 too, and the
+    waters are the residues of the mdout temperature)."""
     import oracle
     sys_ = system.config(0)
     v = system.rigid_velocities(sys_)
-    print("[driver] nve water648 10 steps", flush=True)
+    print(f"[driver] nve water648 10 steps from {fmt}", flush=True)
+    extra = {"write_information_interval": 5, "crd": str(tmp_path / "mdcrd.dat"),
+             "box": str(tmp_path / "mdbox.txt"), "rst": str(tmp_path / "restart"),
+             "mdout": str(tmp_path / "mdout.txt")}
+    if fmt == "rst7":
+        (tmp_path / "res.txt").write_text(f"{sys_.n} {sys_.waters}\n" + "3\n" * sys_.waters)
+        extra["residue_in_file"] = str(tmp_path / "res.txt")
     mdin = system.write_sponge_inputs(sys_, str(tmp_path), step_limit=10, write_mdout_interval=5,
-                                      refresh_interval=20, velocities=v, dt_ps=0.002,
-                                      extra={"write_information_interval": 5,
-                                             "crd": str(tmp_path / "mdcrd.dat"),
-                                             "box": str(tmp_path / "mdbox.txt"),
-                                             "rst": str(tmp_path / "restart")})
+                                      refresh_interval=20, velocities=v, dt_ps=0.002, extra=extra,
+                                      rst7=fmt == "rst7")
+    if fmt == "rst7":                                  # the oracle starts from what the file holds
+        _, _, _, c0, v0, _ = system.read_rst7(str(tmp_path / "system.rst7"))
+        sys_.crd = c0.astype(np.float32)
+        v = v0.astype(np.float32)
     r = run(["-mdin", mdin], timeout=100)
     sys.stdout.write(r.stdout[-1500:])
     assert r.returncode == 0, r.stderr
@@ -201,11 +331,29 @@ def test_driver_nve_matches_oracle(tmp_path):
     frames = np.fromfile(tmp_path / "mdcrd.dat", dtype=np.float32).reshape(-1, sys_.n, 3)
     assert frames.shape[0] == 2
     assert len((tmp_path / "mdbox.txt").read_text().splitlines()) == 2
-    head = (tmp_path / "restart_coordinate.txt").read_text().splitlines()
-    assert head[0].split()[0] == str(sys_.n) and abs(float(head[0].split()[1]) - 0.02) < 1e-9
-    crd = np.loadtxt(tmp_path / "restart_coordinate.txt", skiprows=1, max_rows=sys_.n)
-    vel = np.loadtxt(tmp_path / "restart_velocity.txt", skiprows=1)
+    if fmt == "txt":
+        head = (tmp_path / "restart_coordinate.txt").read_text().splitlines()
+        assert head[0].split()[0] == str(sys_.n) and abs(float(head[0].split()[1]) - 0.02) < 1e-9
+        crd = np.loadtxt(tmp_path / "restart_coordinate.txt", skiprows=1, max_rows=sys_.n)
+        vel = np.loadtxt(tmp_path / "restart_velocity.txt", skiprows=1)
+        nres, res_of = sys_.n, np.arange(sys_.n)
+    else:
+        title, n, time, crd, vel, box = system.read_rst7(str(tmp_path / "restart"))
+        assert title == sys_.name and n == sys_.n and abs(time - 0.02) < 1e-9
+        np.testing.assert_allclose(box, sys_.box, atol=1e-6)
+        nres, res_of = sys_.waters, np.arange(sys_.n) // 3
     np.testing.assert_allclose(crd, frames[-1], atol=1e-6)
+    # the temperature column: residue centre-of-mass kinetic energy
+    # (MD_core.cu:216-239, 1738-1742) of the step-10 velocities
+    m = sys_.mass.astype(np.float64)
+    p_res = np.zeros((nres, 3))
+    np.add.at(p_res, res_of, m[:, None] * vel)
+    m_res = np.bincount(res_of, weights=m, minlength=nres)
+    t_ref = float((0.5 * (p_res ** 2).sum(1) / m_res).sum() * 2.0 / 0.00198716 / nres / 3.0)
+    head_, rows = read_mdout(tmp_path / "mdout.txt")
+    assert head_ == ["step", "time", "temperature", "potential", "LJ", "PME"]
+    assert [r_["step"] for r_ in rows] == ["5", "10"] and rows[1]["time"] == "0.020"
+    assert abs(float(rows[1]["temperature"]) - t_ref) < 0.006 + 1e-4 * t_ref, (rows[1]["temperature"], t_ref)
     groups = oracle.settle_groups(sys_.n, *system.water_constraints(sys_), sys_.mass)
     c_ref, v_ref, _ = oracle.md_run(sys_, v, 10, 0.002 * system.TIME_UNIT_PER_PS, groups)
     L = sys_.box.astype(np.float64)
@@ -237,7 +385,7 @@ def test_driver_npt_matches_oracle(tmp_path, barostat, keys):
     print(f"[driver] npt {barostat} water648 10 steps", flush=True)
     extra = {"mode": "NPT", "barostat": barostat, "target_pressure": 1.0, "write_information_interval": 10,
              "crd": str(tmp_path / "mdcrd.dat"), "box": str(tmp_path / "mdbox.txt"),
-             "rst": str(tmp_path / "restart")}
+             "rst": str(tmp_path / "restart"), "mdout": str(tmp_path / "mdout.txt")}
     extra.update(keys)
     mdin = system.write_sponge_inputs(sys_, str(tmp_path), step_limit=10, write_mdout_interval=5,
                                       refresh_interval=0, velocities=v, dt_ps=0.002, extra=extra)
@@ -259,3 +407,13 @@ def test_driver_npt_matches_oracle(tmp_path, barostat, keys):
     assert abs(box[0] / b_ref[0] - 1.0) < 1e-6 + 1e-6 / b_ref[0]   # mdbox.txt prints 6 decimals
     assert abs(b_ref[0] / float(sys_.box[0]) - 1.0) > 1e-5
     assert rms < 1e-4
+    # the barostat's mdout columns (Berendsen_barostat.cu:70-76,
+    # andersen_barostat.cu:46-53): density of the step-10 box
+    # (MD_core.cu:1510-1514) and the pressure the step printed
+    head, rows = read_mdout(tmp_path / "mdout.txt")
+    assert head == ["step", "time", "temperature", "potential", "LJ", "PME", "density", "pressure"] + \
+        (["dV/dt"] if kind == "andersen" else [])
+    rho = float(sys_.mass.astype(np.float64).sum()) * 1e24 / 6.023e23 / float(np.prod(b_ref))
+    assert abs(float(rows[-1]["density"]) - rho) < 6e-5, (rows[-1]["density"], rho)
+    p10 = float(re.search(r"^step 10 pressure (\S+) bar", r.stdout, flags=re.M).group(1))
+    assert abs(float(rows[-1]["pressure"]) - p10) < 0.006
