@@ -171,6 +171,7 @@ def scale_bench(steps: int, warmup: int, refresh: int, equilibrate: int):
     mde.force(need_energy=1, need_virial=1)
     pairs_in = eng.energy_parts()["pairs_in"]
     listed = eng.nl.status()["pairs"]
+    shapes = eng.shapes().as_dict()
     t = time_md(mde, steps, warmup, refresh, pme_events=True)
     ms = min(t["eager"], t["graph"], t["refresh_graph"])
     n = sys_.n
@@ -192,6 +193,7 @@ def scale_bench(steps: int, warmup: int, refresh: int, equilibrate: int):
         "graph_us_per_step": round(t["graph"] * 1e3, 2),
         "refresh_graph_us_per_step": round(t["refresh_graph"] * 1e3, 2),
         "pairs_in_cutoff": int(pairs_in),
+        "shapes": shapes,
         "pair_kernel_us": round(t["pair_kernel"] * 1e3, 2),
         "pair_gbs": gbs(b_pair, t["pair_kernel"]),
         "pair_tflops": round(PAIR_FLOPS * pairs_in / (t["pair_kernel"] * 1e-3) / 1e12, 2),
@@ -429,6 +431,9 @@ def main():
                     help="frames of the FEP / TI frame-evaluator comparison (0 skips it)")
     ap.add_argument("--scale-steps", type=int, default=100,
                     help="steps of the 96,120-atom measurement (0 skips it)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep the launch shapes on this workload first and time the fastest (default: the "
+                         "shapes the tuned table gives this atom count)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -459,6 +464,11 @@ def main():
     # trajectory is liquid water near 300 K
     mde.prepare()
     mde.equilibrate(args.equilibrate)
+    shapes_source = "tuned table (sponge_amd/csrc/tune_table.hpp) for this atom count"
+    if args.autotune:
+        from sponge_amd import autotune
+        us, _, res = autotune.tune(mde, args.refresh, log=lambda *x: print(*x, file=sys.stderr, flush=True))
+        shapes_source = f"online sweep of {len(res)} candidates ({us:.2f} us/step)"
     t_start_K = mde.temperature()
     crd0 = eng.crd.cpu().numpy().copy()
     vel0 = mde.vel.cpu().numpy().copy()
@@ -513,6 +523,8 @@ def main():
             "md_steps_run": t["md_steps_run"],
             "final_temperature_K": round(temperature, 1),
             "parallelism": "single GPU",
+            "shapes": eng.shapes().as_dict(),
+            "shapes_source": shapes_source,
         },
         "roofline": {
             "kernel": "pair_kernel<false> (LJ + PME direct), timed launched on its own",

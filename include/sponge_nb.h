@@ -104,9 +104,11 @@ int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_list_start, co
  * scheduled prunes, a step at which some atom has moved more than
  * (prune_radius - cutoff) / 2 since the last prune prunes at once (decided on
  * the device), so the inner list never misses a pair inside the cutoff that
- * the full list holds. Default for refresh_interval > 0: interval
- * refresh_interval / 5 and radius cutoff + skin / 2; off otherwise. The other
- * entry points always use the full list. */
+ * the full list holds. Default for refresh_interval > 0: the tuned table's
+ * values for the atom count (sponge_nb_default_shapes); off otherwise. The
+ * next fused step after this call prunes again (a kernel on the null stream
+ * marks it: not during a graph capture). The other entry points always use
+ * the full list. */
 int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius);
 
 /* Synchronises. Number of inner-list prunes that the displacement test above
@@ -271,6 +273,33 @@ int sponge_pme_set_events(sponge_pme* pme, void* spread_start, void* spread_stop
  * selects the one-workgroup-per-plane convolution or the three line launches
  * (0 is refused for planes beyond the LDS). */
 int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_split);
+
+/* Launch shapes and schedule of the fused step as one record (no counterpart
+ * in the reference, whose launch shapes are constants:
+ * Lennard_Jones_force.cu:1327-1350, neighbor_list.cu:435, 480). The handles'
+ * creators take their defaults from a table measured per system size by the
+ * shape sweep of sponge_amd/autotune.py; a caller may sweep its own system
+ * and set the winner. */
+typedef struct sponge_shapes {
+    int pair_waves;     /* i-clusters (waves) per block of the force-only pair kernel: 2, 4 or 8 */
+    int conv_split;     /* 0: one workgroup per kz plane; 1: three line launches */
+    int conv_threads;   /* threads of the per-plane workgroup: 256 or 512 */
+    int step_form;      /* 0: pair kernel on a second stream; 1: every kernel on the caller's stream */
+    int prune_interval; /* fused steps between scheduled prunes of the inner list; 0: no inner list */
+    float prune_radius; /* A, >= cutoff */
+} sponge_shapes;
+
+/* The tuned table's shapes for a system of atom_numbers atoms with the given
+ * list settings (planes beyond the LDS take conv_split 1 whatever the
+ * table says). */
+int sponge_nb_default_shapes(int atom_numbers, int refresh_interval, float cutoff, float skin, sponge_shapes* out);
+
+/* The shapes the handles run with now. */
+int sponge_nb_get_shapes(const sponge_nl* nl, const sponge_lj* lj, const sponge_pme* pme, sponge_shapes* out);
+
+/* Sets every field (validated first: nothing changes on an error). Outside
+ * graph capture: a new prune setting marks the next step for a prune. */
+int sponge_nb_set_shapes(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const sponge_shapes* shapes);
 
 /* Public members the reference's callers read: beta (PME_force.cuh:62) and
  * fftx/ffty/fftz (:30-32). */

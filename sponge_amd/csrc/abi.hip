@@ -2,12 +2,14 @@
 // the energy queries of Particle_Mesh_Ewald::Get_Energy.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <vector>
 
 #include "common.hpp"
+#include "tune_table.hpp"
 
 using namespace snb;
 
@@ -206,5 +208,53 @@ extern "C" int sponge_pme_get_energy(sponge_pme* pme, const sponge_nl* nl, const
         case 3: *h_energy = e_corr; break;
         default: *h_energy = e_self; break;
     }
+    return SPONGE_NB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch shapes (tune_table.hpp holds the measured defaults).
+
+extern "C" int sponge_nb_default_shapes(int atom_numbers, int refresh_interval, float cutoff, float skin,
+                                        sponge_shapes* out) {
+    SNB_ARG(out && atom_numbers > 0 && cutoff > 0 && skin >= 0, "nb_default_shapes");
+    const TuneRow& t = tune_row(atom_numbers);
+    out->pair_waves = t.pair_waves;
+    out->conv_split = t.conv_split < 0 ? -1 : t.conv_split;
+    out->conv_threads = t.conv_threads;
+    out->step_form = t.step_form;
+    out->prune_interval = (refresh_interval > 0 && t.prune_div > 0) ? std::max(1, refresh_interval / t.prune_div) : 0;
+    out->prune_radius = cutoff + t.prune_margin * skin;
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_nb_get_shapes(const sponge_nl* nl, const sponge_lj* lj, const sponge_pme* pme,
+                                    sponge_shapes* out) {
+    SNB_ARG(nl && lj && pme && out, "nb_get_shapes");
+    out->pair_waves = (lj->ps.waves == 2 || lj->ps.waves == 8) ? lj->ps.waves : 4;  // else the 4-wave block
+    out->conv_split = pme->conv_split;
+    out->conv_threads = pme->conv_threads > 0 ? pme->conv_threads : 256;
+    out->step_form = pme->step_form;
+    out->prune_interval = nl->prune_interval;
+    out->prune_radius = nl->prune_interval > 0 ? std::sqrt(nl->prune_r2) : nl->cutoff;
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_nb_set_shapes(sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const sponge_shapes* sh) {
+    SNB_ARG(nl && lj && pme && sh, "nb_set_shapes");
+    SNB_ARG(sh->pair_waves == 2 || sh->pair_waves == 4 || sh->pair_waves == 8, "pair_waves must be 2, 4 or 8");
+    SNB_ARG(sh->conv_threads == 256 || sh->conv_threads == 512, "conv_threads must be 256 or 512");
+    SNB_ARG(sh->step_form == 0 || sh->step_form == 1, "step_form must be 0 or 1");
+    SNB_ARG(sh->conv_split == 0 || sh->conv_split == 1, "conv_split must be 0 or 1");
+    SNB_ARG(sh->prune_interval >= 0, "prune_interval");
+    SNB_ARG(sh->prune_interval == 0 || sh->prune_radius >= nl->cutoff, "prune radius below the cutoff");
+    // the one check that needs the grid: the plane form must fit the LDS
+    int rc = sponge_pme_set_tuning(pme, -1, sh->conv_split);
+    if (rc) return rc;
+    pme->step_form = sh->step_form;
+    pme->conv_threads = sh->conv_threads;
+    lj->ps.waves = sh->pair_waves;
+    if (sh->prune_interval != nl->prune_interval ||
+        (sh->prune_interval > 0 && sh->prune_radius * sh->prune_radius != nl->prune_r2))
+        return sponge_nl_set_prune(nl, sh->prune_interval, sh->prune_radius);
     return SPONGE_NB_OK;
 }

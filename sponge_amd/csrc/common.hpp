@@ -122,6 +122,8 @@ struct PairScratch {
     uint4* pk;                   // packed (ux|t, uy|t, uz|t, q): LJ type in the low bits
     double* acc;                 // SPONGE_NB_NPARTS FP64 totals
     hipEvent_t ev_start, ev_stop;  // optional, recorded around the pair kernel
+    int waves;                   // i-clusters (waves) per block of the force-only pair kernel: 2, 4 or 8
+                                 // (0: 4)
 };
 
 // Mixed-radix pass list of one FFT axis (radix 8 first, then 4, 2, 3, 5, 7),
@@ -259,6 +261,7 @@ struct sponge_pme {
     hipEvent_t ev[4];            // optional: recorded by the spread and gather launches
     int step_form;               // fused step: 0 pair kernel on a second stream, 1 every kernel
                                  // on its own
+    int conv_threads;            // threads of the one-workgroup-per-plane convolution: 256 or 512
 };
 
 // ---- batched systems ------------------------------------------------------

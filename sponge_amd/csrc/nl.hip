@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "tune_table.hpp"
 
 using namespace snb;
 
@@ -658,16 +659,18 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
     if (e == hipSuccess) e = hipMemset(nl->list_in_n, 0, nl->ncl_cap * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->atom_in_n, 0, nl->slot_cap * sizeof(int));
     // Inner list of the fused step (interval mode): pruned every
-    // P = refresh_interval / 5 fused steps to rc + skin / 2, and at once on a
-    // step where an atom has moved more than skin / 4 since the last prune.
-    // Measured on 300 K water (23,570 and 96,120 atoms, refresh 20): P = 4
-    // with a 1 A margin needs no displacement prunes and was the fastest of
-    // P = 4..20 with margins 0.5..2 A.
+    // P = refresh_interval / prune_div fused steps to rc + margin * skin, and
+    // at once on a step where an atom has moved more than half the margin
+    // since the last prune; P and the margin are the measured defaults of
+    // tune_table.hpp for this many atoms (sponge_nl_set_prune changes them).
     if (refresh_interval > 0) {
-        nl->prune_interval = std::max(1, refresh_interval / 5);
-        const float rp = cutoff + 0.5f * skin;
-        nl->prune_r2 = rp * rp;
-        nl->prune_vote_r2 = snb_prune_vote_r2(cutoff, rp);
+        const TuneRow& t = tune_row(atom_numbers);
+        if (t.prune_div > 0) {
+            nl->prune_interval = std::max(1, refresh_interval / t.prune_div);
+            const float rp = cutoff + t.prune_margin * skin;
+            nl->prune_r2 = rp * rp;
+            nl->prune_vote_r2 = snb_prune_vote_r2(cutoff, rp);
+        }
     }
     A((void**)&nl->prune_ref, nl->slot_cap * sizeof(uint4));
     if (e == hipSuccess) e = hipMemset(nl->prune_ref, 0, nl->slot_cap * sizeof(uint4));
@@ -825,12 +828,22 @@ extern "C" int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_lis
     return nl_sym_exclusions(nl, d_excluded_list_start, d_excluded_list, d_excluded_numbers, nullptr);
 }
 
+// The next fused step re-prunes: it sees its own step number in the vote
+// word (flags[13]), as after a displacement vote.
+__global__ void nl_prune_now_kernel(int* flags) {
+    if (threadIdx.x == 0) flags[13] = flags[10] + 1;
+}
+
 extern "C" int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius) {
     SNB_ARG(nl && prune_interval >= 0, "nl_set_prune");
     SNB_ARG(prune_interval == 0 || prune_radius >= nl->cutoff, "prune radius below the cutoff");
     nl->prune_interval = prune_interval;
     nl->prune_r2 = prune_radius * prune_radius;
     nl->prune_vote_r2 = snb_prune_vote_r2(nl->cutoff, prune_radius);
+    // the inner lists were pruned to the old radius: rebuild them on the
+    // next step (ordered on the null stream before any later launch)
+    nl_prune_now_kernel<<<1, 64, 0, nullptr>>>(nl->flags);
+    SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }
 

@@ -27,6 +27,7 @@
 #include "pair_math.hpp"
 #include "pme_prep.hpp"
 #include "pair_body.hpp"
+#include "tune_table.hpp"
 
 using namespace snb;
 
@@ -182,15 +183,17 @@ __global__ __launch_bounds__(256) void step_prep_batch(const BatchSys* __restric
 
 
 
-template <bool ENERGY, bool SOFT, int TB>
-__global__ __launch_bounds__(kPairWaves * 64) void pair_kernel(
+// W: i-clusters (waves) per block; the force-only plain-LJ launch takes the
+// handle's tuned value (2, 4 or 8), every other launch 4.
+template <bool ENERGY, bool SOFT, int TB, int W = kPairWaves>
+__global__ __launch_bounds__(W * 64) void pair_kernel(
     const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list, int list_cap,
     const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz, float rc2,
     float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
     const float4* __restrict__ sc, PruneArgs pa, ExclArgs ex) {
-    pair_body<ENERGY, SOFT, TB>(xcd_block(gridDim.x), flags, list_n, list, list_cap, pk, ab, nt, cx, cy, cz, rc2, beta,
-                                s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc, pa, ex);
+    pair_body<ENERGY, SOFT, TB, W>(xcd_block(gridDim.x), flags, list_n, list, list_cap, pk, ab, nt, cx, cy, cz, rc2,
+                                   beta, s2o, frc, atom_e, atom_v, atom_d, acc, overwrite, sc, pa, ex);
 }
 
 // Batched: logical blocks are spread over the XCDs as in the single-system
@@ -366,6 +369,7 @@ extern "C" int sponge_lj_create(int atom_numbers, int atom_type_numbers, const f
     lj->n = atom_numbers;
     lj->ntypes = T;
     lj->cutoff = cutoff;
+    lj->ps.waves = tune_row(atom_numbers).pair_waves;
     lj->h_A = new float[P];
     lj->h_B = new float[P];
     lj->h_type = new int[atom_numbers];
@@ -585,9 +589,11 @@ void snb_scratch_free(PairScratch* ps) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     hipEvent_t e0 = ps->ev_start, e1 = ps->ev_stop;
+    const int waves = ps->waves;
     memset(ps, 0, sizeof(*ps));
     ps->ev_start = e0;  // events belong to the caller
     ps->ev_stop = e1;
+    ps->waves = waves;  // a setting, not storage
 }
 
 // Gather + pair kernel: forces added to d_frc (original order, may be NULL),
@@ -619,38 +625,44 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
                                                          energy, ps->acc);
         SNB_CHECK_LAUNCH();
     }
-    const int nblk = xcd_grid(ceil_div(nl->ncl_cap, kPairWaves));
+    const int W = (!energy && !sc && (ps->waves == 2 || ps->waves == 8)) ? ps->waves : kPairWaves;
+    const int nblk = xcd_grid(ceil_div(nl->ncl_cap, W));
     const float rc2 = cutoff * cutoff;
     const size_t lds = (size_t)nt * nt * sizeof(float2);
     // With timing events set, the launch itself records them (hipExtLaunchKernel
     // takes the kernel's own start and end times; events recorded around a
     // launch add the marker packets' latency, ~8 us per launch here)
     const bool timed = ps->ev_start && ps->ev_stop;
-#define SNB_PAIR(E, S)                      \
-    do {                                    \
-        if (nt <= 4) SNB_PAIR_TB(E, S, 2);  \
-        else SNB_PAIR_TB(E, S, 6);          \
+#define SNB_PAIR(E, S, W)                      \
+    do {                                       \
+        if (nt <= 4) SNB_PAIR_TB(E, S, 2, W);  \
+        else SNB_PAIR_TB(E, S, 6, W);          \
     } while (0)
-#define SNB_PAIR_TB(E, S, TB)                                                                                        \
+#define SNB_PAIR_TB(E, S, TB, W)                                                                                     \
     do {                                                                                                            \
         if (timed)                                                                                                  \
-            hipExtLaunchKernelGGL(pair_kernel<E, S, TB>, dim3(nblk), dim3(kPairWaves * 64), (std::uint32_t)lds, s,   \
+            hipExtLaunchKernelGGL(pair_kernel<E, S, TB, W>, dim3(nblk), dim3(W * 64), (std::uint32_t)lds, s,         \
                                   ps->ev_start, ps->ev_stop, 0u, nl->flags, list_n, list, nl->list_cap,              \
                                   (const uint4*)ps->pk, ab, nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o, d_frc,   \
                                   E ? atom_e : nullptr, E ? atom_v : nullptr, E ? atom_d : nullptr, ps->acc,         \
                                   overwrite, sc, pa, ex);                                                            \
         else                                                                                                        \
-            pair_kernel<E, S, TB><<<nblk, kPairWaves * 64, lds, s>>>(                                               \
+            pair_kernel<E, S, TB, W><<<nblk, W * 64, lds, s>>>(                                                     \
                 nl->flags, list_n, list, nl->list_cap, ps->pk, ab, nt, cof[0], cof[1], cof[2], rc2, beta, nl->s2o,  \
                 d_frc, E ? atom_e : nullptr, E ? atom_v : nullptr, E ? atom_d : nullptr, ps->acc, overwrite, sc,    \
                 pa, ex);                                                                                            \
     } while (0)
     if (energy) {
-        if (sc) SNB_PAIR(true, true);
-        else SNB_PAIR(true, false);
+        if (sc) SNB_PAIR(true, true, kPairWaves);
+        else SNB_PAIR(true, false, kPairWaves);
+    } else if (sc) {
+        SNB_PAIR(false, true, kPairWaves);
+    } else if (W == 2) {
+        SNB_PAIR(false, false, 2);
+    } else if (W == 8) {
+        SNB_PAIR(false, false, 8);
     } else {
-        if (sc) SNB_PAIR(false, true);
-        else SNB_PAIR(false, false);
+        SNB_PAIR(false, false, kPairWaves);
     }
 #undef SNB_PAIR
 #undef SNB_PAIR_TB

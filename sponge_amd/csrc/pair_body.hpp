@@ -459,8 +459,8 @@ __device__ __forceinline__ void pair_wave(
 
 
 // One block of the pair kernel: the LJ (A, B) table into LDS, then one
-// i-cluster per wave (logical block lb covers clusters kPairWaves lb ..).
-template <bool ENERGY, bool SOFT, int TB>
+// i-cluster per wave (logical block lb covers clusters W lb .. W lb + W - 1).
+template <bool ENERGY, bool SOFT, int TB, int W = kPairWaves>
 __device__ __forceinline__ void pair_body(
     int lb, const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list,
     int list_cap, const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz,
@@ -471,7 +471,7 @@ __device__ __forceinline__ void pair_body(
     extern __shared__ float2 sab[];
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
     __syncthreads();
-    const int ic = lb * kPairWaves + (int)(threadIdx.x >> 6);
+    const int ic = lb * W + (int)(threadIdx.x >> 6);
     if (lb < 0 || ic >= flags[4]) return;
     pair_wave<ENERGY, SOFT, TB>(ic, sab, flags, list_n, list, list_cap, pk, nt, cx, cy, cz, rc2, beta, s2o, frc,
                                 atom_e, atom_v, atom_d, acc, overwrite, sc, pa, ex);

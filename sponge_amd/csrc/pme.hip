@@ -32,6 +32,7 @@
 
 #include "common.hpp"
 #include "pme_prep.hpp"
+#include "tune_table.hpp"
 
 using namespace snb;
 
@@ -516,22 +517,22 @@ __global__ __launch_bounds__(kZThreads) void spread_z_batch(const BatchSys* __re
 // and columns), FFT along y then x, energy 1/2 sum m BC |F|^2 (FP64; m = 2 for
 // the kz that stand for their conjugates too), multiply by BC, inverse FFT
 // along x then y.
-template <int RM = kRadixAll>
+template <int RM = kRadixAll, int NT = kConvThreads>
 __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, const float* __restrict__ bc, int Kx,
                                             int Ky, int Kz, const FftPlan& px, const FftPlan& py,
                                             const float2* __restrict__ twx, const float2* __restrict__ twy, int energy,
                                             double* __restrict__ acc, int* __restrict__ bin_count) {
     __builtin_amdgcn_s_setprio(kPmeWavePriority);
     extern __shared__ float2 sm[];
-    __shared__ double red[kConvThreads / 64];
+    __shared__ double red[NT / 64];
     const int pitch = line_pitch(Ky);
     float2* tx = sm + padded_len(Kx) * pitch;
     float2* ty = tx + Kx;
     // the spread has consumed the x-plane bins: clear them for the next prep
     if (kz == 0)
-        for (int b = threadIdx.x; b < Kx; b += kConvThreads) bin_count[b] = 0;
-    load_twiddles<kConvThreads>(tx, twx, Kx);
-    load_twiddles<kConvThreads>(ty, twy, Ky);
+        for (int b = threadIdx.x; b < Kx; b += NT) bin_count[b] = 0;
+    load_twiddles<NT>(tx, twx, Kx);
+    load_twiddles<NT>(ty, twy, Ky);
     float2* plane = FQ + (size_t)kz * Kx * Ky;
     const float* bcp = bc + (size_t)kz * Kx * Ky;
     const int np = Kx * Ky;
@@ -541,12 +542,12 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
     // through the forward transforms): with one block per CU nothing else
     // hides the memory latency.
     float bv[kConvBatch];
-    const bool onepass = np <= kConvBatch * kConvThreads;
-    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * kConvThreads) {
+    const bool onepass = np <= kConvBatch * NT;
+    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * NT) {
         float2 v[kConvBatch];
 #pragma unroll
         for (int s = 0; s < kConvBatch; s++) {
-            const int idx = s0 + s * kConvThreads;
+            const int idx = s0 + s * NT;
             if (idx < np) {
                 v[s] = plane[idx];
                 if (onepass) bv[s] = bcp[idx];
@@ -554,7 +555,7 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
         }
 #pragma unroll
         for (int s = 0; s < kConvBatch; s++) {
-            const int idx = s0 + s * kConvThreads;
+            const int idx = s0 + s * NT;
             if (idx < np) {
                 const int xx = div_small(idx, rky), y = idx - xx * Ky;
                 sm[pd(xx) * pitch + pd(y)] = v[s];
@@ -562,22 +563,22 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
         }
     }
     __syncthreads();
-    lds_fft<-1, kConvThreads, RM>(sm, Kx, pitch, 1, py, ty);
+    lds_fft<-1, NT, RM>(sm, Kx, pitch, 1, py, ty);
     __syncthreads();
-    lds_fft<-1, kConvThreads, RM>(sm, Ky, 1, pitch, px, tx);
+    lds_fft<-1, NT, RM>(sm, Ky, 1, pitch, px, tx);
     __syncthreads();
     double e = 0.0;
-    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * kConvThreads) {
+    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * NT) {
         if (!onepass) {
 #pragma unroll
             for (int s = 0; s < kConvBatch; s++) {
-                const int idx = s0 + s * kConvThreads;
+                const int idx = s0 + s * NT;
                 if (idx < np) bv[s] = bcp[idx];
             }
         }
 #pragma unroll
         for (int s = 0; s < kConvBatch; s++) {
-            const int idx = s0 + s * kConvThreads;
+            const int idx = s0 + s * NT;
             if (idx < np) {
                 const int xx = div_small(idx, rky), y = idx - xx * Ky;
                 const float b = bv[s];
@@ -588,16 +589,16 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
         }
     }
     __syncthreads();
-    lds_fft<1, kConvThreads, RM>(sm, Ky, 1, pitch, px, tx);
+    lds_fft<1, NT, RM>(sm, Ky, 1, pitch, px, tx);
     __syncthreads();
-    lds_fft<1, kConvThreads, RM>(sm, Kx, pitch, 1, py, ty);
+    lds_fft<1, NT, RM>(sm, Kx, pitch, 1, py, ty);
     __syncthreads();
-    for (int idx = threadIdx.x; idx < np; idx += kConvThreads) {
+    for (int idx = threadIdx.x; idx < np; idx += NT) {
         const int xx = div_small(idx, rky), y = idx - xx * Ky;
         plane[idx] = sm[pd(xx) * pitch + pd(y)];
     }
     if (energy) {
-        const double t = block_sum<kConvThreads>(e, red);
+        const double t = block_sum<NT>(e, red);
         const double m = (kz == 0 || 2 * kz == Kz) ? 1.0 : 2.0;
         if (threadIdx.x == 0) atomicAdd(acc + SPONGE_PART_RECIPROCAL, 0.5 * m * t);
     }
@@ -610,6 +611,16 @@ __global__ __launch_bounds__(kConvThreads) void conv2d_kernel(float2* __restrict
                                                               const float2* __restrict__ twy, int energy,
                                                               double* __restrict__ acc, int* __restrict__ bin_count) {
     conv2d_body<RM>(blockIdx.x, FQ, bc, Kx, Ky, Kz, px, py, twx, twy, energy, acc, bin_count);
+}
+
+// The same with 512 threads (8 waves) per plane: more waves to hide the LDS
+// latency of the transform passes, where the CUs are free to hold them.
+template <int RM>
+__global__ __launch_bounds__(2 * kConvThreads) void conv2d_wide_kernel(
+    float2* __restrict__ FQ, const float* __restrict__ bc, int Kx, int Ky, int Kz, FftPlan px, FftPlan py,
+    const float2* __restrict__ twx, const float2* __restrict__ twy, int energy, double* __restrict__ acc,
+    int* __restrict__ bin_count) {
+    conv2d_body<RM, 2 * kConvThreads>(blockIdx.x, FQ, bc, Kx, Ky, Kz, px, py, twx, twy, energy, acc, bin_count);
 }
 
 __global__ __launch_bounds__(kConvThreads) void conv2d_batch(const BatchSys* __restrict__ bs,
@@ -1058,6 +1069,8 @@ int raise_lds_limits(size_t bytes) {
                          (const void*)spread_z_adjoint_kernel,    (const void*)spread_z_batch,
                          (const void*)conv2d_kernel<kRadixAll>,   (const void*)conv2d_kernel<kRadix8>,
                          (const void*)conv2d_kernel<kRadix34>,    (const void*)conv2d_kernel<kRadix248>,
+                         (const void*)conv2d_wide_kernel<kRadixAll>, (const void*)conv2d_wide_kernel<kRadix8>,
+                         (const void*)conv2d_wide_kernel<kRadix34>,  (const void*)conv2d_wide_kernel<kRadix248>,
                          (const void*)conv2d_batch,               (const void*)inv_z_kernel<kRadixAll>,
                          (const void*)inv_z_kernel<kRadix8>,      (const void*)inv_z_kernel<kRadix34>,
                          (const void*)inv_z_kernel<kRadix248>,    (const void*)inv_z_batch,
@@ -1177,9 +1190,16 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
         max_lds = 64 * 1024;
-    // planes beyond 64 x 64 (or beyond the LDS) run the split convolution;
-    // SPONGE_PME_SPLIT_CONV=0/1 forces either form where it fits
-    pme->conv_split = (Kx * Ky > kConvPlaneMax || conv2d_lds(Kx, Ky) > (size_t)max_lds) ? 1 : 0;
+    // planes beyond 64 x 64 (or beyond the LDS) run the split convolution
+    // unless the tuned table says otherwise; SPONGE_PME_SPLIT_CONV=0/1 forces
+    // either form where it fits
+    const TuneRow& tune = tune_row(atom_numbers);
+    const bool plane_fits = conv2d_lds(Kx, Ky) <= (size_t)max_lds;
+    pme->conv_split = (Kx * Ky > kConvPlaneMax || !plane_fits) ? 1 : 0;
+    if (tune.conv_split == 0 && plane_fits) pme->conv_split = 0;
+    if (tune.conv_split == 1) pme->conv_split = 1;
+    pme->conv_threads = tune.conv_threads == 2 * kConvThreads ? 2 * kConvThreads : kConvThreads;
+    pme->step_form = tune.step_form == 1 ? 1 : 0;
     if (const char* env = getenv("SPONGE_PME_SPLIT_CONV")) {
         if (env[0] == '1') pme->conv_split = 1;
         if (env[0] == '0' && conv2d_lds(Kx, Ky) <= (size_t)max_lds) pme->conv_split = 0;
@@ -1376,6 +1396,11 @@ static int convolve_launch(sponge_pme* pme, int energy, hipStream_t s) {
         SNB_CHECK_LAUNCH();
         SNB_RADIX_DISPATCH(sy, conv_y_inverse_kernel, gy, kLineThreads, lines_lds(Ky), s, pme->FQ, Kx, Ky, py,
                            pme->d_tw[1]);
+        SNB_CHECK_LAUNCH();
+    } else if (pme->conv_threads == 2 * kConvThreads) {
+        SNB_RADIX_DISPATCH(kernel_set(radix_mask(px) | radix_mask(py)), conv2d_wide_kernel, H, 2 * kConvThreads,
+                           conv2d_lds(Kx, Ky), s, pme->FQ, pme->d_bc, Kx, Ky, Kz, px, py, pme->d_tw[0],
+                           pme->d_tw[1], energy, pme->acc, pme->bin_count);
         SNB_CHECK_LAUNCH();
     } else {
         SNB_RADIX_DISPATCH(kernel_set(radix_mask(px) | radix_mask(py)), conv2d_kernel, H, kConvThreads,

@@ -73,6 +73,9 @@ _SIGS = {
     "sponge_pme_update_box_length": (_I, [_P, _FP3]),
     "sponge_pme_set_events": (_I, [_P, _P, _P, _P, _P]),
     "sponge_pme_set_tuning": (_I, [_P, _I, _I]),
+    "sponge_nb_default_shapes": (_I, [_I, _I, _F, _F, _P]),
+    "sponge_nb_get_shapes": (_I, [_P, _P, _P, _P]),
+    "sponge_nb_set_shapes": (_I, [_P, _P, _P, _P]),
     "sponge_nb_force_step": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "sponge_sum_list": (_I, [_I, _P, _P, _P]),
     "sponge_lj_set_tables": (_I, [_P, _P, _P]),
@@ -112,6 +115,22 @@ FEP_FIELDS = ["vdw_intersys", "vdw_intrasys", "coul_direct_intersys", "coul_dire
 
 class SpongeError(RuntimeError):
     pass
+
+
+class Shapes(ctypes.Structure):
+    """sponge_shapes (include/sponge_nb.h): launch shapes and schedule of the
+    fused step."""
+    _fields_ = [("pair_waves", _I), ("conv_split", _I), ("conv_threads", _I), ("step_form", _I),
+                ("prune_interval", _I), ("prune_radius", _F)]
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["prune_radius"] = round(float(d["prune_radius"]), 4)
+        return d
+
+    @classmethod
+    def from_dict(cls, d):
+        return cls(**{name: d[name] for name, _ in cls._fields_})
 
 
 def load_library(path: str = LIB_PATH):
@@ -654,6 +673,30 @@ class NonbondedEngine:
         torch.cuda.synchronize()
         p = self.parts.cpu().numpy()
         return {k: float(v) for k, v in zip(PART_NAMES, p)}
+
+    def shapes(self):
+        """The launch shapes the handles run with (a Shapes record)."""
+        sh = Shapes()
+        _check(load_library().sponge_nb_get_shapes(self.nl.h, self.lj.h, self.pme.h, ctypes.byref(sh)),
+               "sponge_nb_get_shapes")
+        return sh
+
+    def set_shapes(self, sh):
+        """Sets a Shapes record (or a dict of its fields); between steps, not
+        inside a graph capture."""
+        if isinstance(sh, dict):
+            sh = Shapes.from_dict(sh)
+        _check(load_library().sponge_nb_set_shapes(self.nl.h, self.lj.h, self.pme.h, ctypes.byref(sh)),
+               "sponge_nb_set_shapes")
+
+    def default_shapes(self):
+        """The tuned table's shapes for this system (conv_split -1: by plane
+        size)."""
+        sh = Shapes()
+        _check(load_library().sponge_nb_default_shapes(self.n, self.refresh_interval, float(self.sys.cutoff),
+                                                        float(self.sys.skin), ctypes.byref(sh)),
+               "sponge_nb_default_shapes")
+        return sh
 
 
 class _BatchIo(ctypes.Structure):

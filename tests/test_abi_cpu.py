@@ -107,3 +107,39 @@ def test_pme_rejects_unsupported_fft_size():
     rc = lib.sponge_pme_create(1000, box, 10.0, 1e-5, 384, 384, 384, ctypes.byref(h))
     assert rc == -1
     assert b"384" in lib.sponge_nb_last_error()
+
+
+def test_default_shapes_follow_the_tuned_table():
+    """sponge_nb_default_shapes (a table lookup, no device call) returns the
+    row of sponge_amd/csrc/tune_table.hpp that covers the atom count."""
+    from sponge_amd import autotune, nb
+    if not os.path.exists(nb.LIB_PATH):
+        pytest.skip("libsponge_nb.so not built")
+    lib = nb.load_library()
+    rows = autotune.parse_table()
+    assert rows and rows[-1]["max_atoms"] == 0x7fffffff
+    assert [r["max_atoms"] for r in rows] == sorted(r["max_atoms"] for r in rows)
+    for atoms in (100, 648, 5000, 23570, 50001, 96120, 10 ** 7):
+        for refresh in (0, 10, 20):
+            sh = nb.Shapes()
+            assert lib.sponge_nb_default_shapes(atoms, refresh, 10.0, 2.0, ctypes.byref(sh)) == 0
+            r = autotune.table_row(rows, atoms)
+            assert sh.pair_waves == r["pair_waves"] and sh.conv_threads == r["conv_threads"]
+            assert sh.conv_split == r["conv_split"] and sh.step_form == r["step_form"]
+            want = max(1, refresh // r["prune_div"]) if refresh and r["prune_div"] else 0
+            assert sh.prune_interval == want
+            assert sh.prune_radius == pytest.approx(10.0 + r["prune_margin"] * 2.0)
+    for r in rows:
+        assert r["pair_waves"] in (2, 4, 8) and r["conv_threads"] in (256, 512)
+        assert r["conv_split"] in (-1, 0, 1) and r["step_form"] in (0, 1)
+    sh = nb.Shapes()
+    assert lib.sponge_nb_default_shapes(0, 20, 10.0, 2.0, ctypes.byref(sh)) != 0
+
+
+def test_shape_candidates_cover_the_swept_axes():
+    from sponge_amd import autotune
+    c = autotune.candidates(20, 10.0, 2.0)
+    assert len(c) == len(autotune.PAIR_WAVES) * len(autotune.CONV_FORMS) * 2 * len(autotune.PRUNES)
+    assert {x["prune_interval"] for x in c} == {4, 5, 2, 0}
+    assert all(x["prune_radius"] >= 10.0 for x in c)
+    assert not any(x["conv_split"] == 0 for x in autotune.candidates(20, 10.0, 2.0, plane_fits=False))

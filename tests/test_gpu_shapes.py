@@ -1,0 +1,112 @@
+"""Every launch shape the autotuner sweeps (sponge_shapes: pair-kernel waves
+per block, convolution form and threads, step form, inner-list prune
+schedule) computes the same fused force step, within the force contract of
+the FP64 oracle; the shapes are set and read back through the C ABI and bad
+ones are refused without changing the handles."""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from sponge_amd import system
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = {
+    "water648": lambda: system.config(0),
+    "random_noncubic": lambda: system.small_random(900, (21.0, 24.5, 27.3), seed=3, cutoff=7.0, skin=1.5),
+    "water23570": lambda: system.config(1),
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def steps(eng, k=3):
+    import torch
+    for _ in range(k):
+        eng.force_step(0, 0, from_crd=True, overwrite=True)
+    torch.cuda.synchronize()
+    return eng.frc.cpu().numpy().copy()
+
+
+@pytest.mark.parametrize("name", sorted(SYSTEMS))
+def test_every_shape_gives_the_same_forces(name):
+    from sponge_amd import nb
+    sys_ = SYSTEMS[name]()
+    say(f"[case] shapes {name} n={sys_.n}")
+    eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    eng.update_neighbor_list(forced=1)
+    ref = oracle.full_reference(sys_, beta=eng.pme.beta, K=(eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
+    base = eng.shapes().as_dict()
+    f0 = steps(eng)
+    assert rel_rms(f0, ref["frc"]) < 1e-5
+    rc, skin = float(sys_.cutoff), float(sys_.skin)
+    prunes = [(base["prune_interval"], base["prune_radius"]), (0, rc), (2, rc + 0.25 * skin)]
+    by_waves = {}
+    for w, (split, nt), form, (pi, pr) in itertools.product((2, 4, 8), ((0, 256), (0, 512), (1, 256)), (0, 1),
+                                                            prunes):
+        sh = dict(pair_waves=w, conv_split=split, conv_threads=nt, step_form=form, prune_interval=pi,
+                  prune_radius=pr)
+        eng.set_shapes(sh)
+        got = eng.shapes().as_dict()
+        assert {k: got[k] for k in sh if k != "prune_radius"} == {k: sh[k] for k in sh if k != "prune_radius"}
+        if pi:
+            assert got["prune_radius"] == pytest.approx(pr, abs=1e-5)
+        f = steps(eng)
+        err = rel_rms(f, f0)
+        assert err < 1e-6, (sh, err)
+        # the waves per block only regroup the same per-wave arithmetic
+        key = (split, nt, form, pi, pr)
+        if key in by_waves:
+            np.testing.assert_array_equal(f, by_waves[key], err_msg=str(sh))
+        else:
+            by_waves[key] = f
+    eng.set_shapes(base)
+    np.testing.assert_array_equal(steps(eng), f0)
+
+
+def test_bad_shapes_are_refused_and_change_nothing():
+    from sponge_amd import nb
+    sys_ = system.config(0)
+    eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    eng.update_neighbor_list(forced=1)
+    before = eng.shapes().as_dict()
+    good = dict(before)
+    for bad in (dict(pair_waves=3), dict(conv_threads=128), dict(step_form=2), dict(conv_split=-1),
+                dict(prune_interval=-1), dict(prune_interval=4, prune_radius=float(sys_.cutoff) - 0.5)):
+        sh = dict(good)
+        sh.update(bad)
+        with pytest.raises(nb.SpongeError):
+            eng.set_shapes(sh)
+        assert eng.shapes().as_dict() == before
+
+
+def test_new_prune_setting_prunes_at_the_next_step():
+    """sponge_nl_set_prune marks the next fused step: it prunes again (counted
+    with the displacement-vote prunes), so the inner lists follow the new
+    radius at once."""
+    import torch
+    from sponge_amd import nb
+    sys_ = system.config(0)
+    eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    eng.update_neighbor_list(forced=1)
+    f0 = steps(eng, 2)                                  # step 1 prunes (scheduled), step 2 reads
+    v0 = eng.nl.prune_votes()
+    sh = eng.shapes().as_dict()
+    sh["prune_radius"] = float(sys_.cutoff) + 0.9 * float(sys_.skin)
+    eng.set_shapes(sh)
+    f1 = steps(eng, 1)                                  # step 3: not scheduled, marked by the setter
+    assert eng.nl.prune_votes() == v0 + 1
+    np.testing.assert_allclose(f1, f0, rtol=0, atol=1e-5 * float(np.abs(f0).max()))
+    torch.cuda.synchronize()
