@@ -35,6 +35,10 @@ void set_error(const char* fmt, ...);
 
 constexpr int kWave = 64;
 constexpr int kClusterSize = 4;          // atoms per cluster (4x4 cluster pairs, 4 per wave64)
+// Slot records copied on either side of the record array (periodic images of
+// the first and last kPkHalo slots): a 16-bit partner offset from any slot
+// then reads a record inside the allocation.
+constexpr int kPkHalo = 32768;
 constexpr int kMaxLjTypes = 64;          // LDS pair table holds kMaxLjTypes^2 (A,B)
 constexpr int kNumXcd = 8;               // MI355X: 8 XCDs, workgroups dealt round-robin
 // Largest PME grid axis: sponge_pme_create rejects larger axes, and the prep
@@ -120,6 +124,8 @@ __device__ double block_sum(double v, double* smem /* NT/64 */) {
 struct PairScratch {
     int cap;                     // slots (the list's slot capacity)
     uint4* pk;                   // packed (ux|t, uy|t, uz|t, q): LJ type in the low bits
+    uint4* pk_alloc;             // allocation holding pk with kPkHalo records on either side
+                                 // (NULL: pk is a caller's buffer without them)
     double* acc;                 // SPONGE_NB_NPARTS FP64 totals
     hipEvent_t ev_start, ev_stop;  // optional, recorded around the pair kernel
     int waves;                   // i-clusters (waves) per block of the force-only pair kernel: 2, 4 or 8
@@ -191,13 +197,16 @@ struct sponge_nl {
     int* list_in_n;             // ncl_cap
     uint2* list_in;             // ncl_cap * list_cap
     // the inner pairs per i slot (j slots within prune_r2) for plain LJ,
-    // which force-only fused steps read (list_in is soft-core LJ's form).
-    // flags[11] = 1 once a slot held more than atom_in_cap partners at a
-    // prune since the last rebuild (its cluster then reads the full list),
-    // flags[12] counts the launches that read the per-slot form
+    // which force-only fused steps read (list_in is soft-core LJ's form),
+    // each as the 16-bit offset j - i, taken modulo the slot count when that
+    // is the shorter way (the slot records carry kPkHalo copies on either
+    // side, so i + offset is always a readable record). flags[11] = 1 once a
+    // slot held more than atom_in_cap partners, or a partner out of 16-bit
+    // reach, at a prune since the last rebuild (its cluster then reads the
+    // full list); flags[12] counts the launches that read the per-slot form
     int atom_in_cap;
     int* atom_in_n;             // slot_cap
-    int* atom_in;               // slot_cap * atom_in_cap
+    unsigned short* atom_in;    // slot_cap * atom_in_cap
     // symmetric exclusion lists in original atom order (partners of atom a:
     // sym_list[sym_start[a] .. sym_start[a+1])), built on the host from the
     // exclusion CSR the first time an update passes those arrays
@@ -308,7 +317,7 @@ struct PruneArgs {
     int* in_n = nullptr;
     uint2* in_list = nullptr;
     int* atom_n = nullptr;      // per-slot inner lists (NULL: cluster form only)
-    int* atom_list = nullptr;
+    unsigned short* atom_list = nullptr;  // 16-bit slot offsets
     int atom_cap = 0;
     int* diag = nullptr;        // flags + 11: [0] per-slot overflow, [1] per-slot launches,
                                 // [2] step of the last displacement vote, [4] voted prunes

@@ -648,16 +648,18 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
         // on a 128-byte line of its own
         const double rl3 = std::pow((double)cutoff + (double)skin, 3);
         const int est = (int)std::ceil(1.25 * (4.0 / 3.0) * M_PI * rl3 * dens) + 64;
-        nl->atom_in_cap = (std::min(est, nl->slot_cap) + 31) & ~31;
+        nl->atom_in_cap = (std::min(est, nl->slot_cap) + 63) & ~63;
     }
     A((void**)&nl->atom_in_n, nl->slot_cap * sizeof(int));
-    A((void**)&nl->atom_in, (size_t)nl->slot_cap * nl->atom_in_cap * sizeof(int));
+    A((void**)&nl->atom_in, (size_t)nl->slot_cap * nl->atom_in_cap * sizeof(unsigned short));
     A((void**)&nl->pack, nl->slot_cap * sizeof(uint4));
     A((void**)&nl->flags, 16 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->flags, 0, 16 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->list_n, 0, nl->ncl_cap * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->list_in_n, 0, nl->ncl_cap * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->atom_in_n, 0, nl->slot_cap * sizeof(int));
+    if (e == hipSuccess)
+        e = hipMemset(nl->atom_in, 0, (size_t)nl->slot_cap * nl->atom_in_cap * sizeof(unsigned short));
     // Inner list of the fused step (interval mode): pruned every
     // P = refresh_interval / prune_div fused steps to rc + margin * skin, and
     // at once on a step where an atom has moved more than half the margin

@@ -89,7 +89,7 @@ __device__ __forceinline__ void step_prep_body(int blk, const int* __restrict__ 
                                                int* __restrict__ step_counter, const int* __restrict__ dropped,
                                                float* __restrict__ zero_frc, const uint4* __restrict__ prune_ref = nullptr,
                                                float vote_r2 = 0.f, float cx = 0.f, float cy = 0.f, float cz = 0.f,
-                                               int* __restrict__ vote = nullptr) {
+                                               int* __restrict__ vote = nullptr, int halo = 0) {
     const int p = blk * blockDim.x + threadIdx.x;
     if (energy && p < 3) acc[p] = 0.0;  // SPONGE_PART_LJ, _DIRECT, _LJ_VIRIAL
     if (energy && p == 3) acc[SPONGE_PART_PAIRS_IN] = 0.0;
@@ -135,10 +135,21 @@ __device__ __forceinline__ void step_prep_body(int blk, const int* __restrict__ 
         }
         qa = q[a];
         const unsigned t = (unsigned)type[a];
-        pk[p] = make_uint4((u0 & ~3u) | (t & 3u), (u1 & ~3u) | ((t >> 2) & 3u), (u2 & ~3u) | ((t >> 4) & 3u),
-                           __float_as_uint(qa));
+        const uint4 rec = make_uint4((u0 & ~3u) | (t & 3u), (u1 & ~3u) | ((t >> 2) & 3u),
+                                     (u2 & ~3u) | ((t >> 4) & 3u), __float_as_uint(qa));
+        pk[p] = rec;
+        // periodic images for the 16-bit partner offsets: needed only when
+        // some offset wraps, i.e. beyond 32767 slots
+        if (halo && nslot > 32767) {
+            if (p < kPkHalo) pk[nslot + p] = rec;
+            if (p >= nslot - kPkHalo) pk[p - nslot] = rec;
+        }
     } else if (p < nslot) {
         pk[p] = make_uint4(0u, 0u, 0u, 0u);
+        if (halo && nslot > 32767) {
+            if (p < kPkHalo) pk[nslot + p] = make_uint4(0u, 0u, 0u, 0u);
+            if (p >= nslot - kPkHalo) pk[p - nslot] = make_uint4(0u, 0u, 0u, 0u);
+        }
     }
     if (vote) {
         // displacement of the slot's atom since the inner list's last prune
@@ -165,9 +176,9 @@ __global__ __launch_bounds__(256) void step_prep_kernel(const int* __restrict__ 
                                                         int* __restrict__ step_counter, const int* __restrict__ dropped,
                                                         float* __restrict__ zero_frc, const uint4* __restrict__ prune_ref,
                                                         float vote_r2, float cx, float cy, float cz,
-                                                        int* __restrict__ vote) {
+                                                        int* __restrict__ vote, int halo) {
     step_prep_body(blockIdx.x, flags, s2o, crd, cqx, cqy, cqz, uc, q, type, pk, P, energy, acc, step_counter, dropped,
-                   zero_frc, prune_ref, vote_r2, cx, cy, cz, vote);
+                   zero_frc, prune_ref, vote_r2, cx, cy, cz, vote, halo);
 }
 
 __global__ __launch_bounds__(256) void step_prep_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map,
@@ -577,7 +588,13 @@ static int scratch_ensure(PairScratch* ps, const sponge_nl* nl) {
     // graph capture of the step can start).
     snb_scratch_free(ps);
     int cap = nl->slot_cap;
-    SNB_HIP(hipMalloc((void**)&ps->pk, cap * sizeof(uint4)));
+    // the records with kPkHalo slots on either side (16-bit partner offsets
+    // of the per-slot inner lists reach them); zeroed so that no lane ever
+    // reads an undefined record
+    const size_t nrec = (size_t)cap + 2 * (size_t)kPkHalo;
+    SNB_HIP(hipMalloc((void**)&ps->pk_alloc, nrec * sizeof(uint4)));
+    SNB_HIP(hipMemset(ps->pk_alloc, 0, nrec * sizeof(uint4)));
+    ps->pk = ps->pk_alloc + kPkHalo;
     SNB_HIP(hipMalloc((void**)&ps->acc, SPONGE_NB_NPARTS * sizeof(double)));
     SNB_HIP(hipMemset(ps->acc, 0, SPONGE_NB_NPARTS * sizeof(double)));
     ps->cap = cap;
@@ -585,7 +602,7 @@ static int scratch_ensure(PairScratch* ps, const sponge_nl* nl) {
 }
 
 void snb_scratch_free(PairScratch* ps) {
-    void* ptrs[] = {ps->pk, ps->acc};
+    void* ptrs[] = {ps->pk_alloc, ps->acc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     hipEvent_t e0 = ps->ev_start, e1 = ps->ev_stop;
@@ -602,7 +619,9 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
               const unsigned int* d_uint_crd, const float* d_charge, float beta, int energy, PairScratch* ps,
               float* d_frc, float* atom_e, float* atom_v, float* atom_d, hipStream_t s, int gathered,
               int overwrite, const float4* sc, int inner, ExclArgs ex) {
-    inner = inner && nl->prune_interval > 0;
+    // the per-slot inner lists address records through 16-bit offsets,
+    // which needs the record halo of an owned scratch
+    inner = inner && nl->prune_interval > 0 && ps->pk_alloc != nullptr;
     PruneArgs pa{};
     if (inner) {
         pa.interval = nl->prune_interval;
@@ -693,7 +712,8 @@ int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const flo
                                                         lj->ps.acc, prune ? nl->flags + 10 : nullptr, nl->dropped,
                                                         d_frc_assigned, nl->prune_ref, nl->prune_vote_r2,
                                                         lj->cof_dr[0], lj->cof_dr[1], lj->cof_dr[2],
-                                                        prune ? nl->flags + 13 : nullptr);
+                                                        prune ? nl->flags + 13 : nullptr,
+                                                        (prune && lj->ps.pk_alloc) ? 1 : 0);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }

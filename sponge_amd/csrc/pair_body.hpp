@@ -84,16 +84,29 @@ __device__ __forceinline__ void excluded_partners(const ExclArgs& ex, const uint
     }
 }
 
+// 16-bit offset of slot j from slot i in the per-slot inner lists: j - i, or
+// the offset of j's periodic image (modulo the slot count nsl) when j - i
+// does not fit; false when neither fits. i + offset always indexes the slot
+// records or their copies kPkHalo to either side.
+__device__ __forceinline__ bool slot_offset(int i, int j, int nsl, int& off) {
+    int d = j - i;
+    if (d > 32767) d -= nsl;
+    else if (d < -32768) d += nsl;
+    off = d;
+    return d >= -32768 && d <= 32767;
+}
+
 // Per-slot inner lists, written on prune steps: the full list's pairs of
 // cluster ic within pa.r2 at this step's positions, per i slot in list order
 // (lanes ii, ii + 4, ... of a round in lane order). Returns the partner count
 // of slot lane >> 4 (the lane mapping of pair_atoms); a count above atom_cap
-// (partners past the cap dropped) sends the cluster to the full list until
-// the next prune. The list stores are complete and visible to the whole wave
-// on return.
+// (partners past the cap dropped, or a partner beyond 16-bit reach) sends the
+// cluster to the full list until the next prune. The list stores are
+// complete and visible to the whole wave on return.
 template <int TB>
-__device__ __forceinline__ int prune_slots(int ic, const int* __restrict__ list_n, const uint2* __restrict__ list,
-                                           int list_cap, const uint4* __restrict__ pk, float cx, float cy, float cz,
+__device__ __forceinline__ int prune_slots(int ic, int nsl, const int* __restrict__ list_n,
+                                           const uint2* __restrict__ list, int list_cap,
+                                           const uint4* __restrict__ pk, float cx, float cy, float cz,
                                            const PruneArgs& pa) {
     const int lane = threadIdx.x & 63;
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
@@ -102,9 +115,10 @@ __device__ __forceinline__ int prune_slots(int ic, const int* __restrict__ list_
     const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
     const int n = list_n[ic];
     const uint2* lst = list + (size_t)ic * list_cap;
-    int* out = pa.atom_list + (size_t)islot * pa.atom_cap;
+    unsigned short* out = pa.atom_list + (size_t)islot * pa.atom_cap;
     const unsigned long long sel = 0x1111111111111111ull << ii, below = (1ull << lane) - 1ull;
     int count = 0;
+    bool far = false;
     for (int base = 0; base < n; base += 64) {
         const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
         const int cnt = min(64, n - base);
@@ -132,11 +146,15 @@ __device__ __forceinline__ int prune_slots(int ic, const int* __restrict__ list_
                 const bool keep = ((ms[u] >> bit) & 1u) && dr2_exact(dx, dy, dz) < pa.r2;
                 const unsigned long long mi = __ballot(keep) & sel;
                 const int pos = count + __popcll(mi & below);
-                if (keep && pos < pa.atom_cap) out[pos] = jcs[u] * kClusterSize + jj;
+                int off = 0;
+                const bool reach = slot_offset(islot, jcs[u] * kClusterSize + jj, nsl, off);
+                far = far || (keep && !reach);
+                if (keep && pos < pa.atom_cap) out[pos] = (unsigned short)off;
                 count += __popcll(mi);
             }
         }
     }
+    if (__ballot(far) & sel) count = pa.atom_cap + 1;
     if (lane < kClusterSize) {
         pa.atom_n[islot] = count;
         if (count > pa.atom_cap) atomicOr(pa.diag, 1);
@@ -149,12 +167,15 @@ __device__ __forceinline__ int prune_slots(int ic, const int* __restrict__ list_
 }
 
 // Force-only steps of plain LJ from the per-slot inner lists: lane (i, k),
-// i = lane >> 4, takes partners k, k + 16, ... of atom i of the wave's
-// cluster (n of them; nmax the longest of the four lists), so every lane
-// holds a listed pair (the cluster form evaluates all 16 pairs of each listed
-// cluster pair, in cutoff or not). The pair terms are pair_body's, in FP32;
-// the partner index is fetched two rounds ahead and its record one round
-// ahead.
+// i = lane >> 4, takes partners 2k and 2k + 1 of every 32 of atom i of the
+// wave's cluster (n of them; nmax the longest of the four lists) from one
+// dword of two 16-bit slot offsets, so every lane holds listed pairs (the
+// cluster form evaluates all 16 pairs of each listed cluster pair, in cutoff
+// or not). The pair terms are pair_body's, in FP32; a dword of offsets is
+// fetched two rounds ahead and the first record of a round one round ahead.
+// Lanes past the end of a list read a record
+// too (offsets left from earlier prunes, or zero: always inside the records
+// and their copies) and are masked out.
 template <int TB>
 __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4* __restrict__ pk, const float2* sab,
                                            int nt, float cx, float cy, float cz, float rc2, float beta,
@@ -168,38 +189,49 @@ __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4*
     const int ti = TB == 2 ? (int)(a.x & 3u) : unpack_type(a);
     const float2* srow = sab + ti * nt;
     const float mlog2e_b2 = -1.4426950408889634f * (beta * beta);
-    const int* __restrict__ lst = pa.atom_list + (size_t)islot * pa.atom_cap;
+    const unsigned* __restrict__ lw =
+        reinterpret_cast<const unsigned*>(pa.atom_list + (size_t)islot * pa.atom_cap);
+    const int nwords = (nmax + 1) >> 1;  // atom_cap is even: the dword stays inside the slot's list
     float fix = 0.f, fiy = 0.f, fiz = 0.f;
-    int j1 = k + 16 < n ? lst[k + 16] : islot;
-    uint4 b = pk[k < n ? lst[k] : islot];
-    for (int t = 0; t < nmax; t += 16) {
-        const int j2 = t + 32 + k < n ? lst[t + 32 + k] : islot;
-        const uint4 bn = pk[j1];
-        // with at most 4 types the y and z words carry no type bits
-        const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
-        const float dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
-        const float dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
-        const float r2 = dr2_exact(dx, dy, dz);
-        const bool in = t + k < n && r2 < rc2;
-        // unused lanes (past the list's end: the atom itself) may give inf /
-        // NaN below; the final select discards them
-        const float rinv = __builtin_amdgcn_rsqf(r2);
-        const float rinv2 = rinv * rinv;
-        const float rinv6 = rinv2 * rinv2 * rinv2;
-        const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
-        const float2 AB = srow[tj];
-        const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
-        const float x = beta * (r2 * rinv);
-        const float ex_ = __builtin_amdgcn_exp2f(mlog2e_b2 * r2);
-        const float ecx = erfcx_poly(x);
-        const float qq = qi * __uint_as_float(b.w);
-        const float fc = qq * ex_ * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
-        const float f = in ? flj - fc : 0.f;
-        fix += f * dx;
-        fiy += f * dy;
-        fiz += f * dz;
-        b = bn;
-        j1 = j2;
+    // offsets two rounds ahead, the first record of a round one round ahead
+    unsigned w = k < nwords ? lw[k] : 0u;
+    unsigned wn = 16 + k < nwords ? lw[16 + k] : 0u;
+    uint4 b0 = pk[islot + (int)(short)(w & 0xffffu)];
+    for (int t = 0; t < nmax; t += 32) {
+        const int k2 = (t >> 1) + 32 + k;
+        const unsigned w2 = k2 < nwords ? lw[k2] : 0u;
+        const uint4 b1 = pk[islot + ((int)w >> 16)];
+        const uint4 bn = pk[islot + (int)(short)(wn & 0xffffu)];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint4 b = h ? b1 : b0;
+            // with at most 4 types the y and z words carry no type bits
+            const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
+            const float dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
+            const float dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
+            const float r2 = dr2_exact(dx, dy, dz);
+            const bool in = t + 2 * k + h < n && r2 < rc2;
+            // masked lanes may give inf / NaN below; the final select
+            // discards them
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float rinv2 = rinv * rinv;
+            const float rinv6 = rinv2 * rinv2 * rinv2;
+            const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
+            const float2 AB = srow[tj];
+            const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
+            const float x = beta * (r2 * rinv);
+            const float ex_ = __builtin_amdgcn_exp2f(mlog2e_b2 * r2);
+            const float ecx = erfcx_poly(x);
+            const float qq = qi * __uint_as_float(b.w);
+            const float fc = qq * ex_ * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+            const float f = in ? flj - fc : 0.f;
+            fix += f * dx;
+            fiy += f * dy;
+            fiz += f * dz;
+        }
+        b0 = bn;
+        w = wn;
+        wn = w2;
     }
     const int at = s2o[islot];
     if (ex.start && at >= 0) excluded_partners(ex, pk, at, k, 16, ax, ay, az, qi, cx, cy, cz, beta, fix, fiy, fiz);
@@ -265,7 +297,7 @@ __device__ __forceinline__ void pair_wave(
     }
     const bool slots = !SOFT && pa.atom_list != nullptr && pa.interval > 0;
     if (slots && !ENERGY) {
-        const int n = write_inner ? prune_slots<TB>(ic, list_n, list, list_cap, pk, cx, cy, cz, pa)
+        const int n = write_inner ? prune_slots<TB>(ic, flags[3], list_n, list, list_cap, pk, cx, cy, cz, pa)
                                   : pa.atom_n[ic * kClusterSize + (lane >> 4)];
         const int nmax = max(max(__shfl(n, 0), __shfl(n, 16)), max(__shfl(n, 32), __shfl(n, 48)));
         if (ic == 0 && lane == 0) atomicAdd(pa.diag + 1, 1);
@@ -295,6 +327,8 @@ __device__ __forceinline__ void pair_wave(
     const uint2* lst = (use_inner ? pa.in_list : list) + (size_t)ic * list_cap;
     uint2* inner_out = write_cluster ? pa.in_list + (size_t)ic * list_cap : nullptr;
     int inner_count = 0, atom_count = 0;
+    bool atom_far = false;
+    const int nsl = flags[3];
     for (int base = 0; base < n; base += 64) {
         // 64 list entries (16 iterations of 4 j-clusters) held one per lane.
         const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
@@ -328,8 +362,11 @@ __device__ __forceinline__ void pair_wave(
                     // lanes ii, ii + 4, ... append to slot ii's list in lane order
                     const unsigned long long mi = kb & (0x1111111111111111ull << ii);
                     const int pos = atom_count + __popcll(mi & ((1ull << lane) - 1ull));
+                    int off = 0;
+                    const bool reach = slot_offset(islot, jc * kClusterSize + jj, nsl, off);
+                    atom_far = atom_far || (keep && !reach);
                     if (keep && pos < pa.atom_cap)
-                        pa.atom_list[(size_t)islot * pa.atom_cap + pos] = jc * kClusterSize + jj;
+                        pa.atom_list[(size_t)islot * pa.atom_cap + pos] = (unsigned short)off;
                     atom_count += __popcll(mi);
                 }
             }
@@ -399,6 +436,7 @@ __device__ __forceinline__ void pair_wave(
         }
     }
     if (write_cluster && lane == 0) pa.in_n[ic] = inner_count;
+    if (write_atoms && (__ballot(atom_far) & (0x1111111111111111ull << ii))) atom_count = pa.atom_cap + 1;
     if (write_atoms && lane < kClusterSize) {
         pa.atom_n[islot] = atom_count;
         if (atom_count > pa.atom_cap) atomicOr(pa.diag, 1);
