@@ -69,7 +69,7 @@ def cpu_baseline(sys_, crd, vel, dt, refresh, beta, K, steps=200, warmup=20):
     }
 
 
-def time_md(mde, steps, warmup, refresh, pme_events=False):
+def time_md(mde, steps, warmup, refresh, pme_events=False, graphs=True):
     """Times `steps` MD steps three ways: eager launches, one hipGraph per
     kind of step (plain / with the list rebuild), and one hipGraph holding a
     whole refresh period (refresh - 1 plain steps and the rebuilding one;
@@ -151,6 +151,10 @@ def time_md(mde, steps, warmup, refresh, pme_events=False):
         out["refresh_graph" + tag] = replay(lambda k: g_block.replay() if k < periods else per_step(k - periods),
                                             periods + rest) / K
 
+    if not graphs:                       # eager only (profiling runs)
+        out["graph"] = out["refresh_graph"] = out["eager"]
+        out["md_steps_run"] = warmup + 2 * K
+        return out
     time_graphs("")
     n_graph_steps = 2 * K + max(warmup, refresh) + 3 * refresh
     out["md_steps_run"] = warmup + 2 * K + n_graph_steps
@@ -434,6 +438,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="sweep the launch shapes on this workload first and time the fastest (default: the "
                          "shapes the tuned table gives this atom count)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager launches only (for counter collection, which serialises graph replays)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy 5 times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -476,7 +482,7 @@ def main():
     pairs_in0 = eng.energy_parts()["pairs_in"]
     nl_pairs = eng.nl.status()["pairs"]
 
-    t = time_md(mde, args.steps, args.warmup, args.refresh)
+    t = time_md(mde, args.steps, args.warmup, args.refresh, graphs=not args.no_graph)
     eager_ms, graph_ms, block_ms, pair_ms = t["eager"], t["graph"], t["refresh_graph"], t["pair_kernel"]
     # the fastest of the ways of issuing the same steps
     ms = min(eager_ms, graph_ms, block_ms)

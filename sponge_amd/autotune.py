@@ -31,7 +31,10 @@ import time
 PAIR_WAVES = (2, 4, 8)
 CONV_FORMS = ((0, 256), (0, 512), (1, 256))          # (conv_split, conv_threads)
 STEP_FORMS = (0, 1)
-PRUNES = ((5, 0.5), (4, 0.5), (10, 0.5), (5, 0.25), (5, 1.0), (0, 0.0))   # (prune_div, margin / skin)
+# (prune_div, margin / skin): pruned every refresh / prune_div steps to
+# cutoff + margin * skin; (1, 1.0) keeps every listed pair, per slot, from
+# the step after each rebuild (no pruning steps but that one)
+PRUNES = ((5, 0.5), (4, 0.5), (10, 0.5), (5, 0.25), (5, 1.0), (2, 0.75), (2, 1.0), (1, 1.0), (0, 0.0))
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE_PATH = os.path.join(HERE, "csrc", "tune_table.hpp")
@@ -188,7 +191,7 @@ def main(argv=None):
     ap.add_argument("--configs", default="0,1,2", help="benchmark water boxes (system.config indices)")
     ap.add_argument("--refresh", type=int, default=20)
     ap.add_argument("--equilibrate", type=int, default=300)
-    ap.add_argument("--rounds", type=int, default=2)
+    ap.add_argument("--rounds", type=int, default=3)
     ap.add_argument("--periods", type=int, default=4)
     ap.add_argument("--md", default=None, help="write the measured table here (markdown)")
     ap.add_argument("--table", default=None, help="regenerate this tune_table.hpp from the winners")

@@ -140,6 +140,6 @@ def test_shape_candidates_cover_the_swept_axes():
     from sponge_amd import autotune
     c = autotune.candidates(20, 10.0, 2.0)
     assert len(c) == len(autotune.PAIR_WAVES) * len(autotune.CONV_FORMS) * 2 * len(autotune.PRUNES)
-    assert {x["prune_interval"] for x in c} == {4, 5, 2, 0}
+    assert {x["prune_interval"] for x in c} == {20, 10, 5, 4, 2, 0}
     assert all(x["prune_radius"] >= 10.0 for x in c)
     assert not any(x["conv_split"] == 0 for x in autotune.candidates(20, 10.0, 2.0, plane_fits=False))

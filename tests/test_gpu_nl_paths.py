@@ -168,6 +168,7 @@ def test_column_overflow_is_reported_and_defined():
     sys_ = _droplet()
     say(f"[case] column overflow droplet n={sys_.n} box={sys_.box.tolist()}")
     eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    eng.nl.set_prune(0)
     eng.update_neighbor_list(forced=1)
     st = eng.nl.status()
     say(f"    status {st}")
@@ -177,25 +178,30 @@ def test_column_overflow_is_reported_and_defined():
         eng.nl.check()
     with pytest.raises(nb.SpongeError):
         eng.nl.pairs()
-    eng.frc.fill_(1.0e3)
-    forces = []
-    for _ in range(3):
-        eng.force_step(0, 0, from_crd=True, overwrite=True)
-        torch.cuda.synchronize()
-        forces.append(eng.frc.cpu().numpy().copy())
-    f = forces[-1]
-    assert np.isfinite(f).all()
-    for g in forces[:-1]:
-        assert np.array_equal(g, f), "repeated steps at the same coordinates differ"
-    left = np.where(np.all(f == 0.0, axis=1))[0]
-    say(f"    atoms left out: {left.tolist()}")
-    assert 1 <= len(left) <= 8
-    keep = np.setdiff1d(np.arange(sys_.n), left)
-    sub = _subsystem(sys_, keep)
-    ref = oracle.full_reference(sub, beta=eng.pme.beta, K=(eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
-    e = rel_rms(f[keep], ref["frc"])
-    say(f"    slotted atoms vs the oracle without the left-out ones: force rel rms {e:.3e}")
-    assert e < 1e-5
+    # the full list every step, then the per-slot inner lists (pruned every
+    # 4 steps to rc + skin / 2)
+    for prune in (0, 4):
+        if prune:
+            eng.nl.set_prune(prune, float(sys_.cutoff) + 0.5 * float(sys_.skin))
+        eng.frc.fill_(1.0e3)
+        forces = []
+        for _ in range(3):
+            eng.force_step(0, 0, from_crd=True, overwrite=True)
+            torch.cuda.synchronize()
+            forces.append(eng.frc.cpu().numpy().copy())
+        f = forces[-1]
+        assert np.isfinite(f).all()
+        for g in forces[:-1]:
+            assert np.array_equal(g, f), "repeated steps at the same coordinates differ"
+        left = np.where(np.all(f == 0.0, axis=1))[0]
+        say(f"    prune {prune}: atoms left out: {left.tolist()}")
+        assert 1 <= len(left) <= 8
+        keep = np.setdiff1d(np.arange(sys_.n), left)
+        sub = _subsystem(sys_, keep)
+        ref = oracle.full_reference(sub, beta=eng.pme.beta, K=(eng.pme.fftx, eng.pme.ffty, eng.pme.fftz))
+        e = rel_rms(f[keep], ref["frc"])
+        say(f"    slotted atoms vs the oracle without the left-out ones: force rel rms {e:.3e}")
+        assert e < 1e-5
 
 
 def _subsystem(sys_, keep):
@@ -309,8 +315,8 @@ def test_prune_displacement_vote():
 
 
 def test_pruned_list_under_motion():
-    """The 23,570-atom box relaxed to ~300 K, then MD steps with the default
-    prune (every 5 steps to rc + 0.5 A plus displacement votes): at the last
+    """The 23,570-atom box relaxed to ~300 K, then MD steps with the tuned
+    default prune (sponge_nb_default_shapes, plus displacement votes): at the last
     step before a refresh the fused step's forces equal a prune-off
     evaluation, after a forced rebuild, at the same coordinates."""
     import torch

@@ -300,18 +300,17 @@ def test_graph_replay_matches_eager(case):
 
 
 def test_pruned_inner_list(case):
-    """Interval mode prunes the fused step's list to an inner radius (default
-    rc + skin / 2, every refresh / 5 steps); forces stay those of the full list,
-    also when pruned to exactly rc every step. The force-only steps of plain
-    LJ read the per-slot inner lists (all three steps here, the pruning one
-    included; soft-core systems keep the cluster form)."""
+    """Interval mode prunes the fused step's list to an inner radius (here
+    rc + skin / 2 every 4 steps, and exactly rc every step); forces stay those
+    of the full list. The force-only steps of plain LJ read the per-slot
+    inner lists (all three steps here, the pruning one included; soft-core
+    systems keep the cluster form)."""
     import torch
     from sponge_amd import nb
     name, sys_, eng, ref = case
     e2 = nb.NonbondedEngine(sys_, refresh_interval=20)
-    for interval, radius in ((None, None), (1, sys_.cutoff)):
-        if interval is not None:
-            e2.nl.set_prune(interval, radius)
+    for interval, radius in ((4, float(sys_.cutoff) + 0.5 * float(sys_.skin)), (1, sys_.cutoff)):
+        e2.nl.set_prune(interval, radius)
         e2.update_neighbor_list(forced=1)
         before = e2.nl.inner_status()
         for _ in range(3):
@@ -350,6 +349,7 @@ def test_per_slot_overflow_falls_back_to_the_full_list():
     sys_ = dataclasses.replace(base, box=(base.box.astype(np.float64) * 1.5).astype(np.float32))
     say(f"[case] droplet n={sys_.n} box={sys_.box.tolist()}")
     eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    eng.nl.set_prune(4, float(sys_.cutoff) + 0.5 * float(sys_.skin))
     eng.update_neighbor_list(forced=1)
     for _ in range(3):
         eng.force_step(0, 0, from_crd=True, overwrite=True)

@@ -100,6 +100,7 @@ def test_new_prune_setting_prunes_at_the_next_step():
     from sponge_amd import nb
     sys_ = system.config(0)
     eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    eng.nl.set_prune(4, float(sys_.cutoff) + 0.5 * float(sys_.skin))
     eng.update_neighbor_list(forced=1)
     f0 = steps(eng, 2)                                  # step 1 prunes (scheduled), step 2 reads
     v0 = eng.nl.prune_votes()
