@@ -39,6 +39,11 @@ constexpr int kClusterSize = 4;          // atoms per cluster (4x4 cluster pairs
 // the first and last kPkHalo slots): a 16-bit partner offset from any slot
 // then reads a record inside the allocation.
 constexpr int kPkHalo = 32768;
+// Intervals of the tabulated direct Coulomb force factor
+// G(r) = erfc(beta r) / r^3 + 2 beta / sqrt(pi) exp(-beta^2 r^2) / r^2 over
+// [0, rc]: one float4 of cubic Hermite coefficients each (16 KB, held in LDS
+// by the per-slot force loop).
+constexpr int kCoulTab = 1024;
 constexpr int kMaxLjTypes = 64;          // LDS pair table holds kMaxLjTypes^2 (A,B)
 constexpr int kNumXcd = 8;               // MI355X: 8 XCDs, workgroups dealt round-robin
 // Largest PME grid axis: sponge_pme_create rejects larger axes, and the prep
@@ -130,6 +135,9 @@ struct PairScratch {
     hipEvent_t ev_start, ev_stop;  // optional, recorded around the pair kernel
     int waves;                   // i-clusters (waves) per block of the force-only pair kernel: 2, 4 or 8
                                  // (0: 4)
+    float4* ctab;                // kCoulTab cubic pieces of the direct Coulomb force factor (per-slot
+                                 // force loop), built for (ctab_beta, ctab_rc)
+    float ctab_beta, ctab_rc;
 };
 
 // Mixed-radix pass list of one FFT axis (radix 8 first, then 4, 2, 3, 5, 7),
@@ -322,6 +330,8 @@ struct PruneArgs {
     int* diag = nullptr;        // flags + 11: [0] per-slot overflow, [1] per-slot launches,
                                 // [2] step of the last displacement vote, [4] voted prunes
     uint4* ref = nullptr;       // slot records at the last prune
+    const float4* ctab = nullptr;  // the Coulomb table (per-slot force loop), NULL: the formula
+    float ctab_scale = 0.f;        // intervals per Angstrom
 };
 
 // Excluded-pair correction folded into the pair kernel (start NULL: off):

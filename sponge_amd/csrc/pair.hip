@@ -601,8 +601,39 @@ static int scratch_ensure(PairScratch* ps, const sponge_nl* nl) {
     return SPONGE_NB_OK;
 }
 
+// Cubic Hermite pieces of the direct Coulomb force factor
+// G(r) = erfc(beta r) / r^3 + 2 beta / sqrt(pi) exp(-beta^2 r^2) / r^2 (force
+// on atom i from j: -q_i q_j G(r) r_ij, the PME direct term of
+// Lennard_Jones_force.cu:495-510) on kCoulTab intervals of [0, rc (1 + 1e-4)]:
+// piece k holds (G, h G', 3 (G1 - G0) - 2 h G0' - h G1', 2 (G0 - G1) + h G0' +
+// h G1') at its ends, so G(k h + u h) = c0 + u (c1 + u (c2 + u c3)). FP64 on
+// the host; the first piece (r below one interval, never a listed pair) is a
+// copy of the second. Built on first use and when beta or the cutoff changes
+// (a synchronous copy: not during graph capture).
+static int ensure_coulomb_table(PairScratch* ps, float beta, float rc) {
+    if (ps->ctab && ps->ctab_beta == beta && ps->ctab_rc == rc) return SPONGE_NB_OK;
+    if (!ps->ctab) SNB_HIP(hipMalloc((void**)&ps->ctab, kCoulTab * sizeof(float4)));
+    const double b = beta, rmax = (double)rc * 1.0001, h = rmax / kCoulTab, tsp = 2.0 * b / std::sqrt(M_PI);
+    auto G = [&](double r) { return std::erfc(b * r) / (r * r * r) + tsp * std::exp(-b * b * r * r) / (r * r); };
+    auto dG = [&](double r) {
+        return -3.0 * std::erfc(b * r) / (r * r * r * r) - tsp * std::exp(-b * b * r * r) * (3.0 / (r * r * r) + 2.0 * b * b / r);
+    };
+    std::vector<float4> t(kCoulTab);
+    for (int k = 1; k < kCoulTab; k++) {
+        const double r0 = k * h, r1 = r0 + h;
+        const double g0 = G(r0), g1 = G(r1), d0 = h * dG(r0), d1 = h * dG(r1);
+        t[k] = make_float4((float)g0, (float)d0, (float)(3.0 * (g1 - g0) - 2.0 * d0 - d1),
+                           (float)(2.0 * (g0 - g1) + d0 + d1));
+    }
+    t[0] = t[1];
+    SNB_HIP(hipMemcpy(ps->ctab, t.data(), kCoulTab * sizeof(float4), hipMemcpyHostToDevice));
+    ps->ctab_beta = beta;
+    ps->ctab_rc = rc;
+    return SPONGE_NB_OK;
+}
+
 void snb_scratch_free(PairScratch* ps) {
-    void* ptrs[] = {ps->pk_alloc, ps->acc};
+    void* ptrs[] = {ps->pk_alloc, ps->acc, ps->ctab};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     hipEvent_t e0 = ps->ev_start, e1 = ps->ev_stop;
@@ -647,7 +678,16 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     const int W = (!energy && !sc && (ps->waves == 2 || ps->waves == 8)) ? ps->waves : kPairWaves;
     const int nblk = xcd_grid(ceil_div(nl->ncl_cap, W));
     const float rc2 = cutoff * cutoff;
-    const size_t lds = (size_t)nt * nt * sizeof(float2);
+    size_t lds = (size_t)nt * nt * sizeof(float2);
+    // the per-slot force loop of force-only plain steps reads the Coulomb
+    // force factor from a table in LDS
+    if (inner && !energy && !sc) {
+        rc = ensure_coulomb_table(ps, beta, cutoff);
+        if (rc) return rc;
+        pa.ctab = ps->ctab;
+        pa.ctab_scale = (float)kCoulTab / (cutoff * 1.0001f);
+        lds = ((lds + 15) & ~(size_t)15) + kCoulTab * sizeof(float4);
+    }
     // With timing events set, the launch itself records them (hipExtLaunchKernel
     // takes the kernel's own start and end times; events recorded around a
     // launch add the marker packets' latency, ~8 us per launch here)

@@ -166,7 +166,9 @@ __device__ __forceinline__ int prune_slots(int ic, int nsl, const int* __restric
     return __shfl(count, lane >> 4);
 }
 
-// Force-only steps of plain LJ from the per-slot inner lists: lane (i, k),
+// Force-only steps of plain LJ from the per-slot inner lists (TABLE: the
+// Coulomb force factor from its cubic piece in LDS, else from the fitted
+// erfc): lane (i, k),
 // i = lane >> 4, takes partners 2k and 2k + 1 of every 32 of atom i of the
 // wave's cluster (n of them; nmax the longest of the four lists) from one
 // dword of two 16-bit slot offsets, so every lane holds listed pairs (the
@@ -176,11 +178,11 @@ __device__ __forceinline__ int prune_slots(int ic, int nsl, const int* __restric
 // Lanes past the end of a list read a record
 // too (offsets left from earlier prunes, or zero: always inside the records
 // and their copies) and are masked out.
-template <int TB>
+template <int TB, bool TABLE>
 __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4* __restrict__ pk, const float2* sab,
-                                           int nt, float cx, float cy, float cz, float rc2, float beta,
-                                           const int* __restrict__ s2o, float* __restrict__ frc, int overwrite,
-                                           const PruneArgs& pa, const ExclArgs& ex) {
+                                           const float4* stab, int nt, float cx, float cy, float cz, float rc2,
+                                           float beta, const int* __restrict__ s2o, float* __restrict__ frc,
+                                           int overwrite, const PruneArgs& pa, const ExclArgs& ex) {
     const int lane = threadIdx.x & 63, ii = lane >> 4, k = lane & 15;
     const int islot = ic * kClusterSize + ii;
     const uint4 a = pk[islot];
@@ -219,11 +221,21 @@ __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4*
             const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
             const float2 AB = srow[tj];
             const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
-            const float x = beta * (r2 * rinv);
-            const float ex_ = __builtin_amdgcn_exp2f(mlog2e_b2 * r2);
-            const float ecx = erfcx_poly(x);
             const float qq = qi * __uint_as_float(b.w);
-            const float fc = qq * ex_ * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+            float fc;
+            if (TABLE) {
+                // the force factor from its cubic piece on [k h, (k + 1) h]
+                const float t = (r2 * rinv) * pa.ctab_scale;
+                const unsigned kk = min((unsigned)t, (unsigned)(kCoulTab - 1));
+                const float u = t - (float)kk;
+                const float4 c = stab[kk];
+                fc = qq * fmaf(fmaf(fmaf(c.w, u, c.z), u, c.y), u, c.x);
+            } else {
+                const float x = beta * (r2 * rinv);
+                const float ex_ = __builtin_amdgcn_exp2f(mlog2e_b2 * r2);
+                const float ecx = erfcx_poly(x);
+                fc = qq * ex_ * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+            }
             const float f = in ? flj - fc : 0.f;
             fix += f * dx;
             fiy += f * dy;
@@ -269,7 +281,7 @@ __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4*
 // alone (two instructions fewer per pair); TB = 6: the general 6-bit index.
 template <bool ENERGY, bool SOFT, int TB>
 __device__ __forceinline__ void pair_wave(
-    int ic, const float2* sab, const int* __restrict__ flags, const int* __restrict__ list_n,
+    int ic, const float2* sab, const float4* stab, const int* __restrict__ flags, const int* __restrict__ list_n,
     const uint2* __restrict__ list, int list_cap, const uint4* __restrict__ pk, int nt, float cx, float cy, float cz,
     float rc2, float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
@@ -302,7 +314,12 @@ __device__ __forceinline__ void pair_wave(
         const int nmax = max(max(__shfl(n, 0), __shfl(n, 16)), max(__shfl(n, 32), __shfl(n, 48)));
         if (ic == 0 && lane == 0) atomicAdd(pa.diag + 1, 1);
         if (nmax <= pa.atom_cap) {
-            pair_atoms<TB>(ic, n, nmax, pk, sab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite, pa, ex);
+            if (stab)
+                pair_atoms<TB, true>(ic, n, nmax, pk, sab, stab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite, pa,
+                                     ex);
+            else
+                pair_atoms<TB, false>(ic, n, nmax, pk, sab, stab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite, pa,
+                                      ex);
             return;
         }
         write_inner = false;  // a slot overflowed: this cluster's forces from the full list
@@ -508,11 +525,17 @@ __device__ __forceinline__ void pair_body(
     ExclArgs ex = ExclArgs{nullptr, nullptr, nullptr}) {
     extern __shared__ float2 sab[];
     for (int k = threadIdx.x; k < nt * nt; k += blockDim.x) sab[k] = ab[k];
+    // the Coulomb table after the LJ table (16-byte aligned), when given
+    float4* stab = nullptr;
+    if (!ENERGY && !SOFT && pa.ctab) {
+        stab = reinterpret_cast<float4*>(reinterpret_cast<char*>(sab) + ((nt * nt * 8 + 15) & ~15));
+        for (int k = threadIdx.x; k < kCoulTab; k += blockDim.x) stab[k] = pa.ctab[k];
+    }
     __syncthreads();
     const int ic = lb * W + (int)(threadIdx.x >> 6);
     if (lb < 0 || ic >= flags[4]) return;
-    pair_wave<ENERGY, SOFT, TB>(ic, sab, flags, list_n, list, list_cap, pk, nt, cx, cy, cz, rc2, beta, s2o, frc,
-                                atom_e, atom_v, atom_d, acc, overwrite, sc, pa, ex);
+    pair_wave<ENERGY, SOFT, TB>(ic, sab, stab, flags, list_n, list, list_cap, pk, nt, cx, cy, cz, rc2, beta, s2o,
+                                frc, atom_e, atom_v, atom_d, acc, overwrite, sc, pa, ex);
 }
 
 }  // namespace snb
