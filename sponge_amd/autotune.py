@@ -65,6 +65,9 @@ def time_candidate(mde, refresh, shapes, periods=4):
     import torch
     eng = mde.nb
     eng.set_shapes(_shape_fields(shapes))
+    # one eager step first: whatever a shape needs on first use is made
+    # outside the capture
+    mde.step()
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
@@ -89,6 +92,7 @@ def sweep(mde, refresh, rounds=2, periods=4, cands=None, log=None):
     """Times every candidate `rounds` times (alternating order); returns
     [(us_per_step, candidate)] sorted, fastest first. The system keeps moving
     through the sweep; its list is rebuilt at the end of every period."""
+    from sponge_amd import nb
     eng = mde.nb
     if cands is None:
         cands = candidates(refresh, float(eng.sys.cutoff), float(eng.sys.skin), plane_fits=_plane_fits(eng))
@@ -98,7 +102,7 @@ def sweep(mde, refresh, rounds=2, periods=4, cands=None, log=None):
         for i in order:
             try:
                 us = time_candidate(mde, refresh, cands[i], periods)
-            except Exception as e:              # a shape the grid cannot take
+            except nb.SpongeError as e:         # a shape the grid cannot take
                 if log:
                     log(f"    skip {cands[i]}: {e}")
                 continue

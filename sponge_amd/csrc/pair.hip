@@ -652,6 +652,7 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
               int overwrite, const float4* sc, int inner, ExclArgs ex) {
     // the per-slot inner lists address records through 16-bit offsets,
     // which needs the record halo of an owned scratch
+    const bool fused = inner != 0;
     inner = inner && nl->prune_interval > 0 && ps->pk_alloc != nullptr;
     PruneArgs pa{};
     if (inner) {
@@ -680,10 +681,14 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     const float rc2 = cutoff * cutoff;
     size_t lds = (size_t)nt * nt * sizeof(float2);
     // the per-slot force loop of force-only plain steps reads the Coulomb
-    // force factor from a table in LDS
-    if (inner && !energy && !sc) {
+    // force factor from a table in LDS; every fused step makes sure it
+    // exists (built by the first one, so that a prune schedule turned on
+    // later allocates nothing inside a graph capture)
+    if (fused && !energy && !sc && ps->pk_alloc) {
         rc = ensure_coulomb_table(ps, beta, cutoff);
         if (rc) return rc;
+    }
+    if (inner && !energy && !sc) {
         pa.ctab = ps->ctab;
         pa.ctab_scale = (float)kCoulTab / (cutoff * 1.0001f);
         lds = ((lds + 15) & ~(size_t)15) + kCoulTab * sizeof(float4);
