@@ -146,8 +146,11 @@ __device__ __forceinline__ int prune_slots(int ic, int nsl, const int* __restric
                 const bool keep = ((ms[u] >> bit) & 1u) && dr2_exact(dx, dy, dz) < pa.r2;
                 const unsigned long long mi = __ballot(keep) & sel;
                 const int pos = count + __popcll(mi & below);
-                int off = 0;
-                const bool reach = slot_offset(islot, jcs[u] * kClusterSize + jj, nsl, off);
+                // below 32,768 slots every offset fits without the periodic
+                // image (a wave-uniform branch)
+                int off = jcs[u] * kClusterSize + jj - islot;
+                bool reach = true;
+                if (nsl > 32767) reach = slot_offset(islot, jcs[u] * kClusterSize + jj, nsl, off);
                 far = far || (keep && !reach);
                 if (keep && pos < pa.atom_cap) out[pos] = (unsigned short)off;
                 count += __popcll(mi);
