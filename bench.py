@@ -212,20 +212,34 @@ def scale_bench(steps: int, warmup: int, refresh: int, equilibrate: int):
 def fep_batch_bench(steps: int, warmup: int, refresh: int):
     """Batched FEP: 16 soft-core lambda windows of one solvated 12-atom solute
     plus 8 replicas of 1.4k-12k atoms (cutoff 9 A, skin 2 A, PME ~1 A grids),
-    stepped (a) one system after another with per-system launches and (b) as
-    one batch, one launch per stage, eagerly and (c) replayed from one
-    hipGraph per MD step (one graph for steps with the list refresh, one
-    without). Returns window-steps/s of each."""
+    every system taking whole NVE MD steps (force step, leap-frog + SETTLE,
+    list update every `refresh` steps): (a) one system after another, each
+    with its own tuned single-system step, issued eagerly and replayed from
+    each system's own step graphs, (b) as one batch, one launch per stage,
+    eagerly and (c) replayed from one hipGraph per MD step (one graph for
+    steps with the list refresh, one without). Returns window-steps/s of each
+    and the batch's force deviation from the single-system step at the same
+    coordinates."""
     import torch
-    from sponge_amd import nb, system
+    from sponge_amd import md, system
 
     systems = system.fep_batch()
-    engines = [nb.NonbondedEngine(s, refresh_interval=refresh) for s in systems]
-    stream = torch.cuda.current_stream()
+    engines = [md.MDEngine(s, refresh_interval=refresh, seed=7 + k) for k, s in enumerate(systems)]
     for e in engines:
-        e.update_neighbor_list(forced=1)
-        e.force_step(0, 0, from_crd=True, overwrite=True)
+        e.prepare()
+        e.equilibrate(100)          # the lattice start's strain heat rescaled away
+    # same accuracy: the batch's forces against each system's own step
+    for e in engines:
+        e.force()
     torch.cuda.synchronize()
+    own = [e.nb.frc.cpu().numpy().astype(np.float64) for e in engines]
+    bmd = md.BatchedMD(engines)
+    bmd.prepare()
+    bmd.batch.force_step(0, 0)
+    torch.cuda.synchronize()
+    dev = max(float(np.sqrt(((e.nb.frc.cpu().numpy() - f) ** 2).sum() / (f ** 2).sum()))
+              for e, f in zip(engines, own))
+    stream = torch.cuda.current_stream()
 
     def timed(fn, n):
         for k in range(warmup):
@@ -242,46 +256,58 @@ def fep_batch_bench(steps: int, warmup: int, refresh: int):
 
     def seq_step(k):
         for e in engines:
-            e.force_step(0, 0, from_crd=True, overwrite=True)
-            e.update_neighbor_list()
+            e.step(forced_update=1 if k % refresh == 0 else 0)
 
+    for e in engines:
+        e.prepare()
     t_seq = timed(seq_step, steps)
-    batch = nb.BatchedSystems(engines)
-    batch.neighbor_list_update(forced=1)
+    # the same per-system steps replayed from each system's own two graphs,
+    # one system after another (no host issue cost)
+    seq_graphs = []
+    for e in engines:
+        gp, gr = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gp):
+            e.step(stream=torch.cuda.current_stream(), forced_update=0)
+        with torch.cuda.graph(gr):
+            e.step(stream=torch.cuda.current_stream(), forced_update=1)
+        seq_graphs.append((gp, gr))
+
+    def seq_graph_step(k):
+        for gp, gr in seq_graphs:
+            (gr if k % refresh == 0 else gp).replay()
+
+    t_seq_graph = timed(seq_graph_step, steps)
+    bmd.prepare()
 
     def batch_step(k):
-        batch.force_step(0, 0)
-        batch.neighbor_list_update()
+        bmd.step(forced_update=1 if k % refresh == 0 else 0)
 
     t_batch = timed(batch_step, steps)
-
-    # one graph per MD step: with and without the list rebuild
-    g_plain, g_rebuild = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
-    batch.force_step(0, 0)
-    torch.cuda.synchronize()
-    with torch.cuda.graph(g_plain):
-        batch.force_step(0, 0, stream=torch.cuda.current_stream())
-    with torch.cuda.graph(g_rebuild):
-        batch.force_step(0, 0, stream=torch.cuda.current_stream())
-        batch.neighbor_list_update(forced=1, stream=torch.cuda.current_stream())
+    g_plain, g_rebuild = bmd.capture()
 
     def graph_step(k):
         (g_rebuild if k % refresh == 0 else g_plain).replay()
 
     t_graph = timed(graph_step, steps)
+    temps = [e.temperature() for e in engines]
     nsys = len(systems)
     ws = lambda t: round(nsys * steps / t, 1)
     return {
-        "metric": "window-steps/s, 16 soft-core FEP windows + 8 replicas",
+        "metric": "window-steps/s (whole NVE MD steps), 16 soft-core FEP windows + 8 replicas",
         "systems": nsys,
         "atoms": int(sum(s.n for s in systems)),
         "sizes": [int(s.n) for s in systems],
         "steps": steps,
+        "refresh": refresh,
         "sequential": ws(t_seq),
+        "sequential_graph": ws(t_seq_graph),
         "batched": ws(t_batch),
         "batched_graph": ws(t_graph),
         "speedup_batched": round(t_seq / t_batch, 3),
         "speedup_batched_graph": round(t_seq / t_graph, 3),
+        "speedup_batched_graph_vs_sequential_graph": round(t_seq_graph / t_graph, 3),
+        "batch_vs_single_force_rel_rms_max": float(f"{dev:.3g}"),
+        "final_temperature_K_range": [round(min(temps), 1), round(max(temps), 1)],
     }
 
 

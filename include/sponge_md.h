@@ -85,6 +85,34 @@ int sponge_md_iteration(sponge_settle* s, int atom_numbers, float* d_vel, float*
                         float max_velocity, int need_pressure, float* d_pressure, void* stream);
 
 /* ------------------------------------------------------------------ */
+/* Batched integrator. No counterpart in the reference, which runs each FEP
+ * window or replica as its own process (main.cu:282-320 per process): per
+ * system it is sponge_md_iteration, for every system of a sponge_batch in
+ * ONE launch, so a batched MD step is sponge_batch_force_step (FROM_CRD |
+ * OVERWRITE_FORCE) + sponge_batch_md_iteration + sponge_batch_neighbor_
+ * list_update, each one launch per stage, and capturable as one hipGraph.
+ * The coordinates, forces, fixed-point coordinates and box of each system
+ * are the batch's (sponge_batch_io, its handles' box). */
+typedef struct {
+    float* vel;
+    float* acc;                     /* may be NULL */
+    const float* inverse_mass;
+    const float* mass;              /* may be NULL for a system without SETTLE */
+    float dt;
+    float max_velocity;
+} sponge_batch_md_io;
+
+/* Names each system's SETTLE handle (settle may be NULL, or settle[k] NULL:
+ * plain leap-frog of every atom of system k) and integrator arrays; a SETTLE
+ * handle must have the system's atom count and stay alive with the batch.
+ * Replaces an earlier call's setting; on an error the batch keeps it.
+ * sponge_batch_sync re-reads the SETTLE handles' factors
+ * (sponge_settle_set_factors) and the boxes. Allocates; host-synchronous. */
+int sponge_batch_set_integrator(sponge_batch* batch, sponge_settle* const* settle, const sponge_batch_md_io* io);
+/* sponge_md_iteration (need_pressure 0) of every system, one launch. */
+int sponge_batch_md_iteration(sponge_batch* batch, void* stream);
+
+/* ------------------------------------------------------------------ */
 /* Pressure and volume coupling. */
 
 /* MD_INFORMATION::system_information::Get_Pressure (MD_core/MD_core.cu:

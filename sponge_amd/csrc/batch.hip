@@ -41,6 +41,9 @@ static void batch_free(sponge_batch* b) {
     if (b->d) (void)hipFree(b->d);
     for (int st = 0; st < kNumStages; st++)
         if (b->map[st]) (void)hipFree(b->map[st]);
+    if (b->d_md) (void)hipFree(b->d_md);
+    if (b->md_map) (void)hipFree(b->md_map);
+    if (b->d_ident) (void)hipFree(b->d_ident);
     delete b;
 }
 
@@ -63,7 +66,7 @@ extern "C" int sponge_batch_sync(sponge_batch* b) {
         v.lrc = lj->lrc_factor / lj->volume;
     }
     SNB_HIP(hipMemcpy(b->d, b->h.data(), b->nsys * sizeof(BatchSys), hipMemcpyHostToDevice));
-    return SPONGE_NB_OK;
+    return b->d_md ? snb_batch_md_sync(b) : SPONGE_NB_OK;
 }
 
 extern "C" int sponge_batch_create(int nsys, sponge_nl* const* nl, sponge_lj* const* lj, sponge_pme* const* pme,

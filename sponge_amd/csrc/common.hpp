@@ -8,6 +8,7 @@
 #include <cstdio>
 
 #include "../../include/sponge_nb.h"
+#include "../../include/sponge_md.h"
 
 namespace snb {
 
@@ -316,6 +317,15 @@ struct sponge_batch {
     int max_nt, soft;
     size_t lds_pair, lds_fwd, lds_conv, lds_inv;
     int refresh_interval, refresh_count;
+    // integrator (sponge_batch_set_integrator): handles and arrays per
+    // system, their device records, and the block map of its launch
+    std::vector<sponge_settle*> settle;
+    std::vector<sponge_batch_md_io> md_io;
+    void* d_md = nullptr;
+    int2* md_map = nullptr;
+    int md_blocks = 0;
+    int* d_ident = nullptr;     // 0 .. n-1: the atom list of systems without rigid groups
+    int ident_n = 0;
 };
 
 // Inner-list arguments of the pair kernel (interval 0: no inner list).
@@ -386,3 +396,4 @@ int snb_pme_prep(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d
 int snb_pme_adjoint_grid(sponge_pme* pme, const float4* wv, const float* scale, hipStream_t s);
 int snb_pme_adj_gather(sponge_pme* pme, int mode, const float4* wv, float sign, float* out, hipStream_t s);
 int snb_batch_nl(sponge_batch* b, int forced, hipStream_t s);
+int snb_batch_md_sync(sponge_batch* b);

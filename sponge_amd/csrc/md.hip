@@ -13,6 +13,7 @@
 // fused iteration does the reference's Remember_Last_Coordinates, Leap_Frog
 // and Do_SETTLE in one pass over each group: the three atoms of a water are
 // read once and written once, instead of five kernel passes over the arrays.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -277,14 +278,13 @@ __global__ void settle_do_kernel(int ntri, int npair, const SettleTriangle* tri,
     add_pressure(vir, need_pressure, pfac, pressure);
 }
 
-// Remember + leap-frog + SETTLE in one pass; groups: [triangles | pairs |
-// unconstrained atoms].
-__global__ void md_iteration_kernel(int ntri, int npair, int nfree, const SettleTriangle* tri, const SettlePair* prs,
-                                    const int* free_atoms, const unsigned* u, F3 scaler, const float* im,
-                                    const float* mass, float* crd, float* vel, const float* frc, float* acc, F3 L,
-                                    float dt, float vmax, ConstraintFactors cf, int need_pressure, float pfac,
-                                    float* pressure) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+// Remember + leap-frog + SETTLE in one pass for group g of [triangles |
+// pairs | unconstrained atoms]; returns the group's constraint virial.
+__device__ __forceinline__ float md_group(int g, int ntri, int npair, int nfree, const SettleTriangle* tri,
+                                          const SettlePair* prs, const int* free_atoms, const unsigned* u, F3 scaler,
+                                          const float* im, const float* mass, float* crd, float* vel,
+                                          const float* frc, float* acc, F3 L, float dt, float vmax,
+                                          ConstraintFactors cf) {
     float vir = 0.f;
     if (g < ntri) {
         const SettleTriangle t = tri[g];
@@ -318,7 +318,49 @@ __global__ void md_iteration_kernel(int ntri, int npair, int nfree, const Settle
         st3(vel, i, v);
         st3(crd, i, x);
     }
+    return vir;
+}
+
+__global__ void md_iteration_kernel(int ntri, int npair, int nfree, const SettleTriangle* tri, const SettlePair* prs,
+                                    const int* free_atoms, const unsigned* u, F3 scaler, const float* im,
+                                    const float* mass, float* crd, float* vel, const float* frc, float* acc, F3 L,
+                                    float dt, float vmax, ConstraintFactors cf, int need_pressure, float pfac,
+                                    float* pressure) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const float vir = md_group(g, ntri, npair, nfree, tri, prs, free_atoms, u, scaler, im, mass, crd, vel, frc, acc,
+                               L, dt, vmax, cf);
     add_pressure(vir, need_pressure, pfac, pressure);
+}
+
+// One system's integrator record in a batch: its rigid groups and arrays
+// (the coordinates, forces, fixed-point coordinates and box come from the
+// batch's own system record).
+struct MdSys {
+    const SettleTriangle* tri;
+    const SettlePair* prs;
+    const int* free_atoms;
+    int ntri, npair, nfree;
+    float* vel;
+    float* acc;
+    const float* im;
+    const float* mass;
+    float dt, vmax;
+    ConstraintFactors cf;
+};
+
+// md_iteration_kernel over every system of a batch: block -> {system, block
+// within the system} from the map, the system's box from its PME record.
+__global__ __launch_bounds__(256) void batch_md_iteration_kernel(const int2* __restrict__ map,
+                                                                 const BatchSys* __restrict__ bs,
+                                                                 const MdSys* __restrict__ ms) {
+    const int2 e = map[blockIdx.x];
+    if (e.x < 0) return;
+    const MdSys& m = ms[e.x];
+    const BatchSys& b = bs[e.x];
+    const F3 L{b.pme.box[0], b.pme.box[1], b.pme.box[2]};
+    const F3 scaler{L.x / 4294967296.0f, L.y / 4294967296.0f, L.z / 4294967296.0f};
+    (void)md_group(e.y * blockDim.x + threadIdx.x, m.ntri, m.npair, m.nfree, m.tri, m.prs, m.free_atoms,
+                   b.io.uint_crd, scaler, m.im, m.mass, b.io.crd, m.vel, b.io.frc, m.acc, L, m.dt, m.vmax, m.cf);
 }
 
 constexpr int kMdThreads = 256;
@@ -624,6 +666,106 @@ extern "C" int sponge_md_iteration(sponge_settle* s, int atom_numbers, float* d_
         s->ntri, s->npair, s->nfree, s->d_tri, s->d_pair, s->d_free, d_uint_crd, scaler, d_inverse_mass, d_mass,
         d_crd, d_vel, d_frc, d_acc, F3{box_length[0], box_length[1], box_length[2]}, dt, max_velocity,
         ConstraintFactors{s->dt, s->vfac, s->xfac}, need_pressure, pressure_factor(box_length), d_pressure);
+    SNB_CHECK_LAUNCH();
+    return SPONGE_NB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched integrator: the systems' MdSys records follow their SETTLE handles
+// and arrays; the launch map gives each system ceil(groups / 256) blocks.
+
+int snb_batch_md_sync(sponge_batch* b) {
+    std::vector<MdSys> h(b->nsys);
+    for (int s = 0; s < b->nsys; s++) {
+        const sponge_settle* st = b->settle[s];
+        const sponge_batch_md_io& io = b->md_io[s];
+        MdSys& m = h[s];
+        if (st) {
+            m.tri = st->d_tri;
+            m.prs = st->d_pair;
+            m.free_atoms = st->d_free;
+            m.ntri = st->ntri;
+            m.npair = st->npair;
+            m.nfree = st->nfree;
+            m.cf = ConstraintFactors{st->dt, st->vfac, st->xfac};
+        } else {
+            m.tri = nullptr;
+            m.prs = nullptr;
+            m.free_atoms = b->d_ident;
+            m.ntri = m.npair = 0;
+            m.nfree = b->nl[s]->n;
+            m.cf = ConstraintFactors{io.dt, 1.f, 1.f};
+        }
+        m.vel = io.vel;
+        m.acc = io.acc;
+        m.im = io.inverse_mass;
+        m.mass = io.mass;
+        m.dt = io.dt;
+        m.vmax = io.max_velocity;
+    }
+    SNB_HIP(hipMemcpy(b->d_md, h.data(), h.size() * sizeof(MdSys), hipMemcpyHostToDevice));
+    return SPONGE_NB_OK;
+}
+
+extern "C" int sponge_batch_set_integrator(sponge_batch* b, sponge_settle* const* settle,
+                                           const sponge_batch_md_io* io) {
+    SNB_ARG(b && io, "batch_set_integrator");
+    std::vector<int2> map;
+    int plain_n = 0;
+    for (int s = 0; s < b->nsys; s++) {
+        const sponge_settle* st = settle ? settle[s] : nullptr;
+        const int n = b->nl[s]->n;
+        SNB_ARG(io[s].vel && io[s].inverse_mass && io[s].dt > 0.f && io[s].max_velocity >= 0.f,
+                "batch_set_integrator: system arrays or step");
+        SNB_ARG(!st || st->n == n, "batch_set_integrator: a SETTLE handle's atom count differs from its system's");
+        SNB_ARG(!st || st->ntri + st->npair == 0 || io[s].mass, "batch_set_integrator: SETTLE needs the masses");
+        const int groups = st ? st->ntri + st->npair + st->nfree : n;
+        if (!st) plain_n = std::max(plain_n, n);
+        for (int j = 0; j < launch_blocks(groups); j++) map.push_back(make_int2(s, j));
+    }
+    // new buffers first: on an error the batch keeps its previous setting
+    int2* d_map = nullptr;
+    void* d_md = nullptr;
+    int* d_ident = nullptr;
+    bool ok = hipMalloc((void**)&d_md, b->nsys * sizeof(MdSys)) == hipSuccess;
+    if (ok && !map.empty())
+        ok = hipMalloc((void**)&d_map, map.size() * sizeof(int2)) == hipSuccess &&
+             hipMemcpy(d_map, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && plain_n > b->ident_n) {
+        std::vector<int> iota(plain_n);
+        for (int i = 0; i < plain_n; i++) iota[i] = i;
+        ok = hipMalloc((void**)&d_ident, plain_n * sizeof(int)) == hipSuccess &&
+             hipMemcpy(d_ident, iota.data(), plain_n * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!ok) {
+        (void)hipFree(d_map);
+        (void)hipFree(d_md);
+        (void)hipFree(d_ident);
+        set_error("batch_set_integrator: device allocation or upload failed");
+        return SPONGE_NB_EHIP;
+    }
+    (void)hipFree(b->d_md);
+    (void)hipFree(b->md_map);
+    b->d_md = d_md;
+    b->md_map = d_map;
+    b->md_blocks = (int)map.size();
+    if (d_ident) {
+        (void)hipFree(b->d_ident);
+        b->d_ident = d_ident;
+        b->ident_n = plain_n;
+    }
+    b->settle.assign(b->nsys, nullptr);
+    if (settle) b->settle.assign(settle, settle + b->nsys);
+    b->md_io.assign(io, io + b->nsys);
+    return snb_batch_md_sync(b);
+}
+
+extern "C" int sponge_batch_md_iteration(sponge_batch* b, void* stream) {
+    SNB_ARG(b, "batch_md_iteration");
+    SNB_ARG(b->d_md, "batch_md_iteration: no integrator (sponge_batch_set_integrator)");
+    if (b->md_blocks == 0) return SPONGE_NB_OK;
+    batch_md_iteration_kernel<<<b->md_blocks, kMdThreads, 0, as_stream(stream)>>>(b->md_map, b->d,
+                                                                                 (const MdSys*)b->d_md);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;
 }

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import system
-from .nb import _FP3, _I, _F, _P, NonbondedEngine, _check, _ptr, _stream, _vec3, load_library
+from .nb import _FP3, _I, _F, _P, BatchedSystems, NonbondedEngine, _check, _ptr, _stream, _vec3, load_library
 
 
 def leap_frog(atom_numbers, vel, crd, frc, acc, inverse_mass, dt, max_velocity=0.0, stream=None):
@@ -336,3 +336,74 @@ class MDEngine:
     def potential_energy(self):
         p = self.nb.energy_parts()
         return p["lj"] + p["lrc"] + p["direct"] + p["correction"] + p["self"] + p["reciprocal"]
+
+
+class _BatchMdIo(ctypes.Structure):
+    """sponge_batch_md_io (include/sponge_md.h)."""
+    _fields_ = [("vel", _P), ("acc", _P), ("inverse_mass", _P), ("mass", _P), ("dt", _F),
+                ("max_velocity", _F)]
+
+
+class BatchedMD:
+    """MD steps of independent systems (FEP lambda windows, replicas of
+    uneven size) together: each stage of `MDEngine.step` -- the fused force
+    step, the leap-frog + SETTLE iteration and the neighbor-list update -- is
+    ONE launch over every system (sponge_batch_force_step,
+    sponge_batch_md_iteration, sponge_batch_neighbor_list_update), on the
+    engines' own handles and device arrays. Per system the arithmetic is the
+    batch's force step and sponge_md_iteration; the engines must share the
+    list refresh interval. NVE only (no barostat, no pressure)."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        k = len(self.engines)
+        self.batch = BatchedSystems([e.nb for e in self.engines])
+        settle = (_P * k)(*[e.settle.h if e.settle is not None else None for e in self.engines])
+        io = (_BatchMdIo * k)()
+        for i, e in enumerate(self.engines):
+            io[i] = _BatchMdIo(_ptr(e.vel), _ptr(e.acc), _ptr(e.inverse_mass), _ptr(e.mass), float(e.dt),
+                               float(e.max_velocity))
+        self._keep = (settle, io)
+        _check(load_library().sponge_batch_set_integrator(self.batch.h, ctypes.cast(settle, _P),
+                                                          ctypes.cast(io, _P)), "sponge_batch_set_integrator")
+        self.refresh_interval = self.engines[0].nb.refresh_interval if self.engines else 0
+        self.steps = 0
+
+    def prepare(self):
+        """Main_Initial's forced list build of every system; raises if a list
+        overflowed a capacity."""
+        self.batch.neighbor_list_update(forced=1)
+        for e in self.engines:
+            e.nb.nl.check()
+
+    def iterate(self, stream=None):
+        _check(load_library().sponge_batch_md_iteration(self.batch.h, _stream(stream)),
+               "sponge_batch_md_iteration")
+
+    def step(self, stream=None, forced_update=None):
+        """One MD step of every system. forced_update None: the lists'
+        interval rule (host counter, as sponge_batch_neighbor_list_update);
+        1: rebuild now; 0: no list update (graph capture of a plain step)."""
+        self._launch(stream, forced_update)
+        self.steps += 1
+
+    def _launch(self, stream, forced_update):
+        self.batch.force_step(0, 0, from_crd=True, overwrite=True, stream=stream)
+        self.iterate(stream)
+        if forced_update is None:
+            self.batch.neighbor_list_update(stream=stream)
+        elif forced_update:
+            self.batch.neighbor_list_update(forced=1, stream=stream)
+
+    def capture(self):
+        """One hipGraph per MD step form: (plain step, step with the list
+        rebuild). Replay the second on the steps where `step()` would
+        rebuild (every refresh_interval-th update, the first included)."""
+        import torch
+        plain, rebuild = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(plain):
+            self._launch(torch.cuda.current_stream(), 0)
+        with torch.cuda.graph(rebuild):
+            self._launch(torch.cuda.current_stream(), 1)
+        return plain, rebuild

@@ -86,6 +86,8 @@ _SIGS = {
     "sponge_batch_sync": (_I, [_P]),
     "sponge_batch_neighbor_list_update": (_I, [_P, _I, _P]),
     "sponge_batch_force_step": (_I, [_P, _I, _I, _I, _P]),
+    "sponge_batch_set_integrator": (_I, [_P, _P, _P]),
+    "sponge_batch_md_iteration": (_I, [_P, _P]),
     # include/sponge_md.h
     "sponge_md_leap_frog": (_I, [_I, _P, _P, _P, _P, _P, _F, _F, _P]),
     "sponge_settle_create": (_I, [_I, _I, _P, _P, _P, _P, _F, _F, _I, ctypes.POINTER(_P)]),
