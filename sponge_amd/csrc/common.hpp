@@ -264,6 +264,8 @@ struct sponge_pme {
     unsigned* base;              // n grid base indices, x | y << 10 | z << 20
     int* bin_count;              // K0 atoms filed under each x base plane
     int* bin_atoms;              // K0 * bin_cap atom indices
+    unsigned* bin_base;          // K0 * bin_cap: the filed atoms' base and
+    float4* bin_frac;            //   fraction records, in bin order
     int bin_cap;
     float4* frac;                // n fractional parts
     float* corr_atom;            // n correction energies

@@ -72,6 +72,13 @@ __host__ __device__ __forceinline__ int padded_len(int K) { return pd(K - 1) + 1
 __host__ __device__ __forceinline__ int pitch16(int n) { return ((n + 14) / 16) * 16 + 1; }
 // pitch of the packed z-lines and of the y direction of the convolution plane
 __host__ __device__ __forceinline__ int line_pitch(int K) { return pitch16(padded_len(K)); }
+// LDS offset of point idx = x Ky + y of a padded plane (row pitch `pitch`);
+// rky = 1 / Ky. 24-bit multiplies (full rate; every index here < 2^24).
+__device__ __forceinline__ int plane_lds(int idx, float rky, int Ky, int pitch) {
+    const int xx = (int)(((float)idx + 0.5f) * rky);
+    const int y = idx - (int)__umul24((unsigned)xx, (unsigned)Ky);
+    return (int)__umul24((unsigned)pd(xx), (unsigned)pitch) + pd(y);
+}
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -184,30 +191,54 @@ __device__ __forceinline__ void wave_pass(float2* buf, int nlines, int ls, int k
             t[q - 1] = tw[q * k * tws];
             if (SIGN > 0) t[q - 1].y = -t[q - 1].y;
         }
+        // the lane's element offsets within a line, the same for every line
+        // of the pass (24-bit multiplies: full rate, every index < 2^24)
+        int oin[R], oout[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            oin[q] = (int)__umul24((unsigned)pd(j + (int)__umul24(q, per)), (unsigned)ks);
+            oout[q] = (int)__umul24((unsigned)pd(d + (int)__umul24(q, Ns)), (unsigned)ks);
+        }
         for (int i0 = 0; wid + nw * i0 < nlines; i0 += 2 * G) {
             const int la = wid + nw * (i0 + sub), lb = la + nw * G;
             const bool oka = sub < G && la < nlines, okb = sub < G && lb < nlines;
-            const int ra = pd(oka ? la : 0) * ls, rb = pd(okb ? lb : 0) * ls;
+            const int ra = (int)__umul24((unsigned)pd(oka ? la : 0), (unsigned)ls);
+            const int rb = (int)__umul24((unsigned)pd(okb ? lb : 0), (unsigned)ls);
             float2 va[R], vb[R];
+            if (wid + nw * (i0 + G) < nlines) {
+                // some lane of the wave has a line in the second group
 #pragma unroll
-            for (int q = 0; q < R; q++) {
-                va[q] = buf[ra + pd(j + q * per) * ks];
-                vb[q] = buf[rb + pd(j + q * per) * ks];
-            }
+                for (int q = 0; q < R; q++) {
+                    va[q] = buf[ra + oin[q]];
+                    vb[q] = buf[rb + oin[q]];
+                }
 #pragma unroll
-            for (int q = 1; q < R; q++) {
-                va[q] = cmul(va[q], t[q - 1]);
-                vb[q] = cmul(vb[q], t[q - 1]);
-            }
-            dft<R, SIGN>(va);
-            dft<R, SIGN>(vb);
-            if (oka) {
+                for (int q = 1; q < R; q++) {
+                    va[q] = cmul(va[q], t[q - 1]);
+                    vb[q] = cmul(vb[q], t[q - 1]);
+                }
+                dft<R, SIGN>(va);
+                dft<R, SIGN>(vb);
+                if (oka) {
 #pragma unroll
-                for (int q = 0; q < R; q++) buf[ra + pd(d + q * Ns) * ks] = va[q];
-            }
-            if (okb) {
+                    for (int q = 0; q < R; q++) buf[ra + oout[q]] = va[q];
+                }
+                if (okb) {
 #pragma unroll
-                for (int q = 0; q < R; q++) buf[rb + pd(d + q * Ns) * ks] = vb[q];
+                    for (int q = 0; q < R; q++) buf[rb + oout[q]] = vb[q];
+                }
+            } else {
+                // first group only (few lines: the z lines of a row block,
+                // or 8 waves on a 64-line plane)
+#pragma unroll
+                for (int q = 0; q < R; q++) va[q] = buf[ra + oin[q]];
+#pragma unroll
+                for (int q = 1; q < R; q++) va[q] = cmul(va[q], t[q - 1]);
+                dft<R, SIGN>(va);
+                if (oka) {
+#pragma unroll
+                    for (int q = 0; q < R; q++) buf[ra + oout[q]] = va[q];
+                }
             }
         }
     } else {
@@ -346,12 +377,19 @@ __device__ __forceinline__ float bsp_dw(float f, int k) {
 // on the device so that no contribution reaches 2^31).
 template <int MODE = 0, int RM = kRadixAll>
 __device__ __forceinline__ void spread_z_body(
-    int x, int yb, const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
-    const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz,
-    const FftPlan& pz, const float2* __restrict__ twz, double* __restrict__ acc, const float4* __restrict__ wv = nullptr,
-    const float* __restrict__ scale_ptr = nullptr) {
+    int x, int yb, const unsigned* __restrict__ rec_base, const float4* __restrict__ rec_frac,
+    const int* __restrict__ bin_count, const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx,
+    int Ky, int Kz, const FftPlan& pz, const float2* __restrict__ twz, double* __restrict__ acc,
+    const float4* __restrict__ wv = nullptr, const float* __restrict__ scale_ptr = nullptr) {
     __builtin_amdgcn_s_setprio(kPmeWavePriority);
     extern __shared__ float2 sm[];
+    // the four bins' counts are read first: their latency overlaps the
+    // twiddle loads and the clearing of the accumulators
+    int b0 = x, b1 = x + 1, b2 = x + 2, b3 = x + 3;
+    b1 -= b1 >= Kx ? Kx : 0;
+    b2 -= b2 >= Kx ? Kx : 0;
+    b3 -= b3 >= Kx ? Kx : 0;
+    const int c0 = bin_count[b0], c1 = bin_count[b1], c2 = bin_count[b2], c3 = bin_count[b3];
     const float unit = MODE ? scale_ptr[0] : kSpreadScale;
     const int H = Kz / 2 + 1, zp = line_pitch(Kz);
     const int y0 = yb * kRowBlock, nr = min(kRowBlock, Ky - y0), nrp = (nr + 1) / 2;
@@ -364,47 +402,45 @@ __device__ __forceinline__ void spread_z_body(
     const float rkz = 1.0f / (float)Kz, rnr = 1.0f / (float)nr;
     for (int idx = threadIdx.x; idx < npt; idx += kZThreads) qacc[idx] = 0ull;
     __syncthreads();
-    // The four bins are walked as one flat range; each thread issues the
-    // index, base and fraction loads of up to kSpreadPerThread atoms before
-    // any LDS update, so their latencies overlap.
-    // prefix offsets of the four bins, kept in scalars (an array indexed by
-    // a per-lane bin number would live in scratch memory)
-    int b0 = x, b1 = x + 1, b2 = x + 2, b3 = x + 3;
-    b1 -= b1 >= Kx ? Kx : 0;
-    b2 -= b2 >= Kx ? Kx : 0;
-    b3 -= b3 >= Kx ? Kx : 0;
-    const int o1 = min(bin_count[b0], bin_cap);
-    const int o2 = o1 + min(bin_count[b1], bin_cap);
-    const int o3 = o2 + min(bin_count[b2], bin_cap);
-    const int total = o3 + min(bin_count[b3], bin_cap);
+    // The four bins are walked as one flat range of records (base, fraction
+    // and charge filed by the prep in bin order); each thread issues the
+    // record loads of up to kSpreadPerThread atoms before any LDS update, so
+    // their latencies overlap.
+    const int o1 = min(c0, bin_cap);
+    const int o2 = o1 + min(c1, bin_cap);
+    const int o3 = o2 + min(c2, bin_cap);
+    const int total = o3 + min(c3, bin_cap);
     for (int t0 = 0; t0 < total; t0 += kZThreads * kSpreadPerThread) {
-        int ai[kSpreadPerThread], akx[kSpreadPerThread];
+        long sl[kSpreadPerThread];
+        int akx[kSpreadPerThread];
 #pragma unroll
         for (int r = 0; r < kSpreadPerThread; r++) {
             const int t = t0 + r * kZThreads + (int)threadIdx.x;
-            ai[r] = -1;
+            sl[r] = -1;
             akx[r] = 0;
             if (t < total) {
                 const int k = (t >= o1) + (t >= o2) + (t >= o3);
                 const int b = k == 0 ? b0 : k == 1 ? b1 : k == 2 ? b2 : b3;
                 const int o = k == 0 ? 0 : k == 1 ? o1 : k == 2 ? o2 : o3;
-                ai[r] = bin_atoms[(size_t)b * bin_cap + (t - o)];
+                sl[r] = (long)b * bin_cap + (t - o);
                 akx[r] = k;
             }
         }
-        // base and fraction loads of every candidate are issued together (one
-        // memory round trip), the rows are tested afterwards
+        // every candidate's record is loaded together (one memory round
+        // trip), the rows are tested afterwards
         unsigned bb[kSpreadPerThread];
         float4 ff[kSpreadPerThread];
+        int ai[kSpreadPerThread];
 #pragma unroll
         for (int r = 0; r < kSpreadPerThread; r++)
-            if (ai[r] >= 0) {
-                bb[r] = base[ai[r]];
-                ff[r] = frac[ai[r]];
+            if (sl[r] >= 0) {
+                bb[r] = rec_base[sl[r]];
+                ff[r] = rec_frac[sl[r]];
+                if (MODE) ai[r] = bin_atoms[sl[r]];
             }
 #pragma unroll
         for (int r = 0; r < kSpreadPerThread; r++) {
-            if (ai[r] < 0) continue;
+            if (sl[r] < 0) continue;
             // stencil rows by - ky, ky = 0..3, relative to the block
             int d = (int)((bb[r] >> 10) & 1023u) - y0;
             d += d < 0 ? Ky : 0;
@@ -483,21 +519,21 @@ __device__ __forceinline__ void spread_z_body(
 
 template <int RM>
 __global__ __launch_bounds__(kZThreads) void spread_z_kernel(
-    const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
+    const unsigned* __restrict__ rec_base, const float4* __restrict__ rec_frac, const int* __restrict__ bin_count,
     const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, int nyb,
     FftPlan pz, const float2* __restrict__ twz, double* __restrict__ acc) {
     const int x = blockIdx.x / nyb;
-    spread_z_body<0, RM>(x, blockIdx.x - x * nyb, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, twz,
-                         acc);
+    spread_z_body<0, RM>(x, blockIdx.x - x * nyb, rec_base, rec_frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz,
+                         pz, twz, acc);
 }
 
 __global__ __launch_bounds__(kZThreads) void spread_z_adjoint_kernel(
-    const unsigned* __restrict__ base, const float4* __restrict__ frac, const int* __restrict__ bin_count,
+    const unsigned* __restrict__ rec_base, const float4* __restrict__ rec_frac, const int* __restrict__ bin_count,
     const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, int nyb,
     FftPlan pz, const float2* __restrict__ twz, const float4* __restrict__ wv, const float* __restrict__ scale_ptr) {
     const int x = blockIdx.x / nyb;
-    spread_z_body<1>(x, blockIdx.x - x * nyb, base, frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz, twz,
-                     nullptr, wv, scale_ptr);
+    spread_z_body<1>(x, blockIdx.x - x * nyb, rec_base, rec_frac, bin_count, bin_atoms, bin_cap, FQ, Kx, Ky, Kz, pz,
+                     twz, nullptr, wv, scale_ptr);
 }
 
 // Batched: local block l = x * (row blocks) + row block.
@@ -509,8 +545,8 @@ __global__ __launch_bounds__(kZThreads) void spread_z_batch(const BatchSys* __re
     const sponge_pme& P = b.pme;
     const int nyb = row_blocks(P.K[1]);
     const int x = m.y / nyb;
-    spread_z_body(x, m.y - x * nyb, P.base, P.frac, P.bin_count, P.bin_atoms, P.bin_cap, P.FQ, P.K[0], P.K[1], P.K[2],
-                  b.pz, P.d_tw[2], energy ? P.acc : nullptr);
+    spread_z_body(x, m.y - x * nyb, P.bin_base, P.bin_frac, P.bin_count, P.bin_atoms, P.bin_cap, P.FQ, P.K[0], P.K[1],
+                  P.K[2], b.pz, P.d_tw[2], energy ? P.acc : nullptr);
 }
 
 // One block per kz: the Kx x Ky plane of the spectrum in LDS (padded rows
@@ -531,34 +567,38 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
     // the spread has consumed the x-plane bins: clear them for the next prep
     if (kz == 0)
         for (int b = threadIdx.x; b < Kx; b += NT) bin_count[b] = 0;
-    load_twiddles<NT>(tx, twx, Kx);
-    load_twiddles<NT>(ty, twy, Ky);
     float2* plane = FQ + (size_t)kz * Kx * Ky;
     const float* bcp = bc + (size_t)kz * Kx * Ky;
     const int np = Kx * Ky;
     const float rky = 1.0f / (float)Ky;
     // The plane and its BC values are loaded in one memory round trip when
-    // they fit kConvBatch points per thread (the BC values wait in registers
-    // through the forward transforms): with one block per CU nothing else
-    // hides the memory latency.
-    float bv[kConvBatch];
-    const bool onepass = np <= kConvBatch * NT;
-    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * NT) {
-        float2 v[kConvBatch];
+    // they fit CB points per thread (4,096 points: a 64 x 64 plane with 256
+    // or 512 threads; the BC values wait in registers through the forward
+    // transforms): with one block per CU nothing else hides the memory
+    // latency.
+    constexpr int CB = kConvBatch * (512 / NT);
+    float bv[CB];
+    const bool onepass = np <= CB * NT;
+    for (int s0 = threadIdx.x; s0 < np; s0 += CB * NT) {
+        float2 v[CB];
 #pragma unroll
-        for (int s = 0; s < kConvBatch; s++) {
+        for (int s = 0; s < CB; s++) {
             const int idx = s0 + s * NT;
             if (idx < np) {
                 v[s] = plane[idx];
                 if (onepass) bv[s] = bcp[idx];
             }
         }
+        if (s0 == (int)threadIdx.x) {
+            // the twiddle loads join the first round trip
+            load_twiddles<NT>(tx, twx, Kx);
+            load_twiddles<NT>(ty, twy, Ky);
+        }
 #pragma unroll
-        for (int s = 0; s < kConvBatch; s++) {
+        for (int s = 0; s < CB; s++) {
             const int idx = s0 + s * NT;
             if (idx < np) {
-                const int xx = div_small(idx, rky), y = idx - xx * Ky;
-                sm[pd(xx) * pitch + pd(y)] = v[s];
+                sm[plane_lds(idx, rky, Ky, pitch)] = v[s];
             }
         }
     }
@@ -568,23 +608,23 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
     lds_fft<-1, NT, RM>(sm, Ky, 1, pitch, px, tx);
     __syncthreads();
     double e = 0.0;
-    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * NT) {
+    for (int s0 = threadIdx.x; s0 < np; s0 += CB * NT) {
         if (!onepass) {
 #pragma unroll
-            for (int s = 0; s < kConvBatch; s++) {
+            for (int s = 0; s < CB; s++) {
                 const int idx = s0 + s * NT;
                 if (idx < np) bv[s] = bcp[idx];
             }
         }
 #pragma unroll
-        for (int s = 0; s < kConvBatch; s++) {
+        for (int s = 0; s < CB; s++) {
             const int idx = s0 + s * NT;
             if (idx < np) {
-                const int xx = div_small(idx, rky), y = idx - xx * Ky;
+                const int o = plane_lds(idx, rky, Ky, pitch);
                 const float b = bv[s];
-                const float2 w = sm[pd(xx) * pitch + pd(y)];
+                const float2 w = sm[o];
                 if (energy) e += (double)b * ((double)w.x * w.x + (double)w.y * w.y);
-                sm[pd(xx) * pitch + pd(y)] = make_float2(w.x * b, w.y * b);
+                sm[o] = make_float2(w.x * b, w.y * b);
             }
         }
     }
@@ -594,8 +634,7 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
     lds_fft<1, NT, RM>(sm, Kx, pitch, 1, py, ty);
     __syncthreads();
     for (int idx = threadIdx.x; idx < np; idx += NT) {
-        const int xx = div_small(idx, rky), y = idx - xx * Ky;
-        plane[idx] = sm[pd(xx) * pitch + pd(y)];
+        plane[idx] = sm[plane_lds(idx, rky, Ky, pitch)];
     }
     if (energy) {
         const double t = block_sum<NT>(e, red);
@@ -654,12 +693,26 @@ __device__ __forceinline__ void conv_y_body(int bx, int kz, float2* __restrict__
     float2* ty = sm + kLines * pitch;
     const int x0 = bx * kLines, nl = min(kLines, Kx - x0);
     float2* rows = FQ + ((size_t)kz * Kx + x0) * Ky;
-    load_twiddles<kLineThreads>(ty, twy, Ky);
     const int np = nl * Ky;
     const float rky = 1.0f / (float)Ky;
-    for (int idx = threadIdx.x; idx < np; idx += kLineThreads) {
-        const int r = div_small(idx, rky), y = idx - r * Ky;
-        sm[r * pitch + pd(y)] = rows[idx];
+    // kConvBatch loads per thread in flight at a time; the twiddle loads
+    // join the first round
+    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * kLineThreads) {
+        float2 v[kConvBatch];
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * kLineThreads;
+            if (idx < np) v[s] = rows[idx];
+        }
+        if (s0 == (int)threadIdx.x) load_twiddles<kLineThreads>(ty, twy, Ky);
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * kLineThreads;
+            if (idx < np) {
+                const int r = div_small(idx, rky), y = idx - r * Ky;
+                sm[r * pitch + pd(y)] = v[s];
+            }
+        }
     }
     __syncthreads();
     lds_fft<SIGN, kLineThreads, RM>(sm, nl, pitch, 1, py, ty);
@@ -698,7 +751,6 @@ __device__ __forceinline__ void conv_x_body(int by, int kz, float2* __restrict__
     // the spread has consumed the x-plane bins: clear them for the next prep
     if (kz == 0 && by == 0)
         for (int b = threadIdx.x; b < Kx; b += kLineThreads) bin_count[b] = 0;
-    load_twiddles<kLineThreads>(tx, twx, Kx);
     float2* cols = FQ + (size_t)kz * Kx * Ky + y0;
     const float* bcp = bc + (size_t)kz * Kx * Ky + y0;
     const int np = Kx * kLines;
@@ -716,6 +768,7 @@ __device__ __forceinline__ void conv_x_body(int by, int kz, float2* __restrict__
                 if (onepass) bv[s] = bcp[(size_t)x * Ky + c];
             }
         }
+        if (s0 == (int)threadIdx.x) load_twiddles<kLineThreads>(tx, twx, Kx);  // same round trip
 #pragma unroll
         for (int s = 0; s < kConvBatch; s++) {
             const int idx = s0 + s * kLineThreads, x = idx / kLines, c = idx % kLines;
@@ -782,7 +835,6 @@ __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restri
     float2* rows = sm;                      // nr x H
     float2* pk = sm + kRowBlock * H;        // nrp lines at pitch zp
     float2* tz = pk + (kRowBlock / 2) * zp;
-    load_twiddles<kZThreads>(tz, twz, Kz);
     const float rkz = 1.0f / (float)Kz, rnr = 1.0f / (float)nr;
     float2 v[kZPer];
 #pragma unroll
@@ -793,6 +845,8 @@ __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restri
             v[s] = FQ[((size_t)k * Kx + x) * Ky + y0 + r];
         }
     }
+    // the twiddle loads join the same round trip
+    load_twiddles<kZThreads>(tz, twz, Kz);
 #pragma unroll
     for (int s = 0; s < kZPer; s++) {
         const int idx = threadIdx.x + s * kZThreads;
@@ -1236,6 +1290,8 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     pme->bin_cap = atom_numbers;  // a plane can never hold more than every atom
     A((void**)&pme->bin_count, (size_t)Kx * sizeof(int));
     A((void**)&pme->bin_atoms, (size_t)Kx * pme->bin_cap * sizeof(int));
+    A((void**)&pme->bin_base, (size_t)Kx * pme->bin_cap * sizeof(unsigned));
+    A((void**)&pme->bin_frac, (size_t)Kx * pme->bin_cap * sizeof(float4));
     A((void**)&pme->acc, SPONGE_NB_NPARTS * sizeof(double));
     for (int d = 0; d < 3; d++) {
         std::vector<float2> tw(pme->K[d]);
@@ -1283,6 +1339,8 @@ extern "C" void sponge_pme_destroy(sponge_pme* pme) {
         if (p) (void)hipFree(p);
     if (pme->bin_count) (void)hipFree(pme->bin_count);
     if (pme->bin_atoms) (void)hipFree(pme->bin_atoms);
+    if (pme->bin_base) (void)hipFree(pme->bin_base);
+    if (pme->bin_frac) (void)hipFree(pme->bin_frac);
     if (pme->zero_type) (void)hipFree(pme->zero_type);
     if (pme->zero_ab) (void)hipFree(pme->zero_ab);
     snb_fep_free(pme->fep);
@@ -1366,11 +1424,12 @@ int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s) {
     double* acc = energy ? pme->acc : nullptr;
     if (pme->ev[0]) {
         SNB_RADIX_DISPATCH_EV(kernel_set(radix_mask(pz)), spread_z_kernel, Kx * nyb, kZThreads, spread_z_lds(Kz), s,
-                              pme->ev[0], pme->ev[1], pme->base, pme->frac, pme->bin_count, pme->bin_atoms,
+                              pme->ev[0], pme->ev[1], pme->bin_base, pme->bin_frac, pme->bin_count, pme->bin_atoms,
                               pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb, pz, pme->d_tw[2], acc);
     } else {
         SNB_RADIX_DISPATCH(kernel_set(radix_mask(pz)), spread_z_kernel, Kx * nyb, kZThreads, spread_z_lds(Kz), s,
-                           pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz,
+                           pme->bin_base, pme->bin_frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky,
+                           Kz,
                            nyb, pz, pme->d_tw[2], acc);
     }
     SNB_CHECK_LAUNCH();
@@ -1574,7 +1633,7 @@ int snb_pme_adjoint_grid(sponge_pme* pme, const float4* wv, const float* scale, 
     snb_pme_plans(pme, &px, &py, &pz);
     const int nyb = row_blocks(Ky);
     spread_z_adjoint_kernel<<<Kx * nyb, kZThreads, spread_z_lds(Kz), s>>>(
-        pme->base, pme->frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb, pz,
+        pme->bin_base, pme->bin_frac, pme->bin_count, pme->bin_atoms, pme->bin_cap, pme->FQ, Kx, Ky, Kz, nyb, pz,
         pme->d_tw[2], wv, scale);
     SNB_CHECK_LAUNCH();
     (void)H;
