@@ -9,7 +9,7 @@
 //                 coordinate with one 32x32->64-bit multiply (t = u K / 2^32)
 //   spread_z      one block per (x plane, 16 y rows): the stencil rows of every
 //                 atom filed under planes x..x+3 that fall in the block are
-//                 spread in LDS as 64-bit fixed-point sums (no global atomics
+//                 spread in LDS as exact fixed-point sums (no global atomics
 //                 and no charge grid in HBM), then the real z-rows are paired
 //                 into complex z-FFTs and unpacked (R2C)
 //   conv2d        one block per kz: y- and x-FFT of the plane, energy
@@ -364,8 +364,8 @@ __device__ __forceinline__ float bsp_dw(float f, int k) {
 // one block of kRowBlock y rows of one x plane: grid point (base - k) gets
 // q M4(fx+kx) M4(fy+ky) M4(fz+kz); plane x receives the atoms filed under
 // planes x .. x+3 (kx = their offset), of which the block keeps the stencil
-// rows that fall inside it. The rows are summed as 64-bit fixed-point
-// integers in LDS (LDS integer atomics are an order of magnitude faster than
+// rows that fall inside it. The rows are summed as fixed-point integers in
+// LDS (LDS integer atomics are an order of magnitude faster than
 // LDS float atomics on this chip, and integer sums do not depend on the order
 // in which atoms arrive), converted, paired into complex z-lines (row r is
 // the real (r even) or imaginary (r odd) part of line r/2), transformed, and
@@ -393,14 +393,29 @@ __device__ __forceinline__ void spread_z_body(
     const float unit = MODE ? scale_ptr[0] : kSpreadScale;
     const int H = Kz / 2 + 1, zp = line_pitch(Kz);
     const int y0 = yb * kRowBlock, nr = min(kRowBlock, Ky - y0), nrp = (nr + 1) / 2;
-    unsigned long long* qacc = reinterpret_cast<unsigned long long*>(sm);  // nr x Kz
+    // Charge spread: each 32-bit contribution c is summed as two 32-bit
+    // words, hi = c >> 16 (signed) and lo = c & 0xffff, in two arrays; with
+    // fewer than 2^16 contributions per point neither word overflows and
+    // hi 2^16 + lo is the exact 64-bit sum. A wave's 32-bit atomics touch
+    // one bank per lane, not the two of a 64-bit atomic. The adjoint source
+    // (MODE 1) keeps 64-bit atomics.
+    unsigned long long* qacc = reinterpret_cast<unsigned long long*>(sm);  // MODE 1: nr x Kz
+    int* qhi = reinterpret_cast<int*>(sm);                                  // MODE 0: nr x Kz
+    unsigned* qlo = reinterpret_cast<unsigned*>(sm) + kRowBlock * Kz;       //         nr x Kz
     float2* pk = sm + kRowBlock * Kz;                                       // nrp lines at pitch zp
     float2* tz = pk + (kRowBlock / 2) * zp;
     load_twiddles<kZThreads>(tz, twz, Kz);
     if (acc && x == 0 && yb == 0 && threadIdx.x == 0) acc[SPONGE_PART_RECIPROCAL] = 0.0;
     const int npt = nr * Kz;
     const float rkz = 1.0f / (float)Kz, rnr = 1.0f / (float)nr;
-    for (int idx = threadIdx.x; idx < npt; idx += kZThreads) qacc[idx] = 0ull;
+    for (int idx = threadIdx.x; idx < npt; idx += kZThreads) {
+        if (MODE == 0) {
+            qhi[idx] = 0;
+            qlo[idx] = 0u;
+        } else {
+            qacc[idx] = 0ull;
+        }
+    }
     __syncthreads();
     // The four bins are walked as one flat range of records (base, fraction
     // and charge filed by the prep in bin order); each thread issues the
@@ -477,9 +492,14 @@ __device__ __forceinline__ void spread_z_body(
                 unsigned long long* row = qacc + rr * Kz;
                 if (MODE == 0) {
                     const float wy = qs * ax * ay;
+                    int* rhi = qhi + rr * Kz;
+                    unsigned* rlo = qlo + rr * Kz;
 #pragma unroll
-                    for (int kz = 0; kz < 4; kz++)
-                        atomicAdd(row + gz[kz], (unsigned long long)(long long)__float2int_rn(wy * wz[kz]));
+                    for (int kz = 0; kz < 4; kz++) {
+                        const int c = __float2int_rn(wy * wz[kz]);
+                        atomicAdd(rhi + gz[kz], c >> 16);
+                        atomicAdd(rlo + gz[kz], (unsigned)c & 0xffffu);
+                    }
                 } else {
                     const float day = bsp_dw(f.y, ky);
                     const float c1 = qs * (w.x * dax * ay + w.y * ax * day);
@@ -496,7 +516,9 @@ __device__ __forceinline__ void spread_z_body(
     float* pkf = reinterpret_cast<float*>(pk);
     for (int idx = threadIdx.x; idx < npt; idx += kZThreads) {
         const int r = div_small(idx, rkz), z = idx - r * Kz;
-        pkf[2 * ((r >> 1) * zp + pd(z)) + (r & 1)] = (float)((double)(long long)qacc[idx] / (double)unit);
+        const long long q =
+            MODE == 0 ? (long long)qhi[idx] * 65536ll + (long long)qlo[idx] : (long long)qacc[idx];
+        pkf[2 * ((r >> 1) * zp + pd(z)) + (r & 1)] = (float)((double)q / (double)unit);
     }
     if (nr & 1)
         for (int z = threadIdx.x; z < Kz; z += kZThreads) pkf[2 * ((nr >> 1) * zp + pd(z)) + 1] = 0.f;
@@ -914,6 +936,14 @@ __device__ __forceinline__ void pme_gather_body(int blk, int n, const unsigned* 
     const int i = gid >> 4, sub = gid & 15;
     const int kx = sub >> 2, ky = sub & 3;
     float sx = 0.f, sy = 0.f, sz = 0.f, qi = 0.f;
+    // the force being added to is read with the atom's base and fraction
+    // (one memory round trip before the grid reads)
+    float f0x = 0.f, f0y = 0.f, f0z = 0.f;
+    if (i < n && sub == 0) {
+        f0x = frc[3 * i];
+        f0y = frc[3 * i + 1];
+        f0z = frc[3 * i + 2];
+    }
     if (i < n) {
         const unsigned bb = base[i];
         const int4 b = make_int4((int)(bb & 1023u), (int)((bb >> 10) & 1023u), (int)(bb >> 20), 0);
@@ -943,9 +973,9 @@ __device__ __forceinline__ void pme_gather_body(int blk, int n, const unsigned* 
         sz += __shfl_xor(sz, o);
     }
     if (i < n && sub == 0) {
-        frc[3 * i] += -qi * sx * ibx;
-        frc[3 * i + 1] += -qi * sy * iby;
-        frc[3 * i + 2] += -qi * sz * ibz;
+        frc[3 * i] = f0x + -qi * sx * ibx;
+        frc[3 * i + 1] = f0y + -qi * sy * iby;
+        frc[3 * i + 2] = f0z + -qi * sz * ibz;
     }
 }
 
