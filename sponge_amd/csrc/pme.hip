@@ -60,6 +60,14 @@ constexpr int kPmeWavePriority = 3;
 constexpr int kConvBatch = 8;
 constexpr int kZPer = 17;
 constexpr int kGatherThreads = 256;
+// The reciprocal chain's kernels run beside the pair kernel, whose waves hold
+// 64 of a SIMD's 512 VGPRs each: a SIMD with six pair waves has 128 left, so
+// the chain kernels are held to 128 VGPRs (four waves per SIMD) and one of
+// their waves still fits there (radix-8 transforms otherwise take ~132, and
+// wait for pair waves to retire). Held to 64, they spill.
+constexpr int kPmeMinWaves = 4;
+// (radix-8 sets only: the others need more registers and would spill)
+#define SNB_PME_CHAIN __attribute__((amdgpu_waves_per_eu(RM == kRadix8 ? kPmeMinWaves : 1)))
 
 __host__ __device__ __forceinline__ int row_blocks(int Ky) { return (Ky + kRowBlock - 1) / kRowBlock; }
 // LDS layout of transform lines. Stockham passes write each line at strides
@@ -540,7 +548,7 @@ __device__ __forceinline__ void spread_z_body(
 }
 
 template <int RM>
-__global__ __launch_bounds__(kZThreads) void spread_z_kernel(
+__global__ __launch_bounds__(kZThreads) SNB_PME_CHAIN void spread_z_kernel(
     const unsigned* __restrict__ rec_base, const float4* __restrict__ rec_frac, const int* __restrict__ bin_count,
     const int* __restrict__ bin_atoms, int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, int nyb,
     FftPlan pz, const float2* __restrict__ twz, double* __restrict__ acc) {
@@ -666,7 +674,7 @@ __device__ __forceinline__ void conv2d_body(int kz, float2* __restrict__ FQ, con
 }
 
 template <int RM>
-__global__ __launch_bounds__(kConvThreads) void conv2d_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
+__global__ __launch_bounds__(kConvThreads) SNB_PME_CHAIN void conv2d_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
                                                               int Kx, int Ky, int Kz, FftPlan px, FftPlan py,
                                                               const float2* __restrict__ twx,
                                                               const float2* __restrict__ twy, int energy,
@@ -677,7 +685,7 @@ __global__ __launch_bounds__(kConvThreads) void conv2d_kernel(float2* __restrict
 // The same with 512 threads (8 waves) per plane: more waves to hide the LDS
 // latency of the transform passes, where the CUs are free to hold them.
 template <int RM>
-__global__ __launch_bounds__(2 * kConvThreads) void conv2d_wide_kernel(
+__global__ __launch_bounds__(2 * kConvThreads) SNB_PME_CHAIN void conv2d_wide_kernel(
     float2* __restrict__ FQ, const float* __restrict__ bc, int Kx, int Ky, int Kz, FftPlan px, FftPlan py,
     const float2* __restrict__ twx, const float2* __restrict__ twy, int energy, double* __restrict__ acc,
     int* __restrict__ bin_count) {
@@ -746,13 +754,13 @@ __device__ __forceinline__ void conv_y_body(int bx, int kz, float2* __restrict__
 }
 
 template <int RM>
-__global__ __launch_bounds__(kLineThreads) void conv_y_forward_kernel(float2* __restrict__ FQ, int Kx, int Ky,
+__global__ __launch_bounds__(kLineThreads) SNB_PME_CHAIN void conv_y_forward_kernel(float2* __restrict__ FQ, int Kx, int Ky,
                                                                       FftPlan py, const float2* __restrict__ twy) {
     conv_y_body<-1, RM>(blockIdx.x, blockIdx.y, FQ, Kx, Ky, py, twy);
 }
 
 template <int RM>
-__global__ __launch_bounds__(kLineThreads) void conv_y_inverse_kernel(float2* __restrict__ FQ, int Kx, int Ky,
+__global__ __launch_bounds__(kLineThreads) SNB_PME_CHAIN void conv_y_inverse_kernel(float2* __restrict__ FQ, int Kx, int Ky,
                                                                       FftPlan py, const float2* __restrict__ twy) {
     conv_y_body<1, RM>(blockIdx.x, blockIdx.y, FQ, Kx, Ky, py, twy);
 }
@@ -835,7 +843,7 @@ __device__ __forceinline__ void conv_x_body(int by, int kz, float2* __restrict__
 }
 
 template <int RM>
-__global__ __launch_bounds__(kLineThreads) void conv_x_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
+__global__ __launch_bounds__(kLineThreads) SNB_PME_CHAIN void conv_x_kernel(float2* __restrict__ FQ, const float* __restrict__ bc,
                                                               int Kx, int Ky, int Kz, FftPlan px,
                                                               const float2* __restrict__ twx, int energy,
                                                               double* __restrict__ acc, int* __restrict__ bin_count) {
@@ -908,7 +916,7 @@ __device__ __forceinline__ void inv_z_body(int x, int yb, const float2* __restri
 }
 
 template <int RM>
-__global__ __launch_bounds__(kZThreads) void inv_z_kernel(const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
+__global__ __launch_bounds__(kZThreads) SNB_PME_CHAIN void inv_z_kernel(const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
                                                           int Ky, int Kz, int nyb, FftPlan pz,
                                                           const float2* __restrict__ twz) {
     const int x = blockIdx.x / nyb;
