@@ -143,6 +143,68 @@ __global__ void nl_bin_kernel(int n, float* __restrict__ crd, float* __restrict_
                 col_atoms, flags, o2s, dropped);
 }
 
+// nl_bin_body with the column slots taken through a block histogram in LDS:
+// consecutive atoms (the atoms of a molecule, molecules placed next to each
+// other) fall into the same few columns, and one global atomic per atom on
+// the same counters serialises; here a block adds one count per column it
+// touches. The atom order within a column changes, which the sort (by uint z,
+// then atom index) removes. LDS: 2 ncol ints.
+__global__ __launch_bounds__(256) void nl_bin_hist_kernel(
+    int n, float* __restrict__ crd, float* __restrict__ old_crd, unsigned int* __restrict__ nl_uint, float bx,
+    float by, float bz, float cqx, float cqy, float cqz, int ncx, int ncy, int col_cap, int* __restrict__ col_count,
+    int* __restrict__ col_atoms, int* flags, int* __restrict__ o2s, int* __restrict__ dropped) {
+    if (!flags[0]) return;  // uniform: every thread of the grid returns
+    extern __shared__ int hbin[];
+    const int ncol = ncx * ncy;
+    int* hist = hbin;
+    int* hbase = hbin + ncol;
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) hist[c] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = -1, rank = 0;
+    if (i < n) {
+        const float b[3] = {bx, by, bz};
+        const float cq[3] = {cqx, cqy, cqz};
+        unsigned int u[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            float x = crd[3 * i + d];
+            if (x >= 0.0f) {
+                if (!(x < b[d])) x = x - b[d];
+            } else {
+                x = x + b[d];
+            }
+            crd[3 * i + d] = x;
+            old_crd[3 * i + d] = x;
+            const int t = (int)__fmul_rn(x, cq[d]);
+            u[d] = ((unsigned int)t) << 2;
+            nl_uint[3 * i + d] = u[d];
+        }
+        const int cx = (int)(((unsigned long long)u[0] * (unsigned)ncx) >> 32);
+        const int cy = (int)(((unsigned long long)u[1] * (unsigned)ncy) >> 32);
+        c = cx * ncy + cy;
+        rank = atomicAdd(hist + c, 1);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ncol; k += blockDim.x)
+        if (hist[k] > 0) hbase[k] = atomicAdd(col_count + k, hist[k]);
+    __syncthreads();
+    if (c >= 0) {
+        const int s = hbase[c] + rank;
+        if (s < col_cap) {
+            col_atoms[(size_t)c * col_cap + s] = i;
+        } else {
+            // left out of this list, as nl_bin_body does
+            atomicOr(flags + 1, SPONGE_NL_OVERFLOW_COLUMN);
+            o2s[i] = -1;
+            dropped[atomicAdd(flags + 5, 1)] = i;
+        }
+    }
+}
+
+// Largest column count binned through the LDS histogram (64 KB of LDS).
+constexpr int kBinHistMax = 8192;
+
 // Exclusive scan of the padded (multiple of 4) column sizes; one block.
 __device__ __forceinline__ void nl_scan_body(int ncol, int col_cap, const int* __restrict__ col_count,
                                              int* __restrict__ col_start, int* flags) {
@@ -718,10 +780,16 @@ static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el,
     const int n = nl->n;
     nl_begin_kernel<<<1, 256, 0, s>>>(forced, nl->flags, nl->col_count, nl->ncol);
     SNB_CHECK_LAUNCH();
-    nl_bin_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_crd, nl->old_crd, nl->nl_uint, nl->box[0], nl->box[1],
-                                                   nl->box[2], nl->cof_q[0], nl->cof_q[1], nl->cof_q[2], nl->ncx,
-                                                   nl->ncy, nl->col_cap, nl->col_count, nl->col_atoms, nl->flags,
-                                                   nl->o2s, nl->dropped);
+    if (nl->ncol <= kBinHistMax)
+        nl_bin_hist_kernel<<<ceil_div(n, 256), 256, (size_t)2 * nl->ncol * sizeof(int), s>>>(
+            n, d_crd, nl->old_crd, nl->nl_uint, nl->box[0], nl->box[1], nl->box[2], nl->cof_q[0], nl->cof_q[1],
+            nl->cof_q[2], nl->ncx, nl->ncy, nl->col_cap, nl->col_count, nl->col_atoms, nl->flags, nl->o2s,
+            nl->dropped);
+    else
+        nl_bin_kernel<<<ceil_div(n, 256), 256, 0, s>>>(n, d_crd, nl->old_crd, nl->nl_uint, nl->box[0], nl->box[1],
+                                                       nl->box[2], nl->cof_q[0], nl->cof_q[1], nl->cof_q[2], nl->ncx,
+                                                       nl->ncy, nl->col_cap, nl->col_count, nl->col_atoms, nl->flags,
+                                                       nl->o2s, nl->dropped);
     SNB_CHECK_LAUNCH();
     nl_scan_kernel<<<1, 1024, 0, s>>>(nl->ncol, nl->col_cap, nl->col_count, nl->col_start, nl->flags);
     SNB_CHECK_LAUNCH();
