@@ -423,7 +423,10 @@ int sponge_batch_sync(sponge_batch* batch);
  * the interval; 0: each system's own skin check decides on the device). */
 int sponge_batch_neighbor_list_update(sponge_batch* batch, int forced_update, void* stream);
 /* sponge_nb_force_step of every system; SPONGE_STEP_FROM_CRD computes each
- * system's uint coordinates from its crd first. */
+ * system's uint coordinates from its crd first. The reciprocal stages run on
+ * a stream of the batch beside the pair and exclusion stages (forked and
+ * joined with events, so the step stays capturable); SPONGE_STEP_SERIAL keeps
+ * every stage on `stream`. */
 #define SPONGE_STEP_FROM_CRD 4
 int sponge_batch_force_step(sponge_batch* batch, int flags, int need_energy, int need_virial,
                             void* stream);

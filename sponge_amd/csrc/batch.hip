@@ -44,6 +44,12 @@ static void batch_free(sponge_batch* b) {
     if (b->d_md) (void)hipFree(b->d_md);
     if (b->md_map) (void)hipFree(b->md_map);
     if (b->d_ident) (void)hipFree(b->d_ident);
+    if (b->side) {
+        (void)hipStreamSynchronize(b->side);
+        (void)hipStreamDestroy(b->side);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
     delete b;
 }
 
@@ -112,6 +118,13 @@ extern "C" int sponge_batch_create(int nsys, sponge_nl* const* nl, sponge_lj* co
         set_error("batch_create: out of device memory");
         rc = SPONGE_NB_EHIP;
     }
+    // default-priority side stream and the fork / join events of the force step
+    if (rc == SPONGE_NB_OK && (hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+                               hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+                               hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)) {
+        set_error("batch_create: cannot create the side stream or its events");
+        rc = SPONGE_NB_EHIP;
+    }
     for (int st = 0; st < kNumStages && rc == SPONGE_NB_OK; st++) {
         std::vector<int2> m;
         for (int s = 0; s < nsys; s++) {
@@ -155,10 +168,26 @@ extern "C" int sponge_batch_force_step(sponge_batch* b, int flags, int need_ener
     const int energy = (need_energy > 0 || need_virial > 0) ? 1 : 0;
     int rc = snb_batch_prep(b, energy, (flags & SPONGE_STEP_FROM_CRD) ? 1 : 0,
                             (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, s);
-    if (!rc) rc = snb_batch_pairs(b, energy, (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, s);
+    if (rc) return rc;
+    // Fork: the pair and exclusion stages (the longer branch: it stays on
+    // the caller's queue, captured first) and the reciprocal stages on the
+    // side stream, both after the prep; join before the gather, which adds
+    // into the forces. SPONGE_STEP_SERIAL keeps every stage on `stream`.
+    const bool serial = (flags & SPONGE_STEP_SERIAL) != 0;
+    hipStream_t ps = serial ? s : b->side;
+    if (!serial) {
+        SNB_HIP(hipEventRecord(b->fork, s));
+        SNB_HIP(hipStreamWaitEvent(ps, b->fork, 0));
+    }
+    rc = snb_batch_pairs(b, energy, (flags & SPONGE_STEP_OVERWRITE_FORCE) ? 1 : 0, s);
     if (!rc) rc = snb_batch_excluded(b, s);
-    if (!rc) rc = snb_batch_reciprocal(b, energy, s);
-    if (!rc) rc = snb_batch_gather(b, s);
+    if (!rc) rc = snb_batch_reciprocal(b, energy, ps);
+    if (rc) return rc;
+    if (!serial) {
+        SNB_HIP(hipEventRecord(b->join, ps));
+        SNB_HIP(hipStreamWaitEvent(s, b->join, 0));
+    }
+    rc = snb_batch_gather(b, s);
     if (!rc && energy) rc = snb_batch_finalize(b, s);
     return rc;
 }

@@ -328,6 +328,9 @@ struct sponge_batch {
     int md_blocks = 0;
     int* d_ident = nullptr;     // 0 .. n-1: the atom list of systems without rigid groups
     int ident_n = 0;
+    // the force step's reciprocal stages run here, beside the pair stage
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // Inner-list arguments of the pair kernel (interval 0: no inner list).

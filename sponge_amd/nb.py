@@ -736,8 +736,10 @@ class BatchedSystems:
         _check(load_library().sponge_batch_neighbor_list_update(self.h, int(forced), _stream(stream)),
                "sponge_batch_neighbor_list_update")
 
-    def force_step(self, need_energy=0, need_virial=0, from_crd=True, overwrite=True, stream=None):
-        flags = (OVERWRITE_FORCE if overwrite else 0) | (FROM_CRD if from_crd else 0)
+    def force_step(self, need_energy=0, need_virial=0, from_crd=True, overwrite=True, serial=False, stream=None):
+        """serial: every stage on `stream` (else the reciprocal stages run on
+        the batch's side stream beside the pair stage)."""
+        flags = (OVERWRITE_FORCE if overwrite else 0) | (FROM_CRD if from_crd else 0) | (SERIAL if serial else 0)
         _check(load_library().sponge_batch_force_step(self.h, flags, int(need_energy), int(need_virial),
                                                       _stream(stream)), "sponge_batch_force_step")
 

@@ -121,6 +121,38 @@ def test_batch_graph_replays_match_eager(batch_run):
         torch.cuda.synchronize()
 
 
+def test_batch_two_stream_step_equals_serial(batch_run):
+    """The force step's reciprocal stages on the batch's side stream, beside
+    the pair and exclusion stages: the same forces as every stage on one
+    stream, eagerly and replayed from a graph (to the ulps of the exclusion
+    stage's float atomics, whose order the concurrency may change); energy
+    parts to FP64 atomics-order noise."""
+    import torch
+    systems, engines, batch, single, out = batch_run
+    batch.neighbor_list_update(forced=1)
+    batch.force_step(1, 1, serial=True)
+    torch.cuda.synchronize()
+    ref = [(e.frc.cpu().numpy().copy(), e.energy_parts()) for e in engines]
+    batch.force_step(1, 1)
+    torch.cuda.synchronize()
+    for s, e, (f, p) in zip(systems, engines, ref):
+        assert rel_rms(e.frc.cpu().numpy(), f) < 1e-7, s.name
+        for k in ("lj", "direct", "correction", "self", "reciprocal", "lrc"):
+            assert abs(e.energy_parts()[k] - p[k]) <= 1e-9 * max(1.0, abs(p[k])), (s.name, k)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        batch.force_step(0, 0, stream=torch.cuda.current_stream())
+    for r in range(2):
+        for e in engines:
+            e.frc.fill_(float("nan"))
+        g.replay()
+        torch.cuda.synchronize()
+        for s, e, (f, p) in zip(systems, engines, ref):
+            assert rel_rms(e.frc.cpu().numpy(), f) < 1e-7, (r, s.name)
+        batch.force_step(0, 0, serial=True)
+        torch.cuda.synchronize()
+
+
 def test_config4_batch_matches_oracle():
     """The batched-FEP workload the benchmark times: 16 soft-core lambda
     windows of a solvated 12-atom solute and 8 replicas of 1.4k-12k atoms

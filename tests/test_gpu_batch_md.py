@@ -10,7 +10,8 @@ SETTLE at its own time step and velocity cap -- integrated in one launch.
   stay on their constraints; after the run every system's neighbor set,
   forces and energies at its current coordinates match the FP64 oracle.
 * The two graphs of a batched MD step (plain, with the list rebuild)
-  replay the eager steps bit for bit, with eager work between replays.
+  replay the eager steps (to the ulps of the exclusion stage's float
+  atomics), with eager work between replays.
 """
 import dataclasses
 import math
@@ -185,11 +186,20 @@ def test_batched_md_graphs_replay_the_eager_steps():
         for k in range(steps):
             (rebuild if k % REFRESH == 0 else plain).replay()
         torch.cuda.synchronize()
-        # every stage is deterministic (no float atomics on these paths):
-        # the replays are the eager steps bit for bit
+        # the batch's exclusion stage adds its corrections with float
+        # atomics (one-sided exclusion lists, as the reference's kernel), and
+        # it runs beside the reciprocal stages: the order of two corrections
+        # to one atom may differ between runs by an ulp, which 18 steps carry
+        # but do not amplify beyond a few ulps
         for e, (x, v) in zip(engines, eager):
-            np.testing.assert_array_equal(e.crd.cpu().numpy(), x, err_msg=f"replay {rep} {e.sys.name}")
-            np.testing.assert_array_equal(e.vel.cpu().numpy(), v, err_msg=f"replay {rep} {e.sys.name}")
+            L = np.asarray(e.nb.box, np.float64)
+            d = e.crd.cpu().numpy().astype(np.float64) - x
+            d -= L * np.round(d / L)
+            dx = float(np.abs(d).max())
+            dv = rel_rms(e.vel.cpu().numpy(), v)
+            say(f"    replay {rep} {e.sys.name}: max |dx| {dx:.2e} A, velocity rel rms {dv:.2e}")
+            assert dx < 1e-4, (rep, e.sys.name)
+            assert dv < 1e-5, (rep, e.sys.name)
         # eager work between replays
         bmd.step(forced_update=0)
         torch.cuda.synchronize()
