@@ -270,8 +270,10 @@ int sponge_pme_set_events(sponge_pme* pme, void* spread_start, void* spread_stop
  * setting). step_form: how sponge_nb_force_step issues a step when its flags
  * do not say -- 0 the pair kernel on a second stream, concurrent with the
  * reciprocal pipeline (default), 1 every kernel on its own. conv_split 0 / 1
- * selects the one-workgroup-per-plane convolution or the three line launches
- * (0 is refused for planes beyond the LDS). */
+ * / 2 selects the one-workgroup-per-plane convolution, the three line
+ * launches, or the slab form (the spread and the inverse transform take the
+ * y transforms of one x plane each, the convolution the x lines); a form
+ * whose workgroups do not fit the LDS is refused. */
 int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_split);
 
 /* Launch shapes and schedule of the fused step as one record (no counterpart
@@ -282,7 +284,7 @@ int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_split);
  * and set the winner. */
 typedef struct sponge_shapes {
     int pair_waves;     /* i-clusters (waves) per block of the force-only pair kernel: 2, 4 or 8 */
-    int conv_split;     /* 0: one workgroup per kz plane; 1: three line launches */
+    int conv_split;     /* 0: one workgroup per kz plane; 1: three line launches; 2: slab form */
     int conv_threads;   /* threads of the per-plane workgroup: 256 or 512 */
     int step_form;      /* 0: pair kernel on a second stream; 1: every kernel on the caller's stream */
     int prune_interval; /* fused steps between scheduled prunes of the inner list; 0: no inner list */

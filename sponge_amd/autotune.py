@@ -6,8 +6,9 @@ exposes (include/sponge_nb.h, sponge_shapes) is chosen by measurement:
 
 * pair_waves: i-clusters (one wave each) per block of the force-only pair
   kernel, 2 / 4 / 8;
-* the convolution: one 256- or 512-thread workgroup per kz plane, or the three
-  line launches (conv_split);
+* the convolution: one 256- or 512-thread workgroup per kz plane, the three
+  line launches, or the slab form whose spread and inverse transform take the
+  y transforms of an x plane each (conv_split 0 / 1 / 2);
 * step_form: the pair kernel on a second stream or every kernel in turn;
 * the inner list: pruned every refresh / d steps to cutoff + m * skin.
 
@@ -29,7 +30,7 @@ import sys
 import time
 
 PAIR_WAVES = (2, 4, 8)
-CONV_FORMS = ((0, 256), (0, 512), (1, 256))          # (conv_split, conv_threads)
+CONV_FORMS = ((0, 256), (0, 512), (1, 256), (2, 256))  # (conv_split, conv_threads)
 STEP_FORMS = (0, 1)
 # (prune_div, margin / skin): pruned every refresh / prune_div steps to
 # cutoff + margin * skin; (1, 1.0) keeps every listed pair, per slot, from
@@ -185,7 +186,7 @@ def write_table(winners, path=TABLE_PATH):
 
 
 def _fmt(c):
-    conv = "split" if c["conv_split"] else f"plane/{c['conv_threads']}"
+    conv = {1: "split", 2: "slab"}.get(c["conv_split"], f"plane/{c['conv_threads']}")
     prune = "off" if c["prune_div"] == 0 else f"/{c['prune_div']} +{c['prune_margin']}skin"
     return f"| {c['pair_waves']} | {conv} | {'serial' if c['step_form'] else 'two streams'} | {prune} |"
 

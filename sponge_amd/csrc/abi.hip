@@ -244,7 +244,7 @@ extern "C" int sponge_nb_set_shapes(sponge_nl* nl, sponge_lj* lj, sponge_pme* pm
     SNB_ARG(sh->pair_waves == 2 || sh->pair_waves == 4 || sh->pair_waves == 8, "pair_waves must be 2, 4 or 8");
     SNB_ARG(sh->conv_threads == 256 || sh->conv_threads == 512, "conv_threads must be 256 or 512");
     SNB_ARG(sh->step_form == 0 || sh->step_form == 1, "step_form must be 0 or 1");
-    SNB_ARG(sh->conv_split == 0 || sh->conv_split == 1, "conv_split must be 0 or 1");
+    SNB_ARG(sh->conv_split >= 0 && sh->conv_split <= 2, "conv_split must be 0, 1 or 2");
     SNB_ARG(sh->prune_interval >= 0, "prune_interval");
     SNB_ARG(sh->prune_interval == 0 || sh->prune_radius >= nl->cutoff, "prune radius below the cutoff");
     // the one check that needs the grid: the plane form must fit the LDS

@@ -934,6 +934,247 @@ __global__ __launch_bounds__(kZThreads) void inv_z_batch(const BatchSys* __restr
     inv_z_body(x, m.y - x * nyb, P.FQ, P.R, P.K[0], P.K[1], P.K[2], b.pz, P.d_tw[2]);
 }
 
+// ---------------------------------------------------------------------------
+// Slab form (convolution form 2): one workgroup per x plane takes the z and
+// y transforms of its Ky x Kz slab, so that the convolution is left with the
+// x lines (conv_x) and the reciprocal chain is three launches:
+//   spread_yz  spread of every atom filed under planes x..x+3 (all its
+//              stencil rows fall in the slab) into the exact fixed-point sums
+//              of spread_z, paired real rows -> complex z-lines, z-FFT,
+//              R2C unpack into a [kz][y] tile, y-FFT, store FQ[kz][x][y]
+//   conv_x     x-FFT, energy, BC, inverse x-FFT (as in the split form)
+//   inv_yz     load the tile, inverse y-FFT, Hermitian repack into paired
+//              complex z-lines, inverse z-FFT (C2R), store R[x][y][z]
+// Same arithmetic as spread_z + conv_y_forward and conv_y_inverse + inv_z,
+// with the y lines in LDS instead of a round trip through FQ.
+constexpr int kYZThreads = 512;
+
+// LDS (float2 units): max(accumulators, y tile) | z-lines | twiddles z, y.
+// Lines of a transform sit at padded line positions (line l at pd(l) x
+// pitch, as lds_fft addresses them).
+__host__ __device__ __forceinline__ int yz_tile(int Ky, int Kz) { return padded_len(Kz / 2 + 1) * line_pitch(Ky); }
+__host__ __device__ __forceinline__ int yz_lines(int Ky, int Kz) { return padded_len((Ky + 1) / 2) * line_pitch(Kz); }
+__host__ __device__ __forceinline__ int yz_acc(int Ky, int Kz) {
+    const int a = Ky * Kz, t = yz_tile(Ky, Kz);
+    return a > t ? a : t;
+}
+
+template <int RM = kRadixAll>
+__device__ __forceinline__ void spread_yz_body(int x, const unsigned* __restrict__ rec_base,
+                                               const float4* __restrict__ rec_frac,
+                                               const int* __restrict__ bin_count, int bin_cap,
+                                               float2* __restrict__ FQ, int Kx, int Ky, int Kz, const FftPlan& pz,
+                                               const FftPlan& py, const float2* __restrict__ twz,
+                                               const float2* __restrict__ twy, double* __restrict__ acc) {
+    __builtin_amdgcn_s_setprio(kPmeWavePriority);
+    extern __shared__ float2 sm[];
+    constexpr int NT = kYZThreads;
+    int b0 = x, b1 = x + 1, b2 = x + 2, b3 = x + 3;
+    b1 -= b1 >= Kx ? Kx : 0;
+    b2 -= b2 >= Kx ? Kx : 0;
+    b3 -= b3 >= Kx ? Kx : 0;
+    const int c0 = bin_count[b0], c1 = bin_count[b1], c2 = bin_count[b2], c3 = bin_count[b3];
+    const int H = Kz / 2 + 1, zp = line_pitch(Kz), yp = line_pitch(Ky), nrp = (Ky + 1) / 2;
+    int* qhi = reinterpret_cast<int*>(sm);                        // Ky x Kz
+    unsigned* qlo = reinterpret_cast<unsigned*>(sm) + Ky * Kz;    // Ky x Kz
+    float2* T = sm;                                               // H x yp, after the accumulators
+    float2* pk = sm + yz_acc(Ky, Kz);                             // nrp lines at pitch zp
+    float2* tz = pk + yz_lines(Ky, Kz);
+    float2* ty = tz + Kz;
+    load_twiddles<NT>(tz, twz, Kz);
+    load_twiddles<NT>(ty, twy, Ky);
+    if (acc && x == 0 && threadIdx.x == 0) acc[SPONGE_PART_RECIPROCAL] = 0.0;
+    const int npt = Ky * Kz;
+    for (int idx = threadIdx.x; idx < npt; idx += NT) {
+        qhi[idx] = 0;
+        qlo[idx] = 0u;
+    }
+    __syncthreads();
+    const int o1 = min(c0, bin_cap);
+    const int o2 = o1 + min(c1, bin_cap);
+    const int o3 = o2 + min(c2, bin_cap);
+    const int total = o3 + min(c3, bin_cap);
+    const float unit = kSpreadScale;
+    for (int t0 = 0; t0 < total; t0 += NT * kSpreadPerThread) {
+        long sl[kSpreadPerThread];
+        int akx[kSpreadPerThread];
+#pragma unroll
+        for (int r = 0; r < kSpreadPerThread; r++) {
+            const int t = t0 + r * NT + (int)threadIdx.x;
+            sl[r] = -1;
+            akx[r] = 0;
+            if (t < total) {
+                const int k = (t >= o1) + (t >= o2) + (t >= o3);
+                const int b = k == 0 ? b0 : k == 1 ? b1 : k == 2 ? b2 : b3;
+                const int o = k == 0 ? 0 : k == 1 ? o1 : k == 2 ? o2 : o3;
+                sl[r] = (long)b * bin_cap + (t - o);
+                akx[r] = k;
+            }
+        }
+        unsigned bb[kSpreadPerThread];
+        float4 ff[kSpreadPerThread];
+#pragma unroll
+        for (int r = 0; r < kSpreadPerThread; r++)
+            if (sl[r] >= 0) {
+                bb[r] = rec_base[sl[r]];
+                ff[r] = rec_frac[sl[r]];
+            }
+#pragma unroll
+        for (int r = 0; r < kSpreadPerThread; r++) {
+            if (sl[r] < 0) continue;
+            const float4 f = ff[r];
+            const int by = (int)((bb[r] >> 10) & 1023u), bz = (int)(bb[r] >> 20);
+            float wz[4];
+            int gz[4];
+#pragma unroll
+            for (int kz = 0; kz < 4; kz++) {
+                wz[kz] = bsp_w(f.z, kz);
+                int tt = bz - kz;
+                gz[kz] = tt + (tt < 0 ? Kz : 0);
+            }
+            const float qs = f.w * unit;
+            const float ax = bsp_w(f.x, akx[r]);
+#pragma unroll
+            for (int ky = 0; ky < 4; ky++) {
+                int rr = by - ky;
+                rr += rr < 0 ? Ky : 0;
+                const float wy = qs * ax * bsp_w(f.y, ky);
+                int* rhi = qhi + rr * Kz;
+                unsigned* rlo = qlo + rr * Kz;
+#pragma unroll
+                for (int kz = 0; kz < 4; kz++) {
+                    const int cq = __float2int_rn(wy * wz[kz]);
+                    atomicAdd(rhi + gz[kz], cq >> 16);
+                    atomicAdd(rlo + gz[kz], (unsigned)cq & 0xffffu);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // rows (2p, 2p + 1) -> the real and imaginary parts of z-line p
+    float* pkf = reinterpret_cast<float*>(pk);
+    const float rkz = 1.0f / (float)Kz, rky = 1.0f / (float)Ky;
+    for (int idx = threadIdx.x; idx < npt; idx += NT) {
+        const int r = div_small(idx, rkz), z = idx - r * Kz;
+        const long long q = (long long)qhi[idx] * 65536ll + (long long)qlo[idx];
+        pkf[2 * (pd(r >> 1) * zp + pd(z)) + (r & 1)] = (float)((double)q / (double)unit);
+    }
+    if (Ky & 1)
+        for (int z = threadIdx.x; z < Kz; z += NT) pkf[2 * (pd(Ky >> 1) * zp + pd(z)) + 1] = 0.f;
+    __syncthreads();
+    lds_fft<-1, NT, RM>(pk, nrp, zp, 1, pz, tz);
+    __syncthreads();
+    // R2C unpack of the paired transforms into the [kz][y] tile (the
+    // accumulators are dead: the tile takes their place)
+    for (int idx = threadIdx.x; idx < Ky * H; idx += NT) {
+        const int k = div_small(idx, rky), r = idx - k * Ky;
+        const int pl = pd(r >> 1) * zp;
+        const float2 zk = pk[pl + pd(k)];
+        const float2 zm = pk[pl + pd(k == 0 ? 0 : Kz - k)];
+        T[pd(k) * yp + pd(r)] = (r & 1) == 0 ? make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y))
+                                             : make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+    __syncthreads();
+    lds_fft<-1, NT, RM>(T, H, yp, 1, py, ty);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < Ky * H; idx += NT) {
+        const int k = div_small(idx, rky), y = idx - k * Ky;
+        FQ[((size_t)k * Kx + x) * Ky + y] = T[pd(k) * yp + pd(y)];
+    }
+}
+
+template <int RM>
+__global__ __launch_bounds__(kYZThreads) SNB_PME_CHAIN void spread_yz_kernel(
+    const unsigned* __restrict__ rec_base, const float4* __restrict__ rec_frac, const int* __restrict__ bin_count,
+    int bin_cap, float2* __restrict__ FQ, int Kx, int Ky, int Kz, FftPlan pz, FftPlan py,
+    const float2* __restrict__ twz, const float2* __restrict__ twy, double* __restrict__ acc) {
+    spread_yz_body<RM>(blockIdx.x, rec_base, rec_frac, bin_count, bin_cap, FQ, Kx, Ky, Kz, pz, py, twz, twy, acc);
+}
+
+template <int RM = kRadixAll>
+__device__ __forceinline__ void inv_yz_body(int x, const float2* __restrict__ FQ, float* __restrict__ R, int Kx,
+                                            int Ky, int Kz, const FftPlan& pz, const FftPlan& py,
+                                            const float2* __restrict__ twz, const float2* __restrict__ twy) {
+    __builtin_amdgcn_s_setprio(kPmeWavePriority);
+    extern __shared__ float2 sm[];
+    constexpr int NT = kYZThreads;
+    const int H = Kz / 2 + 1, zp = line_pitch(Kz), yp = line_pitch(Ky), nrp = (Ky + 1) / 2;
+    float2* T = sm;                    // H x yp
+    float2* pk = sm + yz_tile(Ky, Kz); // nrp lines at pitch zp
+    float2* tz = pk + yz_lines(Ky, Kz);
+    float2* ty = tz + Kz;
+    const float rky = 1.0f / (float)Ky, rkz = 1.0f / (float)Kz;
+    const int np = Ky * H;
+    // kConvBatch loads per thread in flight at a time; the twiddle loads
+    // join the first round
+    for (int s0 = threadIdx.x; s0 < np; s0 += kConvBatch * NT) {
+        float2 v[kConvBatch];
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * NT;
+            if (idx < np) {
+                const int k = div_small(idx, rky), y = idx - k * Ky;
+                v[s] = FQ[((size_t)k * Kx + x) * Ky + y];
+            }
+        }
+        if (s0 == (int)threadIdx.x) {
+            load_twiddles<NT>(tz, twz, Kz);
+            load_twiddles<NT>(ty, twy, Ky);
+        }
+#pragma unroll
+        for (int s = 0; s < kConvBatch; s++) {
+            const int idx = s0 + s * NT;
+            if (idx < np) {
+                const int k = div_small(idx, rky), y = idx - k * Ky;
+                T[pd(k) * yp + pd(y)] = v[s];
+            }
+        }
+    }
+    __syncthreads();
+    lds_fft<1, NT, RM>(T, H, yp, 1, py, ty);
+    __syncthreads();
+    // rows (2p, 2p + 1) repacked into one complex z-line over all kz, each
+    // extended by Hermitian symmetry (self-conjugate bins keep their real
+    // part), as inv_z_body does
+    for (int o = threadIdx.x; o < nrp * Kz; o += NT) {
+        const int p = div_small(o, rkz), k = o - p * Kz;
+        const bool hi = k > Kz / 2;
+        const int kk = hi ? Kz - k : k;
+        const bool selfc = (kk == 0) || (2 * kk == Kz);
+        const int tl = pd(kk) * yp;
+        float2 A = T[tl + pd(2 * p)];
+        float2 B = (2 * p + 1 < Ky) ? T[tl + pd(2 * p + 1)] : make_float2(0.f, 0.f);
+        if (hi) {
+            A.y = -A.y;
+            B.y = -B.y;
+        }
+        if (selfc) {
+            A.y = 0.f;
+            B.y = 0.f;
+        }
+        pk[pd(p) * zp + pd(k)] = make_float2(A.x - B.y, A.y + B.x);
+    }
+    __syncthreads();
+    lds_fft<1, NT, RM>(pk, nrp, zp, 1, pz, tz);
+    __syncthreads();
+    float* Rp = R + (size_t)x * Ky * Kz;
+    for (int idx = threadIdx.x; idx < nrp * Kz; idx += NT) {
+        const int p = div_small(idx, rkz), z = idx - p * Kz;
+        const float2 v = pk[pd(p) * zp + pd(z)];
+        Rp[(2 * p) * Kz + z] = v.x;
+        if (2 * p + 1 < Ky) Rp[(2 * p + 1) * Kz + z] = v.y;
+    }
+}
+
+template <int RM>
+__global__ __launch_bounds__(kYZThreads) SNB_PME_CHAIN void inv_yz_kernel(const float2* __restrict__ FQ,
+                                                                          float* __restrict__ R, int Kx, int Ky,
+                                                                          int Kz, FftPlan pz, FftPlan py,
+                                                                          const float2* __restrict__ twz,
+                                                                          const float2* __restrict__ twy) {
+    inv_yz_body<RM>(blockIdx.x, FQ, R, Kx, Ky, Kz, pz, py, twz, twy);
+}
+
 // PME_Final (PME_force.cu:485-550): 16 lanes per atom (lane = kx, ky), each
 // walking the 4 kz points; force += -q sum R dtheta (K/L).
 __device__ __forceinline__ void pme_gather_body(int blk, int n, const unsigned* __restrict__ base,
@@ -1150,6 +1391,16 @@ size_t inv_z_lds(int Kz) {
 }
 size_t conv2d_lds(int Kx, int Ky) { return (size_t)(padded_len(Kx) * line_pitch(Ky) + Kx + Ky) * sizeof(float2); }
 size_t lines_lds(int K) { return (size_t)(kLines * line_pitch(K) + K) * sizeof(float2); }
+size_t spread_yz_lds(int Ky, int Kz) {
+    return (size_t)(yz_acc(Ky, Kz) + yz_lines(Ky, Kz) + Kz + Ky) * sizeof(float2);
+}
+size_t inv_yz_lds(int Ky, int Kz) { return (size_t)(yz_tile(Ky, Kz) + yz_lines(Ky, Kz) + Kz + Ky) * sizeof(float2); }
+// LDS of a convolution form's launches (2: the slab form's spread and
+// inverse, which replace spread_z and inv_z, and conv_x)
+size_t form_lds(int form, int Kx, int Ky, int Kz) {
+    if (form == 2) return std::max({spread_yz_lds(Ky, Kz), inv_yz_lds(Ky, Kz), lines_lds(Kx)});
+    return form == 1 ? std::max(lines_lds(Kx), lines_lds(Ky)) : conv2d_lds(Kx, Ky);
+}
 
 // Lets every transform kernel use up to `bytes` of dynamic LDS (once per
 // size; the attribute is per function).
@@ -1171,7 +1422,11 @@ int raise_lds_limits(size_t bytes) {
                          (const void*)conv_y_forward_kernel<kRadixAll>, (const void*)conv_y_forward_kernel<kRadix8>,
                          (const void*)conv_y_forward_kernel<kRadix34>,  (const void*)conv_y_forward_kernel<kRadix248>,
                          (const void*)conv_y_inverse_kernel<kRadixAll>, (const void*)conv_y_inverse_kernel<kRadix8>,
-                         (const void*)conv_y_inverse_kernel<kRadix34>,  (const void*)conv_y_inverse_kernel<kRadix248>};
+                         (const void*)conv_y_inverse_kernel<kRadix34>,  (const void*)conv_y_inverse_kernel<kRadix248>,
+                         (const void*)spread_yz_kernel<kRadixAll>,      (const void*)spread_yz_kernel<kRadix8>,
+                         (const void*)spread_yz_kernel<kRadix34>,       (const void*)spread_yz_kernel<kRadix248>,
+                         (const void*)inv_yz_kernel<kRadixAll>,         (const void*)inv_yz_kernel<kRadix8>,
+                         (const void*)inv_yz_kernel<kRadix34>,          (const void*)inv_yz_kernel<kRadix248>};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
             set_error("cannot raise the LDS limit of the PME kernels to %zu B", bytes);
@@ -1290,13 +1545,14 @@ extern "C" int sponge_pme_create(int atom_numbers, const float box_length[3], fl
     pme->conv_split = (Kx * Ky > kConvPlaneMax || !plane_fits) ? 1 : 0;
     if (tune.conv_split == 0 && plane_fits) pme->conv_split = 0;
     if (tune.conv_split == 1) pme->conv_split = 1;
+    if (tune.conv_split == 2 && form_lds(2, Kx, Ky, Kz) <= (size_t)max_lds) pme->conv_split = 2;
     pme->conv_threads = tune.conv_threads == 2 * kConvThreads ? 2 * kConvThreads : kConvThreads;
     pme->step_form = tune.step_form == 1 ? 1 : 0;
     if (const char* env = getenv("SPONGE_PME_SPLIT_CONV")) {
         if (env[0] == '1') pme->conv_split = 1;
         if (env[0] == '0' && conv2d_lds(Kx, Ky) <= (size_t)max_lds) pme->conv_split = 0;
     }
-    const size_t conv_lds = pme->conv_split ? std::max(lines_lds(Kx), lines_lds(Ky)) : conv2d_lds(Kx, Ky);
+    const size_t conv_lds = form_lds(pme->conv_split, Kx, Ky, Kz);
     const size_t lds_need = std::max({spread_z_lds(Kz), inv_z_lds(Kz), conv_lds});
     if (lds_need > (size_t)max_lds || Kx > kMaxGridAxis || Ky > kMaxGridAxis || Kz > kMaxGridAxis) {
         set_error("PME grid %dx%dx%d too large for the in-LDS transforms (%zu B of LDS, %d available)", Kx, Ky, Kz,
@@ -1460,6 +1716,21 @@ int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s) {
     plan_axis(Kz, &pz);
     const int nyb = row_blocks(Ky);
     double* acc = energy ? pme->acc : nullptr;
+    if (pme->conv_split == 2) {
+        FftPlan py;
+        plan_axis(Ky, &py);
+        const int set = kernel_set(radix_mask(pz) | radix_mask(py));
+        if (pme->ev[0])
+            SNB_RADIX_DISPATCH_EV(set, spread_yz_kernel, Kx, kYZThreads, spread_yz_lds(Ky, Kz), s, pme->ev[0],
+                                  pme->ev[1], pme->bin_base, pme->bin_frac, pme->bin_count, pme->bin_cap, pme->FQ, Kx,
+                                  Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1], acc);
+        else
+            SNB_RADIX_DISPATCH(set, spread_yz_kernel, Kx, kYZThreads, spread_yz_lds(Ky, Kz), s, pme->bin_base,
+                               pme->bin_frac, pme->bin_count, pme->bin_cap, pme->FQ, Kx, Ky, Kz, pz, py,
+                               pme->d_tw[2], pme->d_tw[1], acc);
+        SNB_CHECK_LAUNCH();
+        return SPONGE_NB_OK;
+    }
     if (pme->ev[0]) {
         SNB_RADIX_DISPATCH_EV(kernel_set(radix_mask(pz)), spread_z_kernel, Kx * nyb, kZThreads, spread_z_lds(Kz), s,
                               pme->ev[0], pme->ev[1], pme->bin_base, pme->bin_frac, pme->bin_count, pme->bin_atoms,
@@ -1474,15 +1745,28 @@ int snb_pme_spread(sponge_pme* pme, int energy, hipStream_t s) {
     return SPONGE_NB_OK;
 }
 
-// conv2d_kernel then inv_z_kernel, each in the smallest radix set of its axes.
-static int convolve_launch(sponge_pme* pme, int energy, hipStream_t s) {
+// The convolution of a spectrum that holds the z transforms only (forms 0
+// and 1: the plane or the three line launches, then inv_z) or the z and y
+// transforms (form 2: conv_x, then inv_yz), each kernel in the smallest radix
+// set of its axes.
+static int convolve_launch(sponge_pme* pme, int energy, hipStream_t s, int form) {
     const int Kx = pme->K[0], Ky = pme->K[1], Kz = pme->K[2], H = Kz / 2 + 1;
     FftPlan px, py, pz;
     plan_axis(Kx, &px);
     plan_axis(Ky, &py);
     plan_axis(Kz, &pz);
     const int nyb = row_blocks(Ky);
-    if (pme->conv_split) {
+    if (form == 2) {
+        const dim3 gx(ceil_div(Ky, kLines), H);
+        SNB_RADIX_DISPATCH(kernel_set(radix_mask(px)), conv_x_kernel, gx, kLineThreads, lines_lds(Kx), s, pme->FQ,
+                           pme->d_bc, Kx, Ky, Kz, px, pme->d_tw[0], energy, pme->acc, pme->bin_count);
+        SNB_CHECK_LAUNCH();
+        SNB_RADIX_DISPATCH(kernel_set(radix_mask(pz) | radix_mask(py)), inv_yz_kernel, Kx, kYZThreads,
+                           inv_yz_lds(Ky, Kz), s, pme->FQ, pme->R, Kx, Ky, Kz, pz, py, pme->d_tw[2], pme->d_tw[1]);
+        SNB_CHECK_LAUNCH();
+        return SPONGE_NB_OK;
+    }
+    if (form == 1) {
         const dim3 gy(ceil_div(Kx, kLines), H), gx(ceil_div(Ky, kLines), H);
         const int sy = kernel_set(radix_mask(py)), sx = kernel_set(radix_mask(px));
         SNB_RADIX_DISPATCH(sy, conv_y_forward_kernel, gy, kLineThreads, lines_lds(Ky), s, pme->FQ, Kx, Ky, py,
@@ -1511,7 +1795,9 @@ static int convolve_launch(sponge_pme* pme, int energy, hipStream_t s) {
     return SPONGE_NB_OK;
 }
 
-int snb_pme_convolve(sponge_pme* pme, int energy, hipStream_t s) { return convolve_launch(pme, energy, s); }
+int snb_pme_convolve(sponge_pme* pme, int energy, hipStream_t s) {
+    return convolve_launch(pme, energy, s, pme->conv_split);
+}
 
 // The reciprocal pipeline; forces added to d_frc (original atom order).
 int snb_pme_reciprocal(sponge_pme* pme, const unsigned int* d_uint_crd, const float* d_charge, float* d_frc,
@@ -1548,14 +1834,13 @@ extern "C" int sponge_pme_set_tuning(sponge_pme* pme, int step_form, int conv_sp
     SNB_ARG(pme, "pme_set_tuning");
     SNB_ARG(step_form >= -1 && step_form <= 1, "step_form must be -1, 0 or 1");
     if (step_form >= 0) pme->step_form = step_form;
-    if (conv_split == 0 || conv_split == 1) {
+    if (conv_split >= 0 && conv_split <= 2) {
         int dev = 0, max_lds = 64 * 1024;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
             max_lds = 64 * 1024;
-        const size_t need = conv_split ? std::max(lines_lds(pme->K[0]), lines_lds(pme->K[1]))
-                                       : conv2d_lds(pme->K[0], pme->K[1]);
-        SNB_ARG(need <= (size_t)max_lds, "this grid's planes do not fit one workgroup's LDS");
+        const size_t need = form_lds(conv_split, pme->K[0], pme->K[1], pme->K[2]);
+        SNB_ARG(need <= (size_t)max_lds, "this grid's planes or slabs do not fit one workgroup's LDS");
         const int rc = raise_lds_limits(need);
         if (rc) return rc;
         pme->conv_split = conv_split;
@@ -1677,7 +1962,9 @@ int snb_pme_adjoint_grid(sponge_pme* pme, const float4* wv, const float* scale, 
     (void)H;
     (void)px;
     (void)py;
-    return convolve_launch(pme, 0, s);
+    // the adjoint source was spread with z transforms only: the line form
+    // stands in for the slab form
+    return convolve_launch(pme, 0, s, pme->conv_split == 2 ? 1 : pme->conv_split);
 }
 
 int snb_pme_adj_gather(sponge_pme* pme, int mode, const float4* wv, float sign, float* out, hipStream_t s) {

@@ -131,7 +131,7 @@ def test_default_shapes_follow_the_tuned_table():
             assert sh.prune_radius == pytest.approx(10.0 + r["prune_margin"] * 2.0)
     for r in rows:
         assert r["pair_waves"] in (2, 4, 8) and r["conv_threads"] in (256, 512)
-        assert r["conv_split"] in (-1, 0, 1) and r["step_form"] in (0, 1)
+        assert r["conv_split"] in (-1, 0, 1, 2) and r["step_form"] in (0, 1)
     sh = nb.Shapes()
     assert lib.sponge_nb_default_shapes(0, 20, 10.0, 2.0, ctypes.byref(sh)) != 0
 

@@ -54,8 +54,8 @@ def test_every_shape_gives_the_same_forces(name):
     rc, skin = float(sys_.cutoff), float(sys_.skin)
     prunes = [(base["prune_interval"], base["prune_radius"]), (0, rc), (2, rc + 0.25 * skin)]
     by_waves = {}
-    for w, (split, nt), form, (pi, pr) in itertools.product((2, 4, 8), ((0, 256), (0, 512), (1, 256)), (0, 1),
-                                                            prunes):
+    for w, (split, nt), form, (pi, pr) in itertools.product((2, 4, 8), ((0, 256), (0, 512), (1, 256), (2, 256)),
+                                                            (0, 1), prunes):
         sh = dict(pair_waves=w, conv_split=split, conv_threads=nt, step_form=form, prune_interval=pi,
                   prune_radius=pr)
         eng.set_shapes(sh)
@@ -83,7 +83,7 @@ def test_bad_shapes_are_refused_and_change_nothing():
     eng.update_neighbor_list(forced=1)
     before = eng.shapes().as_dict()
     good = dict(before)
-    for bad in (dict(pair_waves=3), dict(conv_threads=128), dict(step_form=2), dict(conv_split=-1),
+    for bad in (dict(pair_waves=3), dict(conv_threads=128), dict(step_form=2), dict(conv_split=-1), dict(conv_split=3),
                 dict(prune_interval=-1), dict(prune_interval=4, prune_radius=float(sys_.cutoff) - 0.5)):
         sh = dict(good)
         sh.update(bad)
