@@ -97,22 +97,30 @@ int sponge_nl_update(sponge_nl* nl, float* d_crd, const int* d_excluded_list_sta
 int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_list_start, const int* d_excluded_list,
                              const int* d_excluded_numbers);
 
-/* Inner list of the fused step (no counterpart in the reference): every
- * prune_interval fused steps, sponge_nb_force_step keeps only the list
- * entries and pair bits within prune_radius (>= cutoff) of the current
- * coordinates and runs the pair kernel over them; 0 turns it off. Between
- * scheduled prunes, a step at which some atom has moved more than
- * (prune_radius - cutoff) / 2 since the last prune prunes at once (decided on
- * the device), so the inner list never misses a pair inside the cutoff that
- * the full list holds. Default for refresh_interval > 0: the tuned table's
- * values for the atom count (sponge_nb_default_shapes); off otherwise. The
- * next fused step after this call prunes again (a kernel on the null stream
- * marks it: not during a graph capture). The other entry points always use
- * the full list. */
+/* Inner list of the fused step (no counterpart in the reference);
+ * prune_interval 0 turns it off (the cluster-pair list every step). Plain LJ
+ * (force-only steps): the first fused step after each rebuild writes the
+ * list's pairs per atom (every pair within cutoff + skin at the rebuild,
+ * the reference's own list) and the later steps read them; with
+ * prune_radius >= cutoff + skin nothing else is pruned. With a smaller
+ * prune_radius (>= cutoff) those per-atom lists are the outer ones: every
+ * prune_interval fused steps an inner list keeps their pairs within
+ * prune_radius of the current coordinates. Soft-core LJ prunes its
+ * cluster-pair list to prune_radius on the same schedule. Between
+ * scheduled prunes of a distance-pruned list, a step at which some atom has
+ * moved more than (prune_radius - cutoff) / 2 since the last prune prunes at
+ * once (decided on the device), so the inner list never misses a pair
+ * inside the cutoff that the list it is pruned from holds. Default for
+ * refresh_interval > 0: the tuned table's values for the atom count
+ * (sponge_nb_default_shapes); off otherwise. The next fused step after this
+ * call writes the lists again (a kernel on the null stream marks it: not
+ * during a graph capture). The other entry points always use the full
+ * list. */
 int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius);
 
 /* Synchronises. Number of inner-list prunes that the displacement test above
- * caused (beyond the scheduled ones) since the list was created. */
+ * (or a new prune setting) caused beyond the scheduled ones since the list
+ * was created. */
 int sponge_nl_prune_votes(sponge_nl* nl, int* voted_prunes);
 
 /* NEIGHBOR_LIST::Update_Volume (neighbor_list/neighbor_list.cu:645-656). */

@@ -33,9 +33,11 @@ PAIR_WAVES = (2, 4, 8)
 CONV_FORMS = ((0, 256), (0, 512), (1, 256), (2, 256))  # (conv_split, conv_threads)
 STEP_FORMS = (0, 1)
 # (prune_div, margin / skin): pruned every refresh / prune_div steps to
-# cutoff + margin * skin; (1, 1.0) keeps every listed pair, per slot, from
-# the step after each rebuild (no pruning steps but that one)
-PRUNES = ((5, 0.5), (4, 0.5), (10, 0.5), (5, 0.25), (5, 1.0), (2, 0.75), (2, 1.0), (1, 1.0), (0, 0.0))
+# cutoff + margin * skin. A margin of 1 keeps every listed pair, per slot,
+# from the step after each rebuild (one level: no pruning steps but that
+# one, whatever the divisor); below 1 the inner lists are pruned from those
+# (two levels); (0, 0) reads the cluster-pair list every step.
+PRUNES = ((1, 1.0), (5, 0.5), (4, 0.5), (4, 0.75), (2, 0.5), (5, 0.25), (10, 0.25), (0, 0.0))
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE_PATH = os.path.join(HERE, "csrc", "tune_table.hpp")

@@ -216,6 +216,13 @@ struct sponge_nl {
     int atom_in_cap;
     int* atom_in_n;             // slot_cap
     unsigned short* atom_in;    // slot_cap * atom_in_cap
+    // outer per-slot lists (same form and cap), used when the prune radius
+    // is below the list radius: every pair of the full list (within
+    // rc + skin at the rebuild), written by the first fused step after a
+    // rebuild or after a new prune setting (flags[14] holds that step's
+    // number); the inner lists are then pruned from these
+    int* atom_out_n;            // slot_cap
+    unsigned short* atom_out;   // slot_cap * atom_in_cap
     // symmetric exclusion lists in original atom order (partners of atom a:
     // sym_list[sym_start[a] .. sym_start[a+1])), built on the host from the
     // exclusion CSR the first time an update passes those arrays
@@ -343,8 +350,15 @@ struct PruneArgs {
     unsigned short* atom_list = nullptr;  // 16-bit slot offsets
     int atom_cap = 0;
     int* diag = nullptr;        // flags + 11: [0] per-slot overflow, [1] per-slot launches,
-                                // [2] step of the last displacement vote, [4] voted prunes
+                                // [2] step of the last displacement vote, [3] step whose
+                                // per-slot lists are rewritten (a new prune setting),
+                                // [4] voted prunes
     uint4* ref = nullptr;       // slot records at the last prune
+    // two-level per-slot lists (atom_olist non-NULL): the outer lists hold
+    // every pair of the full list, and a prune of the inner lists filters
+    // them instead of the cluster-pair list
+    int* atom_on = nullptr;
+    unsigned short* atom_olist = nullptr;
     const float4* ctab = nullptr;  // the Coulomb table (per-slot force loop), NULL: the formula
     float ctab_scale = 0.f;        // intervals per Angstrom
 };
@@ -388,6 +402,8 @@ int snb_pme_raise_lds(size_t bytes);
 size_t snb_pme_lds_inv(const sponge_pme* pme);
 float snb_nl_rlm(const sponge_nl* nl);
 float snb_prune_vote_r2(float cutoff, float prune_radius);
+// Two-level per-slot lists: an inner prune radius below the list radius.
+inline bool snb_two_level(const sponge_nl* nl) { return nl->prune_interval > 0 && nl->prune_r2 < nl->rl2; }
 int snb_batch_prep(sponge_batch* b, int energy, int from_crd, int overwrite, hipStream_t s);
 int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s);
 int snb_batch_excluded(sponge_batch* b, hipStream_t s);

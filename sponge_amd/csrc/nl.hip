@@ -76,6 +76,7 @@ __device__ __forceinline__ void nl_begin_body(int forced, int* flags, int* col_c
             flags[2] += 1;
             flags[10] = 0;  // the next fused step prunes the new list
             flags[13] = 0;  // no displacement vote on the new list yet
+            flags[14] = 0;
             flags[11] = 0;  // and rewrites the per-slot inner lists
         }
     }
@@ -714,6 +715,8 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
     }
     A((void**)&nl->atom_in_n, nl->slot_cap * sizeof(int));
     A((void**)&nl->atom_in, (size_t)nl->slot_cap * nl->atom_in_cap * sizeof(unsigned short));
+    A((void**)&nl->atom_out_n, nl->slot_cap * sizeof(int));
+    A((void**)&nl->atom_out, (size_t)nl->slot_cap * nl->atom_in_cap * sizeof(unsigned short));
     A((void**)&nl->pack, nl->slot_cap * sizeof(uint4));
     A((void**)&nl->flags, 16 * sizeof(int));
     if (e == hipSuccess) e = hipMemset(nl->flags, 0, 16 * sizeof(int));
@@ -722,6 +725,9 @@ extern "C" int sponge_nl_create(int atom_numbers, const float box_length[3], flo
     if (e == hipSuccess) e = hipMemset(nl->atom_in_n, 0, nl->slot_cap * sizeof(int));
     if (e == hipSuccess)
         e = hipMemset(nl->atom_in, 0, (size_t)nl->slot_cap * nl->atom_in_cap * sizeof(unsigned short));
+    if (e == hipSuccess) e = hipMemset(nl->atom_out_n, 0, nl->slot_cap * sizeof(int));
+    if (e == hipSuccess)
+        e = hipMemset(nl->atom_out, 0, (size_t)nl->slot_cap * nl->atom_in_cap * sizeof(unsigned short));
     // Inner list of the fused step (interval mode): pruned every
     // P = refresh_interval / prune_div fused steps to rc + margin * skin, and
     // at once on a step where an atom has moved more than half the margin
@@ -755,7 +761,7 @@ extern "C" void sponge_nl_destroy(sponge_nl* nl) {
     void* ptrs[] = {nl->old_crd, nl->nl_uint, nl->col_count, nl->col_start, nl->col_atoms, nl->s2o,
                     nl->o2s,     nl->cl_lo,   nl->cl_hi,     nl->list_n,    nl->list,      nl->pack,
                     nl->flags,   nl->list_in, nl->list_in_n, nl->sym_start, nl->sym_list, nl->atom_in_n,
-                    nl->atom_in, nl->dropped, nl->prune_ref};
+                    nl->atom_in, nl->dropped, nl->prune_ref, nl->atom_out_n, nl->atom_out};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete nl;
@@ -899,9 +905,13 @@ extern "C" int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_lis
 }
 
 // The next fused step re-prunes: it sees its own step number in the vote
-// word (flags[13]), as after a displacement vote.
+// word (flags[13]), as after a displacement vote, and rewrites the per-slot
+// lists (flags[14]).
 __global__ void nl_prune_now_kernel(int* flags) {
-    if (threadIdx.x == 0) flags[13] = flags[10] + 1;
+    if (threadIdx.x == 0) {
+        flags[13] = flags[10] + 1;
+        flags[14] = flags[10] + 1;
+    }
 }
 
 extern "C" int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius) {

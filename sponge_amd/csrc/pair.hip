@@ -195,9 +195,13 @@ __global__ __launch_bounds__(256) void step_prep_batch(const BatchSys* __restric
 
 
 // W: i-clusters (waves) per block; the force-only plain-LJ launch takes the
-// handle's tuned value (2, 4 or 8), every other launch 4.
+// handle's tuned value (2, 4 or 8), every other launch 4. The force-only
+// kernels are held to 64 VGPRs (8 waves per SIMD): the per-slot force loops
+// wait on gathered slot records, which the other waves cover; the filtering
+// loop of the two-level lists would take 66 (two registers are spilled
+// around it, outside its loop).
 template <bool ENERGY, bool SOFT, int TB, int W = kPairWaves>
-__global__ __launch_bounds__(W * 64) void pair_kernel(
+__global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(!ENERGY && !SOFT ? 8 : 1))) void pair_kernel(
     const int* __restrict__ flags, const int* __restrict__ list_n, const uint2* __restrict__ list, int list_cap,
     const uint4* __restrict__ pk, const float2* __restrict__ ab, int nt, float cx, float cy, float cz, float rc2,
     float beta, const int* __restrict__ s2o, float* __restrict__ frc, float* __restrict__ atom_e,
@@ -665,6 +669,10 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
         pa.atom_cap = nl->atom_in_cap;
         pa.diag = nl->flags + 11;
         pa.ref = nl->prune_ref;
+        if (!sc && snb_two_level(nl)) {
+            pa.atom_on = nl->atom_out_n;
+            pa.atom_olist = nl->atom_out;
+        }
     }
     const int* list_n = nl->list_n;
     const uint2* list = nl->list;
@@ -752,12 +760,16 @@ int snb_step_prep(const sponge_nl* nl, sponge_lj* lj, sponge_pme* pme, const flo
     for (int d = 0; d < 3; d++) cq[d] = 0.25f * (4294967296.0f / lj->box[d]);
     const int cap = lj->ps.cap;
     const bool prune = nl->prune_interval > 0;
+    // displacement votes: the per-slot lists of plain LJ prune by distance
+    // only in the two-level form (else they hold every pair of the full
+    // list); soft-core LJ prunes its cluster-pair inner list by distance
+    const bool vote = prune && (lj->d_sc != nullptr || snb_two_level(nl));
     step_prep_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(nl->flags, nl->s2o, d_crd, cq[0], cq[1], cq[2], d_uint_crd,
                                                         d_charge, lj->d_type, lj->ps.pk, pme_prep_args(pme), energy,
                                                         lj->ps.acc, prune ? nl->flags + 10 : nullptr, nl->dropped,
                                                         d_frc_assigned, nl->prune_ref, nl->prune_vote_r2,
                                                         lj->cof_dr[0], lj->cof_dr[1], lj->cof_dr[2],
-                                                        prune ? nl->flags + 13 : nullptr,
+                                                        vote ? nl->flags + 13 : nullptr,
                                                         (prune && lj->ps.pk_alloc) ? 1 : 0);
     SNB_CHECK_LAUNCH();
     return SPONGE_NB_OK;

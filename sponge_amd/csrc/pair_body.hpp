@@ -96,70 +96,51 @@ __device__ __forceinline__ bool slot_offset(int i, int j, int nsl, int& off) {
     return d >= -32768 && d <= 32767;
 }
 
-// Per-slot inner lists, written on prune steps: the full list's pairs of
-// cluster ic within pa.r2 at this step's positions, per i slot in list order
-// (lanes ii, ii + 4, ... of a round in lane order). Returns the partner count
-// of slot lane >> 4 (the lane mapping of pair_atoms); a count above atom_cap
+// Per-slot lists of cluster ic: every pair the full list holds (its masks:
+// the pairs within rc + skin at the rebuild, exclusions left out), per i slot
+// in list order (lanes ii, ii + 4, ... of a round in lane order), into
+// `lists` / `counts`; no slot record is read. Returns the partner count of
+// slot lane >> 4 (the lane mapping of pair_atoms); a count above atom_cap
 // (partners past the cap dropped, or a partner beyond 16-bit reach) sends the
-// cluster to the full list until the next prune. The list stores are
-// complete and visible to the whole wave on return.
-template <int TB>
-__device__ __forceinline__ int prune_slots(int ic, int nsl, const int* __restrict__ list_n,
-                                           const uint2* __restrict__ list, int list_cap,
-                                           const uint4* __restrict__ pk, float cx, float cy, float cz,
-                                           const PruneArgs& pa) {
+// cluster to the full list until the lists are written again. The list
+// stores are complete and visible to the whole wave on return.
+__device__ __forceinline__ int slot_lists(int ic, int nsl, const int* __restrict__ list_n,
+                                          const uint2* __restrict__ list, int list_cap, const PruneArgs& pa,
+                                          unsigned short* __restrict__ lists, int* __restrict__ counts) {
     const int lane = threadIdx.x & 63;
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
     const int islot = ic * kClusterSize + ii;
-    const uint4 a = pk[islot];
-    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
     const int n = list_n[ic];
     const uint2* lst = list + (size_t)ic * list_cap;
-    unsigned short* out = pa.atom_list + (size_t)islot * pa.atom_cap;
+    unsigned short* out = lists + (size_t)islot * pa.atom_cap;
     const unsigned long long sel = 0x1111111111111111ull << ii, below = (1ull << lane) - 1ull;
     int count = 0;
     bool far = false;
     for (int base = 0; base < n; base += 64) {
         const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
         const int cnt = min(64, n - base);
-        // four rounds of 4 entries per pass, their j records loaded together
-        // (the loop is latency bound: little arithmetic per record)
-        for (int it = 0; it < cnt; it += 16) {
-            int jcs[4];
-            unsigned ms[4];
-            uint4 bs[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int e = it + 4 * u + g;
-                const int src = min(e, 63);
-                jcs[u] = __shfl((int)my.x, src);
-                const unsigned mm = (unsigned)__shfl((int)my.y, src);
-                ms[u] = e < cnt ? mm : 0u;
-                bs[u] = pk[jcs[u] * kClusterSize + jj];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint4 b = bs[u];
-                const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
-                const float dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
-                const float dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
-                const bool keep = ((ms[u] >> bit) & 1u) && dr2_exact(dx, dy, dz) < pa.r2;
-                const unsigned long long mi = __ballot(keep) & sel;
-                const int pos = count + __popcll(mi & below);
-                // below 32,768 slots every offset fits without the periodic
-                // image (a wave-uniform branch)
-                int off = jcs[u] * kClusterSize + jj - islot;
-                bool reach = true;
-                if (nsl > 32767) reach = slot_offset(islot, jcs[u] * kClusterSize + jj, nsl, off);
-                far = far || (keep && !reach);
-                if (keep && pos < pa.atom_cap) out[pos] = (unsigned short)off;
-                count += __popcll(mi);
-            }
+        // four entries per round, lane (g, jj, ii) holding pair (ii, jj) of
+        // entry g
+        for (int it = 0; it < cnt; it += 4) {
+            const int e = it + g;
+            const int jc = __shfl((int)my.x, min(e, 63));
+            const unsigned m = (unsigned)__shfl((int)my.y, min(e, 63));
+            const bool keep = e < cnt && ((m >> bit) & 1u);
+            const unsigned long long mi = __ballot(keep) & sel;
+            const int pos = count + __popcll(mi & below);
+            // below 32,768 slots every offset fits without the periodic
+            // image (a wave-uniform branch)
+            int off = jc * kClusterSize + jj - islot;
+            bool reach = true;
+            if (nsl > 32767) reach = slot_offset(islot, jc * kClusterSize + jj, nsl, off);
+            far = far || (keep && !reach);
+            if (keep && pos < pa.atom_cap) out[pos] = (unsigned short)off;
+            count += __popcll(mi);
         }
     }
     if (__ballot(far) & sel) count = pa.atom_cap + 1;
     if (lane < kClusterSize) {
-        pa.atom_n[islot] = count;
+        counts[islot] = count;
         if (count > pa.atom_cap) atomicOr(pa.diag, 1);
     }
     // other lanes of the wave read these lists next: the wave's stores must
@@ -169,15 +150,143 @@ __device__ __forceinline__ int prune_slots(int ic, int nsl, const int* __restric
     return __shfl(count, lane >> 4);
 }
 
-// Force-only steps of plain LJ from the per-slot inner lists (TABLE: the
-// Coulomb force factor from its cubic piece in LDS, else from the fitted
-// erfc): lane (i, k),
+// Inner lists of the two-level form from the outer ones, without forces
+// (energy steps, whose forces come from the full list): the partners of each
+// slot of cluster ic within pa.r2 at this step's positions, in outer-list
+// order, with their counts, as pair_atoms_filter writes them. Lane (i, k), i = lane >> 4, takes
+// outer partners 2k and 2k + 1 of every 32 of atom i (one dword of offsets;
+// two dwords in flight per lane); n = the outer count of slot i, nmax the
+// longest of the four (at most atom_cap, so every offset read lies in the
+// slot's list). A partner 2k + h of a round lands after those of the lanes
+// below k, h = 0 before h = 1. Returns the inner count of slot i; the stores
+// are complete and visible to the whole wave on return.
+__device__ __forceinline__ int filter_slots(int ic, int n, int nmax, const uint4* __restrict__ pk, float cx,
+                                            float cy, float cz, const PruneArgs& pa) {
+    const int lane = threadIdx.x & 63, ii = lane >> 4, k = lane & 15;
+    const int islot = ic * kClusterSize + ii;
+    const uint4 a = pk[islot];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
+    const unsigned* __restrict__ lw =
+        reinterpret_cast<const unsigned*>(pa.atom_olist + (size_t)islot * pa.atom_cap);
+    unsigned short* __restrict__ out = pa.atom_list + (size_t)islot * pa.atom_cap;
+    const unsigned below = (1u << k) - 1u;
+    const int nwords = (nmax + 1) >> 1;
+    int count = 0;
+    for (int t = 0; t < nmax; t += 64) {
+        unsigned w[2];
+        uint4 b[4];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int kw = (t >> 1) + 16 * u + k;
+            w[u] = kw < nwords ? lw[kw] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            b[2 * u] = pk[islot + (int)(short)(w[u] & 0xffffu)];
+            b[2 * u + 1] = pk[islot + ((int)w[u] >> 16)];
+        }
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            bool keep[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint4 r = b[2 * u + h];
+                const float dx = (float)(int)((r.x & ~3u) - ax) * cx;
+                const float dy = (float)(int)((r.y & ~3u) - ay) * cy;
+                const float dz = (float)(int)((r.z & ~3u) - az) * cz;
+                keep[h] = t + 32 * u + 2 * k + h < n && dr2_exact(dx, dy, dz) < pa.r2;
+            }
+            const unsigned m0 = (unsigned)(__ballot(keep[0]) >> (16 * ii)) & 0xffffu;
+            const unsigned m1 = (unsigned)(__ballot(keep[1]) >> (16 * ii)) & 0xffffu;
+            const int pos = count + __popc(m0 & below) + __popc(m1 & below);
+            if (keep[0]) out[pos] = (unsigned short)(w[u] & 0xffffu);
+            if (keep[1]) out[pos + (keep[0] ? 1 : 0)] = (unsigned short)(w[u] >> 16);
+            count += __popc(m0) + __popc(m1);
+        }
+    }
+    if (k == 0) pa.atom_n[islot] = count;
+    // other lanes read these lists next (as in slot_lists)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    return count;
+}
+
+// One listed pair of the per-slot force loop (TABLE: the Coulomb force
+// factor from its cubic piece in LDS, else from the fitted erfc): the
+// displacement and r^2 to the partner record b, and the factor f of the
+// pair force f (dx, dy, dz) on atom i; 0 unless listed and r < rc. The pair
+// terms are pair_body's, in FP32.
+template <int TB, bool TABLE>
+__device__ __forceinline__ float slot_pair(const uint4 b, unsigned ax, unsigned ay, unsigned az, float qi,
+                                           const float2* srow, const float4* stab, float cx, float cy, float cz,
+                                           float rc2, float beta, float mlog2e_b2, float ctab_scale, bool listed,
+                                           float& dx, float& dy, float& dz, float& r2) {
+    // with at most 4 types the y and z words carry no type bits
+    dx = (float)(int)((b.x & ~3u) - ax) * cx;
+    dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
+    dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
+    r2 = dr2_exact(dx, dy, dz);
+    const bool in = listed && r2 < rc2;
+    // masked lanes may give inf / NaN below; the final select discards them
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rinv2 = rinv * rinv;
+    const float rinv6 = rinv2 * rinv2 * rinv2;
+    const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
+    const float2 AB = srow[tj];
+    const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
+    const float qq = qi * __uint_as_float(b.w);
+    float fc;
+    if (TABLE) {
+        // the force factor from its cubic piece on [k h, (k + 1) h]
+        const float t = (r2 * rinv) * ctab_scale;
+        const unsigned kk = min((unsigned)t, (unsigned)(kCoulTab - 1));
+        const float u = t - (float)kk;
+        const float4 c = stab[kk];
+        fc = qq * fmaf(fmaf(fmaf(c.w, u, c.z), u, c.y), u, c.x);
+    } else {
+        const float x = beta * (r2 * rinv);
+        const float ex_ = __builtin_amdgcn_exp2f(mlog2e_b2 * r2);
+        const float ecx = erfcx_poly(x);
+        fc = qq * ex_ * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
+    }
+    return in ? flj - fc : 0.f;
+}
+
+// End of the per-slot force loop: the folded exclusion corrections of atom i
+// (lane k takes partners k, k + 16, ...), the 16-lane sum and the store.
+__device__ __forceinline__ void slot_finish(int islot, int k, const uint4* __restrict__ pk, unsigned ax, unsigned ay,
+                                            unsigned az, float qi, float cx, float cy, float cz, float beta,
+                                            const int* __restrict__ s2o, float* __restrict__ frc, int overwrite,
+                                            const ExclArgs& ex, float fix, float fiy, float fiz) {
+    const int at = s2o[islot];
+    if (ex.start && at >= 0) excluded_partners(ex, pk, at, k, 16, ax, ay, az, qi, cx, cy, cz, beta, fix, fiy, fiz);
+    // i force: sum the 16 lanes of atom i
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        fix += __shfl_xor(fix, o);
+        fiy += __shfl_xor(fiy, o);
+        fiz += __shfl_xor(fiz, o);
+    }
+    if (k == 0 && at >= 0 && frc) {
+        // every atom has exactly one slot, so this is the only writer
+        if (overwrite) {
+            frc[3 * at] = fix;
+            frc[3 * at + 1] = fiy;
+            frc[3 * at + 2] = fiz;
+        } else {
+            frc[3 * at] += fix;
+            frc[3 * at + 1] += fiy;
+            frc[3 * at + 2] += fiz;
+        }
+    }
+}
+
+// Force-only steps of plain LJ from the per-slot lists: lane (i, k),
 // i = lane >> 4, takes partners 2k and 2k + 1 of every 32 of atom i of the
 // wave's cluster (n of them; nmax the longest of the four lists) from one
 // dword of two 16-bit slot offsets, so every lane holds listed pairs (the
 // cluster form evaluates all 16 pairs of each listed cluster pair, in cutoff
-// or not). The pair terms are pair_body's, in FP32; a dword of offsets is
-// fetched two rounds ahead and the first record of a round one round ahead.
+// or not). A dword of offsets is fetched two rounds ahead and the first
+// record of a round one round ahead.
 // Lanes past the end of a list read a record
 // too (offsets left from earlier prunes, or zero: always inside the records
 // and their copies) and are masked out.
@@ -209,37 +318,9 @@ __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4*
         const uint4 bn = pk[islot + (int)(short)(wn & 0xffffu)];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            const uint4 b = h ? b1 : b0;
-            // with at most 4 types the y and z words carry no type bits
-            const float dx = (float)(int)((b.x & ~3u) - ax) * cx;
-            const float dy = (float)(int)((TB == 2 ? b.y : b.y & ~3u) - ay) * cy;
-            const float dz = (float)(int)((TB == 2 ? b.z : b.z & ~3u) - az) * cz;
-            const float r2 = dr2_exact(dx, dy, dz);
-            const bool in = t + 2 * k + h < n && r2 < rc2;
-            // masked lanes may give inf / NaN below; the final select
-            // discards them
-            const float rinv = __builtin_amdgcn_rsqf(r2);
-            const float rinv2 = rinv * rinv;
-            const float rinv6 = rinv2 * rinv2 * rinv2;
-            const int tj = TB == 2 ? (int)(b.x & 3u) : unpack_type(b);
-            const float2 AB = srow[tj];
-            const float flj = (AB.y - AB.x * rinv6) * rinv6 * rinv2;
-            const float qq = qi * __uint_as_float(b.w);
-            float fc;
-            if (TABLE) {
-                // the force factor from its cubic piece on [k h, (k + 1) h]
-                const float t = (r2 * rinv) * pa.ctab_scale;
-                const unsigned kk = min((unsigned)t, (unsigned)(kCoulTab - 1));
-                const float u = t - (float)kk;
-                const float4 c = stab[kk];
-                fc = qq * fmaf(fmaf(fmaf(c.w, u, c.z), u, c.y), u, c.x);
-            } else {
-                const float x = beta * (r2 * rinv);
-                const float ex_ = __builtin_amdgcn_exp2f(mlog2e_b2 * r2);
-                const float ecx = erfcx_poly(x);
-                fc = qq * ex_ * fmaf(kTwoOverSqrtPi, x, ecx) * rinv * rinv2;
-            }
-            const float f = in ? flj - fc : 0.f;
+            float dx, dy, dz, r2;
+            const float f = slot_pair<TB, TABLE>(h ? b1 : b0, ax, ay, az, qi, srow, stab, cx, cy, cz, rc2, beta,
+                                                 mlog2e_b2, pa.ctab_scale, t + 2 * k + h < n, dx, dy, dz, r2);
             fix += f * dx;
             fiy += f * dy;
             fiz += f * dz;
@@ -248,27 +329,61 @@ __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4*
         w = wn;
         wn = w2;
     }
-    const int at = s2o[islot];
-    if (ex.start && at >= 0) excluded_partners(ex, pk, at, k, 16, ax, ay, az, qi, cx, cy, cz, beta, fix, fiy, fiz);
-    // i force: sum the 16 lanes of atom i
+    slot_finish(islot, k, pk, ax, ay, az, qi, cx, cy, cz, beta, s2o, frc, overwrite, ex, fix, fiy, fiz);
+}
+
+// The same force loop over the outer lists of the two-level form that also
+// writes the inner lists: the partners within pa.r2 at this step's
+// positions, with their counts, in the order the loop meets them (per round
+// of 32 outer partners, the even ones 2k in k order, then the odd ones
+// 2k + 1). Nothing is
+// fetched ahead across rounds (the kernel keeps its 64 VGPRs; the other waves
+// of the SIMD cover the latency).
+template <int TB, bool TABLE>
+__device__ __forceinline__ void pair_atoms_filter(int ic, int n, int nmax, const uint4* __restrict__ pk,
+                                                  const float2* sab, const float4* stab, int nt, float cx, float cy,
+                                                  float cz, float rc2, float beta, const int* __restrict__ s2o,
+                                                  float* __restrict__ frc, int overwrite, const PruneArgs& pa,
+                                                  const ExclArgs& ex) {
+    const int lane = threadIdx.x & 63, ii = lane >> 4, k = lane & 15;
+    const int islot = ic * kClusterSize + ii;
+    const uint4 a = pk[islot];
+    const unsigned ax = a.x & ~3u, ay = a.y & ~3u, az = a.z & ~3u;
+    const float qi = __uint_as_float(a.w);
+    const int ti = TB == 2 ? (int)(a.x & 3u) : unpack_type(a);
+    const float2* srow = sab + ti * nt;
+    const float mlog2e_b2 = -1.4426950408889634f * (beta * beta);
+    const unsigned lo = (unsigned)islot * (unsigned)pa.atom_cap;  // below 2^32: the lists' allocation
+    const unsigned* __restrict__ lw = reinterpret_cast<const unsigned*>(pa.atom_olist) + (lo >> 1);
+    const int nwords = (nmax + 1) >> 1;
+    float fix = 0.f, fiy = 0.f, fiz = 0.f;
+    int count = 0;
+    for (int t = 0; t < nmax; t += 32) {
+        const int kw = (t >> 1) + k;
+        const unsigned w = kw < nwords ? lw[kw] : 0u;
+        const uint4 b0 = pk[islot + (int)(short)(w & 0xffffu)];
+        const uint4 b1 = pk[islot + ((int)w >> 16)];
+        const unsigned below = (1u << k) - 1u;
 #pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-        fix += __shfl_xor(fix, o);
-        fiy += __shfl_xor(fiy, o);
-        fiz += __shfl_xor(fiz, o);
-    }
-    if (k == 0 && at >= 0 && frc) {
-        // every atom has exactly one slot, so this is the only writer
-        if (overwrite) {
-            frc[3 * at] = fix;
-            frc[3 * at + 1] = fiy;
-            frc[3 * at + 2] = fiz;
-        } else {
-            frc[3 * at] += fix;
-            frc[3 * at + 1] += fiy;
-            frc[3 * at + 2] += fiz;
+        for (int h = 0; h < 2; h++) {
+            float dx, dy, dz, r2;
+            const bool listed = t + 2 * k + h < n;
+            const float f = slot_pair<TB, TABLE>(h ? b1 : b0, ax, ay, az, qi, srow, stab, cx, cy, cz, rc2, beta,
+                                                 mlog2e_b2, pa.ctab_scale, listed, dx, dy, dz, r2);
+            fix += f * dx;
+            fiy += f * dy;
+            fiz += f * dz;
+            // atom ii's kept partners 2k + h of this round (a 16-bit mask
+            // over k), after those of h = 0
+            const bool keep = listed && r2 < pa.r2;
+            const unsigned m = (unsigned)(__ballot(keep) >> (16 * ii)) & 0xffffu;
+            if (keep) pa.atom_list[lo + (unsigned)(count + __popc(m & below))] =
+                (unsigned short)(h ? w >> 16 : w & 0xffffu);
+            count += __popc(m);
         }
     }
+    if (k == 0) pa.atom_n[islot] = count;
+    slot_finish(islot, k, pk, ax, ay, az, qi, cx, cy, cz, beta, s2o, frc, overwrite, ex, fix, fiy, fiz);
 }
 
 // Cutoff test shared with the oracle: FP32 dr2 = (dx*dx + dy*dy) + dz*dz with
@@ -290,45 +405,83 @@ __device__ __forceinline__ void pair_wave(
     float* __restrict__ atom_v, float* __restrict__ atom_d, double* __restrict__ acc, int overwrite,
     const float4* __restrict__ sc, const PruneArgs& pa, const ExclArgs& ex) {
     const int lane = threadIdx.x & 63;
-    // Inner list (pa.interval > 0): on every pa.interval-th fused step since
-    // the last rebuild, and on a step whose prep kernel voted (an atom moved
-    // more than half the prune margin since the last prune), this launch
-    // prunes the full list to the pairs within pa.r2 and keeps the slot
-    // records it pruned at; the other steps read the pruned list. c = this
-    // step's number since the rebuild (the count of completed steps is
-    // advanced by the step's gather, after this kernel). Plain LJ keeps the
-    // pruned pairs per i slot (force-only steps read them, prune steps
-    // included; energy steps read the full list), soft-core LJ as cluster
-    // pairs.
-    bool use_inner = false, write_inner = false;
+    // Inner lists (pa.interval > 0). c = this step's number since the
+    // rebuild (the count of completed steps is advanced by the step's
+    // gather, after this kernel); a step is scheduled every pa.interval-th
+    // step, voted when its prep kernel found an atom that moved more than
+    // half the prune margin since the last prune, and renewing when it is
+    // the first after a rebuild or after a new prune setting.
+    // Plain LJ keeps per-slot lists (pa.atom_list; force-only steps read
+    // them, energy steps the full list). One level (a prune radius at or
+    // beyond rc + skin): the lists hold every pair of the full list, written
+    // on renewing steps; no other step prunes and nothing is voted (the pairs
+    // are those of the list's own radius at the rebuild). Two levels
+    // (pa.atom_olist set): those lists are the outer ones, and the inner
+    // lists, the outer pairs within pa.r2, are written on scheduled, voted
+    // and renewing steps. Soft-core LJ prunes the cluster-pair list to pa.r2
+    // on scheduled and voted steps and reads the pruned one in between. A
+    // step that prunes by distance keeps the slot records it pruned at.
+    const bool two = pa.atom_olist != nullptr;
+    const bool slots = !SOFT && pa.atom_list != nullptr && pa.interval > 0;
+    bool use_inner = false, write_inner = false, write_outer = false;
     if (pa.interval > 0) {
         const int c = flags[10] + 1;
+        const bool renew = c <= 1 || pa.diag[3] == c;
         const bool scheduled = c <= 1 || (c - 1) % pa.interval == 0;
         const bool voted = pa.diag[2] == c;
-        write_inner = scheduled || voted;
+        if (slots) {
+            write_outer = two && renew;
+            write_inner = two ? renew || scheduled || voted : renew;
+        } else {
+            write_inner = scheduled || voted;
+        }
         use_inner = !write_inner;
         if (write_inner && lane < kClusterSize) pa.ref[ic * kClusterSize + lane] = pk[ic * kClusterSize + lane];
-        if (voted && !scheduled && ic == 0 && lane == 0) atomicAdd(pa.diag + 4, 1);
+        if (voted && !scheduled && (!slots || two) && ic == 0 && lane == 0) atomicAdd(pa.diag + 4, 1);
     }
-    const bool slots = !SOFT && pa.atom_list != nullptr && pa.interval > 0;
     if (slots && !ENERGY) {
-        const int n = write_inner ? prune_slots<TB>(ic, flags[3], list_n, list, list_cap, pk, cx, cy, cz, pa)
-                                  : pa.atom_n[ic * kClusterSize + (lane >> 4)];
-        const int nmax = max(max(__shfl(n, 0), __shfl(n, 16)), max(__shfl(n, 32), __shfl(n, 48)));
         if (ic == 0 && lane == 0) atomicAdd(pa.diag + 1, 1);
-        if (nmax <= pa.atom_cap) {
-            if (stab)
-                pair_atoms<TB, true>(ic, n, nmax, pk, sab, stab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite, pa,
-                                     ex);
-            else
-                pair_atoms<TB, false>(ic, n, nmax, pk, sab, stab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite, pa,
-                                      ex);
-            return;
+        if (two && write_inner) {
+            const int n = write_outer ? slot_lists(ic, flags[3], list_n, list, list_cap, pa, pa.atom_olist, pa.atom_on)
+                                      : pa.atom_on[ic * kClusterSize + (lane >> 4)];
+            const int nmax = max(max(__shfl(n, 0), __shfl(n, 16)), max(__shfl(n, 32), __shfl(n, 48)));
+            if (nmax <= pa.atom_cap) {
+                if (stab)
+                    pair_atoms_filter<TB, true>(ic, n, nmax, pk, sab, stab, nt, cx, cy, cz, rc2, beta, s2o, frc,
+                                                overwrite, pa, ex);
+                else
+                    pair_atoms_filter<TB, false>(ic, n, nmax, pk, sab, stab, nt, cx, cy, cz, rc2, beta, s2o, frc,
+                                                 overwrite, pa, ex);
+                return;
+            }
+            // an outer list overflowed: the inner lists follow it to the
+            // full list until the outer ones are written again
+            if (lane < kClusterSize) pa.atom_n[ic * kClusterSize + lane] = pa.atom_cap + 1;
+        } else {
+            const int n = write_inner ? slot_lists(ic, flags[3], list_n, list, list_cap, pa, pa.atom_list, pa.atom_n)
+                                      : pa.atom_n[ic * kClusterSize + (lane >> 4)];
+            const int nmax = max(max(__shfl(n, 0), __shfl(n, 16)), max(__shfl(n, 32), __shfl(n, 48)));
+            if (nmax <= pa.atom_cap) {
+                if (stab)
+                    pair_atoms<TB, true>(ic, n, nmax, pk, sab, stab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite,
+                                         pa, ex);
+                else
+                    pair_atoms<TB, false>(ic, n, nmax, pk, sab, stab, nt, cx, cy, cz, rc2, beta, s2o, frc, overwrite,
+                                          pa, ex);
+                return;
+            }
         }
         write_inner = false;  // a slot overflowed: this cluster's forces from the full list
+        write_outer = false;
     }
-    const bool write_atoms = slots && write_inner;      // energy steps that prune
+    // energy steps that write the per-slot lists take every listed pair from
+    // the full list (the outer ones in the two-level form, which then writes
+    // the inner ones from them below)
+    const bool write_atoms = slots && (two ? write_outer : write_inner);
     const bool write_cluster = !slots && write_inner;
+    unsigned short* const wl = two ? pa.atom_olist : pa.atom_list;
+    int* const wn = two ? pa.atom_on : pa.atom_n;
+    const float wr2 = slots ? __builtin_huge_valf() : pa.r2;
     if (slots) use_inner = false;
     const int ii = lane & 3, jj = (lane >> 2) & 3, g = lane >> 4, bit = lane & 15;
     const int islot = ic * kClusterSize + ii;
@@ -368,7 +521,7 @@ __device__ __forceinline__ void pair_wave(
             const float r2 = dr2_exact(dx, dy, dz);
             const bool in = ((m >> bit) & 1u) && r2 < rc2;
             if (write_inner) {
-                const bool keep = ((m >> bit) & 1u) && r2 < pa.r2;
+                const bool keep = ((m >> bit) & 1u) && r2 < wr2;
                 const unsigned long long kb = __ballot(keep);
                 if (write_cluster) {
                     const unsigned mk = (unsigned)(kb >> (g * 16)) & 0xffffu;
@@ -385,8 +538,7 @@ __device__ __forceinline__ void pair_wave(
                     int off = 0;
                     const bool reach = slot_offset(islot, jc * kClusterSize + jj, nsl, off);
                     atom_far = atom_far || (keep && !reach);
-                    if (keep && pos < pa.atom_cap)
-                        pa.atom_list[(size_t)islot * pa.atom_cap + pos] = (unsigned short)off;
+                    if (keep && pos < pa.atom_cap) wl[(size_t)islot * pa.atom_cap + pos] = (unsigned short)off;
                     atom_count += __popcll(mi);
                 }
             }
@@ -458,8 +610,19 @@ __device__ __forceinline__ void pair_wave(
     if (write_cluster && lane == 0) pa.in_n[ic] = inner_count;
     if (write_atoms && (__ballot(atom_far) & (0x1111111111111111ull << ii))) atom_count = pa.atom_cap + 1;
     if (write_atoms && lane < kClusterSize) {
-        pa.atom_n[islot] = atom_count;
+        wn[islot] = atom_count;
         if (atom_count > pa.atom_cap) atomicOr(pa.diag, 1);
+    }
+    if (ENERGY && slots && two && write_inner) {
+        // the inner lists from the outer ones (no force: this step's forces
+        // come from the full list)
+        if (write_atoms) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const int no = pa.atom_on[ic * kClusterSize + (lane >> 4)];
+        const int nomax = max(max(__shfl(no, 0), __shfl(no, 16)), max(__shfl(no, 32), __shfl(no, 48)));
+        if (nomax <= pa.atom_cap)
+            filter_slots(ic, no, nomax, pk, cx, cy, cz, pa);
+        else if (lane < kClusterSize)
+            pa.atom_n[ic * kClusterSize + lane] = pa.atom_cap + 1;
     }
     if (ex.start) {
         // this cluster's atoms: lane (ii, k) takes the k-th exclusion partner

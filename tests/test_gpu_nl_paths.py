@@ -191,8 +191,15 @@ def test_column_overflow_is_reported_and_defined():
             forces.append(eng.frc.cpu().numpy().copy())
         f = forces[-1]
         assert np.isfinite(f).all()
-        for g in forces[:-1]:
-            assert np.array_equal(g, f), "repeated steps at the same coordinates differ"
+        assert np.array_equal(forces[1], f), "repeated steps at the same coordinates differ"
+        if prune:
+            # the pruning step sums its pairs in the order of the lists it
+            # filters, the later steps in the inner lists': equal but for the
+            # last bits
+            np.testing.assert_allclose(forces[0], f, rtol=0, atol=1e-6 * float(np.abs(f).max()))
+            assert np.array_equal(forces[0] == 0.0, f == 0.0)
+        else:
+            assert np.array_equal(forces[0], f), "repeated steps at the same coordinates differ"
         left = np.where(np.all(f == 0.0, axis=1))[0]
         say(f"    prune {prune}: atoms left out: {left.tolist()}")
         assert 1 <= len(left) <= 8
@@ -314,16 +321,81 @@ def test_prune_displacement_vote():
     assert err < TOL_ABS
 
 
-def test_pruned_list_under_motion():
+def test_two_level_outer_lists_from_the_rebuild():
+    """Two-level per-slot lists (prune radius below the list radius): on a
+    box with skin 0.5 A the outer lists hold the full list's pairs (within
+    rc + 0.5 A at the rebuild), and the inner lists (rc + 0.2 A) are pruned
+    from them at the steps' own positions. An oxygen pair 8.06-8.09 A apart
+    at the rebuild is pulled apart by 0.45 A before the first fused step
+    (8.51-8.54 A there: outer lists pruned at that step's positions would
+    leave it out), then pushed 0.45 A towards the other atom from where it
+    was at the rebuild (7.61-7.64 A: inside rc = 8 A). The second step
+    re-prunes the inner lists on the displacement vote and must find the
+    pair: forces equal the oracle's at both coordinates, and leaving the pair
+    out would move the force on one of its atoms by more than the
+    tolerance."""
+    import torch
+    from sponge_amd import nb
+    sys_ = dataclasses.replace(system.config(0), skin=0.5)
+    say(f"[case] two-level lists, n={sys_.n}, skin {sys_.skin}")
+    eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    eng.nl.set_prune(20, sys_.cutoff + 0.2)
+    eng.update_neighbor_list(forced=1)
+    torch.cuda.synchronize()
+    crd = eng.crd.cpu().numpy().astype(np.float64)
+    L = sys_.box.astype(np.float64)
+    oxy = np.arange(0, 3 * sys_.waters, 3)
+    d = crd[oxy][:, None, :] - crd[oxy][None, :, :]
+    d -= L * np.round(d / L)
+    r = np.linalg.norm(d, axis=2)
+    ii, jj = np.where((r > 8.06) & (r < 8.09))
+    assert len(ii), "no oxygen pair 8.06-8.09 A apart"
+    i, j = int(oxy[ii[0]]), int(oxy[jj[0]])
+    v = crd[j] - crd[i]
+    v -= L * np.round(v / L)
+    u = v / np.linalg.norm(v)
+    votes0 = eng.nl.prune_votes()
+    K = (eng.pme.fftx, eng.pme.ffty, eng.pme.fftz)
+    errs = []
+    for step, shift in enumerate((-0.45, 0.45)):
+        moved = crd.copy()
+        moved[i] = np.mod(crd[i] + shift * u, L)
+        s2 = dataclasses.replace(sys_, crd=moved.astype(np.float32))
+        eng.crd.copy_(torch.from_numpy(moved.astype(np.float32)))
+        eng.force_step(0, 0, from_crd=True, overwrite=True)
+        torch.cuda.synchronize()
+        ref = oracle.full_reference(s2, beta=eng.pme.beta, K=K)
+        f = eng.frc.cpu().numpy()
+        errs.append(float(np.abs(f - ref["frc"]).max()))
+        # the first step prunes on schedule; the second on the vote
+        assert eng.nl.prune_votes() == votes0 + step
+    pairs = ref["pairs"]
+    keep = ~(((pairs[:, 0] == min(i, j)) & (pairs[:, 1] == max(i, j))))
+    assert keep.sum() == len(pairs) - 1
+    full = oracle.direct(s2, ref["uint"], pairs, ref["beta"])
+    without = oracle.direct(s2, ref["uint"], pairs[keep], ref["beta"])
+    miss = float(np.abs(full["frc"][[i, j]] - without["frc"][[i, j]]).max())
+    say(f"    pair ({i}, {j}) at {np.linalg.norm(v):.3f} A: max force error {errs[0]:.2e} (first step), "
+        f"{errs[1]:.2e} (voted re-prune); the pair's force {miss:.2e}")
+    assert miss > 3 * TOL_ABS
+    assert max(errs) < TOL_ABS
+
+
+@pytest.mark.parametrize("schedule", ["tuned", "two-level"])
+def test_pruned_list_under_motion(schedule):
     """The 23,570-atom box relaxed to ~300 K, then MD steps with the tuned
-    default prune (sponge_nb_default_shapes, plus displacement votes): at the last
-    step before a refresh the fused step's forces equal a prune-off
-    evaluation, after a forced rebuild, at the same coordinates."""
+    default prune (sponge_nb_default_shapes, plus displacement votes), or
+    with inner lists pruned every 4 steps to rc + 0.25 A from outer lists at
+    rc + skin: at the last step before a refresh the fused step's forces
+    equal a prune-off evaluation, after a forced rebuild, at the same
+    coordinates."""
     import torch
     from sponge_amd import md, nb
     sys_ = system.config(1)
-    say(f"[case] pruned list under motion, n={sys_.n}")
+    say(f"[case] pruned list under motion, n={sys_.n}, {schedule}")
     mde = md.MDEngine(sys_, refresh_interval=20, temperature=300.0, seed=5)
+    if schedule == "two-level":
+        mde.nb.nl.set_prune(4, float(sys_.cutoff) + 0.25)
     mde.prepare()
     mde.equilibrate(600)
     for _ in range(20):

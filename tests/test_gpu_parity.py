@@ -336,9 +336,9 @@ def test_pruned_inner_list(case):
 def test_per_slot_overflow_falls_back_to_the_full_list():
     """The 648-atom water box as a droplet in a box 1.5 times as long: the
     per-slot lists are sized from the mean density (0.30 of the droplet's:
-    224 partners per slot), so the slots inside the droplet (up to ~300
-    partners within the 9 A prune radius) overflow at the prune and their
-    clusters read the full list;
+    224 partners per slot), so the slots inside the droplet (several hundred
+    partners within the 10 A list radius) overflow when the lists are
+    written and their clusters read the full list;
     the forces stay those of the oracle. (At twice the box length a cell
     column would hold 130 atoms for 128 places: that list overflow is a
     reported error, not this case.)"""
