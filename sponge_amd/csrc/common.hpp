@@ -361,6 +361,7 @@ struct PruneArgs {
     unsigned short* atom_olist = nullptr;
     const float4* ctab = nullptr;  // the Coulomb table (per-slot force loop), NULL: the formula
     float ctab_scale = 0.f;        // intervals per Angstrom
+    int pk_records = 0;            // the slot records with their copies on either side
 };
 
 // Excluded-pair correction folded into the pair kernel (start NULL: off):

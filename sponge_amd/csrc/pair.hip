@@ -679,6 +679,7 @@ int snb_pairs(const sponge_nl* nl, const float2* ab, int nt, const int* d_type, 
     int rc = scratch_ensure(ps, nl);
     if (rc) return rc;
     const int cap = ps->cap;
+    pa.pk_records = cap + 2 * kPkHalo;
     if (!gathered) {
         gather_kernel<<<ceil_div(cap, 256), 256, 0, s>>>(nl->flags, nl->s2o, d_uint_crd, d_charge, d_type, ps->pk,
                                                          energy, ps->acc);

@@ -210,6 +210,37 @@ __device__ __forceinline__ int filter_slots(int ic, int n, int nmax, const uint4
     return count;
 }
 
+// Slot records of the per-slot force loops through a structured buffer
+// resource (16-byte records) over the records and their copies on either
+// side: a partner's record is one indexed buffer load at the slot's own
+// index plus the sign-extended 16-bit offset (one SDWA add to decode; no
+// 64-bit address per pair), and an index outside the allocation reads zeros.
+typedef unsigned SlotRecordBits __attribute__((ext_vector_type(4)));
+__device__ SlotRecordBits snb_struct_buffer_load_b128(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
+                                                      int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+
+struct SlotRecords {
+    __amdgpu_buffer_rsrc_t rsrc;
+    int self;  // index of the slot's own record
+};
+
+__device__ __forceinline__ SlotRecords slot_records(const uint4* pk, int islot, const PruneArgs& pa) {
+    SlotRecords r;
+    r.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(pk - kPkHalo), (short)sizeof(uint4), pa.pk_records,
+                                               0x00020000);
+    r.self = islot + kPkHalo;
+    return r;
+}
+
+// The record at a signed 16-bit slot offset (bits 0-15 or 16-31 of w).
+template <bool HIGH>
+__device__ __forceinline__ uint4 slot_record(const SlotRecords& r, unsigned w) {
+    const int off = HIGH ? (int)w >> 16 : (int)(short)(w & 0xffffu);
+    const SlotRecordBits v = snb_struct_buffer_load_b128(r.rsrc, r.self + off, 0, 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
 // One listed pair of the per-slot force loop (TABLE: the Coulomb force
 // factor from its cubic piece in LDS, else from the fitted erfc): the
 // displacement and r^2 to the partner record b, and the factor f of the
@@ -239,7 +270,8 @@ __device__ __forceinline__ float slot_pair(const uint4 b, unsigned ax, unsigned 
         // the force factor from its cubic piece on [k h, (k + 1) h]
         const float t = (r2 * rinv) * ctab_scale;
         const unsigned kk = min((unsigned)t, (unsigned)(kCoulTab - 1));
-        const float u = t - (float)kk;
+        // t - kk for t below the table's end (beyond it the pair is out)
+        const float u = __builtin_amdgcn_fractf(t);
         const float4 c = stab[kk];
         fc = qq * fmaf(fmaf(fmaf(c.w, u, c.z), u, c.y), u, c.x);
     } else {
@@ -307,27 +339,40 @@ __device__ __forceinline__ void pair_atoms(int ic, int n, int nmax, const uint4*
         reinterpret_cast<const unsigned*>(pa.atom_list + (size_t)islot * pa.atom_cap);
     const int nwords = (nmax + 1) >> 1;  // atom_cap is even: the dword stays inside the slot's list
     float fix = 0.f, fiy = 0.f, fiz = 0.f;
-    // offsets two rounds ahead, the first record of a round one round ahead
-    unsigned w = k < nwords ? lw[k] : 0u;
-    unsigned wn = 16 + k < nwords ? lw[16 + k] : 0u;
-    uint4 b0 = pk[islot + (int)(short)(w & 0xffffu)];
-    for (int t = 0; t < nmax; t += 32) {
+    const SlotRecords rec = slot_records(pk, islot, pa);
+    const int nk = n - 2 * k;  // partner t + 2k + h is listed when t + h < nk
+    // One round of 32 partners per atom: pairs (b0, b1) from offset dword
+    // w; the next round's first record bn and the offsets two rounds ahead
+    // are fetched first. Rounds go in pairs so that the registers rotate
+    // once per two rounds.
+    auto round = [&](int t, unsigned w_, unsigned wn_, const uint4& b0_, unsigned& w2_, uint4& bn_) {
         const int k2 = (t >> 1) + 32 + k;
-        const unsigned w2 = k2 < nwords ? lw[k2] : 0u;
-        const uint4 b1 = pk[islot + ((int)w >> 16)];
-        const uint4 bn = pk[islot + (int)(short)(wn & 0xffffu)];
+        w2_ = k2 < nwords ? lw[k2] : 0u;
+        const uint4 b1 = slot_record<true>(rec, w_);
+        bn_ = slot_record<false>(rec, wn_);
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             float dx, dy, dz, r2;
-            const float f = slot_pair<TB, TABLE>(h ? b1 : b0, ax, ay, az, qi, srow, stab, cx, cy, cz, rc2, beta,
-                                                 mlog2e_b2, pa.ctab_scale, t + 2 * k + h < n, dx, dy, dz, r2);
+            const float f = slot_pair<TB, TABLE>(h ? b1 : b0_, ax, ay, az, qi, srow, stab, cx, cy, cz, rc2, beta,
+                                                 mlog2e_b2, pa.ctab_scale, t + h < nk, dx, dy, dz, r2);
             fix += f * dx;
             fiy += f * dy;
             fiz += f * dz;
         }
-        b0 = bn;
-        w = wn;
-        wn = w2;
+    };
+    // offsets two rounds ahead, the first record of a round one round ahead
+    unsigned w = k < nwords ? lw[k] : 0u;
+    unsigned wn = 16 + k < nwords ? lw[16 + k] : 0u;
+    uint4 b0 = slot_record<false>(rec, w);
+    for (int t = 0; t < nmax; t += 64) {
+        unsigned w2, w3;
+        uint4 bn, bnn;
+        round(t, w, wn, b0, w2, bn);
+        if (t + 32 >= nmax) break;
+        round(t + 32, wn, w2, bn, w3, bnn);
+        w = w2;
+        wn = w3;
+        b0 = bnn;
     }
     slot_finish(islot, k, pk, ax, ay, az, qi, cx, cy, cz, beta, s2o, frc, overwrite, ex, fix, fiy, fiz);
 }
@@ -357,12 +402,13 @@ __device__ __forceinline__ void pair_atoms_filter(int ic, int n, int nmax, const
     const unsigned* __restrict__ lw = reinterpret_cast<const unsigned*>(pa.atom_olist) + (lo >> 1);
     const int nwords = (nmax + 1) >> 1;
     float fix = 0.f, fiy = 0.f, fiz = 0.f;
+    const SlotRecords rec = slot_records(pk, islot, pa);
     int count = 0;
     for (int t = 0; t < nmax; t += 32) {
         const int kw = (t >> 1) + k;
         const unsigned w = kw < nwords ? lw[kw] : 0u;
-        const uint4 b0 = pk[islot + (int)(short)(w & 0xffffu)];
-        const uint4 b1 = pk[islot + ((int)w >> 16)];
+        const uint4 b0 = slot_record<false>(rec, w);
+        const uint4 b1 = slot_record<true>(rec, w);
         const unsigned below = (1u << k) - 1u;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
