@@ -368,6 +368,7 @@ __device__ __forceinline__ void nl_pairs_body(
     __shared__ int cand[kPairWaves][kCandCap];
     __shared__ int colj0[kPairWaves][kColCap];
     __shared__ int colpre[kPairWaves][kColCap];
+    __shared__ int colzc[kPairWaves][kColCap];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int ic = lb * kPairWaves + w;
     const int ncl = flags[4];
@@ -411,19 +412,52 @@ __device__ __forceinline__ void nl_pairs_body(
         // The window's columns as one flat range of j-clusters: a table of
         // each column's first cluster and running cluster count, then 64
         // clusters per step whatever their column (instead of one column per
-        // step, most of whose lanes would idle). Same candidate order.
+        // step, most of whose lanes would idle).
+        // Within a column the clusters are sorted by z (their atoms are), so
+        // only those whose z extent meets [lo.z - rlm, hi.z + rlm] of the
+        // i-cluster (periodically: a circular run of the column's clusters,
+        // found by two binary searches) can pass the box test; a window at
+        // least a box length long takes the whole column.
+        const float zpad = rlm + 1.0e-2f;
+        const float zlo = li.z - zpad, zhi = hi.z + zpad;
+        const bool zwin = zhi - zlo < bz;
+        const float za = zlo < 0.f ? zlo + bz : zlo;   // window start in [0, bz)
+        const float zb = zhi >= bz ? zhi - bz : zhi;   // window end in [0, bz)
+        const bool zwrap = zlo < 0.f || zhi >= bz;
         int total = 0;
         for (int base = 0; base < ncols; base += 64) {
             const int k = base + lane;
-            int j0 = 0, cnt = 0;
+            int j0 = 0, cnt = 0, zs = 0, zl = 0;
             if (k < ncols) {
                 const int ox = k / nyr, oy = k - ox * nyr;
                 const int cx = ((ax0 + ox) % ncx + ncx) % ncx, cy = ((ay0 + oy) % ncy + ncy) % ncy;
                 const int c = cx * ncy + cy;
                 j0 = col_start[c] / kClusterSize;
                 cnt = col_start[c + 1] / kClusterSize - j0;
+                zl = cnt;
+                if (zwin && cnt > 0) {
+                    // s: the first cluster whose top reaches za; e: the first
+                    // whose bottom lies above zb
+                    int lo = 0, up = cnt;
+                    while (lo < up) {
+                        const int mid = (lo + up) >> 1;
+                        if (cl_hi[j0 + mid].z < za) lo = mid + 1;
+                        else up = mid;
+                    }
+                    const int s = lo;
+                    lo = 0;
+                    up = cnt;
+                    while (lo < up) {
+                        const int mid = (lo + up) >> 1;
+                        if (cl_lo[j0 + mid].z <= zb) lo = mid + 1;
+                        else up = mid;
+                    }
+                    const int e = lo;
+                    zs = s == cnt ? 0 : s;
+                    zl = zwrap ? min(cnt - s + e, cnt) : max(e - s, 0);
+                }
             }
-            int incl = cnt;
+            int incl = zl;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const int t = __shfl_up(incl, o);
@@ -431,7 +465,8 @@ __device__ __forceinline__ void nl_pairs_body(
             }
             if (k < ncols) {
                 colj0[w][k] = j0;
-                colpre[w][k] = total + incl - cnt;
+                colpre[w][k] = total + incl - zl;
+                colzc[w][k] = zs | (cnt << 16);
             }
             total += __shfl(incl, 63);
         }
@@ -449,7 +484,9 @@ __device__ __forceinline__ void nl_pairs_body(
                     if (colpre[w][mid] <= t) lo = mid;
                     else hi = mid - 1;
                 }
-                jc = colj0[w][lo] + (t - colpre[w][lo]);
+                const int zc = colzc[w][lo], zs = zc & 0xffff, cnt = zc >> 16;
+                const int u = zs + (t - colpre[w][lo]);
+                jc = colj0[w][lo] + (u >= cnt ? u - cnt : u);
             }
             test_box(jc, t < total);
         }
