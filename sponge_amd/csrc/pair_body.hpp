@@ -120,22 +120,31 @@ __device__ __forceinline__ int slot_lists(int ic, int nsl, const int* __restrict
         const uint2 my = base + lane < n ? lst[base + lane] : make_uint2((unsigned)ic, 0u);
         const int cnt = min(64, n - base);
         // four entries per round, lane (g, jj, ii) holding pair (ii, jj) of
-        // entry g
-        for (int it = 0; it < cnt; it += 4) {
-            const int e = it + g;
-            const int jc = __shfl((int)my.x, min(e, 63));
-            const unsigned m = (unsigned)__shfl((int)my.y, min(e, 63));
-            const bool keep = e < cnt && ((m >> bit) & 1u);
-            const unsigned long long mi = __ballot(keep) & sel;
-            const int pos = count + __popcll(mi & below);
-            // below 32,768 slots every offset fits without the periodic
-            // image (a wave-uniform branch)
-            int off = jc * kClusterSize + jj - islot;
-            bool reach = true;
-            if (nsl > 32767) reach = slot_offset(islot, jc * kClusterSize + jj, nsl, off);
-            far = far || (keep && !reach);
-            if (keep && pos < pa.atom_cap) out[pos] = (unsigned short)off;
-            count += __popcll(mi);
+        // entry g; the lane exchanges of four rounds are issued together
+        for (int it = 0; it < cnt; it += 16) {
+            int jcs[4];
+            unsigned ms[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = min(it + 4 * u + g, 63);
+                jcs[u] = __shfl((int)my.x, e);
+                ms[u] = (unsigned)__shfl((int)my.y, e);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int jc = jcs[u];
+                const bool keep = it + 4 * u + g < cnt && ((ms[u] >> bit) & 1u);
+                const unsigned long long mi = __ballot(keep) & sel;
+                const int pos = count + __popcll(mi & below);
+                // below 32,768 slots every offset fits without the periodic
+                // image (a wave-uniform branch)
+                int off = jc * kClusterSize + jj - islot;
+                bool reach = true;
+                if (nsl > 32767) reach = slot_offset(islot, jc * kClusterSize + jj, nsl, off);
+                far = far || (keep && !reach);
+                if (keep && pos < pa.atom_cap) out[pos] = (unsigned short)off;
+                count += __popcll(mi);
+            }
         }
     }
     if (__ballot(far) & sel) count = pa.atom_cap + 1;
