@@ -40,6 +40,18 @@ constexpr int kClusterSize = 4;          // atoms per cluster (4x4 cluster pairs
 // the first and last kPkHalo slots): a 16-bit partner offset from any slot
 // then reads a record inside the allocation.
 constexpr int kPkHalo = 32768;
+
+// 16-bit offset of slot j from slot i in the per-slot lists: j - i, or the
+// offset of j's periodic image (modulo the slot count nsl) when j - i does
+// not fit; false when neither fits. i + offset always indexes the slot
+// records or their copies kPkHalo to either side.
+__device__ __forceinline__ bool slot_offset(int i, int j, int nsl, int& off) {
+    int d = j - i;
+    if (d > 32767) d -= nsl;
+    else if (d < -32768) d += nsl;
+    off = d;
+    return d >= -32768 && d <= 32767;
+}
 // Intervals of the tabulated direct Coulomb force factor
 // G(r) = erfc(beta r) / r^3 + 2 beta / sqrt(pi) exp(-beta^2 r^2) / r^2 over
 // [0, rc]: one float4 of cubic Hermite coefficients each (16 KB, held in LDS
@@ -405,6 +417,15 @@ float snb_nl_rlm(const sponge_nl* nl);
 float snb_prune_vote_r2(float cutoff, float prune_radius);
 // Two-level per-slot lists: an inner prune radius below the list radius.
 inline bool snb_two_level(const sponge_nl* nl) { return nl->prune_interval > 0 && nl->prune_r2 < nl->rl2; }
+// The per-slot lists of every listed pair, which each rebuild writes after
+// the list (the outer ones of the two-level form); NULL while the inner list
+// is off.
+inline unsigned short* snb_nl_slot_lists(const sponge_nl* nl) {
+    return nl->prune_interval > 0 ? (snb_two_level(nl) ? nl->atom_out : nl->atom_in) : nullptr;
+}
+inline int* snb_nl_slot_counts(const sponge_nl* nl) {
+    return nl->prune_interval > 0 ? (snb_two_level(nl) ? nl->atom_out_n : nl->atom_in_n) : nullptr;
+}
 int snb_batch_prep(sponge_batch* b, int energy, int from_crd, int overwrite, hipStream_t s);
 int snb_batch_pairs(sponge_batch* b, int energy, int overwrite, hipStream_t s);
 int snb_batch_excluded(sponge_batch* b, hipStream_t s);

@@ -590,6 +590,82 @@ __global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
                   es, en, el, list_cap, list_n, list, flags, sym_start, sym_list);
 }
 
+// Per-slot lists of the fused step from the stored list (plain LJ, inner list
+// on): for each slot ii of cluster ic, the partners jc*4 + jj of every entry
+// whose mask holds bit jj*4 + ii, in list order and jj order within an entry
+// (the order in which the pair kernel's own walk writes them), as 16-bit slot
+// offsets. One wave per cluster, one entry per lane: each lane's place in a
+// slot's list is a wave prefix sum of the entries' bit counts, so a wave
+// writes 64 entries' partners at once. A slot with more partners than cap,
+// or one beyond 16-bit reach, gets the count cap + 1 (its cluster then reads
+// the full list) and raises flags[11].
+__global__ __launch_bounds__(256) void nl_slot_lists_kernel(const int* __restrict__ flags,
+                                                            const int* __restrict__ list_n,
+                                                            const uint2* __restrict__ list, int list_cap,
+                                                            unsigned short* __restrict__ slot_list,
+                                                            int* __restrict__ slot_n, int cap, int* diag) {
+    if (!flags[0]) return;
+    const int lane = threadIdx.x & 63;
+    const int lb = xcd_block(gridDim.x);
+    const int ic = lb * 4 + (int)(threadIdx.x >> 6);
+    if (lb < 0 || ic >= flags[4]) return;
+    const int nsl = flags[3];
+    const int n = min(list_n[ic], list_cap);
+    const uint2* lst = list + (size_t)ic * list_cap;
+    int base[kClusterSize] = {0, 0, 0, 0};
+    unsigned farmask = 0u;  // slots of this lane's entries with a partner beyond 16-bit reach
+    for (int b = 0; b < n; b += 64) {
+        const int e = b + lane;
+        const uint2 en = e < n ? lst[e] : make_uint2(0u, 0u);
+        const unsigned m = en.y;
+        // bit counts of slots (0, 1) and (2, 3) in 16-bit fields
+        const unsigned c01 = (unsigned)__popc(m & 0x1111u) | ((unsigned)__popc(m & 0x2222u) << 16);
+        const unsigned c23 = (unsigned)__popc(m & 0x4444u) | ((unsigned)__popc(m & 0x8888u) << 16);
+        unsigned i01 = c01, i23 = c23;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned t01 = (unsigned)__shfl_up((int)i01, o), t23 = (unsigned)__shfl_up((int)i23, o);
+            if (lane >= o) {
+                i01 += t01;
+                i23 += t23;
+            }
+        }
+        const unsigned x01 = i01 - c01, x23 = i23 - c23;
+        const unsigned t01 = (unsigned)__shfl((int)i01, 63), t23 = (unsigned)__shfl((int)i23, 63);
+#pragma unroll
+        for (int ii = 0; ii < kClusterSize; ii++) {
+            const unsigned xx = ii < 2 ? x01 : x23;
+            int pos = base[ii] + (int)((xx >> (16 * (ii & 1))) & 0xffffu);
+            const int islot = ic * kClusterSize + ii;
+            unsigned short* out = slot_list + (size_t)islot * cap;
+#pragma unroll
+            for (int jj = 0; jj < kClusterSize; jj++) {
+                if ((m >> (jj * 4 + ii)) & 1u) {
+                    const int j = (int)en.x * kClusterSize + jj;
+                    int off = j - islot;
+                    bool reach = true;
+                    if (nsl > 32767) reach = slot_offset(islot, j, nsl, off);
+                    if (!reach) farmask |= 1u << ii;
+                    if (pos < cap) out[pos] = (unsigned short)off;
+                    pos++;
+                }
+            }
+            const unsigned tt = ii < 2 ? t01 : t23;
+            base[ii] += (int)((tt >> (16 * (ii & 1))) & 0xffffu);
+        }
+    }
+#pragma unroll
+    for (int ii = 0; ii < kClusterSize; ii++) {
+        const bool slot_far = __ballot((farmask >> ii) & 1u) != 0ull;
+        int count = base[ii];
+        if (slot_far) count = cap + 1;
+        if (lane == ii) {
+            slot_n[ic * kClusterSize + ii] = count;
+            if (count > cap) atomicOr(diag, 1);
+        }
+    }
+}
+
 // ---- batched rebuild: one launch per stage over every system --------------
 __global__ void nl_check_batch(const BatchSys* __restrict__ bs, const int2* __restrict__ map) {
     const int2 m = map[blockIdx.x];
@@ -847,6 +923,11 @@ static int nl_rebuild(sponge_nl* nl, float* d_crd, const int* es, const int* el,
         rlm, nl->col_start, nl->cl_lo, nl->cl_hi, nl->pack, es, en, el, nl->list_cap, nl->list_n, nl->list,
         nl->flags, nl->sym_start, nl->sym_list);
     SNB_CHECK_LAUNCH();
+    if (unsigned short* sl = snb_nl_slot_lists(nl)) {
+        nl_slot_lists_kernel<<<xcd_grid(ceil_div(nl->ncl_cap, 4)), 256, 0, s>>>(
+            nl->flags, nl->list_n, nl->list, nl->list_cap, sl, snb_nl_slot_counts(nl), nl->atom_in_cap, nl->flags + 11);
+        SNB_CHECK_LAUNCH();
+    }
     return SPONGE_NB_OK;
 }
 

@@ -84,18 +84,6 @@ __device__ __forceinline__ void excluded_partners(const ExclArgs& ex, const uint
     }
 }
 
-// 16-bit offset of slot j from slot i in the per-slot inner lists: j - i, or
-// the offset of j's periodic image (modulo the slot count nsl) when j - i
-// does not fit; false when neither fits. i + offset always indexes the slot
-// records or their copies kPkHalo to either side.
-__device__ __forceinline__ bool slot_offset(int i, int j, int nsl, int& off) {
-    int d = j - i;
-    if (d > 32767) d -= nsl;
-    else if (d < -32768) d += nsl;
-    off = d;
-    return d >= -32768 && d <= 32767;
-}
-
 // Per-slot lists of cluster ic: every pair the full list holds (its masks:
 // the pairs within rc + skin at the rebuild, exclusions left out), per i slot
 // in list order (lanes ii, ii + 4, ... of a round in lane order), into
@@ -463,14 +451,16 @@ __device__ __forceinline__ void pair_wave(
     // Inner lists (pa.interval > 0). c = this step's number since the
     // rebuild (the count of completed steps is advanced by the step's
     // gather, after this kernel); a step is scheduled every pa.interval-th
-    // step, voted when its prep kernel found an atom that moved more than
-    // half the prune margin since the last prune, and renewing when it is
-    // the first after a rebuild or after a new prune setting.
+    // step and on the first after a rebuild, voted when its prep kernel
+    // found an atom that moved more than half the prune margin since the
+    // last prune, and renewing when it is the first after a new prune
+    // setting.
     // Plain LJ keeps per-slot lists (pa.atom_list; force-only steps read
     // them, energy steps the full list). One level (a prune radius at or
     // beyond rc + skin): the lists hold every pair of the full list, written
-    // on renewing steps; no other step prunes and nothing is voted (the pairs
-    // are those of the list's own radius at the rebuild). Two levels
+    // by each rebuild (nl_slot_lists_kernel) and on renewing steps; no other
+    // step prunes and nothing is voted (the pairs are those of the list's
+    // own radius at the rebuild). Two levels
     // (pa.atom_olist set): those lists are the outer ones, and the inner
     // lists, the outer pairs within pa.r2, are written on scheduled, voted
     // and renewing steps. Soft-core LJ prunes the cluster-pair list to pa.r2
@@ -481,7 +471,9 @@ __device__ __forceinline__ void pair_wave(
     bool use_inner = false, write_inner = false, write_outer = false;
     if (pa.interval > 0) {
         const int c = flags[10] + 1;
-        const bool renew = c <= 1 || pa.diag[3] == c;
+        // the rebuild writes the lists of a new period; a new prune setting
+        // has the next step write them
+        const bool renew = pa.diag[3] == c;
         const bool scheduled = c <= 1 || (c - 1) % pa.interval == 0;
         const bool voted = pa.diag[2] == c;
         if (slots) {
