@@ -595,7 +595,7 @@ __global__ __launch_bounds__(kPairWaves * 64) void nl_pairs_kernel(
 // whose mask holds bit jj*4 + ii, in list order and jj order within an entry
 // (the order in which the pair kernel's own walk writes them), as 16-bit slot
 // offsets. One wave per cluster, one entry per lane: each lane's place in a
-// slot's list is a wave prefix sum of the entries' bit counts, so a wave
+// slot's list is the sum of the bit counts of the lanes below, so a wave
 // writes 64 entries' partners at once. A slot with more partners than cap,
 // or one beyond 16-bit reach, gets the count cap + 1 (its cluster then reads
 // the full list) and raises flags[11].
@@ -612,46 +612,45 @@ __global__ __launch_bounds__(256) void nl_slot_lists_kernel(const int* __restric
     const int nsl = flags[3];
     const int n = min(list_n[ic], list_cap);
     const uint2* lst = list + (size_t)ic * list_cap;
+    const unsigned long long below = (1ull << lane) - 1ull;
     int base[kClusterSize] = {0, 0, 0, 0};
     unsigned farmask = 0u;  // slots of this lane's entries with a partner beyond 16-bit reach
-    for (int b = 0; b < n; b += 64) {
-        const int e = b + lane;
-        const uint2 en = e < n ? lst[e] : make_uint2(0u, 0u);
-        const unsigned m = en.y;
-        // bit counts of slots (0, 1) and (2, 3) in 16-bit fields
-        const unsigned c01 = (unsigned)__popc(m & 0x1111u) | ((unsigned)__popc(m & 0x2222u) << 16);
-        const unsigned c23 = (unsigned)__popc(m & 0x4444u) | ((unsigned)__popc(m & 0x8888u) << 16);
-        unsigned i01 = c01, i23 = c23;
+    // the entries of four batches of 64 are loaded together; each slot's
+    // place for a lane is the count of its bits in the entries of the lanes
+    // below (three ballots of the count's bits, counted below the lane)
+    for (int b0 = 0; b0 < n; b0 += 256) {
+        uint2 ens[4];
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const unsigned t01 = (unsigned)__shfl_up((int)i01, o), t23 = (unsigned)__shfl_up((int)i23, o);
-            if (lane >= o) {
-                i01 += t01;
-                i23 += t23;
-            }
+        for (int u = 0; u < 4; u++) {
+            const int e = b0 + 64 * u + lane;
+            ens[u] = e < n ? lst[e] : make_uint2(0u, 0u);
         }
-        const unsigned x01 = i01 - c01, x23 = i23 - c23;
-        const unsigned t01 = (unsigned)__shfl((int)i01, 63), t23 = (unsigned)__shfl((int)i23, 63);
 #pragma unroll
-        for (int ii = 0; ii < kClusterSize; ii++) {
-            const unsigned xx = ii < 2 ? x01 : x23;
-            int pos = base[ii] + (int)((xx >> (16 * (ii & 1))) & 0xffffu);
-            const int islot = ic * kClusterSize + ii;
-            unsigned short* out = slot_list + (size_t)islot * cap;
+        for (int u = 0; u < 4; u++) {
+            if (b0 + 64 * u >= n) break;
+            const uint2 en = ens[u];
+            const unsigned m = en.y;
 #pragma unroll
-            for (int jj = 0; jj < kClusterSize; jj++) {
-                if ((m >> (jj * 4 + ii)) & 1u) {
-                    const int j = (int)en.x * kClusterSize + jj;
-                    int off = j - islot;
-                    bool reach = true;
-                    if (nsl > 32767) reach = slot_offset(islot, j, nsl, off);
-                    if (!reach) farmask |= 1u << ii;
-                    if (pos < cap) out[pos] = (unsigned short)off;
-                    pos++;
+            for (int ii = 0; ii < kClusterSize; ii++) {
+                const unsigned c = (unsigned)__popc(m & (0x1111u << ii));
+                const unsigned long long q0 = __ballot(c & 1u), q1 = __ballot(c & 2u), q2 = __ballot(c & 4u);
+                int pos = base[ii] + __popcll(q0 & below) + 2 * __popcll(q1 & below) + 4 * __popcll(q2 & below);
+                const int islot = ic * kClusterSize + ii;
+                unsigned short* out = slot_list + (size_t)islot * cap;
+#pragma unroll
+                for (int jj = 0; jj < kClusterSize; jj++) {
+                    if ((m >> (jj * 4 + ii)) & 1u) {
+                        const int j = (int)en.x * kClusterSize + jj;
+                        int off = j - islot;
+                        bool reach = true;
+                        if (nsl > 32767) reach = slot_offset(islot, j, nsl, off);
+                        if (!reach) farmask |= 1u << ii;
+                        if (pos < cap) out[pos] = (unsigned short)off;
+                        pos++;
+                    }
                 }
+                base[ii] += __popcll(q0) + 2 * __popcll(q1) + 4 * __popcll(q2);
             }
-            const unsigned tt = ii < 2 ? t01 : t23;
-            base[ii] += (int)((tt >> (16 * (ii & 1))) & 0xffffu);
         }
     }
 #pragma unroll
