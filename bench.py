@@ -36,7 +36,7 @@ PAIR_FLOPS = 50              # per in-cutoff pair, force-only variant
 # averaged over the run's pair launches), each scaled by what the counter
 # reads for a 256 MiB device copy (FETCH x 2.0, WRITE x 1.0). Profile
 # constants: a kernel change needs a new counter run.
-PAIR_TRAFFIC_BYTES = {"water23570": 44.40e6, "water96120": 185.70e6}
+PAIR_TRAFFIC_BYTES = {"water23570": 41.75e6, "water96120": 174.47e6}
 # Algorithmic HBM bytes: pair kernel 44 per atom (16-byte record read, force
 # 12 written, 16 of exclusion partners and list bookkeeping) + 4 per listed
 # atom pair; spread 24 per atom (base, fraction + charge, bin entry) + 8 per
