@@ -111,3 +111,27 @@ def test_new_prune_setting_prunes_at_the_next_step():
     assert eng.nl.prune_votes() == v0 + 1
     np.testing.assert_allclose(f1, f0, rtol=0, atol=1e-5 * float(np.abs(f0).max()))
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("name", ["water648", "water23570"])
+def test_rebuild_lists_equal_the_pair_kernels(name):
+    """The per-slot lists of every listed pair come from two writers: the
+    rebuild's lists kernel (each period) and the pair kernel's own walk (the
+    step after a new prune setting). Both keep list order, jj order within
+    an entry, so the force loop sums the same pairs in the same order: the
+    forces of the two steps at the same coordinates are bit-identical, one
+    level and two levels alike."""
+    import torch
+    from sponge_amd import nb
+    sys_ = SYSTEMS[name]()
+    say(f"[case] list writers {name} n={sys_.n}")
+    eng = nb.NonbondedEngine(sys_, refresh_interval=20)
+    for radius in (float(sys_.cutoff + sys_.skin), float(sys_.cutoff) + 0.5 * float(sys_.skin)):
+        eng.nl.set_prune(20, radius)
+        eng.update_neighbor_list(forced=1)         # the rebuild writes the lists
+        f_rebuild = steps(eng, 1)
+        eng.nl.set_prune(20, radius)               # the next step writes them itself
+        f_pair = steps(eng, 1)
+        torch.cuda.synchronize()
+        say(f"    radius {radius}: max |diff| {np.abs(f_rebuild - f_pair).max():.3e}")
+        assert np.array_equal(f_rebuild, f_pair), radius
