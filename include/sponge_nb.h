@@ -99,9 +99,9 @@ int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_list_start, co
 
 /* Inner list of the fused step (no counterpart in the reference);
  * prune_interval 0 turns it off (the cluster-pair list every step). Plain LJ
- * (force-only steps): the first fused step after each rebuild writes the
- * list's pairs per atom (every pair within cutoff + skin at the rebuild,
- * the reference's own list) and the later steps read them; with
+ * (force-only steps): each rebuild also writes the list's pairs per atom
+ * (every pair within cutoff + skin at the rebuild, the reference's own
+ * list) and the fused steps read them; with
  * prune_radius >= cutoff + skin nothing else is pruned. With a smaller
  * prune_radius (>= cutoff) those per-atom lists are the outer ones: every
  * prune_interval fused steps an inner list keeps their pairs within
@@ -113,7 +113,7 @@ int sponge_nl_set_exclusions(sponge_nl* nl, const int* d_excluded_list_start, co
  * inside the cutoff that the list it is pruned from holds. Default for
  * refresh_interval > 0: the tuned table's values for the atom count
  * (sponge_nb_default_shapes); off otherwise. The next fused step after this
- * call writes the lists again (a kernel on the null stream marks it: not
+ * call writes the lists itself (a kernel on the null stream marks it: not
  * during a graph capture). The other entry points always use the full
  * list. */
 int sponge_nl_set_prune(sponge_nl* nl, int prune_interval, float prune_radius);

@@ -230,8 +230,8 @@ struct sponge_nl {
     unsigned short* atom_in;    // slot_cap * atom_in_cap
     // outer per-slot lists (same form and cap), used when the prune radius
     // is below the list radius: every pair of the full list (within
-    // rc + skin at the rebuild), written by the first fused step after a
-    // rebuild or after a new prune setting (flags[14] holds that step's
+    // rc + skin at the rebuild), written by the rebuild or by the first
+    // fused step after a new prune setting (flags[14] holds that step's
     // number); the inner lists are then pruned from these
     int* atom_out_n;            // slot_cap
     unsigned short* atom_out;   // slot_cap * atom_in_cap

@@ -223,7 +223,7 @@ class NEIGHBOR_LIST:
 
     def set_prune(self, interval, radius=0.0):
         """Inner list of the fused step: per-atom lists of the full list's
-        pairs from the first step after each rebuild, pruned every
+        pairs, written by each rebuild, pruned every
         `interval` fused steps to `radius` (>= cutoff) when `radius` is below
         cutoff + skin; interval 0 turns it off."""
         _check(load_library().sponge_nl_set_prune(self.h, int(interval), float(radius)), "sponge_nl_set_prune")
